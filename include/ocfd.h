@@ -1,0 +1,130 @@
+/*
+ * ocfd.h -- C ABI of the MI355X-native OpenCFD-AMR per-box advance.
+ *
+ * Plain C: POD structs, device pointers as double*, int status codes
+ * (0 = ok), no exceptions and no torch types.  A "level" is the drop-in for one
+ * AMReX level's BoxArray + Geometry + BCs: it owns the device box work-lists
+ * and lays out MultiFab storage as one contiguous device buffer holding every
+ * box's FAB (component-major SoA, i fastest, boxes in list order), which is
+ * the amrex::Array4 convention of the reference (Source/Advance.cpp:224-228).
+ *
+ * Every entry point cites the reference interface it replaces.
+ */
+#ifndef OCFD_H
+#define OCFD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCFD_NCONS 5   /* Source/IndexDefines.H:4-9   URHO,UMX,UMY,UMZ,UEDEN */
+#define OCFD_NGROW 4   /* Source/EBR.H:207            NUM_GROW              */
+
+/* status codes */
+#define OCFD_OK           0
+#define OCFD_EINVAL       1
+#define OCFD_EHIP         2
+#define OCFD_ENOMEM       3
+#define OCFD_EUNSUPPORTED 4
+
+/* flags for the flux entry points */
+#define OCFD_VISC       1   /* ebr.do_visc    (Source/EBR.cpp:25)  */
+#define OCFD_GRAVITY    2   /* ebr.do_gravity (Source/EBR.cpp:26)  */
+#define OCFD_FLUX_FP32  4   /* convective/viscous fluxes in FP32, state and accumulation FP64 */
+#define OCFD_PLANAR     8   /* z-uniform data: skip the z sweep (its divergence is exactly 0) */
+
+/* struct Parm, Source/Constants.H:7-64 (non-CHEM members, same order) */
+typedef struct ocfd_parm {
+    double eos_gamma, eos_m, Patm, cv, cp, Ru, Rg, kOverMu, Pr, C_s, T_s,
+           smallr, smallp, eps, twoThirds, oneSix;
+} ocfd_parm;
+
+/* Level geometry + physical BCs: amrex::Geometry (domain, prob_lo, dx,
+ * periodicity) + ebr.lo_bc/hi_bc codes (0 Interior, 1 Inflow, 2 Outflow,
+ * 3 Symmetry, 4 SlipWall, 5 NoSlipWall; Exec/<case>/inputs) + the case's
+ * ext_dir fill (probtype 1 = Exec/1_DMR/prob_bcfill.cpp). */
+typedef struct ocfd_geom {
+    int dom_lo[3], dom_hi[3];
+    double prob_lo[3];
+    double dx[3];
+    int periodic[3];
+    int lo_bc[3], hi_bc[3];
+    int probtype;
+    double left_state[5];
+    double right_state[5];
+} ocfd_geom;
+
+typedef struct ocfd_level_s* ocfd_level;
+
+/* Parm defaults + Parm::Initialize (Source/Constants.H:9-29, Source/Constants.cpp:7-11).
+ * If p->eos_gamma == 0 all defaults are filled first; otherwise only the
+ * derived members Rg, cv, cp, kOverMu are recomputed (EBR::read_params,
+ * Source/EBR.cpp:522-528). */
+int ocfd_parm_initialize(ocfd_parm* p);
+
+/* Build a level from its box list (valid boxes, inclusive lo/hi, 3 ints each):
+ * the BoxArray + Geometry + BCRec set-up of EBR (Source/EBR.cpp:43-65,
+ * Source/Setup.cpp:17-92,181-205).  Allocates the device work-lists and a
+ * dSdt accumulator.  Boxes must not overlap. */
+int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* geom, ocfd_level* out);
+int ocfd_level_destroy(ocfd_level lev);
+
+/* Number of doubles of a MultiFab on this level with ngrow ghosts and ncomp
+ * comps, and the offset (in doubles) of box b's FAB inside it. */
+long ocfd_level_mf_size(ocfd_level lev, int ngrow, int ncomp);
+long ocfd_level_fab_offset(ocfd_level lev, int box, int ngrow, int ncomp);
+int  ocfd_level_nbox(ocfd_level lev);
+
+/* EBR::compute_dSdt regular-box path (Source/Advance.cpp:182-418 with the
+ * kernels of Source/kernel/*.H): dSdt (0 ghosts, 5 comps) = -div F(S) for every
+ * box of the level, S holding 4 filled ghost cells.  dt only feeds flux
+ * registers in the reference and is unused here.  flags: OCFD_VISC |
+ * OCFD_GRAVITY | OCFD_FLUX_FP32 | OCFD_PLANAR.  stream: hipStream_t (NULL =
+ * default stream). */
+int ocfd_compute_dsdt(ocfd_level lev, const double* S, double* dSdt, double dt,
+                      const ocfd_parm* parm, int flags, void* stream);
+
+/* One RK stage fused with compute_dSdt: S_new(valid) = (c_cur*S_cur +
+ * c_old*S_old) + c_dt * dSdt(S_cur), i.e. MultiFab::LinComb + Saxpy of
+ * EBR::flow_advance (Source/Advance.cpp:117-148).  S_old may be NULL (then the
+ * c_old term is absent).  Ghost cells of S_new are not written. */
+int ocfd_rk_stage(ocfd_level lev, const double* S_cur, const double* S_old, double* S_new,
+                  double c_cur, double c_old, double c_dt,
+                  const ocfd_parm* parm, int flags, void* stream);
+
+/* FillPatch on one level (Source/Advance.cpp:119,126,134 + Exec/<case>/prob_bcfill.cpp):
+ * fills the 4 ghost cells of every FAB of S from the level's valid cells,
+ * periodic images and the physical BCs at time `time`. */
+int ocfd_fill_boundary(ocfd_level lev, double* S, double time, void* stream);
+
+/* EBR::estTimeStep (Source/Advance.cpp:61-79): cfl * min over valid cells of
+ * dx_d/(|u_d|+c).  Synchronises the stream and writes the result to *dt_out. */
+int ocfd_estdt(ocfd_level lev, const double* S, double cfl, const ocfd_parm* parm,
+               double* dt_out, void* stream);
+
+/* One coarse time step of EBR::advance without chemistry (Source/Advance.cpp:81-150):
+ * scheme 0 Euler, 1 RK2 (Heun), 2 SSP-RK3.  A holds U^n with filled ghosts on
+ * entry and U^(n+1) with filled ghosts on return; B and C are scratch
+ * MultiFabs of the same layout (4 ghosts, 5 comps). */
+int ocfd_advance(ocfd_level lev, double* A, double* B, double* C, double time, double dt,
+                 int scheme, const ocfd_parm* parm, int flags, void* stream);
+
+/* Case initial data into the valid cells of state MultiFab S (4 ghosts, 5 comps):
+ * kind 0 = Sod (Exec/0_SOD/prob.H:11-38), 1 = double Mach reflection
+ * (Exec/1_DMR/prob.H:11-45), 2 = 3D viscous Kelvin-Helmholtz shear layer
+ * (the 3D counterpart of Exec/8_KH/prob.H:11-42, seeded).  The
+ * amrex_probinit/initData pair of the reference (Source/EBR.cpp:111-170).
+ * Ghost cells are zeroed.  Synchronous. */
+int ocfd_init_state(ocfd_level lev, int kind, double* S, const ocfd_parm* parm, unsigned long long seed);
+
+/* DMR inflow states used by probtype 1 (Exec/1_DMR/prob.cpp:28-40). */
+int ocfd_dmr_states(double* left, double* right);
+
+/* Last error message of the calling thread ("" if none). */
+const char* ocfd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCFD_H */

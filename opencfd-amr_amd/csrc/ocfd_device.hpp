@@ -1,0 +1,308 @@
+// Device-side building blocks of the per-box advance: box/geometry
+// descriptors and the point kernels (primitive recovery, Steger-Warming split,
+// WENO-Z5, viscous face flux, stage update), written for CDNA4.
+//
+// FP64 forms follow the reference's operation order term by term, so the FP64
+// path agrees with the reference arithmetic to rounding (FMA contraction is
+// the only difference).  The FT=float forms are the FP32-flux variant: the
+// same formulas in single precision with hardware reciprocals in the WENO
+// weights; primitive recovery stays FP64 so pressure does not suffer the
+// E - 1/2 rho u^2 cancellation in single precision.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "ocfd.h"
+
+namespace ocfd {
+
+constexpr int NG = OCFD_NGROW;
+constexpr int NC = OCFD_NCONS;
+enum { URHO = 0, UMX = 1, UMY = 2, UMZ = 3, UEDEN = 4 };
+
+// BC types of the ghost fill (amrex::BCType meanings)
+enum : signed char { BC_INT = 0, BC_EVEN = 1, BC_FOEXT = 2, BC_EXT = 3, BC_ODD = -1 };
+
+struct BoxDesc {
+    int lo[3], hi[3];     // valid box, global cell indices
+    int nxg, nyg, nzg;    // extents of the grown (4-ghost) FAB
+    int geom;             // index into the geometry table
+    long off_s;           // offset (doubles) of this box's grown FAB in a state MultiFab
+    long off_d;           // offset (doubles) of this box's valid FAB in a dSdt MultiFab
+    long nptg;            // component stride of the grown FAB
+    long nptv;            // component stride of the valid FAB
+    double dxinv[3];
+    double dx[3];
+};
+
+struct GeomDesc {
+    int dom_lo[3], dom_hi[3];
+    int periodic[3];
+    signed char bc[NC][3][2];   // BC type per component, direction, side (0 lo, 1 hi)
+    int probtype;
+    double prob_lo[3], dx[3];
+    double left[NC], right[NC];
+    int map_shift;              // log2 of the owner-map granularity
+    int map_n[3];
+    const int* map;             // coarse cell -> box index, -1 if not covered
+};
+
+// Parm members the kernels use, by value in the launch arguments
+struct PhysArgs {
+    double gamma, smallr, smallp, Rg, C_s, T_s, kOverMu, twoThirds;
+};
+
+struct StageArgs {
+    const double* scur;   // state being differentiated (4 ghosts, filled)
+    const double* sold;   // U^n for the stage combination, or null
+    double* snew;         // stage output (valid cells), or null for a plain dSdt
+    double* acc;          // dSdt accumulator / output (valid layout)
+    double c_cur, c_old, c_dt;
+    int flags;
+    PhysArgs ph;
+};
+
+__device__ __forceinline__ long sidx(const BoxDesc& B, int i, int j, int k)
+{
+    return B.off_s + ((long)(k - B.lo[2] + NG) * B.nyg + (j - B.lo[1] + NG)) * B.nxg + (i - B.lo[0] + NG);
+}
+
+__device__ __forceinline__ long didx(const BoxDesc& B, int i, int j, int k)
+{
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
+    return B.off_d + ((long)(k - B.lo[2]) * ny + (j - B.lo[1])) * nx + (i - B.lo[0]);
+}
+
+// Primitive state used by the flux kernels
+template <class FT>
+struct Prim {
+    FT r, u, v, w, c, T;
+};
+
+// c2prim, Source/kernel/Kernels.H:13-40 (always FP64, then narrowed)
+template <class FT>
+__device__ __forceinline__ Prim<FT> c2prim(const double* __restrict__ s, long idx, long stride, const PhysArgs& ph)
+{
+    double rho = fmax(s[idx], ph.smallr);
+    double rhoinv = 1.0 / rho;
+    double ux = s[idx + stride] * rhoinv;
+    double uy = s[idx + 2 * stride] * rhoinv;
+    double uz = s[idx + 3 * stride] * rhoinv;
+    double kineng = 0.5 * (ux * ux + uy * uy + uz * uz);
+    double ei = s[idx + 4 * stride] - rho * kineng;
+    double p = fmax((ph.gamma - 1.0) * ei, ph.smallp);
+    double T = p / (rho * ph.Rg);
+    Prim<FT> q;
+    q.r = (FT)rho; q.u = (FT)ux; q.v = (FT)uy; q.w = (FT)uz;
+    q.T = (FT)T;
+    q.c = (FT)sqrt(ph.gamma * ph.Rg * T);
+    return q;
+}
+
+// velocity only (viscous tangential stencils)
+template <class FT>
+__device__ __forceinline__ void c2vel(const double* __restrict__ s, long idx, long stride, const PhysArgs& ph,
+                                      FT& u, FT& v, FT& w)
+{
+    double rho = fmax(s[idx], ph.smallr);
+    double rhoinv = 1.0 / rho;
+    u = (FT)(s[idx + stride] * rhoinv);
+    v = (FT)(s[idx + 2 * stride] * rhoinv);
+    w = (FT)(s[idx + 3 * stride] * rhoinv);
+}
+
+// Steger-Warming split in direction D, Source/kernel/FluxSplit.H:11-193.
+template <int D, class FT>
+__device__ __forceinline__ void sw_split(const Prim<FT>& q, FT gamma, FT* __restrict__ fp, FT* __restrict__ fm)
+{
+    const FT d = q.r, u = q.u, v = q.v, w = q.w, c = q.c;
+    const FT tmp1 = FT(1.0) / (FT(2.0) * gamma);
+    const FT tmp2 = FT(2.0) * (gamma - FT(1.0));
+    const FT tmp3 = (FT(3.0) - gamma) / (FT(2.0) * (gamma - FT(1.0)));
+    const FT un = D == 0 ? u : (D == 1 ? v : w);
+    const FT uc1 = un - c;
+    const FT uc2 = un + c;
+    FT vvc1, vvc2;
+    if (D == 0) {
+        vvc1 = FT(0.5) * (uc1 * uc1 + v * v + w * w);
+        vvc2 = FT(0.5) * (uc2 * uc2 + v * v + w * w);
+    } else if (D == 1) {
+        vvc1 = FT(0.5) * (u * u + uc1 * uc1 + w * w);
+        vvc2 = FT(0.5) * (u * u + uc2 * uc2 + w * w);
+    } else {
+        vvc1 = FT(0.5) * (u * u + v * v + uc1 * uc1);
+        vvc2 = FT(0.5) * (u * u + v * v + uc2 * uc2);
+    }
+    const FT v2 = (gamma - FT(1)) * (u * u + v * v + w * w);
+    const FT w2 = tmp3 * c * c;
+    const FT E1 = un, E2 = un - c, E3 = un + c;
+    const FT E1P = (E1 + fabs(E1)) * FT(0.5);
+    const FT E2P = (E2 + fabs(E2)) * FT(0.5);
+    const FT E3P = (E3 + fabs(E3)) * FT(0.5);
+    const FT E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
+    const FT tmp0 = d * tmp1;
+    const FT sp = tmp2 * E1P + E2P + E3P;
+    const FT sm = tmp2 * E1M + E2M + E3M;
+    fp[URHO] = tmp0 * sp;
+    fm[URHO] = tmp0 * sm;
+    const FT vel[3] = {u, v, w};
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        if (m == D) {
+            fp[UMX + m] = tmp0 * (tmp2 * E1P * un + E2P * uc1 + E3P * uc2);
+            fm[UMX + m] = tmp0 * (tmp2 * E1M * un + E2M * uc1 + E3M * uc2);
+        } else {
+            fp[UMX + m] = tmp0 * vel[m] * sp;
+            fm[UMX + m] = tmp0 * vel[m] * sm;
+        }
+    }
+    fp[UEDEN] = tmp0 * (E1P * v2 + E2P * vvc1 + E3P * vvc2 + w2 * (E2P + E3P));
+    fm[UEDEN] = tmp0 * (E1M * v2 + E2M * vvc1 + E3M * vvc2 + w2 * (E2M + E3M));
+}
+
+// reciprocal: IEEE division in FP64, hardware v_rcp_f32 in FP32
+__device__ __forceinline__ double rcp_(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double div_(double a, double b) { return a / b; }
+__device__ __forceinline__ float div_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// WENO-Z5 from f+ at cells (f-3..f+1) of face f, Source/kernel/Reconstruction.H:83-110
+template <class FT>
+__device__ __forceinline__ FT weno_p(FT V1, FT V2, FT V3, FT V4, FT V5)
+{
+    const FT eps = FT(1e-6);
+    const FT c13 = FT(13.0 / 12.0);
+    FT t1 = V1 - FT(2.0) * V2 + V3, t2 = V1 - FT(4.0) * V2 + FT(3.0) * V3;
+    FT s11 = c13 * t1 * t1 + FT(0.25) * t2 * t2;
+    FT t3 = V2 - FT(2.0) * V3 + V4, t4 = V2 - V4;
+    FT s22 = c13 * t3 * t3 + FT(0.25) * t4 * t4;
+    FT t5 = V3 - FT(2.0) * V4 + V5, t6 = FT(3.0) * V3 - FT(4.0) * V4 + V5;
+    FT s33 = c13 * t5 * t5 + FT(0.25) * t6 * t6;
+    FT s55 = fabs(s11 - s33);
+    FT r1 = div_(s55, eps + s11), r2 = div_(s55, eps + s22), r3 = div_(s55, eps + s33);
+    FT a1 = FT(1.0) + r1 * r1;
+    FT a2 = FT(6.0) * (FT(1.0) + r2 * r2);
+    FT a3 = FT(3.0) * (FT(1.0) + r3 * r3);
+    FT invsum = rcp_(a1 + a2 + a3);
+    FT v1 = FT(2.0) * V1 - FT(7.0) * V2 + FT(11.0) * V3;
+    FT v2 = -V2 + FT(5.0) * V3 + FT(2.0) * V4;
+    FT v3 = FT(2.0) * V3 + FT(5.0) * V4 - V5;
+    return FT(1.0 / 6.0) * invsum * (a1 * v1 + a2 * v2 + a3 * v3);
+}
+
+// WENO-Z5 from f- at cells (f-2..f+2) of face f, Source/kernel/Reconstruction.H:112-137
+template <class FT>
+__device__ __forceinline__ FT weno_m(FT V1, FT V2, FT V3, FT V4, FT V5)
+{
+    const FT eps = FT(1e-6);
+    const FT c13 = FT(13.0 / 12.0);
+    FT t1 = V5 - FT(2.0) * V4 + V3, t2 = V5 - FT(4.0) * V4 + FT(3.0) * V3;
+    FT s11 = c13 * t1 * t1 + FT(0.25) * t2 * t2;
+    FT t3 = V2 - FT(2.0) * V3 + V4, t4 = V4 - V2;
+    FT s22 = c13 * t3 * t3 + FT(0.25) * t4 * t4;
+    FT t5 = V3 - FT(2.0) * V2 + V1, t6 = FT(3.0) * V3 - FT(4.0) * V2 + V1;
+    FT s33 = c13 * t5 * t5 + FT(0.25) * t6 * t6;
+    FT s55 = fabs(s11 - s33);
+    FT r1 = div_(s55, eps + s11), r2 = div_(s55, eps + s22), r3 = div_(s55, eps + s33);
+    FT a1 = FT(1.0) + r1 * r1;
+    FT a2 = FT(6.0) * (FT(1.0) + r2 * r2);
+    FT a3 = FT(3.0) * (FT(1.0) + r3 * r3);
+    FT invsum = rcp_(a1 + a2 + a3);
+    FT v1 = FT(11.0) * V3 - FT(7.0) * V4 + FT(2.0) * V5;
+    FT v2 = -V4 + FT(5.0) * V3 + FT(2.0) * V2;
+    FT v3 = FT(2.0) * V3 + FT(5.0) * V2 - V1;
+    return FT(1.0 / 6.0) * invsum * (a1 * v1 + a2 * v2 + a3 * v3);
+}
+
+// Sutherland viscosity, Source/kernel/Diffusion.H:21-22
+template <class FT>
+__device__ __forceinline__ FT sutherland(FT T, const PhysArgs& ph)
+{
+    return div_(FT(ph.C_s) * T * sqrt(T), T + FT(ph.T_s));
+}
+
+// Tangential 4-point difference q(+,R)+q(+,L)-q(-,R)-q(-,L), Diffusion.H:31-34
+template <class FT>
+__device__ __forceinline__ FT lat(FT pR, FT pL, FT mR, FT mL) { return pR + pL - mR - mL; }
+
+// Viscous + heat flux on the face between cell L (low) and cell R (high) in
+// direction D, added to F (Source/kernel/Diffusion.H:7-131).  The tangential
+// derivatives come in as the two 4-point differences the reference forms:
+//   D=0: ta = lat(u,y) tb = lat(v,y) tc = lat(u,z) td = lat(w,z)
+//   D=1: ta = lat(u,x) tb = lat(v,x) tc = lat(v,z) td = lat(w,z)
+//   D=2: ta = lat(u,x) tb = lat(w,x) tc = lat(v,y) td = lat(w,y)
+template <int D, class FT>
+__device__ __forceinline__ void visc_face(const Prim<FT>& L, const Prim<FT>& R, FT ta, FT tb, FT tc, FT td,
+                                          const double* dxinv, const PhysArgs& ph, FT* F)
+{
+    const FT mutl = sutherland(L.T, ph), mutr = sutherland(R.T, ph);
+    const FT ktl = mutl * FT(ph.kOverMu), ktr = mutr * FT(ph.kOverMu);
+    const FT dn = FT(dxinv[D]);
+    const FT dT = (R.T - L.T) * dn;
+    const FT du = (R.u - L.u) * dn;
+    const FT dv = (R.v - L.v) * dn;
+    const FT dw = (R.w - L.w) * dn;
+    const FT muf = FT(0.5) * (mutl + mutr);
+    const FT tt = FT(ph.twoThirds);
+    FT t1, t2, t3;
+    if (D == 0) {
+        FT dudy = ta * (FT(0.25) * FT(dxinv[1]));
+        FT dvdy = tb * (FT(0.25) * FT(dxinv[1]));
+        FT dudz = tc * (FT(0.25) * FT(dxinv[2]));
+        FT dwdz = td * (FT(0.25) * FT(dxinv[2]));
+        FT divu = du + dvdy + dwdz;
+        t1 = muf * (FT(2.0) * du - tt * divu);   // tauxx
+        t2 = muf * (dudy + dv);                  // tauxy
+        t3 = muf * (dudz + dw);                  // tauxz
+    } else if (D == 1) {
+        FT dudx = ta * (FT(0.25) * FT(dxinv[0]));
+        FT dvdx = tb * (FT(0.25) * FT(dxinv[0]));
+        FT dvdz = tc * (FT(0.25) * FT(dxinv[2]));
+        FT dwdz = td * (FT(0.25) * FT(dxinv[2]));
+        FT divu = dudx + dv + dwdz;
+        t2 = muf * (FT(2.0) * dv - tt * divu);   // tauyy
+        t1 = muf * (du + dvdx);                  // tauxy
+        t3 = muf * (dw + dvdz);                  // tauyz
+    } else {
+        FT dudx = ta * (FT(0.25) * FT(dxinv[0]));
+        FT dwdx = tb * (FT(0.25) * FT(dxinv[0]));
+        FT dvdy = tc * (FT(0.25) * FT(dxinv[1]));
+        FT dwdy = td * (FT(0.25) * FT(dxinv[1]));
+        FT divu = dudx + dvdy + dw;
+        t1 = muf * (du + dwdx);                  // tauxz
+        t2 = muf * (dv + dwdy);                  // tauyz
+        t3 = muf * (FT(2.0) * dw - tt * divu);   // tauzz
+    }
+    F[UMX] += -t1;
+    F[UMY] += -t2;
+    F[UMZ] += -t3;
+    F[UEDEN] += FT(-0.5) * ((R.u + L.u) * t1 + (R.v + L.v) * t2 + (R.w + L.w) * t3 + (ktl + ktr) * dT);
+}
+
+// Gravity source (Source/Advance.cpp:331-342) and the stage combination of
+// MultiFab::LinComb / Saxpy (Source/Advance.cpp:117-148); with no output
+// state the divergence itself is stored (compute_dSdt).
+__device__ __forceinline__ void finish_cell(const StageArgs& a, const BoxDesc& B, int i, int j, int k, double* L)
+{
+    const long s0 = sidx(B, i, j, k);
+    const long st = B.nptg;
+    if (a.flags & OCFD_GRAVITY) {
+        const double g = -9.8;
+        L[UMZ] += g * a.scur[s0];
+        L[UEDEN] += g * a.scur[s0 + 3 * st];
+    }
+    if (a.snew) {
+#pragma unroll
+        for (int n = 0; n < NC; ++n) {
+            double v = a.sold ? a.c_cur * a.scur[s0 + n * st] + a.c_old * a.sold[s0 + n * st]
+                              : a.c_cur * a.scur[s0 + n * st];
+            a.snew[s0 + n * st] = v + a.c_dt * L[n];
+        }
+    } else {
+        const long d0 = didx(B, i, j, k);
+#pragma unroll
+        for (int n = 0; n < NC; ++n) a.acc[d0 + n * B.nptv] = L[n];
+    }
+}
+
+} // namespace ocfd

@@ -1,0 +1,261 @@
+// Flux-divergence sweeps of the per-box advance (EBR::compute_dSdt regular
+// path, Source/Advance.cpp:231-342), one kernel per direction over the box
+// work-list of a whole level (or of many levels/cases at once).
+//
+//   sweep_x : wave = one x-row segment of 58 cells + 3 halo cells each side.
+//             Each lane recovers primitives and splits its cell (FP64 prims,
+//             FT split), the split fluxes are exchanged through LDS, lanes
+//             3..61 evaluate the 59 faces, and the x-part of -div F is stored.
+//   sweep_y : lane = one i column, marches j through a chunk of CH cells with
+//             a register window of split fluxes (6 cells of f+, 5 of f-), and
+//             adds the y-part into the accumulator.
+//   sweep_z : same march along k; adds the z-part and finishes the cell
+//             (gravity, RK stage combination or plain dSdt store).
+//
+// The divergence is accumulated in FP64 in the reference's order
+// dx-term + dy-term + dz-term (Source/kernel/Kernels.H:94-96).
+#include "ocfd_device.hpp"
+#include "ocfd_internal.hpp"
+
+namespace ocfd {
+
+constexpr int XSEG = 58;   // interior cells of one x-row wave
+constexpr int CH = 16;     // cells per march chunk (y and z sweeps)
+
+template <class FT, bool VISC>
+__global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+                                                 StageArgs a)
+{
+    __shared__ FT sf[4][10][64];
+    __shared__ FT sv[VISC ? 4 : 1][VISC ? 12 : 1][64];
+    const int4 t = tiles[blockIdx.x];
+    const BoxDesc B = boxes[t.x];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    const int row = t.z * 4 + w;
+    const bool rowok = row < ny * nz;
+    const int j = B.lo[1] + (rowok ? row % ny : 0);
+    const int k = B.lo[2] + (rowok ? row / ny : 0);
+    const int i = B.lo[0] + t.y * XSEG - 3 + lane;
+    const bool cellok = rowok && i <= B.hi[0] + 3;
+    const FT gam = (FT)a.ph.gamma;
+    (void)nx;
+
+    if (cellok) {
+        const long s0 = sidx(B, i, j, k);
+        Prim<FT> q = c2prim<FT>(a.scur, s0, B.nptg, a.ph);
+        FT fp[5], fm[5];
+        sw_split<0, FT>(q, gam, fp, fm);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) { sf[w][n][lane] = fp[n]; sf[w][5 + n][lane] = fm[n]; }
+        if (VISC) {
+            sv[w][0][lane] = q.T; sv[w][1][lane] = q.u; sv[w][2][lane] = q.v; sv[w][3][lane] = q.w;
+            FT u, v, ww;
+            c2vel<FT>(a.scur, s0 + B.nxg, B.nptg, a.ph, u, v, ww);              // (i, j+1)
+            sv[w][4][lane] = u; sv[w][5][lane] = v;
+            c2vel<FT>(a.scur, s0 - B.nxg, B.nptg, a.ph, u, v, ww);              // (i, j-1)
+            sv[w][6][lane] = u; sv[w][7][lane] = v;
+            c2vel<FT>(a.scur, s0 + (long)B.nxg * B.nyg, B.nptg, a.ph, u, v, ww); // (i, k+1)
+            sv[w][8][lane] = u; sv[w][9][lane] = ww;
+            c2vel<FT>(a.scur, s0 - (long)B.nxg * B.nyg, B.nptg, a.ph, u, v, ww); // (i, k-1)
+            sv[w][10][lane] = u; sv[w][11][lane] = ww;
+        }
+    }
+    __syncthreads();
+
+    const bool faceok = rowok && lane >= 3 && lane <= 61 && i <= B.hi[0] + 1;
+    FT F[5];
+    if (faceok) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            F[n] = weno_p<FT>(sf[w][n][lane - 3], sf[w][n][lane - 2], sf[w][n][lane - 1], sf[w][n][lane], sf[w][n][lane + 1])
+                 + weno_m<FT>(sf[w][5 + n][lane - 2], sf[w][5 + n][lane - 1], sf[w][5 + n][lane], sf[w][5 + n][lane + 1],
+                              sf[w][5 + n][lane + 2]);
+        }
+        if (VISC) {
+            Prim<FT> L, R;
+            L.T = sv[w][0][lane - 1]; L.u = sv[w][1][lane - 1]; L.v = sv[w][2][lane - 1]; L.w = sv[w][3][lane - 1];
+            R.T = sv[w][0][lane];     R.u = sv[w][1][lane];     R.v = sv[w][2][lane];     R.w = sv[w][3][lane];
+            FT ta = lat(sv[w][4][lane], sv[w][4][lane - 1], sv[w][6][lane], sv[w][6][lane - 1]);
+            FT tb = lat(sv[w][5][lane], sv[w][5][lane - 1], sv[w][7][lane], sv[w][7][lane - 1]);
+            FT tc = lat(sv[w][8][lane], sv[w][8][lane - 1], sv[w][10][lane], sv[w][10][lane - 1]);
+            FT td = lat(sv[w][9][lane], sv[w][9][lane - 1], sv[w][11][lane], sv[w][11][lane - 1]);
+            visc_face<0, FT>(L, R, ta, tb, tc, td, B.dxinv, a.ph, F);
+        }
+    }
+    __syncthreads();
+    if (faceok) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) sf[w][n][lane] = F[n];
+    }
+    __syncthreads();
+    if (rowok && lane >= 3 && lane <= 60 && i <= B.hi[0]) {
+        const long d0 = didx(B, i, j, k);
+#pragma unroll
+        for (int n = 0; n < 5; ++n)
+            a.acc[d0 + n * B.nptv] = B.dxinv[0] * ((double)sf[w][n][lane] - (double)sf[w][n][lane + 1]);
+    }
+}
+
+// Marching sweep along direction D (1 = y, 2 = z).  FINAL: this sweep
+// completes the divergence and finishes the cell.
+template <int D, class FT, bool VISC, bool FINAL>
+__global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+                                                     StageArgs a)
+{
+    const int4 t = tiles[blockIdx.x];
+    const BoxDesc B = boxes[t.x];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the "other" transverse direction: z for the y sweep, y for the z sweep
+    constexpr int O = D == 1 ? 2 : 1;
+    const int i = B.lo[0] + t.y * 64 + lane;
+    const int o = B.lo[O] + t.z * 4 + w;
+    if (i > B.hi[0] || o > B.hi[O]) return;
+    const int m0 = B.lo[D] + t.w * CH;
+    const int m1 = min(m0 + CH - 1, B.hi[D]);
+    const FT gam = (FT)a.ph.gamma;
+    const long st = B.nptg;
+    const long dstep = D == 1 ? (long)B.nxg : (long)B.nxg * B.nyg;   // index step along D
+    const long ostep = O == 1 ? (long)B.nxg : (long)B.nxg * B.nyg;   // index step along O
+    const double dxi = B.dxinv[D];
+    int jj = D == 1 ? m0 : o, kk = D == 1 ? o : m0;
+    const long base = sidx(B, i, jj, kk);           // state index of cell m0
+    const long dbase = didx(B, i, jj, kk);
+    const long dd = D == 1 ? (long)(B.hi[0] - B.lo[0] + 1)
+                           : (long)(B.hi[0] - B.lo[0] + 1) * (B.hi[1] - B.lo[1] + 1);
+
+    FT Wp[6][5], Wm[5][5];
+    // prologue: cells m0-3 .. m0+1
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        Prim<FT> q = c2prim<FT>(a.scur, base + (c - 3) * dstep, st, a.ph);
+        FT fp[5], fm[5];
+        sw_split<D, FT>(q, gam, fp, fm);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) Wp[c][n] = fp[n];
+        if (c >= 1) {
+#pragma unroll
+            for (int n = 0; n < 5; ++n) Wm[c - 1][n] = fm[n];
+        }
+    }
+    // viscous state of the low cell of the first face (m0-1)
+    Prim<FT> PL;
+    FT lxpL_a = 0, lxpL_b = 0, lxmL_a = 0, lxmL_b = 0, lopL_a = 0, lopL_b = 0, lomL_a = 0, lomL_b = 0;
+    if (VISC) {
+        const long s = base - dstep;
+        PL = c2prim<FT>(a.scur, s, st, a.ph);
+        FT u, v, w2;
+        c2vel<FT>(a.scur, s + 1, st, a.ph, u, v, w2);
+        lxpL_a = u; lxpL_b = D == 1 ? v : w2;
+        c2vel<FT>(a.scur, s - 1, st, a.ph, u, v, w2);
+        lxmL_a = u; lxmL_b = D == 1 ? v : w2;
+        c2vel<FT>(a.scur, s + ostep, st, a.ph, u, v, w2);
+        lopL_a = D == 1 ? v : v; lopL_b = w2;
+        c2vel<FT>(a.scur, s - ostep, st, a.ph, u, v, w2);
+        lomL_a = v; lomL_b = w2;
+    }
+
+    FT Fprev[5];
+    for (int f = m0; f <= m1 + 1; ++f) {
+        // load cell f+2
+        {
+            Prim<FT> q = c2prim<FT>(a.scur, base + (long)(f + 2 - m0) * dstep, st, a.ph);
+            FT fp[5], fm[5];
+            sw_split<D, FT>(q, gam, fp, fm);
+#pragma unroll
+            for (int n = 0; n < 5; ++n) { Wp[5][n] = fp[n]; Wm[4][n] = fm[n]; }
+        }
+        FT F[5];
+#pragma unroll
+        for (int n = 0; n < 5; ++n)
+            F[n] = weno_p<FT>(Wp[0][n], Wp[1][n], Wp[2][n], Wp[3][n], Wp[4][n])
+                 + weno_m<FT>(Wm[0][n], Wm[1][n], Wm[2][n], Wm[3][n], Wm[4][n]);
+        if (VISC) {
+            const long s = base + (long)(f - m0) * dstep;
+            Prim<FT> PR = c2prim<FT>(a.scur, s, st, a.ph);
+            FT u, v, w2;
+            c2vel<FT>(a.scur, s + 1, st, a.ph, u, v, w2);
+            FT xpa = u, xpb = D == 1 ? v : w2;
+            c2vel<FT>(a.scur, s - 1, st, a.ph, u, v, w2);
+            FT xma = u, xmb = D == 1 ? v : w2;
+            c2vel<FT>(a.scur, s + ostep, st, a.ph, u, v, w2);
+            FT opa = v, opb = w2;
+            c2vel<FT>(a.scur, s - ostep, st, a.ph, u, v, w2);
+            FT oma = v, omb = w2;
+            // D=1: ta=lat(u,x) tb=lat(v,x) tc=lat(v,z) td=lat(w,z)
+            // D=2: ta=lat(u,x) tb=lat(w,x) tc=lat(v,y) td=lat(w,y)
+            FT ta = lat(xpa, lxpL_a, xma, lxmL_a);
+            FT tb = lat(xpb, lxpL_b, xmb, lxmL_b);
+            FT tc = lat(opa, lopL_a, oma, lomL_a);
+            FT td = lat(opb, lopL_b, omb, lomL_b);
+            visc_face<D, FT>(PL, PR, ta, tb, tc, td, B.dxinv, a.ph, F);
+            PL = PR;
+            lxpL_a = xpa; lxpL_b = xpb; lxmL_a = xma; lxmL_b = xmb;
+            lopL_a = opa; lopL_b = opb; lomL_a = oma; lomL_b = omb;
+        }
+        if (f > m0) {
+            // cell f-1 has both faces
+            const long d0 = dbase + (long)(f - 1 - m0) * dd;
+            double L[5];
+#pragma unroll
+            for (int n = 0; n < 5; ++n)
+                L[n] = a.acc[d0 + n * B.nptv] + dxi * ((double)Fprev[n] - (double)F[n]);
+            if (FINAL) {
+                const int c = f - 1;
+                finish_cell(a, B, i, D == 1 ? c : o, D == 1 ? o : c, L);
+            } else {
+#pragma unroll
+                for (int n = 0; n < 5; ++n) a.acc[d0 + n * B.nptv] = L[n];
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < 5; ++n) Fprev[n] = F[n];
+        // shift the window
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+#pragma unroll
+            for (int n = 0; n < 5; ++n) Wp[c][n] = Wp[c + 1][n];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+#pragma unroll
+            for (int n = 0; n < 5; ++n) Wm[c][n] = Wm[c + 1][n];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <class FT, bool VISC>
+static void launch_sweeps_t(const LevelDev& L, const StageArgs& a, bool planar, hipStream_t s)
+{
+    if (L.ntile_x) hipLaunchKernelGGL((k_sweep_x<FT, VISC>), dim3(L.ntile_x), dim3(256), 0, s, L.d_boxes, L.d_tile_x, a);
+    if (planar) {
+        if (L.ntile_y) hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, true>), dim3(L.ntile_y), dim3(256), 0, s,
+                                          L.d_boxes, L.d_tile_y, a);
+    } else {
+        if (L.ntile_y) hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, false>), dim3(L.ntile_y), dim3(256), 0, s,
+                                          L.d_boxes, L.d_tile_y, a);
+        if (L.ntile_z) hipLaunchKernelGGL((k_sweep_march<2, FT, VISC, true>), dim3(L.ntile_z), dim3(256), 0, s,
+                                          L.d_boxes, L.d_tile_z, a);
+    }
+}
+
+void launch_sweeps(const LevelDev& L, const StageArgs& a, hipStream_t s)
+{
+    const bool planar = (a.flags & OCFD_PLANAR) != 0;
+    const bool visc = (a.flags & OCFD_VISC) != 0;
+    if (a.flags & OCFD_FLUX_FP32) {
+        if (visc) launch_sweeps_t<float, true>(L, a, planar, s);
+        else launch_sweeps_t<float, false>(L, a, planar, s);
+    } else {
+        if (visc) launch_sweeps_t<double, true>(L, a, planar, s);
+        else launch_sweeps_t<double, false>(L, a, planar, s);
+    }
+}
+
+int tile_xseg() { return XSEG; }
+int tile_chunk() { return CH; }
+
+} // namespace ocfd

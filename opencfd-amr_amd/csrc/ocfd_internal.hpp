@@ -1,0 +1,41 @@
+// Host-side level object shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "ocfd_device.hpp"
+
+namespace ocfd {
+
+// A level: box list + geometry, with the device work-lists of every kernel.
+struct LevelDev {
+    int nbox = 0;
+    std::vector<BoxDesc> boxes;     // host copy
+    GeomDesc geom{};                // host copy (map pointer is the device map)
+    std::vector<int> owner_map;     // host copy of the owner map
+    BoxDesc* d_boxes = nullptr;
+    GeomDesc* d_geom = nullptr;
+    int* d_map = nullptr;
+    int4* d_tile_x = nullptr; int ntile_x = 0;
+    int4* d_tile_y = nullptr; int ntile_y = 0;
+    int4* d_tile_z = nullptr; int ntile_z = 0;
+    int4* d_tile_g = nullptr; int ntile_g = 0;   // ghost fill: (box, first ghost cell, count, 0)
+    int4* d_tile_c = nullptr; int ntile_c = 0;   // valid-cell chunks: (box, first cell, count, 0)
+    double* d_acc = nullptr;        // dSdt accumulator, valid layout, 5 comps
+    unsigned long long* d_red = nullptr;  // reduction result slot
+    long mf_valid = 0;              // doubles in a 0-ghost 5-comp MultiFab
+    long mf_state = 0;              // doubles in a 4-ghost 5-comp MultiFab
+};
+
+void launch_sweeps(const LevelDev& L, const StageArgs& a, hipStream_t s);
+int tile_xseg();
+int tile_chunk();
+
+void set_error(const std::string& msg);
+
+} // namespace ocfd
+
+struct ocfd_level_s {
+    ocfd::LevelDev dev;
+};

@@ -1,0 +1,500 @@
+// Level set-up, ghost fill (FillPatch on one level), CFL reduction and the
+// C ABI of include/ocfd.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "ocfd_device.hpp"
+#include "ocfd_internal.hpp"
+
+namespace ocfd {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+#define OCFD_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            set_error(std::string(#call) + ": " + hipGetErrorString(e_));               \
+            return OCFD_EHIP;                                                           \
+        }                                                                               \
+    } while (0)
+
+static int check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return OCFD_EHIP;
+    }
+    return OCFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ghost fill
+// ---------------------------------------------------------------------------
+
+// Decode the g-th ghost cell of box B: z slabs (full grown x-y extent), then
+// y slabs (valid k), then x slabs (valid j,k).
+__device__ __forceinline__ void ghost_cell(const BoxDesc& B, int g, int& i, int& j, int& k)
+{
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    const int nxg = B.nxg, nyg = B.nyg;
+    const int zs = NG * nxg * nyg;
+    if (g < 2 * zs) {
+        const int side = g / zs, r = g - side * zs;
+        const int kk = r / (nxg * nyg), rem = r - kk * nxg * nyg;
+        k = side ? B.hi[2] + 1 + kk : B.lo[2] - NG + kk;
+        j = B.lo[1] - NG + rem / nxg;
+        i = B.lo[0] - NG + rem % nxg;
+        return;
+    }
+    g -= 2 * zs;
+    const int ys = NG * nxg * nz;
+    if (g < 2 * ys) {
+        const int side = g / ys, r = g - side * ys;
+        const int kk = r / (NG * nxg), rem = r - kk * NG * nxg;
+        k = B.lo[2] + kk;
+        const int jj = rem / nxg;
+        j = side ? B.hi[1] + 1 + jj : B.lo[1] - NG + jj;
+        i = B.lo[0] - NG + rem % nxg;
+        return;
+    }
+    g -= 2 * ys;
+    const int xs = NG * ny * nz;
+    const int side = g / xs, r = g - side * xs;
+    const int kk = r / (NG * ny), rem = r - kk * NG * ny;
+    k = B.lo[2] + kk;
+    j = B.lo[1] + rem / NG;
+    const int ii = rem % NG;
+    i = side ? B.hi[0] + 1 + ii : B.lo[0] - NG + ii;
+    (void)nx;
+}
+
+// Ghost cells from periodic images, per-direction BC maps (AMReX FilccCell:
+// foextrap copies the boundary cell, reflect_even/odd mirror about the face
+// with sign +-1, composed when a cell is outside in several directions) and
+// the case's ext_dir fill (Exec/1_DMR/prob_bcfill.cpp:14-58).
+__global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes, const GeomDesc* __restrict__ geoms,
+                                              const int4* __restrict__ tiles, double* __restrict__ S, double time)
+{
+    const int4 t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.z) return;
+    const BoxDesc B = boxes[t.x];
+    const GeomDesc& G = geoms[B.geom];
+    int i, j, k;
+    ghost_cell(B, t.y + threadIdx.x, i, j, k);
+    const long dst = sidx(B, i, j, k);
+    int iv[3] = {i, j, k};
+    int outside[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (G.periodic[d]) {
+            const int len = G.dom_hi[d] - G.dom_lo[d] + 1;
+            if (iv[d] < G.dom_lo[d]) iv[d] += len;
+            else if (iv[d] > G.dom_hi[d]) iv[d] -= len;
+        }
+        outside[d] = iv[d] < G.dom_lo[d] ? -1 : (iv[d] > G.dom_hi[d] ? 1 : 0);
+    }
+    for (int n = 0; n < NC; ++n) {
+        int src[3] = {iv[0], iv[1], iv[2]};
+        double sign = 1.0;
+        bool user = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (!outside[d]) continue;
+            const int bc = G.bc[n][d][outside[d] > 0];
+            const int edge = outside[d] < 0 ? G.dom_lo[d] : G.dom_hi[d];
+            if (bc == BC_FOEXT) src[d] = edge;
+            else if (bc == BC_EVEN || bc == BC_ODD) {
+                src[d] = outside[d] < 0 ? 2 * edge - src[d] - 1 : 2 * edge - src[d] + 1;
+                if (bc == BC_ODD) sign = -sign;
+            } else if (bc == BC_EXT) user = true;
+        }
+        if (user) continue;
+        const int mi = (src[0] - G.dom_lo[0]) >> G.map_shift;
+        const int mj = (src[1] - G.dom_lo[1]) >> G.map_shift;
+        const int mk = (src[2] - G.dom_lo[2]) >> G.map_shift;
+        const int m = G.map[((long)mk * G.map_n[1] + mj) * G.map_n[0] + mi];
+        if (m < 0) continue;   // not covered on this level (coarse/fine boundary)
+        const BoxDesc& O = boxes[m];
+        S[dst + n * B.nptg] = sign * S[sidx(O, src[0], src[1], src[2]) + n * O.nptg];
+    }
+    if (G.probtype == 1) {
+        const double x = G.prob_lo[0] + (i + 0.5) * G.dx[0];
+        const double x0 = 1.0 / 6.0 + sqrt(1 / 3.0) + time * 10.0 / sin(M_PI * (1.0 / 3.0));
+        if (G.bc[0][1][1] == BC_EXT && j > G.dom_hi[1]) {
+            for (int n = 0; n < NC; ++n) S[dst + n * B.nptg] = x < x0 ? G.left[n] : G.right[n];
+        }
+        if (G.bc[0][1][0] == BC_EVEN && j < G.dom_lo[1]) {
+            if (x < 1.0 / 6.0)
+                for (int n = 0; n < NC; ++n) S[dst + n * B.nptg] = G.left[n];
+        }
+        if (G.bc[0][0][0] == BC_EXT && i < G.dom_lo[0]) {
+            for (int n = 0; n < NC; ++n) S[dst + n * B.nptg] = G.left[n];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CFL reduction, Source/Advance.cpp:10-79
+// ---------------------------------------------------------------------------
+__global__ void k_red_init(unsigned long long* r)
+{
+    const double big = std::numeric_limits<double>::max();
+    *r = (unsigned long long)__double_as_longlong(big);
+}
+
+__global__ __launch_bounds__(256) void k_estdt(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+                                               const double* __restrict__ S, PhysArgs ph,
+                                               unsigned long long* __restrict__ out)
+{
+    __shared__ double red[4];
+    const int4 t = tiles[blockIdx.x];
+    const BoxDesc B = boxes[t.x];
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
+    double dt = std::numeric_limits<double>::max();
+    for (int c = threadIdx.x; c < t.z; c += 256) {
+        const int g = t.y + c;
+        const int i = B.lo[0] + g % nx;
+        const int j = B.lo[1] + (g / nx) % ny;
+        const int k = B.lo[2] + g / (nx * ny);
+        const long s0 = sidx(B, i, j, k);
+        const long st = B.nptg;
+        double rho = S[s0];
+        double mx = S[s0 + st];
+        double my = S[s0 + 2 * st];
+        double mz = S[s0 + 3 * st];
+        double E = S[s0 + 4 * st];
+        double rhoinv = 1.0 / fmax(rho, ph.smallr);
+        double vx = mx * rhoinv;
+        double vy = my * rhoinv;
+        double vz = mz * rhoinv;
+        double p = fmax((ph.gamma - 1.0) * (E - 0.5 * rho * (vx * vx + vy * vy + vz * vz)), ph.smallp);
+        double cs = sqrt(ph.gamma * p * rhoinv);
+        double dtx = B.dx[0] / (fabs(vx) + cs);
+        double dty = B.dx[1] / (fabs(vy) + cs);
+        double dtz = B.dx[2] / (fabs(vz) + cs);
+        dt = fmin(dt, fmin(dtx, fmin(dty, dtz)));
+    }
+    for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+        // positive doubles order like their bit patterns
+        atomicMin(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+static PhysArgs phys_of(const ocfd_parm& p)
+{
+    PhysArgs a;
+    a.gamma = p.eos_gamma; a.smallr = p.smallr; a.smallp = p.smallp; a.Rg = p.Rg;
+    a.C_s = p.C_s; a.T_s = p.T_s; a.kOverMu = p.kOverMu; a.twoThirds = p.twoThirds;
+    return a;
+}
+
+// Source/Setup.cpp:21-34
+static const signed char scalar_bc[6]   = {BC_INT, BC_EXT, BC_FOEXT, BC_EVEN, BC_EVEN, BC_EVEN};
+static const signed char norm_vel_bc[6] = {BC_INT, BC_EXT, BC_FOEXT, BC_ODD,  BC_ODD,  BC_ODD};
+static const signed char tang_vel_bc[6] = {BC_INT, BC_EXT, BC_FOEXT, BC_EVEN, BC_EVEN, BC_ODD};
+
+template <class T>
+static int upload(T** dptr, const std::vector<T>& h)
+{
+    if (h.empty()) { *dptr = nullptr; return OCFD_OK; }
+    OCFD_HIP(hipMalloc((void**)dptr, sizeof(T) * h.size()));
+    OCFD_HIP(hipMemcpy(*dptr, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return OCFD_OK;
+}
+
+static void free_level(LevelDev& L)
+{
+    hipFree(L.d_boxes); hipFree(L.d_geom); hipFree(L.d_map);
+    hipFree(L.d_tile_x); hipFree(L.d_tile_y); hipFree(L.d_tile_z); hipFree(L.d_tile_g); hipFree(L.d_tile_c);
+    hipFree(L.d_acc); hipFree(L.d_red);
+    L = LevelDev{};
+}
+
+} // namespace ocfd
+
+using namespace ocfd;
+
+extern "C" {
+
+const char* ocfd_last_error(void) { return g_err.c_str(); }
+
+int ocfd_parm_initialize(ocfd_parm* p)
+{
+    if (!p) { set_error("ocfd_parm_initialize: null parm"); return OCFD_EINVAL; }
+    if (p->eos_gamma == 0.0) {
+        p->eos_gamma = 1.4;
+        p->eos_m = 28.97e-3;
+        p->Patm = 101325.0;
+        p->Ru = 8.31446261815324;
+        p->Pr = 0.72;
+        p->C_s = 1.458e-6;
+        p->T_s = 110.4;
+        p->smallr = 1.e-19;
+        p->smallp = 1.e-10;
+        p->eps = 1.e-40;
+        p->twoThirds = 2.0 / 3.0;
+        p->oneSix = 1.0 / 6.0;
+    }
+    p->Rg = p->Ru / p->eos_m;
+    p->cv = p->Rg / (p->eos_gamma - 1.0);
+    p->cp = p->eos_gamma * p->cv;
+    p->kOverMu = p->cp / p->Pr;
+    return OCFD_OK;
+}
+
+int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g, ocfd_level* out)
+{
+    if (!out || !g || nbox <= 0 || !lo || !hi) { set_error("ocfd_level_create: invalid arguments"); return OCFD_EINVAL; }
+    *out = nullptr;
+    ocfd_level lev = new ocfd_level_s;
+    LevelDev& L = lev->dev;
+    L.nbox = nbox;
+    long off_s = 0, off_d = 0;
+    for (int b = 0; b < nbox; ++b) {
+        BoxDesc B{};
+        for (int d = 0; d < 3; ++d) {
+            B.lo[d] = lo[3 * b + d];
+            B.hi[d] = hi[3 * b + d];
+            if (B.hi[d] < B.lo[d] || B.lo[d] < g->dom_lo[d] || B.hi[d] > g->dom_hi[d]) {
+                set_error("ocfd_level_create: box " + std::to_string(b) + " empty or outside the domain");
+                delete lev;
+                return OCFD_EINVAL;
+            }
+            B.dx[d] = g->dx[d];
+            B.dxinv[d] = 1.0 / g->dx[d];
+        }
+        B.nxg = B.hi[0] - B.lo[0] + 1 + 2 * NG;
+        B.nyg = B.hi[1] - B.lo[1] + 1 + 2 * NG;
+        B.nzg = B.hi[2] - B.lo[2] + 1 + 2 * NG;
+        B.nptg = (long)B.nxg * B.nyg * B.nzg;
+        B.nptv = (long)(B.hi[0] - B.lo[0] + 1) * (B.hi[1] - B.lo[1] + 1) * (B.hi[2] - B.lo[2] + 1);
+        B.off_s = off_s;
+        B.off_d = off_d;
+        B.geom = 0;
+        off_s += B.nptg * NC;
+        off_d += B.nptv * NC;
+        L.boxes.push_back(B);
+    }
+    L.mf_state = off_s;
+    L.mf_valid = off_d;
+
+    GeomDesc& G = L.geom;
+    for (int d = 0; d < 3; ++d) {
+        G.dom_lo[d] = g->dom_lo[d]; G.dom_hi[d] = g->dom_hi[d];
+        G.periodic[d] = g->periodic[d];
+        G.prob_lo[d] = g->prob_lo[d]; G.dx[d] = g->dx[d];
+        for (int side = 0; side < 2; ++side) {
+            int phys = side ? g->hi_bc[d] : g->lo_bc[d];
+            if (g->periodic[d]) phys = 0;
+            if (phys < 0 || phys > 5) { set_error("ocfd_level_create: bad BC code"); delete lev; return OCFD_EINVAL; }
+            for (int n = 0; n < NC; ++n) {
+                signed char bc;
+                if (n == URHO || n == UEDEN) bc = scalar_bc[phys];
+                else bc = (n - UMX) == d ? norm_vel_bc[phys] : tang_vel_bc[phys];
+                G.bc[n][d][side] = bc;
+            }
+        }
+    }
+    G.probtype = g->probtype;
+    for (int n = 0; n < NC; ++n) { G.left[n] = g->left_state[n]; G.right[n] = g->right_state[n]; }
+
+    // owner map at the coarsest granularity every box boundary is aligned to
+    int gran = 8;
+    for (int b = 0; b < nbox; ++b)
+        for (int d = 0; d < 3; ++d) {
+            while (gran > 1 && (((L.boxes[b].lo[d] - G.dom_lo[d]) % gran) || ((L.boxes[b].hi[d] + 1 - G.dom_lo[d]) % gran)))
+                gran >>= 1;
+        }
+    int sh = 0;
+    while ((1 << sh) < gran) ++sh;
+    G.map_shift = sh;
+    for (int d = 0; d < 3; ++d) G.map_n[d] = ((G.dom_hi[d] - G.dom_lo[d] + 1) + gran - 1) / gran;
+    L.owner_map.assign((size_t)G.map_n[0] * G.map_n[1] * G.map_n[2], -1);
+    for (int b = 0; b < nbox; ++b) {
+        const BoxDesc& B = L.boxes[b];
+        for (int k = (B.lo[2] - G.dom_lo[2]) >> sh; k <= (B.hi[2] - G.dom_lo[2]) >> sh; ++k)
+            for (int j = (B.lo[1] - G.dom_lo[1]) >> sh; j <= (B.hi[1] - G.dom_lo[1]) >> sh; ++j)
+                for (int i = (B.lo[0] - G.dom_lo[0]) >> sh; i <= (B.hi[0] - G.dom_lo[0]) >> sh; ++i) {
+                    int& m = L.owner_map[((size_t)k * G.map_n[1] + j) * G.map_n[0] + i];
+                    if (m >= 0) { set_error("ocfd_level_create: overlapping boxes"); delete lev; return OCFD_EINVAL; }
+                    m = b;
+                }
+    }
+
+    // work-lists
+    const int xseg = tile_xseg(), ch = tile_chunk();
+    std::vector<int4> tx, ty, tz, tg, tc;
+    for (int b = 0; b < nbox; ++b) {
+        const BoxDesc& B = L.boxes[b];
+        const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+        const int nseg = (nx + xseg - 1) / xseg, nrg = (ny * nz + 3) / 4;
+        for (int r = 0; r < nrg; ++r)
+            for (int s = 0; s < nseg; ++s) tx.push_back(make_int4(b, s, r, 0));
+        const int iseg = (nx + 63) / 64;
+        for (int c = 0; c < (ny + ch - 1) / ch; ++c)
+            for (int kg = 0; kg < (nz + 3) / 4; ++kg)
+                for (int s = 0; s < iseg; ++s) ty.push_back(make_int4(b, s, kg, c));
+        for (int c = 0; c < (nz + ch - 1) / ch; ++c)
+            for (int jg = 0; jg < (ny + 3) / 4; ++jg)
+                for (int s = 0; s < iseg; ++s) tz.push_back(make_int4(b, s, jg, c));
+        const long ngh = B.nptg - B.nptv;
+        for (long s = 0; s < ngh; s += 256) tg.push_back(make_int4(b, (int)s, (int)std::min<long>(256, ngh - s), 0));
+        for (long s = 0; s < B.nptv; s += 2048) tc.push_back(make_int4(b, (int)s, (int)std::min<long>(2048, B.nptv - s), 0));
+    }
+    int rc = OCFD_OK;
+    if ((rc = upload(&L.d_map, L.owner_map)) || (rc = upload(&L.d_tile_x, tx)) || (rc = upload(&L.d_tile_y, ty)) ||
+        (rc = upload(&L.d_tile_z, tz)) || (rc = upload(&L.d_tile_g, tg)) || (rc = upload(&L.d_tile_c, tc))) {
+        free_level(L); delete lev; return rc;
+    }
+    L.ntile_x = (int)tx.size(); L.ntile_y = (int)ty.size(); L.ntile_z = (int)tz.size();
+    L.ntile_g = (int)tg.size(); L.ntile_c = (int)tc.size();
+    G.map = L.d_map;
+    std::vector<GeomDesc> gv(1, G);
+    if ((rc = upload(&L.d_geom, gv)) || (rc = upload(&L.d_boxes, L.boxes))) { free_level(L); delete lev; return rc; }
+    if (hipMalloc((void**)&L.d_acc, sizeof(double) * L.mf_valid) != hipSuccess ||
+        hipMalloc((void**)&L.d_red, sizeof(unsigned long long)) != hipSuccess) {
+        set_error("ocfd_level_create: out of device memory");
+        free_level(L); delete lev; return OCFD_ENOMEM;
+    }
+    *out = lev;
+    return OCFD_OK;
+}
+
+int ocfd_level_destroy(ocfd_level lev)
+{
+    if (!lev) return OCFD_OK;
+    free_level(lev->dev);
+    delete lev;
+    return OCFD_OK;
+}
+
+int ocfd_level_nbox(ocfd_level lev) { return lev ? lev->dev.nbox : 0; }
+
+long ocfd_level_mf_size(ocfd_level lev, int ngrow, int ncomp)
+{
+    if (!lev) return -1;
+    long n = 0;
+    for (const BoxDesc& B : lev->dev.boxes) {
+        long c = 1;
+        for (int d = 0; d < 3; ++d) c *= (B.hi[d] - B.lo[d] + 1 + 2 * ngrow);
+        n += c * ncomp;
+    }
+    return n;
+}
+
+long ocfd_level_fab_offset(ocfd_level lev, int box, int ngrow, int ncomp)
+{
+    if (!lev || box < 0 || box >= lev->dev.nbox) return -1;
+    long n = 0;
+    for (int b = 0; b < box; ++b) {
+        const BoxDesc& B = lev->dev.boxes[b];
+        long c = 1;
+        for (int d = 0; d < 3; ++d) c *= (B.hi[d] - B.lo[d] + 1 + 2 * ngrow);
+        n += c * ncomp;
+    }
+    return n;
+}
+
+static int valid_flags(int flags)
+{
+    return (flags & ~(OCFD_VISC | OCFD_GRAVITY | OCFD_FLUX_FP32 | OCFD_PLANAR)) == 0;
+}
+
+int ocfd_compute_dsdt(ocfd_level lev, const double* S, double* dSdt, double dt, const ocfd_parm* parm, int flags,
+                      void* stream)
+{
+    (void)dt;
+    if (!lev || !S || !dSdt || !parm || !valid_flags(flags)) { set_error("ocfd_compute_dsdt: invalid arguments"); return OCFD_EINVAL; }
+    StageArgs a{};
+    a.scur = S; a.sold = nullptr; a.snew = nullptr; a.acc = dSdt;
+    a.c_cur = 1.0; a.c_old = 0.0; a.c_dt = 0.0;
+    a.flags = flags; a.ph = phys_of(*parm);
+    launch_sweeps(lev->dev, a, (hipStream_t)stream);
+    return check_launch("ocfd_compute_dsdt");
+}
+
+int ocfd_rk_stage(ocfd_level lev, const double* S_cur, const double* S_old, double* S_new, double c_cur, double c_old,
+                  double c_dt, const ocfd_parm* parm, int flags, void* stream)
+{
+    if (!lev || !S_cur || !S_new || !parm || !valid_flags(flags) || S_cur == S_new) {
+        set_error("ocfd_rk_stage: invalid arguments (S_new must differ from S_cur)");
+        return OCFD_EINVAL;
+    }
+    StageArgs a{};
+    a.scur = S_cur; a.sold = S_old; a.snew = S_new; a.acc = lev->dev.d_acc;
+    a.c_cur = c_cur; a.c_old = c_old; a.c_dt = c_dt;
+    a.flags = flags; a.ph = phys_of(*parm);
+    launch_sweeps(lev->dev, a, (hipStream_t)stream);
+    return check_launch("ocfd_rk_stage");
+}
+
+int ocfd_fill_boundary(ocfd_level lev, double* S, double time, void* stream)
+{
+    if (!lev || !S) { set_error("ocfd_fill_boundary: invalid arguments"); return OCFD_EINVAL; }
+    const LevelDev& L = lev->dev;
+    if (L.ntile_g)
+        hipLaunchKernelGGL(k_fill, dim3(L.ntile_g), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_geom, L.d_tile_g, S, time);
+    return check_launch("ocfd_fill_boundary");
+}
+
+int ocfd_estdt(ocfd_level lev, const double* S, double cfl, const ocfd_parm* parm, double* dt_out, void* stream)
+{
+    if (!lev || !S || !parm || !dt_out) { set_error("ocfd_estdt: invalid arguments"); return OCFD_EINVAL; }
+    const LevelDev& L = lev->dev;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_red_init, dim3(1), dim3(1), 0, s, L.d_red);
+    hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, S, phys_of(*parm), L.d_red);
+    int rc = check_launch("ocfd_estdt");
+    if (rc) return rc;
+    unsigned long long bits = 0;
+    OCFD_HIP(hipMemcpyAsync(&bits, L.d_red, sizeof(bits), hipMemcpyDeviceToHost, s));
+    OCFD_HIP(hipStreamSynchronize(s));
+    double m;
+    std::memcpy(&m, &bits, sizeof(m));
+    *dt_out = m * cfl;
+    return OCFD_OK;
+}
+
+int ocfd_advance(ocfd_level lev, double* A, double* B, double* C, double time, double dt, int scheme,
+                 const ocfd_parm* parm, int flags, void* stream)
+{
+    if (!lev || !A || !B || !parm || (scheme == 2 && !C) || scheme < 0 || scheme > 2) {
+        set_error("ocfd_advance: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if ((rc = ocfd_fill_boundary(lev, A, time, s))) return rc;
+    if (scheme == 0) {
+        // Advance.cpp:117-122
+        if ((rc = ocfd_rk_stage(lev, A, nullptr, B, 1.0, 0.0, dt, parm, flags, s))) return rc;
+        OCFD_HIP(hipMemcpyAsync(A, B, sizeof(double) * lev->dev.mf_state, hipMemcpyDeviceToDevice, s));
+    } else if (scheme == 1) {
+        // Advance.cpp:123-139; the last stage writes A in place (it reads A only at its own cell)
+        if ((rc = ocfd_rk_stage(lev, A, nullptr, B, 1.0, 0.0, dt, parm, flags, s))) return rc;
+        if ((rc = ocfd_fill_boundary(lev, B, time + dt, s))) return rc;
+        if ((rc = ocfd_rk_stage(lev, B, A, A, 0.5, 0.5, 0.5 * dt, parm, flags, s))) return rc;
+    } else {
+        // SSP-RK3, stage times t, t+dt, t+dt/2 (Advance.cpp:141-148)
+        if ((rc = ocfd_rk_stage(lev, A, nullptr, B, 1.0, 0.0, dt, parm, flags, s))) return rc;
+        if ((rc = ocfd_fill_boundary(lev, B, time + dt, s))) return rc;
+        if ((rc = ocfd_rk_stage(lev, B, A, C, 0.25, 0.75, 0.25 * dt, parm, flags, s))) return rc;
+        if ((rc = ocfd_fill_boundary(lev, C, time + 0.5 * dt, s))) return rc;
+        if ((rc = ocfd_rk_stage(lev, C, A, A, 2.0 / 3.0, 1.0 / 3.0, (2.0 / 3.0) * dt, parm, flags, s))) return rc;
+    }
+    return OCFD_OK;
+}
+
+} // extern "C"

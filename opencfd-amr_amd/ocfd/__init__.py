@@ -1,0 +1,227 @@
+"""Python binding of the MI355X-native per-box advance (libocfd.so, C ABI of
+include/ocfd.h).
+
+Device memory and streams come from torch (plumbing only); every computation
+runs in the HIP library.  Loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG / "libocfd.so"
+
+NCONS = 5
+NGROW = 4
+
+VISC = 1
+GRAVITY = 2
+FLUX_FP32 = 4
+PLANAR = 8
+
+# physical BC codes (Exec/*/inputs)
+INTERIOR, INFLOW, OUTFLOW, SYMMETRY, SLIPWALL, NOSLIPWALL = range(6)
+
+CASE_SOD, CASE_DMR, CASE_KH3D = 0, 1, 2
+
+
+class Parm(ctypes.Structure):
+    """struct Parm, Source/Constants.H:7-64 (non-CHEM members)."""
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "eos_gamma", "eos_m", "Patm", "cv", "cp", "Ru", "Rg", "kOverMu", "Pr", "C_s", "T_s",
+        "smallr", "smallp", "eps", "twoThirds", "oneSix")]
+
+
+class Geom(ctypes.Structure):
+    _fields_ = [
+        ("dom_lo", ctypes.c_int * 3), ("dom_hi", ctypes.c_int * 3),
+        ("prob_lo", ctypes.c_double * 3), ("dx", ctypes.c_double * 3),
+        ("periodic", ctypes.c_int * 3), ("lo_bc", ctypes.c_int * 3), ("hi_bc", ctypes.c_int * 3),
+        ("probtype", ctypes.c_int),
+        ("left_state", ctypes.c_double * 5), ("right_state", ctypes.c_double * 5),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"HIP library {LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(str(LIB_PATH))
+        vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        L.ocfd_parm_initialize.argtypes = [ctypes.POINTER(Parm)]
+        L.ocfd_level_create.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(Geom), ctypes.POINTER(vp)]
+        L.ocfd_level_destroy.argtypes = [vp]
+        L.ocfd_level_mf_size.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.ocfd_level_mf_size.restype = ctypes.c_long
+        L.ocfd_level_fab_offset.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.ocfd_level_fab_offset.restype = ctypes.c_long
+        L.ocfd_level_nbox.argtypes = [vp]
+        L.ocfd_compute_dsdt.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(Parm), ctypes.c_int, vp]
+        L.ocfd_rk_stage.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.POINTER(Parm), ctypes.c_int, vp]
+        L.ocfd_fill_boundary.argtypes = [vp, vp, ctypes.c_double, vp]
+        L.ocfd_estdt.argtypes = [vp, vp, ctypes.c_double, ctypes.POINTER(Parm), dp, vp]
+        L.ocfd_advance.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   ctypes.POINTER(Parm), ctypes.c_int, vp]
+        L.ocfd_init_state.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(Parm), ctypes.c_ulonglong]
+        L.ocfd_dmr_states.argtypes = [dp, dp]
+        L.ocfd_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().ocfd_last_error().decode()
+        raise RuntimeError(f"{what} failed (status {rc}): {msg}")
+
+
+def default_parm(**kw):
+    p = Parm()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    if "eos_gamma" not in kw:
+        p.eos_gamma = 0.0
+    check(lib().ocfd_parm_initialize(ctypes.byref(p)), "ocfd_parm_initialize")
+    for k, v in kw.items():
+        setattr(p, k, v)
+    check(lib().ocfd_parm_initialize(ctypes.byref(p)), "ocfd_parm_initialize")
+    return p
+
+
+def make_geom(n_cell, prob_lo, prob_hi, periodic=(0, 0, 0), lo_bc=(2, 2, 2), hi_bc=(2, 2, 2), probtype=0,
+              left_state=None, right_state=None):
+    g = Geom()
+    for d in range(3):
+        g.dom_lo[d] = 0
+        g.dom_hi[d] = n_cell[d] - 1
+        g.prob_lo[d] = prob_lo[d]
+        g.dx[d] = (prob_hi[d] - prob_lo[d]) / n_cell[d]
+        g.periodic[d] = periodic[d]
+        g.lo_bc[d] = lo_bc[d]
+        g.hi_bc[d] = hi_bc[d]
+    g.probtype = probtype
+    if left_state is not None:
+        for n in range(5):
+            g.left_state[n] = left_state[n]
+            g.right_state[n] = right_state[n]
+    return g
+
+
+def dmr_states():
+    left = (ctypes.c_double * 5)()
+    right = (ctypes.c_double * 5)()
+    check(lib().ocfd_dmr_states(left, right), "ocfd_dmr_states")
+    return list(left), list(right)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class Level:
+    """One AMR level's box list + geometry (drop-in for the BoxArray/Geometry
+    an EBR level advances, Source/EBR.cpp:43-65)."""
+
+    def __init__(self, boxes, geom):
+        self.boxes = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
+        self.geom = geom
+        n = len(self.boxes)
+        lo = (ctypes.c_int * (3 * n))(*[v for b in self.boxes for v in b[0]])
+        hi = (ctypes.c_int * (3 * n))(*[v for b in self.boxes for v in b[1]])
+        h = ctypes.c_void_p()
+        check(lib().ocfd_level_create(n, lo, hi, ctypes.byref(geom), ctypes.byref(h)), "ocfd_level_create")
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _lib is not None:
+            _lib.ocfd_level_destroy(h)
+            self.handle = None
+
+    @property
+    def nbox(self):
+        return len(self.boxes)
+
+    def mf_size(self, ngrow=NGROW, ncomp=NCONS):
+        return lib().ocfd_level_mf_size(self.handle, ngrow, ncomp)
+
+    def fab_offset(self, b, ngrow=NGROW, ncomp=NCONS):
+        return lib().ocfd_level_fab_offset(self.handle, b, ngrow, ncomp)
+
+    def fab_shape(self, b, ngrow=NGROW, ncomp=NCONS):
+        lo, hi = self.boxes[b]
+        return (ncomp,) + tuple(hi[d] - lo[d] + 1 + 2 * ngrow for d in (2, 1, 0))
+
+    def new_mf(self, ngrow=NGROW, ncomp=NCONS, device="cuda"):
+        import torch
+        return torch.zeros(self.mf_size(ngrow, ncomp), dtype=torch.float64, device=device)
+
+    def fab_view(self, mf, b, ngrow=NGROW, ncomp=NCONS):
+        """(ncomp, nz, ny, nx) view of box b's FAB inside MultiFab storage mf."""
+        off = self.fab_offset(b, ngrow, ncomp)
+        shape = self.fab_shape(b, ngrow, ncomp)
+        return mf[off:off + int(np.prod(shape))].view(shape)
+
+    # ---- entry points -------------------------------------------------
+    def compute_dsdt(self, S, dSdt, parm, flags=0, dt=0.0, stream=None):
+        check(lib().ocfd_compute_dsdt(self.handle, _ptr(S), _ptr(dSdt), dt, ctypes.byref(parm), flags,
+                                      _stream(stream)), "ocfd_compute_dsdt")
+
+    def rk_stage(self, S_cur, S_old, S_new, c_cur, c_old, c_dt, parm, flags=0, stream=None):
+        check(lib().ocfd_rk_stage(self.handle, _ptr(S_cur), _ptr(S_old), _ptr(S_new), c_cur, c_old, c_dt,
+                                  ctypes.byref(parm), flags, _stream(stream)), "ocfd_rk_stage")
+
+    def fill_boundary(self, S, time, stream=None):
+        check(lib().ocfd_fill_boundary(self.handle, _ptr(S), time, _stream(stream)), "ocfd_fill_boundary")
+
+    def estdt(self, S, cfl, parm, stream=None):
+        out = ctypes.c_double()
+        check(lib().ocfd_estdt(self.handle, _ptr(S), cfl, ctypes.byref(parm), ctypes.byref(out), _stream(stream)),
+              "ocfd_estdt")
+        return out.value
+
+    def advance(self, A, B, C, time, dt, scheme, parm, flags=0, stream=None):
+        check(lib().ocfd_advance(self.handle, _ptr(A), _ptr(B), _ptr(C), time, dt, scheme, ctypes.byref(parm),
+                                 flags, _stream(stream)), "ocfd_advance")
+
+    def init_state(self, kind, S, parm, seed=0):
+        check(lib().ocfd_init_state(self.handle, kind, _ptr(S), ctypes.byref(parm), seed), "ocfd_init_state")
+
+
+def run(level, A, B, C, parm, cfl, scheme=2, flags=0, stop_time=-1.0, nsteps=-1, time=0.0, dt=0.0, stream=None):
+    """Single-level time loop of the reference driver: initial dt =
+    estTimeStep (Source/EBR.cpp:189-229,561-563), then per step computeNewDt
+    with change_max 1.1 and the stop_time clip (Source/EBR.cpp:231-306)."""
+    if dt <= 0.0:
+        dt = level.estdt(A, cfl, parm, stream)
+        eps = 0.001 * dt
+        if stop_time >= 0.0 and time + dt > stop_time - eps:
+            dt = stop_time - time
+    steps = 0
+    while (nsteps < 0 or steps < nsteps) and (stop_time < 0.0 or time < stop_time):
+        level.advance(A, B, C, time, dt, scheme, parm, flags, stream)
+        time += dt
+        steps += 1
+        est = level.estdt(A, cfl, parm, stream)
+        dt_new = min(est, 1.1 * dt)
+        eps = 0.001 * dt_new
+        if stop_time >= 0.0 and time + dt_new > stop_time - eps:
+            dt_new = stop_time - time
+        dt = dt_new
+    return steps, time, dt

@@ -1,0 +1,870 @@
+// OpenCFD-AMR per-box advance -- CPU ORACLE (test infrastructure only).
+//
+// This file is the checker, never the product: only tests/, the smoke check in
+// __graft_entry__.py and the cpu_baseline leg of bench.py may load it.  It is a
+// plain FP64 C++/OpenMP restatement of the reference algorithm, written to follow
+// the reference's operation order so that the HIP path can be compared against
+// it at 1e-10 relative.  Build: oracle/Makefile (g++ -O2 -fopenmp
+// -ffp-contract=off, so no FMA contraction changes the reference's rounding).
+//
+// Each function cites the reference file:line it restates.  How each is pinned
+// independently of this file (closed forms, invariants, textbook solutions and
+// the reference's own known answers) is listed in tests/test_oracle_*.py.
+//
+// Layout follows amrex::Array4: component-major SoA, i fastest, indices are
+// global cell indices, a FAB covers an inclusive box [lo, hi].
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdlib>
+#include <algorithm>
+#include <limits>
+#include <vector>
+
+extern "C" {
+
+// Source/Constants.H:7-64 (non-CHEM members, same order)
+typedef struct orc_parm {
+    double eos_gamma, eos_m, Patm, cv, cp, Ru, Rg, kOverMu, Pr, C_s, T_s,
+           smallr, smallp, eps, twoThirds, oneSix;
+} orc_parm;
+
+typedef struct orc_fab {
+    double* p;
+    int lo[3];
+    int hi[3];
+    int ncomp;
+} orc_fab;
+
+// Source/Constants.H:9-29 defaults + Parm::Initialize, Source/Constants.cpp:7-11
+void orc_parm_default(orc_parm* p)
+{
+    p->eos_gamma = 1.4;
+    p->eos_m = 28.97e-3;
+    p->Patm = 101325.0;
+    p->Ru = 8.31446261815324;
+    p->Pr = 0.72;
+    p->C_s = 1.458e-6;
+    p->T_s = 110.4;
+    p->smallr = 1.e-19;
+    p->smallp = 1.e-10;
+    p->eps = 1.e-40;
+    p->twoThirds = 2.0 / 3.0;
+    p->oneSix = 1.0 / 6.0;
+    p->Rg = p->Ru / p->eos_m;
+    p->cv = p->Rg / (p->eos_gamma - 1.0);
+    p->cp = p->eos_gamma * p->cv;
+    p->kOverMu = p->cp / p->Pr;
+}
+
+// Parm::Initialize after the caller set eos_gamma/eos_m/Pr (EBR.cpp:522-528)
+void orc_parm_initialize(orc_parm* p)
+{
+    p->Rg = p->Ru / p->eos_m;
+    p->cv = p->Rg / (p->eos_gamma - 1.0);
+    p->cp = p->eos_gamma * p->cv;
+    p->kOverMu = p->cp / p->Pr;
+}
+
+} // extern "C"
+
+namespace {
+
+enum { URHO = 0, UMX = 1, UMY = 2, UMZ = 3, UEDEN = 4, NCONS = 5 };   // IndexDefines.H:4-9
+enum { QRHO = 0, QU = 1, QV = 2, QW = 3, QPRES = 4, QT = 5, QC = 6, NPRIM = 7 }; // IndexDefines.H:11-22
+
+struct View {
+    double* p;
+    int lo[3], hi[3];
+    long nx, ny, nz;
+    int nc;
+    View() : p(nullptr), nx(0), ny(0), nz(0), nc(0) {}
+    View(const orc_fab& f) : p(f.p), nc(f.ncomp) {
+        for (int d = 0; d < 3; ++d) { lo[d] = f.lo[d]; hi[d] = f.hi[d]; }
+        nx = hi[0] - lo[0] + 1; ny = hi[1] - lo[1] + 1; nz = hi[2] - lo[2] + 1;
+    }
+    View(double* ptr, const int* l, const int* h, int ncomp) : p(ptr), nc(ncomp) {
+        for (int d = 0; d < 3; ++d) { lo[d] = l[d]; hi[d] = h[d]; }
+        nx = hi[0] - lo[0] + 1; ny = hi[1] - lo[1] + 1; nz = hi[2] - lo[2] + 1;
+    }
+    inline double& operator()(int i, int j, int k, int n) const {
+        return p[(((long)n * nz + (k - lo[2])) * ny + (j - lo[1])) * nx + (i - lo[0])];
+    }
+    long npts() const { return nx * ny * nz; }
+};
+
+struct Box {
+    int lo[3], hi[3];
+};
+
+inline Box grow(const Box& b, int g) {
+    Box r = b;
+    for (int d = 0; d < 3; ++d) { r.lo[d] -= g; r.hi[d] += g; }
+    return r;
+}
+
+inline Box surrounding_nodes(const Box& b, int d) {
+    Box r = b;
+    r.hi[d] += 1;
+    return r;
+}
+
+// Source/kernel/Kernels.H:13-40
+inline void c2prim(int i, int j, int k, const View& u, const View& q, const orc_parm& parm)
+{
+    double rho = std::max(u(i, j, k, URHO), parm.smallr);
+    double rhoinv = 1.0 / rho;
+    double ux = u(i, j, k, UMX) * rhoinv;
+    double uy = u(i, j, k, UMY) * rhoinv;
+    double uz = u(i, j, k, UMZ) * rhoinv;
+    double kineng = 0.5 * (ux * ux + uy * uy + uz * uz);
+    double ei = u(i, j, k, UEDEN) - rho * kineng;
+    double p = std::max((parm.eos_gamma - 1.0) * ei, parm.smallp);
+    double T = p / (rho * parm.Rg);
+    q(i, j, k, QRHO) = rho;
+    q(i, j, k, QU) = ux;
+    q(i, j, k, QV) = uy;
+    q(i, j, k, QW) = uz;
+    q(i, j, k, QPRES) = p;
+    q(i, j, k, QT) = T;
+    q(i, j, k, QC) = std::sqrt(parm.eos_gamma * parm.Rg * T);
+}
+
+// Steger-Warming split, Source/kernel/FluxSplit.H:11-70 (x), :72-131 (y), :133-193 (z).
+// The three directions differ only in which velocity is normal; the sums
+// vvc1/vvc2 keep the reference's x,y,z term order with the normal term replaced.
+inline void sw_point(int dir, const double* qv, double* fpv, double* fmv, double gamma)
+{
+    double d = qv[QRHO], u = qv[QU], v = qv[QV], w = qv[QW], c = qv[QC];
+    double tmp1 = 1.0 / (2.0 * gamma);
+    double tmp2 = 2.0 * (gamma - 1.0);
+    double tmp3 = (3.0 - gamma) / (2.0 * (gamma - 1.0));
+    double un = dir == 0 ? u : (dir == 1 ? v : w);
+    double uc1 = un - c;
+    double uc2 = un + c;
+    double vvc1, vvc2;
+    if (dir == 0) {
+        vvc1 = 0.5 * (uc1 * uc1 + v * v + w * w);
+        vvc2 = 0.5 * (uc2 * uc2 + v * v + w * w);
+    } else if (dir == 1) {
+        vvc1 = 0.5 * (u * u + uc1 * uc1 + w * w);
+        vvc2 = 0.5 * (u * u + uc2 * uc2 + w * w);
+    } else {
+        vvc1 = 0.5 * (u * u + v * v + uc1 * uc1);
+        vvc2 = 0.5 * (u * u + v * v + uc2 * uc2);
+    }
+    double v2 = (gamma - 1) * (u * u + v * v + w * w);
+    double w2 = tmp3 * c * c;
+    double E1 = un, E2 = un - c, E3 = un + c;
+    double E1P = (E1 + std::fabs(E1)) * 0.5;
+    double E2P = (E2 + std::fabs(E2)) * 0.5;
+    double E3P = (E3 + std::fabs(E3)) * 0.5;
+    double E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
+    double tmp0 = d * tmp1;
+    const double vel[3] = {u, v, w};
+    for (int side = 0; side < 2; ++side) {
+        double a = side == 0 ? E1P : E1M;
+        double b = side == 0 ? E2P : E2M;
+        double e = side == 0 ? E3P : E3M;
+        double* f = side == 0 ? fpv : fmv;
+        f[URHO] = tmp0 * (tmp2 * a + b + e);
+        for (int m = 0; m < 3; ++m) {
+            if (m == dir) f[UMX + m] = tmp0 * (tmp2 * a * un + b * uc1 + e * uc2);
+            else          f[UMX + m] = tmp0 * vel[m] * (tmp2 * a + b + e);
+        }
+        f[UEDEN] = tmp0 * (a * v2 + b * vvc1 + e * vvc2 + w2 * (b + e));
+    }
+}
+
+// WENO-Z5 upwind-biased value from f+ at cells (f-3..f+1) of face f,
+// Source/kernel/Reconstruction.H:83-110.
+inline double weno_plus(double V1, double V2, double V3, double V4, double V5)
+{
+    double eps = 1e-6;
+    double s11 = (13.0/12.0)*(V1-2.0*V2+V3)*(V1-2.0*V2+V3) + 0.25*(V1-4.0*V2+3.0*V3)*(V1-4.0*V2+3.0*V3);
+    double s22 = (13.0/12.0)*(V2-2.0*V3+V4)*(V2-2.0*V3+V4) + 0.25*(V2-V4)*(V2-V4);
+    double s33 = (13.0/12.0)*(V3-2.0*V4+V5)*(V3-2.0*V4+V5) + 0.25*(3.0*V3-4.0*V4+V5)*(3.0*V3-4.0*V4+V5);
+    double s55 = std::fabs(s11 - s33);
+    s11 = 1.0 + (s55/(eps+s11))*(s55/(eps+s11));
+    s22 = 1.0 + (s55/(eps+s22))*(s55/(eps+s22));
+    s33 = 1.0 + (s55/(eps+s33))*(s55/(eps+s33));
+    double a1 = s11;
+    double a2 = 6.0*s22;
+    double a3 = 3.0*s33;
+    double invsum = 1.0/(a1+a2+a3);
+    double v1 = 2.0*V1-7.0*V2+11.0*V3;
+    double v2 = -V2+5.0*V3+2.0*V4;
+    double v3 = 2.0*V3+5.0*V4-V5;
+    return (1.0/6.0)*invsum*(a1*v1+a2*v2+a3*v3);
+}
+
+// WENO-Z5 value from f- at cells (f-2..f+2) of face f,
+// Source/kernel/Reconstruction.H:112-137.
+inline double weno_minus(double V1, double V2, double V3, double V4, double V5)
+{
+    double eps = 1e-6;
+    double s11 = (13.0/12.0)*(V5-2.0*V4+V3)*(V5-2.0*V4+V3) + 0.25*(V5-4.0*V4+3.0*V3)*(V5-4.0*V4+3.0*V3);
+    double s22 = (13.0/12.0)*(V2-2.0*V3+V4)*(V2-2.0*V3+V4) + 0.25*(V4-V2)*(V4-V2);
+    double s33 = (13.0/12.0)*(V3-2.0*V2+V1)*(V3-2.0*V2+V1) + 0.25*(3.0*V3-4.0*V2+V1)*(3.0*V3-4.0*V2+V1);
+    double s55 = std::fabs(s11 - s33);
+    s11 = 1.0 + (s55/(eps+s11))*(s55/(eps+s11));
+    s22 = 1.0 + (s55/(eps+s22))*(s55/(eps+s22));
+    s33 = 1.0 + (s55/(eps+s33))*(s55/(eps+s33));
+    double a1 = s11;
+    double a2 = 6.0*s22;
+    double a3 = 3.0*s33;
+    double invsum = 1.0/(a1+a2+a3);
+    double v1 = 11.0*V3-7.0*V4+2.0*V5;
+    double v2 = -V4+5.0*V3+2.0*V2;
+    double v3 = 2.0*V3+5.0*V2-V1;
+    return (1.0/6.0)*invsum*(a1*v1+a2*v2+a3*v3);
+}
+
+inline void sutherland(double T, const orc_parm& parm, double& mu)
+{
+    mu = parm.C_s*T*std::sqrt(T)/(T+parm.T_s);
+}
+
+// Source/kernel/Diffusion.H:7-47
+inline void visc_x(int i, int j, int k, const View& q, const View& fx, const double* dxinv, const orc_parm& parm)
+{
+    double Tr = q(i,j,k,QT);
+    double Tl = q(i-1,j,k,QT);
+    double mutl, mutr;
+    sutherland(Tl, parm, mutl);
+    sutherland(Tr, parm, mutr);
+    double ktl = mutl * parm.kOverMu;
+    double ktr = mutr * parm.kOverMu;
+    double dTdx = (Tr-Tl)*dxinv[0];
+    double dudx = (q(i,j,k,QU)-q(i-1,j,k,QU))*dxinv[0];
+    double dvdx = (q(i,j,k,QV)-q(i-1,j,k,QV))*dxinv[0];
+    double dwdx = (q(i,j,k,QW)-q(i-1,j,k,QW))*dxinv[0];
+    double dudy = (q(i,j+1,k,QU)+q(i-1,j+1,k,QU)-q(i,j-1,k,QU)-q(i-1,j-1,k,QU))*(0.25*dxinv[1]);
+    double dvdy = (q(i,j+1,k,QV)+q(i-1,j+1,k,QV)-q(i,j-1,k,QV)-q(i-1,j-1,k,QV))*(0.25*dxinv[1]);
+    double dudz = (q(i,j,k+1,QU)+q(i-1,j,k+1,QU)-q(i,j,k-1,QU)-q(i-1,j,k-1,QU))*(0.25*dxinv[2]);
+    double dwdz = (q(i,j,k+1,QW)+q(i-1,j,k+1,QW)-q(i,j,k-1,QW)-q(i-1,j,k-1,QW))*(0.25*dxinv[2]);
+    double divu = dudx + dvdy + dwdz;
+    double muf = 0.5*(mutl+mutr);
+    double tauxx = muf*(2.0*dudx-parm.twoThirds*divu);
+    double tauxy = muf*(dudy+dvdx);
+    double tauxz = muf*(dudz+dwdx);
+    fx(i,j,k,UMX)   += -tauxx;
+    fx(i,j,k,UMY)   += -tauxy;
+    fx(i,j,k,UMZ)   += -tauxz;
+    fx(i,j,k,UEDEN) += -0.5*((q(i,j,k,QU)+q(i-1,j,k,QU))*tauxx
+                            +(q(i,j,k,QV)+q(i-1,j,k,QV))*tauxy
+                            +(q(i,j,k,QW)+q(i-1,j,k,QW))*tauxz
+                            +(ktl+ktr)*dTdx);
+}
+
+// Source/kernel/Diffusion.H:49-89
+inline void visc_y(int i, int j, int k, const View& q, const View& fy, const double* dxinv, const orc_parm& parm)
+{
+    double Tr = q(i,j,k,QT);
+    double Tl = q(i,j-1,k,QT);
+    double mutl, mutr;
+    sutherland(Tl, parm, mutl);
+    sutherland(Tr, parm, mutr);
+    double ktl = mutl * parm.kOverMu;
+    double ktr = mutr * parm.kOverMu;
+    double dTdy = (Tr-Tl)*dxinv[1];
+    double dudy = (q(i,j,k,QU)-q(i,j-1,k,QU))*dxinv[1];
+    double dvdy = (q(i,j,k,QV)-q(i,j-1,k,QV))*dxinv[1];
+    double dwdy = (q(i,j,k,QW)-q(i,j-1,k,QW))*dxinv[1];
+    double dudx = (q(i+1,j,k,QU)+q(i+1,j-1,k,QU)-q(i-1,j,k,QU)-q(i-1,j-1,k,QU))*(0.25*dxinv[0]);
+    double dvdx = (q(i+1,j,k,QV)+q(i+1,j-1,k,QV)-q(i-1,j,k,QV)-q(i-1,j-1,k,QV))*(0.25*dxinv[0]);
+    double dvdz = (q(i,j,k+1,QV)+q(i,j-1,k+1,QV)-q(i,j,k-1,QV)-q(i,j-1,k-1,QV))*(0.25*dxinv[2]);
+    double dwdz = (q(i,j,k+1,QW)+q(i,j-1,k+1,QW)-q(i,j,k-1,QW)-q(i,j-1,k-1,QW))*(0.25*dxinv[2]);
+    double divu = dudx + dvdy + dwdz;
+    double muf = 0.5*(mutl+mutr);
+    double tauyy = muf*(2.0*dvdy-parm.twoThirds*divu);
+    double tauxy = muf*(dudy+dvdx);
+    double tauyz = muf*(dwdy+dvdz);
+    fy(i,j,k,UMX)   += -tauxy;
+    fy(i,j,k,UMY)   += -tauyy;
+    fy(i,j,k,UMZ)   += -tauyz;
+    fy(i,j,k,UEDEN) += -0.5*((q(i,j,k,QU)+q(i,j-1,k,QU))*tauxy
+                            +(q(i,j,k,QV)+q(i,j-1,k,QV))*tauyy
+                            +(q(i,j,k,QW)+q(i,j-1,k,QW))*tauyz
+                            +(ktl+ktr)*dTdy);
+}
+
+// Source/kernel/Diffusion.H:91-131
+inline void visc_z(int i, int j, int k, const View& q, const View& fz, const double* dxinv, const orc_parm& parm)
+{
+    double Tr = q(i,j,k,QT);
+    double Tl = q(i,j,k-1,QT);
+    double mutl, mutr;
+    sutherland(Tl, parm, mutl);
+    sutherland(Tr, parm, mutr);
+    double ktl = mutl * parm.kOverMu;
+    double ktr = mutr * parm.kOverMu;
+    double dTdz = (Tr-Tl)*dxinv[2];
+    double dudz = (q(i,j,k,QU)-q(i,j,k-1,QU))*dxinv[2];
+    double dvdz = (q(i,j,k,QV)-q(i,j,k-1,QV))*dxinv[2];
+    double dwdz = (q(i,j,k,QW)-q(i,j,k-1,QW))*dxinv[2];
+    double dudx = (q(i+1,j,k,QU)+q(i+1,j,k-1,QU)-q(i-1,j,k,QU)-q(i-1,j,k-1,QU))*(0.25*dxinv[0]);
+    double dwdx = (q(i+1,j,k,QW)+q(i+1,j,k-1,QW)-q(i-1,j,k,QW)-q(i-1,j,k-1,QW))*(0.25*dxinv[0]);
+    double dvdy = (q(i,j+1,k,QV)+q(i,j+1,k-1,QV)-q(i,j-1,k,QV)-q(i,j-1,k-1,QV))*(0.25*dxinv[1]);
+    double dwdy = (q(i,j+1,k,QW)+q(i,j+1,k-1,QW)-q(i,j-1,k,QW)-q(i,j-1,k-1,QW))*(0.25*dxinv[1]);
+    double divu = dudx + dvdy + dwdz;
+    double muf = 0.5*(mutl+mutr);
+    double tauxz = muf*(dudz+dwdx);
+    double tauyz = muf*(dvdz+dwdy);
+    double tauzz = muf*(2.0*dwdz-parm.twoThirds*divu);
+    fz(i,j,k,UMX)   += -tauxz;
+    fz(i,j,k,UMY)   += -tauyz;
+    fz(i,j,k,UMZ)   += -tauzz;
+    fz(i,j,k,UEDEN) += -0.5*((q(i,j,k,QU)+q(i,j,k-1,QU))*tauxz
+                            +(q(i,j,k,QV)+q(i,j,k-1,QV))*tauyz
+                            +(q(i,j,k,QW)+q(i,j,k-1,QW))*tauzz
+                            +(ktl+ktr)*dTdz);
+}
+
+template <class F>
+inline void for_box(const Box& b, F&& f)
+{
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+        for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+            for (int i = b.lo[0]; i <= b.hi[0]; ++i)
+                f(i, j, k);
+}
+
+constexpr int NUM_GROW = 4;  // EBR.H:207
+
+// Regular-box branch of EBR::compute_dSdt, Source/Advance.cpp:219-342:
+// c2prim on grow(bx,4); per direction SW split on grow(bx,4), WENO-Z5 on the
+// face box, optional viscous flux; divop on bx; optional gravity.  Temporaries
+// live in host memory, as the reference's async-arena FABs do on the device.
+void compute_dsdt_box(const Box& bx, const View& s, const View& dsdt, const double* dxinv,
+                      int do_visc, int do_gravity, const orc_parm& parm, double* face_out[3])
+{
+    const Box bxg = grow(bx, NUM_GROW);
+    std::vector<double> qbuf((size_t)NPRIM * (bxg.hi[0]-bxg.lo[0]+1) * (bxg.hi[1]-bxg.lo[1]+1) * (bxg.hi[2]-bxg.lo[2]+1));
+    std::vector<double> fpbuf((size_t)NCONS * (bxg.hi[0]-bxg.lo[0]+1) * (bxg.hi[1]-bxg.lo[1]+1) * (bxg.hi[2]-bxg.lo[2]+1));
+    std::vector<double> fmbuf(fpbuf.size());
+    View q(qbuf.data(), bxg.lo, bxg.hi, NPRIM);
+    View fp(fpbuf.data(), bxg.lo, bxg.hi, NCONS);
+    View fm(fmbuf.data(), bxg.lo, bxg.hi, NCONS);
+
+    Box fb[3];
+    std::vector<double> fbuf[3];
+    View fl[3];
+    for (int d = 0; d < 3; ++d) {
+        fb[d] = surrounding_nodes(bx, d);
+        fbuf[d].assign((size_t)NCONS * (fb[d].hi[0]-fb[d].lo[0]+1) * (fb[d].hi[1]-fb[d].lo[1]+1) * (fb[d].hi[2]-fb[d].lo[2]+1), 0.0);
+        fl[d] = View(fbuf[d].data(), fb[d].lo, fb[d].hi, NCONS);
+    }
+
+    for_box(bxg, [&](int i, int j, int k) { c2prim(i, j, k, s, q, parm); });
+
+    for (int d = 0; d < 3; ++d) {
+        for_box(bxg, [&](int i, int j, int k) {
+            double qv[NPRIM], a[NCONS], b[NCONS];
+            for (int n = 0; n < NPRIM; ++n) qv[n] = q(i, j, k, n);
+            sw_point(d, qv, a, b, parm.eos_gamma);
+            for (int n = 0; n < NCONS; ++n) { fp(i, j, k, n) = a[n]; fm(i, j, k, n) = b[n]; }
+        });
+        const int di = d == 0, dj = d == 1, dk = d == 2;
+        const View F = fl[d];
+        for_box(fb[d], [&](int i, int j, int k) {
+            for (int n = 0; n < NCONS; ++n) {
+                double pv = weno_plus(fp(i-3*di, j-3*dj, k-3*dk, n), fp(i-2*di, j-2*dj, k-2*dk, n),
+                                      fp(i-di, j-dj, k-dk, n), fp(i, j, k, n), fp(i+di, j+dj, k+dk, n));
+                double mv = weno_minus(fm(i-2*di, j-2*dj, k-2*dk, n), fm(i-di, j-dj, k-dk, n),
+                                       fm(i, j, k, n), fm(i+di, j+dj, k+dk, n), fm(i+2*di, j+2*dj, k+2*dk, n));
+                F(i, j, k, n) = pv + mv;
+            }
+        });
+        if (do_visc) {
+            for_box(fb[d], [&](int i, int j, int k) {
+                if (d == 0) visc_x(i, j, k, q, F, dxinv, parm);
+                else if (d == 1) visc_y(i, j, k, q, F, dxinv, parm);
+                else visc_z(i, j, k, q, F, dxinv, parm);
+            });
+        }
+    }
+
+    // divop, Source/kernel/Kernels.H:84-97
+    for_box(bx, [&](int i, int j, int k) {
+        for (int n = 0; n < NCONS; ++n) {
+            dsdt(i, j, k, n) = dxinv[0] * (fl[0](i, j, k, n) - fl[0](i+1, j, k, n))
+                +              dxinv[1] * (fl[1](i, j, k, n) - fl[1](i, j+1, k, n))
+                +              dxinv[2] * (fl[2](i, j, k, n) - fl[2](i, j, k+1, n));
+        }
+    });
+
+    // gravity, Source/Advance.cpp:331-342
+    if (do_gravity) {
+        const double g = -9.8;
+        for_box(bx, [&](int i, int j, int k) {
+            dsdt(i, j, k, UMZ) += g * s(i, j, k, URHO);
+            dsdt(i, j, k, UEDEN) += g * s(i, j, k, UMZ);
+        });
+    }
+
+    if (face_out) {
+        for (int d = 0; d < 3; ++d)
+            if (face_out[d]) std::memcpy(face_out[d], fbuf[d].data(), fbuf[d].size() * sizeof(double));
+    }
+}
+
+// ebr_estdt, Source/Advance.cpp:10-52 (non-CHEM)
+double estdt_box(const Box& bx, const View& state, const double* dx, const orc_parm& parm)
+{
+    double dt = std::numeric_limits<double>::max();
+#pragma omp parallel for collapse(2) reduction(min:dt)
+    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+        for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+            for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+                double rho = state(i,j,k,URHO);
+                double mx  = state(i,j,k,UMX);
+                double my  = state(i,j,k,UMY);
+                double mz  = state(i,j,k,UMZ);
+                double E   = state(i,j,k,UEDEN);
+                double rhoinv = 1.0/std::max(rho,parm.smallr);
+                double vx = mx*rhoinv;
+                double vy = my*rhoinv;
+                double vz = mz*rhoinv;
+                double p = std::max((parm.eos_gamma-1.0)*(E-0.5*rho*(vx*vx+vy*vy+vz*vz)), parm.smallp);
+                double cs = std::sqrt(parm.eos_gamma*p*rhoinv);
+                double dtx = dx[0]/(std::fabs(vx)+cs);
+                double dty = dx[1]/(std::fabs(vy)+cs);
+                double dtz = dx[2]/(std::fabs(vz)+cs);
+                dt = std::min(dt,std::min(dtx,std::min(dty,dtz)));
+            }
+    return dt;
+}
+
+} // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// point-kernel entry points used by the pinning tests
+// ---------------------------------------------------------------------------
+
+// c2prim on n cells; u: n x 5 (row-major), q: n x 7
+void orc_c2prim_many(long n, const double* u, double* q, const orc_parm* parm)
+{
+    for (long m = 0; m < n; ++m) {
+        double uu[NCONS];
+        for (int c = 0; c < NCONS; ++c) uu[c] = u[m * NCONS + c];
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        // a one-cell FAB per component stride 1
+        double ucell[NCONS], qcell[NPRIM];
+        for (int c = 0; c < NCONS; ++c) ucell[c] = uu[c];
+        View uv(ucell, lo, hi, NCONS), qv(qcell, lo, hi, NPRIM);
+        c2prim(0, 0, 0, uv, qv, *parm);
+        for (int c = 0; c < NPRIM; ++c) q[m * NPRIM + c] = qcell[c];
+    }
+}
+
+// SW split on n prim states; q: n x 7, fp/fm: n x 5
+void orc_flux_split_many(int dir, long n, const double* q, double* fp, double* fm, const orc_parm* parm)
+{
+    for (long m = 0; m < n; ++m) sw_point(dir, q + m * NPRIM, fp + m * NCONS, fm + m * NCONS, parm->eos_gamma);
+}
+
+// WENO-Z5 on n stencils of 5 values: side 0 = f+ stencil (cells f-3..f+1),
+// side 1 = f- stencil (cells f-2..f+2)
+void orc_weno_many(int side, long n, const double* v, double* out)
+{
+    for (long m = 0; m < n; ++m) {
+        const double* s = v + 5 * m;
+        out[m] = side == 0 ? weno_plus(s[0], s[1], s[2], s[3], s[4]) : weno_minus(s[0], s[1], s[2], s[3], s[4]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// box-level entry points
+// ---------------------------------------------------------------------------
+
+// EBR::compute_dSdt regular path for one box (Advance.cpp:219-342).
+// S covers grow(bx,4) (or more); dSdt covers bx (or more).  face[d] (optional)
+// receives the face fluxes on surroundingNodes(bx,d), 5 comps.
+void orc_compute_dsdt_box(const int* lo, const int* hi, const orc_fab* S, const orc_fab* dSdt,
+                          const double* dxinv, int do_visc, int do_gravity, const orc_parm* parm,
+                          double* fx, double* fy, double* fz)
+{
+    Box bx;
+    for (int d = 0; d < 3; ++d) { bx.lo[d] = lo[d]; bx.hi[d] = hi[d]; }
+    double* faces[3] = {fx, fy, fz};
+    compute_dsdt_box(bx, View(*S), View(*dSdt), dxinv, do_visc, do_gravity, *parm,
+                     (fx || fy || fz) ? faces : nullptr);
+}
+
+// EBR::estTimeStep (Advance.cpp:61-79): cfl * min over boxes of ebr_estdt.
+double orc_estdt(int nbox, const int* lo, const int* hi, const orc_fab* S, const double* dx,
+                 double cfl, const orc_parm* parm)
+{
+    double dt = std::numeric_limits<double>::max();
+    for (int b = 0; b < nbox; ++b) {
+        Box bx;
+        for (int d = 0; d < 3; ++d) { bx.lo[d] = lo[3*b+d]; bx.hi[d] = hi[3*b+d]; }
+        dt = std::min(dt, estdt_box(bx, View(S[b]), dx, *parm));
+    }
+    return dt * cfl;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ghost fill (FillPatch on one level), RK drivers and case initial data.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+// Physical BC codes of ebr.lo_bc / ebr.hi_bc (Exec/*/inputs:10-14):
+// 0 Interior, 1 Inflow, 2 Outflow, 3 Symmetry, 4 SlipWall, 5 NoSlipWall.
+typedef struct orc_geom {
+    int dom_lo[3], dom_hi[3];
+    double prob_lo[3];
+    double dx[3];
+    int periodic[3];
+    int lo_bc[3], hi_bc[3];
+    int probtype;          // 0: no user fill, 1: double Mach reflection inflow fill
+    double left_state[5];  // DMR post-shock state (Exec/1_DMR/prob.cpp:28-40)
+    double right_state[5]; // DMR pre-shock state
+} orc_geom;
+
+}
+
+namespace {
+
+enum BCType { INT_DIR = 0, EXT_DIR = 3, FOEXTRAP = 2, REFLECT_EVEN = 1, REFLECT_ODD = -1 };
+
+// Source/Setup.cpp:21-34
+const int scalar_bc[6]   = {INT_DIR, EXT_DIR, FOEXTRAP, REFLECT_EVEN, REFLECT_EVEN, REFLECT_EVEN};
+const int norm_vel_bc[6] = {INT_DIR, EXT_DIR, FOEXTRAP, REFLECT_ODD,  REFLECT_ODD,  REFLECT_ODD};
+const int tang_vel_bc[6] = {INT_DIR, EXT_DIR, FOEXTRAP, REFLECT_EVEN, REFLECT_EVEN, REFLECT_ODD};
+
+// BCType of component n on side (d, hi) -- Setup.cpp:36-92 and :201-205
+inline int comp_bc(const orc_geom& g, int n, int d, int hi)
+{
+    int phys = hi ? g.hi_bc[d] : g.lo_bc[d];
+    if (n == URHO || n == UEDEN) return scalar_bc[phys];
+    int veldir = n - UMX;
+    return veldir == d ? norm_vel_bc[phys] : tang_vel_bc[phys];
+}
+
+struct Level {
+    int nbox;
+    std::vector<Box> boxes;
+    std::vector<View> fabs;
+    int find(int i, int j, int k, int hint) const {
+        if (hint >= 0) {
+            const Box& b = boxes[hint];
+            if (i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] && k >= b.lo[2] && k <= b.hi[2]) return hint;
+        }
+        for (int m = 0; m < nbox; ++m) {
+            const Box& b = boxes[m];
+            if (i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] && k >= b.lo[2] && k <= b.hi[2]) return m;
+        }
+        return -1;
+    }
+};
+
+// Fill the ghost cells of every FAB in lev from the valid cells of the level:
+// periodic images, then per-direction BC maps (AMReX FilccCell semantics:
+// foextrap copies the boundary cell, reflect_even/odd mirror about the face
+// with sign +1/-1; a cell outside in several directions composes the maps),
+// then the case's ext_dir fill (Exec/1_DMR/prob_bcfill.cpp:14-58).
+int fill_level(const Level& lev, const orc_geom& g, double time)
+{
+    int bad = 0;
+    for (int b = 0; b < lev.nbox; ++b) {
+        const View& f = lev.fabs[b];
+        const Box& vb = lev.boxes[b];
+#pragma omp parallel for collapse(2) reduction(+:bad)
+        for (int k = f.lo[2]; k <= f.hi[2]; ++k)
+            for (int j = f.lo[1]; j <= f.hi[1]; ++j) {
+                int hint = b;
+                for (int i = f.lo[0]; i <= f.hi[0]; ++i) {
+                    if (i >= vb.lo[0] && i <= vb.hi[0] && j >= vb.lo[1] && j <= vb.hi[1] && k >= vb.lo[2] && k <= vb.hi[2])
+                        continue;
+                    int iv[3] = {i, j, k};
+                    // periodic wrap
+                    for (int d = 0; d < 3; ++d) {
+                        if (!g.periodic[d]) continue;
+                        int len = g.dom_hi[d] - g.dom_lo[d] + 1;
+                        while (iv[d] < g.dom_lo[d]) iv[d] += len;
+                        while (iv[d] > g.dom_hi[d]) iv[d] -= len;
+                    }
+                    int outside[3];
+                    bool any_out = false, ext = false;
+                    for (int d = 0; d < 3; ++d) {
+                        outside[d] = iv[d] < g.dom_lo[d] ? -1 : (iv[d] > g.dom_hi[d] ? 1 : 0);
+                        if (outside[d]) any_out = true;
+                    }
+                    for (int n = 0; n < NCONS; ++n) {
+                        int src[3] = {iv[0], iv[1], iv[2]};
+                        double sign = 1.0;
+                        bool user = false;
+                        for (int d = 0; d < 3; ++d) {
+                            if (!outside[d]) continue;
+                            int bc = comp_bc(g, n, d, outside[d] > 0);
+                            int edge = outside[d] < 0 ? g.dom_lo[d] : g.dom_hi[d];
+                            if (bc == FOEXTRAP) src[d] = edge;
+                            else if (bc == REFLECT_EVEN || bc == REFLECT_ODD) {
+                                src[d] = outside[d] < 0 ? 2 * edge - src[d] - 1 : 2 * edge - src[d] + 1;
+                                if (bc == REFLECT_ODD) sign = -sign;
+                            } else if (bc == EXT_DIR) user = true;
+                        }
+                        if (user) { ext = true; continue; }
+                        int m = lev.find(src[0], src[1], src[2], hint);
+                        if (m < 0) { bad += 1; continue; }
+                        hint = m;
+                        f(i, j, k, n) = sign * lev.fabs[m](src[0], src[1], src[2], n);
+                    }
+                    (void)any_out;
+                    if (g.probtype == 1) {
+                        // Exec/1_DMR/prob_bcfill.cpp:27-57; bc of component 0 decides
+                        double x = g.prob_lo[0] + (i + 0.5) * g.dx[0];
+                        double x0 = 1.0/6.0 + std::sqrt(1/3.0) + time*10.0/std::sin(M_PI*(1.0/3.0));
+                        int bc_hi1 = comp_bc(g, 0, 1, 1), bc_lo1 = comp_bc(g, 0, 1, 0), bc_lo0 = comp_bc(g, 0, 0, 0);
+                        if (bc_hi1 == EXT_DIR && j > g.dom_hi[1]) {
+                            for (int n = 0; n < NCONS; ++n) f(i, j, k, n) = x < x0 ? g.left_state[n] : g.right_state[n];
+                        }
+                        if (bc_lo1 == REFLECT_EVEN && j < g.dom_lo[1]) {
+                            if (x < 1.0/6.0)
+                                for (int n = 0; n < NCONS; ++n) f(i, j, k, n) = g.left_state[n];
+                        }
+                        if (bc_lo0 == EXT_DIR && i < g.dom_lo[0]) {
+                            for (int n = 0; n < NCONS; ++n) f(i, j, k, n) = g.left_state[n];
+                        }
+                    } else if (ext) {
+                        bad += 1;
+                    }
+                }
+            }
+    }
+    return bad;
+}
+
+Level make_level(int nbox, const int* lo, const int* hi, const orc_fab* fabs)
+{
+    Level L;
+    L.nbox = nbox;
+    for (int b = 0; b < nbox; ++b) {
+        Box bx;
+        for (int d = 0; d < 3; ++d) { bx.lo[d] = lo[3*b+d]; bx.hi[d] = hi[3*b+d]; }
+        L.boxes.push_back(bx);
+        L.fabs.push_back(View(fabs[b]));
+    }
+    return L;
+}
+
+} // namespace
+
+extern "C" {
+
+int orc_fill_ghost(int nbox, const int* lo, const int* hi, const orc_fab* fabs, const orc_geom* g, double time)
+{
+    Level L = make_level(nbox, lo, hi, fabs);
+    return fill_level(L, *g, time);
+}
+
+// One time step of EBR::flow_advance on a single level (Advance.cpp:81-150).
+// scheme 0: Euler (:117-122); 1: RK2 Heun (:123-139); 2: SSP-RK3 in Shu-Osher
+// form U1=Un+dtL(Un), U2=3/4Un+1/4U1+dt/4 L(U1), Un+1=1/3Un+2/3U2+2dt/3 L(U2)
+// with stage times t, t+dt, t+dt/2 (AMReX RK(3,...), Advance.cpp:141-148).
+// A: U^n in, U^(n+1) out (ghosts filled on return for the next step's use);
+// B, C: scratch FABs of the same boxes.  All FABs hold 4 ghost cells.
+int orc_flow_advance(int nbox, const int* lo, const int* hi, const orc_fab* A, const orc_fab* B, const orc_fab* C,
+                     const orc_geom* g, const orc_parm* parm, double time, double dt, int scheme,
+                     int do_visc, int do_gravity)
+{
+    Level LA = make_level(nbox, lo, hi, A), LB = make_level(nbox, lo, hi, B), LC = make_level(nbox, lo, hi, C);
+    double dxinv[3] = {1.0 / g->dx[0], 1.0 / g->dx[1], 1.0 / g->dx[2]};
+    int bad = 0;
+    std::vector<std::vector<double>> dsdt(nbox);
+    std::vector<View> dv(nbox);
+    for (int b = 0; b < nbox; ++b) {
+        const Box& bx = LA.boxes[b];
+        dsdt[b].assign((size_t)NCONS * (bx.hi[0]-bx.lo[0]+1) * (bx.hi[1]-bx.lo[1]+1) * (bx.hi[2]-bx.lo[2]+1), 0.0);
+        dv[b] = View(dsdt[b].data(), bx.lo, bx.hi, NCONS);
+    }
+    // stage: out = (c_cur*cur + c_old*old) + c_dt*L(cur)  (LinComb / Saxpy forms)
+    auto stage = [&](const Level& cur, const Level* old, const Level& out, double c_cur, double c_old, double c_dt) {
+        for (int b = 0; b < nbox; ++b)
+            compute_dsdt_box(cur.boxes[b], cur.fabs[b], dv[b], dxinv, do_visc, do_gravity, *parm, nullptr);
+        for (int b = 0; b < nbox; ++b) {
+            const View& u = cur.fabs[b];
+            const View& o = out.fabs[b];
+            const View* uo = old ? &old->fabs[b] : nullptr;
+            const View& L = dv[b];
+            for_box(cur.boxes[b], [&](int i, int j, int k) {
+                for (int n = 0; n < NCONS; ++n) {
+                    double v = uo ? c_cur * u(i, j, k, n) + c_old * (*uo)(i, j, k, n) : c_cur * u(i, j, k, n);
+                    o(i, j, k, n) = v + c_dt * L(i, j, k, n);
+                }
+            });
+        }
+    };
+    bad += fill_level(LA, *g, time);
+    if (scheme == 0) {
+        stage(LA, nullptr, LB, 1.0, 0.0, dt);
+        bad += fill_level(LB, *g, time + dt);
+        for (int b = 0; b < nbox; ++b) std::memcpy(LA.fabs[b].p, LB.fabs[b].p, sizeof(double) * LA.fabs[b].npts() * NCONS);
+    } else if (scheme == 1) {
+        stage(LA, nullptr, LB, 1.0, 0.0, dt);
+        bad += fill_level(LB, *g, time + dt);
+        stage(LB, &LA, LC, 0.5, 0.5, 0.5 * dt);
+        bad += fill_level(LC, *g, time + dt);
+        for (int b = 0; b < nbox; ++b) std::memcpy(LA.fabs[b].p, LC.fabs[b].p, sizeof(double) * LA.fabs[b].npts() * NCONS);
+    } else {
+        stage(LA, nullptr, LB, 1.0, 0.0, dt);
+        bad += fill_level(LB, *g, time + dt);
+        stage(LB, &LA, LC, 0.25, 0.75, 0.25 * dt);
+        bad += fill_level(LC, *g, time + 0.5 * dt);
+        stage(LC, &LA, LB, 2.0 / 3.0, 1.0 / 3.0, (2.0 / 3.0) * dt);
+        bad += fill_level(LB, *g, time + dt);
+        for (int b = 0; b < nbox; ++b) std::memcpy(LA.fabs[b].p, LB.fabs[b].p, sizeof(double) * LA.fabs[b].npts() * NCONS);
+    }
+    return bad;
+}
+
+// Single-level time loop: initial dt = estTimeStep (EBR.cpp:561-563,189-229),
+// then per step computeNewDt with change_max 1.1 and the stop_time clip
+// (EBR.cpp:231-306).  Runs until time reaches stop_time or nsteps steps.
+// Returns the number of steps taken; *time_io and *dt_io carry the time and
+// the next dt.
+int orc_run(int nbox, const int* lo, const int* hi, const orc_fab* A, const orc_fab* B, const orc_fab* C,
+            const orc_geom* g, const orc_parm* parm, double cfl, int scheme, int do_visc, int do_gravity,
+            double stop_time, int nsteps, double* time_io, double* dt_io)
+{
+    Level LA = make_level(nbox, lo, hi, A);
+    double time = *time_io;
+    double dt = *dt_io;
+    if (dt <= 0.0) {
+        dt = orc_estdt(nbox, lo, hi, A, g->dx, cfl, parm);
+        const double eps = 0.001 * dt;
+        if (stop_time >= 0.0 && (time + dt) > (stop_time - eps)) dt = stop_time - time;
+    }
+    int steps = 0;
+    while ((nsteps < 0 || steps < nsteps) && (stop_time < 0.0 || time < stop_time)) {
+        orc_flow_advance(nbox, lo, hi, A, B, C, g, parm, time, dt, scheme, do_visc, do_gravity);
+        time += dt;
+        ++steps;
+        double est = orc_estdt(nbox, lo, hi, A, g->dx, cfl, parm);
+        double dt_new = std::min(est, 1.1 * dt);
+        const double eps = 0.001 * dt_new;
+        if (stop_time >= 0.0 && (time + dt_new) > (stop_time - eps)) dt_new = stop_time - time;
+        dt = dt_new;
+    }
+    *time_io = time;
+    *dt_io = dt;
+    return steps;
+}
+
+// ---------------------------------------------------------------------------
+// Case initial data (valid cells only)
+// ---------------------------------------------------------------------------
+
+// Exec/0_SOD/prob.H:11-38 with Exec/0_SOD/prob_parm.H:9-12 defaults
+void orc_init_sod(const int* lo, const int* hi, const orc_fab* S, const orc_geom* g, const orc_parm* parm)
+{
+    View s(*S);
+    Box bx;
+    for (int d = 0; d < 3; ++d) { bx.lo[d] = lo[d]; bx.hi[d] = hi[d]; }
+    const double p_l = 1.0, p_r = 0.1, rho_l = 1.0, rho_r = 0.125;
+    for_box(bx, [&](int i, int j, int k) {
+        double x = g->prob_lo[0] + (i + 0.5) * g->dx[0];
+        double Pt, rhot;
+        if (x < 0.5) { Pt = p_l; rhot = rho_l; } else { Pt = p_r; rhot = rho_r; }
+        s(i, j, k, URHO) = rhot;
+        s(i, j, k, UMX) = 0.0;
+        s(i, j, k, UMY) = 0.0;
+        s(i, j, k, UMZ) = 0.0;
+        s(i, j, k, UEDEN) = Pt / (parm->eos_gamma - 1.0);
+    });
+}
+
+// Exec/1_DMR/prob.H:11-45 with Exec/1_DMR/prob_parm.H:10-17 defaults
+void orc_init_dmr(const int* lo, const int* hi, const orc_fab* S, const orc_geom* g, const orc_parm* parm)
+{
+    View s(*S);
+    Box bx;
+    for (int d = 0; d < 3; ++d) { bx.lo[d] = lo[d]; bx.hi[d] = hi[d]; }
+    const double p_l = 116.5, p_r = 1.0, rho_l = 8.0, rho_r = 1.4;
+    const double u_l = 8.25 * std::cos(M_PI * (1.0 / 6.0)), u_r = 0.0;
+    const double v_l = -8.25 * std::sin(M_PI * (1.0 / 6.0)), v_r = 0.0;
+    for_box(bx, [&](int i, int j, int k) {
+        double x = g->prob_lo[0] + (i + 0.5) * g->dx[0];
+        double y = g->prob_lo[1] + (j + 0.5) * g->dx[1];
+        double x0 = 1.0 / 6.0;
+        double Pt, rhot, uxt, vxt;
+        if (x < x0 + std::sqrt(1 / 3.0) * y) { Pt = p_l; rhot = rho_l; uxt = u_l; vxt = v_l; }
+        else { Pt = p_r; rhot = rho_r; uxt = u_r; vxt = v_r; }
+        s(i, j, k, URHO) = rhot;
+        s(i, j, k, UMX) = rhot * uxt;
+        s(i, j, k, UMY) = rhot * vxt;
+        s(i, j, k, UMZ) = 0.0;
+        double et = Pt / (parm->eos_gamma - 1.0);
+        s(i, j, k, UEDEN) = et + 0.5 * rhot * (uxt * uxt + vxt * vxt);
+    });
+}
+
+// DMR inflow states, Exec/1_DMR/prob.cpp:28-40 (note the literal 0.4)
+void orc_dmr_states(double* left, double* right)
+{
+    const double p_l = 116.5, p_r = 1.0, rho_l = 8.0, rho_r = 1.4;
+    const double u_l = 8.25 * std::cos(M_PI * (1.0 / 6.0)), u_r = 0.0;
+    const double v_l = -8.25 * std::sin(M_PI * (1.0 / 6.0)), v_r = 0.0;
+    left[0] = rho_l; left[1] = u_l * rho_l; left[2] = v_l * rho_l; left[3] = 0;
+    left[4] = p_l / 0.4 + 0.5 * rho_l * (u_l * u_l + v_l * v_l);
+    right[0] = rho_r; right[1] = u_r * rho_r; right[2] = v_r * rho_r; right[3] = 0;
+    right[4] = p_r / 0.4 + 0.5 * rho_r * (u_r * u_r + v_r * v_r);
+}
+
+// Counter-based uniform variate in [0,1): splitmix64 finaliser of
+// (seed * golden-ratio constant + counter), top 53 bits.
+double orc_rng_uniform(uint64_t seed, uint64_t counter)
+{
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + counter;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// 3D viscous Kelvin-Helmholtz shear layer on a periodic unit cube (the case
+// of the 3D KH workload; the reference's Exec/8_KH is a 2D inviscid variant):
+// air at p0 = 101325 Pa, outer stream at 300 K, density ratio 2 inside the
+// band 1/4 < y < 3/4, tanh profiles of thickness delta = 1/32, streams at
+// +-Mach 0.5 of the outer sound speed, seeded uniform velocity perturbation of
+// amplitude 1e-2 of the stream speed.
+void orc_init_kh3d(const int* lo, const int* hi, const orc_fab* S, const orc_geom* g, const orc_parm* parm,
+                   uint64_t seed)
+{
+    View s(*S);
+    Box bx;
+    for (int d = 0; d < 3; ++d) { bx.lo[d] = lo[d]; bx.hi[d] = hi[d]; }
+    const double gamma = parm->eos_gamma;
+    const double p0 = 101325.0;
+    const double rho_out = p0 / (parm->Rg * 300.0);
+    const double rho_in = 2.0 * rho_out;
+    const double c_out = std::sqrt(gamma * p0 / rho_out);
+    const double U = 0.5 * c_out;
+    const double delta = 1.0 / 32.0;
+    const double amp = 1.0e-2 * U;
+    const long nx = g->dom_hi[0] - g->dom_lo[0] + 1, ny = g->dom_hi[1] - g->dom_lo[1] + 1;
+    for_box(bx, [&](int i, int j, int k) {
+        double y = g->prob_lo[1] + (j + 0.5) * g->dx[1];
+        double sh = 0.5 * (std::tanh((y - 0.25) / delta) - std::tanh((y - 0.75) / delta));
+        double rho = rho_out + (rho_in - rho_out) * sh;
+        uint64_t cell = ((uint64_t)(k - g->dom_lo[2]) * ny + (j - g->dom_lo[1])) * nx + (i - g->dom_lo[0]);
+        double u = U * (2.0 * sh - 1.0) + amp * (2.0 * orc_rng_uniform(seed, 3 * cell + 0) - 1.0);
+        double v = amp * (2.0 * orc_rng_uniform(seed, 3 * cell + 1) - 1.0);
+        double w = amp * (2.0 * orc_rng_uniform(seed, 3 * cell + 2) - 1.0);
+        s(i, j, k, URHO) = rho;
+        s(i, j, k, UMX) = rho * u;
+        s(i, j, k, UMY) = rho * v;
+        s(i, j, k, UMZ) = rho * w;
+        s(i, j, k, UEDEN) = p0 / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
+    });
+}
+
+} // extern "C"

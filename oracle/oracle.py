@@ -1,0 +1,224 @@
+"""ctypes binding of the CPU oracle (liboracle.so) -- test infrastructure only.
+
+Only tests/, the smoke check and the cpu_baseline leg of bench.py use this
+module, as the checker.  The product path never imports it.
+"""
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+NCONS = 5
+NGROW = 4
+
+
+class OParm(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "eos_gamma", "eos_m", "Patm", "cv", "cp", "Ru", "Rg", "kOverMu", "Pr", "C_s", "T_s",
+        "smallr", "smallp", "eps", "twoThirds", "oneSix")]
+
+
+class OFab(ctypes.Structure):
+    _fields_ = [("p", ctypes.POINTER(ctypes.c_double)), ("lo", ctypes.c_int * 3), ("hi", ctypes.c_int * 3),
+                ("ncomp", ctypes.c_int)]
+
+
+class OGeom(ctypes.Structure):
+    _fields_ = [
+        ("dom_lo", ctypes.c_int * 3), ("dom_hi", ctypes.c_int * 3),
+        ("prob_lo", ctypes.c_double * 3), ("dx", ctypes.c_double * 3),
+        ("periodic", ctypes.c_int * 3), ("lo_bc", ctypes.c_int * 3), ("hi_bc", ctypes.c_int * 3),
+        ("probtype", ctypes.c_int),
+        ("left_state", ctypes.c_double * 5), ("right_state", ctypes.c_double * 5),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_parm_default.argtypes = [ctypes.POINTER(OParm)]
+        L.orc_parm_initialize.argtypes = [ctypes.POINTER(OParm)]
+        L.orc_c2prim_many.argtypes = [ctypes.c_long, dp, dp, ctypes.POINTER(OParm)]
+        L.orc_flux_split_many.argtypes = [ctypes.c_int, ctypes.c_long, dp, dp, dp, ctypes.POINTER(OParm)]
+        L.orc_weno_many.argtypes = [ctypes.c_int, ctypes.c_long, dp, dp]
+        L.orc_compute_dsdt_box.argtypes = [ip, ip, ctypes.POINTER(OFab), ctypes.POINTER(OFab), dp, ctypes.c_int,
+                                           ctypes.c_int, ctypes.POINTER(OParm), dp, dp, dp]
+        L.orc_estdt.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(OFab), dp, ctypes.c_double,
+                                ctypes.POINTER(OParm)]
+        L.orc_estdt.restype = ctypes.c_double
+        L.orc_fill_ghost.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(OFab), ctypes.POINTER(OGeom),
+                                     ctypes.c_double]
+        L.orc_flow_advance.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(OFab), ctypes.POINTER(OFab),
+                                       ctypes.POINTER(OFab), ctypes.POINTER(OGeom), ctypes.POINTER(OParm),
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_run.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(OFab), ctypes.POINTER(OFab), ctypes.POINTER(OFab),
+                              ctypes.POINTER(OGeom), ctypes.POINTER(OParm), ctypes.c_double, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp]
+        for name in ("orc_init_sod", "orc_init_dmr"):
+            getattr(L, name).argtypes = [ip, ip, ctypes.POINTER(OFab), ctypes.POINTER(OGeom), ctypes.POINTER(OParm)]
+        L.orc_init_kh3d.argtypes = [ip, ip, ctypes.POINTER(OFab), ctypes.POINTER(OGeom), ctypes.POINTER(OParm),
+                                    ctypes.c_uint64]
+        L.orc_dmr_states.argtypes = [dp, dp]
+        L.orc_rng_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.orc_rng_uniform.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def parm(**kw):
+    p = OParm()
+    lib().orc_parm_default(ctypes.byref(p))
+    if kw:
+        for k, v in kw.items():
+            setattr(p, k, v)
+        lib().orc_parm_initialize(ctypes.byref(p))
+    return p
+
+
+def geom(n_cell, prob_lo, prob_hi, periodic=(0, 0, 0), lo_bc=(2, 2, 2), hi_bc=(2, 2, 2), probtype=0,
+         left_state=None, right_state=None):
+    g = OGeom()
+    for d in range(3):
+        g.dom_lo[d] = 0
+        g.dom_hi[d] = n_cell[d] - 1
+        g.prob_lo[d] = prob_lo[d]
+        g.dx[d] = (prob_hi[d] - prob_lo[d]) / n_cell[d]
+        g.periodic[d] = periodic[d]
+        g.lo_bc[d] = lo_bc[d]
+        g.hi_bc[d] = hi_bc[d]
+    g.probtype = probtype
+    if probtype == 1 and left_state is None:
+        left_state, right_state = dmr_states()
+    if left_state is not None:
+        for n in range(5):
+            g.left_state[n] = left_state[n]
+            g.right_state[n] = right_state[n]
+    return g
+
+
+def dmr_states():
+    l = np.zeros(5)
+    r = np.zeros(5)
+    lib().orc_dmr_states(_dp(l), _dp(r))
+    return l, r
+
+
+class Fab:
+    """A numpy-backed FAB (ncomp, nz, ny, nx) over the inclusive box [lo, hi]."""
+
+    def __init__(self, lo, hi, ncomp=NCONS, data=None):
+        self.lo = tuple(lo)
+        self.hi = tuple(hi)
+        shape = (ncomp,) + tuple(hi[d] - lo[d] + 1 for d in (2, 1, 0))
+        self.a = np.zeros(shape) if data is None else np.ascontiguousarray(data, dtype=np.float64).reshape(shape)
+        self.c = OFab()
+        self.c.p = _dp(self.a)
+        for d in range(3):
+            self.c.lo[d] = self.lo[d]
+            self.c.hi[d] = self.hi[d]
+        self.c.ncomp = ncomp
+
+    def valid(self, lo, hi):
+        sl = tuple(slice(lo[d] - self.lo[d], hi[d] - self.lo[d] + 1) for d in (2, 1, 0))
+        return self.a[(slice(None),) + sl]
+
+
+def grown(lo, hi, g=NGROW):
+    return tuple(v - g for v in lo), tuple(v + g for v in hi)
+
+
+def _boxarrays(boxes):
+    n = len(boxes)
+    lo = (ctypes.c_int * (3 * n))(*[v for b in boxes for v in b[0]])
+    hi = (ctypes.c_int * (3 * n))(*[v for b in boxes for v in b[1]])
+    return n, lo, hi
+
+
+def _fabarray(fabs):
+    arr = (OFab * len(fabs))()
+    for m, f in enumerate(fabs):
+        arr[m] = f.c
+    return arr
+
+
+def compute_dsdt_box(lo, hi, S, p, dxinv, do_visc=0, do_gravity=0, faces=False):
+    """dSdt on box [lo,hi] from FAB S (covering grow(bx,4)); returns (dSdt, faces)."""
+    D = Fab(lo, hi)
+    lo_c = (ctypes.c_int * 3)(*lo)
+    hi_c = (ctypes.c_int * 3)(*hi)
+    dxi = np.ascontiguousarray(dxinv, dtype=np.float64)
+    fx = fy = fz = None
+    fptr = [None, None, None]
+    if faces:
+        out = []
+        for d in range(3):
+            h = list(hi)
+            h[d] += 1
+            out.append(np.zeros((5,) + tuple(h[e] - lo[e] + 1 for e in (2, 1, 0))))
+        fx, fy, fz = out
+        fptr = [_dp(fx), _dp(fy), _dp(fz)]
+    lib().orc_compute_dsdt_box(lo_c, hi_c, ctypes.byref(S.c), ctypes.byref(D.c), _dp(dxi), do_visc, do_gravity,
+                               ctypes.byref(p), *fptr)
+    return D.a, (fx, fy, fz)
+
+
+def estdt(boxes, fabs, dx, cfl, p):
+    n, lo, hi = _boxarrays(boxes)
+    return lib().orc_estdt(n, lo, hi, _fabarray(fabs), _dp(np.asarray(dx, dtype=np.float64)), cfl, ctypes.byref(p))
+
+
+def fill_ghost(boxes, fabs, g, time):
+    n, lo, hi = _boxarrays(boxes)
+    return lib().orc_fill_ghost(n, lo, hi, _fabarray(fabs), ctypes.byref(g), time)
+
+
+def flow_advance(boxes, A, B, C, g, p, time, dt, scheme, do_visc=0, do_gravity=0):
+    n, lo, hi = _boxarrays(boxes)
+    return lib().orc_flow_advance(n, lo, hi, _fabarray(A), _fabarray(B), _fabarray(C), ctypes.byref(g),
+                                  ctypes.byref(p), time, dt, scheme, do_visc, do_gravity)
+
+
+def run(boxes, A, B, C, g, p, cfl, scheme=2, do_visc=0, do_gravity=0, stop_time=-1.0, nsteps=-1, time=0.0, dt=0.0):
+    n, lo, hi = _boxarrays(boxes)
+    t = ctypes.c_double(time)
+    d = ctypes.c_double(dt)
+    steps = lib().orc_run(n, lo, hi, _fabarray(A), _fabarray(B), _fabarray(C), ctypes.byref(g), ctypes.byref(p), cfl,
+                          scheme, do_visc, do_gravity, stop_time, nsteps, ctypes.byref(t), ctypes.byref(d))
+    return steps, t.value, d.value
+
+
+def init_case(kind, boxes, fabs, g, p, seed=0):
+    for (lo, hi), f in zip(boxes, fabs):
+        lo_c = (ctypes.c_int * 3)(*lo)
+        hi_c = (ctypes.c_int * 3)(*hi)
+        if kind == 0:
+            lib().orc_init_sod(lo_c, hi_c, ctypes.byref(f.c), ctypes.byref(g), ctypes.byref(p))
+        elif kind == 1:
+            lib().orc_init_dmr(lo_c, hi_c, ctypes.byref(f.c), ctypes.byref(g), ctypes.byref(p))
+        else:
+            lib().orc_init_kh3d(lo_c, hi_c, ctypes.byref(f.c), ctypes.byref(g), ctypes.byref(p), seed)
+
+
+def new_level_fabs(boxes, g=NGROW):
+    return [Fab(*grown(lo, hi, g)) for lo, hi in boxes]
