@@ -1,0 +1,156 @@
+"""Benchmark of the MI355X-native per-box advance.
+
+Workload: 3D viscous Kelvin-Helmholtz shear layer, periodic 256^3 single
+level, SSP-RK3, Sutherland viscosity, Pr 0.72 -- one step = one full RK3 time
+step (3 ghost fills, 9 flux sweeps with the fused stage updates, one CFL
+reduction) over all 256^3 cells, state resident in HBM.  The headline value is
+the FP32-flux / FP64-state form; the plain FP64 form is measured beside it.
+
+Prints one JSON line (rank 0, N=1).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "opencfd-amr_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256, help="cells per side of the KH cube")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=128, help="cells per side of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--skip-fp64", action="store_true")
+    return ap.parse_args()
+
+
+def timed_run(lev, A, B, C, p, flags, steps, warmup):
+    import torch
+    import ocfd
+    cfl = 0.5
+    _, t, dt = ocfd.run(lev, A, B, C, p, cfl, 2, flags, nsteps=warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ocfd.run(lev, A, B, C, p, cfl, 2, flags, nsteps=steps, time=t, dt=dt)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_roofline(lev, A, B, C, p, flags, ncell):
+    """Per-kernel average durations of one RK stage (stage-2 form, HIP events on
+    the launch stream) and the roofline figures of the dominant kernel."""
+    lev.fill_boundary(A, 0.0)
+    ms = lev.profile_stage(A, B, C, 0.25, 0.75, 1e-9, p, flags, reps=5)
+    names = ["sweep_x", "sweep_y", "sweep_z_update", "fill_boundary"]
+    # algorithmic HBM bytes per valid cell of each kernel (FP64 state, 5 comps):
+    #   sweep_x: read S 40, write dSdt partial 40
+    #   sweep_y: read S 40, read+write partial 80
+    #   sweep_z_update: read S 40, partial 40, U^n 40, write U^(k+1) 40
+    #   fill: ghost cells only, read+write 80 B per ghost cell
+    nx = lev.boxes[0][1][0] - lev.boxes[0][0][0] + 1
+    ghost = (nx + 8) ** 3 - nx ** 3
+    bytes_ = [80.0 * ncell, 120.0 * ncell, 160.0 * ncell, 80.0 * ghost]
+    kern = {}
+    for nm, t, b in zip(names, ms, bytes_):
+        kern[nm] = {"ms": t, "GBps": b / (t * 1e-3) / 1e9}
+    dom = max(range(3), key=lambda i: ms[i])
+    achieved = bytes_[dom] / (ms[dom] * 1e-3) / 1e9
+    return names[dom], achieved, kern
+
+
+def cpu_baseline(n, steps):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    sys.path.insert(0, str(ROOT / "tests"))
+    import ctypes
+    import oracle as orc
+    from ocfd import cases
+    boxes, g, kind = cases.kh3d(n)
+    og = orc.OGeom()
+    ctypes.memmove(ctypes.byref(og), ctypes.byref(g), ctypes.sizeof(og))
+    A, B, C = (orc.new_level_fabs(boxes) for _ in range(3))
+    orc.init_case(kind, boxes, A, og, orc.parm(), seed=42)
+    t0 = time.perf_counter()
+    st, _, _ = orc.run(boxes, A, B, C, og, orc.parm(), 0.5, 2, 1, 0, -1.0, steps)
+    el = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n ** 3 * st / el / 1e6, "unit": "Mcell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 C++/OpenMP oracle, KH {n}^3 periodic viscous, {st} SSP-RK3 steps ({el:.1f} s)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a "
+              "design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import ocfd
+    from ocfd import cases
+
+    torch.cuda.set_device(0)
+    n = args.n
+    boxes, g, kind = cases.kh3d(n)
+    lev = ocfd.Level(boxes, g)
+    p = ocfd.default_parm()
+    A, B, C = lev.new_mf(), lev.new_mf(), lev.new_mf()
+    ncell = n ** 3
+
+    flags32 = ocfd.VISC | ocfd.FLUX_FP32
+    lev.init_state(kind, A, p, seed=42)
+    t32 = timed_run(lev, A, B, C, p, flags32, args.steps, args.warmup)
+    dom, achieved, kern32 = kernel_roofline(lev, A, B, C, p, flags32, ncell)
+    value = ncell * args.steps / t32 / 1e6
+
+    res64 = None
+    if not args.skip_fp64:
+        flags64 = ocfd.VISC
+        lev.init_state(kind, A, p, seed=42)
+        t64 = timed_run(lev, A, B, C, p, flags64, args.steps, args.warmup)
+        _, _, kern64 = kernel_roofline(lev, A, B, C, p, flags64, ncell)
+        res64 = {"value": ncell * args.steps / t64 / 1e6, "ms_per_step": 1e3 * t64 / args.steps,
+                 "kernels": kern64}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_n, args.cpu_steps)
+
+    out = {
+        "metric": "Mcell-updates/s + % HBM peak; p50/p99 case latency, FP32-flux vs FP64",
+        "value": value,
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t32 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-flux/f64-state",
+        "data": "synthetic (seeded KH initial data)",
+        "config": {"workload": f"3D viscous Kelvin-Helmholtz, periodic {n}^3 single level, SSP-RK3, "
+                               "FP32 fluxes / FP64 state", "cells": ncell, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "hbm_frac_step": 320.0 * value * 1e6 / 1e9 / HBM_PEAK_GBS,
+        "kernels": kern32,
+        "fp64_form": res64,
+        "fp32_over_fp64": (value / res64["value"]) if res64 else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -255,6 +255,25 @@ void launch_sweeps(const LevelDev& L, const StageArgs& a, hipStream_t s)
     }
 }
 
+// One sweep alone (0 = x, 1 = y, 2 = z final), for per-kernel timing.
+void launch_sweep_one(const LevelDev& L, const StageArgs& a, int which, hipStream_t s)
+{
+    const bool visc = (a.flags & OCFD_VISC) != 0;
+    const bool f32 = (a.flags & OCFD_FLUX_FP32) != 0;
+#define OCFD_SW(FT, V)                                                                                           \
+    do {                                                                                                         \
+        if (which == 0) hipLaunchKernelGGL((k_sweep_x<FT, V>), dim3(L.ntile_x), dim3(256), 0, s, L.d_boxes,      \
+                                           L.d_tile_x, a);                                                       \
+        else if (which == 1) hipLaunchKernelGGL((k_sweep_march<1, FT, V, false>), dim3(L.ntile_y), dim3(256), 0, \
+                                                s, L.d_boxes, L.d_tile_y, a);                                     \
+        else hipLaunchKernelGGL((k_sweep_march<2, FT, V, true>), dim3(L.ntile_z), dim3(256), 0, s, L.d_boxes,    \
+                                L.d_tile_z, a);                                                                  \
+    } while (0)
+    if (f32) { if (visc) OCFD_SW(float, true); else OCFD_SW(float, false); }
+    else { if (visc) OCFD_SW(double, true); else OCFD_SW(double, false); }
+#undef OCFD_SW
+}
+
 int tile_xseg() { return XSEG; }
 int tile_chunk() { return CH; }
 

@@ -29,6 +29,7 @@ struct LevelDev {
 };
 
 void launch_sweeps(const LevelDev& L, const StageArgs& a, hipStream_t s);
+void launch_sweep_one(const LevelDev& L, const StageArgs& a, int which, hipStream_t s);
 int tile_xseg();
 int tile_chunk();
 

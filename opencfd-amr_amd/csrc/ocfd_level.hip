@@ -117,6 +117,10 @@ __global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes,
             } else if (bc == BC_EXT) user = true;
         }
         if (user) continue;
+        // a domain thinner than the ghost width leaves the source outside: no fill
+        if (src[0] < G.dom_lo[0] || src[0] > G.dom_hi[0] || src[1] < G.dom_lo[1] || src[1] > G.dom_hi[1] ||
+            src[2] < G.dom_lo[2] || src[2] > G.dom_hi[2])
+            continue;
         const int mi = (src[0] - G.dom_lo[0]) >> G.map_shift;
         const int mj = (src[1] - G.dom_lo[1]) >> G.map_shift;
         const int mk = (src[2] - G.dom_lo[2]) >> G.map_shift;
@@ -495,6 +499,41 @@ int ocfd_advance(ocfd_level lev, double* A, double* B, double* C, double time, d
         if ((rc = ocfd_rk_stage(lev, C, A, A, 2.0 / 3.0, 1.0 / 3.0, (2.0 / 3.0) * dt, parm, flags, s))) return rc;
     }
     return OCFD_OK;
+}
+
+int ocfd_profile_stage(ocfd_level lev, const double* S_cur, const double* S_old, double* S_new, double c_cur,
+                       double c_old, double c_dt, const ocfd_parm* parm, int flags, int reps, double* ms_out,
+                       void* stream)
+{
+    if (!lev || !S_cur || !S_new || !parm || !ms_out || reps <= 0 || !valid_flags(flags) || S_cur == S_new ||
+        (flags & OCFD_PLANAR)) {
+        set_error("ocfd_profile_stage: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    StageArgs a{};
+    a.scur = S_cur; a.sold = S_old; a.snew = S_new; a.acc = lev->dev.d_acc;
+    a.c_cur = c_cur; a.c_old = c_old; a.c_dt = c_dt;
+    a.flags = flags; a.ph = phys_of(*parm);
+    hipEvent_t e0, e1;
+    OCFD_HIP(hipEventCreate(&e0));
+    OCFD_HIP(hipEventCreate(&e1));
+    for (int which = 0; which < 4; ++which) {
+        OCFD_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) {
+            if (which < 3) launch_sweep_one(lev->dev, a, which, s);
+            else hipLaunchKernelGGL(k_fill, dim3(lev->dev.ntile_g), dim3(256), 0, s, lev->dev.d_boxes,
+                                    lev->dev.d_geom, lev->dev.d_tile_g, (double*)S_cur, 0.0);
+        }
+        OCFD_HIP(hipEventRecord(e1, s));
+        OCFD_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        OCFD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms_out[which] = ms / reps;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return check_launch("ocfd_profile_stage");
 }
 
 } // extern "C"

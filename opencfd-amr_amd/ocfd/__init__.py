@@ -71,6 +71,8 @@ def lib():
                                    ctypes.POINTER(Parm), ctypes.c_int, vp]
         L.ocfd_init_state.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(Parm), ctypes.c_ulonglong]
         L.ocfd_dmr_states.argtypes = [dp, dp]
+        L.ocfd_profile_stage.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, dp, vp]
         L.ocfd_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -199,6 +201,13 @@ class Level:
     def advance(self, A, B, C, time, dt, scheme, parm, flags=0, stream=None):
         check(lib().ocfd_advance(self.handle, _ptr(A), _ptr(B), _ptr(C), time, dt, scheme, ctypes.byref(parm),
                                  flags, _stream(stream)), "ocfd_advance")
+
+    def profile_stage(self, S_cur, S_old, S_new, c_cur, c_old, c_dt, parm, flags=0, reps=5, stream=None):
+        """Average ms of (x sweep, y sweep, z sweep + update, ghost fill)."""
+        out = (ctypes.c_double * 4)()
+        check(lib().ocfd_profile_stage(self.handle, _ptr(S_cur), _ptr(S_old), _ptr(S_new), c_cur, c_old, c_dt,
+                                       ctypes.byref(parm), flags, reps, out, _stream(stream)), "ocfd_profile_stage")
+        return list(out)
 
     def init_state(self, kind, S, parm, seed=0):
         check(lib().ocfd_init_state(self.handle, kind, _ptr(S), ctypes.byref(parm), seed), "ocfd_init_state")

@@ -745,7 +745,8 @@ int orc_run(int nbox, const int* lo, const int* hi, const orc_fab* A, const orc_
     }
     int steps = 0;
     while ((nsteps < 0 || steps < nsteps) && (stop_time < 0.0 || time < stop_time)) {
-        orc_flow_advance(nbox, lo, hi, A, B, C, g, parm, time, dt, scheme, do_visc, do_gravity);
+        if (orc_flow_advance(nbox, lo, hi, A, B, C, g, parm, time, dt, scheme, do_visc, do_gravity) != 0)
+            return -1;   // a ghost cell had no source (domain thinner than the ghost width, or uncovered)
         time += dt;
         ++steps;
         double est = orc_estdt(nbox, lo, hi, A, g->dx, cfl, parm);

@@ -1,0 +1,211 @@
+"""GPU parity of the HIP per-box advance against the CPU oracle.
+
+FP64 form: 1e-10 relative (L1 and Linf per conserved variable, normalised by
+the oracle's L1 / max of that variable).  FP32-flux form: the tolerances
+written in each test.  Ghost fill, initial data and the CFL reduction are
+bit-exact (pure copies / exact min).
+"""
+import numpy as np
+import pytest
+
+import gpu_util as gu
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch
+
+
+def _level(case):
+    import ocfd
+    boxes, g, kind = case
+    return ocfd.Level(boxes, g), boxes, g, kind
+
+
+def _cases():
+    from ocfd import cases
+    return {
+        "sod_multibox": cases.sod(64, 8, 8, max_grid=(16, 8, 8)),
+        "dmr_uneven": cases.dmr(128, 32, 2, sizes=(8, 16, 32), seed=3),
+        "kh_multibox": cases.kh3d(24, max_grid=(12, 12, 24)),
+    }
+
+
+@pytest.mark.parametrize("name", ["sod_multibox", "dmr_uneven", "kh_multibox"])
+def test_init_state_matches_oracle(torch_cuda, orc, name):
+    import ocfd
+    lev, boxes, g, kind = _level(_cases()[name])
+    p = ocfd.default_parm()
+    S = lev.new_mf()
+    lev.init_state(kind, S, p, seed=42)
+    fabs = orc.new_level_fabs(boxes)
+    orc.init_case(kind, boxes, fabs, gu.to_ogeom(g), orc.parm(), seed=42)
+    got = gu.download(lev, S)
+    for b, f in enumerate(fabs):
+        assert np.array_equal(got[b], f.a), f"box {b}"
+
+
+@pytest.mark.parametrize("name", ["sod_multibox", "dmr_uneven", "kh_multibox"])
+def test_fill_boundary_matches_oracle(torch_cuda, orc, name):
+    lev, boxes, g, kind = _level(_cases()[name])
+    rng = np.random.default_rng(7)
+    fabs = orc.new_level_fabs(boxes)
+    for f in fabs:
+        f.a[...] = rng.uniform(-1, 1, size=f.a.shape)
+    S = lev.new_mf()
+    gu.upload(lev, S, fabs)
+    lev.fill_boundary(S, 0.0731)
+    bad = orc.fill_ghost(boxes, fabs, gu.to_ogeom(g), 0.0731)
+    assert bad == 0
+    got = gu.download(lev, S)
+    for b, f in enumerate(fabs):
+        assert np.array_equal(got[b], f.a), f"box {b}"
+
+
+@pytest.mark.parametrize("kind", ["smooth", "rough"])
+@pytest.mark.parametrize("visc,grav", [(0, 0), (1, 0), (1, 1)])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_compute_dsdt(torch_cuda, orc, kind, visc, grav, fp32):
+    import ocfd
+    from ocfd import cases
+    boxes, g, _ = cases.kh3d(24, max_grid=(24, 12, 8))
+    # non-periodic data on purpose: ghost cells are random too
+    lev = ocfd.Level(boxes, g)
+    rng = np.random.default_rng(11)
+    fabs = orc.new_level_fabs(boxes)
+    for f in fabs:
+        gu.random_state(f, rng, kind)
+    S = lev.new_mf()
+    gu.upload(lev, S, fabs)
+    D = lev.new_mf(ngrow=0)
+    p = ocfd.default_parm()
+    flags = (ocfd.VISC if visc else 0) | (ocfd.GRAVITY if grav else 0) | (ocfd.FLUX_FP32 if fp32 else 0)
+    lev.compute_dsdt(S, D, p, flags)
+    got = gu.download(lev, D, ngrow=0)
+    op = orc.parm()
+    dxinv = [1.0 / g.dx[d] for d in range(3)]
+    worst = 0.0
+    for b, (lo, hi) in enumerate(boxes):
+        ref, _ = orc.compute_dsdt_box(lo, hi, fabs[b], op, dxinv, visc, grav)
+        worst = max(worst, gu.max_rel(got[b], ref))
+    tol = 2e-5 if fp32 else TOL64
+    assert worst < tol, worst
+
+
+def test_planar_equals_full(torch_cuda, orc):
+    """z-uniform data: skipping the z sweep gives the full 3D result."""
+    import ocfd
+    from ocfd import cases
+    boxes, g, _ = cases.dmr(64, 32, 2, max_grid=(32, 16, 2))
+    lev = ocfd.Level(boxes, g)
+    rng = np.random.default_rng(5)
+    fabs = orc.new_level_fabs(boxes)
+    for f in fabs:
+        gu.random_state(f, rng, "rough")
+        f.a[:] = f.a[:, :1, :, :]   # z-uniform
+        f.a[3] = 0.0
+    S = lev.new_mf()
+    gu.upload(lev, S, fabs)
+    p = ocfd.default_parm()
+    D1 = lev.new_mf(ngrow=0)
+    D2 = lev.new_mf(ngrow=0)
+    lev.compute_dsdt(S, D1, p, 0)
+    lev.compute_dsdt(S, D2, p, ocfd.PLANAR)
+    assert np.array_equal(D1.cpu().numpy(), D2.cpu().numpy())
+
+
+def test_estdt_exact(torch_cuda, orc):
+    import ocfd
+    lev, boxes, g, kind = _level(_cases()["dmr_uneven"])
+    rng = np.random.default_rng(3)
+    fabs = orc.new_level_fabs(boxes)
+    for f in fabs:
+        gu.random_state(f, rng, "rough")
+    S = lev.new_mf()
+    gu.upload(lev, S, fabs)
+    p = ocfd.default_parm()
+    got = lev.estdt(S, 0.5, p)
+    ref = orc.estdt(boxes, fabs, [g.dx[0], g.dx[1], g.dx[2]], 0.5, orc.parm())
+    assert got == ref
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2])
+def test_one_step_matches_oracle(torch_cuda, orc, scheme):
+    import ocfd
+    lev, boxes, g, kind = _level(_cases()["dmr_uneven"])
+    p = ocfd.default_parm()
+    A = lev.new_mf(); B = lev.new_mf(); C = lev.new_mf()
+    lev.init_state(kind, A, p)
+    dt = 0.3 * lev.estdt(A, 1.0, p)
+    lev.advance(A, B, C, 0.0, dt, scheme, p, 0)
+    fa = orc.new_level_fabs(boxes); fb = orc.new_level_fabs(boxes); fc = orc.new_level_fabs(boxes)
+    og = gu.to_ogeom(g)
+    orc.init_case(kind, boxes, fa, og, orc.parm())
+    assert orc.flow_advance(boxes, fa, fb, fc, og, orc.parm(), 0.0, dt, scheme) == 0
+    got = gu.download(lev, A)
+    for b, (lo, hi) in enumerate(boxes):
+        v_got = got[b][:, 4:-4, 4:-4, 4:-4]
+        v_ref = fa[b].a[:, 4:-4, 4:-4, 4:-4]
+        assert gu.max_rel(v_got, v_ref) < TOL64
+
+
+def _run_pair(case, cfl, stop_time, nsteps, flags, scheme=2, visc=0):
+    import ocfd
+    import oracle as orc
+    boxes, g, kind = case
+    lev = ocfd.Level(boxes, g)
+    p = ocfd.default_parm()
+    A = lev.new_mf(); B = lev.new_mf(); C = lev.new_mf()
+    lev.init_state(kind, A, p, seed=42)
+    st_g, t_g, _ = ocfd.run(lev, A, B, C, p, cfl, scheme, flags, stop_time=stop_time, nsteps=nsteps)
+    fa = orc.new_level_fabs(boxes); fb = orc.new_level_fabs(boxes); fc = orc.new_level_fabs(boxes)
+    og = gu.to_ogeom(g)
+    orc.init_case(kind, boxes, fa, og, orc.parm(), seed=42)
+    st_o, t_o, _ = orc.run(boxes, fa, fb, fc, og, orc.parm(), cfl, scheme, visc, 0, stop_time, nsteps)
+    got = gu.download(lev, A)
+    errs = []
+    for b in range(len(boxes)):
+        errs.append(gu.rel_err(got[b][:, 4:-4, 4:-4, 4:-4], fa[b].a[:, 4:-4, 4:-4, 4:-4]))
+    return (st_g, t_g), (st_o, t_o), errs
+
+
+def test_sod_to_t02_fp64(torch_cuda):
+    """Sod 512x8 slab (nz=8), SW + WENO-Z5 + SSP-RK3, CFL 0.5, to t = 0.2."""
+    from ocfd import cases
+    (sg, tg), (so, to), errs = _run_pair(cases.sod(512, 8, 8), 0.5, 0.2, -1, 0)
+    assert sg == so and tg == to
+    worst = max(max(max(e) for e in be) for be in errs)
+    print("sod 512x8 fp64 worst rel err", worst)
+    assert worst < TOL64
+
+
+def test_kh3d_steps_fp64_and_fp32(torch_cuda):
+    import ocfd
+    from ocfd import cases
+    case = cases.kh3d(32, max_grid=(32, 16, 32))
+    (sg, tg), (so, to), errs = _run_pair(case, 0.5, -1.0, 5, ocfd.VISC, visc=1)
+    worst = max(max(max(e) for e in be) for be in errs)
+    assert sg == so == 5
+    assert worst < TOL64, worst
+    (_, _), (_, _), errs32 = _run_pair(case, 0.5, -1.0, 5, ocfd.VISC | ocfd.FLUX_FP32, visc=1)
+    l1_rho = max(be[0][0] for be in errs32)
+    print("kh fp32 rel L1 rho", l1_rho)
+    assert l1_rho < 1e-5
+
+
+def test_dmr_uneven_steps_fp64(torch_cuda):
+    import ocfd
+    from ocfd import cases
+    case = cases.dmr(128, 32, 2, sizes=(8, 16, 32), seed=3)
+    (sg, tg), (so, to), errs = _run_pair(case, 0.5, -1.0, 10, 0)
+    worst = max(max(max(e) for e in be) for be in errs)
+    assert worst < TOL64, worst
+    (_, _), (_, _), errs_p = _run_pair(case, 0.5, -1.0, 10, ocfd.PLANAR)
+    worst_p = max(max(max(e) for e in be) for be in errs_p)
+    assert worst_p < TOL64, worst_p
