@@ -60,5 +60,28 @@ def rel_err(a, b):
 
 
 def max_rel(a, b):
-    e = rel_err(a, b)
-    return max(max(x) for x in e)
+    return level_max_rel([a], [b])
+
+
+def level_rel_err(got, ref):
+    """Whole-level (L1, Linf) error per conserved variable: sum/max over all
+    boxes of |got-ref| over sum/max of |ref|.  The momentum components share
+    the norm of the momentum vector, so a component that is identically zero
+    (e.g. z-momentum of a planar case) is measured against the flow's momentum."""
+    nc = got[0].shape[0]
+    num1 = np.zeros(nc); numi = np.zeros(nc); den1 = np.zeros(nc); deni = np.zeros(nc)
+    for a, b in zip(got, ref):
+        a = np.asarray(a).reshape(nc, -1)
+        b = np.asarray(b).reshape(nc, -1)
+        d = np.abs(a - b)
+        num1 += d.sum(1); numi = np.maximum(numi, d.max(1))
+        den1 += np.abs(b).sum(1); deni = np.maximum(deni, np.abs(b).max(1))
+    if nc >= 4:
+        den1[1:4] = den1[1:4].sum()
+        deni[1:4] = deni[1:4].max()
+    den1 = np.maximum(den1, 1e-300); deni = np.maximum(deni, 1e-300)
+    return list(zip(num1 / den1, numi / deni))
+
+
+def level_max_rel(got, ref):
+    return max(max(e) for e in level_rel_err(got, ref))

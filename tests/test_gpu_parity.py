@@ -90,10 +90,8 @@ def test_compute_dsdt(torch_cuda, orc, kind, visc, grav, fp32):
     got = gu.download(lev, D, ngrow=0)
     op = orc.parm()
     dxinv = [1.0 / g.dx[d] for d in range(3)]
-    worst = 0.0
-    for b, (lo, hi) in enumerate(boxes):
-        ref, _ = orc.compute_dsdt_box(lo, hi, fabs[b], op, dxinv, visc, grav)
-        worst = max(worst, gu.max_rel(got[b], ref))
+    refs = [orc.compute_dsdt_box(lo, hi, fabs[b], op, dxinv, visc, grav)[0] for b, (lo, hi) in enumerate(boxes)]
+    worst = gu.level_max_rel(got, refs)
     tol = 2e-5 if fp32 else TOL64
     assert worst < tol, worst
 
@@ -149,10 +147,9 @@ def test_one_step_matches_oracle(torch_cuda, orc, scheme):
     orc.init_case(kind, boxes, fa, og, orc.parm())
     assert orc.flow_advance(boxes, fa, fb, fc, og, orc.parm(), 0.0, dt, scheme) == 0
     got = gu.download(lev, A)
-    for b, (lo, hi) in enumerate(boxes):
-        v_got = got[b][:, 4:-4, 4:-4, 4:-4]
-        v_ref = fa[b].a[:, 4:-4, 4:-4, 4:-4]
-        assert gu.max_rel(v_got, v_ref) < TOL64
+    v_got = [x[:, 4:-4, 4:-4, 4:-4] for x in got]
+    v_ref = [f.a[:, 4:-4, 4:-4, 4:-4] for f in fa]
+    assert gu.level_max_rel(v_got, v_ref) < TOL64
 
 
 def _run_pair(case, cfl, stop_time, nsteps, flags, scheme=2, visc=0):
@@ -169,9 +166,7 @@ def _run_pair(case, cfl, stop_time, nsteps, flags, scheme=2, visc=0):
     orc.init_case(kind, boxes, fa, og, orc.parm(), seed=42)
     st_o, t_o, _ = orc.run(boxes, fa, fb, fc, og, orc.parm(), cfl, scheme, visc, 0, stop_time, nsteps)
     got = gu.download(lev, A)
-    errs = []
-    for b in range(len(boxes)):
-        errs.append(gu.rel_err(got[b][:, 4:-4, 4:-4, 4:-4], fa[b].a[:, 4:-4, 4:-4, 4:-4]))
+    errs = gu.level_rel_err([x[:, 4:-4, 4:-4, 4:-4] for x in got], [f.a[:, 4:-4, 4:-4, 4:-4] for f in fa])
     return (st_g, t_g), (st_o, t_o), errs
 
 
@@ -180,8 +175,8 @@ def test_sod_to_t02_fp64(torch_cuda):
     from ocfd import cases
     (sg, tg), (so, to), errs = _run_pair(cases.sod(512, 8, 8), 0.5, 0.2, -1, 0)
     assert sg == so and tg == to
-    worst = max(max(max(e) for e in be) for be in errs)
-    print("sod 512x8 fp64 worst rel err", worst)
+    worst = max(max(e) for e in errs)
+    print("sod 512x8 fp64 (L1, Linf) rel err per variable", errs)
     assert worst < TOL64
 
 
@@ -190,11 +185,13 @@ def test_kh3d_steps_fp64_and_fp32(torch_cuda):
     from ocfd import cases
     case = cases.kh3d(32, max_grid=(32, 16, 32))
     (sg, tg), (so, to), errs = _run_pair(case, 0.5, -1.0, 5, ocfd.VISC, visc=1)
-    worst = max(max(max(e) for e in be) for be in errs)
+    worst = max(max(e) for e in errs)
+    print("kh fp64 (L1, Linf) per variable", errs)
     assert sg == so == 5
     assert worst < TOL64, worst
     (_, _), (_, _), errs32 = _run_pair(case, 0.5, -1.0, 5, ocfd.VISC | ocfd.FLUX_FP32, visc=1)
-    l1_rho = max(be[0][0] for be in errs32)
+    l1_rho = errs32[0][0]
+    print("kh fp32 (L1, Linf) per variable", errs32)
     print("kh fp32 rel L1 rho", l1_rho)
     assert l1_rho < 1e-5
 
@@ -204,8 +201,9 @@ def test_dmr_uneven_steps_fp64(torch_cuda):
     from ocfd import cases
     case = cases.dmr(128, 32, 2, sizes=(8, 16, 32), seed=3)
     (sg, tg), (so, to), errs = _run_pair(case, 0.5, -1.0, 10, 0)
-    worst = max(max(max(e) for e in be) for be in errs)
+    worst = max(max(e) for e in errs)
+    print("dmr fp64 (L1, Linf) per variable", errs)
     assert worst < TOL64, worst
     (_, _), (_, _), errs_p = _run_pair(case, 0.5, -1.0, 10, ocfd.PLANAR)
-    worst_p = max(max(max(e) for e in be) for be in errs_p)
+    worst_p = max(max(e) for e in errs_p)
     assert worst_p < TOL64, worst_p
