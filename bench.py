@@ -33,7 +33,22 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=128, help="cells per side of the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (HBM counter calibration)")
     return ap.parse_args()
+
+
+def calib_copy():
+    """Plain device-to-device copy of a known size: FETCH_SIZE / WRITE_SIZE of
+    this kernel divided by 256 MiB gives the counter's scale factor."""
+    import torch
+    n = 256 * 1024 * 1024 // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calib_copy_bytes": n * 8, "repeats": 3}))
 
 
 def timed_run(lev, A, B, C, p, flags, steps, warmup):
@@ -100,6 +115,9 @@ def main():
     from ocfd import cases
 
     torch.cuda.set_device(0)
+    if args.calib_copy:
+        calib_copy()
+        return
     n = args.n
     boxes, g, kind = cases.kh3d(n)
     lev = ocfd.Level(boxes, g)
