@@ -68,15 +68,17 @@ def kernel_roofline(lev, A, B, C, p, flags, ncell):
     the launch stream) and the roofline figures of the dominant kernel."""
     lev.fill_boundary(A, 0.0)
     ms = lev.profile_stage(A, B, C, 0.25, 0.75, 1e-9, p, flags, reps=5)
-    names = ["sweep_x", "sweep_y", "sweep_z_update", "fill_boundary"]
+    names = ["sweep_x", "sweep_y", "sweep_z_update", "fill_boundary", "primitives"]
     # algorithmic HBM bytes per valid cell of each kernel (FP64 state, 5 comps):
     #   sweep_x: read S 40, write dSdt partial 40
     #   sweep_y: read S 40, read+write partial 80
     #   sweep_z_update: read S 40, partial 40, U^n 40, write U^(k+1) 40
     #   fill: ghost cells only, read+write 80 B per ghost cell
+    #   primitives: read S 40, write 6 primitives (FP32 24 / FP64 48) per cell of grow(bx,3)
     nx = lev.boxes[0][1][0] - lev.boxes[0][0][0] + 1
     ghost = (nx + 8) ** 3 - nx ** 3
-    bytes_ = [80.0 * ncell, 120.0 * ncell, 160.0 * ncell, 80.0 * ghost]
+    pb = 24.0 if flags & 4 else 48.0
+    bytes_ = [80.0 * ncell, 120.0 * ncell, 160.0 * ncell, 80.0 * ghost, (40.0 + pb) * (nx + 6) ** 3]
     kern = {}
     for nm, t, b in zip(names, ms, bytes_):
         kern[nm] = {"ms": t, "GBps": b / (t * 1e-3) / 1e9}

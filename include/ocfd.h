@@ -123,8 +123,9 @@ int ocfd_dmr_states(double* left, double* right);
 /* Per-kernel timing of one RK stage (the reference's BL_PROFILE scopes,
  * Source/Advance.cpp:184): launches each kernel of ocfd_rk_stage `reps`
  * times between HIP events on `stream` and writes the average duration in ms
- * of the x sweep, y sweep, z sweep (with the stage update) and the ghost fill
- * to ms_out[0..3].  The ghost fill runs on S_cur.  Not for OCFD_PLANAR. */
+ * of the x sweep, y sweep, z sweep (with the stage update), the ghost fill
+ * and the primitive recovery to ms_out[0..4].  The ghost fill runs on S_cur.
+ * Not for OCFD_PLANAR. */
 int ocfd_profile_stage(ocfd_level lev, const double* S_cur, const double* S_old, double* S_new,
                        double c_cur, double c_old, double c_dt, const ocfd_parm* parm, int flags,
                        int reps, double* ms_out, void* stream);

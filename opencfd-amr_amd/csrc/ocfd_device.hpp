@@ -31,6 +31,7 @@ struct BoxDesc {
     long off_d;           // offset (doubles) of this box's valid FAB in a dSdt MultiFab
     long nptg;            // component stride of the grown FAB
     long nptv;            // component stride of the valid FAB
+    long off_p;           // offset (elements) of this box's primitive FAB (6 comps, grown layout)
     double dxinv[3];
     double dx[3];
 };
@@ -53,6 +54,7 @@ struct PhysArgs {
 };
 
 struct StageArgs {
+    const void* prim;     // primitives of scur (FT, 6 comps r,u,v,w,c,T, grown layout)
     const double* scur;   // state being differentiated (4 ghosts, filled)
     const double* sold;   // U^n for the stage combination, or null
     double* snew;         // stage output (valid cells), or null for a plain dSdt
@@ -61,6 +63,15 @@ struct StageArgs {
     int flags;
     PhysArgs ph;
 };
+
+// Workgroups are dealt round-robin over the 8 XCDs (workgroup b runs on XCD
+// b % 8), each with its own L2.  Give every XCD a contiguous range of the
+// work-list instead, so tiles that share halo cells share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+    const int q = n >> 3, r = n & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
 
 __device__ __forceinline__ long sidx(const BoxDesc& B, int i, int j, int k)
 {
@@ -97,6 +108,22 @@ __device__ __forceinline__ Prim<FT> c2prim(const double* __restrict__ s, long id
     q.T = (FT)T;
     q.c = (FT)sqrt(ph.gamma * ph.Rg * T);
     return q;
+}
+
+// primitive state of one cell from the primitive MultiFab
+template <class FT>
+__device__ __forceinline__ Prim<FT> ldprim(const FT* __restrict__ P, long idx, long stride)
+{
+    Prim<FT> q;
+    q.r = P[idx]; q.u = P[idx + stride]; q.v = P[idx + 2 * stride]; q.w = P[idx + 3 * stride];
+    q.c = P[idx + 4 * stride]; q.T = P[idx + 5 * stride];
+    return q;
+}
+
+template <class FT>
+__device__ __forceinline__ void ldvel(const FT* __restrict__ P, long idx, long stride, FT& u, FT& v, FT& w)
+{
+    u = P[idx + stride]; v = P[idx + 2 * stride]; w = P[idx + 3 * stride];
 }
 
 // velocity only (viscous tangential stencils)

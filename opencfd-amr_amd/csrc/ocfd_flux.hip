@@ -20,7 +20,7 @@
 namespace ocfd {
 
 constexpr int XSEG = 58;   // interior cells of one x-row wave
-constexpr int CH = 16;     // cells per march chunk (y and z sweeps)
+constexpr int CH = 32;     // cells per march chunk (y and z sweeps)
 
 template <class FT, bool VISC>
 __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
@@ -28,7 +28,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
 {
     __shared__ FT sf[4][10][64];
     __shared__ FT sv[VISC ? 4 : 1][VISC ? 12 : 1][64];
-    const int4 t = tiles[blockIdx.x];
+    const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const BoxDesc B = boxes[t.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
@@ -41,9 +41,10 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
     const FT gam = (FT)a.ph.gamma;
     (void)nx;
 
+    const FT* __restrict__ P = (const FT*)a.prim;
     if (cellok) {
-        const long s0 = sidx(B, i, j, k);
-        Prim<FT> q = c2prim<FT>(a.scur, s0, B.nptg, a.ph);
+        const long s0 = sidx(B, i, j, k) - B.off_s + B.off_p;
+        Prim<FT> q = ldprim<FT>(P, s0, B.nptg);
         FT fp[5], fm[5];
         sw_split<0, FT>(q, gam, fp, fm);
 #pragma unroll
@@ -51,13 +52,13 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
         if (VISC) {
             sv[w][0][lane] = q.T; sv[w][1][lane] = q.u; sv[w][2][lane] = q.v; sv[w][3][lane] = q.w;
             FT u, v, ww;
-            c2vel<FT>(a.scur, s0 + B.nxg, B.nptg, a.ph, u, v, ww);              // (i, j+1)
+            ldvel<FT>(P, s0 + B.nxg, B.nptg, u, v, ww);              // (i, j+1)
             sv[w][4][lane] = u; sv[w][5][lane] = v;
-            c2vel<FT>(a.scur, s0 - B.nxg, B.nptg, a.ph, u, v, ww);              // (i, j-1)
+            ldvel<FT>(P, s0 - B.nxg, B.nptg, u, v, ww);              // (i, j-1)
             sv[w][6][lane] = u; sv[w][7][lane] = v;
-            c2vel<FT>(a.scur, s0 + (long)B.nxg * B.nyg, B.nptg, a.ph, u, v, ww); // (i, k+1)
+            ldvel<FT>(P, s0 + (long)B.nxg * B.nyg, B.nptg, u, v, ww); // (i, k+1)
             sv[w][8][lane] = u; sv[w][9][lane] = ww;
-            c2vel<FT>(a.scur, s0 - (long)B.nxg * B.nyg, B.nptg, a.ph, u, v, ww); // (i, k-1)
+            ldvel<FT>(P, s0 - (long)B.nxg * B.nyg, B.nptg, u, v, ww); // (i, k-1)
             sv[w][10][lane] = u; sv[w][11][lane] = ww;
         }
     }
@@ -103,7 +104,7 @@ template <int D, class FT, bool VISC, bool FINAL>
 __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                      StageArgs a)
 {
-    const int4 t = tiles[blockIdx.x];
+    const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const BoxDesc B = boxes[t.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // the "other" transverse direction: z for the y sweep, y for the z sweep
@@ -119,7 +120,8 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
     const long ostep = O == 1 ? (long)B.nxg : (long)B.nxg * B.nyg;   // index step along O
     const double dxi = B.dxinv[D];
     int jj = D == 1 ? m0 : o, kk = D == 1 ? o : m0;
-    const long base = sidx(B, i, jj, kk);           // state index of cell m0
+    const long base = sidx(B, i, jj, kk) - B.off_s + B.off_p;   // primitive index of cell m0
+    const FT* __restrict__ P = (const FT*)a.prim;
     const long dbase = didx(B, i, jj, kk);
     const long dd = D == 1 ? (long)(B.hi[0] - B.lo[0] + 1)
                            : (long)(B.hi[0] - B.lo[0] + 1) * (B.hi[1] - B.lo[1] + 1);
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
     // prologue: cells m0-3 .. m0+1
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-        Prim<FT> q = c2prim<FT>(a.scur, base + (c - 3) * dstep, st, a.ph);
+        Prim<FT> q = ldprim<FT>(P, base + (c - 3) * dstep, st);
         FT fp[5], fm[5];
         sw_split<D, FT>(q, gam, fp, fm);
 #pragma unroll
@@ -143,15 +145,15 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
     FT lxpL_a = 0, lxpL_b = 0, lxmL_a = 0, lxmL_b = 0, lopL_a = 0, lopL_b = 0, lomL_a = 0, lomL_b = 0;
     if (VISC) {
         const long s = base - dstep;
-        PL = c2prim<FT>(a.scur, s, st, a.ph);
+        PL = ldprim<FT>(P, s, st);
         FT u, v, w2;
-        c2vel<FT>(a.scur, s + 1, st, a.ph, u, v, w2);
+        ldvel<FT>(P, s + 1, st, u, v, w2);
         lxpL_a = u; lxpL_b = D == 1 ? v : w2;
-        c2vel<FT>(a.scur, s - 1, st, a.ph, u, v, w2);
+        ldvel<FT>(P, s - 1, st, u, v, w2);
         lxmL_a = u; lxmL_b = D == 1 ? v : w2;
-        c2vel<FT>(a.scur, s + ostep, st, a.ph, u, v, w2);
+        ldvel<FT>(P, s + ostep, st, u, v, w2);
         lopL_a = D == 1 ? v : v; lopL_b = w2;
-        c2vel<FT>(a.scur, s - ostep, st, a.ph, u, v, w2);
+        ldvel<FT>(P, s - ostep, st, u, v, w2);
         lomL_a = v; lomL_b = w2;
     }
 
@@ -159,7 +161,7 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
     for (int f = m0; f <= m1 + 1; ++f) {
         // load cell f+2
         {
-            Prim<FT> q = c2prim<FT>(a.scur, base + (long)(f + 2 - m0) * dstep, st, a.ph);
+            Prim<FT> q = ldprim<FT>(P, base + (long)(f + 2 - m0) * dstep, st);
             FT fp[5], fm[5];
             sw_split<D, FT>(q, gam, fp, fm);
 #pragma unroll
@@ -172,15 +174,15 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
                  + weno_m<FT>(Wm[0][n], Wm[1][n], Wm[2][n], Wm[3][n], Wm[4][n]);
         if (VISC) {
             const long s = base + (long)(f - m0) * dstep;
-            Prim<FT> PR = c2prim<FT>(a.scur, s, st, a.ph);
+            Prim<FT> PR = ldprim<FT>(P, s, st);
             FT u, v, w2;
-            c2vel<FT>(a.scur, s + 1, st, a.ph, u, v, w2);
+            ldvel<FT>(P, s + 1, st, u, v, w2);
             FT xpa = u, xpb = D == 1 ? v : w2;
-            c2vel<FT>(a.scur, s - 1, st, a.ph, u, v, w2);
+            ldvel<FT>(P, s - 1, st, u, v, w2);
             FT xma = u, xmb = D == 1 ? v : w2;
-            c2vel<FT>(a.scur, s + ostep, st, a.ph, u, v, w2);
+            ldvel<FT>(P, s + ostep, st, u, v, w2);
             FT opa = v, opb = w2;
-            c2vel<FT>(a.scur, s - ostep, st, a.ph, u, v, w2);
+            ldvel<FT>(P, s - ostep, st, u, v, w2);
             FT oma = v, omb = w2;
             // D=1: ta=lat(u,x) tb=lat(v,x) tc=lat(v,z) td=lat(w,z)
             // D=2: ta=lat(u,x) tb=lat(w,x) tc=lat(v,y) td=lat(w,y)
@@ -224,12 +226,42 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
     }
 }
 
+// Primitive recovery (c2prim, Source/kernel/Kernels.H:13-40) once per cell
+// of the stencil region of every box, in FP64, stored narrowed to FT.
+// tiles: (box, first cell of the region, count, z growth of the region).
+template <class FT>
+__global__ __launch_bounds__(256) void k_prims(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+                                               StageArgs a)
+{
+    const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    if ((int)threadIdx.x >= t.z) return;
+    const BoxDesc B = boxes[t.x];
+    const int gz = t.w;
+    const int ex = B.hi[0] - B.lo[0] + 7, ey = B.hi[1] - B.lo[1] + 7;
+    const int g = t.y + threadIdx.x;
+    const int i = B.lo[0] - 3 + g % ex;
+    const int j = B.lo[1] - 3 + (g / ex) % ey;
+    const int k = B.lo[2] - gz + g / (ex * ey);
+    const long s0 = sidx(B, i, j, k);
+    const Prim<FT> q = c2prim<FT>(a.scur, s0, B.nptg, a.ph);
+    FT* __restrict__ P = (FT*)a.prim;
+    const long p0 = s0 - B.off_s + B.off_p;
+    const long st = B.nptg;
+    P[p0] = q.r; P[p0 + st] = q.u; P[p0 + 2 * st] = q.v; P[p0 + 3 * st] = q.w;
+    P[p0 + 4 * st] = q.c; P[p0 + 5 * st] = q.T;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <class FT, bool VISC>
-static void launch_sweeps_t(const LevelDev& L, const StageArgs& a, bool planar, hipStream_t s)
+static void launch_sweeps_t(const LevelDev& L, const StageArgs& a0, bool planar, hipStream_t s)
 {
+    StageArgs a = a0;
+    a.prim = sizeof(FT) == 4 ? (const void*)L.d_prim32 : (const void*)L.d_prim64;
+    const int np = planar ? L.ntile_pp : L.ntile_p;
+    const int4* tp = planar ? L.d_tile_pp : L.d_tile_p;
+    if (np) hipLaunchKernelGGL((k_prims<FT>), dim3(np), dim3(256), 0, s, L.d_boxes, tp, a);
     if (L.ntile_x) hipLaunchKernelGGL((k_sweep_x<FT, VISC>), dim3(L.ntile_x), dim3(256), 0, s, L.d_boxes, L.d_tile_x, a);
     if (planar) {
         if (L.ntile_y) hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, true>), dim3(L.ntile_y), dim3(256), 0, s,
@@ -256,13 +288,17 @@ void launch_sweeps(const LevelDev& L, const StageArgs& a, hipStream_t s)
 }
 
 // One sweep alone (0 = x, 1 = y, 2 = z final), for per-kernel timing.
-void launch_sweep_one(const LevelDev& L, const StageArgs& a, int which, hipStream_t s)
+void launch_sweep_one(const LevelDev& L, const StageArgs& a0, int which, hipStream_t s)
 {
-    const bool visc = (a.flags & OCFD_VISC) != 0;
-    const bool f32 = (a.flags & OCFD_FLUX_FP32) != 0;
+    const bool visc = (a0.flags & OCFD_VISC) != 0;
+    const bool f32 = (a0.flags & OCFD_FLUX_FP32) != 0;
+    StageArgs a = a0;
+    a.prim = f32 ? (const void*)L.d_prim32 : (const void*)L.d_prim64;
 #define OCFD_SW(FT, V)                                                                                           \
     do {                                                                                                         \
-        if (which == 0) hipLaunchKernelGGL((k_sweep_x<FT, V>), dim3(L.ntile_x), dim3(256), 0, s, L.d_boxes,      \
+        if (which == 4) hipLaunchKernelGGL((k_prims<FT>), dim3(L.ntile_p), dim3(256), 0, s, L.d_boxes,           \
+                                           L.d_tile_p, a);                                                       \
+        else if (which == 0) hipLaunchKernelGGL((k_sweep_x<FT, V>), dim3(L.ntile_x), dim3(256), 0, s, L.d_boxes,      \
                                            L.d_tile_x, a);                                                       \
         else if (which == 1) hipLaunchKernelGGL((k_sweep_march<1, FT, V, false>), dim3(L.ntile_y), dim3(256), 0, \
                                                 s, L.d_boxes, L.d_tile_y, a);                                     \

@@ -222,7 +222,8 @@ static void free_level(LevelDev& L)
 {
     hipFree(L.d_boxes); hipFree(L.d_geom); hipFree(L.d_map);
     hipFree(L.d_tile_x); hipFree(L.d_tile_y); hipFree(L.d_tile_z); hipFree(L.d_tile_g); hipFree(L.d_tile_c);
-    hipFree(L.d_acc); hipFree(L.d_red);
+    hipFree(L.d_acc); hipFree(L.d_red); hipFree(L.d_tile_p); hipFree(L.d_tile_pp);
+    hipFree(L.d_prim32); hipFree(L.d_prim64);
     L = LevelDev{};
 }
 
@@ -286,6 +287,7 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
         B.nptv = (long)(B.hi[0] - B.lo[0] + 1) * (B.hi[1] - B.lo[1] + 1) * (B.hi[2] - B.lo[2] + 1);
         B.off_s = off_s;
         B.off_d = off_d;
+        B.off_p = off_s / NC * 6;
         B.geom = 0;
         off_s += B.nptg * NC;
         off_d += B.nptv * NC;
@@ -339,35 +341,48 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
 
     // work-lists
     const int xseg = tile_xseg(), ch = tile_chunk();
-    std::vector<int4> tx, ty, tz, tg, tc;
+    std::vector<int4> tx, ty, tz, tg, tc, tp, tpp;
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
         const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
         const int nseg = (nx + xseg - 1) / xseg, nrg = (ny * nz + 3) / 4;
         for (int r = 0; r < nrg; ++r)
             for (int s = 0; s < nseg; ++s) tx.push_back(make_int4(b, s, r, 0));
+        // march tiles: i segment fastest, then chunk, then row group, so a
+        // contiguous range of the list (one XCD's share) holds tiles that
+        // share chunk halos and transverse neighbour rows
         const int iseg = (nx + 63) / 64;
-        for (int c = 0; c < (ny + ch - 1) / ch; ++c)
-            for (int kg = 0; kg < (nz + 3) / 4; ++kg)
+        for (int kg = 0; kg < (nz + 3) / 4; ++kg)
+            for (int c = 0; c < (ny + ch - 1) / ch; ++c)
                 for (int s = 0; s < iseg; ++s) ty.push_back(make_int4(b, s, kg, c));
-        for (int c = 0; c < (nz + ch - 1) / ch; ++c)
-            for (int jg = 0; jg < (ny + 3) / 4; ++jg)
+        for (int jg = 0; jg < (ny + 3) / 4; ++jg)
+            for (int c = 0; c < (nz + ch - 1) / ch; ++c)
                 for (int s = 0; s < iseg; ++s) tz.push_back(make_int4(b, s, jg, c));
         const long ngh = B.nptg - B.nptv;
         for (long s = 0; s < ngh; s += 256) tg.push_back(make_int4(b, (int)s, (int)std::min<long>(256, ngh - s), 0));
         for (long s = 0; s < B.nptv; s += 2048) tc.push_back(make_int4(b, (int)s, (int)std::min<long>(2048, B.nptv - s), 0));
+        for (int gz : {3, 1}) {
+            const long np = (long)(nx + 6) * (ny + 6) * (nz + 2 * gz);
+            std::vector<int4>& tl = gz == 3 ? tp : tpp;
+            for (long s = 0; s < np; s += 256) tl.push_back(make_int4(b, (int)s, (int)std::min<long>(256, np - s), gz));
+        }
     }
     int rc = OCFD_OK;
     if ((rc = upload(&L.d_map, L.owner_map)) || (rc = upload(&L.d_tile_x, tx)) || (rc = upload(&L.d_tile_y, ty)) ||
-        (rc = upload(&L.d_tile_z, tz)) || (rc = upload(&L.d_tile_g, tg)) || (rc = upload(&L.d_tile_c, tc))) {
+        (rc = upload(&L.d_tile_z, tz)) || (rc = upload(&L.d_tile_g, tg)) || (rc = upload(&L.d_tile_c, tc)) ||
+        (rc = upload(&L.d_tile_p, tp)) || (rc = upload(&L.d_tile_pp, tpp))) {
         free_level(L); delete lev; return rc;
     }
     L.ntile_x = (int)tx.size(); L.ntile_y = (int)ty.size(); L.ntile_z = (int)tz.size();
     L.ntile_g = (int)tg.size(); L.ntile_c = (int)tc.size();
+    L.ntile_p = (int)tp.size(); L.ntile_pp = (int)tpp.size();
+    L.mf_prim = L.mf_state / NC * 6;
     G.map = L.d_map;
     std::vector<GeomDesc> gv(1, G);
     if ((rc = upload(&L.d_geom, gv)) || (rc = upload(&L.d_boxes, L.boxes))) { free_level(L); delete lev; return rc; }
     if (hipMalloc((void**)&L.d_acc, sizeof(double) * L.mf_valid) != hipSuccess ||
+        hipMalloc((void**)&L.d_prim32, sizeof(float) * L.mf_prim) != hipSuccess ||
+        hipMalloc((void**)&L.d_prim64, sizeof(double) * L.mf_prim) != hipSuccess ||
         hipMalloc((void**)&L.d_red, sizeof(unsigned long long)) != hipSuccess) {
         set_error("ocfd_level_create: out of device memory");
         free_level(L); delete lev; return OCFD_ENOMEM;
@@ -518,10 +533,11 @@ int ocfd_profile_stage(ocfd_level lev, const double* S_cur, const double* S_old,
     hipEvent_t e0, e1;
     OCFD_HIP(hipEventCreate(&e0));
     OCFD_HIP(hipEventCreate(&e1));
-    for (int which = 0; which < 4; ++which) {
+    launch_sweep_one(lev->dev, a, 4, s);   // primitives of S_cur for the sweeps
+    for (int which = 0; which < 5; ++which) {
         OCFD_HIP(hipEventRecord(e0, s));
         for (int r = 0; r < reps; ++r) {
-            if (which < 3) launch_sweep_one(lev->dev, a, which, s);
+            if (which < 3 || which == 4) launch_sweep_one(lev->dev, a, which, s);
             else hipLaunchKernelGGL(k_fill, dim3(lev->dev.ntile_g), dim3(256), 0, s, lev->dev.d_boxes,
                                     lev->dev.d_geom, lev->dev.d_tile_g, (double*)S_cur, 0.0);
         }

@@ -203,8 +203,8 @@ class Level:
                                  flags, _stream(stream)), "ocfd_advance")
 
     def profile_stage(self, S_cur, S_old, S_new, c_cur, c_old, c_dt, parm, flags=0, reps=5, stream=None):
-        """Average ms of (x sweep, y sweep, z sweep + update, ghost fill)."""
-        out = (ctypes.c_double * 4)()
+        """Average ms of (x sweep, y sweep, z sweep + update, ghost fill, primitives)."""
+        out = (ctypes.c_double * 5)()
         check(lib().ocfd_profile_stage(self.handle, _ptr(S_cur), _ptr(S_old), _ptr(S_new), c_cur, c_old, c_dt,
                                        ctypes.byref(parm), flags, reps, out, _stream(stream)), "ocfd_profile_stage")
         return list(out)
