@@ -58,7 +58,8 @@ struct StageArgs {
     const double* scur;   // state being differentiated (4 ghosts, filled)
     const double* sold;   // U^n for the stage combination, or null
     double* snew;         // stage output (valid cells), or null for a plain dSdt
-    double* acc;          // dSdt accumulator / output (valid layout)
+    void* part;           // partial divergence dx-term (+dy-term), FT, valid layout, level-owned
+    double* acc;          // dSdt output of a plain compute_dSdt (valid layout)
     double c_cur, c_old, c_dt;
     int flags;
     PhysArgs ph;

@@ -12,8 +12,11 @@
 //   sweep_z : same march along k; adds the z-part and finishes the cell
 //             (gravity, RK stage combination or plain dSdt store).
 //
-// The divergence is accumulated in FP64 in the reference's order
-// dx-term + dy-term + dz-term (Source/kernel/Kernels.H:94-96).
+// The divergence is summed in FP64 in the reference's order
+// dx-term + dy-term + dz-term (Source/kernel/Kernels.H:94-96); the partial
+// sums between sweeps are stored in the flux type FT (FP64 in the plain form,
+// FP32 in the FP32-flux form, where they are differences of FP32 fluxes).
+// Every sweep reads the primitive MultiFab written once per stage by k_prims.
 #include "ocfd_device.hpp"
 #include "ocfd_internal.hpp"
 
@@ -92,9 +95,10 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
     __syncthreads();
     if (rowok && lane >= 3 && lane <= 60 && i <= B.hi[0]) {
         const long d0 = didx(B, i, j, k);
+        FT* __restrict__ part = (FT*)a.part;
 #pragma unroll
         for (int n = 0; n < 5; ++n)
-            a.acc[d0 + n * B.nptv] = B.dxinv[0] * ((double)sf[w][n][lane] - (double)sf[w][n][lane + 1]);
+            part[d0 + n * B.nptv] = (FT)(B.dxinv[0] * ((double)sf[w][n][lane] - (double)sf[w][n][lane + 1]));
     }
 }
 
@@ -201,13 +205,13 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
             double L[5];
 #pragma unroll
             for (int n = 0; n < 5; ++n)
-                L[n] = a.acc[d0 + n * B.nptv] + dxi * ((double)Fprev[n] - (double)F[n]);
+                L[n] = (double)((const FT*)a.part)[d0 + n * B.nptv] + dxi * ((double)Fprev[n] - (double)F[n]);
             if (FINAL) {
                 const int c = f - 1;
                 finish_cell(a, B, i, D == 1 ? c : o, D == 1 ? o : c, L);
             } else {
 #pragma unroll
-                for (int n = 0; n < 5; ++n) a.acc[d0 + n * B.nptv] = L[n];
+                for (int n = 0; n < 5; ++n) ((FT*)a.part)[d0 + n * B.nptv] = (FT)L[n];
             }
         }
 #pragma unroll
@@ -259,6 +263,7 @@ static void launch_sweeps_t(const LevelDev& L, const StageArgs& a0, bool planar,
 {
     StageArgs a = a0;
     a.prim = sizeof(FT) == 4 ? (const void*)L.d_prim32 : (const void*)L.d_prim64;
+    a.part = (void*)L.d_acc;
     const int np = planar ? L.ntile_pp : L.ntile_p;
     const int4* tp = planar ? L.d_tile_pp : L.d_tile_p;
     if (np) hipLaunchKernelGGL((k_prims<FT>), dim3(np), dim3(256), 0, s, L.d_boxes, tp, a);
@@ -294,6 +299,7 @@ void launch_sweep_one(const LevelDev& L, const StageArgs& a0, int which, hipStre
     const bool f32 = (a0.flags & OCFD_FLUX_FP32) != 0;
     StageArgs a = a0;
     a.prim = f32 ? (const void*)L.d_prim32 : (const void*)L.d_prim64;
+    a.part = (void*)L.d_acc;
 #define OCFD_SW(FT, V)                                                                                           \
     do {                                                                                                         \
         if (which == 4) hipLaunchKernelGGL((k_prims<FT>), dim3(L.ntile_p), dim3(256), 0, s, L.d_boxes,           \
