@@ -222,3 +222,168 @@ def init_case(kind, boxes, fabs, g, p, seed=0):
 
 def new_level_fabs(boxes, g=NGROW):
     return [Fab(*grown(lo, hi, g)) for lo, hi in boxes]
+
+
+# ---------------------------------------------------------------------------
+# chemistry (Li-Dryer H2/air, 9 species: H2 O2 H2O H O OH HO2 H2O2 N2)
+# ---------------------------------------------------------------------------
+NSPEC = 9
+SPECIES = ["H2", "O2", "H2O", "H", "O", "OH", "HO2", "H2O2", "N2"]
+
+
+def _chem_sigs():
+    L = lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    if getattr(L, "_chem_ready", False):
+        return L
+    L.orc_chem_tables.argtypes = [dp, dp, dp]
+    L.orc_production_rate.argtypes = [dp, ctypes.c_double, dp, dp]
+    L.orc_gauss.argtypes = [dp, dp, dp]
+    L.orc_T_given_ey.argtypes = [ctypes.c_double, dp, ip]
+    L.orc_T_given_ey.restype = ctypes.c_double
+    for nm in ("orc_CKUBMS", "orc_CKCVBS", "orc_CKCPBS", "orc_CKGAMMA"):
+        getattr(L, nm).argtypes = [ctypes.c_double, dp]
+        getattr(L, nm).restype = ctypes.c_double
+    L.orc_CKPY.argtypes = [dp, ctypes.c_double]
+    L.orc_CKPY.restype = ctypes.c_double
+    L.orc_CKRHOY.argtypes = [ctypes.c_double, ctypes.c_double, dp]
+    L.orc_CKRHOY.restype = ctypes.c_double
+    L.orc_CKXTY.argtypes = [dp, dp]
+    L.orc_CKYTX.argtypes = [dp, dp]
+    L.orc_species_enthalpy.argtypes = [ctypes.c_double, dp]
+    L.orc_chem_step_many.argtypes = [ctypes.c_long, dp, dp, ctypes.c_double, ctypes.c_double]
+    L.orc_renorm_many.argtypes = [ctypes.c_long, dp, dp]
+    L.orc_reactor_init.argtypes = [ctypes.c_long, dp, dp, dp, dp, dp]
+    L.orc_reactor_fixed.argtypes = [ctypes.c_long, dp, dp, ctypes.c_double, ctypes.c_int]
+    L.orc_reactor_adaptive.argtypes = [ctypes.c_long, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ip, ip]
+    L.orc_reactor_T.argtypes = [ctypes.c_long, dp, dp, dp]
+    L._chem_ready = True
+    return L
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def chem_tables():
+    mw = np.zeros(9); lo = np.zeros(63); hi = np.zeros(63)
+    _chem_sigs().orc_chem_tables(_dp(mw), _dp(lo), _dp(hi))
+    return mw, lo.reshape(9, 7), hi.reshape(9, 7)
+
+
+def production_rate(c, T):
+    c = _c(c)
+    w = np.zeros(9); J = np.zeros(81)
+    _chem_sigs().orc_production_rate(_dp(c), T, _dp(w), _dp(J))
+    return w, J.reshape(9, 9)
+
+
+def gauss(A, b):
+    A = _c(A); b = _c(b); x = np.zeros(9)
+    _chem_sigs().orc_gauss(_dp(A), _dp(b), _dp(x))
+    return x
+
+
+def T_given_ey(e, rhoi):
+    it = ctypes.c_int()
+    T = _chem_sigs().orc_T_given_ey(e, _dp(_c(rhoi)), ctypes.byref(it))
+    return T, it.value
+
+
+def thermo(name, *args):
+    L = _chem_sigs()
+    conv = [_dp(_c(a)) if isinstance(a, (list, np.ndarray)) else a for a in args]
+    return getattr(L, "orc_" + name)(*conv)
+
+
+def CKXTY(x):
+    y = np.zeros(9)
+    _chem_sigs().orc_CKXTY(_dp(_c(x)), _dp(y))
+    return y
+
+
+def CKYTX(y):
+    x = np.zeros(9)
+    _chem_sigs().orc_CKYTX(_dp(_c(y)), _dp(x))
+    return x
+
+
+def chem_step(u, rhoi, dt, smallr=1e-19):
+    u = _c(u).copy(); r = _c(rhoi).copy()
+    _chem_sigs().orc_chem_step_many(len(u) if u.ndim == 2 else 1, _dp(u), _dp(r), dt, smallr)
+    return u, r
+
+
+def renorm(u, rhoi):
+    u = _c(u); r = _c(rhoi).copy()
+    _chem_sigs().orc_renorm_many(len(u) if u.ndim == 2 else 1, _dp(u), _dp(r))
+    return r
+
+
+def reactor_init(T0, p0, phi):
+    T0, p0, phi = _c(T0), _c(p0), _c(phi)
+    n = len(T0)
+    u = np.zeros((n, 5)); r = np.zeros((n, 9))
+    _chem_sigs().orc_reactor_init(n, _dp(T0), _dp(p0), _dp(phi), _dp(u), _dp(r))
+    return u, r
+
+
+def reactor_fixed(u, r, dt, nsteps):
+    u = _c(u).copy(); r = _c(r).copy()
+    _chem_sigs().orc_reactor_fixed(len(u), _dp(u), _dp(r), dt, nsteps)
+    return u, r
+
+
+def reactor_adaptive(u, r, t_end, h0=1e-8, rtol=1e-6, atol=1e-12, max_steps=200000):
+    u = _c(u).copy(); r = _c(r).copy()
+    n = len(u)
+    steps = np.zeros(n, dtype=np.int32); rej = np.zeros(n, dtype=np.int32)
+    _chem_sigs().orc_reactor_adaptive(n, _dp(u), _dp(r), t_end, h0, rtol, atol, max_steps,
+                                      steps.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                      rej.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return u, r, steps, rej
+
+
+def reactor_T(u, r):
+    u = _c(u); r = _c(r); T = np.zeros(len(u))
+    _chem_sigs().orc_reactor_T(len(u), _dp(u), _dp(r), _dp(T))
+    return T
+
+
+# the reference's own AMReX-free kinetics (Test/chemical_test.cpp), built by oracle/ref.mk
+REF_DIR = HERE / "_ref"
+_ref = None
+
+
+def ref_chem():
+    """ctypes handle of oracle/_ref/libchemref.so, or None when it was not built."""
+    global _ref
+    if _ref is None:
+        p = REF_DIR / "libchemref.so"
+        if not p.exists():
+            return None
+        L = ctypes.CDLL(str(p))
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.vproductionRate = L._Z15vproductionRatePdRA9_A9_dS_d
+        L.vproductionRate.argtypes = [dp, dp, dp, ctypes.c_double]
+        L.GET_T_GIVEN_EY = L._Z14GET_T_GIVEN_EYdPdRd
+        L.GET_T_GIVEN_EY.argtypes = [ctypes.c_double, dp, dp]
+        L.gauss = L._Z5gaussPdRA9_A9_dS_
+        L.gauss.argtypes = [dp, dp, dp]
+        for nm, mang in (("CKUBMS", "_Z6CKUBMSdPdRd"), ("CKCVBS", "_Z6CKCVBSdPdRd"), ("CKCPBS", "_Z6CKCPBSdPdRd"),
+                         ("CKGAMMA", "_Z7CKGAMMAdPdRd")):
+            f = getattr(L, mang)
+            f.argtypes = [ctypes.c_double, dp, dp]
+            setattr(L, nm, f)
+        L.CKPY = L._Z4CKPYPddRd
+        L.CKPY.argtypes = [dp, ctypes.c_double, dp]
+        L.CKRHOY = L._Z6CKRHOYddPdRd
+        L.CKRHOY.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp]
+        L.CKXTY = L._Z5CKXTYPdS_
+        L.CKXTY.argtypes = [dp, dp]
+        L.CKYTX = L._Z5CKYTXPdS_
+        L.CKYTX.argtypes = [dp, dp]
+        _ref = L
+    return _ref
