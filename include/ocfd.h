@@ -130,6 +130,47 @@ int ocfd_profile_stage(ocfd_level lev, const double* S_cur, const double* S_old,
                        double c_cur, double c_old, double c_dt, const ocfd_parm* parm, int flags,
                        int reps, double* ms_out, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Chemistry: Li-Dryer 9-species / 19-reaction H2/air mechanism, species order
+ * H2 O2 H2O H O OH HO2 H2O2 N2 (Source/Constants.cpp:13-22).
+ * Species MultiFabs use the level layout with 4 ghosts and 9 components
+ * (Spec_Type, Source/Setup.cpp:242-246); size ocfd_level_mf_size(lev, 4, 9).
+ * ------------------------------------------------------------------------- */
+
+/* EBR::chemical_advance (Source/CHEM/ChemAdvance.cpp:7-85): one
+ * linearised-implicit step of dt per valid cell (Newton temperature from the
+ * sensible energy, production rates and mass-action Jacobian, 9x9 pivoted
+ * Gauss solve, clip at 0, heat release into E), in place on S and Spec. */
+int ocfd_chem_advance(ocfd_level lev, double* S, double* Spec, double dt, const ocfd_parm* parm, void* stream);
+
+/* Species clip and rescale to rho (EBR::post_timestep, Source/EBR.cpp:333-357). */
+int ocfd_species_renorm(ocfd_level lev, const double* S, double* Spec, void* stream);
+
+/* Batched constant-volume reactors (0D ignition).  state: device array of
+ * 11 rows of n doubles (SoA): rho, sensible energy density E (J/m^3), then the
+ * 9 partial densities (kg/m^3).  ocfd_reactor_init fills it for H2/air at
+ * (T0 [K], p0 [Pa], equivalence ratio phi) with X(H2):X(O2):X(N2) = 2phi:1:3.76
+ * (CKXTY/CKRHOY/CKUBMS, Source/CHEM/LiDryer.H:21-66,697-715). */
+#define OCFD_REACTOR_NCOMP 11
+int ocfd_reactor_init(long n, const double* T0, const double* p0, const double* phi, double* state, void* stream);
+
+/* nsteps linearised-implicit steps of dt per reactor: the reference's
+ * fixed-step chemistry (Source/CHEM/ChemAdvance.cpp:48-81) applied to a
+ * reactor at rest. */
+int ocfd_reactor_advance_fixed(long n, double* state, double dt, int nsteps, void* stream);
+
+/* Integration to t_end with per-reactor adaptive substeps: step doubling on
+ * the same linearised-implicit step, error max_k |y2-y1|/(atol rho + rtol|y2|)
+ * over the partial densities, h <- h*clamp(0.9/sqrt(err), 0.2, 4).  One
+ * persistent launch; lanes take reactors from a device work counter.
+ * steps_out / rejects_out (device int[n], may be NULL): accepted steps (-1 if
+ * max_steps was hit) and rejected steps.  work: 8 bytes of device memory. */
+int ocfd_reactor_advance_adaptive(long n, double* state, double t_end, double h0, double rtol, double atol,
+                                  int max_steps, int* steps_out, int* rejects_out, void* work, void* stream);
+
+/* Temperature of each reactor (GET_T_GIVEN_EY, Source/CHEM/LiDryer.H:836-888). */
+int ocfd_reactor_temperature(long n, const double* state, double* T, void* stream);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ocfd_last_error(void);
 

@@ -25,6 +25,11 @@ HIP_FLAGS = [
 ]
 
 
+# per-file extra flags: the chemistry follows the reference's FP64 operation
+# order without FMA contraction
+EXTRA = {"ocfd_chem.hip": ["-ffp-contract=off"]}
+
+
 def sources():
     return sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
 
@@ -43,7 +48,7 @@ def build(verbose=False):
     for src in sources():
         obj = BUILD / (src.name + ".o")
         lang = ["-x", "hip"] if src.suffix == ".hip" else []
-        cmd = [HIPCC] + HIP_FLAGS + lang + ["-c", str(src), "-o", str(obj)]
+        cmd = [HIPCC] + HIP_FLAGS + EXTRA.get(src.name, []) + lang + ["-c", str(src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)

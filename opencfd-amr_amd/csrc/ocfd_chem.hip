@@ -1,0 +1,276 @@
+// Chemistry kernels: the per-cell linearised-implicit step over a level
+// (EBR::chemical_advance, Source/CHEM/ChemAdvance.cpp:7-85), the species
+// renormalisation (EBR::post_timestep, Source/EBR.cpp:333-357) and batched
+// constant-volume reactors (0D ignition) with fixed or adaptive substeps.
+//
+// The adaptive reactor kernel is persistent: every lane takes reactors from a
+// device work counter until the batch is exhausted, so lanes that finish an
+// easy reactor pick up the next one instead of idling behind a stiff one.
+#include <algorithm>
+#include <string>
+
+#include "ocfd_chem_device.hpp"
+#include "ocfd_device.hpp"
+#include "ocfd_internal.hpp"
+
+namespace ocfd {
+
+using chem::NS;
+
+// reactor batch layout: SoA of OCFD_REACTOR_NCOMP rows of n doubles:
+// row 0 rho, row 1 sensible energy density E, rows 2..10 partial densities
+constexpr int RROW_RHO = 0, RROW_E = 1, RROW_Y = 2;
+
+__global__ __launch_bounds__(64) void k_chem_level(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+                                                   double* __restrict__ S, double* __restrict__ Spec, double dt,
+                                                   double smallr)
+{
+    const int4 t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.z) return;
+    const BoxDesc B = boxes[t.x];
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
+    const int g = t.y + threadIdx.x;
+    const int i = B.lo[0] + g % nx, j = B.lo[1] + (g / nx) % ny, k = B.lo[2] + g / (nx * ny);
+    const long s0 = sidx(B, i, j, k);
+    const long st = B.nptg;
+    const long p0 = s0 - B.off_s + B.off_s / NC * NS;   // species FAB: same grown layout, 9 comps
+    // Source/CHEM/ChemAdvance.cpp:36-46
+    const double rho = fmax(S[s0], smallr);
+    const double rhoinv = 1.0 / rho;
+    const double ux = S[s0 + st] * rhoinv, uy = S[s0 + 2 * st] * rhoinv, uz = S[s0 + 3 * st] * rhoinv;
+    const double kineng = 0.5 * (ux * ux + uy * uy + uz * uz);
+    const double ei = S[s0 + 4 * st] - rho * kineng;
+    double rhoi[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) rhoi[n] = Spec[p0 + n * st];
+    const double dei = chem::cell_step(ei, rhoi, dt);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) Spec[p0 + n * st] = rhoi[n];
+    S[s0 + 4 * st] += dei;
+}
+
+__global__ __launch_bounds__(256) void k_renorm_level(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+                                                      const double* __restrict__ S, double* __restrict__ Spec)
+{
+    const int4 t = tiles[blockIdx.x];
+    const BoxDesc B = boxes[t.x];
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
+    for (int c = threadIdx.x; c < t.z; c += 256) {
+        const int g = t.y + c;
+        const int i = B.lo[0] + g % nx, j = B.lo[1] + (g / nx) % ny, k = B.lo[2] + g / (nx * ny);
+        const long s0 = sidx(B, i, j, k);
+        const long st = B.nptg;
+        const long p0 = s0 - B.off_s + B.off_s / NC * NS;
+        // Source/EBR.cpp:344-354
+        double r[NS];
+        double rho0 = 0;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            r[n] = Spec[p0 + n * st];
+            if (r[n] < 0) r[n] = 0.0;
+            rho0 += r[n];
+        }
+        const double tmp = S[s0] / rho0;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) Spec[p0 + n * st] = r[n] * tmp;
+    }
+}
+
+// constant-volume H2/air reactor at (T0, p0, phi): X(H2):X(O2):X(N2) = 2phi:1:3.76
+__global__ void k_reactor_init(long n, const double* __restrict__ T0, const double* __restrict__ p0,
+                               const double* __restrict__ phi, double* __restrict__ st)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double X[NS] = {0}, Y[NS];
+    const double tot = 2.0 * phi[r] + 1.0 + 3.76;
+    X[0] = 2.0 * phi[r] / tot;
+    X[1] = 1.0 / tot;
+    X[8] = 3.76 / tot;
+    // CKXTY, CKRHOY (Source/CHEM/LiDryer.H:21-66)
+    double XW = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) XW += X[i] * chem::mw[i];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) Y[i] = X[i] * chem::mw[i] / XW;
+    double YOW = 0, tmp[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) tmp[i] = Y[i] / chem::mw[i];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) YOW += tmp[i];
+    const double rho = p0[r] / (chem::Ru * T0[r] * YOW);
+    double rhoi[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) rhoi[k] = Y[k] * rho;
+    st[RROW_RHO * n + r] = rho;
+    st[RROW_E * n + r] = chem::ckubms(T0[r], rhoi);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) st[(RROW_Y + k) * n + r] = rhoi[k];
+}
+
+__global__ __launch_bounds__(64) void k_reactor_fixed(long n, double* __restrict__ st, double dt, int nsteps)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double rhoi[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) rhoi[k] = st[(RROW_Y + k) * n + r];
+    double E = st[RROW_E * n + r];
+    for (int s = 0; s < nsteps; ++s) E += chem::cell_step(E, rhoi, dt);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) st[(RROW_Y + k) * n + r] = rhoi[k];
+    st[RROW_E * n + r] = E;
+}
+
+__global__ void k_counter_init(unsigned long long* c) { *c = 0ull; }
+
+// Adaptive integration with step doubling on the linearised-implicit step:
+// one step of h against two of h/2, error max_k |y2-y1|/(atol rho + rtol |y2|),
+// accept y2 and rescale h by clamp(0.9/sqrt(err), 0.2, 4).
+__global__ __launch_bounds__(64) void k_reactor_adaptive(long n, double* __restrict__ st, double t_end, double h0,
+                                                         double rtol, double atol, int max_steps,
+                                                         int* __restrict__ steps_out, int* __restrict__ rej_out,
+                                                         unsigned long long* __restrict__ counter)
+{
+    long r = (long)atomicAdd(counter, 1ull);
+    double rhoi[NS], E = 0, rho = 0, t = 0, h = h0;
+    int acc = 0, rej = 0;
+    auto load = [&](long q) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) rhoi[k] = st[(RROW_Y + k) * n + q];
+        E = st[RROW_E * n + q];
+        rho = st[RROW_RHO * n + q];
+        t = 0.0; h = h0; acc = 0; rej = 0;
+    };
+    if (r < n) load(r);
+    while (r < n) {
+        bool done = false, failed = false;
+        if (t >= t_end) done = true;
+        else if (acc + rej >= max_steps) { done = true; failed = true; }
+        if (!done) {
+            if (t + h > t_end) h = t_end - t;
+            double r1[NS], r2[NS];
+#pragma unroll
+            for (int k = 0; k < NS; ++k) { r1[k] = rhoi[k]; r2[k] = rhoi[k]; }
+            const double E1 = E + chem::cell_step(E, r1, h);
+            double E2 = E + chem::cell_step(E, r2, 0.5 * h);
+            E2 += chem::cell_step(E2, r2, 0.5 * h);
+            double err = 0.0;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) err = fmax(err, fabs(r2[k] - r1[k]) / (atol * rho + rtol * fabs(r2[k])));
+            (void)E1;
+            if (err <= 1.0) {
+#pragma unroll
+                for (int k = 0; k < NS; ++k) rhoi[k] = r2[k];
+                E = E2;
+                t += h;
+                ++acc;
+                const double f = err > 0.0 ? 0.9 / sqrt(err) : 4.0;
+                h *= fmin(4.0, fmax(0.2, f));
+            } else {
+                ++rej;
+                h *= fmax(0.2, 0.9 / sqrt(err));
+            }
+        }
+        if (done) {
+#pragma unroll
+            for (int k = 0; k < NS; ++k) st[(RROW_Y + k) * n + r] = rhoi[k];
+            st[RROW_E * n + r] = E;
+            if (steps_out) steps_out[r] = failed ? -1 : acc;
+            if (rej_out) rej_out[r] = rej;
+            r = (long)atomicAdd(counter, 1ull);
+            if (r < n) load(r);
+        }
+    }
+}
+
+__global__ void k_reactor_T(long n, const double* __restrict__ st, double* __restrict__ T)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double rhoi[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) rhoi[k] = st[(RROW_Y + k) * n + r];
+    T[r] = chem::T_given_ey(st[RROW_E * n + r], rhoi);
+}
+
+static int launch_check(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return OCFD_EHIP;
+    }
+    return OCFD_OK;
+}
+
+} // namespace ocfd
+
+using namespace ocfd;
+
+extern "C" {
+
+int ocfd_chem_advance(ocfd_level lev, double* S, double* Spec, double dt, const ocfd_parm* parm, void* stream)
+{
+    if (!lev || !S || !Spec || !parm) { set_error("ocfd_chem_advance: invalid arguments"); return OCFD_EINVAL; }
+    const LevelDev& L = lev->dev;
+    if (L.ntile_v)
+        hipLaunchKernelGGL(k_chem_level, dim3(L.ntile_v), dim3(64), 0, (hipStream_t)stream, L.d_boxes, L.d_tile_v, S,
+                           Spec, dt, parm->smallr);
+    return launch_check("ocfd_chem_advance");
+}
+
+int ocfd_species_renorm(ocfd_level lev, const double* S, double* Spec, void* stream)
+{
+    if (!lev || !S || !Spec) { set_error("ocfd_species_renorm: invalid arguments"); return OCFD_EINVAL; }
+    const LevelDev& L = lev->dev;
+    if (L.ntile_c)
+        hipLaunchKernelGGL(k_renorm_level, dim3(L.ntile_c), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_tile_c,
+                           S, Spec);
+    return launch_check("ocfd_species_renorm");
+}
+
+int ocfd_reactor_init(long n, const double* T0, const double* p0, const double* phi, double* state, void* stream)
+{
+    if (n <= 0 || !T0 || !p0 || !phi || !state) { set_error("ocfd_reactor_init: invalid arguments"); return OCFD_EINVAL; }
+    hipLaunchKernelGGL(k_reactor_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n, T0, p0,
+                       phi, state);
+    return launch_check("ocfd_reactor_init");
+}
+
+int ocfd_reactor_advance_fixed(long n, double* state, double dt, int nsteps, void* stream)
+{
+    if (n <= 0 || !state || nsteps < 0) { set_error("ocfd_reactor_advance_fixed: invalid arguments"); return OCFD_EINVAL; }
+    hipLaunchKernelGGL(k_reactor_fixed, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, (hipStream_t)stream, n, state,
+                       dt, nsteps);
+    return launch_check("ocfd_reactor_advance_fixed");
+}
+
+int ocfd_reactor_advance_adaptive(long n, double* state, double t_end, double h0, double rtol, double atol,
+                                  int max_steps, int* steps_out, int* rejects_out, void* work, void* stream)
+{
+    if (n <= 0 || !state || !work || t_end < 0 || h0 <= 0 || rtol <= 0 || atol < 0 || max_steps <= 0) {
+        set_error("ocfd_reactor_advance_adaptive: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long* ctr = (unsigned long long*)work;
+    hipLaunchKernelGGL(k_counter_init, dim3(1), dim3(1), 0, s, ctr);
+    int dev = 0, ncu = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    // persistent grid: a few 64-lane workgroups per CU, never more lanes than reactors
+    long wgs = std::min<long>((n + 63) / 64, (long)std::max(ncu, 1) * 8);
+    hipLaunchKernelGGL(k_reactor_adaptive, dim3((unsigned)wgs), dim3(64), 0, s, n, state, t_end, h0, rtol, atol,
+                       max_steps, steps_out, rejects_out, ctr);
+    return launch_check("ocfd_reactor_advance_adaptive");
+}
+
+int ocfd_reactor_temperature(long n, const double* state, double* T, void* stream)
+{
+    if (n <= 0 || !state || !T) { set_error("ocfd_reactor_temperature: invalid arguments"); return OCFD_EINVAL; }
+    hipLaunchKernelGGL(k_reactor_T, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n, state, T);
+    return launch_check("ocfd_reactor_temperature");
+}
+
+} // extern "C"

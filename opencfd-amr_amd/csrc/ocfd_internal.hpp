@@ -22,6 +22,7 @@ struct LevelDev {
     int4* d_tile_z = nullptr; int ntile_z = 0;
     int4* d_tile_g = nullptr; int ntile_g = 0;   // ghost fill: (box, first ghost cell, count, 0)
     int4* d_tile_c = nullptr; int ntile_c = 0;   // valid-cell chunks: (box, first cell, count, 0)
+    int4* d_tile_v = nullptr; int ntile_v = 0;   // valid-cell chunks of 64 (chemistry)
     int4* d_tile_p = nullptr; int ntile_p = 0;   // primitive region grow(bx,3): (box, first, count, 3)
     int4* d_tile_pp = nullptr; int ntile_pp = 0; // planar: z grown by 1 only
     float* d_prim32 = nullptr;      // primitives, FP32, 6 comps, grown layout

@@ -223,7 +223,7 @@ static void free_level(LevelDev& L)
     hipFree(L.d_boxes); hipFree(L.d_geom); hipFree(L.d_map);
     hipFree(L.d_tile_x); hipFree(L.d_tile_y); hipFree(L.d_tile_z); hipFree(L.d_tile_g); hipFree(L.d_tile_c);
     hipFree(L.d_acc); hipFree(L.d_red); hipFree(L.d_tile_p); hipFree(L.d_tile_pp);
-    hipFree(L.d_prim32); hipFree(L.d_prim64);
+    hipFree(L.d_prim32); hipFree(L.d_prim64); hipFree(L.d_tile_v);
     L = LevelDev{};
 }
 
@@ -341,7 +341,7 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
 
     // work-lists
     const int xseg = tile_xseg(), ch = tile_chunk();
-    std::vector<int4> tx, ty, tz, tg, tc, tp, tpp;
+    std::vector<int4> tx, ty, tz, tg, tc, tp, tpp, tv;
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
         const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
@@ -361,6 +361,7 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
         const long ngh = B.nptg - B.nptv;
         for (long s = 0; s < ngh; s += 256) tg.push_back(make_int4(b, (int)s, (int)std::min<long>(256, ngh - s), 0));
         for (long s = 0; s < B.nptv; s += 2048) tc.push_back(make_int4(b, (int)s, (int)std::min<long>(2048, B.nptv - s), 0));
+        for (long s = 0; s < B.nptv; s += 64) tv.push_back(make_int4(b, (int)s, (int)std::min<long>(64, B.nptv - s), 0));
         for (int gz : {3, 1}) {
             const long np = (long)(nx + 6) * (ny + 6) * (nz + 2 * gz);
             std::vector<int4>& tl = gz == 3 ? tp : tpp;
@@ -370,12 +371,12 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
     int rc = OCFD_OK;
     if ((rc = upload(&L.d_map, L.owner_map)) || (rc = upload(&L.d_tile_x, tx)) || (rc = upload(&L.d_tile_y, ty)) ||
         (rc = upload(&L.d_tile_z, tz)) || (rc = upload(&L.d_tile_g, tg)) || (rc = upload(&L.d_tile_c, tc)) ||
-        (rc = upload(&L.d_tile_p, tp)) || (rc = upload(&L.d_tile_pp, tpp))) {
+        (rc = upload(&L.d_tile_p, tp)) || (rc = upload(&L.d_tile_pp, tpp)) || (rc = upload(&L.d_tile_v, tv))) {
         free_level(L); delete lev; return rc;
     }
     L.ntile_x = (int)tx.size(); L.ntile_y = (int)ty.size(); L.ntile_z = (int)tz.size();
     L.ntile_g = (int)tg.size(); L.ntile_c = (int)tc.size();
-    L.ntile_p = (int)tp.size(); L.ntile_pp = (int)tpp.size();
+    L.ntile_p = (int)tp.size(); L.ntile_pp = (int)tpp.size(); L.ntile_v = (int)tv.size();
     L.mf_prim = L.mf_state / NC * 6;
     G.map = L.d_map;
     std::vector<GeomDesc> gv(1, G);

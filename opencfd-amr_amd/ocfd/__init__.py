@@ -73,6 +73,13 @@ def lib():
         L.ocfd_dmr_states.argtypes = [dp, dp]
         L.ocfd_profile_stage.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                          ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, dp, vp]
+        L.ocfd_chem_advance.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(Parm), vp]
+        L.ocfd_species_renorm.argtypes = [vp, vp, vp, vp]
+        L.ocfd_reactor_init.argtypes = [ctypes.c_long, vp, vp, vp, vp, vp]
+        L.ocfd_reactor_advance_fixed.argtypes = [ctypes.c_long, vp, ctypes.c_double, ctypes.c_int, vp]
+        L.ocfd_reactor_advance_adaptive.argtypes = [ctypes.c_long, vp, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp, vp, vp]
+        L.ocfd_reactor_temperature.argtypes = [ctypes.c_long, vp, vp, vp]
         L.ocfd_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -209,6 +216,13 @@ class Level:
                                        ctypes.byref(parm), flags, reps, out, _stream(stream)), "ocfd_profile_stage")
         return list(out)
 
+    def chem_advance(self, S, Spec, dt, parm, stream=None):
+        check(lib().ocfd_chem_advance(self.handle, _ptr(S), _ptr(Spec), dt, ctypes.byref(parm), _stream(stream)),
+              "ocfd_chem_advance")
+
+    def species_renorm(self, S, Spec, stream=None):
+        check(lib().ocfd_species_renorm(self.handle, _ptr(S), _ptr(Spec), _stream(stream)), "ocfd_species_renorm")
+
     def init_state(self, kind, S, parm, seed=0):
         check(lib().ocfd_init_state(self.handle, kind, _ptr(S), ctypes.byref(parm), seed), "ocfd_init_state")
 
@@ -234,3 +248,55 @@ def run(level, A, B, C, parm, cfl, scheme=2, flags=0, stop_time=-1.0, nsteps=-1,
             dt_new = stop_time - time
         dt = dt_new
     return steps, time, dt
+
+
+# ---------------------------------------------------------------------------
+# batched constant-volume reactors (0D ignition)
+# ---------------------------------------------------------------------------
+NSPEC = 9
+REACTOR_NCOMP = 11   # rows: rho, E, 9 partial densities
+
+
+class Reactors:
+    """A device-resident batch of n constant-volume H2/air reactors."""
+
+    def __init__(self, T0, p0, phi, stream=None):
+        import torch
+        self.T0 = torch.as_tensor(T0, dtype=torch.float64, device="cuda").contiguous()
+        p0 = torch.as_tensor(p0, dtype=torch.float64, device="cuda").contiguous()
+        phi = torch.as_tensor(phi, dtype=torch.float64, device="cuda").contiguous()
+        self.n = int(self.T0.numel())
+        self.state = torch.zeros(REACTOR_NCOMP * self.n, dtype=torch.float64, device="cuda")
+        self.steps = torch.zeros(self.n, dtype=torch.int32, device="cuda")
+        self.rejects = torch.zeros(self.n, dtype=torch.int32, device="cuda")
+        self.work = torch.zeros(1, dtype=torch.int64, device="cuda")
+        check(lib().ocfd_reactor_init(self.n, _ptr(self.T0), _ptr(p0), _ptr(phi), _ptr(self.state),
+                                      _stream(stream)), "ocfd_reactor_init")
+
+    def rows(self):
+        return self.state.view(REACTOR_NCOMP, self.n)
+
+    def advance_fixed(self, dt, nsteps, stream=None):
+        check(lib().ocfd_reactor_advance_fixed(self.n, _ptr(self.state), dt, nsteps, _stream(stream)),
+              "ocfd_reactor_advance_fixed")
+
+    def advance_adaptive(self, t_end, h0=1e-8, rtol=1e-6, atol=1e-12, max_steps=200000, stream=None):
+        check(lib().ocfd_reactor_advance_adaptive(self.n, _ptr(self.state), t_end, h0, rtol, atol, max_steps,
+                                                  _ptr(self.steps), _ptr(self.rejects), _ptr(self.work),
+                                                  _stream(stream)), "ocfd_reactor_advance_adaptive")
+
+    def temperature(self, stream=None):
+        import torch
+        T = torch.empty(self.n, dtype=torch.float64, device="cuda")
+        check(lib().ocfd_reactor_temperature(self.n, _ptr(self.state), _ptr(T), _stream(stream)),
+              "ocfd_reactor_temperature")
+        return T
+
+
+def ignition_batch(n, seed=7):
+    """Seeded 0D ignition inputs: T0 ~ U[900,1500] K, p0 ~ U[1,10] atm, phi ~ U[0.5,2]."""
+    rng = np.random.default_rng(seed)
+    T0 = rng.uniform(900.0, 1500.0, n)
+    p0 = rng.uniform(1.0, 10.0, n) * 101325.0
+    phi = rng.uniform(0.5, 2.0, n)
+    return T0, p0, phi
