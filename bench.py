@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=128, help="cells per side of the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--no-chem", action="store_true", help="skip the 0D ignition batch")
+    ap.add_argument("--reactors", type=int, default=65536)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (HBM counter calibration)")
     return ap.parse_args()
@@ -85,6 +87,41 @@ def kernel_roofline(lev, A, B, C, p, flags, ncell):
     dom = max(range(3), key=lambda i: ms[i])
     achieved = bytes_[dom] / (ms[dom] * 1e-3) / 1e9
     return names[dom], achieved, kern
+
+
+def chemistry_0d(n_react, cpu_sample=64):
+    """0D constant-volume H2/air ignition: n_react seeded reactors (T0 900-1500 K,
+    p0 1-10 atm, phi 0.5-2) integrated to 1 ms with per-reactor adaptive
+    substeps in one persistent launch; the oracle's identical integrator is
+    timed on a sample of the same reactors."""
+    import torch
+    import ocfd
+    T0, p0, phi = ocfd.ignition_batch(n_react, seed=7)
+    R = ocfd.Reactors(T0, p0, phi)
+    R.advance_adaptive(1e-3, h0=1e-8, rtol=1e-6, atol=1e-12)   # warm-up (also first-touch of the code)
+    R = ocfd.Reactors(T0, p0, phi)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    R.advance_adaptive(1e-3, h0=1e-8, rtol=1e-6, atol=1e-12)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    steps = R.steps.cpu().numpy()
+    rej = R.rejects.cpu().numpy()
+    T = R.temperature().cpu().numpy()
+    out = {"reactors": n_react, "t_end_s": 1e-3, "ms": el * 1e3, "reactors_per_s": n_react / el,
+           "substeps_per_s": float((steps.clip(0) + rej).sum() * 3 / el),
+           "accepted_steps_mean": float(steps.mean()), "failed": int((steps < 0).sum()),
+           "ignited_frac": float((T > 1.5 * T0).mean())}
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    u, r = orc.reactor_init(T0[:cpu_sample], p0[:cpu_sample], phi[:cpu_sample])
+    t0 = time.perf_counter()
+    orc.reactor_adaptive(u, r, 1e-3, h0=1e-8, rtol=1e-6, atol=1e-12)
+    el_c = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out["cpu_baseline"] = {"value": cpu_sample / el_c, "unit": "reactors/s", "cores": cores, "kind": "port",
+                           "sample": f"oracle adaptive integrator, first {cpu_sample} reactors ({el_c:.1f} s)"}
+    return out
 
 
 def cpu_baseline(n, steps):
@@ -142,6 +179,10 @@ def main():
         res64 = {"value": ncell * args.steps / t64 / 1e6, "ms_per_step": 1e3 * t64 / args.steps,
                  "kernels": kern64}
 
+    chem = None
+    if not args.no_chem:
+        chem = chemistry_0d(args.reactors)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_n, args.cpu_steps)
@@ -168,6 +209,7 @@ def main():
         "fp64_form": res64,
         "fp32_over_fp64": (value / res64["value"]) if res64 else None,
         "cpu_baseline": cpu,
+        "chemistry_0d": chem,
     }
     print(json.dumps(out))
 
