@@ -42,38 +42,36 @@ def test_reactor_init_matches_oracle(torch_cuda, orc):
     assert np.allclose(T, T0, rtol=1e-9)
 
 
-@pytest.mark.parametrize("dt,nsteps", [(1e-5, 100), (1e-7, 200)])
+@pytest.mark.parametrize("dt,nsteps", [(1e-5, 100), (1e-6, 100), (1e-7, 200)])
 def test_reactor_fixed_steps_match_oracle(torch_cuda, orc, dt, nsteps):
-    """4096 seeded reactors, nsteps reference steps of dt.  At dt = 1e-5 s the
-    reference's fixed step diverges for some high-pressure reactors (negative
-    temperature, then NaN); both sides must diverge on the same reactors and
-    agree on the others."""
+    """4096 seeded reactors, nsteps reference steps of dt.  The reference's
+    fixed step is ill-conditioned for part of the batch at large dt (a 1e-14
+    perturbation of the initial energy changes the final state, or it
+    diverges): those reactors are identified with the oracle itself and
+    excluded; the rest (>= 90 %) must agree with the oracle to 1e-8."""
     import ocfd
     T0, p0, phi = ocfd.ignition_batch(4096, seed=7)
     R = ocfd.Reactors(T0, p0, phi)
     R.advance_fixed(dt, nsteps)
     rho, E, Y = _gpu_rows(R)
-    u, r = _oracle_state(orc, T0, p0, phi)
-    u, r = orc.reactor_fixed(u, r, dt, nsteps)
-    Tg_all = R.temperature().cpu().numpy()
-    To_all = orc.reactor_T(u, r)
-
-    def physical(Yv, Ev, Tv):
-        return np.isfinite(Yv).all(1) & np.isfinite(Ev) & np.isfinite(Tv) & (Tv > 200.0) & (Tv < 5000.0)
-
-    ok_g = physical(Y, E, Tg_all)
-    ok_o = physical(r, u[:, 4], To_all)
-    # a diverging trajectory of the reference scheme is decided by rounding
-    assert np.sum(ok_g != ok_o) <= 0.01 * len(ok_o)
-    ok = ok_g & ok_o
-    Y, r, E, uE = Y[ok], r[ok], E[ok], u[ok, 4]
-    err = np.abs(Y - r).sum(0) / np.abs(r).sum(0).clip(1e-300)
-    errE = np.abs(E - uE).sum() / np.abs(uE).sum()
-    Tg, To = Tg_all[ok], To_all[ok]
-    print(f"dt {dt}: {np.sum(~ok_o)} diverged (oracle), {np.sum(ok_g != ok_o)} disagree; rel L1 per species", err,
-          "E", errE, "T max rel", np.max(np.abs(Tg - To) / To))
-    assert err.max() < 1e-8 and errE < 1e-10
-    assert np.max(np.abs(Tg - To) / To) < 1e-8
+    u0, r0 = _oracle_state(orc, T0, p0, phi)
+    u, r = orc.reactor_fixed(u0, r0, dt, nsteps)
+    up = u0.copy()
+    up[:, 4] *= 1.0 + 1e-14
+    ub, rb = orc.reactor_fixed(up, r0, dt, nsteps)
+    To = orc.reactor_T(u, r)
+    Tb = orc.reactor_T(ub, rb)
+    Tg = R.temperature().cpu().numpy()
+    with np.errstate(invalid="ignore"):
+        well = np.isfinite(To) & np.isfinite(Tb) & (np.abs(To - Tb) < 1e-8 * np.abs(To))
+    assert well.mean() >= 0.9
+    Yw, rw, Ew, uEw = Y[well], r[well], E[well], u[well, 4]
+    err = np.abs(Yw - rw).sum(0) / np.abs(rw).sum(0).clip(1e-300)
+    errE = np.abs(Ew - uEw).sum() / np.abs(uEw).sum()
+    relT = np.abs(Tg[well] - To[well]) / To[well]
+    print(f"dt {dt}: well-conditioned {well.mean():.4f}; rel L1 per species", err, "E", errE, "T max rel",
+          relT.max())
+    assert err.max() < 1e-8 and errE < 1e-10 and relT.max() < 1e-8
 
 
 def test_reactor_adaptive_matches_oracle(torch_cuda, orc):
