@@ -98,17 +98,18 @@ def chemistry_0d(n_react, cpu_sample=64):
     import ocfd
     T0, p0, phi = ocfd.ignition_batch(n_react, seed=7)
     R = ocfd.Reactors(T0, p0, phi)
-    R.advance_adaptive(1e-3, h0=1e-8, rtol=1e-6, atol=1e-12)   # warm-up (also first-touch of the code)
+    R.advance_adaptive(1e-3, h0=1e-8, rtol=1e-4, atol=1e-9)   # warm-up (also first-touch of the code)
     R = ocfd.Reactors(T0, p0, phi)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    R.advance_adaptive(1e-3, h0=1e-8, rtol=1e-6, atol=1e-12)
+    R.advance_adaptive(1e-3, h0=1e-8, rtol=1e-4, atol=1e-9)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     steps = R.steps.cpu().numpy()
     rej = R.rejects.cpu().numpy()
     T = R.temperature().cpu().numpy()
-    out = {"reactors": n_react, "t_end_s": 1e-3, "ms": el * 1e3, "reactors_per_s": n_react / el,
+    out = {"reactors": n_react, "t_end_s": 1e-3, "rtol": 1e-4, "atol": 1e-9, "ms": el * 1e3,
+           "reactors_per_s": n_react / el,
            "substeps_per_s": float((steps.clip(0) + rej).sum() * 3 / el),
            "accepted_steps_mean": float(steps.mean()), "failed": int((steps < 0).sum()),
            "ignited_frac": float((T > 1.5 * T0).mean())}
@@ -116,7 +117,7 @@ def chemistry_0d(n_react, cpu_sample=64):
     import oracle as orc
     u, r = orc.reactor_init(T0[:cpu_sample], p0[:cpu_sample], phi[:cpu_sample])
     t0 = time.perf_counter()
-    orc.reactor_adaptive(u, r, 1e-3, h0=1e-8, rtol=1e-6, atol=1e-12)
+    orc.reactor_adaptive(u, r, 1e-3, h0=1e-8, rtol=1e-4, atol=1e-9)
     el_c = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     out["cpu_baseline"] = {"value": cpu_sample / el_c, "unit": "reactors/s", "cores": cores, "kind": "port",

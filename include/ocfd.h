@@ -161,7 +161,8 @@ int ocfd_reactor_advance_fixed(long n, double* state, double dt, int nsteps, voi
 
 /* Integration to t_end with per-reactor adaptive substeps: step doubling on
  * the same linearised-implicit step, error max_k |y2-y1|/(atol rho + rtol|y2|)
- * over the partial densities, h <- h*clamp(0.9/sqrt(err), 0.2, 4).  One
+ * over the partial densities, accepted value 2 y2 - y1 (Richardson, second
+ * order), h <- h*clamp(0.9/sqrt(err), 0.2, 4).  One
  * persistent launch; lanes take reactors from a device work counter.
  * steps_out / rejects_out (device int[n], may be NULL): accepted steps (-1 if
  * max_steps was hit) and rejected steps.  work: 8 bytes of device memory. */

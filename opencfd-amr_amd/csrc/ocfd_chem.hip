@@ -125,8 +125,9 @@ __global__ __launch_bounds__(64) void k_reactor_fixed(long n, double* __restrict
 __global__ void k_counter_init(unsigned long long* c) { *c = 0ull; }
 
 // Adaptive integration with step doubling on the linearised-implicit step:
-// one step of h against two of h/2, error max_k |y2-y1|/(atol rho + rtol |y2|),
-// accept y2 and rescale h by clamp(0.9/sqrt(err), 0.2, 4).
+// one step of h (y1) against two of h/2 (y2), error max_k |y2-y1| /
+// (atol rho + rtol |y2|); on acceptance keep the Richardson value 2 y2 - y1
+// (second order, densities clipped at 0); rescale h by clamp(0.9/sqrt(err), 0.2, 4).
 __global__ __launch_bounds__(64) void k_reactor_adaptive(long n, double* __restrict__ st, double t_end, double h0,
                                                          double rtol, double atol, int max_steps,
                                                          int* __restrict__ steps_out, int* __restrict__ rej_out,
@@ -158,11 +159,10 @@ __global__ __launch_bounds__(64) void k_reactor_adaptive(long n, double* __restr
             double err = 0.0;
 #pragma unroll
             for (int k = 0; k < NS; ++k) err = fmax(err, fabs(r2[k] - r1[k]) / (atol * rho + rtol * fabs(r2[k])));
-            (void)E1;
             if (err <= 1.0) {
 #pragma unroll
-                for (int k = 0; k < NS; ++k) rhoi[k] = r2[k];
-                E = E2;
+                for (int k = 0; k < NS; ++k) rhoi[k] = fmax(2.0 * r2[k] - r1[k], 0.0);
+                E = 2.0 * E2 - E1;
                 t += h;
                 ++acc;
                 const double f = err > 0.0 ? 0.9 / sqrt(err) : 4.0;

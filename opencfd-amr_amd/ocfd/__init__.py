@@ -280,7 +280,7 @@ class Reactors:
         check(lib().ocfd_reactor_advance_fixed(self.n, _ptr(self.state), dt, nsteps, _stream(stream)),
               "ocfd_reactor_advance_fixed")
 
-    def advance_adaptive(self, t_end, h0=1e-8, rtol=1e-6, atol=1e-12, max_steps=200000, stream=None):
+    def advance_adaptive(self, t_end, h0=1e-8, rtol=1e-4, atol=1e-9, max_steps=200000, stream=None):
         check(lib().ocfd_reactor_advance_adaptive(self.n, _ptr(self.state), t_end, h0, rtol, atol, max_steps,
                                                   _ptr(self.steps), _ptr(self.rejects), _ptr(self.work),
                                                   _stream(stream)), "ocfd_reactor_advance_adaptive")

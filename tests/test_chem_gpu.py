@@ -78,11 +78,11 @@ def test_reactor_adaptive_matches_oracle(torch_cuda, orc):
     import ocfd
     T0, p0, phi = ocfd.ignition_batch(512, seed=9)
     R = ocfd.Reactors(T0, p0, phi)
-    R.advance_adaptive(1e-3, h0=1e-8, rtol=1e-6, atol=1e-12)
+    R.advance_adaptive(1e-3, h0=1e-8, rtol=1e-4, atol=1e-9)
     Tg = R.temperature().cpu().numpy()
     steps = R.steps.cpu().numpy()
     u, r = _oracle_state(orc, T0, p0, phi)
-    u, r, so, ro = orc.reactor_adaptive(u, r, 1e-3, h0=1e-8, rtol=1e-6, atol=1e-12)
+    u, r, so, ro = orc.reactor_adaptive(u, r, 1e-3, h0=1e-8, rtol=1e-4, atol=1e-9)
     To = orc.reactor_T(u, r)
     assert np.all(steps > 0) and np.all(so > 0)
     rel = np.abs(Tg - To) / To

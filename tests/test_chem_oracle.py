@@ -164,17 +164,17 @@ def test_adaptive_integration_matches_library_stiff_solver(orc):
         return out
 
     y0 = np.concatenate([r0[0], [u0[0, 4]]])
-    sol = solve_ivp(rhs, (0, 1e-3), y0, method="Radau", jac=jac, rtol=1e-9, atol=1e-13)
+    sol = solve_ivp(rhs, (0, 1e-3), y0, method="Radau", jac=jac, rtol=1e-10, atol=1e-14)
     assert sol.success
-    u, r, steps, rej = orc.reactor_adaptive(u0, r0, 1e-3, h0=1e-8, rtol=1e-6, atol=1e-12)
+    u, r, steps, rej = orc.reactor_adaptive(u0, r0, 1e-3, h0=1e-8, rtol=1e-4, atol=1e-9)
     assert steps[0] > 0
     T_lib, _ = orc.T_given_ey(sol.y[9, -1], np.maximum(sol.y[:9, -1], 0))
     T_ad = orc.reactor_T(u, r)[0]
     assert T_lib > 2000.0                              # ignited
-    assert abs(T_ad - T_lib) < 2e-3 * T_lib, (T_ad, T_lib)
+    assert abs(T_ad - T_lib) < 5e-5 * T_lib, (T_ad, T_lib)
     yw = r[0] / r[0].sum()
     yl = sol.y[:9, -1] / sol.y[:9, -1].sum()
-    assert abs(yw[2] - yl[2]) < 5e-3 * yl[2]           # H2O
+    assert abs(yw[2] - yl[2]) < 2e-4 * yl[2]           # H2O
 
 
 def test_renorm(orc):
