@@ -56,6 +56,24 @@ typedef struct ocfd_geom {
 
 typedef struct ocfd_level_s* ocfd_level;
 
+/* Device-resident time control of one level: the state of the coarse time
+ * loop (Source/main.cpp:61-70 with EBR::computeInitialDt / computeNewDt,
+ * Source/EBR.cpp:189-306), kept in device memory so a whole time step --
+ * ghost fills, RK stages, CFL reduction and the dt update -- runs without a
+ * host round trip and can be captured into a hipGraph.  Allocate
+ * sizeof(ocfd_step_ctl) bytes of device memory per level. */
+typedef struct ocfd_step_ctl {
+    double time;            /* current time */
+    double dt;              /* dt of the next step */
+    double cfl;
+    double stop_time;       /* < 0: none */
+    unsigned long long red; /* CFL reduction scratch (bit pattern of a positive double) */
+    int steps;              /* steps taken */
+    int max_steps;          /* < 0: none */
+    int done;               /* set when stop_time or max_steps is reached */
+    int pad;
+} ocfd_step_ctl;
+
 /* Parm defaults + Parm::Initialize (Source/Constants.H:9-29, Source/Constants.cpp:7-11).
  * If p->eos_gamma == 0 all defaults are filled first; otherwise only the
  * derived members Rg, cv, cp, kOverMu are recomputed (EBR::read_params,
@@ -73,10 +91,15 @@ int ocfd_level_destroy(ocfd_level lev);
  * comps, and the offset (in doubles) of box b's FAB inside it. */
 long ocfd_level_mf_size(ocfd_level lev, int ngrow, int ncomp);
 long ocfd_level_fab_offset(ocfd_level lev, int box, int ngrow, int ncomp);
+
+/* Device memory held by the level itself (work-lists, primitive and partial
+ * divergence scratch), in bytes; the caller's MultiFabs are not included.
+ * For admission control by HBM footprint. */
+long ocfd_level_device_bytes(ocfd_level lev);
 int  ocfd_level_nbox(ocfd_level lev);
 
 /* EBR::compute_dSdt regular-box path (Source/Advance.cpp:182-418 with the
- * kernels of Source/kernel/*.H): dSdt (0 ghosts, 5 comps) = -div F(S) for every
+ * kernels of the Source/kernel headers): dSdt (0 ghosts, 5 comps) = -div F(S) for every
  * box of the level, S holding 4 filled ghost cells.  dt only feeds flux
  * registers in the reference and is unused here.  flags: OCFD_VISC |
  * OCFD_GRAVITY | OCFD_FLUX_FP32 | OCFD_PLANAR.  stream: hipStream_t (NULL =
@@ -171,6 +194,30 @@ int ocfd_reactor_advance_adaptive(long n, double* state, double t_end, double h0
 
 /* Temperature of each reactor (GET_T_GIVEN_EY, Source/CHEM/LiDryer.H:836-888). */
 int ocfd_reactor_temperature(long n, const double* state, double* T, void* stream);
+
+/* Initialise device time control: time, cfl, stop_time, max_steps, and the
+ * initial dt = cfl * estdt(A) clipped to stop_time (EBR::computeInitialDt,
+ * Source/EBR.cpp:189-229,561-563).  No host synchronisation. */
+int ocfd_ctl_start(ocfd_level lev, const double* A, ocfd_step_ctl* ctl, double time, double cfl, double stop_time,
+                   int max_steps, const ocfd_parm* parm, void* stream);
+
+/* One time step driven by ctl (the stage sequence of ocfd_advance for the
+ * scheme, with dt and time read from ctl), followed by the CFL reduction of
+ * the new state and the dt update of EBR::computeNewDt (change_max 1.1,
+ * stop_time clip, Source/EBR.cpp:231-306).  Once ctl->done is set every kernel
+ * of the step returns at once, so extra steps are no-ops.  No host
+ * synchronisation, no allocation. */
+int ocfd_step_device(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
+                     const ocfd_parm* parm, int flags, void* stream);
+
+/* hipGraph of nsteps device-driven time steps of one level, captured on
+ * `stream` (which must not be the NULL stream) and replayed with
+ * ocfd_graph_launch on any stream. */
+typedef struct ocfd_graph_s* ocfd_graph;
+int ocfd_graph_capture_steps(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
+                             const ocfd_parm* parm, int flags, int nsteps, void* stream, ocfd_graph* out);
+int ocfd_graph_launch(ocfd_graph g, void* stream);
+int ocfd_graph_destroy(ocfd_graph g);
 
 /* Last error message of the calling thread ("" if none). */
 const char* ocfd_last_error(void);

@@ -60,10 +60,28 @@ struct StageArgs {
     double* snew;         // stage output (valid cells), or null for a plain dSdt
     void* part;           // partial divergence dx-term (+dy-term), FT, valid layout, level-owned
     double* acc;          // dSdt output of a plain compute_dSdt (valid layout)
-    double c_cur, c_old, c_dt;
+    double c_cur, c_old, c_dt;  // with ctl: c_dt multiplies ctl->dt
     int flags;
     PhysArgs ph;
+    const ocfd_step_ctl* ctl;    // device time control, or null (host-driven)
 };
+
+// The control block is written by one kernel and read by every workgroup of
+// the next ones on all XCDs; its fields are accessed with device-scope atomic
+// loads and stores so no XCD can read a line its own L2 holds stale (the CFL
+// minimum arrives there by device-scope atomics).
+template <class T>
+__device__ __forceinline__ T ctl_ld(const T* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void ctl_st(T* p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool ctl_done(const ocfd_step_ctl* c) { return c && ctl_ld(&c->done); }
 
 // Workgroups are dealt round-robin over the 8 XCDs (workgroup b runs on XCD
 // b % 8), each with its own L2.  Give every XCD a contiguous range of the
@@ -320,11 +338,12 @@ __device__ __forceinline__ void finish_cell(const StageArgs& a, const BoxDesc& B
         L[UEDEN] += g * a.scur[s0 + 3 * st];
     }
     if (a.snew) {
+        const double cdt = a.ctl ? a.c_dt * ctl_ld(&a.ctl->dt) : a.c_dt;
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
             double v = a.sold ? a.c_cur * a.scur[s0 + n * st] + a.c_old * a.sold[s0 + n * st]
                               : a.c_cur * a.scur[s0 + n * st];
-            a.snew[s0 + n * st] = v + a.c_dt * L[n];
+            a.snew[s0 + n * st] = v + cdt * L[n];
         }
     } else {
         const long d0 = didx(B, i, j, k);

@@ -31,6 +31,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
 {
     __shared__ FT sf[4][10][64];
     __shared__ FT sv[VISC ? 4 : 1][VISC ? 12 : 1][64];
+    if (ctl_done(a.ctl)) return;   // uniform over the block: before any barrier
     const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const BoxDesc B = boxes[t.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -108,6 +109,7 @@ template <int D, class FT, bool VISC, bool FINAL>
 __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                      StageArgs a)
 {
+    if (ctl_done(a.ctl)) return;
     const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const BoxDesc B = boxes[t.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -237,6 +239,7 @@ template <class FT>
 __global__ __launch_bounds__(256) void k_prims(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                StageArgs a)
 {
+    if (ctl_done(a.ctl)) return;
     const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     if ((int)threadIdx.x >= t.z) return;
     const BoxDesc B = boxes[t.x];

@@ -32,6 +32,7 @@ struct LevelDev {
     unsigned long long* d_red = nullptr;  // reduction result slot
     long mf_valid = 0;              // doubles in a 0-ghost 5-comp MultiFab
     long mf_state = 0;              // doubles in a 4-ghost 5-comp MultiFab
+    long dev_bytes = 0;             // device memory held by the level
 };
 
 void launch_sweeps(const LevelDev& L, const StageArgs& a, hipStream_t s);

@@ -80,9 +80,16 @@ __device__ __forceinline__ void ghost_cell(const BoxDesc& B, int g, int& i, int&
 // foextrap copies the boundary cell, reflect_even/odd mirror about the face
 // with sign +-1, composed when a cell is outside in several directions) and
 // the case's ext_dir fill (Exec/1_DMR/prob_bcfill.cpp:14-58).
+// With device time control the fill time is ctl->time + toff * ctl->dt and a
+// finished run fills nothing.
 __global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes, const GeomDesc* __restrict__ geoms,
-                                              const int4* __restrict__ tiles, double* __restrict__ S, double time)
+                                              const int4* __restrict__ tiles, double* __restrict__ S, double time,
+                                              const ocfd_step_ctl* __restrict__ ctl, double toff)
 {
+    if (ctl) {
+        if (ctl_ld(&ctl->done)) return;
+        time = ctl_ld(&ctl->time) + toff * ctl_ld(&ctl->dt);
+    }
     const int4 t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.z) return;
     const BoxDesc B = boxes[t.x];
@@ -156,9 +163,10 @@ __global__ void k_red_init(unsigned long long* r)
 
 __global__ __launch_bounds__(256) void k_estdt(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                const double* __restrict__ S, PhysArgs ph,
-                                               unsigned long long* __restrict__ out)
+                                               unsigned long long* __restrict__ out, const ocfd_step_ctl* ctl)
 {
     __shared__ double red[4];
+    if (ctl_done(ctl)) return;
     const int4 t = tiles[blockIdx.x];
     const BoxDesc B = boxes[t.x];
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
@@ -194,6 +202,66 @@ __global__ __launch_bounds__(256) void k_estdt(const BoxDesc* __restrict__ boxes
         // positive doubles order like their bit patterns
         atomicMin(out, (unsigned long long)__double_as_longlong(m));
     }
+}
+
+// ---------------------------------------------------------------------------
+// device time control: the driver's dt sequence (Source/EBR.cpp:189-306)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double red_value(unsigned long long bits) { return __longlong_as_double((long long)bits); }
+__device__ __forceinline__ unsigned long long red_big()
+{
+    return (unsigned long long)__double_as_longlong(std::numeric_limits<double>::max());
+}
+
+__global__ void k_ctl_init(ocfd_step_ctl* c, double time, double cfl, double stop_time, int max_steps)
+{
+    ctl_st(&c->time, time); ctl_st(&c->dt, 0.0); ctl_st(&c->cfl, cfl); ctl_st(&c->stop_time, stop_time);
+    atomicExch(&c->red, red_big());
+    ctl_st(&c->steps, 0); ctl_st(&c->max_steps, max_steps); ctl_st(&c->done, 0); ctl_st(&c->pad, 0);
+}
+
+// dt_new from the CFL minimum, clipped so the run ends on stop_time; then the
+// done flag (the driver's loop condition, Source/main.cpp:61-70)
+__device__ void ctl_set_dt(ocfd_step_ctl* c, double dt_new, double time, int steps)
+{
+#pragma clang fp contract(off)
+    const double stop = ctl_ld(&c->stop_time);
+    const int max_steps = ctl_ld(&c->max_steps);
+    const double eps = 0.001 * dt_new;
+    if (stop >= 0.0 && time + dt_new > stop - eps) dt_new = stop - time;
+    ctl_st(&c->dt, dt_new);
+    ctl_st(&c->time, time);
+    ctl_st(&c->steps, steps);
+    ctl_st(&c->done, (int)((max_steps >= 0 && steps >= max_steps) || (stop >= 0.0 && !(time < stop))));
+}
+
+// initial dt: cfl * estdt (EBR::computeInitialDt, Source/EBR.cpp:189-229 with
+// the single-level ratio 1)
+__global__ void k_ctl_first(ocfd_step_ctl* c)
+{
+#pragma clang fp contract(off)
+    const double dt = red_value(atomicExch(&c->red, red_big())) * ctl_ld(&c->cfl);
+    ctl_set_dt(c, dt, ctl_ld(&c->time), ctl_ld(&c->steps));
+}
+
+// after a step: t += dt, then dt_new = min(cfl*estdt, change_max*dt) with
+// change_max 1.1 and the stop_time clip (EBR::computeNewDt, Source/EBR.cpp:231-306)
+__global__ void k_ctl_update(ocfd_step_ctl* c)
+{
+#pragma clang fp contract(off)
+    if (ctl_ld(&c->done)) return;
+    const double est = red_value(atomicExch(&c->red, red_big())) * ctl_ld(&c->cfl);
+    const double dt = ctl_ld(&c->dt);
+    const double time = ctl_ld(&c->time) + dt;
+    ctl_set_dt(c, fmin(est, 1.1 * dt), time, ctl_ld(&c->steps) + 1);
+}
+
+// A <- B for the forward-Euler step under device control (a no-op once done)
+__global__ __launch_bounds__(256) void k_copy_ctl(double* __restrict__ dst, const double* __restrict__ src, long n,
+                                                  const ocfd_step_ctl* __restrict__ ctl)
+{
+    if (ctl_done(ctl)) return;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) dst[i] = src[i];
 }
 
 static PhysArgs phys_of(const ocfd_parm& p)
@@ -388,6 +456,11 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
         set_error("ocfd_level_create: out of device memory");
         free_level(L); delete lev; return OCFD_ENOMEM;
     }
+    L.dev_bytes = (long)sizeof(int) * (long)L.owner_map.size() +
+                  (long)sizeof(int4) * (L.ntile_x + L.ntile_y + L.ntile_z + L.ntile_g + L.ntile_c + L.ntile_p +
+                                        L.ntile_pp + L.ntile_v) +
+                  (long)sizeof(GeomDesc) + (long)sizeof(BoxDesc) * L.nbox + (long)sizeof(double) * L.mf_valid +
+                  (long)(sizeof(float) + sizeof(double)) * L.mf_prim + (long)sizeof(unsigned long long);
     *out = lev;
     return OCFD_OK;
 }
@@ -401,6 +474,8 @@ int ocfd_level_destroy(ocfd_level lev)
 }
 
 int ocfd_level_nbox(ocfd_level lev) { return lev ? lev->dev.nbox : 0; }
+
+long ocfd_level_device_bytes(ocfd_level lev) { return lev ? lev->dev.dev_bytes : -1; }
 
 long ocfd_level_mf_size(ocfd_level lev, int ngrow, int ncomp)
 {
@@ -465,7 +540,8 @@ int ocfd_fill_boundary(ocfd_level lev, double* S, double time, void* stream)
     if (!lev || !S) { set_error("ocfd_fill_boundary: invalid arguments"); return OCFD_EINVAL; }
     const LevelDev& L = lev->dev;
     if (L.ntile_g)
-        hipLaunchKernelGGL(k_fill, dim3(L.ntile_g), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_geom, L.d_tile_g, S, time);
+        hipLaunchKernelGGL(k_fill, dim3(L.ntile_g), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_geom, L.d_tile_g, S, time,
+                           (const ocfd_step_ctl*)nullptr, 0.0);
     return check_launch("ocfd_fill_boundary");
 }
 
@@ -475,7 +551,8 @@ int ocfd_estdt(ocfd_level lev, const double* S, double cfl, const ocfd_parm* par
     const LevelDev& L = lev->dev;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_red_init, dim3(1), dim3(1), 0, s, L.d_red);
-    hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, S, phys_of(*parm), L.d_red);
+    hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, S, phys_of(*parm), L.d_red,
+                       (const ocfd_step_ctl*)nullptr);
     int rc = check_launch("ocfd_estdt");
     if (rc) return rc;
     unsigned long long bits = 0;
@@ -540,7 +617,8 @@ int ocfd_profile_stage(ocfd_level lev, const double* S_cur, const double* S_old,
         for (int r = 0; r < reps; ++r) {
             if (which < 3 || which == 4) launch_sweep_one(lev->dev, a, which, s);
             else hipLaunchKernelGGL(k_fill, dim3(lev->dev.ntile_g), dim3(256), 0, s, lev->dev.d_boxes,
-                                    lev->dev.d_geom, lev->dev.d_tile_g, (double*)S_cur, 0.0);
+                                    lev->dev.d_geom, lev->dev.d_tile_g, (double*)S_cur, 0.0,
+                                    (const ocfd_step_ctl*)nullptr, 0.0);
         }
         OCFD_HIP(hipEventRecord(e1, s));
         OCFD_HIP(hipEventSynchronize(e1));
@@ -551,6 +629,135 @@ int ocfd_profile_stage(ocfd_level lev, const double* S_cur, const double* S_old,
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     return check_launch("ocfd_profile_stage");
+}
+
+// ---------------------------------------------------------------------------
+// device-driven time steps and their graphs
+// ---------------------------------------------------------------------------
+int ocfd_ctl_start(ocfd_level lev, const double* A, ocfd_step_ctl* ctl, double time, double cfl, double stop_time,
+                   int max_steps, const ocfd_parm* parm, void* stream)
+{
+    if (!lev || !A || !ctl || !parm || !(cfl > 0.0)) { set_error("ocfd_ctl_start: invalid arguments"); return OCFD_EINVAL; }
+    const LevelDev& L = lev->dev;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, ctl, time, cfl, stop_time, max_steps);
+    hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, A, phys_of(*parm), &ctl->red,
+                       (const ocfd_step_ctl*)nullptr);
+    hipLaunchKernelGGL(k_ctl_first, dim3(1), dim3(1), 0, s, ctl);
+    return check_launch("ocfd_ctl_start");
+}
+
+static int step_device(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
+                       const ocfd_parm* parm, int flags, hipStream_t s)
+{
+    const LevelDev& L = lev->dev;
+    StageArgs a{};
+    a.flags = flags; a.ph = phys_of(*parm); a.ctl = ctl; a.acc = L.d_acc;
+    auto fill = [&](double* S, double toff) {
+        if (L.ntile_g)
+            hipLaunchKernelGGL(k_fill, dim3(L.ntile_g), dim3(256), 0, s, L.d_boxes, L.d_geom, L.d_tile_g, S, 0.0,
+                               (const ocfd_step_ctl*)ctl, toff);
+    };
+    auto stage = [&](const double* cur, const double* old, double* nw, double cc, double co, double cdt) {
+        a.scur = cur; a.sold = old; a.snew = nw; a.c_cur = cc; a.c_old = co; a.c_dt = cdt;
+        launch_sweeps(L, a, s);
+    };
+    // the same stage sequence as ocfd_advance, with dt and time read from ctl
+    fill(A, 0.0);
+    if (scheme == 0) {
+        stage(A, nullptr, B, 1.0, 0.0, 1.0);
+        const long n = L.mf_state;
+        const int nb = (int)std::min<long>((n + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_copy_ctl, dim3(nb), dim3(256), 0, s, A, B, n, (const ocfd_step_ctl*)ctl);
+    } else if (scheme == 1) {
+        stage(A, nullptr, B, 1.0, 0.0, 1.0);
+        fill(B, 1.0);
+        stage(B, A, A, 0.5, 0.5, 0.5);
+    } else {
+        stage(A, nullptr, B, 1.0, 0.0, 1.0);
+        fill(B, 1.0);
+        stage(B, A, C, 0.25, 0.75, 0.25);
+        fill(C, 0.5);
+        stage(C, A, A, 2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0);
+    }
+    hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, A, a.ph, &ctl->red,
+                       (const ocfd_step_ctl*)ctl);
+    hipLaunchKernelGGL(k_ctl_update, dim3(1), dim3(1), 0, s, ctl);
+    return check_launch("ocfd_step_device");
+}
+
+static bool step_args_ok(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
+                         const ocfd_parm* parm, int flags)
+{
+    return lev && A && B && ctl && parm && valid_flags(flags) && scheme >= 0 && scheme <= 2 && (scheme != 2 || C) &&
+           A != B && (scheme != 2 || (C != A && C != B));
+}
+
+int ocfd_step_device(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
+                     const ocfd_parm* parm, int flags, void* stream)
+{
+    if (!step_args_ok(lev, A, B, C, ctl, scheme, parm, flags)) {
+        set_error("ocfd_step_device: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    return step_device(lev, A, B, C, ctl, scheme, parm, flags, (hipStream_t)stream);
+}
+
+} // extern "C"
+
+struct ocfd_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int ocfd_graph_capture_steps(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
+                             const ocfd_parm* parm, int flags, int nsteps, void* stream, ocfd_graph* out)
+{
+    if (!step_args_ok(lev, A, B, C, ctl, scheme, parm, flags) || nsteps <= 0 || !out || !stream) {
+        set_error("ocfd_graph_capture_steps: invalid arguments (a non-null stream is required)");
+        return OCFD_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    OCFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = OCFD_OK;
+    for (int n = 0; n < nsteps && rc == OCFD_OK; ++n) rc = step_device(lev, A, B, C, ctl, scheme, parm, flags, s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != OCFD_OK || e != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        if (rc == OCFD_OK) set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return rc != OCFD_OK ? rc : OCFD_EHIP;
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        return OCFD_EHIP;
+    }
+    ocfd_graph G = new ocfd_graph_s;
+    G->graph = g;
+    G->exec = x;
+    *out = G;
+    return OCFD_OK;
+}
+
+int ocfd_graph_launch(ocfd_graph g, void* stream)
+{
+    if (!g) { set_error("ocfd_graph_launch: null graph"); return OCFD_EINVAL; }
+    OCFD_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return OCFD_OK;
+}
+
+int ocfd_graph_destroy(ocfd_graph g)
+{
+    if (!g) return OCFD_OK;
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return OCFD_OK;
 }
 
 } // extern "C"

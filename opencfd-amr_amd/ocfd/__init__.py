@@ -44,6 +44,14 @@ class Geom(ctypes.Structure):
     ]
 
 
+class StepCtl(ctypes.Structure):
+    """Device time control of one level (struct ocfd_step_ctl)."""
+    _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("stop_time", ctypes.c_double), ("red", ctypes.c_ulonglong), ("steps", ctypes.c_int),
+                ("max_steps", ctypes.c_int), ("done", ctypes.c_int), ("pad", ctypes.c_int)]
+
+
+CTL_BYTES = ctypes.sizeof(StepCtl)
 _lib = None
 
 
@@ -62,6 +70,8 @@ def lib():
         L.ocfd_level_fab_offset.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.ocfd_level_fab_offset.restype = ctypes.c_long
         L.ocfd_level_nbox.argtypes = [vp]
+        L.ocfd_level_device_bytes.argtypes = [vp]
+        L.ocfd_level_device_bytes.restype = ctypes.c_long
         L.ocfd_compute_dsdt.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(Parm), ctypes.c_int, vp]
         L.ocfd_rk_stage.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     ctypes.POINTER(Parm), ctypes.c_int, vp]
@@ -80,6 +90,13 @@ def lib():
         L.ocfd_reactor_advance_adaptive.argtypes = [ctypes.c_long, vp, ctypes.c_double, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp, vp, vp]
         L.ocfd_reactor_temperature.argtypes = [ctypes.c_long, vp, vp, vp]
+        L.ocfd_ctl_start.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                     ctypes.POINTER(Parm), vp]
+        L.ocfd_step_device.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, ctypes.POINTER(Parm), ctypes.c_int, vp]
+        L.ocfd_graph_capture_steps.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int, ctypes.POINTER(Parm), ctypes.c_int,
+                                               ctypes.c_int, vp, ctypes.POINTER(vp)]
+        L.ocfd_graph_launch.argtypes = [vp, vp]
+        L.ocfd_graph_destroy.argtypes = [vp]
         L.ocfd_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -167,6 +184,10 @@ class Level:
     def nbox(self):
         return len(self.boxes)
 
+    def device_bytes(self):
+        """Device memory the level holds itself (not the caller's MultiFabs)."""
+        return lib().ocfd_level_device_bytes(self.handle)
+
     def mf_size(self, ngrow=NGROW, ncomp=NCONS):
         return lib().ocfd_level_mf_size(self.handle, ngrow, ncomp)
 
@@ -225,6 +246,57 @@ class Level:
 
     def init_state(self, kind, S, parm, seed=0):
         check(lib().ocfd_init_state(self.handle, kind, _ptr(S), ctypes.byref(parm), seed), "ocfd_init_state")
+
+    # ---- device-driven time steps ---------------------------------------
+    def ctl_start(self, A, ctl, parm, cfl, time=0.0, stop_time=-1.0, max_steps=-1, stream=None):
+        """Initial dt on the device into ctl (a CTL_BYTES-byte device buffer)."""
+        check(lib().ocfd_ctl_start(self.handle, _ptr(A), _ptr(ctl), time, cfl, stop_time, max_steps,
+                                   ctypes.byref(parm), _stream(stream)), "ocfd_ctl_start")
+
+    def step_device(self, A, B, C, ctl, parm, scheme=2, flags=0, stream=None):
+        check(lib().ocfd_step_device(self.handle, _ptr(A), _ptr(B), _ptr(C), _ptr(ctl), scheme, ctypes.byref(parm),
+                                     flags, _stream(stream)), "ocfd_step_device")
+
+    def capture_steps(self, A, B, C, ctl, parm, nsteps, scheme=2, flags=0, stream=None):
+        """A StepGraph of nsteps device-driven steps, captured on `stream`
+        (a non-default torch stream)."""
+        return StepGraph(self, A, B, C, ctl, parm, nsteps, scheme, flags, stream)
+
+
+def new_ctl(device="cuda"):
+    import torch
+    return torch.zeros(CTL_BYTES, dtype=torch.uint8, device=device)
+
+
+def read_ctl(ctl):
+    """Host copy of a device StepCtl (synchronises with its stream)."""
+    return StepCtl.from_buffer_copy(bytes(ctl.cpu().numpy().tobytes()))
+
+
+class StepGraph:
+    """hipGraph of device-driven time steps; replayed with launch()."""
+
+    def __init__(self, level, A, B, C, ctl, parm, nsteps, scheme=2, flags=0, stream=None):
+        if stream is None:
+            raise ValueError("graph capture needs a non-default stream")
+        self.level = level                     # keeps the level's work-lists alive
+        self.keep = (A, B, C, ctl)            # and the buffers the graph writes
+        self.parm = parm
+        h = ctypes.c_void_p()
+        check(lib().ocfd_graph_capture_steps(level.handle, _ptr(A), _ptr(B), _ptr(C), _ptr(ctl), scheme,
+                                             ctypes.byref(parm), flags, nsteps, _stream(stream), ctypes.byref(h)),
+              "ocfd_graph_capture_steps")
+        self.handle = h
+        self.nsteps = nsteps
+
+    def launch(self, stream=None):
+        check(lib().ocfd_graph_launch(self.handle, _stream(stream)), "ocfd_graph_launch")
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _lib is not None:
+            _lib.ocfd_graph_destroy(h)
+            self.handle = None
 
 
 def run(level, A, B, C, parm, cfl, scheme=2, flags=0, stop_time=-1.0, nsteps=-1, time=0.0, dt=0.0, stream=None):

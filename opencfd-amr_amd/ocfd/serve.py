@@ -1,0 +1,285 @@
+"""Request loop serving independent cases on one GPU.
+
+Each request is a whole case: a flow case (a level, its three state
+MultiFabs and a device time control block) run to its stop time, or a batch
+of 0D reactors integrated to t_end.  State stays resident on the device from
+admission to completion.  Admission is by device-memory footprint against a
+budget and by free slots (one HIP stream per slot); requests that arrive while
+others are on the GPU are admitted as soon as a slot and memory free up, and
+their work overlaps with the running cases' on the device (continuous
+batching).  A flow case advances in chunks of device-driven time steps
+replayed from a hipGraph captured at admission, so the host only enqueues
+graph launches and polls a pinned copy of the control block; a finished case
+turns the rest of its chunk into no-ops on the device.
+
+The reference runs one case per process (Source/main.cpp); this loop is the
+serving form of its per-level advance (Source/Advance.cpp:81-177) and time
+loop (Source/EBR.cpp:189-306).
+"""
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import (CTL_BYTES, NCONS, NGROW, Level, Reactors, StepCtl, default_parm, ignition_batch, new_ctl,
+               REACTOR_NCOMP)
+
+MiB = 1 << 20
+
+
+@dataclass
+class FlowRequest:
+    """A flow case: `make()` returns (boxes, geom, init kind) as in ocfd.cases."""
+    name: str
+    make: object
+    flags: int = 0
+    cfl: float = 0.5
+    stop_time: float = -1.0
+    max_steps: int = -1
+    scheme: int = 2
+    seed: int = 0
+    parm_kw: dict = field(default_factory=dict)
+    arrival: float = 0.0          # seconds after the loop starts
+
+    def footprint(self, boxes=None):
+        """Device bytes while resident: 3 state MultiFabs (4 ghosts, 5 comps,
+        FP64), the level's FP32+FP64 primitive scratch (grow 4, 6 comps), the
+        partial divergence scratch (5 comps valid) and the small tables."""
+        if boxes is None:
+            boxes = self.make()[0]
+        ng = nv = 0
+        for lo, hi in boxes:
+            n = [hi[d] - lo[d] + 1 for d in range(3)]
+            nv += n[0] * n[1] * n[2]
+            ng += (n[0] + 2 * NGROW) * (n[1] + 2 * NGROW) * (n[2] + 2 * NGROW)
+        return 3 * 8 * NCONS * ng + 12 * 6 * ng + 8 * NCONS * nv + 128 * len(boxes) * 16 + MiB
+
+
+@dataclass
+class ReactorRequest:
+    """A batch of n constant-volume reactors integrated to t_end."""
+    name: str
+    n: int
+    seed: int = 7
+    t_end: float = 1e-3
+    rtol: float = 1e-4
+    atol: float = 1e-9
+    arrival: float = 0.0
+
+    def footprint(self):
+        return self.n * (REACTOR_NCOMP * 8 + 3 * 8 + 2 * 4) + MiB
+
+
+class _FlowJob:
+    def __init__(self, req, stream, chunk, torch):
+        self.req = req
+        self.stream = stream
+        boxes, geom, kind = req.make()
+        self.level = Level(boxes, geom)
+        self.parm = default_parm(**req.parm_kw)
+        with torch.cuda.stream(stream):
+            self.A = self.level.new_mf()
+            self.B = self.level.new_mf()
+            self.C = self.level.new_mf()
+            self.ctl = new_ctl()
+            self.host_ctl = torch.zeros(CTL_BYTES, dtype=torch.uint8).pin_memory()
+        stream.synchronize()   # the zero fills above ran on this stream
+        self.level.init_state(kind, self.A, self.parm, req.seed)
+        self.level.ctl_start(self.A, self.ctl, self.parm, req.cfl, 0.0, req.stop_time, req.max_steps, stream)
+        self.graph = self.level.capture_steps(self.A, self.B, self.C, self.ctl, self.parm, chunk, req.scheme,
+                                              req.flags, stream)
+        self.event = torch.cuda.Event()
+        self.in_flight = False
+        self.cells = sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for lo, hi in boxes)
+
+    def launch(self, torch):
+        self.graph.launch(self.stream)
+        with torch.cuda.stream(self.stream):
+            self.host_ctl.copy_(self.ctl, non_blocking=True)
+        self.event.record(self.stream)
+        self.in_flight = True
+
+    def finished(self):
+        """True once the last launched chunk completed and the run is done."""
+        if not self.event.query():
+            return False
+        self.in_flight = False
+        c = StepCtl.from_buffer_copy(self.host_ctl.numpy().tobytes())
+        self.steps, self.time = c.steps, c.time
+        return bool(c.done)
+
+    def result(self):
+        return self.A
+
+    def release(self):
+        """Keep what the device may still be reading until the loop ends:
+        a level's destructor frees with hipFree, which waits for the device."""
+        g = (self.level, self.graph)
+        self.A = self.B = self.C = self.ctl = None
+        self.level = self.graph = None
+        return g
+
+
+class _ReactorJob:
+    def __init__(self, req, stream, torch):
+        self.req = req
+        self.stream = stream
+        T0, p0, phi = ignition_batch(req.n, req.seed)
+        with torch.cuda.stream(stream):
+            self.r = Reactors(T0, p0, phi, stream)
+            self.r.advance_adaptive(req.t_end, rtol=req.rtol, atol=req.atol, stream=stream)
+        self.event = torch.cuda.Event()
+        self.event.record(stream)
+        self.in_flight = True
+        self.cells = 0
+
+    def launch(self, torch):
+        pass
+
+    def finished(self):
+        if not self.event.query():
+            return False
+        self.in_flight = False
+        self.steps = None
+        return True
+
+    def result(self):
+        return self.r.state
+
+    def release(self):
+        self.r = None
+        return ()
+
+
+class Server:
+    """Continuous-batching request loop over one GPU.
+
+    hbm_budget: bytes of device memory the resident cases may hold together.
+    slots: streams, i.e. the number of cases resident at once.
+    chunk: device-driven time steps per graph launch of a flow case.
+    keep_results: copy every finished case's state to the host (for checks).
+    """
+
+    def __init__(self, hbm_budget=64 << 30, slots=8, chunk=8, keep_results=False):
+        import torch
+        self.torch = torch
+        self.budget = hbm_budget
+        self.slots = slots
+        self.chunk = chunk
+        self.keep_results = keep_results
+        self.streams = [torch.cuda.Stream() for _ in range(slots)]   # default priority
+        self.records = []
+        self.results = {}
+        self.peak_used = 0     # largest admitted footprint sum
+        self.peak_active = 0
+
+    def run(self, requests, real_time=True):
+        """Serve `requests` (their `arrival` offsets in seconds; with
+        real_time False they are all present at the start).  Returns per-case
+        records (name, arrival, admit, done, latency, steps, cells) and the
+        loop's wall time."""
+        torch = self.torch
+        order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
+        pending = []           # arrived, not admitted (FIFO with first-fit)
+        active = {}            # slot -> (job, record)
+        free_slots = list(range(self.slots))
+        used = 0
+        graveyard = []
+        nxt = 0
+        t0 = time.perf_counter()
+        while nxt < len(order) or pending or active:
+            now = time.perf_counter() - t0
+            while nxt < len(order) and (not real_time or requests[order[nxt]].arrival <= now):
+                pending.append(order[nxt])
+                nxt += 1
+            # admission: first pending request that fits in memory and a slot
+            admitted = True
+            while admitted and pending and free_slots:
+                admitted = False
+                for pi, ri in enumerate(pending):
+                    req = requests[ri]
+                    fp = req.footprint()
+                    if fp > self.budget:
+                        raise RuntimeError(f"request {req.name} needs {fp} B, above the budget {self.budget} B")
+                    if used + fp <= self.budget:
+                        slot = free_slots.pop(0)
+                        s = self.streams[slot]
+                        t_adm = time.perf_counter() - t0
+                        job = (_FlowJob(req, s, self.chunk, torch) if isinstance(req, FlowRequest)
+                               else _ReactorJob(req, s, torch))
+                        used += fp
+                        self.peak_used = max(self.peak_used, used)
+                        self.peak_active = max(self.peak_active, len(active) + 1)
+                        rec = dict(name=req.name, arrival=req.arrival if real_time else 0.0, admit=t_adm,
+                                   footprint=fp, cells=job.cells)
+                        active[slot] = (job, rec)
+                        pending.pop(pi)
+                        admitted = True
+                        break
+            # keep every flow case's stream fed, then collect finished cases
+            for slot, (job, rec) in list(active.items()):
+                if not job.in_flight:
+                    job.launch(torch)
+                if job.finished():
+                    now = time.perf_counter() - t0
+                    rec["done"] = now
+                    rec["latency"] = now - rec["arrival"]
+                    rec["steps"] = job.steps
+                    if self.keep_results:
+                        with torch.cuda.stream(job.stream):
+                            self.results[rec["name"]] = job.result().cpu()
+                    graveyard.append(job.release())
+                    used -= rec["footprint"]
+                    del active[slot]
+                    free_slots.append(slot)
+                    self.records.append(rec)
+                elif not job.in_flight and isinstance(job, _FlowJob):
+                    job.launch(torch)
+            if real_time and not active and not pending and nxt < len(order):
+                wait = requests[order[nxt]].arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        del graveyard
+        return self.records, wall
+
+
+def latency_stats(records):
+    lat = np.array([r["latency"] for r in records])
+    return dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+                mean_ms=float(lat.mean() * 1e3), n=len(records))
+
+
+def poisson_mix(n, rate, seed=99, kinds=None):
+    """n requests with exponential inter-arrival times at `rate` per second
+    over the served case mix: 2D Sod/DMR slabs (64^2 .. 1024x256, FP32 flux),
+    3D periodic KH boxes (64^3 .. 128^3, viscous, FP32 flux) and 0D ignition
+    batches (1k .. 16k reactors)."""
+    from . import FLUX_FP32, PLANAR, VISC, cases
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    kinds = kinds or ("sod", "dmr", "kh", "reactors")
+    for q in range(n):
+        t += rng.exponential(1.0 / rate)
+        kind = kinds[int(rng.integers(len(kinds)))]
+        if kind == "sod":
+            nx = int(rng.choice([64, 128, 256, 512]))
+            ny = max(8, nx // 8)
+            out.append(FlowRequest(f"sod{q}", (lambda nx=nx, ny=ny: cases.sod(nx, ny, 2, max_grid=(128, 128, 2))),
+                                   flags=FLUX_FP32 | PLANAR, cfl=0.5, stop_time=0.2, arrival=t))
+        elif kind == "dmr":
+            ny = int(rng.choice([64, 128, 256]))
+            nx = 4 * ny
+            out.append(FlowRequest(f"dmr{q}", (lambda nx=nx, ny=ny: cases.dmr(nx, ny, 2, sizes=(32, 64, 128),
+                                                                             seed=1234 + q)),
+                                   flags=FLUX_FP32 | PLANAR, cfl=0.5, stop_time=0.01, arrival=t))
+        elif kind == "kh":
+            n3 = int(rng.choice([64, 96, 128]))
+            out.append(FlowRequest(f"kh{q}", (lambda n3=n3: cases.kh3d(n3)), flags=FLUX_FP32 | VISC, cfl=0.5,
+                                   max_steps=10, seed=42, arrival=t))
+        else:
+            nr = int(rng.choice([1024, 4096, 16384]))
+            out.append(ReactorRequest(f"ign{q}", nr, seed=int(rng.integers(1 << 30)), t_end=1e-4, arrival=t))
+    return out

@@ -15,13 +15,6 @@ pytestmark = pytest.mark.gpu
 TOL64 = 1e-10
 
 
-@pytest.fixture(scope="module")
-def torch_cuda():
-    import torch
-    assert torch.cuda.is_available(), "GPU tests need a HIP device"
-    return torch
-
-
 def _level(case):
     import ocfd
     boxes, g, kind = case
