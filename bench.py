@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-chem", action="store_true", help="skip the 0D ignition batch")
     ap.add_argument("--reactors", type=int, default=65536)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served mixed stream")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (HBM counter calibration)")
     return ap.parse_args()
@@ -54,6 +57,8 @@ def calib_copy():
 
 
 def timed_run(lev, A, B, C, p, flags, steps, warmup):
+    """Host-driven steps (the reference driver's form: the CFL minimum goes to
+    the host every step)."""
     import torch
     import ocfd
     cfl = 0.5
@@ -63,6 +68,30 @@ def timed_run(lev, A, B, C, p, flags, steps, warmup):
     ocfd.run(lev, A, B, C, p, cfl, 2, flags, nsteps=steps, time=t, dt=dt)
     torch.cuda.synchronize()
     return time.perf_counter() - t0
+
+
+def timed_graph(lev, A, B, C, p, flags, steps, warmup):
+    """Device-driven steps replayed from hipGraphs (dt rule on the device, no
+    host round trip): `warmup` one-step replays, then one replay of a
+    `steps`-step graph is timed on its stream."""
+    import torch
+    import ocfd
+    s = torch.cuda.Stream()
+    ctl = ocfd.new_ctl()
+    with torch.cuda.stream(s):
+        lev.ctl_start(A, ctl, p, 0.5, stream=s)
+        g1 = lev.capture_steps(A, B, C, ctl, p, 1, 2, flags, stream=s)
+        gk = lev.capture_steps(A, B, C, ctl, p, steps, 2, flags, stream=s)
+    for _ in range(warmup):
+        g1.launch(s)
+    s.synchronize()
+    t0 = time.perf_counter()
+    gk.launch(s)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    c = ocfd.read_ctl(ctl)
+    assert c.steps == warmup + steps and not c.done, (c.steps, c.done)
+    return el
 
 
 def kernel_roofline(lev, A, B, C, p, flags, ncell):
@@ -125,6 +154,30 @@ def chemistry_0d(n_react, cpu_sample=64):
     return out
 
 
+def serving(nreq, rate, seed=99):
+    """Served mixed stream: Poisson arrivals (seeded) of 2D Sod/DMR slabs,
+    3D periodic KH boxes and 0D ignition batches through the continuous-
+    batching request loop; latency = completion - arrival."""
+    import numpy as np
+    from ocfd import serve
+    warm = serve.poisson_mix(6, rate=1e3, seed=seed + 1)
+    serve.Server(hbm_budget=32 << 30, slots=8, chunk=8).run(warm, real_time=False)   # first-use costs
+    reqs = serve.poisson_mix(nreq, rate=rate, seed=seed)
+    srv = serve.Server(hbm_budget=32 << 30, slots=8, chunk=8)
+    recs, wall = srv.run(reqs, real_time=True)
+    st = serve.latency_stats(recs)
+    cell_updates = sum(r["cells"] * (r["steps"] or 0) for r in recs)
+    kinds = {}
+    for r in recs:
+        k = r["name"].rstrip("0123456789")
+        kinds.setdefault(k, []).append(r["latency"] * 1e3)
+    return {"requests": len(recs), "rate_per_s": rate, "wall_s": wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+            "mean_ms": st["mean_ms"], "requests_per_s": len(recs) / wall,
+            "flow_Mcell_updates_per_s": cell_updates / wall / 1e6,
+            "p50_ms_by_kind": {k: float(np.percentile(v, 50)) for k, v in kinds.items()},
+            "hbm_budget_GiB": 32, "slots": 8, "peak_resident": srv.peak_active}
+
+
 def cpu_baseline(n, steps):
     sys.path.insert(0, str(ROOT / "oracle"))
     sys.path.insert(0, str(ROOT / "tests"))
@@ -167,7 +220,9 @@ def main():
 
     flags32 = ocfd.VISC | ocfd.FLUX_FP32
     lev.init_state(kind, A, p, seed=42)
-    t32 = timed_run(lev, A, B, C, p, flags32, args.steps, args.warmup)
+    t32 = timed_graph(lev, A, B, C, p, flags32, args.steps, args.warmup)
+    lev.init_state(kind, A, p, seed=42)
+    t32h = timed_run(lev, A, B, C, p, flags32, args.steps, args.warmup)
     dom, achieved, kern32 = kernel_roofline(lev, A, B, C, p, flags32, ncell)
     value = ncell * args.steps / t32 / 1e6
 
@@ -176,9 +231,18 @@ def main():
         flags64 = ocfd.VISC
         lev.init_state(kind, A, p, seed=42)
         t64 = timed_run(lev, A, B, C, p, flags64, args.steps, args.warmup)
+        lev.init_state(kind, A, p, seed=42)
+        t64g = timed_graph(lev, A, B, C, p, flags64, args.steps, args.warmup)
         _, _, kern64 = kernel_roofline(lev, A, B, C, p, flags64, ncell)
         res64 = {"value": ncell * args.steps / t64 / 1e6, "ms_per_step": 1e3 * t64 / args.steps,
-                 "kernels": kern64}
+                 "form": "host-driven steps (plain FP64 GPU form)",
+                 "graph_value": ncell * args.steps / t64g / 1e6, "kernels": kern64}
+    del A, B, C, lev
+    torch.cuda.empty_cache()
+
+    serve_res = None
+    if not args.no_serve:
+        serve_res = serving(args.serve_requests, args.serve_rate)
 
     chem = None
     if not args.no_chem:
@@ -207,8 +271,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "hbm_frac_step": 320.0 * value * 1e6 / 1e9 / HBM_PEAK_GBS,
         "kernels": kern32,
+        "form": "FP32-flux device-driven steps replayed from a hipGraph",
+        "host_driven_fp32": {"value": ncell * args.steps / t32h / 1e6, "ms_per_step": 1e3 * t32h / args.steps},
         "fp64_form": res64,
         "fp32_over_fp64": (value / res64["value"]) if res64 else None,
+        "serving": serve_res,
         "cpu_baseline": cpu,
         "chemistry_0d": chem,
     }

@@ -61,17 +61,23 @@ typedef struct ocfd_level_s* ocfd_level;
  * Source/EBR.cpp:189-306), kept in device memory so a whole time step --
  * ghost fills, RK stages, CFL reduction and the dt update -- runs without a
  * host round trip and can be captured into a hipGraph.  Allocate
- * sizeof(ocfd_step_ctl) bytes of device memory per level. */
+ * sizeof(ocfd_step_ctl) (256) bytes of 128-byte aligned device memory per
+ * level. */
 typedef struct ocfd_step_ctl {
     double time;            /* current time */
     double dt;              /* dt of the next step */
     double cfl;
     double stop_time;       /* < 0: none */
-    unsigned long long red; /* CFL reduction scratch (bit pattern of a positive double) */
     int steps;              /* steps taken */
     int max_steps;          /* < 0: none */
     int done;               /* set when stop_time or max_steps is reached */
-    int pad;
+    int pad0;
+    char pad1[80];
+    /* CFL reduction slot (bit pattern of a positive double), on a cache line
+     * of its own: it is only ever accessed with device-scope atomics, the
+     * fields above only with plain loads and stores */
+    unsigned long long red;
+    char pad2[120];
 } ocfd_step_ctl;
 
 /* Parm defaults + Parm::Initialize (Source/Constants.H:9-29, Source/Constants.cpp:7-11).

@@ -66,22 +66,12 @@ struct StageArgs {
     const ocfd_step_ctl* ctl;    // device time control, or null (host-driven)
 };
 
-// The control block is written by one kernel and read by every workgroup of
-// the next ones on all XCDs; its fields are accessed with device-scope atomic
-// loads and stores so no XCD can read a line its own L2 holds stale (the CFL
-// minimum arrives there by device-scope atomics).
-template <class T>
-__device__ __forceinline__ T ctl_ld(const T* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <class T>
-__device__ __forceinline__ void ctl_st(T* p, T v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ bool ctl_done(const ocfd_step_ctl* c) { return c && ctl_ld(&c->done); }
+// The control block's fields are written by single-thread kernels with plain
+// stores and read with plain loads by the following kernels, like any other
+// buffer; its reduction slot sits on a separate cache line and is touched only
+// by device-scope atomics (a plain load of a line that an atomic updated may
+// hit a stale copy in the reading XCD's L2).
+__device__ __forceinline__ bool ctl_done(const ocfd_step_ctl* c) { return c && c->done; }
 
 // Workgroups are dealt round-robin over the 8 XCDs (workgroup b runs on XCD
 // b % 8), each with its own L2.  Give every XCD a contiguous range of the
@@ -338,7 +328,7 @@ __device__ __forceinline__ void finish_cell(const StageArgs& a, const BoxDesc& B
         L[UEDEN] += g * a.scur[s0 + 3 * st];
     }
     if (a.snew) {
-        const double cdt = a.ctl ? a.c_dt * ctl_ld(&a.ctl->dt) : a.c_dt;
+        const double cdt = a.ctl ? a.c_dt * a.ctl->dt : a.c_dt;
 #pragma unroll
         for (int n = 0; n < NC; ++n) {
             double v = a.sold ? a.c_cur * a.scur[s0 + n * st] + a.c_old * a.sold[s0 + n * st]

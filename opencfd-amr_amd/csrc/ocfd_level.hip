@@ -87,8 +87,8 @@ __global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes,
                                               const ocfd_step_ctl* __restrict__ ctl, double toff)
 {
     if (ctl) {
-        if (ctl_ld(&ctl->done)) return;
-        time = ctl_ld(&ctl->time) + toff * ctl_ld(&ctl->dt);
+        if (ctl->done) return;
+        time = ctl->time + toff * ctl->dt;
     }
     const int4 t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.z) return;
@@ -215,9 +215,9 @@ __device__ __forceinline__ unsigned long long red_big()
 
 __global__ void k_ctl_init(ocfd_step_ctl* c, double time, double cfl, double stop_time, int max_steps)
 {
-    ctl_st(&c->time, time); ctl_st(&c->dt, 0.0); ctl_st(&c->cfl, cfl); ctl_st(&c->stop_time, stop_time);
+    c->time = time; c->dt = 0.0; c->cfl = cfl; c->stop_time = stop_time;
+    c->steps = 0; c->max_steps = max_steps; c->done = 0; c->pad0 = 0;
     atomicExch(&c->red, red_big());
-    ctl_st(&c->steps, 0); ctl_st(&c->max_steps, max_steps); ctl_st(&c->done, 0); ctl_st(&c->pad, 0);
 }
 
 // dt_new from the CFL minimum, clipped so the run ends on stop_time; then the
@@ -225,14 +225,14 @@ __global__ void k_ctl_init(ocfd_step_ctl* c, double time, double cfl, double sto
 __device__ void ctl_set_dt(ocfd_step_ctl* c, double dt_new, double time, int steps)
 {
 #pragma clang fp contract(off)
-    const double stop = ctl_ld(&c->stop_time);
-    const int max_steps = ctl_ld(&c->max_steps);
+    const double stop = c->stop_time;
+    const int max_steps = c->max_steps;
     const double eps = 0.001 * dt_new;
     if (stop >= 0.0 && time + dt_new > stop - eps) dt_new = stop - time;
-    ctl_st(&c->dt, dt_new);
-    ctl_st(&c->time, time);
-    ctl_st(&c->steps, steps);
-    ctl_st(&c->done, (int)((max_steps >= 0 && steps >= max_steps) || (stop >= 0.0 && !(time < stop))));
+    c->dt = dt_new;
+    c->time = time;
+    c->steps = steps;
+    c->done = (int)((max_steps >= 0 && steps >= max_steps) || (stop >= 0.0 && !(time < stop)));
 }
 
 // initial dt: cfl * estdt (EBR::computeInitialDt, Source/EBR.cpp:189-229 with
@@ -240,8 +240,8 @@ __device__ void ctl_set_dt(ocfd_step_ctl* c, double dt_new, double time, int ste
 __global__ void k_ctl_first(ocfd_step_ctl* c)
 {
 #pragma clang fp contract(off)
-    const double dt = red_value(atomicExch(&c->red, red_big())) * ctl_ld(&c->cfl);
-    ctl_set_dt(c, dt, ctl_ld(&c->time), ctl_ld(&c->steps));
+    const double dt = red_value(atomicExch(&c->red, red_big())) * c->cfl;
+    ctl_set_dt(c, dt, c->time, c->steps);
 }
 
 // after a step: t += dt, then dt_new = min(cfl*estdt, change_max*dt) with
@@ -249,11 +249,11 @@ __global__ void k_ctl_first(ocfd_step_ctl* c)
 __global__ void k_ctl_update(ocfd_step_ctl* c)
 {
 #pragma clang fp contract(off)
-    if (ctl_ld(&c->done)) return;
-    const double est = red_value(atomicExch(&c->red, red_big())) * ctl_ld(&c->cfl);
-    const double dt = ctl_ld(&c->dt);
-    const double time = ctl_ld(&c->time) + dt;
-    ctl_set_dt(c, fmin(est, 1.1 * dt), time, ctl_ld(&c->steps) + 1);
+    if (c->done) return;
+    const double est = red_value(atomicExch(&c->red, red_big())) * c->cfl;
+    const double dt = c->dt;
+    const double time = c->time + dt;
+    ctl_set_dt(c, fmin(est, 1.1 * dt), time, c->steps + 1);
 }
 
 // A <- B for the forward-Euler step under device control (a no-op once done)

@@ -47,8 +47,9 @@ class Geom(ctypes.Structure):
 class StepCtl(ctypes.Structure):
     """Device time control of one level (struct ocfd_step_ctl)."""
     _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double), ("cfl", ctypes.c_double),
-                ("stop_time", ctypes.c_double), ("red", ctypes.c_ulonglong), ("steps", ctypes.c_int),
-                ("max_steps", ctypes.c_int), ("done", ctypes.c_int), ("pad", ctypes.c_int)]
+                ("stop_time", ctypes.c_double), ("steps", ctypes.c_int), ("max_steps", ctypes.c_int),
+                ("done", ctypes.c_int), ("pad0", ctypes.c_int), ("pad1", ctypes.c_char * 80),
+                ("red", ctypes.c_ulonglong), ("pad2", ctypes.c_char * 120)]
 
 
 CTL_BYTES = ctypes.sizeof(StepCtl)
