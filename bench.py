@@ -94,25 +94,40 @@ def timed_graph(lev, A, B, C, p, flags, steps, warmup):
     return el
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_kh3d_256_fp32.txt"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration)."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    txt = "".join(l for l in TRAFFIC_FILE.read_text().splitlines(True) if not l.startswith("#"))
+    k = json.loads(txt)["kernels"].get(kernel)
+    return k["traffic_bytes"] if k else None
+
+
 def kernel_roofline(lev, A, B, C, p, flags, ncell):
     """Per-kernel average durations of one RK stage (stage-2 form, HIP events on
     the launch stream) and the roofline figures of the dominant kernel."""
     lev.fill_boundary(A, 0.0)
     ms = lev.profile_stage(A, B, C, 0.25, 0.75, 1e-9, p, flags, reps=5)
     names = ["sweep_x", "sweep_y", "sweep_z_update", "fill_boundary", "primitives"]
-    # algorithmic HBM bytes per valid cell of each kernel (FP64 state, 5 comps):
-    #   sweep_x: read S 40, write dSdt partial 40
-    #   sweep_y: read S 40, read+write partial 80
-    #   sweep_z_update: read S 40, partial 40, U^n 40, write U^(k+1) 40
+    # algorithmic HBM bytes per valid cell of each kernel; f = bytes of the flux
+    # type (4 FP32, 8 FP64), state FP64 with 5 comps:
+    #   sweep_x: 6 primitives in, 5-comp partial divergence out       = 11 f
+    #   sweep_y: primitives in, partial in and out                     = 16 f
+    #   sweep_z_update: primitives, partial, U^(k) and U^n in (80 B),
+    #                   U^(k+1) out (40 B)                             = 11 f + 120
     #   fill: ghost cells only, read+write 80 B per ghost cell
-    #   primitives: read S 40, write 6 primitives (FP32 24 / FP64 48) per cell of grow(bx,3)
+    #   primitives: conserved in (40 B), 6 primitives out, on grow(bx,3)
     nx = lev.boxes[0][1][0] - lev.boxes[0][0][0] + 1
     ghost = (nx + 8) ** 3 - nx ** 3
-    pb = 24.0 if flags & 4 else 48.0
-    bytes_ = [80.0 * ncell, 120.0 * ncell, 160.0 * ncell, 80.0 * ghost, (40.0 + pb) * (nx + 6) ** 3]
+    f = 4.0 if flags & 4 else 8.0
+    bytes_ = [11 * f * ncell, 16 * f * ncell, (11 * f + 120.0) * ncell, 80.0 * ghost, (40.0 + 6 * f) * (nx + 6) ** 3]
     kern = {}
     for nm, t, b in zip(names, ms, bytes_):
-        kern[nm] = {"ms": t, "GBps": b / (t * 1e-3) / 1e9}
+        kern[nm] = {"ms": t, "GBps": b / (t * 1e-3) / 1e9, "algorithmic_bytes": b}
     dom = max(range(3), key=lambda i: ms[i])
     achieved = bytes_[dom] / (ms[dom] * 1e-3) / 1e9
     return names[dom], achieved, kern
@@ -268,7 +283,10 @@ def main():
         "config": {"workload": f"3D viscous Kelvin-Helmholtz, periodic {n}^3 single level, SSP-RK3, "
                                "FP32 fluxes / FP64 state", "cells": ncell, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "algorithmic_bytes": kern32[dom]["algorithmic_bytes"],
+                     "traffic": measured_traffic(dom) if n == 256 else None,
+                     "traffic_source": "profiles/hbm_traffic_kh3d_256_fp32.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"},
         "hbm_frac_step": 320.0 * value * 1e6 / 1e9 / HBM_PEAK_GBS,
         "kernels": kern32,
         "form": "FP32-flux device-driven steps replayed from a hipGraph",
