@@ -32,6 +32,7 @@ extern "C" {
 #define OCFD_GRAVITY    2   /* ebr.do_gravity (Source/EBR.cpp:26)  */
 #define OCFD_FLUX_FP32  4   /* convective/viscous fluxes in FP32, state and accumulation FP64 */
 #define OCFD_PLANAR     8   /* z-uniform data: skip the z sweep (its divergence is exactly 0) */
+#define OCFD_EB_REDIST 16   /* ebr.do_redistribute (Source/EBR.cpp:27): small cut cells scaled by vfrac */
 
 /* struct Parm, Source/Constants.H:7-64 (non-CHEM members, same order) */
 typedef struct ocfd_parm {
@@ -42,7 +43,8 @@ typedef struct ocfd_parm {
 /* Level geometry + physical BCs: amrex::Geometry (domain, prob_lo, dx,
  * periodicity) + ebr.lo_bc/hi_bc codes (0 Interior, 1 Inflow, 2 Outflow,
  * 3 Symmetry, 4 SlipWall, 5 NoSlipWall; Exec/<case>/inputs) + the case's
- * ext_dir fill (probtype 1 = Exec/1_DMR/prob_bcfill.cpp). */
+ * ext_dir fill (probtype 1 = Exec/1_DMR/prob_bcfill.cpp; probtype 2 = the inflow
+ * state left_state at x-lo, Exec/3_FSTEP/prob_bcfill.cpp). */
 typedef struct ocfd_geom {
     int dom_lo[3], dom_hi[3];
     double prob_lo[3];
@@ -55,6 +57,26 @@ typedef struct ocfd_geom {
 } ocfd_geom;
 
 typedef struct ocfd_level_s* ocfd_level;
+
+/* Embedded boundary: a body extruded along z, given by an implicit function
+ * (> 0 inside the body) -- the eb2.geom_type choices the cases use
+ * (Exec/3_FSTEP/inputs box, Exec/4_WEDGE/inputs plane) plus a cylinder.
+ * fluid_inside flips the sign (fluid inside the cylinder or box).
+ * Cells whose volume fraction is below small_volfrac (eb2.small_volfrac;
+ * <= 0 means 1e-14) are covered. */
+#define OCFD_EB_NONE     0
+#define OCFD_EB_CYLINDER 1   /* centre (center[0], center[1]), radius */
+#define OCFD_EB_BOX      2   /* [lo, hi] in x and y */
+#define OCFD_EB_PLANE    3   /* body where (x - point) . normal > 0 (x and y components) */
+typedef struct ocfd_eb_spec {
+    int kind;
+    int fluid_inside;
+    double center[3];
+    double radius;
+    double lo[3], hi[3];
+    double point[3], normal[3];
+    double small_volfrac;
+} ocfd_eb_spec;
 
 /* Device-resident time control of one level: the state of the coarse time
  * loop (Source/main.cpp:61-70 with EBR::computeInitialDt / computeNewDt,
@@ -97,6 +119,30 @@ int ocfd_level_destroy(ocfd_level lev);
  * comps, and the offset (in doubles) of box b's FAB inside it. */
 long ocfd_level_mf_size(ocfd_level lev, int ngrow, int ncomp);
 long ocfd_level_fab_offset(ocfd_level lev, int box, int ngrow, int ncomp);
+
+/* Cut-cell geometry of the box [lo, hi] grown by ng cells, on the host: the
+ * fields EB2::Build provides to the EB box path (Source/Geometry.cpp:9-29,
+ * Source/EB/AdvanceBox.cpp:15-31), each an array over the grown box (i
+ * fastest): flag (EBCellFlag bit layout: bits 0-1 type 0 regular / 1 cut /
+ * 3 covered, bits 2-4 volume count, bit 5+(i+1)+3(j+1)+9(k+1) neighbour
+ * (i,j,k) connected), vfrac, apx/apy/apz (area fraction of the cell's low
+ * x/y/z face), fcx/fcy/fcz (2 comps: face centroid offsets in cell units),
+ * bcent (3 comps: boundary centroid).  Any output pointer may be NULL.
+ * Needs no GPU. */
+int ocfd_eb_geometry(const ocfd_eb_spec* spec, const ocfd_geom* g, const int* lo, const int* hi, int ng,
+                     unsigned* flag, double* vfrac, double* apx, double* apy, double* apz,
+                     double* fcx, double* fcy, double* fcz, double* bcent);
+
+/* ocfd_level_create with an embedded boundary: boxes are classified as in
+ * EBR::compute_dSdt (Source/Advance.cpp:213,231): covered (every valid cell
+ * covered: dSdt = 0), regular (grow(bx,4) all regular: WENO path) or cut
+ * (the EB path of Source/EB/AdvanceBox.cpp).  spec NULL or kind
+ * OCFD_EB_NONE gives an all-regular level. */
+int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom* g, const ocfd_eb_spec* spec,
+                         ocfd_level* out);
+
+/* Box classification of the level: types[b] = 0 regular, 1 cut, 2 covered. */
+int ocfd_level_box_types(ocfd_level lev, int* types);
 
 /* Device memory held by the level itself (work-lists, primitive and partial
  * divergence scratch), in bytes; the caller's MultiFabs are not included.
@@ -141,10 +187,17 @@ int ocfd_advance(ocfd_level lev, double* A, double* B, double* C, double time, d
 /* Case initial data into the valid cells of state MultiFab S (4 ghosts, 5 comps):
  * kind 0 = Sod (Exec/0_SOD/prob.H:11-38), 1 = double Mach reflection
  * (Exec/1_DMR/prob.H:11-45), 2 = 3D viscous Kelvin-Helmholtz shear layer
- * (the 3D counterpart of Exec/8_KH/prob.H:11-42, seeded).  The
+ * (the 3D counterpart of Exec/8_KH/prob.H:11-42, seeded), 3 = uniform
+ * inflow over an embedded body (Exec/3_FSTEP/prob.H:11-40: the inflow state,
+ * covered cells at rest with the same energy; needs an EB level).  The
  * amrex_probinit/initData pair of the reference (Source/EBR.cpp:111-170).
  * Ghost cells are zeroed.  Synchronous. */
 int ocfd_init_state(ocfd_level lev, int kind, double* S, const ocfd_parm* parm, unsigned long long seed);
+
+/* Inflow state of the EB cases (Exec/3_FSTEP/prob.cpp:24-29, literal 0.4) for
+ * p, rho, u (defaults of Exec/3_FSTEP/prob_parm.H when the pointer args are
+ * negative). */
+int ocfd_inflow_state(double p, double rho, double u, double* state);
 
 /* DMR inflow states used by probtype 1 (Exec/1_DMR/prob.cpp:28-40). */
 int ocfd_dmr_states(double* left, double* right);

@@ -111,16 +111,63 @@ void init_kh3d(const ocfd::LevelDev& L, const ocfd_parm& parm, uint64_t seed, st
             }
 }
 
+// Exec/3_FSTEP/prob.H:11-40 with prob_parm.H defaults: the inflow state
+// everywhere, covered cells with zero momentum and the same energy
+int init_eb_inflow(const ocfd::LevelDev& L, const ocfd_parm& parm, std::vector<double>& h)
+{
+    if (!L.has_eb) return OCFD_EINVAL;
+    const double Pt = 2026.5, rhot = 0.08938, uxt = 534.49;
+    ocfd_geom g{};
+    for (int d = 0; d < 3; ++d) {
+        g.dom_lo[d] = L.geom.dom_lo[d]; g.dom_hi[d] = L.geom.dom_hi[d];
+        g.prob_lo[d] = L.geom.prob_lo[d]; g.dx[d] = L.geom.dx[d];
+    }
+    for (const BoxDesc& B : L.boxes) {
+        const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+        std::vector<unsigned> fl((size_t)nx * ny * nz);
+        ocfd_eb_geometry(&L.eb_spec, &g, B.lo, B.hi, 0, fl.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                         nullptr, nullptr);
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+            for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+                for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+                    const bool cov = (fl[((size_t)(k - B.lo[2]) * ny + (j - B.lo[1])) * nx + (i - B.lo[0])] & 3u) == 3u;
+                    const double et = Pt / (parm.eos_gamma - 1.0);
+                    h[hidx(B, i, j, k, 0)] = rhot;
+                    h[hidx(B, i, j, k, 1)] = cov ? 0.0 : rhot * uxt;
+                    h[hidx(B, i, j, k, 2)] = 0.0;
+                    h[hidx(B, i, j, k, 3)] = 0.0;
+                    h[hidx(B, i, j, k, 4)] = et + 0.5 * rhot * uxt * uxt;
+                }
+    }
+    return OCFD_OK;
+}
+
 } // namespace
 
 extern "C" {
 
-// Initial data of case `kind` (0 Sod, 1 double Mach reflection, 2 3D KH) into
+// inflow_state of Exec/3_FSTEP/prob.cpp:24-29 (literal 0.4)
+int ocfd_inflow_state(double p, double rho, double u, double* state)
+{
+    if (!state) return OCFD_EINVAL;
+    if (p < 0) p = 2026.5;
+    if (rho < 0) rho = 0.08938;
+    if (u < 0) u = 534.49;
+    state[0] = rho;
+    state[1] = u * rho;
+    state[2] = 0;
+    state[3] = 0;
+    state[4] = p / 0.4 + 0.5 * rho * u * u;
+    return OCFD_OK;
+}
+
+// Initial data of case `kind` (0 Sod, 1 double Mach reflection, 2 3D KH,
+// 3 inflow over an embedded body) into
 // the valid cells of state MultiFab S (4 ghosts, 5 comps) of level lev;
 // ghost cells are zeroed.  Synchronous.
 int ocfd_init_state(ocfd_level lev, int kind, double* S, const ocfd_parm* parm, unsigned long long seed)
 {
-    if (!lev || !S || !parm || kind < 0 || kind > 2) {
+    if (!lev || !S || !parm || kind < 0 || kind > 3) {
         ocfd::set_error("ocfd_init_state: invalid arguments");
         return OCFD_EINVAL;
     }
@@ -128,7 +175,11 @@ int ocfd_init_state(ocfd_level lev, int kind, double* S, const ocfd_parm* parm, 
     std::vector<double> h((size_t)L.mf_state, 0.0);
     if (kind == 0) init_sod(L, *parm, h);
     else if (kind == 1) init_dmr(L, *parm, h);
-    else init_kh3d(L, *parm, seed, h);
+    else if (kind == 2) init_kh3d(L, *parm, seed, h);
+    else if (init_eb_inflow(L, *parm, h)) {
+        ocfd::set_error("ocfd_init_state: the inflow-over-body case needs a level with an embedded boundary");
+        return OCFD_EINVAL;
+    }
     hipError_t e = hipMemcpy(S, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         ocfd::set_error(std::string("ocfd_init_state: ") + hipGetErrorString(e));

@@ -32,6 +32,9 @@ struct BoxDesc {
     long nptg;            // component stride of the grown FAB
     long nptv;            // component stride of the valid FAB
     long off_p;           // offset (elements) of this box's primitive FAB (6 comps, grown layout)
+    long off_e;           // cut boxes: offset (grown-FAB units) in the embedded-boundary arrays, else -1
+    int type;             // 0 regular, 1 cut (embedded-boundary path), 2 covered
+    int pad;
     double dxinv[3];
     double dx[3];
 };

@@ -293,6 +293,7 @@ void launch_sweeps(const LevelDev& L, const StageArgs& a, hipStream_t s)
         if (visc) launch_sweeps_t<double, true>(L, a, planar, s);
         else launch_sweeps_t<double, false>(L, a, planar, s);
     }
+    launch_eb(L, a, s);   // cut and covered boxes
 }
 
 // One sweep alone (0 = x, 1 = y, 2 = z final), for per-kernel timing.

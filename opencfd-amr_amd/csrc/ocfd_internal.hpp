@@ -33,7 +33,21 @@ struct LevelDev {
     long mf_valid = 0;              // doubles in a 0-ghost 5-comp MultiFab
     long mf_state = 0;              // doubles in a 4-ghost 5-comp MultiFab
     long dev_bytes = 0;             // device memory held by the level
+    // embedded boundary (cut boxes only, grown layout, box order)
+    bool has_eb = false;
+    ocfd_eb_spec eb_spec{};
+    int n_cut = 0, n_covered = 0;
+    long mf_eb = 0;                 // grown cells over all cut boxes
+    uint32_t* d_ebflag = nullptr;   // EBCellFlag words
+    double* d_ebgeo = nullptr;      // vfrac, apx, apy, apz
+    double* d_ebq = nullptr;        // primitives, FP64, 7 comps (rho,u,v,w,p,T,c)
+    double* d_ebF[3] = {nullptr, nullptr, nullptr};   // face fluxes per direction, 5 comps
+    int4* d_tile_eq = nullptr; int ntile_eq = 0;      // cut boxes: grow(bx,2) cells
+    int4* d_tile_ef[3] = {nullptr, nullptr, nullptr}; int ntile_ef[3] = {0, 0, 0};   // faces
+    int4* d_tile_ed = nullptr; int ntile_ed = 0;      // cut and covered boxes: valid cells
 };
+
+void launch_eb(const LevelDev& L, const StageArgs& a, hipStream_t s);
 
 void launch_sweeps(const LevelDev& L, const StageArgs& a, hipStream_t s);
 void launch_sweep_one(const LevelDev& L, const StageArgs& a, int which, hipStream_t s);

@@ -102,9 +102,11 @@ __global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes,
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         if (G.periodic[d]) {
+            // periodic image; a domain thinner than the ghost width wraps more than once
             const int len = G.dom_hi[d] - G.dom_lo[d] + 1;
-            if (iv[d] < G.dom_lo[d]) iv[d] += len;
-            else if (iv[d] > G.dom_hi[d]) iv[d] -= len;
+            int r = (iv[d] - G.dom_lo[d]) % len;
+            if (r < 0) r += len;
+            iv[d] = G.dom_lo[d] + r;
         }
         outside[d] = iv[d] < G.dom_lo[d] ? -1 : (iv[d] > G.dom_hi[d] ? 1 : 0);
     }
@@ -149,6 +151,10 @@ __global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes,
         if (G.bc[0][0][0] == BC_EXT && i < G.dom_lo[0]) {
             for (int n = 0; n < NC; ++n) S[dst + n * B.nptg] = G.left[n];
         }
+    } else if (G.probtype == 2) {
+        // inflow state at x-lo (Exec/3_FSTEP/prob_bcfill.cpp:8-33)
+        if (G.bc[0][0][0] == BC_EXT && i < G.dom_lo[0])
+            for (int n = 0; n < NC; ++n) S[dst + n * B.nptg] = G.left[n];
     }
 }
 
@@ -292,6 +298,8 @@ static void free_level(LevelDev& L)
     hipFree(L.d_tile_x); hipFree(L.d_tile_y); hipFree(L.d_tile_z); hipFree(L.d_tile_g); hipFree(L.d_tile_c);
     hipFree(L.d_acc); hipFree(L.d_red); hipFree(L.d_tile_p); hipFree(L.d_tile_pp);
     hipFree(L.d_prim32); hipFree(L.d_prim64); hipFree(L.d_tile_v);
+    hipFree(L.d_ebflag); hipFree(L.d_ebgeo); hipFree(L.d_ebq); hipFree(L.d_tile_eq); hipFree(L.d_tile_ed);
+    for (int d = 0; d < 3; ++d) { hipFree(L.d_ebF[d]); hipFree(L.d_tile_ef[d]); }
     L = LevelDev{};
 }
 
@@ -329,6 +337,19 @@ int ocfd_parm_initialize(ocfd_parm* p)
 
 int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g, ocfd_level* out)
 {
+    return ocfd_level_create_eb(nbox, lo, hi, g, nullptr, out);
+}
+
+int ocfd_level_box_types(ocfd_level lev, int* types)
+{
+    if (!lev || !types) { set_error("ocfd_level_box_types: invalid arguments"); return OCFD_EINVAL; }
+    for (int b = 0; b < lev->dev.nbox; ++b) types[b] = lev->dev.boxes[b].type;
+    return OCFD_OK;
+}
+
+int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom* g, const ocfd_eb_spec* spec,
+                         ocfd_level* out)
+{
     if (!out || !g || nbox <= 0 || !lo || !hi) { set_error("ocfd_level_create: invalid arguments"); return OCFD_EINVAL; }
     *out = nullptr;
     ocfd_level lev = new ocfd_level_s;
@@ -356,6 +377,8 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
         B.off_s = off_s;
         B.off_d = off_d;
         B.off_p = off_s / NC * 6;
+        B.off_e = -1;
+        B.type = 0;
         B.geom = 0;
         off_s += B.nptg * NC;
         off_d += B.nptv * NC;
@@ -363,6 +386,43 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
     }
     L.mf_state = off_s;
     L.mf_valid = off_d;
+
+    // embedded boundary: per-box geometry over grow(bx,4) and the box kinds of
+    // EBR::compute_dSdt (Source/Advance.cpp:213,231)
+    std::vector<uint32_t> eb_flag;
+    std::vector<double> eb_geo;
+    if (spec && spec->kind != OCFD_EB_NONE) {
+        for (int b = 0; b < nbox; ++b) {
+            BoxDesc& B = L.boxes[b];
+            std::vector<uint32_t> fl(B.nptg);
+            std::vector<double> vf(B.nptg), ax(B.nptg), ay(B.nptg), az(B.nptg);
+            int rc = ocfd_eb_geometry(spec, g, B.lo, B.hi, NG, fl.data(), vf.data(), ax.data(), ay.data(), az.data(),
+                                      nullptr, nullptr, nullptr, nullptr);
+            if (rc) { set_error("ocfd_level_create_eb: bad embedded-boundary spec"); delete lev; return rc; }
+            bool allreg = true, allcov = true;
+            for (long c = 0; c < B.nptg; ++c) allreg = allreg && (fl[c] & 3u) == 0u;
+            for (int k = 0; k <= B.hi[2] - B.lo[2]; ++k)
+                for (int j = 0; j <= B.hi[1] - B.lo[1]; ++j)
+                    for (int i = 0; i <= B.hi[0] - B.lo[0]; ++i)
+                        allcov = allcov && (fl[((long)(k + NG) * B.nyg + j + NG) * B.nxg + i + NG] & 3u) == 3u;
+            B.type = allcov ? 2 : (allreg ? 0 : 1);
+            L.has_eb = true;
+            L.eb_spec = *spec;
+            if (B.type == 1) {
+                B.off_e = L.mf_eb;
+                L.mf_eb += B.nptg;
+                eb_flag.insert(eb_flag.end(), fl.begin(), fl.end());
+                // geometry comps box by box: vfrac, apx, apy, apz FABs
+                eb_geo.insert(eb_geo.end(), vf.begin(), vf.end());
+                eb_geo.insert(eb_geo.end(), ax.begin(), ax.end());
+                eb_geo.insert(eb_geo.end(), ay.begin(), ay.end());
+                eb_geo.insert(eb_geo.end(), az.begin(), az.end());
+                ++L.n_cut;
+            } else if (B.type == 2) {
+                ++L.n_covered;
+            }
+        }
+    }
 
     GeomDesc& G = L.geom;
     for (int d = 0; d < 3; ++d) {
@@ -409,10 +469,27 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
 
     // work-lists
     const int xseg = tile_xseg(), ch = tile_chunk();
-    std::vector<int4> tx, ty, tz, tg, tc, tp, tpp, tv;
+    std::vector<int4> tx, ty, tz, tg, tc, tp, tpp, tv, teq, tef[3], ted;
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
         const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+        const long ngh = B.nptg - B.nptv;
+        for (long s = 0; s < ngh; s += 256) tg.push_back(make_int4(b, (int)s, (int)std::min<long>(256, ngh - s), 0));
+        for (long s = 0; s < B.nptv; s += 2048) tc.push_back(make_int4(b, (int)s, (int)std::min<long>(2048, B.nptv - s), 0));
+        for (long s = 0; s < B.nptv; s += 64) tv.push_back(make_int4(b, (int)s, (int)std::min<long>(64, B.nptv - s), 0));
+        if (B.type != 0) {
+            if (B.type == 1) {
+                const long nq = (long)(nx + 4) * (ny + 4) * (nz + 4);
+                for (long s = 0; s < nq; s += 256) teq.push_back(make_int4(b, (int)s, (int)std::min<long>(256, nq - s), 2));
+                for (int d = 0; d < 3; ++d) {
+                    const long nf = (long)(nx + (d == 0)) * (ny + (d == 1)) * (nz + (d == 2));
+                    for (long s = 0; s < nf; s += 256)
+                        tef[d].push_back(make_int4(b, (int)s, (int)std::min<long>(256, nf - s), d));
+                }
+            }
+            for (long s = 0; s < B.nptv; s += 256) ted.push_back(make_int4(b, (int)s, (int)std::min<long>(256, B.nptv - s), 0));
+            continue;
+        }
         const int nseg = (nx + xseg - 1) / xseg, nrg = (ny * nz + 3) / 4;
         for (int r = 0; r < nrg; ++r)
             for (int s = 0; s < nseg; ++s) tx.push_back(make_int4(b, s, r, 0));
@@ -426,10 +503,6 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
         for (int jg = 0; jg < (ny + 3) / 4; ++jg)
             for (int c = 0; c < (nz + ch - 1) / ch; ++c)
                 for (int s = 0; s < iseg; ++s) tz.push_back(make_int4(b, s, jg, c));
-        const long ngh = B.nptg - B.nptv;
-        for (long s = 0; s < ngh; s += 256) tg.push_back(make_int4(b, (int)s, (int)std::min<long>(256, ngh - s), 0));
-        for (long s = 0; s < B.nptv; s += 2048) tc.push_back(make_int4(b, (int)s, (int)std::min<long>(2048, B.nptv - s), 0));
-        for (long s = 0; s < B.nptv; s += 64) tv.push_back(make_int4(b, (int)s, (int)std::min<long>(64, B.nptv - s), 0));
         for (int gz : {3, 1}) {
             const long np = (long)(nx + 6) * (ny + 6) * (nz + 2 * gz);
             std::vector<int4>& tl = gz == 3 ? tp : tpp;
@@ -439,8 +512,18 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
     int rc = OCFD_OK;
     if ((rc = upload(&L.d_map, L.owner_map)) || (rc = upload(&L.d_tile_x, tx)) || (rc = upload(&L.d_tile_y, ty)) ||
         (rc = upload(&L.d_tile_z, tz)) || (rc = upload(&L.d_tile_g, tg)) || (rc = upload(&L.d_tile_c, tc)) ||
-        (rc = upload(&L.d_tile_p, tp)) || (rc = upload(&L.d_tile_pp, tpp)) || (rc = upload(&L.d_tile_v, tv))) {
+        (rc = upload(&L.d_tile_p, tp)) || (rc = upload(&L.d_tile_pp, tpp)) || (rc = upload(&L.d_tile_v, tv)) ||
+        (rc = upload(&L.d_tile_eq, teq)) || (rc = upload(&L.d_tile_ef[0], tef[0])) ||
+        (rc = upload(&L.d_tile_ef[1], tef[1])) || (rc = upload(&L.d_tile_ef[2], tef[2])) ||
+        (rc = upload(&L.d_tile_ed, ted)) || (rc = upload(&L.d_ebflag, eb_flag)) || (rc = upload(&L.d_ebgeo, eb_geo))) {
         free_level(L); delete lev; return rc;
+    }
+    L.ntile_eq = (int)teq.size(); L.ntile_ed = (int)ted.size();
+    for (int d = 0; d < 3; ++d) L.ntile_ef[d] = (int)tef[d].size();
+    if (L.mf_eb) {
+        bool ok = hipMalloc((void**)&L.d_ebq, sizeof(double) * 7 * L.mf_eb) == hipSuccess;
+        for (int d = 0; d < 3 && ok; ++d) ok = hipMalloc((void**)&L.d_ebF[d], sizeof(double) * NC * L.mf_eb) == hipSuccess;
+        if (!ok) { set_error("ocfd_level_create_eb: out of device memory"); free_level(L); delete lev; return OCFD_ENOMEM; }
     }
     L.ntile_x = (int)tx.size(); L.ntile_y = (int)ty.size(); L.ntile_z = (int)tz.size();
     L.ntile_g = (int)tg.size(); L.ntile_c = (int)tc.size();
@@ -460,7 +543,9 @@ int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g
                   (long)sizeof(int4) * (L.ntile_x + L.ntile_y + L.ntile_z + L.ntile_g + L.ntile_c + L.ntile_p +
                                         L.ntile_pp + L.ntile_v) +
                   (long)sizeof(GeomDesc) + (long)sizeof(BoxDesc) * L.nbox + (long)sizeof(double) * L.mf_valid +
-                  (long)(sizeof(float) + sizeof(double)) * L.mf_prim + (long)sizeof(unsigned long long);
+                  (long)(sizeof(float) + sizeof(double)) * L.mf_prim + (long)sizeof(unsigned long long) +
+                  (long)sizeof(int4) * (L.ntile_eq + L.ntile_ef[0] + L.ntile_ef[1] + L.ntile_ef[2] + L.ntile_ed) +
+                  L.mf_eb * (long)(sizeof(uint32_t) + sizeof(double) * (4 + 7 + 3 * NC));
     *out = lev;
     return OCFD_OK;
 }
@@ -504,7 +589,7 @@ long ocfd_level_fab_offset(ocfd_level lev, int box, int ngrow, int ncomp)
 
 static int valid_flags(int flags)
 {
-    return (flags & ~(OCFD_VISC | OCFD_GRAVITY | OCFD_FLUX_FP32 | OCFD_PLANAR)) == 0;
+    return (flags & ~(OCFD_VISC | OCFD_GRAVITY | OCFD_FLUX_FP32 | OCFD_PLANAR | OCFD_EB_REDIST)) == 0;
 }
 
 int ocfd_compute_dsdt(ocfd_level lev, const double* S, double* dSdt, double dt, const ocfd_parm* parm, int flags,

@@ -24,7 +24,7 @@ PLANAR = 8
 # physical BC codes (Exec/*/inputs)
 INTERIOR, INFLOW, OUTFLOW, SYMMETRY, SLIPWALL, NOSLIPWALL = range(6)
 
-CASE_SOD, CASE_DMR, CASE_KH3D = 0, 1, 2
+CASE_SOD, CASE_DMR, CASE_KH3D, CASE_EB_INFLOW = 0, 1, 2, 3
 
 
 class Parm(ctypes.Structure):
@@ -42,6 +42,44 @@ class Geom(ctypes.Structure):
         ("probtype", ctypes.c_int),
         ("left_state", ctypes.c_double * 5), ("right_state", ctypes.c_double * 5),
     ]
+
+
+EB_NONE, EB_CYLINDER, EB_BOX, EB_PLANE = range(4)
+EB_REDIST = 16
+
+
+class EbSpec(ctypes.Structure):
+    """struct ocfd_eb_spec: a z-extruded body for the cut-cell geometry."""
+    _fields_ = [("kind", ctypes.c_int), ("fluid_inside", ctypes.c_int), ("center", ctypes.c_double * 3),
+                ("radius", ctypes.c_double), ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3),
+                ("point", ctypes.c_double * 3), ("normal", ctypes.c_double * 3), ("small_volfrac", ctypes.c_double)]
+
+
+def eb_cylinder(cx, cy, r, small_volfrac=0.0):
+    s = EbSpec()
+    s.kind = EB_CYLINDER
+    s.center[0], s.center[1] = cx, cy
+    s.radius = r
+    s.small_volfrac = small_volfrac
+    return s
+
+
+def eb_box(lo, hi, small_volfrac=0.0):
+    s = EbSpec()
+    s.kind = EB_BOX
+    for d in range(2):
+        s.lo[d], s.hi[d] = lo[d], hi[d]
+    s.small_volfrac = small_volfrac
+    return s
+
+
+def eb_plane(point, normal, small_volfrac=0.0):
+    s = EbSpec()
+    s.kind = EB_PLANE
+    for d in range(2):
+        s.point[d], s.normal[d] = point[d], normal[d]
+    s.small_volfrac = small_volfrac
+    return s
 
 
 class StepCtl(ctypes.Structure):
@@ -71,6 +109,12 @@ def lib():
         L.ocfd_level_fab_offset.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.ocfd_level_fab_offset.restype = ctypes.c_long
         L.ocfd_level_nbox.argtypes = [vp]
+        L.ocfd_level_create_eb.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(Geom), ctypes.POINTER(EbSpec),
+                                           ctypes.POINTER(vp)]
+        L.ocfd_level_box_types.argtypes = [vp, ip]
+        L.ocfd_eb_geometry.argtypes = [ctypes.POINTER(EbSpec), ctypes.POINTER(Geom), ip, ip, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_uint32), dp, dp, dp, dp, dp, dp, dp, dp]
+        L.ocfd_inflow_state.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
         L.ocfd_level_device_bytes.argtypes = [vp]
         L.ocfd_level_device_bytes.restype = ctypes.c_long
         L.ocfd_compute_dsdt.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(Parm), ctypes.c_int, vp]
@@ -141,6 +185,31 @@ def make_geom(n_cell, prob_lo, prob_hi, periodic=(0, 0, 0), lo_bc=(2, 2, 2), hi_
     return g
 
 
+def eb_geometry(spec, geom, lo, hi, ng=NGROW):
+    """Cut-cell fields of box [lo,hi] grown by ng (host computation, no GPU):
+    dict of numpy arrays shaped (nz, ny, nx) (fc*: (2, nz, ny, nx), bcent: (3, ...))."""
+    shape = tuple(hi[d] - lo[d] + 1 + 2 * ng for d in (2, 1, 0))
+    out = {"flag": np.zeros(shape, dtype=np.uint32)}
+    for k in ("vfrac", "apx", "apy", "apz"):
+        out[k] = np.zeros(shape)
+    for k in ("fcx", "fcy", "fcz"):
+        out[k] = np.zeros((2,) + shape)
+    out["bcent"] = np.zeros((3,) + shape)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    check(lib().ocfd_eb_geometry(ctypes.byref(spec), ctypes.byref(geom), (ctypes.c_int * 3)(*lo),
+                                 (ctypes.c_int * 3)(*hi), ng,
+                                 out["flag"].ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                 *(dp(out[k]) for k in ("vfrac", "apx", "apy", "apz", "fcx", "fcy", "fcz", "bcent"))),
+          "ocfd_eb_geometry")
+    return out
+
+
+def inflow_state(p=-1.0, rho=-1.0, u=-1.0):
+    st = (ctypes.c_double * 5)()
+    check(lib().ocfd_inflow_state(p, rho, u, st), "ocfd_inflow_state")
+    return list(st)
+
+
 def dmr_states():
     left = (ctypes.c_double * 5)()
     right = (ctypes.c_double * 5)()
@@ -165,15 +234,23 @@ class Level:
     """One AMR level's box list + geometry (drop-in for the BoxArray/Geometry
     an EBR level advances, Source/EBR.cpp:43-65)."""
 
-    def __init__(self, boxes, geom):
+    def __init__(self, boxes, geom, eb=None):
         self.boxes = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
         self.geom = geom
+        self.eb = eb
         n = len(self.boxes)
         lo = (ctypes.c_int * (3 * n))(*[v for b in self.boxes for v in b[0]])
         hi = (ctypes.c_int * (3 * n))(*[v for b in self.boxes for v in b[1]])
         h = ctypes.c_void_p()
-        check(lib().ocfd_level_create(n, lo, hi, ctypes.byref(geom), ctypes.byref(h)), "ocfd_level_create")
+        check(lib().ocfd_level_create_eb(n, lo, hi, ctypes.byref(geom), ctypes.byref(eb) if eb is not None else None,
+                                         ctypes.byref(h)), "ocfd_level_create_eb")
         self.handle = h
+
+    def box_types(self):
+        """Per box: 0 regular, 1 cut (embedded-boundary path), 2 covered."""
+        t = (ctypes.c_int * self.nbox)()
+        check(lib().ocfd_level_box_types(self.handle, t), "ocfd_level_box_types")
+        return list(t)
 
     def __del__(self):
         h = getattr(self, "handle", None)

@@ -2,8 +2,8 @@
 of the reference, restated as level set-ups)."""
 import numpy as np
 
-from . import (Geom, Level, make_geom, dmr_states, INTERIOR, INFLOW, OUTFLOW, SLIPWALL,
-               CASE_SOD, CASE_DMR, CASE_KH3D)
+from . import (Geom, Level, make_geom, dmr_states, eb_cylinder, inflow_state, INTERIOR, INFLOW, OUTFLOW, SLIPWALL,
+               CASE_SOD, CASE_DMR, CASE_KH3D, CASE_EB_INFLOW)
 
 
 def chop(n_cell, max_grid):
@@ -77,3 +77,20 @@ def kh3d(n=256, max_grid=None):
                   lo_bc=(INTERIOR,) * 3, hi_bc=(INTERIOR,) * 3)
     boxes = chop(nc, max_grid or nc)
     return boxes, g, CASE_KH3D
+
+
+def eb_channel(nx=512, ny=128, nz=2, max_grid=32, radius=0.1, center=(0.5, 0.25), small_volfrac=0.05):
+    """Supersonic channel past a cylinder (diameter 0.2 in a 2 x 0.5 channel):
+    Mach 3 inflow at x-lo (the inflow state of Exec/3_FSTEP), outflow at x-hi,
+    slip walls in y, periodic in z.  Cells under small_volfrac are covered
+    (eb2.small_volfrac of Exec/Corner/inputs); run it with OCFD_EB_REDIST, the
+    small-cell scaling that keeps the explicit cut-cell update stable.
+    Returns (boxes, geom, kind, eb spec)."""
+    n = (nx, ny, nz)
+    dx = 2.0 / nx
+    st = inflow_state()
+    g = make_geom(n, (0.0, 0.0, 0.0), (2.0, 0.5 * ny / (nx / 4), nz * dx), periodic=(0, 0, 1),
+                  lo_bc=(INFLOW, SLIPWALL, INTERIOR), hi_bc=(OUTFLOW, SLIPWALL, INTERIOR), probtype=2,
+                  left_state=st, right_state=st)
+    boxes = chop(n, (max_grid, max_grid, nz))
+    return boxes, g, CASE_EB_INFLOW, eb_cylinder(center[0], center[1], radius, small_volfrac)

@@ -37,6 +37,18 @@ typedef struct orc_fab {
     int ncomp;
 } orc_fab;
 
+// Embedded-boundary fields of one box over an index range [lo, hi] (at least
+// grow(bx,4)): EBCellFlag words, volume fraction, low-face area fractions.
+typedef struct orc_ebfab {
+    const uint32_t* flag;
+    const double* vfrac;
+    const double* apx;
+    const double* apy;
+    const double* apz;
+    int lo[3];
+    int hi[3];
+} orc_ebfab;
+
 // Source/Constants.H:9-29 defaults + Parm::Initialize, Source/Constants.cpp:7-11
 void orc_parm_default(orc_parm* p)
 {
@@ -411,6 +423,320 @@ void compute_dsdt_box(const Box& bx, const View& s, const View& dsdt, const doub
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Embedded-boundary box path, Source/EB/AdvanceBox.cpp:15-228 with
+// Source/EB/EBkernels.H and Source/EB/EBdiffusion.H
+// ---------------------------------------------------------------------------
+struct EBV {
+    const orc_ebfab* e;
+    long nx, ny;
+    explicit EBV(const orc_ebfab* f) : e(f), nx(f->hi[0] - f->lo[0] + 1), ny(f->hi[1] - f->lo[1] + 1) {}
+    long at(int i, int j, int k) const { return ((long)(k - e->lo[2]) * ny + (j - e->lo[1])) * nx + (i - e->lo[0]); }
+    uint32_t flag(int i, int j, int k) const { return e->flag[at(i, j, k)]; }
+    double vf(int i, int j, int k) const { return e->vfrac[at(i, j, k)]; }
+    double ax(int i, int j, int k) const { return e->apx[at(i, j, k)]; }
+    double ay(int i, int j, int k) const { return e->apy[at(i, j, k)]; }
+    double az(int i, int j, int k) const { return e->apz[at(i, j, k)]; }
+};
+
+// amrex::EBCellFlag (bit layout of AMReX_EBCellFlag.H): type in bits 0-1
+// (0 regular, 1 single-valued, 3 covered), neighbour (i,j,k) connected at bit
+// 5 + (i+1) + 3(j+1) + 9(k+1).  The reference also asks for offsets of 2
+// (EBkernels.H:20,40,60); the same bit formula is applied to them, which is
+// what the flag word answers for such an offset.
+inline bool eb_conn(uint32_t f, int i, int j, int k) { return (f >> (5 + (i + 1) + 3 * (j + 1) + 9 * (k + 1))) & 1u; }
+inline bool eb_covered(uint32_t f) { return (f & 3u) == 3u; }
+inline bool eb_regular(uint32_t f) { return (f & 3u) == 0u; }
+
+// Source/kernel/Reconstruction.H:9-19
+inline double minmod(double a, double b)
+{
+    if (a * b > 0) return std::fabs(a) > std::fabs(b) ? b : a;
+    return 0.0;
+}
+
+// eb_recon_{x,y,z}, Source/EB/EBkernels.H:10-71: left/right face states from
+// the neighbour cells, with half minmod slopes where the cells connect
+inline void eb_recon(int d, int i, int j, int k, int n, const View& q, const EBV& g, double& ql, double& qr)
+{
+    const int di = d == 0, dj = d == 1, dk = d == 2;
+    const uint32_t f = g.flag(i, j, k);
+    ql = q(i - di, j - dj, k - dk, n);
+    qr = q(i, j, k, n);
+    if (eb_conn(f, -di, -dj, -dk)) {
+        if (eb_conn(f, -2 * di, -2 * dj, -2 * dk))
+            ql = q(i - di, j - dj, k - dk, n) + 0.5 * minmod(q(i, j, k, n) - q(i - di, j - dj, k - dk, n),
+                                                              q(i - di, j - dj, k - dk, n) - q(i - 2 * di, j - 2 * dj, k - 2 * dk, n));
+        if (eb_conn(f, di, dj, dk))
+            qr = q(i, j, k, n) - 0.5 * minmod(q(i + di, j + dj, k + dk, n) - q(i, j, k, n),
+                                             q(i, j, k, n) - q(i - di, j - dj, k - dk, n));
+    }
+}
+
+// Van Leer flux vector splitting, Source/EB/EBkernels.H:73-171 (non-CHEM)
+inline void eb_flux_point(int cdir, const double* ql, const double* qr, double* F, const orc_parm& parm)
+{
+    double dl = ql[QRHO], ul = ql[QU], vl = ql[QV], wl = ql[QW], pl = std::max(ql[QPRES], parm.smallp);
+    double dr = qr[QRHO], ur = qr[QU], vr = qr[QV], wr = qr[QW], pr = std::max(qr[QPRES], parm.smallp);
+    double gamma = parm.eos_gamma;
+    double cL = ql[QC], cR = qr[QC];
+    double fp[NCONS] = {0, 0, 0, 0, 0}, fm[NCONS] = {0, 0, 0, 0, 0};
+    double ML, MR, UL, UR;
+    int nx = 0, ny = 0, nz = 0;
+    if (cdir == 0) { ML = ul / cL; MR = ur / cR; UL = ul; UR = ur; nx = 1; }
+    else if (cdir == 1) { ML = vl / cL; MR = vr / cR; UL = vl; UR = vr; ny = 1; }
+    else { ML = wl / cL; MR = wr / cR; UL = wl; UR = wr; nz = 1; }
+    if (ML >= 1.0) {
+        fp[0] = dl * UL;
+        fp[1] = dl * UL * ul + nx * pl;
+        fp[2] = dl * UL * vl + ny * pl;
+        fp[3] = dl * UL * wl + nz * pl;
+        fp[4] = UL * (gamma * pl / (gamma - 1.0) + 0.5 * dl * (ul * ul + vl * vl + wl * wl));
+    } else if (std::fabs(ML) < 1.0) {
+        double Mp = 0.250 * (1.0 + ML) * (1.0 + ML);
+        double tmp0 = dl * cL * Mp;
+        fp[0] = tmp0;
+        fp[1] = tmp0 * (nx * (-UL + 2.0 * cL) / gamma + ul);
+        fp[2] = tmp0 * (ny * (-UL + 2.0 * cL) / gamma + vl);
+        fp[3] = tmp0 * (nz * (-UL + 2.0 * cL) / gamma + wl);
+        fp[4] = tmp0 * (((gamma - 1) * UL + 2 * cL) * ((gamma - 1) * UL + 2 * cL) / (2 * (gamma * gamma - 1))
+                        + (ul * ul + vl * vl + wl * wl - UL * UL) / 2);
+    }
+    if (std::fabs(MR) < 1.0) {
+        double Mm = -0.250 * (MR - 1.0) * (MR - 1.0);
+        double tmp0 = dr * cR * Mm;
+        fm[0] = tmp0;
+        fm[1] = tmp0 * (nx * (-UR - 2.0 * cR) / gamma + ur);
+        fm[2] = tmp0 * (ny * (-UR - 2.0 * cR) / gamma + vr);
+        fm[3] = tmp0 * (nz * (-UR - 2.0 * cR) / gamma + wr);
+        fm[4] = tmp0 * (((gamma - 1) * UR - 2 * cR) * ((gamma - 1) * UR - 2 * cR) / (2 * (gamma * gamma - 1))
+                        + (ur * ur + vr * vr + wr * wr - UR * UR) / 2);
+    } else if (MR <= -1.0) {
+        fm[0] = dr * UR;
+        fm[1] = dr * UR * ur + nx * pr;
+        fm[2] = dr * UR * vr + ny * pr;
+        fm[3] = dr * UR * wr + nz * pr;
+        fm[4] = UR * (gamma * pr / (gamma - 1.0) + 0.5 * dr * (ur * ur + vr * vr + wr * wr));
+    }
+    for (int n = 0; n < NCONS; ++n) F[n] = fp[n] + fm[n];
+}
+
+// eb_compute_visc_{x,y,z}, Source/EB/EBdiffusion.H:8-203: as the regular
+// viscous flux, with tangential differences one-sided where a cell does not
+// connect, weighted by {0, 1, 0.5}[hi - lo] (AdvanceBox.cpp:40-43).  Written
+// for direction d with (t1, t2) the two tangential directions in x,y,z order.
+inline void eb_visc(int d, int i, int j, int k, const View& q, double* F, const EBV& g, const double* dxinv,
+                    const orc_parm& parm)
+{
+    static const double weights[3] = {0.0, 1.0, 0.5};
+    int e[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    const int* ed = e[d];
+    const int im = i - ed[0], jm = j - ed[1], km = k - ed[2];
+    double Tl = q(i, j, k, QT);
+    double Tr = q(im, jm, km, QT);
+    double mutl = parm.C_s * Tl * std::sqrt(Tl) / (Tl + parm.T_s);
+    double mutr = parm.C_s * Tr * std::sqrt(Tr) / (Tr + parm.T_s);
+    double muf = 0.5 * (mutl + mutr);
+    double ktl = mutl * parm.kOverMu;
+    double ktr = mutr * parm.kOverMu;
+    double dTn = (Tl - Tr) * dxinv[d];
+    double dn[3];   // normal derivatives of u, v, w
+    for (int c = 0; c < 3; ++c) dn[c] = (q(i, j, k, QU + c) - q(im, jm, km, QU + c)) * dxinv[d];
+    // tangential derivative of velocity component c along direction t
+    auto tang = [&](int t, int c1, int c2, double& o1, double& o2) {
+        const int* et = e[t];
+        const uint32_t fh = g.flag(i, j, k), fl = g.flag(im, jm, km);
+        int hp = eb_conn(fh, et[0], et[1], et[2]) ? 1 : 0;
+        int hm = eb_conn(fh, -et[0], -et[1], -et[2]) ? 1 : 0;
+        int lp = eb_conn(fl, et[0], et[1], et[2]) ? 1 : 0;
+        int lm = eb_conn(fl, -et[0], -et[1], -et[2]) ? 1 : 0;
+        double whi = weights[hp + hm];
+        double wlo = weights[lp + lm];
+        auto Q = [&](int ii, int jj, int kk, int sgn, int comp) {
+            return q(ii + sgn * et[0], jj + sgn * et[1], kk + sgn * et[2], comp);
+        };
+        o1 = (0.5 * dxinv[t]) * ((Q(i, j, k, hp, c1) - Q(i, j, k, -hm, c1)) * whi
+                                 + (Q(im, jm, km, lp, c1) - Q(im, jm, km, -lm, c1)) * wlo);
+        o2 = (0.5 * dxinv[t]) * ((Q(i, j, k, hp, c2) - Q(i, j, k, -hm, c2)) * whi
+                                 + (Q(im, jm, km, lp, c2) - Q(im, jm, km, -lm, c2)) * wlo);
+    };
+    double t1, t2, t3;   // the three stress components of this face, reference order
+    if (d == 0) {
+        double dudy, dvdy, dudz, dwdz;
+        tang(1, QU, QV, dudy, dvdy);
+        tang(2, QU, QW, dudz, dwdz);
+        double divu = dn[0] + dvdy + dwdz;
+        t1 = muf * (2.0 * dn[0] - parm.twoThirds * divu);   // tauxx
+        t2 = muf * (dudy + dn[1]);                           // tauxy
+        t3 = muf * (dudz + dn[2]);                           // tauxz
+    } else if (d == 1) {
+        double dudx, dvdx, dvdz, dwdz;
+        tang(0, QU, QV, dudx, dvdx);
+        tang(2, QV, QW, dvdz, dwdz);
+        double divu = dudx + dn[1] + dwdz;
+        double tauyy = muf * (2.0 * dn[1] - parm.twoThirds * divu);
+        double tauxy = muf * (dn[0] + dvdx);
+        double tauyz = muf * (dn[2] + dvdz);
+        t1 = tauxy; t2 = tauyy; t3 = tauyz;
+    } else {
+        double dudx, dwdx, dvdy, dwdy;
+        tang(0, QU, QW, dudx, dwdx);
+        tang(1, QV, QW, dvdy, dwdy);
+        double divu = dudx + dvdy + dn[2];
+        double tauxz = muf * (dn[0] + dwdx);
+        double tauyz = muf * (dn[1] + dwdy);
+        double tauzz = muf * (2.0 * dn[2] - parm.twoThirds * divu);
+        t1 = tauxz; t2 = tauyz; t3 = tauzz;
+    }
+    F[UMX] += -t1;
+    F[UMY] += -t2;
+    F[UMZ] += -t3;
+    F[UEDEN] += -0.5 * ((q(i, j, k, QU) + q(im, jm, km, QU)) * t1 + (q(i, j, k, QV) + q(im, jm, km, QV)) * t2
+                        + (q(i, j, k, QW) + q(im, jm, km, QW)) * t3 + (ktl + ktr) * dTn);
+}
+
+// eb_wallflux, Source/EB/EBkernels.H:173-214
+inline void eb_wallflux(const double* qw, double axm, double axp, double aym, double ayp, double azm, double azp,
+                        double* fluxw, const orc_parm& parm)
+{
+    double fp[NCONS], fm[NCONS];
+    double d = qw[QRHO], u = qw[QU], v = qw[QV], w = qw[QW], p = qw[QPRES], c = qw[QC];
+    double apnorm = std::sqrt((axm - axp) * (axm - axp) + (aym - ayp) * (aym - ayp) + (azm - azp) * (azm - azp));
+    double un = u * (axm - axp) + v * (aym - ayp) + w * (azm - azp);
+    double lambda0 = std::fabs(un) + c * apnorm;
+    double gamma = parm.eos_gamma;
+    double E0 = p / (gamma - 1) + 0.50 * d * (u * u + v * v + w * w);
+    fp[0] = 0.5 * (d * un + lambda0 * d);
+    fp[1] = 0.5 * (d * u * un + (axm - axp) * p + lambda0 * d * u);
+    fp[2] = 0.5 * (d * v * un + (aym - ayp) * p + lambda0 * d * v);
+    fp[3] = 0.5 * (d * w * un + (azm - azp) * p + lambda0 * d * w);
+    fp[4] = 0.5 * ((E0 + p) * un + lambda0 * E0);
+    fm[0] = 0.5 * (d * (-un) - lambda0 * d);
+    fm[1] = 0.5 * (d * u * un + (axm - axp) * p + lambda0 * d * u);
+    fm[2] = 0.5 * (d * v * un + (aym - ayp) * p + lambda0 * d * v);
+    fm[3] = 0.5 * (d * w * un + (azm - azp) * p + lambda0 * d * w);
+    fm[4] = 0.5 * (-(E0 + p) * un - lambda0 * E0);
+    for (int n = 0; n < NCONS; ++n) fluxw[n] = fp[n] + fm[n];
+}
+
+// FabType of a region: 0 regular (all regular), 2 covered (all covered), else 1
+inline int eb_region_type(const EBV& g, const Box& b)
+{
+    bool allreg = true, allcov = true;
+    for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+        for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+            for (int i = b.lo[0]; i <= b.hi[0]; ++i) {
+                const uint32_t f = g.flag(i, j, k);
+                allreg = allreg && eb_regular(f);
+                allcov = allcov && eb_covered(f);
+            }
+    return allcov ? 2 : (allreg ? 0 : 1);
+}
+
+// Box kind of EBR::compute_dSdt (Source/Advance.cpp:213,231): 2 covered (bx all
+// covered), 0 regular (grow(bx,4) all regular), 1 the EB path.
+inline int eb_box_kind(const EBV& g, const Box& bx)
+{
+    if (eb_region_type(g, bx) == 2) return 2;
+    return eb_region_type(g, grow(bx, NUM_GROW)) == 0 ? 0 : 1;
+}
+
+// EBR::eb_compute_dSdt_box, Source/EB/AdvanceBox.cpp:15-228
+void eb_compute_dsdt_box(const Box& bx, const View& s, const View& dsdt, const double* dxinv, const EBV& g,
+                         int do_visc, int do_gravity, int do_redist, const orc_parm& parm)
+{
+    const Box bxg = grow(bx, NUM_GROW);
+    std::vector<double> qbuf((size_t)NPRIM * (bxg.hi[0]-bxg.lo[0]+1) * (bxg.hi[1]-bxg.lo[1]+1) * (bxg.hi[2]-bxg.lo[2]+1));
+    View q(qbuf.data(), bxg.lo, bxg.hi, NPRIM);
+    for_box(bxg, [&](int i, int j, int k) { c2prim(i, j, k, s, q, parm); });
+
+    Box fb[3];
+    std::vector<double> fbuf[3];
+    View fl[3];
+    for (int d = 0; d < 3; ++d) {
+        fb[d] = surrounding_nodes(bx, d);
+        fbuf[d].assign((size_t)NCONS * (fb[d].hi[0]-fb[d].lo[0]+1) * (fb[d].hi[1]-fb[d].lo[1]+1) * (fb[d].hi[2]-fb[d].lo[2]+1), 0.0);
+        fl[d] = View(fbuf[d].data(), fb[d].lo, fb[d].hi, NCONS);
+        const View F = fl[d];
+        for_box(fb[d], [&](int i, int j, int k) {
+            double ql[NPRIM], qr[NPRIM], f[NCONS];
+            for (int n = 0; n < NPRIM; ++n) eb_recon(d, i, j, k, n, q, g, ql[n], qr[n]);
+            eb_flux_point(d, ql, qr, f, parm);
+            for (int n = 0; n < NCONS; ++n) F(i, j, k, n) = f[n];
+        });
+        if (do_visc) {
+            for_box(fb[d], [&](int i, int j, int k) {
+                double f[NCONS];
+                for (int n = 0; n < NCONS; ++n) f[n] = F(i, j, k, n);
+                eb_visc(d, i, j, k, q, f, g, dxinv, parm);
+                for (int n = 0; n < NCONS; ++n) F(i, j, k, n) = f[n];
+            });
+        }
+    }
+    // eb_compute_div(_visc), EBkernels.H:216-338, then flux_redist (:341-377)
+    // or a copy (AdvanceBox.cpp:192-214)
+    for_box(bx, [&](int i, int j, int k) {
+        const uint32_t f = g.flag(i, j, k);
+        double L[NCONS];
+        double tmp = dxinv[0] / g.vf(i, j, k);
+        if (eb_covered(f)) {
+            for (int n = 0; n < NCONS; ++n) L[n] = 0.0;
+        } else if (eb_regular(f)) {
+            for (int n = 0; n < NCONS; ++n)
+                L[n] = dxinv[0] * (fl[0](i + 1, j, k, n) - fl[0](i, j, k, n)) +
+                       dxinv[1] * (fl[1](i, j + 1, k, n) - fl[1](i, j, k, n)) +
+                       dxinv[2] * (fl[2](i, j, k + 1, n) - fl[2](i, j, k, n));
+        } else {
+            for (int n = 0; n < NCONS; ++n)
+                L[n] = tmp * (g.ax(i + 1, j, k) * fl[0](i + 1, j, k, n) - g.ax(i, j, k) * fl[0](i, j, k, n) +
+                              g.ay(i, j + 1, k) * fl[1](i, j + 1, k, n) - g.ay(i, j, k) * fl[1](i, j, k, n) +
+                              g.az(i, j, k + 1) * fl[2](i, j, k + 1, n) - g.az(i, j, k) * fl[2](i, j, k, n));
+            double fw[NCONS];
+            if (do_visc) {
+                double qw[NPRIM];
+                for (int n = 0; n < NPRIM; ++n) qw[n] = q(i, j, k, n);
+                eb_wallflux(qw, g.ax(i, j, k), g.ax(i + 1, j, k), g.ay(i, j, k), g.ay(i, j + 1, k), g.az(i, j, k),
+                            g.az(i, j, k + 1), fw, parm);
+            } else {
+                fw[URHO] = 0.0;
+                fw[UMX] = (g.ax(i, j, k) - g.ax(i + 1, j, k)) * q(i, j, k, QPRES);
+                fw[UMY] = (g.ay(i, j, k) - g.ay(i, j + 1, k)) * q(i, j, k, QPRES);
+                fw[UMZ] = (g.az(i, j, k) - g.az(i, j, k + 1)) * q(i, j, k, QPRES);
+                fw[UEDEN] = 0.0;
+            }
+            for (int n = 0; n < NCONS; ++n) L[n] += fw[n] * tmp;
+        }
+        for (int n = 0; n < NCONS; ++n) L[n] *= -1.0;
+        const double vf = g.vf(i, j, k);
+        const bool scale = do_redist && vf < 0.5 && vf > 0.0;
+        for (int n = 0; n < NCONS; ++n) dsdt(i, j, k, n) = scale ? vf * L[n] : L[n];
+    });
+    if (do_gravity) {
+        const double gr = -9.8;
+        for_box(bx, [&](int i, int j, int k) {
+            dsdt(i, j, k, UMZ) += gr * s(i, j, k, URHO);
+            dsdt(i, j, k, UEDEN) += gr * s(i, j, k, UMZ);
+        });
+    }
+}
+
+// Per-box dispatch of EBR::compute_dSdt (Source/Advance.cpp:207-231)
+void compute_dsdt_any(const Box& bx, const View& s, const View& dsdt, const double* dxinv, const orc_ebfab* eb,
+                      int do_visc, int do_gravity, int do_redist, const orc_parm& parm)
+{
+    if (!eb) { compute_dsdt_box(bx, s, dsdt, dxinv, do_visc, do_gravity, parm, nullptr); return; }
+    EBV g(eb);
+    const int kind = eb_box_kind(g, bx);
+    if (kind == 2) for_box(bx, [&](int i, int j, int k) { for (int n = 0; n < NCONS; ++n) dsdt(i, j, k, n) = 0.0; });
+    else if (kind == 0) compute_dsdt_box(bx, s, dsdt, dxinv, do_visc, do_gravity, parm, nullptr);
+    else eb_compute_dsdt_box(bx, s, dsdt, dxinv, g, do_visc, do_gravity, do_redist, parm);
+}
+
+// EB fields of the level the box-level drivers below use (per box, or empty)
+std::vector<orc_ebfab> g_eb;
+int g_eb_redist = 0;
+
 // ebr_estdt, Source/Advance.cpp:10-52 (non-CHEM)
 double estdt_box(const Box& bx, const View& state, const double* dx, const orc_parm& parm)
 {
@@ -494,6 +820,39 @@ void orc_compute_dsdt_box(const int* lo, const int* hi, const orc_fab* S, const 
     double* faces[3] = {fx, fy, fz};
     compute_dsdt_box(bx, View(*S), View(*dSdt), dxinv, do_visc, do_gravity, *parm,
                      (fx || fy || fz) ? faces : nullptr);
+}
+
+// EBR::compute_dSdt for one box with embedded-boundary fields (Advance.cpp:207-231
+// dispatch: covered / regular / EB path); eb covers at least grow(bx,4).
+void orc_compute_dsdt_box_eb(const int* lo, const int* hi, const orc_fab* S, const orc_fab* dSdt,
+                             const double* dxinv, const orc_ebfab* eb, int do_visc, int do_gravity, int do_redist,
+                             const orc_parm* parm)
+{
+    Box bx;
+    for (int d = 0; d < 3; ++d) { bx.lo[d] = lo[d]; bx.hi[d] = hi[d]; }
+    compute_dsdt_any(bx, View(*S), View(*dSdt), dxinv, eb, do_visc, do_gravity, do_redist, *parm);
+}
+
+// Van Leer face flux on n (left, right) state pairs; ql, qr: n x 7, F: n x 5
+void orc_eb_flux_many(int dir, long n, const double* ql, const double* qr, double* F, const orc_parm* parm)
+{
+    for (long m = 0; m < n; ++m) eb_flux_point(dir, ql + 7 * m, qr + 7 * m, F + 5 * m, *parm);
+}
+
+// Box kind as EBR::compute_dSdt decides it: 0 regular, 1 EB path, 2 covered.
+int orc_eb_box_kind(const int* lo, const int* hi, const orc_ebfab* eb)
+{
+    Box bx;
+    for (int d = 0; d < 3; ++d) { bx.lo[d] = lo[d]; bx.hi[d] = hi[d]; }
+    return eb_box_kind(EBV(eb), bx);
+}
+
+// Set (nbox > 0) or clear (nbox == 0) the EB fields the level drivers
+// (orc_flow_advance, orc_run) use for box b: eb[b].
+void orc_eb_set(int nbox, const orc_ebfab* eb, int do_redist)
+{
+    g_eb.assign(eb, eb + nbox);
+    g_eb_redist = do_redist;
 }
 
 // EBR::estTimeStep (Advance.cpp:61-79): cfl * min over boxes of ebr_estdt.
@@ -634,6 +993,13 @@ int fill_level(const Level& lev, const orc_geom& g, double time)
                         if (bc_lo0 == EXT_DIR && i < g.dom_lo[0]) {
                             for (int n = 0; n < NCONS; ++n) f(i, j, k, n) = g.left_state[n];
                         }
+                    } else if (g.probtype == 2) {
+                        // inflow at x-lo, Exec/3_FSTEP/prob_bcfill.cpp:8-33
+                        if (comp_bc(g, 0, 0, 0) == EXT_DIR && i < g.dom_lo[0]) {
+                            for (int n = 0; n < NCONS; ++n) f(i, j, k, n) = g.left_state[n];
+                        } else if (ext) {
+                            bad += 1;
+                        }
                     } else if (ext) {
                         bad += 1;
                     }
@@ -689,7 +1055,8 @@ int orc_flow_advance(int nbox, const int* lo, const int* hi, const orc_fab* A, c
     // stage: out = (c_cur*cur + c_old*old) + c_dt*L(cur)  (LinComb / Saxpy forms)
     auto stage = [&](const Level& cur, const Level* old, const Level& out, double c_cur, double c_old, double c_dt) {
         for (int b = 0; b < nbox; ++b)
-            compute_dsdt_box(cur.boxes[b], cur.fabs[b], dv[b], dxinv, do_visc, do_gravity, *parm, nullptr);
+            compute_dsdt_any(cur.boxes[b], cur.fabs[b], dv[b], dxinv, g_eb.empty() ? nullptr : &g_eb[b], do_visc,
+                             do_gravity, g_eb_redist, *parm);
         for (int b = 0; b < nbox; ++b) {
             const View& u = cur.fabs[b];
             const View& o = out.fabs[b];
@@ -806,6 +1173,46 @@ void orc_init_dmr(const int* lo, const int* hi, const orc_fab* S, const orc_geom
         double et = Pt / (parm->eos_gamma - 1.0);
         s(i, j, k, UEDEN) = et + 0.5 * rhot * (uxt * uxt + vxt * vxt);
     });
+}
+
+// Exec/3_FSTEP/prob.H:11-40 with Exec/3_FSTEP/prob_parm.H defaults; flag holds the
+// EBCellFlag words of the valid box (i fastest)
+void orc_init_eb_inflow(const int* lo, const int* hi, const orc_fab* S, const uint32_t* flag, const orc_parm* parm)
+{
+    View s(*S);
+    const double Pt = 2026.5, rhot = 0.08938, uxt = 534.49;
+    const long nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1;
+    for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i) {
+                const bool cov = (flag[((long)(k - lo[2]) * ny + (j - lo[1])) * nx + (i - lo[0])] & 3u) == 3u;
+                if (cov) {
+                    s(i, j, k, URHO) = rhot;
+                    s(i, j, k, UMX) = 0.0;
+                    s(i, j, k, UMY) = 0.0;
+                    s(i, j, k, UMZ) = 0.0;
+                    double et = Pt / (parm->eos_gamma - 1.0);
+                    s(i, j, k, UEDEN) = et + 0.5 * rhot * uxt * uxt;
+                } else {
+                    s(i, j, k, URHO) = rhot;
+                    s(i, j, k, UMX) = rhot * uxt;
+                    s(i, j, k, UMY) = 0.0;
+                    s(i, j, k, UMZ) = 0.0;
+                    double et = Pt / (parm->eos_gamma - 1.0);
+                    s(i, j, k, UEDEN) = et + 0.5 * rhot * uxt * uxt;
+                }
+            }
+}
+
+// Exec/3_FSTEP/prob.cpp:24-29 (literal 0.4)
+void orc_inflow_state(double* st)
+{
+    const double p = 2026.5, rho = 0.08938, u = 534.49;
+    st[0] = rho;
+    st[1] = u * rho;
+    st[2] = 0;
+    st[3] = 0;
+    st[4] = p / 0.4 + 0.5 * rho * u * u;
 }
 
 // DMR inflow states, Exec/1_DMR/prob.cpp:28-40 (note the literal 0.4)

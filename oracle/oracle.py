@@ -225,6 +225,110 @@ def new_level_fabs(boxes, g=NGROW):
 
 
 # ---------------------------------------------------------------------------
+# embedded boundary
+# ---------------------------------------------------------------------------
+class OEbFab(ctypes.Structure):
+    _fields_ = [("flag", ctypes.POINTER(ctypes.c_uint32)), ("vfrac", ctypes.POINTER(ctypes.c_double)),
+                ("apx", ctypes.POINTER(ctypes.c_double)), ("apy", ctypes.POINTER(ctypes.c_double)),
+                ("apz", ctypes.POINTER(ctypes.c_double)), ("lo", ctypes.c_int * 3), ("hi", ctypes.c_int * 3)]
+
+
+class EbFab:
+    """EB fields of one box over [lo, hi] (numpy arrays (nz, ny, nx)): flag
+    (uint32 EBCellFlag words), vfrac, apx, apy, apz."""
+
+    def __init__(self, lo, hi, flag, vfrac, apx, apy, apz):
+        self.lo, self.hi = tuple(lo), tuple(hi)
+        shape = tuple(hi[d] - lo[d] + 1 for d in (2, 1, 0))
+        self.flag = np.ascontiguousarray(flag, dtype=np.uint32).reshape(shape)
+        self.arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(shape) for a in (vfrac, apx, apy, apz)]
+        self.c = OEbFab()
+        self.c.flag = self.flag.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        self.c.vfrac, self.c.apx, self.c.apy, self.c.apz = (_dp(a) for a in self.arrs)
+        for d in range(3):
+            self.c.lo[d] = self.lo[d]
+            self.c.hi[d] = self.hi[d]
+
+
+def _eb_sigs():
+    L = lib()
+    if getattr(L, "_eb_ready", False):
+        return L
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.orc_compute_dsdt_box_eb.argtypes = [ip, ip, ctypes.POINTER(OFab), ctypes.POINTER(OFab), dp,
+                                          ctypes.POINTER(OEbFab), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(OParm)]
+    L.orc_eb_box_kind.argtypes = [ip, ip, ctypes.POINTER(OEbFab)]
+    L.orc_eb_set.argtypes = [ctypes.c_int, ctypes.POINTER(OEbFab), ctypes.c_int]
+    L.orc_init_eb_inflow.argtypes = [ip, ip, ctypes.POINTER(OFab), ctypes.POINTER(ctypes.c_uint32),
+                                     ctypes.POINTER(OParm)]
+    L.orc_inflow_state.argtypes = [dp]
+    L.orc_eb_flux_many.argtypes = [ctypes.c_int, ctypes.c_long, dp, dp, dp, ctypes.POINTER(OParm)]
+    L._eb_ready = True
+    return L
+
+
+def compute_dsdt_box_eb(lo, hi, S, p, dxinv, eb, do_visc=0, do_gravity=0, do_redist=0):
+    L = _eb_sigs()
+    D = Fab(lo, hi)
+    lo_c = (ctypes.c_int * 3)(*lo)
+    hi_c = (ctypes.c_int * 3)(*hi)
+    dxi = np.ascontiguousarray(dxinv, dtype=np.float64)
+    L.orc_compute_dsdt_box_eb(lo_c, hi_c, ctypes.byref(S.c), ctypes.byref(D.c), _dp(dxi), ctypes.byref(eb.c),
+                              do_visc, do_gravity, do_redist, ctypes.byref(p))
+    return D.a
+
+
+def eb_flux(d, ql, qr, p):
+    """Van Leer flux of (left, right) primitive states (n x 7 each)."""
+    L = _eb_sigs()
+    ql = np.ascontiguousarray(ql, dtype=np.float64)
+    qr = np.ascontiguousarray(qr, dtype=np.float64)
+    F = np.zeros((ql.shape[0], 5))
+    L.orc_eb_flux_many(d, ql.shape[0], _dp(ql), _dp(qr), _dp(F), ctypes.byref(p))
+    return F
+
+
+def eb_box_kind(lo, hi, eb):
+    L = _eb_sigs()
+    return L.orc_eb_box_kind((ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi), ctypes.byref(eb.c))
+
+
+_eb_keep = None
+
+
+def set_eb(ebfabs, do_redist=0):
+    """EB fields per box for flow_advance / run (None or [] clears them)."""
+    global _eb_keep
+    L = _eb_sigs()
+    if not ebfabs:
+        _eb_keep = None
+        L.orc_eb_set(0, None, 0)
+        return
+    arr = (OEbFab * len(ebfabs))()
+    for m, e in enumerate(ebfabs):
+        arr[m] = e.c
+    _eb_keep = (arr, list(ebfabs))
+    L.orc_eb_set(len(ebfabs), arr, do_redist)
+
+
+def init_eb_inflow(boxes, fabs, valid_flags, p):
+    L = _eb_sigs()
+    for (lo, hi), f, fl in zip(boxes, fabs, valid_flags):
+        fl = np.ascontiguousarray(fl, dtype=np.uint32)
+        L.orc_init_eb_inflow((ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi), ctypes.byref(f.c),
+                             fl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.byref(p))
+
+
+def inflow_state():
+    L = _eb_sigs()
+    st = np.zeros(5)
+    L.orc_inflow_state(_dp(st))
+    return st
+
+
+# ---------------------------------------------------------------------------
 # chemistry (Li-Dryer H2/air, 9 species: H2 O2 H2O H O OH HO2 H2O2 N2)
 # ---------------------------------------------------------------------------
 NSPEC = 9

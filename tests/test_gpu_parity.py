@@ -22,11 +22,16 @@ def _level(case):
 
 
 def _cases():
+    import ocfd
     from ocfd import cases
     return {
         "sod_multibox": cases.sod(64, 8, 8, max_grid=(16, 8, 8)),
         "dmr_uneven": cases.dmr(128, 32, 2, sizes=(8, 16, 32), seed=3),
         "kh_multibox": cases.kh3d(24, max_grid=(12, 12, 24)),
+        # periodic in z with 2 cells: ghost images wrap around more than once
+        "thin_periodic": (cases.chop((32, 16, 2), (16, 8, 2)),
+                          ocfd.make_geom((32, 16, 2), (0, 0, 0), (1.0, 0.5, 1.0 / 16), periodic=(1, 0, 1),
+                                         lo_bc=(0, 4, 0), hi_bc=(0, 2, 0)), ocfd.CASE_KH3D),
     }
 
 
@@ -44,7 +49,7 @@ def test_init_state_matches_oracle(torch_cuda, orc, name):
         assert np.array_equal(got[b], f.a), f"box {b}"
 
 
-@pytest.mark.parametrize("name", ["sod_multibox", "dmr_uneven", "kh_multibox"])
+@pytest.mark.parametrize("name", ["sod_multibox", "dmr_uneven", "kh_multibox", "thin_periodic"])
 def test_fill_boundary_matches_oracle(torch_cuda, orc, name):
     lev, boxes, g, kind = _level(_cases()[name])
     rng = np.random.default_rng(7)
