@@ -930,7 +930,9 @@ struct Level {
 // foextrap copies the boundary cell, reflect_even/odd mirror about the face
 // with sign +1/-1; a cell outside in several directions composes the maps),
 // then the case's ext_dir fill (Exec/1_DMR/prob_bcfill.cpp:14-58).
-int fill_level(const Level& lev, const orc_geom& g, double time)
+// species: fill a 9-component species MultiFab; every component takes the
+// scalar (density) BC, Source/Setup.cpp:241-247, and no ext_dir values exist.
+int fill_level(const Level& lev, const orc_geom& g, double time, int species = 0)
 {
     int bad = 0;
     for (int b = 0; b < lev.nbox; ++b) {
@@ -957,13 +959,14 @@ int fill_level(const Level& lev, const orc_geom& g, double time)
                         outside[d] = iv[d] < g.dom_lo[d] ? -1 : (iv[d] > g.dom_hi[d] ? 1 : 0);
                         if (outside[d]) any_out = true;
                     }
-                    for (int n = 0; n < NCONS; ++n) {
+                    const int ncomp = species ? f.nc : NCONS;
+                    for (int n = 0; n < ncomp; ++n) {
                         int src[3] = {iv[0], iv[1], iv[2]};
                         double sign = 1.0;
                         bool user = false;
                         for (int d = 0; d < 3; ++d) {
                             if (!outside[d]) continue;
-                            int bc = comp_bc(g, n, d, outside[d] > 0);
+                            int bc = comp_bc(g, species ? 0 : n, d, outside[d] > 0);
                             int edge = outside[d] < 0 ? g.dom_lo[d] : g.dom_hi[d];
                             if (bc == FOEXTRAP) src[d] = edge;
                             else if (bc == REFLECT_EVEN || bc == REFLECT_ODD) {
@@ -978,7 +981,9 @@ int fill_level(const Level& lev, const orc_geom& g, double time)
                         f(i, j, k, n) = sign * lev.fabs[m](src[0], src[1], src[2], n);
                     }
                     (void)any_out;
-                    if (g.probtype == 1) {
+                    if (species) {
+                        if (ext) bad += 1;
+                    } else if (g.probtype == 1) {
                         // Exec/1_DMR/prob_bcfill.cpp:27-57; bc of component 0 decides
                         double x = g.prob_lo[0] + (i + 0.5) * g.dx[0];
                         double x0 = 1.0/6.0 + std::sqrt(1/3.0) + time*10.0/std::sin(M_PI*(1.0/3.0));
@@ -1273,6 +1278,377 @@ void orc_init_kh3d(const int* lo, const int* hi, const orc_fab* S, const orc_geo
         s(i, j, k, UMZ) = rho * w;
         s(i, j, k, UEDEN) = p0 / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
     });
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Multispecies (CHEM) flow path: EBR::compute_dSdt_multi on regular boxes
+// (Source/CHEM/FlowAdvance.cpp:92-496), EBR::flow_advance_multi (:9-90) and
+// the Strang-split EBR::advance (Source/Advance.cpp:152-177).
+// ---------------------------------------------------------------------------
+extern "C" {
+double orc_T_given_ey(double e, const double* rhoi, int* iters);
+double orc_CKPY(const double* rhoi, double T);
+void orc_CKYTX(const double* y, double* x);
+void orc_species_enthalpy(double T, double* h);
+void orc_mixture_properties(double T, const double* X, double* mu, double* lambda);
+void orc_mix_diff_coeffs_mass(double T, double p, const double* X, double* D);
+void orc_chem_step_many(long n, double* u, double* rhoi, double dt, double smallr);
+void orc_renorm_many(long n, const double* u, double* rhoi);
+}
+
+namespace {
+
+constexpr int NSP = 9;
+constexpr int QGAMA = 7, NPRIM_M = 8;   // IndexDefines.H:19-21 (CHEM)
+
+// c2prim_rgas, Source/kernel/Kernels.H:42-81
+inline void c2prim_rgas(int i, int j, int k, const View& u, const View& spec, const View& q, const orc_parm& parm)
+{
+    double rho = std::max(u(i, j, k, URHO), parm.smallr);
+    double rhoinv = 1.0 / rho;
+    double ux = u(i, j, k, UMX) * rhoinv;
+    double uy = u(i, j, k, UMY) * rhoinv;
+    double uz = u(i, j, k, UMZ) * rhoinv;
+    double kineng = 0.5 * (ux * ux + uy * uy + uz * uz);
+    double ei = u(i, j, k, UEDEN) - rho * kineng;
+    double rhoi[NSP];
+    q(i, j, k, QRHO) = rho;
+    q(i, j, k, QU) = ux;
+    q(i, j, k, QV) = uy;
+    q(i, j, k, QW) = uz;
+    for (int n = 0; n < NSP; ++n) rhoi[n] = spec(i, j, k, n);
+    double T = orc_T_given_ey(ei, rhoi, nullptr);
+    double p = orc_CKPY(rhoi, T);
+    q(i, j, k, QT) = T;
+    q(i, j, k, QPRES) = p;
+    double gama = p / ei + 1.0;
+    q(i, j, k, QGAMA) = gama;
+    q(i, j, k, QC) = std::sqrt(gama * p / rho);
+}
+
+// Steger-Warming split of the partial densities, FlowAdvance.cpp:214-242
+inline void sw_species(int d, const View& q, const View& spec, int i, int j, int k, double* fp, double* fm)
+{
+    double un = q(i, j, k, QU + d);
+    double c = q(i, j, k, QC);
+    double gamma = q(i, j, k, QGAMA);
+    double E1 = un, E2 = un - c, E3 = un + c;
+    double E1P = (E1 + std::fabs(E1)) * 0.5;
+    double E2P = (E2 + std::fabs(E2)) * 0.5;
+    double E3P = (E3 + std::fabs(E3)) * 0.5;
+    double E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
+    double tmp1 = 1.0 / (2.0 * gamma);
+    double tmp2 = 2.0 * (gamma - 1.0);
+    double tmp2_p = tmp2 * E1P + E2P + E3P;
+    double tmp2_m = tmp2 * E1M + E2M + E3M;
+    for (int n = 0; n < NSP; ++n) {
+        double tmp0 = spec(i, j, k, n) * tmp1;
+        fp[n] = tmp0 * tmp2_p;
+        fm[n] = tmp0 * tmp2_m;
+    }
+}
+
+// compute_dSdt_multi on one regular box (FlowAdvance.cpp:173-403)
+void compute_dsdt_multi_box(const Box& bx, const View& s, const View& spec, const View& dsdt, const View& dsdt_spec,
+                            const double* dxinv, int do_visc, const orc_parm& parm)
+{
+    const Box bxg = grow(bx, NUM_GROW);
+    const size_t ng = (size_t)(bxg.hi[0]-bxg.lo[0]+1) * (bxg.hi[1]-bxg.lo[1]+1) * (bxg.hi[2]-bxg.lo[2]+1);
+    std::vector<double> qb(NPRIM_M * ng), mub(ng), lamb(ng), Db(NSP * ng);
+    std::vector<double> fpb(NCONS * ng), fmb(NCONS * ng), fpsb(NSP * ng), fmsb(NSP * ng);
+    View q(qb.data(), bxg.lo, bxg.hi, NPRIM_M), mu(mub.data(), bxg.lo, bxg.hi, 1), lam(lamb.data(), bxg.lo, bxg.hi, 1);
+    View D(Db.data(), bxg.lo, bxg.hi, NSP);
+    View fp(fpb.data(), bxg.lo, bxg.hi, NCONS), fm(fmb.data(), bxg.lo, bxg.hi, NCONS);
+    View fps(fpsb.data(), bxg.lo, bxg.hi, NSP), fms(fmsb.data(), bxg.lo, bxg.hi, NSP);
+
+    for_box(bxg, [&](int i, int j, int k) { c2prim_rgas(i, j, k, s, spec, q, parm); });
+    if (do_visc) {
+        // FlowAdvance.cpp:188-202
+        for_box(bxg, [&](int i, int j, int k) {
+            double Yt[NSP], Xt[NSP], Dk[NSP];
+            double T = q(i, j, k, QT);
+            double p = q(i, j, k, QPRES);
+            for (int n = 0; n < NSP; ++n) Yt[n] = spec(i, j, k, n) / q(i, j, k, QRHO);
+            orc_CKYTX(Yt, Xt);
+            orc_mixture_properties(T, Xt, &mu(i, j, k, 0), &lam(i, j, k, 0));
+            orc_mix_diff_coeffs_mass(T, p, Xt, Dk);
+            for (int n = 0; n < NSP; ++n) D(i, j, k, n) = Dk[n];
+        });
+    }
+    Box fb[3];
+    std::vector<double> fbuf[3], fsbuf[3], fdbuf[3];
+    View F[3], Fs[3], Fd[3];
+    for (int d = 0; d < 3; ++d) {
+        fb[d] = surrounding_nodes(bx, d);
+        const size_t nf = (size_t)(fb[d].hi[0]-fb[d].lo[0]+1) * (fb[d].hi[1]-fb[d].lo[1]+1) * (fb[d].hi[2]-fb[d].lo[2]+1);
+        fbuf[d].assign(NCONS * nf, 0.0); fsbuf[d].assign(NSP * nf, 0.0); fdbuf[d].assign(NSP * nf, 0.0);
+        F[d] = View(fbuf[d].data(), fb[d].lo, fb[d].hi, NCONS);
+        Fs[d] = View(fsbuf[d].data(), fb[d].lo, fb[d].hi, NSP);
+        Fd[d] = View(fdbuf[d].data(), fb[d].lo, fb[d].hi, NSP);
+        const int di = d == 0, dj = d == 1, dk = d == 2;
+        for_box(bxg, [&](int i, int j, int k) {
+            double qv[NPRIM], a[NCONS], b[NCONS], as[NSP], bs[NSP];
+            for (int n = 0; n < NPRIM; ++n) qv[n] = q(i, j, k, n);
+            sw_point(d, qv, a, b, q(i, j, k, QGAMA));
+            for (int n = 0; n < NCONS; ++n) { fp(i, j, k, n) = a[n]; fm(i, j, k, n) = b[n]; }
+            sw_species(d, q, spec, i, j, k, as, bs);
+            for (int n = 0; n < NSP; ++n) { fps(i, j, k, n) = as[n]; fms(i, j, k, n) = bs[n]; }
+        });
+        auto weno = [&](const View& P, const View& M, const View& out, int ncomp) {
+            for_box(fb[d], [&](int i, int j, int k) {
+                for (int n = 0; n < ncomp; ++n) {
+                    double pv = weno_plus(P(i-3*di, j-3*dj, k-3*dk, n), P(i-2*di, j-2*dj, k-2*dk, n),
+                                          P(i-di, j-dj, k-dk, n), P(i, j, k, n), P(i+di, j+dj, k+dk, n));
+                    double mv = weno_minus(M(i-2*di, j-2*dj, k-2*dk, n), M(i-di, j-dj, k-dk, n),
+                                           M(i, j, k, n), M(i+di, j+dj, k+dk, n), M(i+2*di, j+2*dj, k+2*dk, n));
+                    out(i, j, k, n) = pv + mv;
+                }
+            });
+        };
+        weno(fp, fm, F[d], NCONS);
+        weno(fps, fms, Fs[d], NSP);
+        if (do_visc) {
+            const View Fv = F[d], Fsv = Fs[d], Fdv = Fd[d];
+            // diffusion_{x,y,z}, Source/CHEM/ChemViscous.H:108-199
+            for_box(fb[d], [&](int i, int j, int k) {
+                const int im = i - di, jm = j - dj, km = k - dk;
+                double rhor = q(i, j, k, QRHO);
+                double rhol = q(im, jm, km, QRHO);
+                double rho = 0.5 * (rhol + rhor);
+                double dn[NSP], Ytr[NSP], Ytl[NSP];
+                for (int n = 0; n < NSP; ++n) {
+                    Ytr[n] = spec(i, j, k, n) / rhor;
+                    Ytl[n] = spec(im, jm, km, n) / rhol;
+                    dn[n] = 0.5 * (D(i, j, k, n) + D(im, jm, km, n));
+                }
+                for (int n = 0; n < NSP; ++n) {
+                    double tmp = rho * dn[n] * (Ytl[n] - Ytr[n]) * dxinv[d];
+                    Fdv(i, j, k, n) = tmp;
+                    Fsv(i, j, k, n) += tmp;
+                }
+            });
+            // compute_visc_{x,y,z}_multi, Source/CHEM/ChemViscous.H:201-357
+            for_box(fb[d], [&](int i, int j, int k) {
+                const int e[3] = {di, dj, dk};
+                const int im = i - di, jm = j - dj, km = k - dk;
+                double Tr = q(i, j, k, QT);
+                double Tl = q(im, jm, km, QT);
+                double T = 0.5 * (Tl + Tr);
+                double muf = 0.5 * (mu(i, j, k, 0) + mu(im, jm, km, 0));
+                double lamf = 0.5 * (lam(i, j, k, 0) + lam(im, jm, km, 0));
+                double hi[NSP];
+                orc_species_enthalpy(T, hi);
+                double tmp = 0;
+                for (int n = 0; n < NSP; ++n) tmp += Fdv(i, j, k, n) * hi[n];
+                double dTn = (Tr - Tl) * dxinv[d];
+                double dn[3];
+                for (int c = 0; c < 3; ++c) dn[c] = (q(i, j, k, QU + c) - q(im, jm, km, QU + c)) * dxinv[d];
+                // along each tangential direction t: 4-point differences of u_d and u_t
+                double g_d[3] = {0, 0, 0}, g_t[3] = {0, 0, 0};
+                for (int t = 0; t < 3; ++t) {
+                    if (t == d) continue;
+                    const int ti = t == 0, tj = t == 1, tk = t == 2;
+                    auto four = [&](int comp) {
+                        return (q(i + ti, j + tj, k + tk, comp) + q(im + ti, jm + tj, km + tk, comp) -
+                                q(i - ti, j - tj, k - tk, comp) - q(im - ti, jm - tj, km - tk, comp)) *
+                               (0.25 * dxinv[t]);
+                    };
+                    g_d[t] = four(QU + d);
+                    g_t[t] = four(QU + t);
+                    (void)e;
+                }
+                double divu = 0;
+                {
+                    double c0 = d == 0 ? dn[0] : g_t[0];
+                    double c1 = d == 1 ? dn[1] : g_t[1];
+                    double c2 = d == 2 ? dn[2] : g_t[2];
+                    divu = c0 + c1 + c2;
+                }
+                double tau[3];
+                for (int t = 0; t < 3; ++t) {
+                    if (t == d) tau[t] = muf * (2.0 * dn[d] - parm.twoThirds * divu);
+                    else tau[t] = muf * (g_d[t] + dn[t]);
+                }
+                Fv(i, j, k, UMX) += -tau[0];
+                Fv(i, j, k, UMY) += -tau[1];
+                Fv(i, j, k, UMZ) += -tau[2];
+                Fv(i, j, k, UEDEN) += -0.5 * ((q(i, j, k, QU) + q(im, jm, km, QU)) * tau[0] +
+                                              (q(i, j, k, QV) + q(im, jm, km, QV)) * tau[1] +
+                                              (q(i, j, k, QW) + q(im, jm, km, QW)) * tau[2]) - lamf * dTn - tmp;
+            });
+        }
+    }
+    // divop for both, FlowAdvance.cpp:394-403
+    for_box(bx, [&](int i, int j, int k) {
+        for (int n = 0; n < NCONS; ++n)
+            dsdt(i, j, k, n) = dxinv[0] * (F[0](i, j, k, n) - F[0](i+1, j, k, n))
+                +              dxinv[1] * (F[1](i, j, k, n) - F[1](i, j+1, k, n))
+                +              dxinv[2] * (F[2](i, j, k, n) - F[2](i, j, k+1, n));
+        for (int n = 0; n < NSP; ++n)
+            dsdt_spec(i, j, k, n) = dxinv[0] * (Fs[0](i, j, k, n) - Fs[0](i+1, j, k, n))
+                +                   dxinv[1] * (Fs[1](i, j, k, n) - Fs[1](i, j+1, k, n))
+                +                   dxinv[2] * (Fs[2](i, j, k, n) - Fs[2](i, j, k+1, n));
+    });
+}
+
+// chemical_advance(dt) on the valid cells of every box (ChemAdvance.cpp:7-85)
+void chem_level(const Level& S, const Level& Sp, double dt, const orc_parm& parm)
+{
+    for (int b = 0; b < S.nbox; ++b) {
+        const Box& bx = S.boxes[b];
+        std::vector<double> u, r;
+        for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+            for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+                for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+                    for (int n = 0; n < NCONS; ++n) u.push_back(S.fabs[b](i, j, k, n));
+                    for (int n = 0; n < NSP; ++n) r.push_back(Sp.fabs[b](i, j, k, n));
+                }
+        orc_chem_step_many((long)(u.size() / NCONS), u.data(), r.data(), dt, parm.smallr);
+        size_t c = 0;
+        for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+            for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+                for (int i = bx.lo[0]; i <= bx.hi[0]; ++i, ++c) {
+                    S.fabs[b](i, j, k, UEDEN) = u[c * NCONS + UEDEN];
+                    for (int n = 0; n < NSP; ++n) Sp.fabs[b](i, j, k, n) = r[c * NSP + n];
+                }
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+// compute_dSdt_multi for one regular box; S, Spec cover grow(bx,4)
+void orc_compute_dsdt_multi_box(const int* lo, const int* hi, const orc_fab* S, const orc_fab* Spec,
+                                const orc_fab* dSdt, const orc_fab* dSdt_spec, const double* dxinv, int do_visc,
+                                const orc_parm* parm)
+{
+    Box bx;
+    for (int d = 0; d < 3; ++d) { bx.lo[d] = lo[d]; bx.hi[d] = hi[d]; }
+    compute_dsdt_multi_box(bx, View(*S), View(*Spec), View(*dSdt), View(*dSdt_spec), dxinv, do_visc, *parm);
+}
+
+int orc_fill_ghost_spec(int nbox, const int* lo, const int* hi, const orc_fab* fabs, const orc_geom* g, double time)
+{
+    Level L = make_level(nbox, lo, hi, fabs);
+    return fill_level(L, *g, time, 1);
+}
+
+// EBR::flow_advance_multi (FlowAdvance.cpp:9-90): scheme 0 Euler, 1 RK2.
+// A/As: state and species (U^n in, U^(n+1) out); B/Bs scratch.
+int orc_flow_advance_multi(int nbox, const int* lo, const int* hi, const orc_fab* A, const orc_fab* As,
+                           const orc_fab* B, const orc_fab* Bs, const orc_geom* g, const orc_parm* parm,
+                           double time, double dt, int scheme, int do_visc)
+{
+    Level LA = make_level(nbox, lo, hi, A), LAs = make_level(nbox, lo, hi, As);
+    Level LB = make_level(nbox, lo, hi, B), LBs = make_level(nbox, lo, hi, Bs);
+    double dxinv[3] = {1.0 / g->dx[0], 1.0 / g->dx[1], 1.0 / g->dx[2]};
+    int bad = 0;
+    std::vector<std::vector<double>> d1(nbox), d2(nbox);
+    std::vector<View> v1(nbox), v2(nbox);
+    for (int b = 0; b < nbox; ++b) {
+        const Box& bx = LA.boxes[b];
+        const size_t nv = (size_t)(bx.hi[0]-bx.lo[0]+1) * (bx.hi[1]-bx.lo[1]+1) * (bx.hi[2]-bx.lo[2]+1);
+        d1[b].assign(NCONS * nv, 0.0); d2[b].assign(NSP * nv, 0.0);
+        v1[b] = View(d1[b].data(), bx.lo, bx.hi, NCONS);
+        v2[b] = View(d2[b].data(), bx.lo, bx.hi, NSP);
+    }
+    auto stage = [&](const Level& cur, const Level& curs, const Level* old, const Level* olds, const Level& out,
+                     const Level& outs, double c_cur, double c_old, double c_dt) {
+        for (int b = 0; b < nbox; ++b)
+            compute_dsdt_multi_box(cur.boxes[b], cur.fabs[b], curs.fabs[b], v1[b], v2[b], dxinv, do_visc, *parm);
+        for (int b = 0; b < nbox; ++b) {
+            for_box(cur.boxes[b], [&](int i, int j, int k) {
+                for (int n = 0; n < NCONS; ++n) {
+                    double v = old ? c_cur * cur.fabs[b](i, j, k, n) + c_old * old->fabs[b](i, j, k, n)
+                                   : c_cur * cur.fabs[b](i, j, k, n);
+                    out.fabs[b](i, j, k, n) = v + c_dt * v1[b](i, j, k, n);
+                }
+                for (int n = 0; n < NSP; ++n) {
+                    double v = olds ? c_cur * curs.fabs[b](i, j, k, n) + c_old * olds->fabs[b](i, j, k, n)
+                                    : c_cur * curs.fabs[b](i, j, k, n);
+                    outs.fabs[b](i, j, k, n) = v + c_dt * v2[b](i, j, k, n);
+                }
+            });
+        }
+    };
+    bad += fill_level(LA, *g, time);
+    bad += fill_level(LAs, *g, time, 1);
+    stage(LA, LAs, nullptr, nullptr, LB, LBs, 1.0, 0.0, dt);
+    if (scheme == 0) {
+        for (int b = 0; b < nbox; ++b) {
+            std::memcpy(LA.fabs[b].p, LB.fabs[b].p, sizeof(double) * LA.fabs[b].npts() * NCONS);
+            std::memcpy(LAs.fabs[b].p, LBs.fabs[b].p, sizeof(double) * LAs.fabs[b].npts() * NSP);
+        }
+    } else {
+        bad += fill_level(LB, *g, time + dt);
+        bad += fill_level(LBs, *g, time + dt, 1);
+        stage(LB, LBs, &LA, &LAs, LA, LAs, 0.5, 0.5, 0.5 * dt);
+    }
+    return bad;
+}
+
+// EBR::advance with chemistry (Source/Advance.cpp:152-177): iter = 4/2^level
+// (level <= 2, else 1) chemistry steps of dt/(2 iter), the flow step, and
+// iter more chemistry steps; then the species renormalisation of
+// EBR::post_timestep (Source/EBR.cpp:333-357).
+int orc_advance_chem(int nbox, const int* lo, const int* hi, const orc_fab* A, const orc_fab* As, const orc_fab* B,
+                     const orc_fab* Bs, const orc_geom* g, const orc_parm* parm, double time, double dt, int level,
+                     int scheme, int do_visc, int renorm)
+{
+    Level LA = make_level(nbox, lo, hi, A), LAs = make_level(nbox, lo, hi, As);
+    int iter = level <= 2 ? 4 / (int)std::pow(2, level) : 1;
+    double dt1 = 0.5 * dt;
+    double dt2 = dt1 / iter;
+    for (int n = 0; n < iter; ++n) chem_level(LA, LAs, dt2, *parm);
+    int bad = orc_flow_advance_multi(nbox, lo, hi, A, As, B, Bs, g, parm, time, dt, scheme, do_visc);
+    for (int n = 0; n < iter; ++n) chem_level(LA, LAs, dt2, *parm);
+    if (renorm) {
+        for (int b = 0; b < nbox; ++b) {
+            const Box& bx = LA.boxes[b];
+            for_box(bx, [&](int i, int j, int k) {
+                double u[NCONS], r[NSP];
+                for (int n = 0; n < NCONS; ++n) u[n] = LA.fabs[b](i, j, k, n);
+                for (int n = 0; n < NSP; ++n) r[n] = LAs.fabs[b](i, j, k, n);
+                orc_renorm_many(1, u, r);
+                for (int n = 0; n < NSP; ++n) LAs.fabs[b](i, j, k, n) = r[n];
+            });
+        }
+    }
+    return bad;
+}
+
+// Exec/5_CHEMSOD/prob.H:11-82 with prob_parm.H defaults: H2:O2 = 0.667:0.333,
+// 20 atm and 1500 K for x < 0.0004, 0.2 atm and 295 K elsewhere, at rest
+void orc_init_chemsod(const int* lo, const int* hi, const orc_fab* S, const orc_fab* Spec, const orc_geom* g)
+{
+    extern double orc_CKRHOY(double P, double T, const double* y);
+    extern void orc_CKXTY(const double* x, double* y);
+    extern double orc_CKUBMS(double T, const double* rhoi);
+    View s(*S), sp(*Spec);
+    for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i) {
+                double Pt = 0.2 * 101325.0, temp = 295;
+                double Xt[NSP] = {0.0}, Yt[NSP], rhoi[NSP];
+                Xt[0] = 0.667;
+                Xt[1] = 0.333;
+                double x = g->prob_lo[0] + (i + 0.5) * g->dx[0];
+                if (x < 0.0004) { Pt = 20 * 101325.0; temp = 1500; }
+                orc_CKXTY(Xt, Yt);
+                double rho = orc_CKRHOY(Pt, temp, Yt);
+                for (int n = 0; n < NSP; ++n) rhoi[n] = rho * Yt[n];
+                double et = orc_CKUBMS(temp, rhoi);
+                s(i, j, k, URHO) = rho;
+                s(i, j, k, UMX) = 0.0;
+                s(i, j, k, UMY) = 0.0;
+                s(i, j, k, UMZ) = 0.0;
+                s(i, j, k, UEDEN) = et;
+                // ebr_initspec: rho of the state times the same Y
+                for (int n = 0; n < NSP; ++n) sp(i, j, k, n) = s(i, j, k, URHO) * Yt[n];
+            }
 }
 
 } // extern "C"

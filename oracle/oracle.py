@@ -356,6 +356,8 @@ def _chem_sigs():
     L.orc_CKXTY.argtypes = [dp, dp]
     L.orc_CKYTX.argtypes = [dp, dp]
     L.orc_species_enthalpy.argtypes = [ctypes.c_double, dp]
+    L.orc_mixture_properties.argtypes = [ctypes.c_double, dp, dp, dp]
+    L.orc_mix_diff_coeffs_mass.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp]
     L.orc_chem_step_many.argtypes = [ctypes.c_long, dp, dp, ctypes.c_double, ctypes.c_double]
     L.orc_renorm_many.argtypes = [ctypes.c_long, dp, dp]
     L.orc_reactor_init.argtypes = [ctypes.c_long, dp, dp, dp, dp, dp]
@@ -461,6 +463,42 @@ REF_DIR = HERE / "_ref"
 _ref = None
 
 
+_ref_t = None
+
+
+def ref_trans():
+    """ctypes handle of oracle/_ref/libtransref.so (Test/transport_test.cpp), or None."""
+    global _ref_t
+    if _ref_t is None:
+        p = REF_DIR / "libtransref.so"
+        if not p.exists():
+            return None
+        L = ctypes.CDLL(str(p))
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.mixtureProperties = L._Z17mixturePropertiesdPdRdS0_
+        L.mixtureProperties.argtypes = [ctypes.c_double, dp, dp, dp]
+        L.getMixDiffCoeffsMass = L._Z20getMixDiffCoeffsMassddPdS_
+        L.getMixDiffCoeffsMass.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp]
+        _ref_t = L
+    return _ref_t
+
+
+def mixture_properties(T, X):
+    L = _chem_sigs()
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    mu = ctypes.c_double(); lam = ctypes.c_double()
+    L.orc_mixture_properties(T, _dp(X), ctypes.byref(mu), ctypes.byref(lam))
+    return mu.value, lam.value
+
+
+def mix_diff_coeffs(T, p, X):
+    L = _chem_sigs()
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    D = np.zeros(NSPEC)
+    L.orc_mix_diff_coeffs_mass(T, p, _dp(X), _dp(D))
+    return D
+
+
 def ref_chem():
     """ctypes handle of oracle/_ref/libchemref.so, or None when it was not built."""
     global _ref
@@ -491,3 +529,67 @@ def ref_chem():
         L.CKYTX.argtypes = [dp, dp]
         _ref = L
     return _ref
+
+
+# ---------------------------------------------------------------------------
+# multispecies (CHEM) flow
+# ---------------------------------------------------------------------------
+def _multi_sigs():
+    L = lib()
+    if getattr(L, "_multi_ready", False):
+        return L
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    F = ctypes.POINTER(OFab)
+    L.orc_compute_dsdt_multi_box.argtypes = [ip, ip, F, F, F, F, dp, ctypes.c_int, ctypes.POINTER(OParm)]
+    L.orc_fill_ghost_spec.argtypes = [ctypes.c_int, ip, ip, F, ctypes.POINTER(OGeom), ctypes.c_double]
+    L.orc_flow_advance_multi.argtypes = [ctypes.c_int, ip, ip, F, F, F, F, ctypes.POINTER(OGeom), ctypes.POINTER(OParm),
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    L.orc_advance_chem.argtypes = [ctypes.c_int, ip, ip, F, F, F, F, ctypes.POINTER(OGeom), ctypes.POINTER(OParm),
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int]
+    L.orc_init_chemsod.argtypes = [ip, ip, F, F, ctypes.POINTER(OGeom)]
+    L._multi_ready = True
+    return L
+
+
+def compute_dsdt_multi_box(lo, hi, S, Sp, p, dxinv, do_visc=0):
+    L = _multi_sigs()
+    D = Fab(lo, hi)
+    Ds = Fab(lo, hi, ncomp=NSPEC)
+    dxi = np.ascontiguousarray(dxinv, dtype=np.float64)
+    L.orc_compute_dsdt_multi_box((ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi), ctypes.byref(S.c),
+                                 ctypes.byref(Sp.c), ctypes.byref(D.c), ctypes.byref(Ds.c), _dp(dxi), do_visc,
+                                 ctypes.byref(p))
+    return D.a, Ds.a
+
+
+def fill_ghost_spec(boxes, fabs, g, time):
+    L = _multi_sigs()
+    n, lo, hi = _boxarrays(boxes)
+    return L.orc_fill_ghost_spec(n, lo, hi, _fabarray(fabs), ctypes.byref(g), time)
+
+
+def flow_advance_multi(boxes, A, As, B, Bs, g, p, time, dt, scheme=1, do_visc=0):
+    L = _multi_sigs()
+    n, lo, hi = _boxarrays(boxes)
+    return L.orc_flow_advance_multi(n, lo, hi, _fabarray(A), _fabarray(As), _fabarray(B), _fabarray(Bs),
+                                    ctypes.byref(g), ctypes.byref(p), time, dt, scheme, do_visc)
+
+
+def advance_chem(boxes, A, As, B, Bs, g, p, time, dt, level=0, scheme=1, do_visc=0, renorm=1):
+    L = _multi_sigs()
+    n, lo, hi = _boxarrays(boxes)
+    return L.orc_advance_chem(n, lo, hi, _fabarray(A), _fabarray(As), _fabarray(B), _fabarray(Bs),
+                              ctypes.byref(g), ctypes.byref(p), time, dt, level, scheme, do_visc, renorm)
+
+
+def init_chemsod(boxes, fabs, sfabs, g):
+    L = _multi_sigs()
+    for (lo, hi), f, fs in zip(boxes, fabs, sfabs):
+        L.orc_init_chemsod((ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi), ctypes.byref(f.c), ctypes.byref(fs.c),
+                           ctypes.byref(g))
+
+
+def new_spec_fabs(boxes, g=NGROW):
+    return [Fab(*grown(lo, hi, g), ncomp=NSPEC) for lo, hi in boxes]

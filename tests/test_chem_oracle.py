@@ -186,3 +186,52 @@ def test_renorm(orc):
     assert np.allclose(out.sum(1), u[:, 0], rtol=1e-14)
     clipped = np.maximum(r, 0)
     assert np.allclose(out, clipped * (u[:, 0] / clipped.sum(1))[:, None], rtol=1e-14)
+
+
+@pytest.fixture(scope="module")
+def reft(orc):
+    r = orc.ref_trans()
+    if r is None:
+        pytest.skip("oracle/_ref/libtransref.so not built (needs the reference sources)")
+    return r
+
+
+def test_transport_matches_reference_build(orc, reft):
+    """mixtureProperties / getMixDiffCoeffsMass bitwise against the reference's
+    own AMReX-free copy (Test/transport_test.cpp)."""
+    rng = np.random.default_rng(8)
+    for _ in range(200):
+        X = rng.uniform(0, 1, 9) ** 2
+        X /= X.sum()
+        T = rng.uniform(250, 3000)
+        p = rng.uniform(0.1, 20) * 101325.0
+        mu, lam = orc.mixture_properties(T, X)
+        mr = ctypes.c_double(); lr = ctypes.c_double()
+        Xc = np.ascontiguousarray(X)
+        reft.mixtureProperties(T, _dp(Xc), ctypes.byref(mr), ctypes.byref(lr))
+        assert mu == mr.value and lam == lr.value
+        D = orc.mix_diff_coeffs(T, p, X)
+        Dr = np.zeros(9)
+        reft.getMixDiffCoeffsMass(T, p, _dp(Xc), _dp(Dr))
+        assert np.array_equal(D, Dr)
+
+
+def test_transport_limits(orc):
+    """A pure species gets its own viscosity and conductivity (Wilke's
+    phi_kk = 1, lambda = (X lambda + 1/(X/lambda))/2); air-like mixtures at
+    300 K have mu ~ 1.8e-5 Pa s and lambda ~ 0.026 W/m/K; D scales as 1/p."""
+    T = 300.0
+    mus = []
+    for k in range(9):
+        X = np.zeros(9); X[k] = 1.0
+        mu, lam = orc.mixture_properties(T, X)
+        assert mu > 0 and lam > 0
+        mus.append(mu)
+    X = np.zeros(9); X[1] = 0.21; X[8] = 0.79
+    mu, lam = orc.mixture_properties(T, X)
+    assert 1.6e-5 < mu < 2.0e-5 and 0.022 < lam < 0.030
+    X2 = X.copy(); X2[0] = 1e-3; X2 /= X2.sum()
+    D1 = orc.mix_diff_coeffs(T, 101325.0, X2)
+    D2 = orc.mix_diff_coeffs(T, 2 * 101325.0, X2)
+    assert np.allclose(D1, 2 * D2, rtol=1e-14)
+    assert 5e-5 < D1[0] < 1.5e-4          # H2 in air ~ 7.8e-5 m^2/s at 300 K
