@@ -254,6 +254,48 @@ int ocfd_reactor_advance_adaptive(long n, double* state, double t_end, double h0
 /* Temperature of each reactor (GET_T_GIVEN_EY, Source/CHEM/LiDryer.H:836-888). */
 int ocfd_reactor_temperature(long n, const double* state, double* T, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Multispecies flow (the CHEM build of the reference): state S (5 comps) and
+ * partial densities Spec (9 comps), both 4 ghosts.  FP64; regular boxes only
+ * (a level with cut boxes returns OCFD_EUNSUPPORTED).  flags: OCFD_VISC.
+ * ------------------------------------------------------------------------- */
+#define OCFD_CASE_CHEMSOD 4   /* Exec/5_CHEMSOD: H2/O2 shock tube with a hot driver */
+
+/* FillPatch of the species MultiFab on one level: scalar BCs for every
+ * component (Source/Setup.cpp:241-247). */
+int ocfd_fill_boundary_spec(ocfd_level lev, double* Spec, double time, void* stream);
+
+/* EBR::compute_dSdt_multi, regular boxes (Source/CHEM/FlowAdvance.cpp:92-496):
+ * c2prim_rgas, mixture transport (viscous), Steger-Warming split of flow and
+ * species with the cell's gamma, WENO-Z5 for 5 + 9 comps, Fickian diffusion,
+ * multispecies viscous flux, divop.  dSdt: 5 comps, dSdt_spec: 9 comps, valid
+ * layout (ocfd_level_mf_size(lev, 0, 5 or 9)). */
+int ocfd_compute_dsdt_multi(ocfd_level lev, const double* S, const double* Spec, double* dSdt, double* dSdt_spec,
+                            double dt, const ocfd_parm* parm, int flags, void* stream);
+
+/* One stage of EBR::flow_advance_multi for state and species together
+ * (LinComb / Saxpy of Source/CHEM/FlowAdvance.cpp:59-86). */
+int ocfd_rk_stage_multi(ocfd_level lev, const double* S_cur, const double* Spec_cur, const double* S_old,
+                        const double* Spec_old, double* S_new, double* Spec_new, double c_cur, double c_old,
+                        double c_dt, const ocfd_parm* parm, int flags, void* stream);
+
+/* EBR::flow_advance_multi (Source/CHEM/FlowAdvance.cpp:9-90): scheme 0 Euler,
+ * 1 RK2 (its RK3 is an error in the reference).  A/Aspec in and out, B/Bspec
+ * scratch. */
+int ocfd_advance_multi(ocfd_level lev, double* A, double* Aspec, double* B, double* Bspec, double time, double dt,
+                       int scheme, const ocfd_parm* parm, int flags, void* stream);
+
+/* EBR::advance with chemistry (Source/Advance.cpp:152-177): iter = 4/2^level
+ * (level <= 2, else 1) chemistry steps of dt/(2 iter), the flow step, iter
+ * more chemistry steps; renorm != 0 adds the species renormalisation of
+ * EBR::post_timestep (Source/EBR.cpp:333-357). */
+int ocfd_advance_chem(ocfd_level lev, double* A, double* Aspec, double* B, double* Bspec, double time, double dt,
+                      int level, int scheme, const ocfd_parm* parm, int flags, int renorm, void* stream);
+
+/* Initial state and species of a multispecies case (kind OCFD_CASE_CHEMSOD:
+ * Exec/5_CHEMSOD/prob.H:11-82).  Ghost cells are zeroed.  Synchronous. */
+int ocfd_init_species_state(ocfd_level lev, int kind, double* S, double* Spec, const ocfd_parm* parm);
+
 /* Initialise device time control: time, cfl, stop_time, max_steps, and the
  * initial dt = cfl * estdt(A) clipped to stop_time (EBR::computeInitialDt,
  * Source/EBR.cpp:189-229,561-563).  No host synchronisation. */

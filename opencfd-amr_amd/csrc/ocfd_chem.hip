@@ -28,6 +28,7 @@ __global__ __launch_bounds__(64) void k_chem_level(const BoxDesc* __restrict__ b
     const int4 t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.z) return;
     const BoxDesc B = boxes[t.x];
+    if (B.type == 2) return;   // covered boxes are skipped (ChemAdvance.cpp:20)
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
     const int g = t.y + threadIdx.x;
     const int i = B.lo[0] + g % nx, j = B.lo[1] + (g / nx) % ny, k = B.lo[2] + g / (nx * ny);

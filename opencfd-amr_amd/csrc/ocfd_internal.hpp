@@ -45,9 +45,22 @@ struct LevelDev {
     int4* d_tile_eq = nullptr; int ntile_eq = 0;      // cut boxes: grow(bx,2) cells
     int4* d_tile_ef[3] = {nullptr, nullptr, nullptr}; int ntile_ef[3] = {0, 0, 0};   // faces
     int4* d_tile_ed = nullptr; int ntile_ed = 0;      // cut and covered boxes: valid cells
+    // multispecies flow (regular boxes)
+    int4* d_tile_mf[3] = {nullptr, nullptr, nullptr}; int ntile_mf[3] = {0, 0, 0};   // faces
+    int4* d_tile_md = nullptr; int ntile_md = 0;      // valid cells
+    double* d_mq = nullptr;         // 19 comps, grown layout (allocated on first use)
+    double* d_mF[3] = {nullptr, nullptr, nullptr};    // 14-comp face fluxes per direction
 };
 
 void launch_eb(const LevelDev& L, const StageArgs& a, hipStream_t s);
+
+inline PhysArgs phys_of(const ocfd_parm& p)
+{
+    PhysArgs a;
+    a.gamma = p.eos_gamma; a.smallr = p.smallr; a.smallp = p.smallp; a.Rg = p.Rg;
+    a.C_s = p.C_s; a.T_s = p.T_s; a.kOverMu = p.kOverMu; a.twoThirds = p.twoThirds;
+    return a;
+}
 
 void launch_sweeps(const LevelDev& L, const StageArgs& a, hipStream_t s);
 void launch_sweep_one(const LevelDev& L, const StageArgs& a, int which, hipStream_t s);

@@ -39,43 +39,6 @@ static int check_launch(const char* what)
 // ghost fill
 // ---------------------------------------------------------------------------
 
-// Decode the g-th ghost cell of box B: z slabs (full grown x-y extent), then
-// y slabs (valid k), then x slabs (valid j,k).
-__device__ __forceinline__ void ghost_cell(const BoxDesc& B, int g, int& i, int& j, int& k)
-{
-    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
-    const int nxg = B.nxg, nyg = B.nyg;
-    const int zs = NG * nxg * nyg;
-    if (g < 2 * zs) {
-        const int side = g / zs, r = g - side * zs;
-        const int kk = r / (nxg * nyg), rem = r - kk * nxg * nyg;
-        k = side ? B.hi[2] + 1 + kk : B.lo[2] - NG + kk;
-        j = B.lo[1] - NG + rem / nxg;
-        i = B.lo[0] - NG + rem % nxg;
-        return;
-    }
-    g -= 2 * zs;
-    const int ys = NG * nxg * nz;
-    if (g < 2 * ys) {
-        const int side = g / ys, r = g - side * ys;
-        const int kk = r / (NG * nxg), rem = r - kk * NG * nxg;
-        k = B.lo[2] + kk;
-        const int jj = rem / nxg;
-        j = side ? B.hi[1] + 1 + jj : B.lo[1] - NG + jj;
-        i = B.lo[0] - NG + rem % nxg;
-        return;
-    }
-    g -= 2 * ys;
-    const int xs = NG * ny * nz;
-    const int side = g / xs, r = g - side * xs;
-    const int kk = r / (NG * ny), rem = r - kk * NG * ny;
-    k = B.lo[2] + kk;
-    j = B.lo[1] + rem / NG;
-    const int ii = rem % NG;
-    i = side ? B.hi[0] + 1 + ii : B.lo[0] - NG + ii;
-    (void)nx;
-}
-
 // Ghost cells from periodic images, per-direction BC maps (AMReX FilccCell:
 // foextrap copies the boundary cell, reflect_even/odd mirror about the face
 // with sign +-1, composed when a cell is outside in several directions) and
@@ -270,13 +233,6 @@ __global__ __launch_bounds__(256) void k_copy_ctl(double* __restrict__ dst, cons
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) dst[i] = src[i];
 }
 
-static PhysArgs phys_of(const ocfd_parm& p)
-{
-    PhysArgs a;
-    a.gamma = p.eos_gamma; a.smallr = p.smallr; a.smallp = p.smallp; a.Rg = p.Rg;
-    a.C_s = p.C_s; a.T_s = p.T_s; a.kOverMu = p.kOverMu; a.twoThirds = p.twoThirds;
-    return a;
-}
 
 // Source/Setup.cpp:21-34
 static const signed char scalar_bc[6]   = {BC_INT, BC_EXT, BC_FOEXT, BC_EVEN, BC_EVEN, BC_EVEN};
@@ -299,7 +255,8 @@ static void free_level(LevelDev& L)
     hipFree(L.d_acc); hipFree(L.d_red); hipFree(L.d_tile_p); hipFree(L.d_tile_pp);
     hipFree(L.d_prim32); hipFree(L.d_prim64); hipFree(L.d_tile_v);
     hipFree(L.d_ebflag); hipFree(L.d_ebgeo); hipFree(L.d_ebq); hipFree(L.d_tile_eq); hipFree(L.d_tile_ed);
-    for (int d = 0; d < 3; ++d) { hipFree(L.d_ebF[d]); hipFree(L.d_tile_ef[d]); }
+    for (int d = 0; d < 3; ++d) { hipFree(L.d_ebF[d]); hipFree(L.d_tile_ef[d]); hipFree(L.d_tile_mf[d]); hipFree(L.d_mF[d]); }
+    hipFree(L.d_tile_md); hipFree(L.d_mq);
     L = LevelDev{};
 }
 
@@ -469,7 +426,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
 
     // work-lists
     const int xseg = tile_xseg(), ch = tile_chunk();
-    std::vector<int4> tx, ty, tz, tg, tc, tp, tpp, tv, teq, tef[3], ted;
+    std::vector<int4> tx, ty, tz, tg, tc, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
         const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
@@ -490,6 +447,11 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
             for (long s = 0; s < B.nptv; s += 256) ted.push_back(make_int4(b, (int)s, (int)std::min<long>(256, B.nptv - s), 0));
             continue;
         }
+        for (int d = 0; d < 3; ++d) {
+            const long nf = (long)(nx + (d == 0)) * (ny + (d == 1)) * (nz + (d == 2));
+            for (long s = 0; s < nf; s += 256) tmf[d].push_back(make_int4(b, (int)s, (int)std::min<long>(256, nf - s), d));
+        }
+        for (long s = 0; s < B.nptv; s += 256) tmd.push_back(make_int4(b, (int)s, (int)std::min<long>(256, B.nptv - s), 0));
         const int nseg = (nx + xseg - 1) / xseg, nrg = (ny * nz + 3) / 4;
         for (int r = 0; r < nrg; ++r)
             for (int s = 0; s < nseg; ++s) tx.push_back(make_int4(b, s, r, 0));
@@ -515,10 +477,13 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         (rc = upload(&L.d_tile_p, tp)) || (rc = upload(&L.d_tile_pp, tpp)) || (rc = upload(&L.d_tile_v, tv)) ||
         (rc = upload(&L.d_tile_eq, teq)) || (rc = upload(&L.d_tile_ef[0], tef[0])) ||
         (rc = upload(&L.d_tile_ef[1], tef[1])) || (rc = upload(&L.d_tile_ef[2], tef[2])) ||
-        (rc = upload(&L.d_tile_ed, ted)) || (rc = upload(&L.d_ebflag, eb_flag)) || (rc = upload(&L.d_ebgeo, eb_geo))) {
+        (rc = upload(&L.d_tile_ed, ted)) || (rc = upload(&L.d_ebflag, eb_flag)) || (rc = upload(&L.d_ebgeo, eb_geo)) ||
+        (rc = upload(&L.d_tile_mf[0], tmf[0])) || (rc = upload(&L.d_tile_mf[1], tmf[1])) ||
+        (rc = upload(&L.d_tile_mf[2], tmf[2])) || (rc = upload(&L.d_tile_md, tmd))) {
         free_level(L); delete lev; return rc;
     }
-    L.ntile_eq = (int)teq.size(); L.ntile_ed = (int)ted.size();
+    L.ntile_eq = (int)teq.size(); L.ntile_ed = (int)ted.size(); L.ntile_md = (int)tmd.size();
+    for (int d = 0; d < 3; ++d) L.ntile_mf[d] = (int)tmf[d].size();
     for (int d = 0; d < 3; ++d) L.ntile_ef[d] = (int)tef[d].size();
     if (L.mf_eb) {
         bool ok = hipMalloc((void**)&L.d_ebq, sizeof(double) * 7 * L.mf_eb) == hipSuccess;

@@ -15,6 +15,7 @@ LIB_PATH = PKG / "libocfd.so"
 
 NCONS = 5
 NGROW = 4
+NSPEC = 9
 
 VISC = 1
 GRAVITY = 2
@@ -24,7 +25,7 @@ PLANAR = 8
 # physical BC codes (Exec/*/inputs)
 INTERIOR, INFLOW, OUTFLOW, SYMMETRY, SLIPWALL, NOSLIPWALL = range(6)
 
-CASE_SOD, CASE_DMR, CASE_KH3D, CASE_EB_INFLOW = 0, 1, 2, 3
+CASE_SOD, CASE_DMR, CASE_KH3D, CASE_EB_INFLOW, CASE_CHEMSOD = 0, 1, 2, 3, 4
 
 
 class Parm(ctypes.Structure):
@@ -142,6 +143,15 @@ def lib():
                                                ctypes.c_int, vp, ctypes.POINTER(vp)]
         L.ocfd_graph_launch.argtypes = [vp, vp]
         L.ocfd_graph_destroy.argtypes = [vp]
+        L.ocfd_fill_boundary_spec.argtypes = [vp, vp, ctypes.c_double, vp]
+        L.ocfd_compute_dsdt_multi.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.POINTER(Parm), ctypes.c_int, vp]
+        L.ocfd_rk_stage_multi.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, ctypes.POINTER(Parm), ctypes.c_int, vp]
+        L.ocfd_advance_multi.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.POINTER(Parm), ctypes.c_int, vp]
+        L.ocfd_advance_chem.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int, ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, vp]
+        L.ocfd_init_species_state.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(Parm)]
         L.ocfd_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -325,6 +335,29 @@ class Level:
     def init_state(self, kind, S, parm, seed=0):
         check(lib().ocfd_init_state(self.handle, kind, _ptr(S), ctypes.byref(parm), seed), "ocfd_init_state")
 
+    # ---- multispecies flow ----------------------------------------------
+    def new_spec_mf(self, ngrow=NGROW, device="cuda"):
+        return self.new_mf(ngrow, NSPEC, device)
+
+    def fill_boundary_spec(self, Spec, time, stream=None):
+        check(lib().ocfd_fill_boundary_spec(self.handle, _ptr(Spec), time, _stream(stream)), "ocfd_fill_boundary_spec")
+
+    def compute_dsdt_multi(self, S, Spec, dSdt, dSdt_spec, parm, flags=0, dt=0.0, stream=None):
+        check(lib().ocfd_compute_dsdt_multi(self.handle, _ptr(S), _ptr(Spec), _ptr(dSdt), _ptr(dSdt_spec), dt,
+                                            ctypes.byref(parm), flags, _stream(stream)), "ocfd_compute_dsdt_multi")
+
+    def advance_multi(self, A, As, B, Bs, time, dt, scheme, parm, flags=0, stream=None):
+        check(lib().ocfd_advance_multi(self.handle, _ptr(A), _ptr(As), _ptr(B), _ptr(Bs), time, dt, scheme,
+                                       ctypes.byref(parm), flags, _stream(stream)), "ocfd_advance_multi")
+
+    def advance_chem(self, A, As, B, Bs, time, dt, parm, level=0, scheme=1, flags=0, renorm=True, stream=None):
+        check(lib().ocfd_advance_chem(self.handle, _ptr(A), _ptr(As), _ptr(B), _ptr(Bs), time, dt, level, scheme,
+                                      ctypes.byref(parm), flags, int(renorm), _stream(stream)), "ocfd_advance_chem")
+
+    def init_species_state(self, kind, S, Spec, parm):
+        check(lib().ocfd_init_species_state(self.handle, kind, _ptr(S), _ptr(Spec), ctypes.byref(parm)),
+              "ocfd_init_species_state")
+
     # ---- device-driven time steps ---------------------------------------
     def ctl_start(self, A, ctl, parm, cfl, time=0.0, stop_time=-1.0, max_steps=-1, stream=None):
         """Initial dt on the device into ctl (a CTL_BYTES-byte device buffer)."""
@@ -339,6 +372,30 @@ class Level:
         """A StepGraph of nsteps device-driven steps, captured on `stream`
         (a non-default torch stream)."""
         return StepGraph(self, A, B, C, ctl, parm, nsteps, scheme, flags, stream)
+
+
+CHEM_ESTDT = 1e-8   # the CHEM build's ebr_estdt (Source/Advance.cpp:54-57)
+
+
+def run_chem(level, A, As, B, Bs, parm, cfl, nsteps=-1, stop_time=-1.0, scheme=1, flags=0, lev=0, time=0.0,
+             stream=None):
+    """Time loop of a multispecies level: dt = cfl * 1e-8 (the CHEM build's
+    estimate), computeNewDt's change_max and stop_time clip
+    (Source/EBR.cpp:231-306), one Strang-split EBR::advance plus the species
+    renormalisation per step."""
+    dt = cfl * CHEM_ESTDT
+    if stop_time >= 0.0 and time + dt > stop_time - 0.001 * dt:
+        dt = stop_time - time
+    steps = 0
+    while (nsteps < 0 or steps < nsteps) and (stop_time < 0.0 or time < stop_time):
+        level.advance_chem(A, As, B, Bs, time, dt, parm, lev, scheme, flags, True, stream)
+        time += dt
+        steps += 1
+        dt_new = min(cfl * CHEM_ESTDT, 1.1 * dt)
+        if stop_time >= 0.0 and time + dt_new > stop_time - 0.001 * dt_new:
+            dt_new = stop_time - time
+        dt = dt_new
+    return steps, time, dt
 
 
 def new_ctl(device="cuda"):
@@ -403,7 +460,6 @@ def run(level, A, B, C, parm, cfl, scheme=2, flags=0, stop_time=-1.0, nsteps=-1,
 # ---------------------------------------------------------------------------
 # batched constant-volume reactors (0D ignition)
 # ---------------------------------------------------------------------------
-NSPEC = 9
 REACTOR_NCOMP = 11   # rows: rho, E, 9 partial densities
 
 

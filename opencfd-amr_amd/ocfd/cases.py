@@ -3,7 +3,7 @@ of the reference, restated as level set-ups)."""
 import numpy as np
 
 from . import (Geom, Level, make_geom, dmr_states, eb_cylinder, inflow_state, INTERIOR, INFLOW, OUTFLOW, SLIPWALL,
-               CASE_SOD, CASE_DMR, CASE_KH3D, CASE_EB_INFLOW)
+               CASE_SOD, CASE_DMR, CASE_KH3D, CASE_EB_INFLOW, CASE_CHEMSOD)
 
 
 def chop(n_cell, max_grid):
@@ -94,3 +94,13 @@ def eb_channel(nx=512, ny=128, nz=2, max_grid=32, radius=0.1, center=(0.5, 0.25)
                   left_state=st, right_state=st)
     boxes = chop(n, (max_grid, max_grid, nz))
     return boxes, g, CASE_EB_INFLOW, eb_cylinder(center[0], center[1], radius, small_volfrac)
+
+
+def chemsod(nx=1280, ny=4, nz=4, max_grid=128):
+    """H2/O2 shock tube with a hot high-pressure driver (Exec/5_CHEMSOD/inputs):
+    dx = 1e-4 m, slip wall at x-lo, outflow at x-hi, periodic in y and z."""
+    n = (nx, ny, nz)
+    dx = 1e-4
+    g = make_geom(n, (0.0, 0.0, 0.0), (nx * dx, ny * dx, nz * dx), periodic=(0, 1, 1),
+                  lo_bc=(SLIPWALL, INTERIOR, INTERIOR), hi_bc=(OUTFLOW, INTERIOR, INTERIOR))
+    return chop(n, (max_grid, max_grid, max_grid)), g, CASE_CHEMSOD
