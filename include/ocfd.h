@@ -256,8 +256,10 @@ int ocfd_reactor_temperature(long n, const double* state, double* T, void* strea
 
 /* ---------------------------------------------------------------------------
  * Multispecies flow (the CHEM build of the reference): state S (5 comps) and
- * partial densities Spec (9 comps), both 4 ghosts.  FP64; regular boxes only
- * (a level with cut boxes returns OCFD_EUNSUPPORTED).  flags: OCFD_VISC.
+ * partial densities Spec (9 comps), both 4 ghosts.  FP64.  Cut boxes take the
+ * EB-CHEM path of EBR::ebchem_compute_dSdt_box (Source/CHEM/FlowAdvanceBox.cpp:
+ * species only; the reference leaves their flow dSdt at 0), covered boxes get
+ * dSdt = 0.  flags: OCFD_VISC.
  * ------------------------------------------------------------------------- */
 #define OCFD_CASE_CHEMSOD 4   /* Exec/5_CHEMSOD: H2/O2 shock tube with a hot driver */
 

@@ -506,6 +506,174 @@ __global__ __launch_bounds__(256) void k_fill_spec(const BoxDesc* __restrict__ b
     for (int n = 0; n < NS; ++n) P[dst + n * B.nptg] = P[s0 + n * O.nptg];
 }
 
+
+// ---------------------------------------------------------------------------
+// EB-CHEM boxes: EBR::ebchem_compute_dSdt_box (Source/CHEM/FlowAdvanceBox.cpp:16-525)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool mconn(uint32_t f, int i, int j, int k)
+{
+    return (f >> (5 + (i + 1) + 3 * (j + 1) + 9 * (k + 1))) & 1u;
+}
+
+__device__ __forceinline__ double mminmod(double a, double b)
+{
+    if (a * b > 0) return fabs(a) > fabs(b) ? b : a;
+    return 0.0;
+}
+
+// eb_recon (Source/EB/EBkernels.H:10-71) of one component x (stride ds) at the
+// low face of cell l0 with the right cell's flag f
+__device__ __forceinline__ void mrecon(const double* x, long l0, long ds, bool cm, bool cm2, bool cp, double& ql,
+                                       double& qr)
+{
+    const double q0 = x[l0], qm = x[l0 - ds];
+    ql = qm;
+    qr = q0;
+    if (cm) {
+        if (cm2) ql = qm + 0.5 * mminmod(q0 - qm, qm - x[l0 - 2 * ds]);
+        if (cp) qr = q0 - 0.5 * mminmod(x[l0 + ds] - q0, q0 - qm);
+    }
+}
+
+// species face flux of a cut box: MUSCL states of u_D, c and rho_k, Van Leer
+// mass-flux split (FlowAdvanceBox.cpp:174-223), Fickian flux when viscous
+template <int D>
+__global__ __launch_bounds__(64) void k_ebchem_face(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+                                                    MultiArgs m, const uint32_t* __restrict__ ebflag)
+{
+    if (ctl_done(m.a.ctl)) return;
+    const int4 t = tiles[blockIdx.x];
+    for (int c = threadIdx.x; c < t.z; c += 64) {
+        const BoxDesc B = boxes[t.x];
+        const int di = D == 0, dj = D == 1, dk = D == 2;
+        const int nx = B.hi[0] - B.lo[0] + 1 + di, ny = B.hi[1] - B.lo[1] + 1 + dj;
+        const int cc = t.y + c;
+        const int i = B.lo[0] + cc % nx, j = B.lo[1] + (cc / nx) % ny, k = B.lo[2] + cc / (nx * ny);
+        const long st = B.nptg;
+        const long ds = D == 0 ? 1 : (D == 1 ? (long)B.nxg : (long)B.nxg * B.nyg);
+        const long l0 = gl(B, i, j, k);
+        const double* q = m.mq + B.off_s / NC * MQ_N;
+        const double* P = m.pcur + B.off_s / NC * NS;
+        const uint32_t f = ebflag[B.off_e + l0];
+        const bool cm = mconn(f, -di, -dj, -dk), cm2 = mconn(f, -2 * di, -2 * dj, -2 * dk), cp = mconn(f, di, dj, dk);
+        double ul, ur, cL, cR;
+        mrecon(q + (MQ_U + D) * st, l0, ds, cm, cm2, cp, ul, ur);
+        mrecon(q + MQ_C * st, l0, ds, cm, cm2, cp, cL, cR);
+        const double ML = ul / cL, MR = ur / cR, UL = ul, UR = ur;
+        double F[NS];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            double rl, rr;
+            mrecon(P + n * st, l0, ds, cm, cm2, cp, rl, rr);
+            double fp = 0.0, fm = 0.0;
+            if (ML >= 1.0) fp = rl * UL;
+            else if (fabs(ML) < 1.0) { const double Mp = 0.250 * (1.0 + ML) * (1.0 + ML); fp = rl * cL * Mp; }
+            if (fabs(MR) < 1.0) { const double Mm = -0.250 * (MR - 1.0) * (MR - 1.0); fm = rr * cR * Mm; }
+            else if (MR <= -1.0) fm = rr * UR;
+            F[n] = fp + fm;
+        }
+        if (m.a.flags & OCFD_VISC) {
+            const long lm = l0 - ds;
+            const double dxn = B.dxinv[D];
+            const double rhor = q[MQ_R * st + l0], rhol = q[MQ_R * st + lm];
+            const double rho = 0.5 * (rhol + rhor);
+#pragma unroll
+            for (int n = 0; n < NS; ++n) {
+                const double Ytr = P[n * st + l0] / rhor;
+                const double Ytl = P[n * st + lm] / rhol;
+                const double dn = 0.5 * (q[(MQ_D + n) * st + l0] + q[(MQ_D + n) * st + lm]);
+                F[n] += rho * dn * (Ytl - Ytr) * dxn;
+            }
+        }
+        double* Fo = m.mF[D] + B.off_s / NC * NF + l0 + NC * st;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) Fo[n * st] = F[n];
+    }
+}
+
+// per valid cell of cut and covered boxes: the flow dSdt is 0 (covered boxes,
+// Source/CHEM/FlowAdvance.cpp:127-129; cut boxes, FlowAdvanceBox.cpp:501-524);
+// the species divergence with area fractions and the wall term (:414-460)
+__global__ __launch_bounds__(256) void k_ebchem_div(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+                                                    MultiArgs m, const uint32_t* __restrict__ ebflag,
+                                                    const double* __restrict__ ebgeo)
+{
+    if (ctl_done(m.a.ctl)) return;
+    const int4 t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.z) return;
+    const BoxDesc B = boxes[t.x];
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
+    const int cc = t.y + threadIdx.x;
+    const int i = B.lo[0] + cc % nx, j = B.lo[1] + (cc / nx) % ny, k = B.lo[2] + cc / (nx * ny);
+    const long st = B.nptg, l0 = gl(B, i, j, k);
+    double L[NF];
+#pragma unroll
+    for (int n = 0; n < NF; ++n) L[n] = 0.0;
+    if (B.type == 1) {
+        const long sy = B.nxg, sz = (long)B.nxg * B.nyg;
+        const uint32_t f = ebflag[B.off_e + l0];
+        const double* geo = ebgeo + B.off_e * 4;
+        const double vf = geo[l0];
+        const double* ax = geo + st;
+        const double* ay = geo + 2 * st;
+        const double* az = geo + 3 * st;
+        const long f0 = B.off_s / NC * NF + l0 + NC * st;
+        const double* Fx = m.mF[0] + f0;
+        const double* Fy = m.mF[1] + f0;
+        const double* Fz = m.mF[2] + f0;
+        const double tmp = B.dxinv[0] / vf;
+        double Ls[NS];
+        if ((f & 3u) == 3u) {
+#pragma unroll
+            for (int n = 0; n < NS; ++n) Ls[n] = 0.0;
+        } else if ((f & 3u) == 0u) {
+#pragma unroll
+            for (int n = 0; n < NS; ++n)
+                Ls[n] = B.dxinv[0] * (Fx[n * st + 1] - Fx[n * st]) + B.dxinv[1] * (Fy[n * st + sy] - Fy[n * st]) +
+                        B.dxinv[2] * (Fz[n * st + sz] - Fz[n * st]);
+        } else {
+            const double axm = ax[l0], axp = ax[l0 + 1], aym = ay[l0], ayp = ay[l0 + sy], azm = az[l0], azp = az[l0 + sz];
+#pragma unroll
+            for (int n = 0; n < NS; ++n)
+                Ls[n] = tmp * (axp * Fx[n * st + 1] - axm * Fx[n * st] + ayp * Fy[n * st + sy] - aym * Fy[n * st] +
+                               azp * Fz[n * st + sz] - azm * Fz[n * st]);
+            if (m.a.flags & OCFD_VISC) {
+                const double* q = m.mq + B.off_s / NC * MQ_N + l0;
+                const double* P = m.pcur + B.off_s / NC * NS + l0;
+                const double apnorm =
+                    sqrt((axm - axp) * (axm - axp) + (aym - ayp) * (aym - ayp) + (azm - azp) * (azm - azp));
+                const double un = q[MQ_U * st] * (axm - axp) + q[MQ_V * st] * (aym - ayp) + q[MQ_W * st] * (azm - azp);
+                const double lambda0 = fabs(un) + q[MQ_C * st] * apnorm;
+#pragma unroll
+                for (int n = 0; n < NS; ++n) {
+                    const double fp = 0.5 * (P[n * st] * un + lambda0 * P[n * st]);
+                    const double fm = -0.5 * (P[n * st] * un + lambda0 * P[n * st]);
+                    Ls[n] += (fp + fm) * tmp;
+                }
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < NS; ++n) L[NC + n] = -1.0 * Ls[n];
+    }
+    StageArgs a = m.a;
+    a.flags &= ~OCFD_GRAVITY;
+    finish_cell(a, B, i, j, k, L);
+    const long p0 = B.off_s / NC * NS + l0;
+    if (m.pnew) {
+        const double cdt = a.ctl ? a.c_dt * a.ctl->dt : a.c_dt;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            const double v = m.pold ? a.c_cur * m.pcur[p0 + n * st] + a.c_old * m.pold[p0 + n * st]
+                                    : a.c_cur * m.pcur[p0 + n * st];
+            m.pnew[p0 + n * st] = v + cdt * L[NC + n];
+        }
+    } else {
+        const long d0 = B.off_d / NC * NS + didx(B, i, j, k) - B.off_d;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) m.pacc[d0 + n * B.nptv] = L[NC + n];
+    }
+}
+
 void launch_multi(const LevelDev& L, const MultiArgs& m, hipStream_t s)
 {
     if (L.ntile_p) hipLaunchKernelGGL(k_mprims, dim3(L.ntile_p), dim3(64), 0, s, L.d_boxes, L.d_tile_p, m);
@@ -515,6 +683,19 @@ void launch_multi(const LevelDev& L, const MultiArgs& m, hipStream_t s)
         hipLaunchKernelGGL(k_mface<2>, dim3(L.ntile_mf[2]), dim3(64), 0, s, L.d_boxes, L.d_tile_mf[2], m);
     }
     if (L.ntile_md) hipLaunchKernelGGL(k_mdiv, dim3(L.ntile_md), dim3(256), 0, s, L.d_boxes, L.d_tile_md, m);
+    // cut and covered boxes
+    if (L.ntile_eq) {
+        hipLaunchKernelGGL(k_mprims, dim3(L.ntile_eq), dim3(64), 0, s, L.d_boxes, L.d_tile_eq, m);
+        hipLaunchKernelGGL(k_ebchem_face<0>, dim3(L.ntile_ef[0]), dim3(64), 0, s, L.d_boxes, L.d_tile_ef[0], m,
+                           (const uint32_t*)L.d_ebflag);
+        hipLaunchKernelGGL(k_ebchem_face<1>, dim3(L.ntile_ef[1]), dim3(64), 0, s, L.d_boxes, L.d_tile_ef[1], m,
+                           (const uint32_t*)L.d_ebflag);
+        hipLaunchKernelGGL(k_ebchem_face<2>, dim3(L.ntile_ef[2]), dim3(64), 0, s, L.d_boxes, L.d_tile_ef[2], m,
+                           (const uint32_t*)L.d_ebflag);
+    }
+    if (L.ntile_ed)
+        hipLaunchKernelGGL(k_ebchem_div, dim3(L.ntile_ed), dim3(256), 0, s, L.d_boxes, L.d_tile_ed, m,
+                           (const uint32_t*)L.d_ebflag, (const double*)L.d_ebgeo);
 }
 
 void launch_fill_spec(const LevelDev& L, double* P, const ocfd_step_ctl* ctl, hipStream_t s)
@@ -577,7 +758,6 @@ static int mcheck(const char* what)
 
 static int ensure_multi(LevelDev& L)
 {
-    if (L.n_cut) { set_error("multispecies flow: levels with cut boxes are not supported"); return OCFD_EUNSUPPORTED; }
     if (L.d_mq) return OCFD_OK;
     const long ng = L.mf_state / NC;
     if (hipMalloc((void**)&L.d_mq, sizeof(double) * MQ_N * ng) != hipSuccess) {

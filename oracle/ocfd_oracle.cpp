@@ -1493,11 +1493,141 @@ void compute_dsdt_multi_box(const Box& bx, const View& s, const View& spec, cons
     });
 }
 
+// Fickian species flux of face (i,j,k) in direction d, Source/CHEM/ChemViscous.H:108-199
+inline void diffusion_face(int d, int i, int j, int k, const View& q, const View& spec, const View& D,
+                           const double* dxinv, double* J)
+{
+    const int im = i - (d == 0), jm = j - (d == 1), km = k - (d == 2);
+    double rhor = q(i, j, k, QRHO);
+    double rhol = q(im, jm, km, QRHO);
+    double rho = 0.5 * (rhol + rhor);
+    for (int n = 0; n < NSP; ++n) {
+        double Ytr = spec(i, j, k, n) / rhor;
+        double Ytl = spec(im, jm, km, n) / rhol;
+        double dn = 0.5 * (D(i, j, k, n) + D(im, jm, km, n));
+        J[n] = rho * dn * (Ytl - Ytr) * dxinv[d];
+    }
+}
+
+// EBR::ebchem_compute_dSdt_box, Source/CHEM/FlowAdvanceBox.cpp:16-525: MUSCL
+// left/right states of the normal velocity, sound speed and partial densities
+// (eb_recon), the Van Leer mass-flux split for the species (:174-223), the
+// Fickian flux (viscous), the EB divergence with the species wall term
+// (:414-460).  The flow divergence is computed by the reference but never
+// copied into dSdt (:501-524), so the flow dSdt of these boxes is 0.
+void ebchem_dsdt_box(const Box& bx, const View& s, const View& spec, const View& dsdt, const View& dsdt_spec,
+                     const double* dxinv, const EBV& g, int do_visc, const orc_parm& parm)
+{
+    const Box bxg = grow(bx, NUM_GROW);
+    const size_t ng = (size_t)(bxg.hi[0]-bxg.lo[0]+1) * (bxg.hi[1]-bxg.lo[1]+1) * (bxg.hi[2]-bxg.lo[2]+1);
+    std::vector<double> qb(NPRIM_M * ng), mub(ng), lamb(ng), Db(NSP * ng);
+    View q(qb.data(), bxg.lo, bxg.hi, NPRIM_M), D(Db.data(), bxg.lo, bxg.hi, NSP);
+    for_box(bxg, [&](int i, int j, int k) { c2prim_rgas(i, j, k, s, spec, q, parm); });
+    if (do_visc) {
+        for_box(bxg, [&](int i, int j, int k) {
+            double Yt[NSP], Xt[NSP], Dk[NSP], mu, lam;
+            for (int n = 0; n < NSP; ++n) Yt[n] = spec(i, j, k, n) / q(i, j, k, QRHO);
+            orc_CKYTX(Yt, Xt);
+            orc_mixture_properties(q(i, j, k, QT), Xt, &mu, &lam);
+            orc_mix_diff_coeffs_mass(q(i, j, k, QT), q(i, j, k, QPRES), Xt, Dk);
+            for (int n = 0; n < NSP; ++n) D(i, j, k, n) = Dk[n];
+        });
+    }
+    Box fb[3];
+    std::vector<double> fsbuf[3];
+    View Fs[3];
+    for (int d = 0; d < 3; ++d) {
+        fb[d] = surrounding_nodes(bx, d);
+        const size_t nf = (size_t)(fb[d].hi[0]-fb[d].lo[0]+1) * (fb[d].hi[1]-fb[d].lo[1]+1) * (fb[d].hi[2]-fb[d].lo[2]+1);
+        fsbuf[d].assign(NSP * nf, 0.0);
+        Fs[d] = View(fsbuf[d].data(), fb[d].lo, fb[d].hi, NSP);
+        const View F = Fs[d];
+        for_box(fb[d], [&](int i, int j, int k) {
+            double ul, ur, cL, cR, rl[NSP], rr[NSP];
+            eb_recon(d, i, j, k, QU + d, q, g, ul, ur);
+            eb_recon(d, i, j, k, QC, q, g, cL, cR);
+            for (int n = 0; n < NSP; ++n) eb_recon(d, i, j, k, n, spec, g, rl[n], rr[n]);
+            double fp[NSP] = {0}, fm[NSP] = {0};
+            double ML = ul / cL, MR = ur / cR, UL = ul, UR = ur;
+            if (ML >= 1.0) {
+                for (int n = 0; n < NSP; ++n) fp[n] = rl[n] * UL;
+            } else if (std::fabs(ML) < 1.0) {
+                double Mp = 0.250 * (1.0 + ML) * (1.0 + ML);
+                for (int n = 0; n < NSP; ++n) fp[n] = rl[n] * cL * Mp;
+            }
+            if (std::fabs(MR) < 1.0) {
+                double Mm = -0.250 * (MR - 1.0) * (MR - 1.0);
+                for (int n = 0; n < NSP; ++n) fm[n] = rr[n] * cR * Mm;
+            } else if (MR <= -1.0) {
+                for (int n = 0; n < NSP; ++n) fm[n] = rr[n] * UR;
+            }
+            for (int n = 0; n < NSP; ++n) F(i, j, k, n) = fp[n] + fm[n];
+            if (do_visc) {
+                double J[NSP];
+                diffusion_face(d, i, j, k, q, spec, D, dxinv, J);
+                for (int n = 0; n < NSP; ++n) F(i, j, k, n) += J[n];
+            }
+        });
+    }
+    for_box(bx, [&](int i, int j, int k) {
+        for (int n = 0; n < NCONS; ++n) dsdt(i, j, k, n) = 0.0;
+        const uint32_t f = g.flag(i, j, k);
+        double tmp = dxinv[0] / g.vf(i, j, k);
+        double L[NSP];
+        if (eb_covered(f)) {
+            for (int n = 0; n < NSP; ++n) L[n] = 0.0;
+        } else if (eb_regular(f)) {
+            for (int n = 0; n < NSP; ++n)
+                L[n] = dxinv[0] * (Fs[0](i + 1, j, k, n) - Fs[0](i, j, k, n)) +
+                       dxinv[1] * (Fs[1](i, j + 1, k, n) - Fs[1](i, j, k, n)) +
+                       dxinv[2] * (Fs[2](i, j, k + 1, n) - Fs[2](i, j, k, n));
+        } else {
+            for (int n = 0; n < NSP; ++n)
+                L[n] = tmp * (g.ax(i + 1, j, k) * Fs[0](i + 1, j, k, n) - g.ax(i, j, k) * Fs[0](i, j, k, n) +
+                              g.ay(i, j + 1, k) * Fs[1](i, j + 1, k, n) - g.ay(i, j, k) * Fs[1](i, j, k, n) +
+                              g.az(i, j, k + 1) * Fs[2](i, j, k + 1, n) - g.az(i, j, k) * Fs[2](i, j, k, n));
+            if (do_visc) {
+                double axm = g.ax(i, j, k), axp = g.ax(i + 1, j, k), aym = g.ay(i, j, k), ayp = g.ay(i, j + 1, k);
+                double azm = g.az(i, j, k), azp = g.az(i, j, k + 1);
+                double apnorm = std::sqrt((axm - axp) * (axm - axp) + (aym - ayp) * (aym - ayp) + (azm - azp) * (azm - azp));
+                double un = q(i, j, k, QU) * (axm - axp) + q(i, j, k, QV) * (aym - ayp) + q(i, j, k, QW) * (azm - azp);
+                double lambda0 = std::fabs(un) + q(i, j, k, QC) * apnorm;
+                for (int n = 0; n < NSP; ++n) {
+                    double fp = 0.5 * (spec(i, j, k, n) * un + lambda0 * spec(i, j, k, n));
+                    double fm = -0.5 * (spec(i, j, k, n) * un + lambda0 * spec(i, j, k, n));
+                    L[n] += (fp + fm) * tmp;
+                }
+            }
+        }
+        for (int n = 0; n < NSP; ++n) dsdt_spec(i, j, k, n) = -1.0 * L[n];
+    });
+}
+
+// Per-box dispatch of EBR::compute_dSdt_multi (FlowAdvance.cpp:127-173)
+void compute_dsdt_multi_any(const Box& bx, const View& s, const View& spec, const View& dsdt, const View& dsdt_spec,
+                            const double* dxinv, const orc_ebfab* eb, int do_visc, const orc_parm& parm)
+{
+    if (!eb) { compute_dsdt_multi_box(bx, s, spec, dsdt, dsdt_spec, dxinv, do_visc, parm); return; }
+    EBV g(eb);
+    const int kind = eb_box_kind(g, bx);
+    if (kind == 2) {
+        for_box(bx, [&](int i, int j, int k) {
+            for (int n = 0; n < NCONS; ++n) dsdt(i, j, k, n) = 0.0;
+            for (int n = 0; n < NSP; ++n) dsdt_spec(i, j, k, n) = 0.0;
+        });
+    } else if (kind == 0) {
+        compute_dsdt_multi_box(bx, s, spec, dsdt, dsdt_spec, dxinv, do_visc, parm);
+    } else {
+        ebchem_dsdt_box(bx, s, spec, dsdt, dsdt_spec, dxinv, g, do_visc, parm);
+    }
+}
+
 // chemical_advance(dt) on the valid cells of every box (ChemAdvance.cpp:7-85)
 void chem_level(const Level& S, const Level& Sp, double dt, const orc_parm& parm)
 {
     for (int b = 0; b < S.nbox; ++b) {
         const Box& bx = S.boxes[b];
+        if (!g_eb.empty() && eb_region_type(EBV(&g_eb[b]), bx) == 2) continue;   // ChemAdvance.cpp:20
         std::vector<double> u, r;
         for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
             for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
@@ -1530,6 +1660,16 @@ void orc_compute_dsdt_multi_box(const int* lo, const int* hi, const orc_fab* S, 
     compute_dsdt_multi_box(bx, View(*S), View(*Spec), View(*dSdt), View(*dSdt_spec), dxinv, do_visc, *parm);
 }
 
+// compute_dSdt_multi for one box with EB fields (covered / regular / EB-CHEM)
+void orc_compute_dsdt_multi_box_eb(const int* lo, const int* hi, const orc_fab* S, const orc_fab* Spec,
+                                   const orc_fab* dSdt, const orc_fab* dSdt_spec, const double* dxinv,
+                                   const orc_ebfab* eb, int do_visc, const orc_parm* parm)
+{
+    Box bx;
+    for (int d = 0; d < 3; ++d) { bx.lo[d] = lo[d]; bx.hi[d] = hi[d]; }
+    compute_dsdt_multi_any(bx, View(*S), View(*Spec), View(*dSdt), View(*dSdt_spec), dxinv, eb, do_visc, *parm);
+}
+
 int orc_fill_ghost_spec(int nbox, const int* lo, const int* hi, const orc_fab* fabs, const orc_geom* g, double time)
 {
     Level L = make_level(nbox, lo, hi, fabs);
@@ -1558,7 +1698,8 @@ int orc_flow_advance_multi(int nbox, const int* lo, const int* hi, const orc_fab
     auto stage = [&](const Level& cur, const Level& curs, const Level* old, const Level* olds, const Level& out,
                      const Level& outs, double c_cur, double c_old, double c_dt) {
         for (int b = 0; b < nbox; ++b)
-            compute_dsdt_multi_box(cur.boxes[b], cur.fabs[b], curs.fabs[b], v1[b], v2[b], dxinv, do_visc, *parm);
+            compute_dsdt_multi_any(cur.boxes[b], cur.fabs[b], curs.fabs[b], v1[b], v2[b], dxinv,
+                                   g_eb.empty() ? nullptr : &g_eb[b], do_visc, *parm);
         for (int b = 0; b < nbox; ++b) {
             for_box(cur.boxes[b], [&](int i, int j, int k) {
                 for (int n = 0; n < NCONS; ++n) {

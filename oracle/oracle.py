@@ -549,6 +549,8 @@ def _multi_sigs():
                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int]
     L.orc_init_chemsod.argtypes = [ip, ip, F, F, ctypes.POINTER(OGeom)]
+    L.orc_compute_dsdt_multi_box_eb.argtypes = [ip, ip, F, F, F, F, dp, ctypes.POINTER(OEbFab), ctypes.c_int,
+                                                ctypes.POINTER(OParm)]
     L._multi_ready = True
     return L
 
@@ -561,6 +563,17 @@ def compute_dsdt_multi_box(lo, hi, S, Sp, p, dxinv, do_visc=0):
     L.orc_compute_dsdt_multi_box((ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi), ctypes.byref(S.c),
                                  ctypes.byref(Sp.c), ctypes.byref(D.c), ctypes.byref(Ds.c), _dp(dxi), do_visc,
                                  ctypes.byref(p))
+    return D.a, Ds.a
+
+
+def compute_dsdt_multi_box_eb(lo, hi, S, Sp, p, dxinv, eb, do_visc=0):
+    L = _multi_sigs()
+    D = Fab(lo, hi)
+    Ds = Fab(lo, hi, ncomp=NSPEC)
+    dxi = np.ascontiguousarray(dxinv, dtype=np.float64)
+    L.orc_compute_dsdt_multi_box_eb((ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi), ctypes.byref(S.c),
+                                    ctypes.byref(Sp.c), ctypes.byref(D.c), ctypes.byref(Ds.c), _dp(dxi),
+                                    ctypes.byref(eb.c), do_visc, ctypes.byref(p))
     return D.a, Ds.a
 
 
