@@ -322,6 +322,73 @@ int ocfd_graph_capture_steps(ocfd_level lev, double* A, double* B, double* C, oc
 int ocfd_graph_launch(ocfd_graph g, void* stream);
 int ocfd_graph_destroy(ocfd_graph g);
 
+/* ---------------------------------------------------------------------------
+ * AMR between two levels of refinement ratio 2 (regular levels).  The fine
+ * level's domain is the coarse domain refined by 2 and its boxes are aligned
+ * to 2.  The subcycled time step that strings these together is the host
+ * driver's (AmrLevel::timeStep semantics: one coarse step, then two fine steps
+ * of dt/2, then reflux and average-down; Source/EBR.cpp:231-330).
+ * ------------------------------------------------------------------------- */
+
+/* tag_dengrad (Source/Tagging.H:8-36) as called by EBR::errorEst
+ * (Source/Tagging.cpp:48-80): tags[cell] = 1 where the larger one-sided
+ * density jump >= threshold.  S has filled ghost cells; tags holds one byte per
+ * valid cell in the level's 0-ghost layout (box order, i fastest). */
+int ocfd_tag_dengrad(ocfd_level lev, const double* S, double threshold, unsigned char* tags, void* stream);
+
+/* New fine boxes from coarse tags (the AmrLevel regrid's grid_places that
+ * errorEst feeds; amr.blocking_factor / amr.grid_eff / amr.max_grid_size of
+ * Exec/<case>/inputs): tags dense over [lo, hi] (i fastest, host memory),
+ * coarsened by `block` cells, Berger-Rigoutsos clustering to efficiency
+ * grid_eff, chopped to max_size_cells.  Boxes (lo[3], hi[3]) in the tag index
+ * space go to boxes_out[6*cap]; *nbox_out = count.  Host only. */
+int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigned char* tags, int block, double grid_eff,
+                      int max_size_cells, int cap, int* boxes_out, int* nbox_out);
+
+/* out[0..n) = a*x + b*y: the two-time data combination of FillPatch at a time
+ * between a level's old and new states (a = (t1-t)/(t1-t0), b = (t-t0)/(t1-t0)). */
+int ocfd_lincomb(long n, double a, const double* x, double b, const double* y, double* out, void* stream);
+
+/* The coarse part of the two-level FillPatch (FillPatchTwoLevels with the
+ * state's cell-conservative linear interpolater, Source/Setup.cpp:196): after
+ * ocfd_fill_boundary(fine, S_fine, t), every fine ghost cell whose source cell
+ * no fine box owns is interpolated from S_coarse, the coarse state at time t
+ * with filled ghost cells. */
+int ocfd_fill_from_coarse(ocfd_level fine, double* S_fine, ocfd_level coarse, const double* S_coarse, void* stream);
+
+/* Valid cells of a new fine level interpolated from the coarse state (the
+ * same interpolater; AmrLevel::init of a regridded level), skipping cells the
+ * old fine level covers when old_fine is given. */
+int ocfd_interp_from_coarse(ocfd_level fine, double* S_fine, ocfd_level coarse, const double* S_coarse,
+                            ocfd_level old_fine, void* stream);
+
+/* Valid cells of level dst copied from level src (same domain) where src covers them. */
+int ocfd_copy_level(ocfd_level dst, double* S_dst, ocfd_level src, const double* S_src, void* stream);
+
+/* Flux register of one coarse/fine interface (EBFluxRegister of EBR,
+ * Source/EBR.H flux_reg; CrseAdd/FineAdd at Source/Advance.cpp:344-360,
+ * Reflux at Source/EBR.cpp:322-330). */
+typedef struct ocfd_cf_s* ocfd_cf;
+int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out);
+int ocfd_cf_destroy(ocfd_cf c);
+int ocfd_cf_nfaces(ocfd_cf c);
+/* interface faces as (coarse box, i, j, k, dir, high-face) sextuples */
+int ocfd_cf_faces(ocfd_cf c, int* out);
+/* zero both registers (flux_reg.reset(), Source/Advance.cpp:98-103) */
+int ocfd_cf_reset(ocfd_cf c, void* stream);
+/* After an RK stage of the coarse (fine_side 0) or fine (fine_side 1) level:
+ * add w_dt * F to every interface face, F recomputed from that stage's
+ * primitives (w_dt = the stage's dt weight: RK3 dt/6, dt/6, 2dt/3; RK2 dt/2;
+ * Euler dt).  flags as for the stage. */
+int ocfd_cf_add(ocfd_cf c, int fine_side, double w_dt, const ocfd_parm* parm, int flags, void* stream);
+/* copy the registers to host: regc[5*nface], regf[20*nface] (4 sub-faces per face) */
+int ocfd_cf_registers(ocfd_cf c, double* regc, double* regf);
+/* S_coarse += the flux mismatch of every interface face on the coarse side */
+int ocfd_cf_reflux(ocfd_cf c, double* S_coarse, void* stream);
+/* EBR::avgDown (Source/EBR.cpp:535-559): coarse cells under the fine level
+ * take the mean of their 8 fine cells; ncomp components (5 state, 9 species). */
+int ocfd_cf_avg_down(ocfd_cf c, const double* S_fine, double* S_coarse, int ncomp, void* stream);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ocfd_last_error(void);
 

@@ -1,0 +1,785 @@
+// AMR level machinery between two levels of refinement ratio 2:
+//   * density-gradient tagging, tag_dengrad (Source/Tagging.H:8-36);
+//   * the coarse part of the two-level FillPatch: fine ghost cells that no
+//     fine box owns are interpolated from the coarse level, already combined
+//     in time and ghost-filled at the stage time, with the cell-conservative
+//     linear interpolation of the state descriptor (Source/Setup.cpp:196);
+//   * the same interpolation for the valid cells of a new fine region, and the
+//     copy of an old level's cells into a new one (regrid);
+//   * average-down, EBR::avgDown (Source/EBR.cpp:535-559) with unit volumes;
+//   * the flux register: coarse and fine face fluxes of the coarse/fine
+//     interface weighted by the RK stage dt (the CrseAdd / FineAdd calls of
+//     Source/Advance.cpp:344-360) and the reflux of EBR::post_timestep
+//     (Source/EBR.cpp:322-330).
+//
+// The interface fluxes are recomputed at the interface faces only, from the
+// primitive scratch the stage's sweeps have just used, so the fused sweeps
+// carry no flux-register work.
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ocfd_device.hpp"
+#include "ocfd_internal.hpp"
+
+namespace ocfd {
+
+// ---------------------------------------------------------------------------
+// face flux of the regular path at one face
+// ---------------------------------------------------------------------------
+// Steger-Warming + WENO-Z5 (+ viscous) flux of the low face of the cell at
+// primitive index s (grown layout of box B) in direction D: the value the
+// sweeps compute for that face, with the same operations in the same order.
+template <int D, class FT, bool VISC>
+__device__ __forceinline__ void face_flux(const FT* __restrict__ P, long s, const BoxDesc& B, const PhysArgs& ph,
+                                          double* Fout)
+{
+    const long st = B.nptg;
+    const long ds = D == 0 ? 1 : (D == 1 ? (long)B.nxg : (long)B.nxg * B.nyg);
+    const FT gam = (FT)ph.gamma;
+    FT fp[6][5], fm[6][5];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        Prim<FT> q = ldprim<FT>(P, s + (c - 3) * ds, st);
+        sw_split<D, FT>(q, gam, fp[c], fm[c]);
+    }
+    FT F[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+        F[n] = weno_p<FT>(fp[0][n], fp[1][n], fp[2][n], fp[3][n], fp[4][n]) +
+               weno_m<FT>(fm[1][n], fm[2][n], fm[3][n], fm[4][n], fm[5][n]);
+    if (VISC) {
+        const long sL = s - ds;
+        const Prim<FT> L = ldprim<FT>(P, sL, st), R = ldprim<FT>(P, s, st);
+        // lateral directions a (first of x,y,z other than D) and b (second)
+        constexpr int A = D == 0 ? 1 : 0;
+        constexpr int Bd = D == 2 ? 1 : 2;
+        const long sa = A == 0 ? 1 : (long)B.nxg;
+        const long sb = Bd == 1 ? (long)B.nxg : (long)B.nxg * B.nyg;
+        // velocity components of the four 4-point differences:
+        //   x face: (u,v) along y, (u,w) along z
+        //   y face: (u,v) along x, (v,w) along z
+        //   z face: (u,w) along x, (v,w) along y
+        constexpr int ca = 0, cb = D == 2 ? 2 : 1, cc = D == 0 ? 0 : 1, cd = 2;
+        FT v[8][3];
+        const long off[8] = {s + sa, sL + sa, s - sa, sL - sa, s + sb, sL + sb, s - sb, sL - sb};
+#pragma unroll
+        for (int m = 0; m < 8; ++m) ldvel<FT>(P, off[m], st, v[m][0], v[m][1], v[m][2]);
+        const FT ta = lat(v[0][ca], v[1][ca], v[2][ca], v[3][ca]);
+        const FT tb = lat(v[0][cb], v[1][cb], v[2][cb], v[3][cb]);
+        const FT tc = lat(v[4][cc], v[5][cc], v[6][cc], v[7][cc]);
+        const FT td = lat(v[4][cd], v[5][cd], v[6][cd], v[7][cd]);
+        visc_face<D, FT>(L, R, ta, tb, tc, td, B.dxinv, ph, F);
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) Fout[n] = (double)F[n];
+}
+
+// one interface face, seen from the cell (box, i, j, k) that owns it on its
+// level: dir, and up = 1 when it is the high face of that cell
+struct CFFace {
+    int box, i, j, k;
+    int dir, up;
+    int pad0, pad1;
+};
+
+// reg[f] += w * F(face f) for every listed face; one thread per face, each
+// slot written by one thread per launch, so the sums are deterministic
+template <class FT, bool VISC>
+__global__ __launch_bounds__(128) void k_cf_flux(const BoxDesc* __restrict__ boxes, const CFFace* __restrict__ faces,
+                                                 int nface, const void* prim, PhysArgs ph, double w,
+                                                 double* __restrict__ reg)
+{
+    const int f = blockIdx.x * 128 + threadIdx.x;
+    if (f >= nface) return;
+    const CFFace c = faces[f];
+    const BoxDesc B = boxes[c.box];
+    const int i = c.i + (c.up && c.dir == 0), j = c.j + (c.up && c.dir == 1), k = c.k + (c.up && c.dir == 2);
+    const long s = sidx(B, i, j, k) - B.off_s + B.off_p;
+    const FT* P = (const FT*)prim;
+    double F[5];
+    if (c.dir == 0) face_flux<0, FT, VISC>(P, s, B, ph, F);
+    else if (c.dir == 1) face_flux<1, FT, VISC>(P, s, B, ph, F);
+    else face_flux<2, FT, VISC>(P, s, B, ph, F);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) reg[(long)f * 5 + n] += w * F[n];
+}
+
+__global__ void k_zero(double* __restrict__ p, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = 0.0;
+}
+
+// reflux: per coarse cell next to the fine level, over its interface faces in
+// the fixed order (x lo, x hi, y lo, y hi, z lo, z hi),
+//   U += -s * (mean of the 4 fine sub-face sums - coarse sum) * dxinv,
+// s = +1 for the cell's high face, -1 for its low face
+__global__ __launch_bounds__(128) void k_reflux(const BoxDesc* __restrict__ boxes, const int4* __restrict__ cells,
+                                                const int* __restrict__ cellfaces, int ncell,
+                                                const CFFace* __restrict__ cfaces, const double* __restrict__ regc,
+                                                const double* __restrict__ regf, double* __restrict__ S)
+{
+    const int c = blockIdx.x * 128 + threadIdx.x;
+    if (c >= ncell) return;
+    const int4 cl = cells[c];
+    const BoxDesc B = boxes[cl.x];
+    const long s0 = sidx(B, cl.y, cl.z, cl.w), st = B.nptg;
+    double U[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) U[n] = S[s0 + n * st];
+    for (int m = 0; m < 6; ++m) {
+        const int f = cellfaces[c * 6 + m];
+        if (f < 0) continue;
+        const CFFace F = cfaces[f];
+        const double sgn = F.up ? 1.0 : -1.0;
+        const double dxi = B.dxinv[F.dir];
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+            const double fine = (((regf[(4L * f + 0) * 5 + n] + regf[(4L * f + 1) * 5 + n]) + regf[(4L * f + 2) * 5 + n]) +
+                                 regf[(4L * f + 3) * 5 + n]) * 0.25;
+            U[n] += -sgn * (fine - regc[(long)f * 5 + n]) * dxi;
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) S[s0 + n * st] = U[n];
+}
+
+__device__ __forceinline__ int owner(const GeomDesc& G, int i, int j, int k)
+{
+    return G.map[((long)((k - G.dom_lo[2]) >> G.map_shift) * G.map_n[1] + ((j - G.dom_lo[1]) >> G.map_shift)) *
+                     G.map_n[0] + ((i - G.dom_lo[0]) >> G.map_shift)];
+}
+
+__device__ __forceinline__ void valid_cell(const BoxDesc& B, int c, int& i, int& j, int& k)
+{
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
+    i = B.lo[0] + c % nx;
+    j = B.lo[1] + (c / nx) % ny;
+    k = B.lo[2] + c / (nx * ny);
+}
+
+// average-down: every coarse cell under a fine box takes the volume-weighted
+// mean of its 8 fine cells (EB_average_down with unit volume fractions):
+// sum in i-fastest order, then divided by the summed weight 8.
+// tiles: (fine box, first coarse cell of the coarsened box, count, 0)
+__global__ __launch_bounds__(256) void k_avg_down(const BoxDesc* __restrict__ fboxes, const BoxDesc* __restrict__ cboxes,
+                                                  const GeomDesc* __restrict__ cgeom, const int4* __restrict__ tiles,
+                                                  const double* __restrict__ Sf, double* __restrict__ Sc, int ncomp)
+{
+    const int4 t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.z) return;
+    const BoxDesc Bf = fboxes[t.x];
+    const int nx = (Bf.hi[0] - Bf.lo[0] + 1) >> 1, ny = (Bf.hi[1] - Bf.lo[1] + 1) >> 1;
+    const int c = t.y + threadIdx.x;
+    const int I = (Bf.lo[0] >> 1) + c % nx, J = (Bf.lo[1] >> 1) + (c / nx) % ny, K = (Bf.lo[2] >> 1) + c / (nx * ny);
+    const int m = owner(*cgeom, I, J, K);
+    if (m < 0) return;
+    const BoxDesc Bc = cboxes[m];
+    const long dst = Bc.off_s / NC * ncomp + (sidx(Bc, I, J, K) - Bc.off_s);
+    const long src = Bf.off_s / NC * ncomp + (sidx(Bf, 2 * I, 2 * J, 2 * K) - Bf.off_s);
+    const long sy = Bf.nxg, sz = (long)Bf.nxg * Bf.nyg;
+    for (int n = 0; n < ncomp; ++n) {
+        const double* f = Sf + src + n * Bf.nptg;
+        double v = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 2; ++ii) v += f[kk * sz + jj * sy + ii];
+        Sc[dst + n * Bc.nptg] = v / 8.0;
+    }
+}
+
+// Central slope of one component along one direction; at a domain edge with an
+// external-Dirichlet BC, the third-order one-sided form that takes the ghost
+// value as the boundary-face value (the slopes of AMReX's cell-conservative
+// linear interpolation).
+__device__ __forceinline__ double cc_slope(const double* u, long s, long d, bool ext_lo, bool ext_hi)
+{
+    double dc = 0.5 * (u[s + d] - u[s - d]);
+    if (ext_lo) dc = -(16.0 / 15.0) * u[s - d] + 0.5 * u[s] + (2.0 / 3.0) * u[s + d] - 0.1 * u[s + 2 * d];
+    if (ext_hi) dc = (16.0 / 15.0) * u[s + d] - 0.5 * u[s] - (2.0 / 3.0) * u[s - d] + 0.1 * u[s - 2 * d];
+    return dc;
+}
+
+// MC limiting of a central slope by twice the one-sided differences
+__device__ __forceinline__ double mc_limit(double dc, double um, double u0, double up)
+{
+    const double df = 2.0 * (up - u0), db = 2.0 * (u0 - um);
+    const double s = (df * db >= 0.0) ? fmin(fabs(df), fabs(db)) : 0.0;
+    return copysign(1.0, dc) * fmin(s, fabs(dc));
+}
+
+// Cell-conservative linear interpolation (refinement ratio 2) of component n
+// of the coarse state C at fine cell (i,j,k): MC-limited central slopes, then
+// one factor alpha <= 1 that keeps every fine value of the parent within the
+// min/max of the parent's 27 coarse neighbours; fine value =
+// u + xoff*sx + yoff*sy + zoff*sz with offsets +-1/4.  Returns false when no
+// coarse box holds the parent cell.
+__device__ __forceinline__ bool cons_interp(const BoxDesc* __restrict__ cboxes, const GeomDesc& CG,
+                                            const double* __restrict__ C, int i, int j, int k, int n, double& v)
+{
+    const int I = i >> 1, J = j >> 1, K = k >> 1;
+    if (I < CG.dom_lo[0] || I > CG.dom_hi[0] || J < CG.dom_lo[1] || J > CG.dom_hi[1] || K < CG.dom_lo[2] ||
+        K > CG.dom_hi[2])
+        return false;
+    const int m = owner(CG, I, J, K);
+    if (m < 0) return false;
+    const BoxDesc Bc = cboxes[m];
+    const double* u = C + n * Bc.nptg;
+    const long s = sidx(Bc, I, J, K);
+    const long dy = Bc.nxg, dz = (long)Bc.nxg * Bc.nyg;
+    const long dd[3] = {1, dy, dz};
+    const int iv[3] = {I, J, K};
+    const double u0 = u[s];
+    double sl[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const bool elo = iv[d] == CG.dom_lo[d] && CG.bc[n][d][0] == BC_EXT && !CG.periodic[d];
+        const bool ehi = iv[d] == CG.dom_hi[d] && CG.bc[n][d][1] == BC_EXT && !CG.periodic[d];
+        const double dc = cc_slope(u, s, dd[d], elo, ehi);
+        sl[d] = mc_limit(dc, u[s - dd[d]], u0, u[s + dd[d]]);
+    }
+    double alpha = 1.0;
+    if (sl[0] != 0.0 || sl[1] != 0.0 || sl[2] != 0.0) {
+        const double dumax = fabs(sl[0]) * 0.25 + fabs(sl[1]) * 0.25 + fabs(sl[2]) * 0.25;
+        double umax = u0, umin = u0;
+        for (int kk = -1; kk <= 1; ++kk)
+            for (int jj = -1; jj <= 1; ++jj)
+                for (int ii = -1; ii <= 1; ++ii) {
+                    const double w = u[s + kk * dz + jj * dy + ii];
+                    umax = fmax(umax, w);
+                    umin = fmin(umin, w);
+                }
+        if (dumax * alpha > umax - u0) alpha = (umax - u0) / dumax;
+        if (dumax * alpha > u0 - umin) alpha = (u0 - umin) / dumax;
+    }
+    const double xo = (i & 1) ? 0.25 : -0.25, yo = (j & 1) ? 0.25 : -0.25, zo = (k & 1) ? 0.25 : -0.25;
+    v = u0 + xo * (sl[0] * alpha) + yo * (sl[1] * alpha) + zo * (sl[2] * alpha);
+    return true;
+}
+
+// the coarse part of the two-level ghost fill: fine ghost cells whose source
+// cell (after the periodic wrap and the BC map) no fine box owns take the
+// interpolated coarse value at that source; runs after the level's own fill,
+// which leaves exactly those cells untouched.  Cells the case's own boundary
+// function sets (external BCs, the DMR wall override) are left alone too.
+__global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fboxes, const GeomDesc* __restrict__ fgeom,
+                                                 const int4* __restrict__ tiles, double* __restrict__ Sf,
+                                                 const BoxDesc* __restrict__ cboxes, const GeomDesc* __restrict__ cgeom,
+                                                 const double* __restrict__ Sc)
+{
+    const int4 t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.z) return;
+    const BoxDesc B = fboxes[t.x];
+    const GeomDesc& G = *fgeom;
+    int i, j, k;
+    ghost_cell(B, t.y + threadIdx.x, i, j, k);
+    if (G.probtype == 1 && j < G.dom_lo[1] && G.bc[0][1][0] == BC_EVEN) {
+        // the DMR post-shock wall region x < 1/6 below y-lo is set by the case
+        const double x = G.prob_lo[0] + (i + 0.5) * G.dx[0];
+        if (x < 1.0 / 6.0) return;
+    }
+    const long dst = sidx(B, i, j, k);
+    int iv[3] = {i, j, k}, outside[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (G.periodic[d]) {
+            const int len = G.dom_hi[d] - G.dom_lo[d] + 1;
+            int r = (iv[d] - G.dom_lo[d]) % len;
+            if (r < 0) r += len;
+            iv[d] = G.dom_lo[d] + r;
+        }
+        outside[d] = iv[d] < G.dom_lo[d] ? -1 : (iv[d] > G.dom_hi[d] ? 1 : 0);
+    }
+    for (int n = 0; n < NC; ++n) {
+        int src[3] = {iv[0], iv[1], iv[2]};
+        double sign = 1.0;
+        bool user = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (!outside[d]) continue;
+            const int bc = G.bc[n][d][outside[d] > 0];
+            const int edge = outside[d] < 0 ? G.dom_lo[d] : G.dom_hi[d];
+            if (bc == BC_FOEXT) src[d] = edge;
+            else if (bc == BC_EVEN || bc == BC_ODD) {
+                src[d] = outside[d] < 0 ? 2 * edge - src[d] - 1 : 2 * edge - src[d] + 1;
+                if (bc == BC_ODD) sign = -sign;
+            } else if (bc == BC_EXT) user = true;
+        }
+        if (user) continue;
+        if (src[0] < G.dom_lo[0] || src[0] > G.dom_hi[0] || src[1] < G.dom_lo[1] || src[1] > G.dom_hi[1] ||
+            src[2] < G.dom_lo[2] || src[2] > G.dom_hi[2])
+            continue;
+        if (owner(G, src[0], src[1], src[2]) >= 0) continue;   // the level fill copied it
+        double v;
+        if (cons_interp(cboxes, *cgeom, Sc, src[0], src[1], src[2], n, v)) Sf[dst + n * B.nptg] = sign * v;
+    }
+}
+
+// valid cells of the fine level from the coarse level (a new fine region);
+// cells that the old fine level (when given) covers are skipped
+__global__ __launch_bounds__(256) void k_interp_valid(const BoxDesc* __restrict__ fboxes, const int4* __restrict__ tiles,
+                                                      double* __restrict__ Sf, const BoxDesc* __restrict__ cboxes,
+                                                      const GeomDesc* __restrict__ cgeom, const double* __restrict__ Sc,
+                                                      const BoxDesc* __restrict__ oboxes, const GeomDesc* __restrict__ ogeom)
+{
+    const int4 t = tiles[blockIdx.x];
+    const BoxDesc B = fboxes[t.x];
+    for (int c = threadIdx.x; c < t.z; c += 256) {
+        int i, j, k;
+        valid_cell(B, t.y + c, i, j, k);
+        if (ogeom) {
+            const int m = owner(*ogeom, i, j, k);
+            if (m >= 0) {
+                const BoxDesc& O = oboxes[m];
+                if (i >= O.lo[0] && i <= O.hi[0] && j >= O.lo[1] && j <= O.hi[1] && k >= O.lo[2] && k <= O.hi[2])
+                    continue;
+            }
+        }
+        const long dst = sidx(B, i, j, k);
+        for (int n = 0; n < NC; ++n) {
+            double v;
+            if (cons_interp(cboxes, *cgeom, Sc, i, j, k, n, v)) Sf[dst + n * B.nptg] = v;
+        }
+    }
+}
+
+// valid cells of a new level from an old level of the same resolution where
+// the old one covers them (regrid)
+__global__ __launch_bounds__(256) void k_copy_level(const BoxDesc* __restrict__ nboxes, const int4* __restrict__ tiles,
+                                                    double* __restrict__ Sn, const BoxDesc* __restrict__ oboxes,
+                                                    const GeomDesc* __restrict__ ogeom, const double* __restrict__ So)
+{
+    const int4 t = tiles[blockIdx.x];
+    const BoxDesc B = nboxes[t.x];
+    for (int c = threadIdx.x; c < t.z; c += 256) {
+        int i, j, k;
+        valid_cell(B, t.y + c, i, j, k);
+        const int m = owner(*ogeom, i, j, k);
+        if (m < 0) continue;
+        const BoxDesc O = oboxes[m];
+        if (i < O.lo[0] || i > O.hi[0] || j < O.lo[1] || j > O.hi[1] || k < O.lo[2] || k > O.hi[2]) continue;
+        const long dst = sidx(B, i, j, k), src = sidx(O, i, j, k);
+        for (int n = 0; n < NC; ++n) Sn[dst + n * B.nptg] = So[src + n * O.nptg];
+    }
+}
+
+// tag_dengrad, Source/Tagging.H:8-36: the larger one-sided density jump in
+// any direction >= threshold (ghost cells of S filled); covered boxes untagged
+__global__ __launch_bounds__(256) void k_tag(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+                                             const double* __restrict__ S, double thr, uint8_t* __restrict__ tags)
+{
+    const int4 t = tiles[blockIdx.x];
+    const BoxDesc B = boxes[t.x];
+    for (int c = threadIdx.x; c < t.z; c += 256) {
+        int i, j, k;
+        valid_cell(B, t.y + c, i, j, k);
+        uint8_t tg = 0;
+        if (B.type != 2) {
+            const long s = sidx(B, i, j, k);
+            const long sy = B.nxg, sz = (long)B.nxg * B.nyg;
+            const double r = S[s];
+            double ax = fabs(S[s + 1] - r), ay = fabs(S[s + sy] - r), az = fabs(S[s + sz] - r);
+            ax = fmax(ax, fabs(r - S[s - 1]));
+            ay = fmax(ay, fabs(r - S[s - sy]));
+            az = fmax(az, fabs(r - S[s - sz]));
+            tg = fmax(ax, fmax(ay, az)) >= thr ? 1 : 0;
+        }
+        tags[B.off_d / NC + (didx(B, i, j, k) - B.off_d)] = tg;
+    }
+}
+
+// out = a*x + b*y, the two-time combination of FillPatchSingleLevel
+__global__ void k_lincomb(long n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
+                          double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) out[i] = a * x[i] + b * y[i];
+}
+
+} // namespace ocfd
+
+using namespace ocfd;
+
+// ---------------------------------------------------------------------------
+// host side: the coarse/fine interface and the C ABI
+// ---------------------------------------------------------------------------
+struct ocfd_cf_s {
+    ocfd_level coarse = nullptr, fine = nullptr;
+    int nface = 0, ncell = 0;
+    CFFace* d_cf = nullptr;      // coarse faces
+    CFFace* d_ff = nullptr;      // 4 fine sub-faces per coarse face
+    int4* d_cells = nullptr;     // coarse cells next to the fine level (box, i, j, k)
+    int* d_cellfaces = nullptr;  // 6 face indices per cell, -1 = none
+    double* d_regc = nullptr;    // 5 per coarse face
+    double* d_regf = nullptr;    // 5 per fine sub-face
+    int4* d_tile_avg = nullptr;  // coarsened fine boxes, 256 coarse cells per tile
+    int ntile_avg = 0;
+};
+
+namespace {
+
+int find_box(const LevelDev& L, int i, int j, int k)
+{
+    const GeomDesc& G = L.geom;
+    if (i < G.dom_lo[0] || i > G.dom_hi[0] || j < G.dom_lo[1] || j > G.dom_hi[1] || k < G.dom_lo[2] || k > G.dom_hi[2])
+        return -1;
+    const int m = L.owner_map[((size_t)((k - G.dom_lo[2]) >> G.map_shift) * G.map_n[1] +
+                               ((j - G.dom_lo[1]) >> G.map_shift)) * G.map_n[0] + ((i - G.dom_lo[0]) >> G.map_shift)];
+    if (m < 0) return -1;
+    const BoxDesc& B = L.boxes[m];
+    if (i < B.lo[0] || i > B.hi[0] || j < B.lo[1] || j > B.hi[1] || k < B.lo[2] || k > B.hi[2]) return -1;
+    return m;
+}
+
+int wrap(const GeomDesc& G, int d, int v)
+{
+    if (!G.periodic[d]) return v;
+    const int len = G.dom_hi[d] - G.dom_lo[d] + 1;
+    return G.dom_lo[d] + (((v - G.dom_lo[d]) % len) + len) % len;
+}
+
+template <class T>
+int upload_vec(T** d, const std::vector<T>& h)
+{
+    *d = nullptr;
+    if (h.empty()) return OCFD_OK;
+    if (hipMalloc((void**)d, sizeof(T) * h.size()) != hipSuccess) return OCFD_ENOMEM;
+    if (hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice) != hipSuccess) return OCFD_EHIP;
+    return OCFD_OK;
+}
+
+int launched(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return OCFD_EHIP;
+    }
+    return OCFD_OK;
+}
+
+void free_cf(ocfd_cf c)
+{
+    hipFree(c->d_cf); hipFree(c->d_ff); hipFree(c->d_cells); hipFree(c->d_cellfaces);
+    hipFree(c->d_regc); hipFree(c->d_regf); hipFree(c->d_tile_avg);
+}
+
+bool refined_by_2(const LevelDev& C, const LevelDev& F)
+{
+    for (int d = 0; d < 3; ++d)
+        if (F.geom.dom_lo[d] != 2 * C.geom.dom_lo[d] || F.geom.dom_hi[d] != 2 * C.geom.dom_hi[d] + 1 ||
+            F.geom.periodic[d] != C.geom.periodic[d])
+            return false;
+    return true;
+}
+
+bool aligned_by_2(const LevelDev& F)
+{
+    for (const BoxDesc& B : F.boxes)
+        for (int d = 0; d < 3; ++d)
+            if ((B.lo[d] & 1) || !(B.hi[d] & 1)) return false;
+    return true;
+}
+
+} // namespace
+
+extern "C" {
+
+int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
+{
+    if (!coarse || !fine || !out) {
+        set_error("ocfd_cf_create: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    const LevelDev& C = coarse->dev;
+    const LevelDev& F = fine->dev;
+    if (!refined_by_2(C, F) || !aligned_by_2(F)) {
+        set_error("ocfd_cf_create: the fine level must refine the coarse domain by 2 with boxes aligned to 2");
+        return OCFD_EINVAL;
+    }
+    if (C.has_eb || F.has_eb) {
+        set_error("ocfd_cf_create: the flux register covers regular levels only");
+        return OCFD_EUNSUPPORTED;
+    }
+    // 1 = covered by the fine level, 0 = not, -1 = outside a non-periodic domain side
+    auto covered = [&](int I, int J, int K) {
+        int iv[3] = {I, J, K};
+        for (int d = 0; d < 3; ++d) {
+            iv[d] = wrap(C.geom, d, iv[d]);
+            if (iv[d] < C.geom.dom_lo[d] || iv[d] > C.geom.dom_hi[d]) return -1;
+        }
+        return find_box(F, 2 * iv[0], 2 * iv[1], 2 * iv[2]) >= 0 ? 1 : 0;
+    };
+    std::vector<CFFace> cf, ff;
+    std::vector<int4> cells;
+    std::vector<int> cellfaces;
+    for (int b = 0; b < C.nbox; ++b) {
+        const BoxDesc& B = C.boxes[b];
+        for (int K = B.lo[2]; K <= B.hi[2]; ++K)
+            for (int J = B.lo[1]; J <= B.hi[1]; ++J)
+                for (int I = B.lo[0]; I <= B.hi[0]; ++I) {
+                    if (covered(I, J, K) != 0) continue;
+                    int fidx[6] = {-1, -1, -1, -1, -1, -1};
+                    for (int d = 0; d < 3; ++d)
+                        for (int side = 0; side < 2; ++side) {
+                            int nb[3] = {I, J, K};
+                            nb[d] += side ? 1 : -1;
+                            if (covered(nb[0], nb[1], nb[2]) != 1) continue;
+                            CFFace f{};
+                            f.box = b; f.i = I; f.j = J; f.k = K; f.dir = d; f.up = side;
+                            fidx[2 * d + side] = (int)cf.size();
+                            cf.push_back(f);
+                            // the 4 fine cells across the face: the first fine layer of the neighbour
+                            int fc[3] = {2 * I, 2 * J, 2 * K};
+                            fc[d] = side ? 2 * nb[d] : 2 * nb[d] + 1;
+                            const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+                            for (int a2 = 0; a2 < 2; ++a2)
+                                for (int a1 = 0; a1 < 2; ++a1) {
+                                    int g[3] = {fc[0], fc[1], fc[2]};
+                                    g[t1] += a1;
+                                    g[t2] += a2;
+                                    for (int e = 0; e < 3; ++e) g[e] = wrap(F.geom, e, g[e]);
+                                    const int fb = find_box(F, g[0], g[1], g[2]);
+                                    if (fb < 0) {
+                                        set_error("ocfd_cf_create: a fine cell across an interface face has no box");
+                                        return OCFD_EINVAL;
+                                    }
+                                    CFFace h{};
+                                    h.box = fb; h.i = g[0]; h.j = g[1]; h.k = g[2]; h.dir = d;
+                                    h.up = side ? 0 : 1;   // fine level above the face: its low face
+                                    ff.push_back(h);
+                                }
+                        }
+                    bool any = false;
+                    for (int m = 0; m < 6; ++m) any = any || fidx[m] >= 0;
+                    if (any) {
+                        cells.push_back(make_int4(b, I, J, K));
+                        cellfaces.insert(cellfaces.end(), fidx, fidx + 6);
+                    }
+                }
+    }
+    std::vector<int4> ta;
+    for (int b = 0; b < F.nbox; ++b) {
+        const BoxDesc& B = F.boxes[b];
+        const long n = (long)((B.hi[0] - B.lo[0] + 1) / 2) * ((B.hi[1] - B.lo[1] + 1) / 2) * ((B.hi[2] - B.lo[2] + 1) / 2);
+        for (long s = 0; s < n; s += 256) ta.push_back(make_int4(b, (int)s, (int)std::min<long>(256, n - s), 0));
+    }
+    ocfd_cf c = new ocfd_cf_s;
+    c->coarse = coarse; c->fine = fine;
+    c->nface = (int)cf.size(); c->ncell = (int)cells.size();
+    c->ntile_avg = (int)ta.size();
+    int rc;
+    if ((rc = upload_vec(&c->d_cf, cf)) || (rc = upload_vec(&c->d_ff, ff)) || (rc = upload_vec(&c->d_cells, cells)) ||
+        (rc = upload_vec(&c->d_cellfaces, cellfaces)) || (rc = upload_vec(&c->d_tile_avg, ta))) {
+        set_error("ocfd_cf_create: device allocation failed");
+        free_cf(c);
+        delete c;
+        return rc;
+    }
+    if (c->nface && (hipMalloc((void**)&c->d_regc, sizeof(double) * 5 * c->nface) != hipSuccess ||
+                     hipMalloc((void**)&c->d_regf, sizeof(double) * 20 * c->nface) != hipSuccess)) {
+        set_error("ocfd_cf_create: out of device memory");
+        free_cf(c);
+        delete c;
+        return OCFD_ENOMEM;
+    }
+    *out = c;
+    return OCFD_OK;
+}
+
+int ocfd_cf_destroy(ocfd_cf c)
+{
+    if (!c) return OCFD_OK;
+    free_cf(c);
+    delete c;
+    return OCFD_OK;
+}
+
+int ocfd_cf_nfaces(ocfd_cf c) { return c ? c->nface : -1; }
+
+int ocfd_cf_faces(ocfd_cf c, int* out)
+{
+    if (!c || !out) {
+        set_error("ocfd_cf_faces: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    std::vector<CFFace> h(c->nface);
+    if (c->nface && hipMemcpy(h.data(), c->d_cf, sizeof(CFFace) * c->nface, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("ocfd_cf_faces: copy failed");
+        return OCFD_EHIP;
+    }
+    for (int f = 0; f < c->nface; ++f) {
+        const CFFace& x = h[f];
+        const int v[6] = {x.box, x.i, x.j, x.k, x.dir, x.up};
+        std::copy(v, v + 6, out + 6 * f);
+    }
+    return OCFD_OK;
+}
+
+int ocfd_cf_reset(ocfd_cf c, void* stream)
+{
+    if (!c) {
+        set_error("ocfd_cf_reset: null register");
+        return OCFD_EINVAL;
+    }
+    if (c->nface) {
+        hipLaunchKernelGGL(k_zero, dim3(std::min(1024, (c->nface * 5 + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                           c->d_regc, (long)c->nface * 5);
+        hipLaunchKernelGGL(k_zero, dim3(std::min(1024, (c->nface * 20 + 255) / 256)), dim3(256), 0,
+                           (hipStream_t)stream, c->d_regf, (long)c->nface * 20);
+    }
+    return launched("ocfd_cf_reset");
+}
+
+int ocfd_cf_add(ocfd_cf c, int fine_side, double w_dt, const ocfd_parm* parm, int flags, void* stream)
+{
+    if (!c || !parm || (fine_side != 0 && fine_side != 1)) {
+        set_error("ocfd_cf_add: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    if (!c->nface) return OCFD_OK;
+    const LevelDev& L = fine_side ? c->fine->dev : c->coarse->dev;
+    const CFFace* faces = fine_side ? c->d_ff : c->d_cf;
+    double* reg = fine_side ? c->d_regf : c->d_regc;
+    const int n = fine_side ? 4 * c->nface : c->nface;
+    const PhysArgs ph = phys_of(*parm);
+    const bool f32 = (flags & OCFD_FLUX_FP32) != 0, visc = (flags & OCFD_VISC) != 0;
+    const void* prim = f32 ? (const void*)L.d_prim32 : (const void*)L.d_prim64;
+    const dim3 g((n + 127) / 128), blk(128);
+    hipStream_t s = (hipStream_t)stream;
+    if (f32) {
+        if (visc) hipLaunchKernelGGL((k_cf_flux<float, true>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg);
+        else hipLaunchKernelGGL((k_cf_flux<float, false>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg);
+    } else {
+        if (visc) hipLaunchKernelGGL((k_cf_flux<double, true>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg);
+        else hipLaunchKernelGGL((k_cf_flux<double, false>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg);
+    }
+    return launched("ocfd_cf_add");
+}
+
+int ocfd_cf_registers(ocfd_cf c, double* regc, double* regf)
+{
+    if (!c || !regc || !regf) {
+        set_error("ocfd_cf_registers: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    if (c->nface && (hipMemcpy(regc, c->d_regc, sizeof(double) * 5 * c->nface, hipMemcpyDeviceToHost) != hipSuccess ||
+                     hipMemcpy(regf, c->d_regf, sizeof(double) * 20 * c->nface, hipMemcpyDeviceToHost) != hipSuccess)) {
+        set_error("ocfd_cf_registers: copy failed");
+        return OCFD_EHIP;
+    }
+    return OCFD_OK;
+}
+
+int ocfd_cf_reflux(ocfd_cf c, double* S_coarse, void* stream)
+{
+    if (!c || !S_coarse) {
+        set_error("ocfd_cf_reflux: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    if (c->ncell)
+        hipLaunchKernelGGL(k_reflux, dim3((c->ncell + 127) / 128), dim3(128), 0, (hipStream_t)stream,
+                           c->coarse->dev.d_boxes, c->d_cells, c->d_cellfaces, c->ncell, c->d_cf, c->d_regc, c->d_regf,
+                           S_coarse);
+    return launched("ocfd_cf_reflux");
+}
+
+int ocfd_cf_avg_down(ocfd_cf c, const double* S_fine, double* S_coarse, int ncomp, void* stream)
+{
+    if (!c || !S_fine || !S_coarse || ncomp <= 0) {
+        set_error("ocfd_cf_avg_down: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    if (c->ntile_avg)
+        hipLaunchKernelGGL(k_avg_down, dim3(c->ntile_avg), dim3(256), 0, (hipStream_t)stream, c->fine->dev.d_boxes,
+                           c->coarse->dev.d_boxes, c->coarse->dev.d_geom, c->d_tile_avg, S_fine, S_coarse, ncomp);
+    return launched("ocfd_cf_avg_down");
+}
+
+int ocfd_fill_from_coarse(ocfd_level fine, double* S_fine, ocfd_level coarse, const double* S_coarse, void* stream)
+{
+    if (!fine || !coarse || !S_fine || !S_coarse) {
+        set_error("ocfd_fill_from_coarse: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    const LevelDev& F = fine->dev;
+    const LevelDev& C = coarse->dev;
+    if (!refined_by_2(C, F) || !aligned_by_2(F)) {
+        set_error("ocfd_fill_from_coarse: the fine level must refine the coarse domain by 2 with boxes aligned to 2");
+        return OCFD_EINVAL;
+    }
+    if (F.ntile_g)
+        hipLaunchKernelGGL(k_fill_cf, dim3(F.ntile_g), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_geom, F.d_tile_g,
+                           S_fine, C.d_boxes, C.d_geom, S_coarse);
+    return launched("ocfd_fill_from_coarse");
+}
+
+int ocfd_interp_from_coarse(ocfd_level fine, double* S_fine, ocfd_level coarse, const double* S_coarse,
+                            ocfd_level old_fine, void* stream)
+{
+    if (!fine || !coarse || !S_fine || !S_coarse) {
+        set_error("ocfd_interp_from_coarse: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    const LevelDev& F = fine->dev;
+    const LevelDev& C = coarse->dev;
+    if (!refined_by_2(C, F) || !aligned_by_2(F) || (old_fine && !refined_by_2(C, old_fine->dev))) {
+        set_error("ocfd_interp_from_coarse: the fine levels must refine the coarse domain by 2");
+        return OCFD_EINVAL;
+    }
+    if (F.ntile_c)
+        hipLaunchKernelGGL(k_interp_valid, dim3(F.ntile_c), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_tile_c,
+                           S_fine, C.d_boxes, C.d_geom, S_coarse, old_fine ? old_fine->dev.d_boxes : nullptr,
+                           old_fine ? old_fine->dev.d_geom : nullptr);
+    return launched("ocfd_interp_from_coarse");
+}
+
+int ocfd_copy_level(ocfd_level dst, double* S_dst, ocfd_level src, const double* S_src, void* stream)
+{
+    if (!dst || !src || !S_dst || !S_src) {
+        set_error("ocfd_copy_level: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    const LevelDev& N = dst->dev;
+    const LevelDev& O = src->dev;
+    for (int d = 0; d < 3; ++d)
+        if (N.geom.dom_lo[d] != O.geom.dom_lo[d] || N.geom.dom_hi[d] != O.geom.dom_hi[d]) {
+            set_error("ocfd_copy_level: the levels must share the domain");
+            return OCFD_EINVAL;
+        }
+    if (N.ntile_c)
+        hipLaunchKernelGGL(k_copy_level, dim3(N.ntile_c), dim3(256), 0, (hipStream_t)stream, N.d_boxes, N.d_tile_c,
+                           S_dst, O.d_boxes, O.d_geom, S_src);
+    return launched("ocfd_copy_level");
+}
+
+int ocfd_tag_dengrad(ocfd_level lev, const double* S, double threshold, unsigned char* tags, void* stream)
+{
+    if (!lev || !S || !tags) {
+        set_error("ocfd_tag_dengrad: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    const LevelDev& L = lev->dev;
+    if (L.ntile_c)
+        hipLaunchKernelGGL(k_tag, dim3(L.ntile_c), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_tile_c, S,
+                           threshold, (uint8_t*)tags);
+    return launched("ocfd_tag_dengrad");
+}
+
+int ocfd_lincomb(long n, double a, const double* x, double b, const double* y, double* out, void* stream)
+{
+    if (n < 0 || (n > 0 && (!x || !y || !out))) {
+        set_error("ocfd_lincomb: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    if (n)
+        hipLaunchKernelGGL(k_lincomb, dim3((unsigned)std::min<long>(4096, (n + 255) / 256)), dim3(256), 0,
+                           (hipStream_t)stream, n, a, x, b, y, out);
+    return launched("ocfd_lincomb");
+}
+
+} // extern "C"

@@ -1,0 +1,221 @@
+// Fine-grid generation from tagged coarse cells (the grid_places step of an
+// AMR regrid, which EBR's errorEst feeds: Source/Tagging.cpp:10-81):
+// tagged cells are coarsened by the blocking factor, grouped by
+// Berger-Rigoutsos signature clustering until every cluster reaches the grid
+// efficiency (the cluster/cut rules of AMReX's ClusterList: hole cuts first,
+// then the strongest inflection of the signature's second difference, then
+// bisection; among equal kinds the direction whose smaller part is largest),
+// refined back by the blocking factor and chopped to the maximum grid size
+// (balanced pieces, AMReX BoxList::maxSize).  Host code; no device work.
+#include <algorithm>
+#include <cstdlib>
+#include <list>
+#include <string>
+#include <vector>
+
+#include "ocfd.h"
+
+namespace ocfd {
+void set_error(const std::string& msg);
+}
+
+namespace {
+
+struct IBox {
+    int lo[3], hi[3];
+    long volume() const { return (long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1); }
+};
+
+struct Pt {
+    int v[3];
+};
+
+enum CutKind { HOLE = 0, INFLECTION = 1, BISECT = 2, NONE = 3 };
+
+struct Cluster {
+    std::vector<Pt> pts;
+    IBox box{};
+    void min_box()
+    {
+        for (int d = 0; d < 3; ++d) {
+            box.lo[d] = pts[0].v[d];
+            box.hi[d] = pts[0].v[d];
+        }
+        for (const Pt& p : pts)
+            for (int d = 0; d < 3; ++d) {
+                box.lo[d] = std::min(box.lo[d], p.v[d]);
+                box.hi[d] = std::max(box.hi[d], p.v[d]);
+            }
+    }
+    double eff() const { return (double)pts.size() / (double)box.volume(); }
+};
+
+// cut index along one direction from the signature hist[0..len-1] over [lo, lo+len-1]
+int find_cut(const std::vector<int>& hist, int lo, CutKind& kind)
+{
+    const int MINOFF = 2, CUT_THRESH = 2;
+    kind = NONE;
+    const int len = (int)hist.size();
+    if (len <= 1) return lo;
+    const int mid = len / 2;
+    int cut = -1;
+    for (int i = 0; i < len; ++i) {
+        if (hist[i] == 0) {
+            kind = HOLE;
+            if (std::abs(cut - mid) > std::abs(i - mid)) {
+                cut = i;
+                if (i > mid) break;
+            }
+        }
+    }
+    if (kind == HOLE) return lo + cut;
+    std::vector<int> d2(len, 0);
+    for (int i = 1; i < len - 1; ++i) d2[i] = hist[i + 1] - 2 * hist[i] + hist[i - 1];
+    int locmax = -1;
+    for (int i = MINOFF; i < len - MINOFF; ++i) {
+        const int prev = d2[i - 1], cur = d2[i];
+        const int dif = std::abs(prev - cur);
+        if (prev * cur < 0 && dif >= locmax) {
+            if (dif > locmax) {
+                kind = INFLECTION;
+                cut = i;
+                locmax = dif;
+            } else if (std::abs(i - mid) < std::abs(cut - mid)) {
+                cut = i;
+            }
+        }
+    }
+    if (locmax <= CUT_THRESH) {
+        cut = mid;
+        kind = BISECT;
+    }
+    return lo + cut;
+}
+
+// split cluster c in two; returns the high part, c keeps the low part
+Cluster chop(Cluster& c)
+{
+    std::vector<int> hist[3];
+    for (int d = 0; d < 3; ++d) hist[d].assign(c.box.hi[d] - c.box.lo[d] + 1, 0);
+    for (const Pt& p : c.pts)
+        for (int d = 0; d < 3; ++d) hist[d][p.v[d] - c.box.lo[d]]++;
+    CutKind kind[3];
+    int cut[3];
+    CutKind best = NONE;
+    for (int d = 0; d < 3; ++d) {
+        cut[d] = find_cut(hist[d], c.box.lo[d], kind[d]);
+        best = std::min(best, kind[d]);
+    }
+    int dir = -1, minlen = -1;
+    for (int d = 0; d < 3; ++d) {
+        if (kind[d] != best) continue;
+        const int m = std::min(cut[d] - c.box.lo[d], c.box.hi[d] - cut[d]);
+        if (m >= minlen) {
+            dir = d;
+            minlen = m;
+        }
+    }
+    Cluster hi;
+    std::vector<Pt> lo;
+    for (const Pt& p : c.pts) (p.v[dir] < cut[dir] ? lo : hi.pts).push_back(p);
+    c.pts.swap(lo);
+    c.min_box();
+    hi.min_box();
+    return hi;
+}
+
+// chop every box of the list to at most `chunk` cells per direction, in
+// balanced pieces taken from the high end
+void max_size(std::vector<IBox>& boxes, int chunk)
+{
+    for (int d = 0; d < 3; ++d) {
+        std::vector<IBox> out;
+        for (IBox b : boxes) {
+            const int len = b.hi[d] - b.lo[d] + 1;
+            if (len > chunk) {
+                int ratio = 1, bs = chunk, nlen = len;
+                while (bs % 2 == 0 && nlen % 2 == 0) {
+                    ratio *= 2;
+                    bs /= 2;
+                    nlen /= 2;
+                }
+                const int numblk = nlen / bs + (nlen % bs ? 1 : 0);
+                const int sz = nlen / numblk, extra = nlen % numblk;
+                for (int k = 0; k < numblk - 1; ++k) {
+                    const int ksize = (k < extra ? sz + 1 : sz) * ratio;
+                    IBox top = b;
+                    top.lo[d] = b.hi[d] - ksize + 1;
+                    b.hi[d] = top.lo[d] - 1;
+                    out.push_back(top);
+                }
+            }
+            out.push_back(b);
+        }
+        boxes.swap(out);
+    }
+}
+
+} // namespace
+
+extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigned char* tags, int block,
+                                 double grid_eff, int max_size_cells, int cap, int* boxes_out, int* nbox_out)
+{
+    using ocfd::set_error;
+    if (!lo || !hi || !tags || !nbox_out || block < 1 || max_size_cells < block || max_size_cells % block ||
+        grid_eff <= 0.0 || grid_eff > 1.0 || cap < 0 || (cap > 0 && !boxes_out)) {
+        set_error("ocfd_cluster_tags: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    int n[3];
+    for (int d = 0; d < 3; ++d) {
+        n[d] = hi[d] - lo[d] + 1;
+        if (n[d] <= 0 || (lo[d] % block) || (n[d] % block)) {
+            set_error("ocfd_cluster_tags: the tag box must be non-empty and aligned to the blocking factor");
+            return OCFD_EINVAL;
+        }
+    }
+    // tagged points at blocking-factor resolution
+    const int nb[3] = {n[0] / block, n[1] / block, n[2] / block};
+    std::vector<unsigned char> bt((size_t)nb[0] * nb[1] * nb[2], 0);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i)
+                if (tags[((size_t)k * n[1] + j) * n[0] + i])
+                    bt[((size_t)(k / block) * nb[1] + j / block) * nb[0] + i / block] = 1;
+    Cluster all;
+    for (int k = 0; k < nb[2]; ++k)
+        for (int j = 0; j < nb[1]; ++j)
+            for (int i = 0; i < nb[0]; ++i)
+                if (bt[((size_t)k * nb[1] + j) * nb[0] + i])
+                    all.pts.push_back(Pt{{lo[0] / block + i, lo[1] / block + j, lo[2] / block + k}});
+    std::vector<IBox> boxes;
+    if (!all.pts.empty()) {
+        all.min_box();
+        std::list<Cluster> lst;
+        lst.push_back(std::move(all));
+        for (auto it = lst.begin(); it != lst.end();) {
+            if (it->eff() < grid_eff) lst.push_back(chop(*it));
+            else ++it;
+        }
+        for (const Cluster& c : lst) {
+            IBox b = c.box;
+            for (int d = 0; d < 3; ++d) {
+                b.lo[d] *= block;
+                b.hi[d] = (b.hi[d] + 1) * block - 1;
+            }
+            boxes.push_back(b);
+        }
+        max_size(boxes, max_size_cells);
+    }
+    *nbox_out = (int)boxes.size();
+    if ((int)boxes.size() > cap) {
+        set_error("ocfd_cluster_tags: more boxes than the output capacity");
+        return OCFD_EINVAL;
+    }
+    for (size_t b = 0; b < boxes.size(); ++b)
+        for (int d = 0; d < 3; ++d) {
+            boxes_out[6 * b + d] = boxes[b].lo[d];
+            boxes_out[6 * b + 3 + d] = boxes[b].hi[d];
+        }
+    return OCFD_OK;
+}

@@ -152,6 +152,22 @@ def lib():
         L.ocfd_advance_chem.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, vp]
         L.ocfd_init_species_state.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(Parm)]
+        L.ocfd_tag_dengrad.argtypes = [vp, vp, ctypes.c_double, vp, vp]
+        L.ocfd_cluster_tags.argtypes = [ip, ip, ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_int, ip, ip]
+        L.ocfd_lincomb.argtypes = [ctypes.c_long, ctypes.c_double, vp, ctypes.c_double, vp, vp, vp]
+        L.ocfd_fill_from_coarse.argtypes = [vp, vp, vp, vp, vp]
+        L.ocfd_interp_from_coarse.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.ocfd_copy_level.argtypes = [vp, vp, vp, vp, vp]
+        L.ocfd_cf_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
+        L.ocfd_cf_destroy.argtypes = [vp]
+        L.ocfd_cf_nfaces.argtypes = [vp]
+        L.ocfd_cf_faces.argtypes = [vp, ip]
+        L.ocfd_cf_reset.argtypes = [vp, vp]
+        L.ocfd_cf_add.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Parm), ctypes.c_int, vp]
+        L.ocfd_cf_registers.argtypes = [vp, dp, dp]
+        L.ocfd_cf_reflux.argtypes = [vp, vp, vp]
+        L.ocfd_cf_avg_down.argtypes = [vp, vp, vp, ctypes.c_int, vp]
         L.ocfd_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib

@@ -1,0 +1,216 @@
+"""Pins of the AMR oracle (oracle/amr_oracle.py) and of the host-side grid
+generation of the library.
+
+AMReX is not vendored with the reference and no reference test covers its AMR
+machinery, so the oracle is pinned by what the mathematics fixes:
+* the cell-conservative linear interpolation reproduces linear data exactly,
+  keeps every fine value inside the min/max of the parent's coarse
+  neighbours, conserves the parent's value, and its one-sided boundary slope
+  is the exact derivative of any quadratic whose ghost value is the face value;
+* average-down of the interpolation gives the coarse field back;
+* the grid generator covers every tag with disjoint boxes, stops at the grid
+  efficiency, and finds the obvious decompositions (separate blobs, an L);
+* reflux makes the two-level periodic advance conserve mass, momentum and
+  energy to rounding (and without reflux it does not);
+* a fine level covering the whole domain reproduces the uniform fine run bit
+  for bit, which pins the subcycled stage sequence and time interpolation.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import amr_oracle as ao
+
+
+def _og(orc, n, periodic=(1, 1, 1), lo_bc=(0, 0, 0), hi_bc=(0, 0, 0), L=1.0):
+    return orc.geom(n, (0.0, 0.0, 0.0), tuple(L * n[d] / n[0] for d in range(3)), periodic=periodic, lo_bc=lo_bc,
+                    hi_bc=hi_bc)
+
+
+def _dense_from_func(g, fn, ncomp=5):
+    lo, hi = ao.dom(g)
+    n = [hi[d] - lo[d] + 1 for d in range(3)]
+    K, J, I = np.meshgrid(*(np.arange(-4, n[a] + 4) for a in (2, 1, 0)), indexing="ij")
+    return np.stack([fn(I, J, K, c) for c in range(ncomp)])
+
+
+def test_interp_reproduces_linear_data(orc):
+    g = _og(orc, (8, 6, 4), periodic=(0, 0, 0), lo_bc=(2, 2, 2), hi_bc=(2, 2, 2))
+    coef = np.array([[1.0, 0.3, -0.2, 0.05], [2.0, -0.1, 0.4, 0.0], [0.5, 0.0, 0.0, 0.7], [3.0, 1.0, 1.0, 1.0],
+                     [-1.0, 0.2, 0.1, -0.3]])
+    C = _dense_from_func(g, lambda I, J, K, c: coef[c, 0] + coef[c, 1] * I + coef[c, 2] * J + coef[c, 3] * K)
+    F = ao.interp_dense(C, g)
+    k, j, i = np.meshgrid(*(np.arange(F.shape[a]) for a in (1, 2, 3)), indexing="ij")
+    xc, yc, zc = (i + 0.5) / 2 - 0.5, (j + 0.5) / 2 - 0.5, (k + 0.5) / 2 - 0.5
+    for c in range(5):
+        want = coef[c, 0] + coef[c, 1] * xc + coef[c, 2] * yc + coef[c, 3] * zc
+        assert np.abs(F[c] - want).max() < 1e-13
+
+
+def test_interp_conserves_and_stays_bounded(orc):
+    g = _og(orc, (6, 6, 6))
+    rng = np.random.default_rng(5)
+    C = rng.uniform(0.5, 2.0, (5, 14, 14, 14))
+    F = ao.interp_dense(C, g)
+    mean = sum(F[:, kk::2, jj::2, ii::2] for kk in (0, 1) for jj in (0, 1) for ii in (0, 1)) / 8.0
+    assert np.abs(mean - C[:, 4:-4, 4:-4, 4:-4]).max() < 1e-14
+    nb = np.lib.stride_tricks.sliding_window_view(C[:, 3:-3, 3:-3, 3:-3], (3, 3, 3), axis=(1, 2, 3))
+    umax = nb.max(axis=(-1, -2, -3)).repeat(2, 1).repeat(2, 2).repeat(2, 3)
+    umin = nb.min(axis=(-1, -2, -3)).repeat(2, 1).repeat(2, 2).repeat(2, 3)
+    assert (F <= umax + 1e-14).all() and (F >= umin - 1e-14).all()
+
+
+def test_interp_one_sided_slope_at_external_boundary(orc):
+    """x-lo inflow (ext_dir): u = -x^2 + 4x with the ghost holding the face
+    value; the boundary cell's children are u0 -+ u'(0)/4 exactly (MC and
+    the min/max limiter inactive for this profile), where a central slope
+    would give (u(1)-u(-1/2))/2 = 2.625 instead of 4."""
+    g = _og(orc, (6, 4, 4), periodic=(0, 1, 1), lo_bc=(1, 0, 0), hi_bc=(2, 0, 0))
+
+    def fn(I, J, K, c):
+        x = np.where(I < 0, -0.5 * np.ones_like(I, dtype=float), I.astype(float))
+        x = np.where(I < -1, -0.5, x)
+        return -x * x + 4.0 * x + 10.0 * (c + 1)
+
+    C = _dense_from_func(g, fn)
+    F = ao.interp_dense(C, g)
+    for c in range(5):
+        u0 = 10.0 * (c + 1)
+        assert np.allclose(F[c, :, :, 0], u0 - 1.0, rtol=0, atol=1e-13)
+        assert np.allclose(F[c, :, :, 1], u0 + 1.0, rtol=0, atol=1e-13)
+
+
+def test_avg_down_of_interp_is_identity(orc):
+    g = _og(orc, (8, 8, 4))
+    rng = np.random.default_rng(9)
+    boxes = [((0, 0, 0), (7, 7, 3))]
+    fabs = orc.new_level_fabs(boxes)
+    fabs[0].valid(*boxes[0])[:] = rng.uniform(1, 2, (5, 4, 8, 8))
+    orc.fill_ghost(boxes, fabs, g, 0.0)
+    C = ao.dense_grown(boxes, fabs, g)
+    F = ao.interp_dense(C, g)
+    fboxes = [((0, 0, 0), (15, 15, 7))]
+    ff = orc.new_level_fabs(fboxes)
+    ff[0].valid(*fboxes[0])[:] = F
+    cf = orc.new_level_fabs(boxes)
+    ao.avg_down(fboxes, ff, boxes, cf, g)
+    assert np.abs(cf[0].valid(*boxes[0]) - fabs[0].valid(*boxes[0])).max() < 1e-15
+
+
+def _covers(boxes, dense, lo=(0, 0, 0)):
+    cov = np.zeros_like(dense, dtype=int)
+    for blo, bhi in boxes:
+        cov[blo[2] - lo[2]:bhi[2] - lo[2] + 1, blo[1] - lo[1]:bhi[1] - lo[1] + 1, blo[0] - lo[0]:bhi[0] - lo[0] + 1] += 1
+    return cov
+
+
+@pytest.mark.parametrize("eff", [0.5, 0.7, 1.0])
+def test_cluster_properties(eff):
+    rng = np.random.default_rng(int(eff * 10))
+    for trial in range(5):
+        d = (rng.uniform(size=(4, 24, 40)) < 0.08).astype(np.uint8)
+        d[:, 5:9, 10:30] = 1
+        boxes = ao.cluster(d, (0, 0, 0), 1, eff, 10 ** 6)
+        cov = _covers(boxes, d)
+        assert cov.max() == 1                      # disjoint
+        assert (cov[d == 1] == 1).all()            # every tag covered
+        for blo, bhi in boxes:
+            sub = d[blo[2]:bhi[2] + 1, blo[1]:bhi[1] + 1, blo[0]:bhi[0] + 1]
+            assert sub.mean() >= eff - 1e-12
+            # bounding boxes: every face plane holds a tag
+            assert sub[0].any() and sub[-1].any() and sub[:, 0].any() and sub[:, -1].any()
+            assert sub[:, :, 0].any() and sub[:, :, -1].any()
+
+
+def test_cluster_known_answers():
+    d = np.zeros((2, 16, 32), dtype=np.uint8)
+    d[:, 2:6, 3:9] = 1
+    d[:, 10:14, 20:30] = 1
+    assert sorted(ao.cluster(d, (0, 0, 0), 1, 0.7, 64)) == [((3, 2, 0), (8, 5, 1)), ((20, 10, 0), (29, 13, 1))]
+    L = np.zeros((1, 8, 8), dtype=np.uint8)
+    L[0, 0:8, 0:2] = 1
+    L[0, 0:2, 0:8] = 1
+    boxes = ao.cluster(L, (0, 0, 0), 1, 1.0, 64)
+    assert len(boxes) == 2 and _covers(boxes, L).sum() == L.sum()
+    # blocking factor 2: boxes aligned to 2; max size chops a long box evenly
+    d = np.zeros((2, 8, 64), dtype=np.uint8)
+    d[:, 3, 5:61] = 1
+    boxes = ao.cluster(d, (0, 0, 0), 2, 0.7, 16)
+    for blo, bhi in boxes:
+        assert all(v % 2 == 0 for v in blo) and all((v + 1) % 2 == 0 for v in bhi)
+        assert all(bhi[q] - blo[q] + 1 <= 16 for q in range(3))
+    assert sorted(b[0][0] for b in boxes) == [4, 18, 32, 46]
+
+
+@pytest.mark.parametrize("block,eff,mx", [(1, 0.7, 32), (1, 1.0, 16), (2, 0.7, 8), (1, 0.5, 64)])
+def test_library_clustering_matches_oracle(block, eff, mx):
+    """ocfd_cluster_tags (host code of the HIP library) against the oracle."""
+    from ocfd.amr import cluster_tags
+    rng = np.random.default_rng(block * 100 + mx)
+    for trial in range(6):
+        shape = (2 * int(rng.integers(1, 4)), 2 * int(rng.integers(4, 20)), 2 * int(rng.integers(4, 40)))
+        d = (rng.uniform(size=shape) < rng.uniform(0.02, 0.3)).astype(np.uint8)
+        lo = (2 * int(rng.integers(0, 3)), 0, 2)
+        want = sorted(ao.cluster(d, lo, block, eff, mx))
+        got = sorted(cluster_tags(d, lo, block, eff, mx))
+        assert got == want, (trial, shape)
+
+
+def _kh_two_level(orc, n, fine, reflux=True, nsteps=3, scheme=2, visc=1):
+    g = _og(orc, (n, n, n))
+    boxes = [((0, 0, 0), (n - 1, n - 1, n - 1))]
+    T = ao.TwoLevel(boxes, g, orc.parm(), 0.4, 2, scheme=scheme, visc=visc, regrid_int=0, fine_boxes=fine,
+                    seed=3, do_reflux=reflux)
+    return T, g, boxes
+
+
+def _totals(T):
+    tot = np.zeros(5)
+    cov = np.zeros((T.gc.dom_hi[2] + 1, T.gc.dom_hi[1] + 1, T.gc.dom_hi[0] + 1), dtype=bool)
+    for blo, bhi in T.fboxes:
+        cov[blo[2] // 2:bhi[2] // 2 + 1, blo[1] // 2:bhi[1] // 2 + 1, blo[0] // 2:bhi[0] // 2 + 1] = True
+    for (lo, hi), f in zip(T.cboxes, T.S[0]):
+        v = f.valid(lo, hi)
+        m = ~cov[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+        tot += 8.0 * v[:, m].sum(axis=1)
+    for (lo, hi), f in zip(T.fboxes, T.S[1]):
+        tot += f.valid(lo, hi).reshape(5, -1).sum(axis=1)
+    return tot
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2])
+def test_reflux_conserves_on_a_periodic_domain(orc, scheme):
+    fine = [((2, 1, 2), (5, 4, 4)), ((6, 1, 2), (7, 6, 5))]
+    T, g, boxes = _kh_two_level(orc, 12, fine, scheme=scheme)
+    t0 = _totals(T)
+    for _ in range(3):
+        T.step()
+    t1 = _totals(T)
+    scale = np.array([t0[0], np.abs(t0[1:4]).max() + t0[0], 1, 1, t0[4]])
+    scale[2] = scale[3] = scale[1]
+    assert (np.abs(t1 - t0) / scale < 2e-14).all(), (t1 - t0) / scale
+    U, _, _ = _kh_two_level(orc, 12, fine, reflux=False, scheme=scheme)
+    u0 = _totals(U)
+    for _ in range(3):
+        U.step()
+    assert np.abs(_totals(U) - u0)[0] / scale[0] > 1e-10
+
+
+def test_fully_refined_equals_uniform_fine(orc):
+    n = 6
+    T, g, boxes = _kh_two_level(orc, n, [((0, 0, 0), (n - 1, n - 1, n - 1))], nsteps=0)
+    gf = ao.refine_ogeom(g)
+    fb = ao.refine_boxes(boxes)
+    A = orc.new_level_fabs(fb)
+    B = orc.new_level_fabs(fb)
+    C = orc.new_level_fabs(fb)
+    orc.init_case(2, fb, A, gf, orc.parm(), 3)
+    time = 0.0
+    for _ in range(3):
+        dt0 = T.step()
+        for it in range(2):
+            orc.flow_advance(fb, A, B, C, gf, orc.parm(), time + it * dt0 / 2, dt0 / 2, 2, 1)
+        time += dt0
+    for (lo, hi), f, a in zip(fb, T.S[1], A):
+        assert np.array_equal(f.valid(lo, hi), a.valid(lo, hi))
