@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the served mixed stream")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--no-amr", action="store_true", help="skip the two-level DMR")
+    ap.add_argument("--amr-only", action="store_true", help="only the two-level DMR (for profiling)")
+    ap.add_argument("--amr-steps", type=int, default=10)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (HBM counter calibration)")
     return ap.parse_args()
@@ -193,6 +196,93 @@ def serving(nreq, rate, seed=99):
             "hbm_budget_GiB": 32, "slots": 8, "peak_resident": srv.peak_active}
 
 
+def _cells(boxes):
+    n = 0
+    for lo, hi in boxes:
+        n += (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1)
+    return n
+
+
+def amr_dmr(steps, warmup=60, nx=960, ny=240, cpu_sample=True):
+    """Two-level double Mach reflection: the reference's DMR inputs (960x240x2,
+    max_grid_size 64, blocking_factor 2, regrid_int 2, refine_dengrad 0.05,
+    SSP-RK3, CFL 0.5) with amr.max_level = 1 and AMReX's default grid
+    efficiency 0.7 (the inputs' grid_eff 1 only matters once a fine level
+    exists; at 1 the fine level falls apart into ~1000 boxes of a few dozen
+    cells); FP64 fluxes, planar stepping of the z-uniform flow.  A cell update
+    is one cell advanced by its level's dt (fine cells count twice per coarse
+    step).  The same time span on a uniform grid at the fine resolution is
+    timed beside it."""
+    import torch
+    import ocfd
+    from ocfd import amr, cases
+    p = ocfd.default_parm()
+    boxes, g, kind = cases.dmr(nx, ny, 2, max_grid=(64, 64, 2))
+    kw = dict(scheme=2, refine_dengrad=0.05, regrid_int=2, blocking_factor=2, grid_eff=0.7, max_grid_size=64)
+    T = amr.TwoLevel(boxes, g, p, 0.5, kind, flags=ocfd.PLANAR, **kw)
+    for _ in range(warmup):
+        T.step()
+    nc = _cells(boxes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    upd, nfine, nfbox, dts = 0, [], [], []
+    for _ in range(steps):
+        dts.append(T.step())
+        nf = _cells(T.fine.boxes) if T.fine is not None else 0
+        nfine.append(nf)
+        nfbox.append(T.fine.nbox if T.fine is not None else 0)
+        upd += nc + 2 * nf
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    out = {"workload": f"DMR {nx}x{ny}x2 + 1 level (ratio 2, grid_eff 0.7), SSP-RK3, FP64, regrid every 2 "
+                       "coarse steps",
+           "coarse_steps": steps, "ms_per_coarse_step": 1e3 * el / steps,
+           "cell_updates_per_s": upd / el, "coarse_cells": nc, "fine_cells_mean": float(sum(nfine) / steps),
+           "fine_boxes_mean": float(sum(nfbox) / steps), "fine_fraction_of_domain": float(sum(nfine) / steps / (8 * nc))}
+    del T
+    torch.cuda.empty_cache()
+    # the same time span on the uniform fine grid (two fine steps per coarse dt)
+    fb, fg, _ = cases.dmr(2 * nx, 2 * ny, 4, max_grid=(64, 64, 4))
+    lev = ocfd.Level(fb, fg)
+    A, B, C = lev.new_mf(), lev.new_mf(), lev.new_mf()
+    lev.init_state(kind, A, p)
+    for _ in range(2):
+        lev.advance(A, B, C, 0.0, dts[0] / 2, 2, p, ocfd.PLANAR)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tt = 0.0
+    for dt in dts:
+        for _ in range(2):
+            lev.advance(A, B, C, tt, dt / 2, 2, p, ocfd.PLANAR)
+            tt += dt / 2
+    torch.cuda.synchronize()
+    elu = time.perf_counter() - t0
+    out["uniform_fine_ms_per_coarse_step"] = 1e3 * elu / steps
+    out["speedup_vs_uniform_fine"] = elu / el
+    del lev, A, B, C
+    torch.cuda.empty_cache()
+    if cpu_sample:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import ctypes
+        import oracle as orc
+        import amr_oracle as ao
+        sb, sg, _ = cases.dmr(240, 60, 2, max_grid=(64, 64, 2))
+        og = orc.OGeom()
+        ctypes.memmove(ctypes.byref(og), ctypes.byref(sg), ctypes.sizeof(og))
+        O = ao.TwoLevel(sb, og, orc.parm(), 0.5, kind, **kw)
+        t0 = time.perf_counter()
+        upd_c = 0
+        for _ in range(2):
+            O.step()
+            upd_c += _cells(sb) + 2 * _cells(O.fboxes)
+        el_c = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": upd_c / el_c, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+                               "sample": f"AMR oracle (C++ fluxes, numpy AMR steps), DMR 240x60x2 + 1 level, "
+                                         f"2 coarse steps ({el_c:.1f} s)"}
+    return out
+
+
 def cpu_baseline(n, steps):
     sys.path.insert(0, str(ROOT / "oracle"))
     sys.path.insert(0, str(ROOT / "tests"))
@@ -226,6 +316,9 @@ def main():
     if args.calib_copy:
         calib_copy()
         return
+    if args.amr_only:
+        print(json.dumps({"amr_two_level": amr_dmr(args.amr_steps, cpu_sample=False)}))
+        return
     n = args.n
     boxes, g, kind = cases.kh3d(n)
     lev = ocfd.Level(boxes, g)
@@ -254,6 +347,10 @@ def main():
                  "graph_value": ncell * args.steps / t64g / 1e6, "kernels": kern64}
     del A, B, C, lev
     torch.cuda.empty_cache()
+
+    amr_res = None
+    if not args.no_amr:
+        amr_res = amr_dmr(args.amr_steps, cpu_sample=not args.no_cpu_baseline)
 
     serve_res = None
     if not args.no_serve:
@@ -294,6 +391,7 @@ def main():
         "fp64_form": res64,
         "fp32_over_fp64": (value / res64["value"]) if res64 else None,
         "serving": serve_res,
+        "amr_two_level": amr_res,
         "cpu_baseline": cpu,
         "chemistry_0d": chem,
     }

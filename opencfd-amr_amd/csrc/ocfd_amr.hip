@@ -505,15 +505,40 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
         set_error("ocfd_cf_create: the flux register covers regular levels only");
         return OCFD_EUNSUPPORTED;
     }
+    // coverage of the coarse domain by the fine level, and the uncovered coarse
+    // cells next to it (the only ones that can own an interface face)
+    const GeomDesc& CG = C.geom;
+    const long n0 = CG.dom_hi[0] - CG.dom_lo[0] + 1, n1 = CG.dom_hi[1] - CG.dom_lo[1] + 1,
+               n2 = CG.dom_hi[2] - CG.dom_lo[2] + 1;
+    auto lin = [&](int I, int J, int K) {
+        return ((long)(K - CG.dom_lo[2]) * n1 + (J - CG.dom_lo[1])) * n0 + (I - CG.dom_lo[0]);
+    };
+    std::vector<unsigned char> cov((size_t)(n0 * n1 * n2), 0), cand((size_t)(n0 * n1 * n2), 0);
+    for (const BoxDesc& B : F.boxes)
+        for (int K = B.lo[2] >> 1; K <= B.hi[2] >> 1; ++K)
+            for (int J = B.lo[1] >> 1; J <= B.hi[1] >> 1; ++J)
+                for (int I = B.lo[0] >> 1; I <= B.hi[0] >> 1; ++I) cov[lin(I, J, K)] = 1;
     // 1 = covered by the fine level, 0 = not, -1 = outside a non-periodic domain side
     auto covered = [&](int I, int J, int K) {
         int iv[3] = {I, J, K};
         for (int d = 0; d < 3; ++d) {
-            iv[d] = wrap(C.geom, d, iv[d]);
-            if (iv[d] < C.geom.dom_lo[d] || iv[d] > C.geom.dom_hi[d]) return -1;
+            iv[d] = wrap(CG, d, iv[d]);
+            if (iv[d] < CG.dom_lo[d] || iv[d] > CG.dom_hi[d]) return -1;
         }
-        return find_box(F, 2 * iv[0], 2 * iv[1], 2 * iv[2]) >= 0 ? 1 : 0;
+        return (int)cov[lin(iv[0], iv[1], iv[2])];
     };
+    for (const BoxDesc& B : F.boxes)
+        for (int K = (B.lo[2] >> 1) - 1; K <= (B.hi[2] >> 1) + 1; ++K)
+            for (int J = (B.lo[1] >> 1) - 1; J <= (B.hi[1] >> 1) + 1; ++J)
+                for (int I = (B.lo[0] >> 1) - 1; I <= (B.hi[0] >> 1) + 1; ++I) {
+                    int iv[3] = {I, J, K};
+                    bool in = true;
+                    for (int d = 0; d < 3; ++d) {
+                        iv[d] = wrap(CG, d, iv[d]);
+                        in = in && iv[d] >= CG.dom_lo[d] && iv[d] <= CG.dom_hi[d];
+                    }
+                    if (in && !cov[lin(iv[0], iv[1], iv[2])]) cand[lin(iv[0], iv[1], iv[2])] = 1;
+                }
     std::vector<CFFace> cf, ff;
     std::vector<int4> cells;
     std::vector<int> cellfaces;
@@ -522,7 +547,7 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
         for (int K = B.lo[2]; K <= B.hi[2]; ++K)
             for (int J = B.lo[1]; J <= B.hi[1]; ++J)
                 for (int I = B.lo[0]; I <= B.hi[0]; ++I) {
-                    if (covered(I, J, K) != 0) continue;
+                    if (!cand[lin(I, J, K)]) continue;
                     int fidx[6] = {-1, -1, -1, -1, -1, -1};
                     for (int d = 0; d < 3; ++d)
                         for (int side = 0; side < 2; ++side) {

@@ -5,7 +5,8 @@
 // efficiency (the cluster/cut rules of AMReX's ClusterList: hole cuts first,
 // then the strongest inflection of the signature's second difference, then
 // bisection; among equal kinds the direction whose smaller part is largest),
-// refined back by the blocking factor and chopped to the maximum grid size
+// refined back by the blocking factor, joined where two boxes share a face
+// (the BoxList::simplify step), and chopped to the maximum grid size
 // (balanced pieces, AMReX BoxList::maxSize).  Host code; no device work.
 #include <algorithm>
 #include <cstdlib>
@@ -124,6 +125,42 @@ Cluster chop(Cluster& c)
     return hi;
 }
 
+// join boxes that share their extent in two directions and touch in the
+// third, until no pair joins (list kept sorted by lower corner, z major);
+// the union is unchanged, the boxes get fewer and larger
+void merge(std::vector<IBox>& bx)
+{
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        std::sort(bx.begin(), bx.end(), [](const IBox& a, const IBox& b) {
+            if (a.lo[2] != b.lo[2]) return a.lo[2] < b.lo[2];
+            if (a.lo[1] != b.lo[1]) return a.lo[1] < b.lo[1];
+            return a.lo[0] < b.lo[0];
+        });
+        for (size_t i = 0; i < bx.size(); ++i) {
+            size_t j = i + 1;
+            while (j < bx.size()) {
+                IBox& a = bx[i];
+                const IBox& b = bx[j];
+                int same = 0, d = -1;
+                for (int q = 0; q < 3; ++q) {
+                    if (a.lo[q] == b.lo[q] && a.hi[q] == b.hi[q]) ++same;
+                    else d = q;
+                }
+                if (same == 2 && (b.lo[d] == a.hi[d] + 1 || a.lo[d] == b.hi[d] + 1)) {
+                    a.lo[d] = std::min(a.lo[d], b.lo[d]);
+                    a.hi[d] = std::max(a.hi[d], b.hi[d]);
+                    bx.erase(bx.begin() + j);
+                    changed = true;
+                    continue;
+                }
+                ++j;
+            }
+        }
+    }
+}
+
 // chop every box of the list to at most `chunk` cells per direction, in
 // balanced pieces taken from the high end
 void max_size(std::vector<IBox>& boxes, int chunk)
@@ -205,6 +242,7 @@ extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigne
             }
             boxes.push_back(b);
         }
+        merge(boxes);
         max_size(boxes, max_size_cells);
     }
     *nbox_out = (int)boxes.size();

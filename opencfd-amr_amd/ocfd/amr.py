@@ -175,6 +175,9 @@ class TwoLevel:
         self.parm, self.cfl, self.scheme, self.flags = parm, cfl, scheme, flags
         self.thr, self.regrid_int = refine_dengrad, regrid_int
         self.block, self.grid_eff, self.max_c = blocking_factor // RATIO, grid_eff, max_grid_size // RATIO
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream()
         self.stop_time, self.do_reflux, self.stream = stop_time, do_reflux, stream
         self.kind, self.seed = kind, seed
         self.coarse = Level(coarse_boxes, geom)

@@ -5,7 +5,8 @@ restates, in numpy over the C oracle's level fill and box fluxes:
 
 * tag_dengrad, Source/Tagging.H:8-36;
 * fine-grid generation from tags (Berger-Rigoutsos signature clustering with
-  AMReX ClusterList's cut rules, blocking factor, max grid size);
+  AMReX ClusterList's cut rules, blocking factor, joining of boxes that share
+  a face, max grid size);
 * AMReX's cell-conservative linear interpolation (MC-limited central slopes,
   third-order one-sided slopes at external-Dirichlet domain faces, one min/max
   limiter per coarse cell) for fine ghost cells and new fine regions: the
@@ -161,6 +162,34 @@ def _bbox(pts):
     return pts.min(0), pts.max(0)
 
 
+def merge_boxes(boxes):
+    """Join boxes that share their extent in two directions and touch in the
+    third until no pair joins (sorted by lower corner, z major)."""
+    bx = [(list(a), list(b)) for a, b in boxes]
+    changed = True
+    while changed:
+        changed = False
+        bx.sort(key=lambda q: (q[0][2], q[0][1], q[0][0]))
+        i = 0
+        while i < len(bx):
+            a = bx[i]
+            j = i + 1
+            while j < len(bx):
+                b = bx[j]
+                same = [a[0][d] == b[0][d] and a[1][d] == b[1][d] for d in range(3)]
+                if sum(same) == 2:
+                    d = same.index(False)
+                    if b[0][d] == a[1][d] + 1 or a[0][d] == b[1][d] + 1:
+                        a[0][d] = min(a[0][d], b[0][d])
+                        a[1][d] = max(a[1][d], b[1][d])
+                        del bx[j]
+                        changed = True
+                        continue
+                j += 1
+            i += 1
+    return bx
+
+
 def cluster(dense, lo, block=1, grid_eff=0.7, max_size=32):
     """Boxes covering the tagged cells of dense (nz, ny, nx) (index origin lo)."""
     nz, ny, nx = dense.shape
@@ -201,6 +230,7 @@ def cluster(dense, lo, block=1, grid_eff=0.7, max_size=32):
     for p in clusters:
         blo, bhi = _bbox(p)
         boxes.append(([int(v) * b for v in blo], [(int(v) + 1) * b - 1 for v in bhi]))
+    boxes = merge_boxes(boxes)
     for d in range(3):
         out = []
         for blo, bhi in boxes:
