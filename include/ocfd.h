@@ -389,6 +389,29 @@ int ocfd_cf_reflux(ocfd_cf c, double* S_coarse, void* stream);
  * take the mean of their 8 fine cells; ncomp components (5 state, 9 species). */
 int ocfd_cf_avg_down(ocfd_cf c, const double* S_fine, double* S_coarse, int ncomp, void* stream);
 
+/* State buffers of one two-level advance (MultiFabs with 4 ghosts, 5 comps). */
+typedef struct ocfd_amr_bufs {
+    double* c_old;    /* coarse U^n; its ghosts are filled at `time` on return */
+    double* c_new;    /* coarse U^(n+1), refluxed and averaged down on return */
+    double* c_s1;     /* coarse stage scratch */
+    double* c_s2;     /* coarse stage scratch (SSP-RK3) */
+    double* c_t;      /* coarse state at a fine stage time (scratch) */
+    double* f_state;  /* fine U, advanced in place by two steps of dt/2 */
+    double* f_s1;     /* fine stage scratch */
+    double* f_s2;     /* fine stage scratch (SSP-RK3) */
+} ocfd_amr_bufs;
+
+/* One coarse time step of the two-level hierarchy on `stream`, no host
+ * synchronisation (Amr::timeStep of level 0 with subcycling): the coarse
+ * advance of EBR::flow_advance (Source/Advance.cpp:81-150) with the coarse
+ * register adds, two fine steps of dt0/2 whose stage ghosts come from the
+ * fine level and from the coarse level at the stage time (old/new combination
+ * of FillPatch), then reflux and average-down (EBR::post_timestep,
+ * Source/EBR.cpp:313-330).  cf NULL: the coarse advance alone.  scheme 0
+ * Euler, 1 RK2, 2 SSP-RK3; do_reflux as ebr.do_reflux. */
+int ocfd_amr_advance(ocfd_level coarse, ocfd_cf cf, const ocfd_amr_bufs* b, double time, double dt0, int scheme,
+                     const ocfd_parm* parm, int flags, int do_reflux, void* stream);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ocfd_last_error(void);
 

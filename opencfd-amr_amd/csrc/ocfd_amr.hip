@@ -16,6 +16,7 @@
 // primitive scratch the stage's sweeps have just used, so the fused sweeps
 // carry no flux-register work.
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -808,3 +809,105 @@ int ocfd_lincomb(long n, double a, const double* x, double b, const double* y, d
 }
 
 } // extern "C"
+
+namespace {
+
+// (stage time offset, register weight) in units of dt, per scheme
+const double kStageT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 1.0, 0.5}};
+const double kStageW[3][3] = {{1.0, 0.0, 0.0}, {0.5, 0.5, 0.0}, {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0}};
+
+// stage s of the scheme: out = (c_cur*cur + c_old*old) + c_dt*dt*L(cur)
+void stage_coefs(int scheme, int s, double& c_cur, double& c_old, double& c_dt)
+{
+    if (scheme == 0) { c_cur = 1.0; c_old = 0.0; c_dt = 1.0; return; }
+    if (scheme == 1) {
+        if (s == 0) { c_cur = 1.0; c_old = 0.0; c_dt = 1.0; }
+        else { c_cur = 0.5; c_old = 0.5; c_dt = 0.5; }
+        return;
+    }
+    if (s == 0) { c_cur = 1.0; c_old = 0.0; c_dt = 1.0; }
+    else if (s == 1) { c_cur = 0.25; c_old = 0.75; c_dt = 0.25; }
+    else { c_cur = 2.0 / 3.0; c_old = 1.0 / 3.0; c_dt = 2.0 / 3.0; }
+}
+
+// The stage sequence of one level from U (old) into `out`: scratch s1, s2;
+// `fill` fills the ghosts of a stage input at a time; the register add runs
+// after each stage when cf is given.  out may be U (the last stage reads U
+// only at its own cell).
+template <class Fill>
+int run_stages(ocfd_level lev, double* U, double* out, double* s1, double* s2, double t, double dt, int scheme,
+               const ocfd_parm* parm, int flags, ocfd_cf cf, int side, Fill&& fill, hipStream_t st)
+{
+    const int ns = scheme + 1;
+    double* in[3] = {U, s1, s2};
+    double* dst[3] = {s1, s2, out};
+    if (ns == 1) dst[0] = out;
+    if (ns == 2) dst[1] = out;
+    int rc;
+    for (int s = 0; s < ns; ++s) {
+        if ((rc = fill(in[s], t + kStageT[scheme][s] * dt))) return rc;
+        double c_cur, c_old, c_dt;
+        stage_coefs(scheme, s, c_cur, c_old, c_dt);
+        if ((rc = ocfd_rk_stage(lev, in[s], s ? U : nullptr, dst[s], c_cur, c_old, c_dt * dt, parm, flags, st)))
+            return rc;
+        if (cf && (rc = ocfd_cf_add(cf, side, kStageW[scheme][s] * dt, parm, flags, st))) return rc;
+    }
+    return OCFD_OK;
+}
+
+} // namespace
+
+extern "C" int ocfd_amr_advance(ocfd_level coarse, ocfd_cf cf, const ocfd_amr_bufs* b, double time, double dt0,
+                                int scheme, const ocfd_parm* parm, int flags, int do_reflux, void* stream)
+{
+    if (!coarse || !b || !parm || scheme < 0 || scheme > 2 || !(dt0 > 0.0) || !b->c_old || !b->c_new || !b->c_s1 ||
+        (scheme == 2 && !b->c_s2) || (cf && (cf->coarse != coarse || !b->c_t || !b->f_state || !b->f_s1 ||
+                                            (scheme == 2 && !b->f_s2)))) {
+        set_error("ocfd_amr_advance: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    ocfd_cf reg = (cf && do_reflux) ? cf : nullptr;
+    if (reg && (rc = ocfd_cf_reset(reg, st))) return rc;
+    auto fill_c = [&](double* X, double ts) { return ocfd_fill_boundary(coarse, X, ts, st); };
+    if ((rc = run_stages(coarse, b->c_old, b->c_new, b->c_s1, b->c_s2, time, dt0, scheme, parm, flags, reg, 0,
+                         fill_c, st)))
+        return rc;
+    if (!cf) return OCFD_OK;
+    const double t0 = time, t1 = time + dt0, span = dt0;
+    ocfd_level fine = cf->fine;
+    const long nc = coarse->dev.mf_state;
+    // fine stage ghosts: the fine level, then the coarse state at the stage time
+    auto fill_f = [&](double* X, double ts) {
+        int r;
+        if ((r = ocfd_fill_boundary(fine, X, ts, st))) return r;
+        const double* C;
+        if (std::fabs(ts - t0) <= 1e-10 * span) {
+            C = b->c_old;                      // ghosts filled at t0 by the coarse stage 1
+        } else if (std::fabs(ts - t1) <= 1e-10 * span) {
+            if ((r = ocfd_fill_boundary(coarse, b->c_new, t1, st))) return r;
+            C = b->c_new;
+        } else {
+            if ((r = ocfd_lincomb(nc, (t1 - ts) / span, b->c_old, (ts - t0) / span, b->c_new, b->c_t, st))) return r;
+            if ((r = ocfd_fill_boundary(coarse, b->c_t, ts, st))) return r;
+            C = b->c_t;
+        }
+        return ocfd_fill_from_coarse(fine, X, coarse, C, st);
+    };
+    const double dt1 = 0.5 * dt0;
+    for (int it = 0; it < 2; ++it) {
+        double* out = b->f_state;
+        if (scheme == 0) out = b->f_s1;   // Euler: out of place, copied back below
+        if ((rc = run_stages(fine, b->f_state, out, b->f_s1, b->f_s2, t0 + it * dt1, dt1, scheme, parm, flags, reg, 1,
+                             fill_f, st)))
+            return rc;
+        if (scheme == 0 && hipMemcpyAsync(b->f_state, b->f_s1, sizeof(double) * fine->dev.mf_state,
+                                          hipMemcpyDeviceToDevice, st) != hipSuccess) {
+            set_error("ocfd_amr_advance: copy failed");
+            return OCFD_EHIP;
+        }
+    }
+    if (reg && (rc = ocfd_cf_reflux(reg, b->c_new, st))) return rc;
+    return ocfd_cf_avg_down(cf, b->f_state, b->c_new, OCFD_NCONS, st);
+}

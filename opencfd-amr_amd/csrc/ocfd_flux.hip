@@ -6,7 +6,7 @@
 //             Each lane recovers primitives and splits its cell (FP64 prims,
 //             FT split), the split fluxes are exchanged through LDS, lanes
 //             3..61 evaluate the 59 faces, and the x-part of -div F is stored.
-//   sweep_y : lane = one i column, marches j through a chunk of CH cells with
+//   sweep_y : lane = one i column, marches j through a chunk of cells with
 //             a register window of split fluxes (6 cells of f+, 5 of f-), and
 //             adds the y-part into the accumulator.
 //   sweep_z : same march along k; adds the z-part and finishes the cell
@@ -23,7 +23,7 @@
 namespace ocfd {
 
 constexpr int XSEG = 58;   // interior cells of one x-row wave
-constexpr int CH = 32;     // cells per march chunk (y and z sweeps)
+constexpr int CH = 32;     // cells per march chunk (y and z sweeps); small levels use shorter ones
 
 template <class FT, bool VISC>
 __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
@@ -118,8 +118,8 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
     const int i = B.lo[0] + t.y * 64 + lane;
     const int o = B.lo[O] + t.z * 4 + w;
     if (i > B.hi[0] || o > B.hi[O]) return;
-    const int m0 = B.lo[D] + t.w * CH;
-    const int m1 = min(m0 + CH - 1, B.hi[D]);
+    const int m0 = B.lo[D] + (t.w & 0xfffff);
+    const int m1 = min(m0 + (t.w >> 20) - 1, B.hi[D]);
     const FT gam = (FT)a.ph.gamma;
     const long st = B.nptg;
     const long dstep = D == 1 ? (long)B.nxg : (long)B.nxg * B.nyg;   // index step along D

@@ -323,6 +323,11 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
                 delete lev;
                 return OCFD_EINVAL;
             }
+            if (B.hi[d] - B.lo[d] + 1 >= (1 << 20)) {   // march tiles hold in-box offsets in 20 bits
+                set_error("ocfd_level_create: box " + std::to_string(b) + " has 2^20 cells or more along a side");
+                delete lev;
+                return OCFD_EINVAL;
+            }
             B.dx[d] = g->dx[d];
             B.dxinv[d] = 1.0 / g->dx[d];
         }
@@ -425,7 +430,23 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     }
 
     // work-lists
-    const int xseg = tile_xseg(), ch = tile_chunk();
+    const int xseg = tile_xseg();
+    // march chunk length: the full chunk unless the level has too few march
+    // tiles to fill the chip (small or thin levels), where shorter chunks
+    // trade a longer halo prologue per cell for more independent waves
+    int ch = tile_chunk();
+    auto march_tiles = [&](int c) {
+        long ny_t = 0, nz_t = 0;
+        for (const BoxDesc& B : L.boxes) {
+            if (B.type != 0) continue;
+            const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+            const long is = (nx + 63) / 64;
+            ny_t += is * ((nz + 3) / 4) * ((ny + c - 1) / c);
+            nz_t += is * ((ny + 3) / 4) * ((nz + c - 1) / c);
+        }
+        return std::max(ny_t, nz_t);
+    };
+    while (ch > 8 && march_tiles(ch) < 2048) ch /= 2;
     std::vector<int4> tx, ty, tz, tg, tc, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
@@ -459,12 +480,13 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         // contiguous range of the list (one XCD's share) holds tiles that
         // share chunk halos and transverse neighbour rows
         const int iseg = (nx + 63) / 64;
+        // .w = first cell of the chunk (low 20 bits) | chunk length << 20
         for (int kg = 0; kg < (nz + 3) / 4; ++kg)
             for (int c = 0; c < (ny + ch - 1) / ch; ++c)
-                for (int s = 0; s < iseg; ++s) ty.push_back(make_int4(b, s, kg, c));
+                for (int s = 0; s < iseg; ++s) ty.push_back(make_int4(b, s, kg, (c * ch) | (ch << 20)));
         for (int jg = 0; jg < (ny + 3) / 4; ++jg)
             for (int c = 0; c < (nz + ch - 1) / ch; ++c)
-                for (int s = 0; s < iseg; ++s) tz.push_back(make_int4(b, s, jg, c));
+                for (int s = 0; s < iseg; ++s) tz.push_back(make_int4(b, s, jg, (c * ch) | (ch << 20)));
         for (int gz : {3, 1}) {
             const long np = (long)(nx + 6) * (ny + 6) * (nz + 2 * gz);
             std::vector<int4>& tl = gz == 3 ? tp : tpp;

@@ -168,6 +168,8 @@ def lib():
         L.ocfd_cf_registers.argtypes = [vp, dp, dp]
         L.ocfd_cf_reflux.argtypes = [vp, vp, vp]
         L.ocfd_cf_avg_down.argtypes = [vp, vp, vp, ctypes.c_int, vp]
+        L.ocfd_amr_advance.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                       ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, vp]
         L.ocfd_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib

@@ -31,13 +31,6 @@ CHANGE_MAX = 1.1
 RATIO = 2
 _pkg = importlib.import_module(__package__)
 
-# (stage time offset in units of dt, register weight in units of dt) per scheme
-STAGES = {
-    0: [(0.0, 1.0)],
-    1: [(0.0, 0.5), (1.0, 0.5)],
-    2: [(0.0, 1.0 / 6.0), (1.0, 1.0 / 6.0), (0.5, 2.0 / 3.0)],
-}
-
 
 def refine_geom(g):
     """Geometry of the level refined by 2 (same physical box and BCs)."""
@@ -111,6 +104,11 @@ def copy_level(dst, S_dst, src, S_src, stream=None):
     check(lib().ocfd_copy_level(dst.handle, _ptr(S_dst), src.handle, _ptr(S_src), _stream(stream)), "ocfd_copy_level")
 
 
+class AmrBufs(ctypes.Structure):
+    """struct ocfd_amr_bufs: the state buffers of one two-level advance."""
+    _fields_ = [(n, ctypes.c_void_p) for n in ("c_old", "c_new", "c_s1", "c_s2", "c_t", "f_state", "f_s1", "f_s2")]
+
+
 class FluxRegister:
     """The coarse/fine interface of one fine level (EBR's flux_reg)."""
 
@@ -168,7 +166,7 @@ class TwoLevel:
     def __init__(self, coarse_boxes, geom, parm, cfl, kind=None, scheme=2, flags=0, refine_dengrad=0.05,
                  regrid_int=2, blocking_factor=2, grid_eff=0.7, max_grid_size=64, fine_boxes=None, seed=0,
                  stop_time=-1.0, do_reflux=True, stream=None):
-        if scheme not in STAGES:
+        if scheme not in (0, 1, 2):
             raise ValueError("scheme must be 0 (Euler), 1 (RK2) or 2 (SSP-RK3)")
         if blocking_factor % RATIO or max_grid_size % blocking_factor:
             raise ValueError("blocking_factor must be a multiple of 2 and divide max_grid_size")
@@ -270,84 +268,14 @@ class TwoLevel:
             dt0 = self.stop_time - self.time
         self.dt_level = [dt0, dt0 / RATIO]
 
-    def _coarse_at(self, ts, t0, t1):
-        """Coarse state at time ts with filled ghosts (FillPatchSingleLevel of
-        the old/new pair)."""
-        c, s = self.coarse, self.stream
-        span = t1 - t0
-        if abs(ts - t0) <= 1e-10 * span:
-            return self.Oc                       # ghosts filled at t0 by the coarse stage 1
-        if abs(ts - t1) <= 1e-10 * span:
-            c.fill_boundary(self.S[0], t1, s)
-            return self.S[0]
-        T = self.scratch_c[2]
-        lincomb((t1 - ts) / span, self.Oc, (ts - t0) / span, self.S[0], T, s)
-        c.fill_boundary(T, ts, s)
-        return T
-
-    def _advance_coarse(self, t, dt):
-        c, p, fl, s = self.coarse, self.parm, self.flags, self.stream
-        self.Oc, self.S[0] = self.S[0], self.Oc          # old <- current, new is written below
-        O, N = self.Oc, self.S[0]
-        B, C = self.scratch_c[0], self.scratch_c[1]
-        reg = self.cf if (self.do_reflux and self.cf is not None) else None
-        st = STAGES[self.scheme]
-        if reg is not None:
-            reg.reset(s)
-        c.fill_boundary(O, t, s)
-        if self.scheme == 0:
-            c.rk_stage(O, None, N, 1.0, 0.0, dt, p, fl, s)
-        elif self.scheme == 1:
-            c.rk_stage(O, None, B, 1.0, 0.0, dt, p, fl, s)
-        else:
-            c.rk_stage(O, None, B, 1.0, 0.0, dt, p, fl, s)
-        if reg is not None:
-            reg.add(0, st[0][1] * dt, p, fl, s)
-        if self.scheme >= 1:
-            c.fill_boundary(B, t + dt, s)
-            if self.scheme == 1:
-                c.rk_stage(B, O, N, 0.5, 0.5, 0.5 * dt, p, fl, s)
-            else:
-                c.rk_stage(B, O, C, 0.25, 0.75, 0.25 * dt, p, fl, s)
-            if reg is not None:
-                reg.add(0, st[1][1] * dt, p, fl, s)
-        if self.scheme == 2:
-            c.fill_boundary(C, t + 0.5 * dt, s)
-            c.rk_stage(C, O, N, 2.0 / 3.0, 1.0 / 3.0, (2.0 / 3.0) * dt, p, fl, s)
-            if reg is not None:
-                reg.add(0, st[2][1] * dt, p, fl, s)
-
-    def _fill_fine(self, X, ts, t0, t1):
-        f, s = self.fine, self.stream
-        f.fill_boundary(X, ts, s)
-        fill_from_coarse(f, X, self.coarse, self._coarse_at(ts, t0, t1), s)
-
-    def _advance_fine(self, t, dt, t0, t1):
-        f, p, fl, s = self.fine, self.parm, self.flags, self.stream
-        A = self.S[1]
-        B, C = self.scratch_f
-        reg = self.cf if self.do_reflux else None
-        st = STAGES[self.scheme]
-        self._fill_fine(A, t, t0, t1)
-        f.rk_stage(A, None, B, 1.0, 0.0, dt, p, fl, s)
-        if reg is not None:
-            reg.add(1, st[0][1] * dt, p, fl, s)
-        if self.scheme == 0:
-            self.S[1], self.scratch_f[0] = B, A
-            return
-        self._fill_fine(B, t + dt, t0, t1)
-        if self.scheme == 1:
-            f.rk_stage(B, A, A, 0.5, 0.5, 0.5 * dt, p, fl, s)
-            if reg is not None:
-                reg.add(1, st[1][1] * dt, p, fl, s)
-            return
-        f.rk_stage(B, A, C, 0.25, 0.75, 0.25 * dt, p, fl, s)
-        if reg is not None:
-            reg.add(1, st[1][1] * dt, p, fl, s)
-        self._fill_fine(C, t + 0.5 * dt, t0, t1)
-        f.rk_stage(C, A, A, 2.0 / 3.0, 1.0 / 3.0, (2.0 / 3.0) * dt, p, fl, s)
-        if reg is not None:
-            reg.add(1, st[2][1] * dt, p, fl, s)
+    def _bufs(self):
+        b = AmrBufs()
+        b.c_old, b.c_new = self.Oc.data_ptr(), self.S[0].data_ptr()
+        b.c_s1, b.c_s2, b.c_t = (x.data_ptr() for x in self.scratch_c)
+        if self.fine is not None:
+            b.f_state = self.S[1].data_ptr()
+            b.f_s1, b.f_s2 = (x.data_ptr() for x in self.scratch_f)
+        return b
 
     def step(self):
         """One coarse time step (Amr::coarseTimeStep with two subcycled fine steps)."""
@@ -356,14 +284,11 @@ class TwoLevel:
             self.regrid()
             self.count = 0
         t, dt0 = self.time, self.dt_level[0]
-        self._advance_coarse(t, dt0)
-        if self.fine is not None:
-            dt1 = dt0 / RATIO
-            for it in range(RATIO):
-                self._advance_fine(t + it * dt1, dt1, t, t + dt0)
-            if self.do_reflux:
-                self.cf.reflux(self.S[0], self.stream)
-            self.cf.avg_down(self.S[1], self.S[0], NCONS, self.stream)
+        self.Oc, self.S[0] = self.S[0], self.Oc          # old <- current; the new state is written
+        b = self._bufs()
+        check(lib().ocfd_amr_advance(self.coarse.handle, self.cf.handle if self.cf is not None else None,
+                                     ctypes.byref(b), t, dt0, self.scheme, ctypes.byref(self.parm), self.flags,
+                                     int(self.do_reflux), _stream(self.stream)), "ocfd_amr_advance")
         self.time = t + dt0
         self.steps += 1
         self.count += 1
