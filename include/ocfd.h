@@ -412,6 +412,37 @@ typedef struct ocfd_amr_bufs {
 int ocfd_amr_advance(ocfd_level coarse, ocfd_cf cf, const ocfd_amr_bufs* b, double time, double dt0, int scheme,
                      const ocfd_parm* parm, int flags, int do_reflux, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * On-disk formats
+ * ------------------------------------------------------------------------- */
+
+/* Plot variables: rho xmom ymom zmom Eden (State_Type, Source/Setup.cpp:202-206),
+ * Cost (Cost_Type, written as 0), ux uy uz and pressure (the derived
+ * "velocity" and "pressure" of derive_plot_vars, Source/Setup.cpp:94-179,
+ * Exec/1_DMR/inputs) and vfrac (the EB volume fraction AmrLevel adds). */
+#define OCFD_PLOT_NVARS 11
+
+/* The plot variables of every valid cell of S (4 ghosts, 5 comps) into out
+ * (device, the level's 0-ghost layout with OCFD_PLOT_NVARS comps). */
+int ocfd_plot_vars(ocfd_level lev, const double* S, double* out, const ocfd_parm* parm, void* stream);
+
+/* Write an AMReX "HyperCLaw-V1.1" plotfile (EBR::thePlotFileType,
+ * Source/EBR.H:33-35; the AmrLevel::writePlotFile layout of Source/IO.cpp:32-262):
+ * dir/Header, dir/Level_<l>/Cell_H and dir/Level_<l>/Cell_D_00000 for nlev
+ * levels (each refining the one below by 2), S[l] the state of level l,
+ * level_steps[l] its step count.  Synchronises the stream. */
+int ocfd_plotfile_write(const char* dir, int nlev, const ocfd_level* levels, const double* const* S, double time,
+                        const int* level_steps, const ocfd_parm* parm, void* stream);
+
+/* fcompare (Util/Plotfile/fcompare.cpp:19-390): compare two plotfiles level by
+ * level and variable by variable in norm 0 (max), 1 or 2 (the latter scaled by
+ * the cell volume to the power 1/norm), relative to plotfile A.  *result: 1
+ * identical, 2 within the tolerances (absolute atol or relative rtol per
+ * variable), 0 different, -1 NaN present.  The fcompare report text goes to
+ * report[report_cap].  Host only; needs no GPU. */
+int ocfd_plotfile_compare(const char* file_a, const char* file_b, int norm, double rtol, double atol,
+                          int allow_diff_grids, int* result, char* report, long report_cap);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ocfd_last_error(void);
 

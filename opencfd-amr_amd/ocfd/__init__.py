@@ -168,6 +168,11 @@ def lib():
         L.ocfd_cf_registers.argtypes = [vp, dp, dp]
         L.ocfd_cf_reflux.argtypes = [vp, vp, vp]
         L.ocfd_cf_avg_down.argtypes = [vp, vp, vp, ctypes.c_int, vp]
+        L.ocfd_plot_vars.argtypes = [vp, vp, vp, ctypes.POINTER(Parm), vp]
+        L.ocfd_plotfile_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                          ctypes.c_double, ip, ctypes.POINTER(Parm), vp]
+        L.ocfd_plotfile_compare.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_int, ip, ctypes.c_char_p, ctypes.c_long]
         L.ocfd_amr_advance.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, vp]
         L.ocfd_last_error.restype = ctypes.c_char_p
