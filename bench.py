@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-amr", action="store_true", help="skip the two-level DMR")
     ap.add_argument("--amr-only", action="store_true", help="only the two-level DMR (for profiling)")
     ap.add_argument("--amr-steps", type=int, default=10)
+    ap.add_argument("--dmr-to-t", action="store_true",
+                    help="also run the AMR double Mach reflection to t = 0.2 in both precision forms")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (HBM counter calibration)")
     return ap.parse_args()
@@ -203,84 +205,115 @@ def _cells(boxes):
     return n
 
 
-def amr_dmr(steps, warmup=60, nx=960, ny=240, cpu_sample=True):
-    """Two-level double Mach reflection: the reference's DMR inputs (960x240x2,
-    max_grid_size 64, blocking_factor 2, regrid_int 2, refine_dengrad 0.05,
-    SSP-RK3, CFL 0.5) with amr.max_level = 1 and AMReX's default grid
-    efficiency 0.7 (the inputs' grid_eff 1 only matters once a fine level
-    exists; at 1 the fine level falls apart into ~1000 boxes of a few dozen
-    cells); FP64 fluxes, planar stepping of the z-uniform flow.  A cell update
-    is one cell advanced by its level's dt (fine cells count twice per coarse
-    step).  The same time span on a uniform grid at the fine resolution is
+def amr_dmr(steps, warmup=40, nx=1024, ny=256, max_level=2, cpu_sample=True, uniform=True):
+    """Double Mach reflection with AMR: base 1024x256x2 in uneven boxes of 32 to
+    128 cells (seeded), max_level 2 at ratio 2, tagged on density gradient
+    (refine_dengrad 0.05), regrid every 2 steps of each level, blocking factor 2,
+    max_grid_size 64, AMReX's default grid efficiency 0.7; SSP-RK3, CFL 0.5,
+    FP64, planar stepping of the z-uniform flow.  A cell update is one cell
+    advanced by its level's dt (a level-l cell counts 2^l times per coarse
+    step).  The same time span on the uniform grid at the finest resolution is
     timed beside it."""
     import torch
     import ocfd
     from ocfd import amr, cases
     p = ocfd.default_parm()
-    boxes, g, kind = cases.dmr(nx, ny, 2, max_grid=(64, 64, 2))
-    kw = dict(scheme=2, refine_dengrad=0.05, regrid_int=2, blocking_factor=2, grid_eff=0.7, max_grid_size=64)
-    T = amr.TwoLevel(boxes, g, p, 0.5, kind, flags=ocfd.PLANAR, **kw)
+    boxes, g, kind = cases.dmr(nx, ny, 2, sizes=(32, 64, 128), seed=1234)
+    kw = dict(scheme=2, max_level=max_level, refine_dengrad=0.05, regrid_int=2, blocking_factor=2, grid_eff=0.7,
+              max_grid_size=64)
+    H = amr.Hierarchy(boxes, g, p, 0.5, kind, flags=ocfd.PLANAR, **kw)
     for _ in range(warmup):
-        T.step()
-    nc = _cells(boxes)
+        H.step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    upd, nfine, nfbox, dts = 0, [], [], []
+    upd, dts, per_level, nbox = 0, [], [0.0] * (max_level + 1), [0.0] * (max_level + 1)
     for _ in range(steps):
-        dts.append(T.step())
-        nf = _cells(T.fine.boxes) if T.fine is not None else 0
-        nfine.append(nf)
-        nfbox.append(T.fine.nbox if T.fine is not None else 0)
-        upd += nc + 2 * nf
+        dts.append(H.step())
+        c = H.cells()
+        for l, n in enumerate(c):
+            upd += n * 2 ** l
+            per_level[l] += n / steps
+            nbox[l] += H.levels[l].nbox / steps
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    out = {"workload": f"DMR {nx}x{ny}x2 + 1 level (ratio 2, grid_eff 0.7), SSP-RK3, FP64, regrid every 2 "
-                       "coarse steps",
-           "coarse_steps": steps, "ms_per_coarse_step": 1e3 * el / steps,
-           "cell_updates_per_s": upd / el, "coarse_cells": nc, "fine_cells_mean": float(sum(nfine) / steps),
-           "fine_boxes_mean": float(sum(nfbox) / steps), "fine_fraction_of_domain": float(sum(nfine) / steps / (8 * nc))}
-    del T
+    nc = _cells(boxes)
+    out = {"workload": f"DMR {nx}x{ny}x2 base (uneven boxes 32-128) + {max_level} levels of ratio 2, SSP-RK3, FP64",
+           "coarse_steps": steps, "after_steps": warmup, "ms_per_coarse_step": 1e3 * el / steps,
+           "cell_updates_per_s": upd / el, "cells_per_level": per_level, "boxes_per_level": nbox,
+           "fraction_of_domain_per_level": [per_level[l] / (nc * 8 ** l) for l in range(max_level + 1)]}
+    del H
     torch.cuda.empty_cache()
-    # the same time span on the uniform fine grid (two fine steps per coarse dt)
-    fb, fg, _ = cases.dmr(2 * nx, 2 * ny, 4, max_grid=(64, 64, 4))
-    lev = ocfd.Level(fb, fg)
-    A, B, C = lev.new_mf(), lev.new_mf(), lev.new_mf()
-    lev.init_state(kind, A, p)
-    for _ in range(2):
-        lev.advance(A, B, C, 0.0, dts[0] / 2, 2, p, ocfd.PLANAR)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    tt = 0.0
-    for dt in dts:
-        for _ in range(2):
-            lev.advance(A, B, C, tt, dt / 2, 2, p, ocfd.PLANAR)
-            tt += dt / 2
-    torch.cuda.synchronize()
-    elu = time.perf_counter() - t0
-    out["uniform_fine_ms_per_coarse_step"] = 1e3 * elu / steps
-    out["speedup_vs_uniform_fine"] = elu / el
-    del lev, A, B, C
-    torch.cuda.empty_cache()
+    if uniform:
+        r = 2 ** max_level
+        fb, fg, _ = cases.dmr(r * nx, r * ny, 2 * r, max_grid=(128, 128, 2 * r))
+        lev = ocfd.Level(fb, fg)
+        A, B, C = lev.new_mf(), lev.new_mf(), lev.new_mf()
+        lev.init_state(kind, A, p)
+        lev.advance(A, B, C, 0.0, dts[0] / r, 2, p, ocfd.PLANAR)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tt = 0.0
+        for dt in dts:
+            for _ in range(r):
+                lev.advance(A, B, C, tt, dt / r, 2, p, ocfd.PLANAR)
+                tt += dt / r
+        torch.cuda.synchronize()
+        elu = time.perf_counter() - t0
+        out["uniform_finest"] = f"{r * nx}x{r * ny}x{2 * r}, {r} steps per coarse step"
+        out["uniform_finest_ms_per_coarse_step"] = 1e3 * elu / steps
+        out["speedup_vs_uniform_finest"] = elu / el
+        del lev, A, B, C
+        torch.cuda.empty_cache()
     if cpu_sample:
         sys.path.insert(0, str(ROOT / "oracle"))
         import ctypes
         import oracle as orc
         import amr_oracle as ao
-        sb, sg, _ = cases.dmr(240, 60, 2, max_grid=(64, 64, 2))
+        sb, sg, _ = cases.dmr(256, 64, 2, max_grid=(64, 64, 2))
         og = orc.OGeom()
         ctypes.memmove(ctypes.byref(og), ctypes.byref(sg), ctypes.sizeof(og))
-        O = ao.TwoLevel(sb, og, orc.parm(), 0.5, kind, **kw)
+        O = ao.Hierarchy(sb, og, orc.parm(), 0.5, kind, **kw)
         t0 = time.perf_counter()
         upd_c = 0
         for _ in range(2):
             O.step()
-            upd_c += _cells(sb) + 2 * _cells(O.fboxes)
+            upd_c += sum(_cells(O.boxes[l]) * 2 ** l for l in range(O.finest + 1))
         el_c = time.perf_counter() - t0
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = {"value": upd_c / el_c, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-                               "sample": f"AMR oracle (C++ fluxes, numpy AMR steps), DMR 240x60x2 + 1 level, "
-                                         f"2 coarse steps ({el_c:.1f} s)"}
+                               "sample": f"AMR oracle (C++ fluxes, numpy AMR steps), DMR 256x64x2 + {max_level} "
+                                         f"levels, 2 coarse steps ({el_c:.1f} s)"}
     return out
+
+
+def amr_dmr_to_time(stop_time=0.2, nx=1024, ny=256, max_level=2):
+    """The DMR configuration run to t = 0.2 in the FP64 form and in the
+    FP32-flux / FP64-state form: wall time, coarse steps, and the relative L1
+    density difference between the two forms on the base level (averaged down
+    from the finer levels)."""
+    import torch
+    import ocfd
+    from ocfd import amr, cases
+    p = ocfd.default_parm()
+    boxes, g, kind = cases.dmr(nx, ny, 2, sizes=(32, 64, 128), seed=1234)
+    kw = dict(scheme=2, max_level=max_level, refine_dengrad=0.05, regrid_int=2, blocking_factor=2, grid_eff=0.7,
+              max_grid_size=64, stop_time=stop_time)
+    res, rho = {}, {}
+    for name, flags in (("fp64", ocfd.PLANAR), ("fp32_flux", ocfd.PLANAR | ocfd.FLUX_FP32)):
+        H = amr.Hierarchy(boxes, g, p, 0.5, kind, flags=flags, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n = H.run()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[name] = {"wall_s": el, "coarse_steps": n, "time": H.time, "cells_per_level": H.cells()}
+        rho[name] = torch.cat([H.levels[0].fab_view(H.S[0], b)[0, 4:-4, 4:-4, 4:-4].reshape(-1)
+                               for b in range(H.levels[0].nbox)]).cpu()
+        del H
+        torch.cuda.empty_cache()
+    d = (rho["fp32_flux"] - rho["fp64"]).abs().sum() / rho["fp64"].abs().sum()
+    res["rel_L1_density_fp32_vs_fp64"] = float(d)
+    return res
 
 
 def cpu_baseline(n, steps):
@@ -317,7 +350,10 @@ def main():
         calib_copy()
         return
     if args.amr_only:
-        print(json.dumps({"amr_two_level": amr_dmr(args.amr_steps, cpu_sample=False)}))
+        out = {"amr_dmr": amr_dmr(args.amr_steps, cpu_sample=False)}
+        if args.dmr_to_t:
+            out["amr_dmr_to_t02"] = amr_dmr_to_time()
+        print(json.dumps(out))
         return
     n = args.n
     boxes, g, kind = cases.kh3d(n)
@@ -351,6 +387,8 @@ def main():
     amr_res = None
     if not args.no_amr:
         amr_res = amr_dmr(args.amr_steps, cpu_sample=not args.no_cpu_baseline)
+        if args.dmr_to_t:
+            amr_res["to_t02"] = amr_dmr_to_time()
 
     serve_res = None
     if not args.no_serve:
@@ -391,7 +429,7 @@ def main():
         "fp64_form": res64,
         "fp32_over_fp64": (value / res64["value"]) if res64 else None,
         "serving": serve_res,
-        "amr_two_level": amr_res,
+        "amr_dmr": amr_res,
         "cpu_baseline": cpu,
         "chemistry_0d": chem,
     }

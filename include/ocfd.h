@@ -323,11 +323,12 @@ int ocfd_graph_launch(ocfd_graph g, void* stream);
 int ocfd_graph_destroy(ocfd_graph g);
 
 /* ---------------------------------------------------------------------------
- * AMR between two levels of refinement ratio 2 (regular levels).  The fine
- * level's domain is the coarse domain refined by 2 and its boxes are aligned
- * to 2.  The subcycled time step that strings these together is the host
- * driver's (AmrLevel::timeStep semantics: one coarse step, then two fine steps
- * of dt/2, then reflux and average-down; Source/EBR.cpp:231-330).
+ * AMR hierarchies of refinement ratio 2 (regular levels).  Each fine level's
+ * domain is the coarser domain refined by 2, its boxes are aligned to 2 and
+ * nested in the coarser level.  The subcycled time step that strings these
+ * together is the host driver's (Amr::timeStep semantics: a level's step,
+ * then two steps of dt/2 of the next finer level, then reflux and
+ * average-down; Source/EBR.cpp:231-330).
  * ------------------------------------------------------------------------- */
 
 /* tag_dengrad (Source/Tagging.H:8-36) as called by EBR::errorEst
@@ -340,27 +341,16 @@ int ocfd_tag_dengrad(ocfd_level lev, const double* S, double threshold, unsigned
  * errorEst feeds; amr.blocking_factor / amr.grid_eff / amr.max_grid_size of
  * Exec/<case>/inputs): tags dense over [lo, hi] (i fastest, host memory),
  * coarsened by `block` cells, Berger-Rigoutsos clustering to efficiency
- * grid_eff, chopped to max_size_cells.  Boxes (lo[3], hi[3]) in the tag index
- * space go to boxes_out[6*cap]; *nbox_out = count.  Host only. */
-int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigned char* tags, int block, double grid_eff,
-                      int max_size_cells, int cap, int* boxes_out, int* nbox_out);
+ * grid_eff, intersected with the region `mask` allows (dense like tags; the
+ * proper nesting domain; NULL = everywhere), chopped to max_size_cells.
+ * Boxes (lo[3], hi[3]) in the tag index space go to boxes_out[6*cap];
+ * *nbox_out = count.  Host only. */
+int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigned char* tags, const unsigned char* mask, int block,
+                      double grid_eff, int max_size_cells, int cap, int* boxes_out, int* nbox_out);
 
 /* out[0..n) = a*x + b*y: the two-time data combination of FillPatch at a time
  * between a level's old and new states (a = (t1-t)/(t1-t0), b = (t-t0)/(t1-t0)). */
 int ocfd_lincomb(long n, double a, const double* x, double b, const double* y, double* out, void* stream);
-
-/* The coarse part of the two-level FillPatch (FillPatchTwoLevels with the
- * state's cell-conservative linear interpolater, Source/Setup.cpp:196): after
- * ocfd_fill_boundary(fine, S_fine, t), every fine ghost cell whose source cell
- * no fine box owns is interpolated from S_coarse, the coarse state at time t
- * with filled ghost cells. */
-int ocfd_fill_from_coarse(ocfd_level fine, double* S_fine, ocfd_level coarse, const double* S_coarse, void* stream);
-
-/* Valid cells of a new fine level interpolated from the coarse state (the
- * same interpolater; AmrLevel::init of a regridded level), skipping cells the
- * old fine level covers when old_fine is given. */
-int ocfd_interp_from_coarse(ocfd_level fine, double* S_fine, ocfd_level coarse, const double* S_coarse,
-                            ocfd_level old_fine, void* stream);
 
 /* Valid cells of level dst copied from level src (same domain) where src covers them. */
 int ocfd_copy_level(ocfd_level dst, double* S_dst, ocfd_level src, const double* S_src, void* stream);
@@ -385,32 +375,53 @@ int ocfd_cf_add(ocfd_cf c, int fine_side, double w_dt, const ocfd_parm* parm, in
 int ocfd_cf_registers(ocfd_cf c, double* regc, double* regf);
 /* S_coarse += the flux mismatch of every interface face on the coarse side */
 int ocfd_cf_reflux(ocfd_cf c, double* S_coarse, void* stream);
+/* The coarse part of the two-level FillPatch (FillPatchTwoLevels with the
+ * state's cell-conservative linear interpolater, Source/Setup.cpp:196): after
+ * ocfd_fill_boundary(fine, S_fine, t), every fine ghost cell whose source cell
+ * no fine box owns is interpolated from S_coarse, the coarse state at time t
+ * with filled ghost cells. */
+int ocfd_cf_fill_fine(ocfd_cf c, double* S_fine, const double* S_coarse, void* stream);
+
+/* Valid cells of a new fine level interpolated from the coarse state (the
+ * same interpolater; AmrLevel::init of a regridded level), skipping cells the
+ * old fine level covers when old_fine is given. */
+int ocfd_cf_interp_fine(ocfd_cf c, double* S_fine, const double* S_coarse, ocfd_level old_fine, void* stream);
+
 /* EBR::avgDown (Source/EBR.cpp:535-559): coarse cells under the fine level
  * take the mean of their 8 fine cells; ncomp components (5 state, 9 species). */
 int ocfd_cf_avg_down(ocfd_cf c, const double* S_fine, double* S_coarse, int ncomp, void* stream);
 
-/* State buffers of one two-level advance (MultiFabs with 4 ghosts, 5 comps). */
-typedef struct ocfd_amr_bufs {
-    double* c_old;    /* coarse U^n; its ghosts are filled at `time` on return */
-    double* c_new;    /* coarse U^(n+1), refluxed and averaged down on return */
-    double* c_s1;     /* coarse stage scratch */
-    double* c_s2;     /* coarse stage scratch (SSP-RK3) */
-    double* c_t;      /* coarse state at a fine stage time (scratch) */
-    double* f_state;  /* fine U, advanced in place by two steps of dt/2 */
-    double* f_s1;     /* fine stage scratch */
-    double* f_s2;     /* fine stage scratch (SSP-RK3) */
-} ocfd_amr_bufs;
+/* One level of an AMR hierarchy during a time step (MultiFabs with 4 ghosts,
+ * 5 comps; the caller keeps the array across calls). */
+typedef struct ocfd_amr_level_state {
+    double* old;     /* the level's state at t_old */
+    double* cur;     /* the level's state at t_new (the current state) */
+    double* s1;      /* stage scratch */
+    double* s2;      /* stage scratch (SSP-RK3) */
+    double* tmp;     /* the state at an intermediate time (used when a finer level fills from it) */
+    double t_old, t_new;
+} ocfd_amr_level_state;
 
-/* One coarse time step of the two-level hierarchy on `stream`, no host
- * synchronisation (Amr::timeStep of level 0 with subcycling): the coarse
- * advance of EBR::flow_advance (Source/Advance.cpp:81-150) with the coarse
- * register adds, two fine steps of dt0/2 whose stage ghosts come from the
- * fine level and from the coarse level at the stage time (old/new combination
- * of FillPatch), then reflux and average-down (EBR::post_timestep,
- * Source/EBR.cpp:313-330).  cf NULL: the coarse advance alone.  scheme 0
- * Euler, 1 RK2, 2 SSP-RK3; do_reflux as ebr.do_reflux. */
-int ocfd_amr_advance(ocfd_level coarse, ocfd_cf cf, const ocfd_amr_bufs* b, double time, double dt0, int scheme,
-                     const ocfd_parm* parm, int flags, int do_reflux, void* stream);
+/* Advance level l of an nlev hierarchy from t by dt (AmrLevel::advance inside
+ * Amr::timeStep, EBR::flow_advance of Source/Advance.cpp:81-150): st[l].old and
+ * st[l].cur are swapped (cur then holds the state at t) and the stages write the
+ * new state into st[l].cur; each stage's ghost cells come from the level and,
+ * recursively, from every coarser level m at the stage time -- its old/new pair
+ * combined between st[m].t_old and st[m].t_new and ghost-filled the same way
+ * (FillPatch with the two-level interpolater).  Register adds after each stage:
+ * cfs[l+1] as the coarse side (reset first), cfs[l] as the fine side.
+ * cfs[m] joins levels m-1 and m (cfs[0] unused).  The caller runs the finer
+ * levels' subcycles after this call and then ocfd_cf_reflux / ocfd_cf_avg_down
+ * (EBR::post_timestep, Source/EBR.cpp:313-330).  No host synchronisation. */
+int ocfd_amr_advance_level(int nlev, const ocfd_level* levels, const ocfd_cf* cfs, ocfd_amr_level_state* st, int l,
+                           double t, double dt, int scheme, const ocfd_parm* parm, int flags, int do_reflux,
+                           void* stream);
+
+/* Ghost cells of X, a state of level l at time t, filled as a stage's are in
+ * ocfd_amr_advance_level (FillPatch of level l: the level, then the coarser
+ * levels at time t, recursively).  For regrids and output. */
+int ocfd_amr_fill_level(int nlev, const ocfd_level* levels, const ocfd_cf* cfs, ocfd_amr_level_state* st, int l,
+                        double* X, double t, void* stream);
 
 /* ---------------------------------------------------------------------------
  * On-disk formats

@@ -217,16 +217,20 @@ __device__ __forceinline__ double mc_limit(double dc, double um, double u0, doub
 // of the coarse state C at fine cell (i,j,k): MC-limited central slopes, then
 // one factor alpha <= 1 that keeps every fine value of the parent within the
 // min/max of the parent's 27 coarse neighbours; fine value =
-// u + xoff*sx + yoff*sy + zoff*sz with offsets +-1/4.  Returns false when no
-// coarse box holds the parent cell.
+// u + xoff*sx + yoff*sy + zoff*sz with offsets +-1/4.  cmap gives, per coarse
+// cell, the coarse box to read it from: its owner, or else a box whose filled
+// ghost region holds the cell and its neighbours (a coarse level that does not
+// cover the domain).  Returns false when there is none.
 __device__ __forceinline__ bool cons_interp(const BoxDesc* __restrict__ cboxes, const GeomDesc& CG,
-                                            const double* __restrict__ C, int i, int j, int k, int n, double& v)
+                                            const int* __restrict__ cmap, const double* __restrict__ C, int i, int j,
+                                            int k, int n, double& v)
 {
     const int I = i >> 1, J = j >> 1, K = k >> 1;
     if (I < CG.dom_lo[0] || I > CG.dom_hi[0] || J < CG.dom_lo[1] || J > CG.dom_hi[1] || K < CG.dom_lo[2] ||
         K > CG.dom_hi[2])
         return false;
-    const int m = owner(CG, I, J, K);
+    const long nx = CG.dom_hi[0] - CG.dom_lo[0] + 1, ny = CG.dom_hi[1] - CG.dom_lo[1] + 1;
+    const int m = cmap[((long)(K - CG.dom_lo[2]) * ny + (J - CG.dom_lo[1])) * nx + (I - CG.dom_lo[0])];
     if (m < 0) return false;
     const BoxDesc Bc = cboxes[m];
     const double* u = C + n * Bc.nptg;
@@ -270,7 +274,7 @@ __device__ __forceinline__ bool cons_interp(const BoxDesc* __restrict__ cboxes, 
 __global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fboxes, const GeomDesc* __restrict__ fgeom,
                                                  const int4* __restrict__ tiles, double* __restrict__ Sf,
                                                  const BoxDesc* __restrict__ cboxes, const GeomDesc* __restrict__ cgeom,
-                                                 const double* __restrict__ Sc)
+                                                 const int* __restrict__ cmap, const double* __restrict__ Sc)
 {
     const int4 t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.z) return;
@@ -316,7 +320,7 @@ __global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fbo
             continue;
         if (owner(G, src[0], src[1], src[2]) >= 0) continue;   // the level fill copied it
         double v;
-        if (cons_interp(cboxes, *cgeom, Sc, src[0], src[1], src[2], n, v)) Sf[dst + n * B.nptg] = sign * v;
+        if (cons_interp(cboxes, *cgeom, cmap, Sc, src[0], src[1], src[2], n, v)) Sf[dst + n * B.nptg] = sign * v;
     }
 }
 
@@ -324,8 +328,9 @@ __global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fbo
 // cells that the old fine level (when given) covers are skipped
 __global__ __launch_bounds__(256) void k_interp_valid(const BoxDesc* __restrict__ fboxes, const int4* __restrict__ tiles,
                                                       double* __restrict__ Sf, const BoxDesc* __restrict__ cboxes,
-                                                      const GeomDesc* __restrict__ cgeom, const double* __restrict__ Sc,
-                                                      const BoxDesc* __restrict__ oboxes, const GeomDesc* __restrict__ ogeom)
+                                                      const GeomDesc* __restrict__ cgeom, const int* __restrict__ cmap,
+                                                      const double* __restrict__ Sc, const BoxDesc* __restrict__ oboxes,
+                                                      const GeomDesc* __restrict__ ogeom)
 {
     const int4 t = tiles[blockIdx.x];
     const BoxDesc B = fboxes[t.x];
@@ -343,7 +348,7 @@ __global__ __launch_bounds__(256) void k_interp_valid(const BoxDesc* __restrict_
         const long dst = sidx(B, i, j, k);
         for (int n = 0; n < NC; ++n) {
             double v;
-            if (cons_interp(cboxes, *cgeom, Sc, i, j, k, n, v)) Sf[dst + n * B.nptg] = v;
+            if (cons_interp(cboxes, *cgeom, cmap, Sc, i, j, k, n, v)) Sf[dst + n * B.nptg] = v;
         }
     }
 }
@@ -419,6 +424,8 @@ struct ocfd_cf_s {
     double* d_regf = nullptr;    // 5 per fine sub-face
     int4* d_tile_avg = nullptr;  // coarsened fine boxes, 256 coarse cells per tile
     int ntile_avg = 0;
+    int* d_cmap = nullptr;       // coarse cell -> box to interpolate from (owner, else a box whose
+                                 // grown-by-3 region holds it), -1 none; over the coarse domain
 };
 
 namespace {
@@ -466,7 +473,7 @@ int launched(const char* what)
 void free_cf(ocfd_cf c)
 {
     hipFree(c->d_cf); hipFree(c->d_ff); hipFree(c->d_cells); hipFree(c->d_cellfaces);
-    hipFree(c->d_regc); hipFree(c->d_regf); hipFree(c->d_tile_avg);
+    hipFree(c->d_regc); hipFree(c->d_regf); hipFree(c->d_tile_avg); hipFree(c->d_cmap);
 }
 
 bool refined_by_2(const LevelDev& C, const LevelDev& F)
@@ -594,13 +601,45 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
         const long n = (long)((B.hi[0] - B.lo[0] + 1) / 2) * ((B.hi[1] - B.lo[1] + 1) / 2) * ((B.hi[2] - B.lo[2] + 1) / 2);
         for (long s = 0; s < n; s += 256) ta.push_back(make_int4(b, (int)s, (int)std::min<long>(256, n - s), 0));
     }
+    // coarse cell -> box to interpolate from: the owner, else the first box
+    // whose grown-by-3 region holds the cell (its 27 neighbours then lie in
+    // that box's filled ghost region)
+    std::vector<int> cmap((size_t)(n0 * n1 * n2), -1);
+    for (int b = 0; b < C.nbox; ++b) {
+        const BoxDesc& B = C.boxes[b];
+        for (int K = B.lo[2]; K <= B.hi[2]; ++K)
+            for (int J = B.lo[1]; J <= B.hi[1]; ++J)
+                for (int I = B.lo[0]; I <= B.hi[0]; ++I) cmap[lin(I, J, K)] = b;
+    }
+    for (int b = 0; b < C.nbox; ++b) {
+        const BoxDesc& B = C.boxes[b];
+        int lo3[3], hi3[3];
+        for (int d = 0; d < 3; ++d) {
+            lo3[d] = std::max(B.lo[d] - 3, CG.dom_lo[d]);
+            hi3[d] = std::min(B.hi[d] + 3, CG.dom_hi[d]);
+        }
+        for (int K = lo3[2]; K <= hi3[2]; ++K)
+            for (int J = lo3[1]; J <= hi3[1]; ++J)
+                for (int I = lo3[0]; I <= hi3[0]; ++I)
+                    if (cmap[lin(I, J, K)] < 0) cmap[lin(I, J, K)] = b;
+    }
+    // the fine level must lie in the coarse level's valid cells
+    for (const BoxDesc& B : F.boxes)
+        for (int K = B.lo[2] >> 1; K <= B.hi[2] >> 1; ++K)
+            for (int J = B.lo[1] >> 1; J <= B.hi[1] >> 1; ++J)
+                for (int I = B.lo[0] >> 1; I <= B.hi[0] >> 1; ++I)
+                    if (find_box(C, I, J, K) < 0) {
+                        set_error("ocfd_cf_create: the fine level is not nested in the coarse level");
+                        return OCFD_EINVAL;
+                    }
     ocfd_cf c = new ocfd_cf_s;
     c->coarse = coarse; c->fine = fine;
     c->nface = (int)cf.size(); c->ncell = (int)cells.size();
     c->ntile_avg = (int)ta.size();
     int rc;
     if ((rc = upload_vec(&c->d_cf, cf)) || (rc = upload_vec(&c->d_ff, ff)) || (rc = upload_vec(&c->d_cells, cells)) ||
-        (rc = upload_vec(&c->d_cellfaces, cellfaces)) || (rc = upload_vec(&c->d_tile_avg, ta))) {
+        (rc = upload_vec(&c->d_cellfaces, cellfaces)) || (rc = upload_vec(&c->d_tile_avg, ta)) ||
+        (rc = upload_vec(&c->d_cmap, cmap))) {
         set_error("ocfd_cf_create: device allocation failed");
         free_cf(c);
         delete c;
@@ -726,42 +765,40 @@ int ocfd_cf_avg_down(ocfd_cf c, const double* S_fine, double* S_coarse, int ncom
     return launched("ocfd_cf_avg_down");
 }
 
-int ocfd_fill_from_coarse(ocfd_level fine, double* S_fine, ocfd_level coarse, const double* S_coarse, void* stream)
+int ocfd_cf_fill_fine(ocfd_cf c, double* S_fine, const double* S_coarse, void* stream)
 {
-    if (!fine || !coarse || !S_fine || !S_coarse) {
-        set_error("ocfd_fill_from_coarse: invalid arguments");
+    if (!c || !S_fine || !S_coarse) {
+        set_error("ocfd_cf_fill_fine: invalid arguments");
         return OCFD_EINVAL;
     }
-    const LevelDev& F = fine->dev;
-    const LevelDev& C = coarse->dev;
-    if (!refined_by_2(C, F) || !aligned_by_2(F)) {
-        set_error("ocfd_fill_from_coarse: the fine level must refine the coarse domain by 2 with boxes aligned to 2");
-        return OCFD_EINVAL;
-    }
+    const LevelDev& F = c->fine->dev;
+    const LevelDev& C = c->coarse->dev;
     if (F.ntile_g)
         hipLaunchKernelGGL(k_fill_cf, dim3(F.ntile_g), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_geom, F.d_tile_g,
-                           S_fine, C.d_boxes, C.d_geom, S_coarse);
-    return launched("ocfd_fill_from_coarse");
+                           S_fine, C.d_boxes, C.d_geom, c->d_cmap, S_coarse);
+    return launched("ocfd_cf_fill_fine");
 }
 
-int ocfd_interp_from_coarse(ocfd_level fine, double* S_fine, ocfd_level coarse, const double* S_coarse,
-                            ocfd_level old_fine, void* stream)
+int ocfd_cf_interp_fine(ocfd_cf c, double* S_fine, const double* S_coarse, ocfd_level old_fine, void* stream)
 {
-    if (!fine || !coarse || !S_fine || !S_coarse) {
-        set_error("ocfd_interp_from_coarse: invalid arguments");
+    if (!c || !S_fine || !S_coarse) {
+        set_error("ocfd_cf_interp_fine: invalid arguments");
         return OCFD_EINVAL;
     }
-    const LevelDev& F = fine->dev;
-    const LevelDev& C = coarse->dev;
-    if (!refined_by_2(C, F) || !aligned_by_2(F) || (old_fine && !refined_by_2(C, old_fine->dev))) {
-        set_error("ocfd_interp_from_coarse: the fine levels must refine the coarse domain by 2");
-        return OCFD_EINVAL;
+    const LevelDev& F = c->fine->dev;
+    const LevelDev& C = c->coarse->dev;
+    if (old_fine) {
+        for (int d = 0; d < 3; ++d)
+            if (old_fine->dev.geom.dom_lo[d] != F.geom.dom_lo[d] || old_fine->dev.geom.dom_hi[d] != F.geom.dom_hi[d]) {
+                set_error("ocfd_cf_interp_fine: the old fine level must share the fine domain");
+                return OCFD_EINVAL;
+            }
     }
     if (F.ntile_c)
         hipLaunchKernelGGL(k_interp_valid, dim3(F.ntile_c), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_tile_c,
-                           S_fine, C.d_boxes, C.d_geom, S_coarse, old_fine ? old_fine->dev.d_boxes : nullptr,
+                           S_fine, C.d_boxes, C.d_geom, c->d_cmap, S_coarse, old_fine ? old_fine->dev.d_boxes : nullptr,
                            old_fine ? old_fine->dev.d_geom : nullptr);
-    return launched("ocfd_interp_from_coarse");
+    return launched("ocfd_cf_interp_fine");
 }
 
 int ocfd_copy_level(ocfd_level dst, double* S_dst, ocfd_level src, const double* S_src, void* stream)
@@ -830,84 +867,111 @@ void stage_coefs(int scheme, int s, double& c_cur, double& c_old, double& c_dt)
     else { c_cur = 2.0 / 3.0; c_old = 1.0 / 3.0; c_dt = 2.0 / 3.0; }
 }
 
-// The stage sequence of one level from U (old) into `out`: scratch s1, s2;
-// `fill` fills the ghosts of a stage input at a time; the register add runs
-// after each stage when cf is given.  out may be U (the last stage reads U
-// only at its own cell).
-template <class Fill>
-int run_stages(ocfd_level lev, double* U, double* out, double* s1, double* s2, double t, double dt, int scheme,
-               const ocfd_parm* parm, int flags, ocfd_cf cf, int side, Fill&& fill, hipStream_t st)
-{
-    const int ns = scheme + 1;
-    double* in[3] = {U, s1, s2};
-    double* dst[3] = {s1, s2, out};
-    if (ns == 1) dst[0] = out;
-    if (ns == 2) dst[1] = out;
-    int rc;
-    for (int s = 0; s < ns; ++s) {
-        if ((rc = fill(in[s], t + kStageT[scheme][s] * dt))) return rc;
-        double c_cur, c_old, c_dt;
-        stage_coefs(scheme, s, c_cur, c_old, c_dt);
-        if ((rc = ocfd_rk_stage(lev, in[s], s ? U : nullptr, dst[s], c_cur, c_old, c_dt * dt, parm, flags, st)))
+} // namespace
+
+namespace {
+
+// A hierarchy during one level advance: fills stage ghosts of level l from the
+// level itself and, recursively, from the coarser levels at the same time.
+struct Hier {
+    int nlev;
+    const ocfd_level* levels;
+    const ocfd_cf* cfs;
+    ocfd_amr_level_state* st;
+    hipStream_t s;
+
+    // ghost cells of X (a state of level l) at time ts
+    int fill(int l, double* X, double ts)
+    {
+        int rc;
+        if ((rc = ocfd_fill_boundary(levels[l], X, ts, s))) return rc;
+        if (l == 0) return OCFD_OK;
+        const double* C = nullptr;
+        if ((rc = at(l - 1, ts, C))) return rc;
+        return ocfd_cf_fill_fine(cfs[l], X, C, s);
+    }
+
+    // state of level m at time ts (between its t_old and t_new) with filled ghosts
+    int at(int m, double ts, const double*& out)
+    {
+        ocfd_amr_level_state& q = st[m];
+        const double span = q.t_new - q.t_old;
+        int rc;
+        if (std::fabs(ts - q.t_new) <= 1e-10 * span) {   // also a level not advanced yet (span 0)
+            if ((rc = fill(m, q.cur, q.t_new))) return rc;
+            out = q.cur;
+            return OCFD_OK;
+        }
+        if (std::fabs(ts - q.t_old) <= 1e-10 * span) {
+            out = q.old;        // ghosts filled at t_old by the level's first stage
+            return OCFD_OK;
+        }
+        if ((rc = ocfd_lincomb(levels[m]->dev.mf_state, (q.t_new - ts) / span, q.old, (ts - q.t_old) / span, q.cur,
+                               q.tmp, s)))
             return rc;
-        if (cf && (rc = ocfd_cf_add(cf, side, kStageW[scheme][s] * dt, parm, flags, st))) return rc;
+        if ((rc = fill(m, q.tmp, ts))) return rc;
+        out = q.tmp;
+        return OCFD_OK;
+    }
+};
+
+} // namespace
+
+extern "C" int ocfd_amr_advance_level(int nlev, const ocfd_level* levels, const ocfd_cf* cfs, ocfd_amr_level_state* st,
+                                      int l, double t, double dt, int scheme, const ocfd_parm* parm, int flags,
+                                      int do_reflux, void* stream)
+{
+    if (nlev < 1 || !levels || !st || l < 0 || l >= nlev || !parm || scheme < 0 || scheme > 2 || !(dt > 0.0) ||
+        (nlev > 1 && !cfs)) {
+        set_error("ocfd_amr_advance_level: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    for (int m = 0; m <= std::min(l + 1, nlev - 1); ++m) {
+        const ocfd_amr_level_state& q = st[m];
+        if (!levels[m] || !q.old || !q.cur || (m == l && (!q.s1 || (scheme == 2 && !q.s2))) ||
+            (m < l && !q.tmp) || (m > 0 && (!cfs[m] || cfs[m]->coarse != levels[m - 1] || cfs[m]->fine != levels[m]))) {
+            set_error("ocfd_amr_advance_level: level " + std::to_string(m) + " is incomplete");
+            return OCFD_EINVAL;
+        }
+    }
+    hipStream_t s = (hipStream_t)stream;
+    Hier h{nlev, levels, cfs, st, s};
+    ocfd_amr_level_state& q = st[l];
+    std::swap(q.old, q.cur);
+    q.t_old = t;
+    q.t_new = t + dt;
+    ocfd_cf reg_c = (do_reflux && l + 1 < nlev) ? cfs[l + 1] : nullptr;   // this level as the coarse side
+    ocfd_cf reg_f = (do_reflux && l > 0) ? cfs[l] : nullptr;              // this level as the fine side
+    int rc;
+    if (reg_c && (rc = ocfd_cf_reset(reg_c, s))) return rc;
+    const int ns = scheme + 1;
+    double* in[3] = {q.old, q.s1, q.s2};
+    double* dst[3] = {q.s1, q.s2, q.cur};
+    dst[ns - 1] = q.cur;
+    for (int k = 0; k < ns; ++k) {
+        if ((rc = h.fill(l, in[k], t + kStageT[scheme][k] * dt))) return rc;
+        double c_cur, c_old, c_dt;
+        stage_coefs(scheme, k, c_cur, c_old, c_dt);
+        if ((rc = ocfd_rk_stage(levels[l], in[k], k ? q.old : nullptr, dst[k], c_cur, c_old, c_dt * dt, parm, flags, s)))
+            return rc;
+        if (reg_c && (rc = ocfd_cf_add(reg_c, 0, kStageW[scheme][k] * dt, parm, flags, s))) return rc;
+        if (reg_f && (rc = ocfd_cf_add(reg_f, 1, kStageW[scheme][k] * dt, parm, flags, s))) return rc;
     }
     return OCFD_OK;
 }
 
-} // namespace
-
-extern "C" int ocfd_amr_advance(ocfd_level coarse, ocfd_cf cf, const ocfd_amr_bufs* b, double time, double dt0,
-                                int scheme, const ocfd_parm* parm, int flags, int do_reflux, void* stream)
+extern "C" int ocfd_amr_fill_level(int nlev, const ocfd_level* levels, const ocfd_cf* cfs, ocfd_amr_level_state* st,
+                                   int l, double* X, double t, void* stream)
 {
-    if (!coarse || !b || !parm || scheme < 0 || scheme > 2 || !(dt0 > 0.0) || !b->c_old || !b->c_new || !b->c_s1 ||
-        (scheme == 2 && !b->c_s2) || (cf && (cf->coarse != coarse || !b->c_t || !b->f_state || !b->f_s1 ||
-                                            (scheme == 2 && !b->f_s2)))) {
-        set_error("ocfd_amr_advance: invalid arguments");
+    if (nlev < 1 || !levels || !st || l < 0 || l >= nlev || !X || (l > 0 && !cfs)) {
+        set_error("ocfd_amr_fill_level: invalid arguments");
         return OCFD_EINVAL;
     }
-    hipStream_t st = (hipStream_t)stream;
-    int rc;
-    ocfd_cf reg = (cf && do_reflux) ? cf : nullptr;
-    if (reg && (rc = ocfd_cf_reset(reg, st))) return rc;
-    auto fill_c = [&](double* X, double ts) { return ocfd_fill_boundary(coarse, X, ts, st); };
-    if ((rc = run_stages(coarse, b->c_old, b->c_new, b->c_s1, b->c_s2, time, dt0, scheme, parm, flags, reg, 0,
-                         fill_c, st)))
-        return rc;
-    if (!cf) return OCFD_OK;
-    const double t0 = time, t1 = time + dt0, span = dt0;
-    ocfd_level fine = cf->fine;
-    const long nc = coarse->dev.mf_state;
-    // fine stage ghosts: the fine level, then the coarse state at the stage time
-    auto fill_f = [&](double* X, double ts) {
-        int r;
-        if ((r = ocfd_fill_boundary(fine, X, ts, st))) return r;
-        const double* C;
-        if (std::fabs(ts - t0) <= 1e-10 * span) {
-            C = b->c_old;                      // ghosts filled at t0 by the coarse stage 1
-        } else if (std::fabs(ts - t1) <= 1e-10 * span) {
-            if ((r = ocfd_fill_boundary(coarse, b->c_new, t1, st))) return r;
-            C = b->c_new;
-        } else {
-            if ((r = ocfd_lincomb(nc, (t1 - ts) / span, b->c_old, (ts - t0) / span, b->c_new, b->c_t, st))) return r;
-            if ((r = ocfd_fill_boundary(coarse, b->c_t, ts, st))) return r;
-            C = b->c_t;
+    for (int m = 0; m < l; ++m)
+        if (!levels[m] || !st[m].cur || !st[m].old || !st[m].tmp || !cfs[m + 1]) {
+            set_error("ocfd_amr_fill_level: coarser level " + std::to_string(m) + " is incomplete");
+            return OCFD_EINVAL;
         }
-        return ocfd_fill_from_coarse(fine, X, coarse, C, st);
-    };
-    const double dt1 = 0.5 * dt0;
-    for (int it = 0; it < 2; ++it) {
-        double* out = b->f_state;
-        if (scheme == 0) out = b->f_s1;   // Euler: out of place, copied back below
-        if ((rc = run_stages(fine, b->f_state, out, b->f_s1, b->f_s2, t0 + it * dt1, dt1, scheme, parm, flags, reg, 1,
-                             fill_f, st)))
-            return rc;
-        if (scheme == 0 && hipMemcpyAsync(b->f_state, b->f_s1, sizeof(double) * fine->dev.mf_state,
-                                          hipMemcpyDeviceToDevice, st) != hipSuccess) {
-            set_error("ocfd_amr_advance: copy failed");
-            return OCFD_EHIP;
-        }
-    }
-    if (reg && (rc = ocfd_cf_reflux(reg, b->c_new, st))) return rc;
-    return ocfd_cf_avg_down(cf, b->f_state, b->c_new, OCFD_NCONS, st);
+    Hier h{nlev, levels, cfs, st, (hipStream_t)stream};
+    return h.fill(l, X, t);
 }

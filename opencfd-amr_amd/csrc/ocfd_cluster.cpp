@@ -5,7 +5,9 @@
 // efficiency (the cluster/cut rules of AMReX's ClusterList: hole cuts first,
 // then the strongest inflection of the signature's second difference, then
 // bisection; among equal kinds the direction whose smaller part is largest),
-// refined back by the blocking factor, joined where two boxes share a face
+// intersected with the region the boxes may cover (the proper nesting
+// domain of the level being refined), refined back by the blocking factor,
+// joined where two boxes share a face
 // (the BoxList::simplify step), and chopped to the maximum grid size
 // (balanced pieces, AMReX BoxList::maxSize).  Host code; no device work.
 #include <algorithm>
@@ -194,8 +196,8 @@ void max_size(std::vector<IBox>& boxes, int chunk)
 
 } // namespace
 
-extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigned char* tags, int block,
-                                 double grid_eff, int max_size_cells, int cap, int* boxes_out, int* nbox_out)
+extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigned char* tags, const unsigned char* mask,
+                                 int block, double grid_eff, int max_size_cells, int cap, int* boxes_out, int* nbox_out)
 {
     using ocfd::set_error;
     if (!lo || !hi || !tags || !nbox_out || block < 1 || max_size_cells < block || max_size_cells % block ||
@@ -225,6 +227,17 @@ extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigne
             for (int i = 0; i < nb[0]; ++i)
                 if (bt[((size_t)k * nb[1] + j) * nb[0] + i])
                     all.pts.push_back(Pt{{lo[0] / block + i, lo[1] / block + j, lo[2] / block + k}});
+    // the region boxes may cover, at blocking-factor resolution (a block only
+    // when all its cells are in the mask)
+    std::vector<unsigned char> bm;
+    if (mask) {
+        bm.assign(bt.size(), 1);
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i)
+                    if (!mask[((size_t)k * n[1] + j) * n[0] + i])
+                        bm[((size_t)(k / block) * nb[1] + j / block) * nb[0] + i / block] = 0;
+    }
     std::vector<IBox> boxes;
     if (!all.pts.empty()) {
         all.min_box();
@@ -234,7 +247,31 @@ extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigne
             if (it->eff() < grid_eff) lst.push_back(chop(*it));
             else ++it;
         }
-        for (const Cluster& c : lst) {
+        // clusters intersected with the mask region (ClusterList::intersect with
+        // the proper nesting domain): a box partly outside is replaced by an
+        // exact box cover of its part inside
+        std::list<Cluster> kept;
+        for (Cluster& c : lst) {
+            if (!mask) { kept.push_back(std::move(c)); continue; }
+            Cluster in;
+            for (int k = c.box.lo[2]; k <= c.box.hi[2]; ++k)
+                for (int j = c.box.lo[1]; j <= c.box.hi[1]; ++j)
+                    for (int i = c.box.lo[0]; i <= c.box.hi[0]; ++i) {
+                        const int ii = i - lo[0] / block, jj = j - lo[1] / block, kk = k - lo[2] / block;
+                        if (bm[((size_t)kk * nb[1] + jj) * nb[0] + ii]) in.pts.push_back(Pt{{i, j, k}});
+                    }
+            if ((long)in.pts.size() == c.box.volume()) { kept.push_back(std::move(c)); continue; }
+            if (in.pts.empty()) continue;
+            in.min_box();
+            std::list<Cluster> part;
+            part.push_back(std::move(in));
+            for (auto it = part.begin(); it != part.end();) {
+                if (it->eff() < 1.0) part.push_back(chop(*it));
+                else ++it;
+            }
+            for (Cluster& q : part) kept.push_back(std::move(q));
+        }
+        for (const Cluster& c : kept) {
             IBox b = c.box;
             for (int d = 0; d < 3; ++d) {
                 b.lo[d] *= block;

@@ -153,11 +153,11 @@ def lib():
                                         ctypes.c_int, ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, vp]
         L.ocfd_init_species_state.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(Parm)]
         L.ocfd_tag_dengrad.argtypes = [vp, vp, ctypes.c_double, vp, vp]
-        L.ocfd_cluster_tags.argtypes = [ip, ip, ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int, ctypes.c_double,
-                                        ctypes.c_int, ctypes.c_int, ip, ip]
+        L.ocfd_cluster_tags.argtypes = [ip, ip, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte),
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ip, ip]
         L.ocfd_lincomb.argtypes = [ctypes.c_long, ctypes.c_double, vp, ctypes.c_double, vp, vp, vp]
-        L.ocfd_fill_from_coarse.argtypes = [vp, vp, vp, vp, vp]
-        L.ocfd_interp_from_coarse.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.ocfd_cf_fill_fine.argtypes = [vp, vp, vp, vp]
+        L.ocfd_cf_interp_fine.argtypes = [vp, vp, vp, vp, vp]
         L.ocfd_copy_level.argtypes = [vp, vp, vp, vp, vp]
         L.ocfd_cf_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
         L.ocfd_cf_destroy.argtypes = [vp]
@@ -173,8 +173,9 @@ def lib():
                                           ctypes.c_double, ip, ctypes.POINTER(Parm), vp]
         L.ocfd_plotfile_compare.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_int, ip, ctypes.c_char_p, ctypes.c_long]
-        L.ocfd_amr_advance.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
-                                       ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, vp]
+        L.ocfd_amr_advance_level.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, vp]
+        L.ocfd_amr_fill_level.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, vp, ctypes.c_double, vp]
         L.ocfd_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib

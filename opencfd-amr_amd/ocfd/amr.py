@@ -74,9 +74,10 @@ def dense_tags(level, tags):
     return out
 
 
-def cluster_tags(dense, lo, block=1, grid_eff=0.7, max_size=32):
+def cluster_tags(dense, lo, block=1, grid_eff=0.7, max_size=32, mask=None):
     """Boxes (in the tags' index space, domain lower corner `lo`) covering the
-    tagged cells of a dense (nz, ny, nx) array."""
+    tagged cells of a dense (nz, ny, nx) array, within the region `mask`
+    allows (same shape) when given."""
     dense = np.ascontiguousarray(dense, dtype=np.uint8)
     nz, ny, nx = dense.shape
     lo_c = (ctypes.c_int * 3)(*lo)
@@ -84,33 +85,61 @@ def cluster_tags(dense, lo, block=1, grid_eff=0.7, max_size=32):
     cap = max(16, dense.size)
     out = (ctypes.c_int * (6 * cap))()
     nb = ctypes.c_int()
-    check(lib().ocfd_cluster_tags(lo_c, hi_c, dense.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), block, grid_eff,
+    up = ctypes.POINTER(ctypes.c_ubyte)
+    mk = None
+    if mask is not None:
+        mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        mk = mask.ctypes.data_as(up)
+    check(lib().ocfd_cluster_tags(lo_c, hi_c, dense.ctypes.data_as(up), mk, block, grid_eff,
                                   max_size, cap, out, ctypes.byref(nb)), "ocfd_cluster_tags")
     return [(tuple(out[6 * b:6 * b + 3]), tuple(out[6 * b + 3:6 * b + 6])) for b in range(nb.value)]
-
-
-def fill_from_coarse(fine, S_fine, coarse, S_coarse, stream=None):
-    check(lib().ocfd_fill_from_coarse(fine.handle, _ptr(S_fine), coarse.handle, _ptr(S_coarse), _stream(stream)),
-          "ocfd_fill_from_coarse")
-
-
-def interp_from_coarse(fine, S_fine, coarse, S_coarse, old_fine=None, stream=None):
-    check(lib().ocfd_interp_from_coarse(fine.handle, _ptr(S_fine), coarse.handle, _ptr(S_coarse),
-                                        old_fine.handle if old_fine is not None else None, _stream(stream)),
-          "ocfd_interp_from_coarse")
 
 
 def copy_level(dst, S_dst, src, S_src, stream=None):
     check(lib().ocfd_copy_level(dst.handle, _ptr(S_dst), src.handle, _ptr(S_src), _stream(stream)), "ocfd_copy_level")
 
 
-class AmrBufs(ctypes.Structure):
-    """struct ocfd_amr_bufs: the state buffers of one two-level advance."""
-    _fields_ = [(n, ctypes.c_void_p) for n in ("c_old", "c_new", "c_s1", "c_s2", "c_t", "f_state", "f_s1", "f_s2")]
+def _dilate(mask, n, periodic):
+    """mask grown by n cells (Chebyshev); periodic axes wrap, other sides see False."""
+    if n <= 0:
+        return mask.copy()
+    from scipy.ndimage import maximum_filter
+    pads = []
+    a = mask
+    for ax, d in ((0, 2), (1, 1), (2, 0)):
+        mode = "wrap" if periodic[d] else "constant"
+        w = [(0, 0)] * 3
+        w[ax] = (n, n)
+        a = np.pad(a, w, mode=mode)
+        pads.append(n)
+    a = maximum_filter(a.astype(np.uint8), size=2 * n + 1, mode="constant", cval=0).astype(bool)
+    return a[n:-n, n:-n, n:-n]
+
+
+def coverage(boxes, geom):
+    """Dense (nz, ny, nx) coverage of a level's domain by its boxes."""
+    lo = [geom.dom_lo[d] for d in range(3)]
+    n = [geom.dom_hi[d] - lo[d] + 1 for d in range(3)]
+    m = np.zeros((n[2], n[1], n[0]), dtype=bool)
+    for blo, bhi in boxes:
+        m[blo[2] - lo[2]:bhi[2] - lo[2] + 1, blo[1] - lo[1]:bhi[1] - lo[1] + 1, blo[0] - lo[0]:bhi[0] - lo[0] + 1] = True
+    return m
+
+
+def proper_nesting(boxes, geom, n_proper):
+    """Cells of a level whose n_proper-neighbourhood (inside the domain, periodic
+    images included) is covered by the level: where the next finer level may go."""
+    per = [geom.periodic[d] for d in range(3)]
+    return ~_dilate(~coverage(boxes, geom), n_proper, per)
+
+
+def coarsen_boxes(boxes):
+    return [(tuple(v // RATIO for v in lo), tuple(v // RATIO for v in hi)) for lo, hi in boxes]
 
 
 class FluxRegister:
-    """The coarse/fine interface of one fine level (EBR's flux_reg)."""
+    """The coarse/fine interface of one fine level (EBR's flux_reg), with the
+    coarse-to-fine interpolation of that pair."""
 
     def __init__(self, coarse, fine):
         self.coarse, self.fine = coarse, fine
@@ -152,147 +181,305 @@ class FluxRegister:
         check(lib().ocfd_cf_avg_down(self.handle, _ptr(S_fine), _ptr(S_coarse), ncomp, _stream(stream)),
               "ocfd_cf_avg_down")
 
+    def fill_fine(self, S_fine, S_coarse, stream=None):
+        """Fine ghosts no fine box owns, from the coarse state (ghosts filled)."""
+        check(lib().ocfd_cf_fill_fine(self.handle, _ptr(S_fine), _ptr(S_coarse), _stream(stream)), "ocfd_cf_fill_fine")
 
-class TwoLevel:
-    """A coarse level covering the domain plus one fine level of ratio 2.
+    def interp_fine(self, S_fine, S_coarse, old_fine=None, stream=None):
+        """Valid fine cells from the coarse state, except where old_fine covers them."""
+        check(lib().ocfd_cf_interp_fine(self.handle, _ptr(S_fine), _ptr(S_coarse),
+                                        old_fine.handle if old_fine is not None else None, _stream(stream)),
+              "ocfd_cf_interp_fine")
 
-    coarse_boxes, geom: the base level.  kind: case initial data of
-    ocfd_init_state (None: set the states yourself and call start()).
-    fine_boxes: fixed fine grids (coarse-index boxes are refined), else the
-    fine grids come from tagging.  regrid_int 0 keeps the grids fixed.
-    blocking_factor and max_grid_size are in fine cells (amr.* inputs).
+
+class LevelState(ctypes.Structure):
+    """struct ocfd_amr_level_state."""
+    _fields_ = [("old", ctypes.c_void_p), ("cur", ctypes.c_void_p), ("s1", ctypes.c_void_p), ("s2", ctypes.c_void_p),
+                ("tmp", ctypes.c_void_p), ("t_old", ctypes.c_double), ("t_new", ctypes.c_double)]
+
+
+class Hierarchy:
+    """A base level covering the domain and up to max_level finer levels of
+    ratio 2, advanced with subcycling.
+
+    coarse_boxes, geom: level 0.  kind: case initial data of ocfd_init_state
+    (None: set level 0 with set_state and call start()).  fine_boxes: fixed
+    grids -- a list per finer level, each in the index space of the level
+    below (refined when built) -- instead of grids from tagging.  regrid_int 0
+    keeps the grids.  blocking_factor and max_grid_size are in the finer
+    level's cells (amr.* inputs); n_error_buf, n_proper, grid_eff as in AMReX.
     """
 
-    def __init__(self, coarse_boxes, geom, parm, cfl, kind=None, scheme=2, flags=0, refine_dengrad=0.05,
-                 regrid_int=2, blocking_factor=2, grid_eff=0.7, max_grid_size=64, fine_boxes=None, seed=0,
-                 stop_time=-1.0, do_reflux=True, stream=None):
+    def __init__(self, coarse_boxes, geom, parm, cfl, kind=None, scheme=2, flags=0, max_level=1,
+                 refine_dengrad=0.05, regrid_int=2, blocking_factor=2, grid_eff=0.7, max_grid_size=64,
+                 n_error_buf=0, n_proper=1, fine_boxes=None, seed=0, stop_time=-1.0, do_reflux=True, stream=None):
         if scheme not in (0, 1, 2):
             raise ValueError("scheme must be 0 (Euler), 1 (RK2) or 2 (SSP-RK3)")
         if blocking_factor % RATIO or max_grid_size % blocking_factor:
             raise ValueError("blocking_factor must be a multiple of 2 and divide max_grid_size")
-        self.parm, self.cfl, self.scheme, self.flags = parm, cfl, scheme, flags
-        self.thr, self.regrid_int = refine_dengrad, regrid_int
-        self.block, self.grid_eff, self.max_c = blocking_factor // RATIO, grid_eff, max_grid_size // RATIO
         if stream is None:
             import torch
             stream = torch.cuda.current_stream()
+        self.parm, self.cfl, self.scheme, self.flags = parm, cfl, scheme, flags
+        self.max_level = max_level
+        self.thr, self.regrid_int = refine_dengrad, regrid_int
+        self.block, self.grid_eff, self.max_c = blocking_factor // RATIO, grid_eff, max_grid_size // RATIO
+        self.n_error_buf, self.n_proper = n_error_buf, n_proper
         self.stop_time, self.do_reflux, self.stream = stop_time, do_reflux, stream
         self.kind, self.seed = kind, seed
-        self.coarse = Level(coarse_boxes, geom)
-        self.gf = refine_geom(geom)
-        self.S = [self.coarse.new_mf(), None]          # new state per level
-        self.Oc = self.coarse.new_mf()                 # coarse old state
-        self.scratch_c = [self.coarse.new_mf(), self.coarse.new_mf(), self.coarse.new_mf()]
-        self.fine = None
-        self.cf = None
-        self.scratch_f = None
+        if fine_boxes is not None and fine_boxes and not isinstance(fine_boxes[0][0][0], (tuple, list)):
+            fine_boxes = [fine_boxes]          # one fixed fine level, boxes in level-0 index space
+        self.fixed = fine_boxes
+        self.geoms = [geom]
+        for _ in range(max_level):
+            self.geoms.append(refine_geom(self.geoms[-1]))
+        n = max_level + 1
+        self.levels = [None] * n
+        self.cfs = [None] * n
+        self.bufs = [None] * n
+        self.st = (LevelState * n)()
+        self.finest = 0
         self.time = 0.0
-        self.dt_level = [0.0, 0.0]
+        self.dt_level = [0.0] * n
+        self.level_steps = [0] * n
+        self.count = [0] * n
         self.steps = 0
-        self.count = 0
-        self.fixed_fine = fine_boxes
+        self._set_level(0, Level(coarse_boxes, geom), None, 0.0)
         if kind is not None:
-            self.coarse.init_state(kind, self.S[0], parm, seed)
+            self.levels[0].init_state(kind, self.S[0], parm, seed)
             self.start()
 
-    # -- grids ------------------------------------------------------------
-    def _fine_boxes_from_tags(self):
-        c = self.coarse
-        c.fill_boundary(self.S[0], self.time, self.stream)
-        tags = tag_dengrad(c, self.S[0], self.thr, self.stream).cpu().numpy()
-        dense = dense_tags(c, tags)
-        g = c.geom
-        boxes = cluster_tags(dense, (g.dom_lo[0], g.dom_lo[1], g.dom_lo[2]), self.block, self.grid_eff, self.max_c)
-        return refine_boxes(boxes)
-
-    def _set_fine(self, level, S):
-        self.fine = level
-        self.S[1] = S
-        if level is None:
-            self.cf, self.scratch_f = None, None
-            return
-        self.scratch_f = [level.new_mf(), level.new_mf()]
-        self.cf = FluxRegister(self.coarse, level)
-        if (self.flags & PLANAR) and self.cf.nfaces and (self.cf.faces()[:, 4] == 2).any():
+    # -- buffers ------------------------------------------------------------
+    def _link(self, l):
+        cf = FluxRegister(self.levels[l - 1], self.levels[l])
+        if (self.flags & PLANAR) and cf.nfaces and (cf.faces()[:, 4] == 2).any():
             raise ValueError("planar stepping needs fine grids that span the domain in z")
+        self.cfs[l] = cf
+
+    def _set_level(self, l, level, S=None, t=0.0):
+        """Install level l at time t (None removes it and everything finer)."""
+        if level is None:
+            for m in range(l, self.max_level + 1):
+                self.levels[m], self.bufs[m], self.cfs[m] = None, None, None
+                self.st[m] = LevelState()
+            self.finest = l - 1
+            return
+        self.levels[l] = level
+        b = {k: level.new_mf() for k in ("a", "b", "s1", "s2", "tmp")}
+        if S is not None:
+            b["a"] = S
+        self.bufs[l] = b
+        q = self.st[l]
+        q.cur, q.old = b["a"].data_ptr(), b["b"].data_ptr()
+        q.s1, q.s2, q.tmp = b["s1"].data_ptr(), b["s2"].data_ptr(), b["tmp"].data_ptr()
+        q.t_old = q.t_new = t
+        if l > 0:
+            self._link(l)
+        # the next finer level now hangs off a new level; it is relinked, or
+        # rebuilt by the caller when it no longer nests
+        if l + 1 <= self.max_level and self.levels[l + 1] is not None:
+            self.cfs[l + 1] = None
+            try:
+                self._link(l + 1)
+            except RuntimeError:
+                pass
+
+    def _tensor(self, l, ptr):
+        for t in self.bufs[l].values():
+            if t.data_ptr() == ptr:
+                return t
+        raise RuntimeError("unknown state buffer")
+
+    @property
+    def S(self):
+        """Current state of every existing level."""
+        return [self._tensor(l, self.st[l].cur) for l in range(self.finest + 1)]
+
+    @property
+    def coarse(self):
+        return self.levels[0]
+
+    @property
+    def fine(self):
+        return self.levels[1] if self.finest >= 1 else None
+
+    @property
+    def gf(self):
+        return self.geoms[1]
+
+    def _arrays(self):
+        n = self.finest + 1
+        lv = (ctypes.c_void_p * n)(*[self.levels[l].handle for l in range(n)])
+        cf = (ctypes.c_void_p * n)(*[self.cfs[l].handle if l > 0 else None for l in range(n)])
+        return n, lv, cf
+
+    def fill(self, l, X=None, t=None):
+        """Ghost cells of level l's state (default: its current one) at time t."""
+        X = self.S[l] if X is None else X
+        n, lv, cf = self._arrays()
+        check(lib().ocfd_amr_fill_level(n, lv, cf, self.st, l, _ptr(X), self.time if t is None else t,
+                                        _stream(self.stream)), "ocfd_amr_fill_level")
+
+    # -- grids ----------------------------------------------------------------
+    def _tags(self, l, t):
+        lev = self.levels[l]
+        self.fill(l, None, t)
+        tg = dense_tags(lev, tag_dengrad(lev, self.S[l], self.thr, self.stream).cpu().numpy()).astype(bool)
+        g = self.geoms[l]
+        return _dilate(tg, self.n_error_buf, [g.periodic[d] for d in range(3)])
+
+    def grid_places(self, lbase, t):
+        """New grids for levels lbase+1.. (Amr::grid_places): top-down from the
+        finest level that may refine; each level's tags are widened to cover
+        the next finer new grids plus n_proper cells, and kept inside the
+        level's proper nesting region."""
+        new = {}
+        new_finest = lbase
+        max_crse = min(self.finest, self.max_level - 1)
+        for levc in range(max_crse, lbase - 1, -1):
+            levf = levc + 1
+            g = self.geoms[levc]
+            tags = self._tags(levc, t)
+            if levf + 1 in new and new[levf + 1]:
+                proj = coverage(coarsen_boxes(new[levf + 1]), self.geoms[levf])
+                proj = _dilate(proj, self.n_proper, [self.geoms[levf].periodic[d] for d in range(3)])
+                nz, ny, nx = proj.shape
+                tags |= proj.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).any(axis=(1, 3, 5))
+            pn = proper_nesting(self.levels[levc].boxes, g, self.n_proper) if levc > 0 else None
+            if pn is not None:
+                tags &= pn
+            if not tags.any():
+                new[levf] = []
+                continue
+            boxes = cluster_tags(tags, (g.dom_lo[0], g.dom_lo[1], g.dom_lo[2]), self.block, self.grid_eff, self.max_c,
+                                 pn)
+            new[levf] = refine_boxes(boxes)
+            new_finest = max(new_finest, levf)
+        return {l: b for l, b in new.items() if l <= new_finest}, new_finest
+
+    def _fixed_grids(self):
+        out, boxes = {}, None
+        for l, fb in enumerate(self.fixed or [], start=1):
+            if l > self.max_level:
+                break
+            out[l] = refine_boxes(fb)
+        return out
+
+    def _build(self, l, boxes, t, init_from_case):
+        """(Re)build level l on `boxes` at time t: case data, or the old level's
+        cells where it had them and the interpolated coarser level elsewhere."""
+        old_level = self.levels[l] if l <= self.finest else None
+        old_S = self.S[l] if old_level is not None else None
+        lev = Level(boxes, self.geoms[l])
+        Sn = lev.new_mf()
+        cf = FluxRegister(self.levels[l - 1], lev)
+        if init_from_case and self.kind is not None:
+            lev.init_state(self.kind, Sn, self.parm, self.seed)
+        else:
+            if old_level is not None:
+                copy_level(lev, Sn, old_level, old_S, self.stream)
+            self.fill(l - 1, None, t)
+            cf.interp_fine(Sn, self.S[l - 1], old_level, self.stream)
+        del cf
+        self._set_level(l, lev, Sn, t)
+        self.finest = max(self.finest, l)
 
     def start(self):
-        """Initial grids (bldFineLevels + post_init): fine grids from the
-        coarse tags or the fixed list, the case on the fine level, average-down."""
-        boxes = refine_boxes(self.fixed_fine) if self.fixed_fine is not None else self._fine_boxes_from_tags()
-        if not boxes:
-            self._set_fine(None, None)
-            return
-        f = Level(boxes, self.gf)
-        Sf = f.new_mf()
-        if self.kind is not None:
-            f.init_state(self.kind, Sf, self.parm, self.seed)
+        """Initial grids (Amr::bldFineLevels, then up to 4 top-down passes until
+        the grids settle) and the average-down of EBR::post_init."""
+        t = self.time
+        if self.fixed is not None:
+            for l, b in sorted(self._fixed_grids().items()):
+                g = self.geoms[l - 1]
+                inner = _dilate(coverage(coarsen_boxes(b), g), self.n_proper, [g.periodic[d] for d in range(3)])
+                if (inner & ~coverage(self.levels[l - 1].boxes, g)).any():
+                    raise ValueError(f"fixed grids of level {l} are not properly nested in level {l - 1} "
+                                     f"(n_proper {self.n_proper}, periodic images included)")
+                self._build(l, b, t, True)
         else:
-            self.coarse.fill_boundary(self.S[0], self.time, self.stream)
-            interp_from_coarse(f, Sf, self.coarse, self.S[0], None, self.stream)
-        self._set_fine(f, Sf)
-        self.cf.avg_down(Sf, self.S[0], NCONS, self.stream)
+            while self.finest < self.max_level:
+                new, nf = self.grid_places(self.finest, t)
+                if nf <= self.finest:
+                    break
+                self._build(nf, new[nf], t, True)
+            if self.max_level > 0:
+                for _ in range(4):
+                    new, nf = self.grid_places(0, t)
+                    if nf < self.finest:
+                        break
+                    same = True
+                    for l in range(1, nf + 1):
+                        if l > self.finest or sorted(new[l]) != sorted(self.levels[l].boxes):
+                            same = False
+                            self._build(l, new[l], t, True)
+                    if same:
+                        break
+        self._check_links()
+        for l in range(self.finest - 1, -1, -1):
+            self.cfs[l + 1].avg_down(self.S[l + 1], self.S[l], NCONS, self.stream)
 
-    def regrid(self):
-        """New fine grids from the current coarse state; the new fine level takes
-        the old fine level's cells where it had them and the interpolated
-        coarse state elsewhere (AmrLevel::init(old))."""
-        boxes = self._fine_boxes_from_tags()      # fills the coarse ghosts at self.time
-        if not boxes:
-            self._set_fine(None, None)
-            return
-        f = Level(boxes, self.gf)
-        Sf = f.new_mf()
-        if self.fine is not None:
-            copy_level(f, Sf, self.fine, self.S[1], self.stream)
-        interp_from_coarse(f, Sf, self.coarse, self.S[0], self.fine, self.stream)
-        self._set_fine(f, Sf)
+    def regrid(self, lbase, t):
+        """Amr::regrid(lbase): new grids for levels lbase+1..; levels that lose
+        their grids are removed."""
+        new, nf = self.grid_places(lbase, t)
+        for l in range(lbase + 1, nf + 1):
+            if new[l]:
+                self._build(l, new[l], t, False)
+        if nf < self.finest:
+            self._set_level(nf + 1, None)
+        self._check_links()
+        for l in range(lbase, self.max_level + 1):
+            self.count[l] = 0
 
-    # -- time step ----------------------------------------------------------
-    def _estdt(self):
-        est = [self.coarse.estdt(self.S[0], self.cfl, self.parm, self.stream)]
-        if self.fine is not None:
-            est.append(self.fine.estdt(self.S[1], self.cfl, self.parm, self.stream))
-        return est
+    def _check_links(self):
+        for l in range(1, self.finest + 1):
+            if self.cfs[l] is None:
+                raise RuntimeError(f"level {l} is not nested in level {l - 1}")
 
+    # -- time stepping ----------------------------------------------------------
     def _compute_dt(self):
-        est = self._estdt()
+        est = [self.levels[l].estdt(self.S[l], self.cfl, self.parm, self.stream) for l in range(self.finest + 1)]
         if self.steps > 0:
-            prev = [self.dt_level[0], self.dt_level[1] if self.dt_level[1] > 0 else self.dt_level[0] / RATIO]
-            est = [min(e, CHANGE_MAX * prev[i]) for i, e in enumerate(est)]
-        dt0 = est[0]
-        if len(est) > 1:
-            dt0 = min(dt0, RATIO * est[1])
+            prev = [self.dt_level[l] if self.dt_level[l] > 0 else self.dt_level[0] / RATIO ** l
+                    for l in range(self.finest + 1)]
+            est = [min(e, CHANGE_MAX * prev[l]) for l, e in enumerate(est)]
+        dt0 = min(RATIO ** l * e for l, e in enumerate(est))
         eps = 0.001 * dt0
         if self.stop_time >= 0.0 and self.time + dt0 > self.stop_time - eps:
             dt0 = self.stop_time - self.time
-        self.dt_level = [dt0, dt0 / RATIO]
+        self.dt_level = [dt0 / RATIO ** l for l in range(self.max_level + 1)]
 
-    def _bufs(self):
-        b = AmrBufs()
-        b.c_old, b.c_new = self.Oc.data_ptr(), self.S[0].data_ptr()
-        b.c_s1, b.c_s2, b.c_t = (x.data_ptr() for x in self.scratch_c)
-        if self.fine is not None:
-            b.f_state = self.S[1].data_ptr()
-            b.f_s1, b.f_s2 = (x.data_ptr() for x in self.scratch_f)
-        return b
+    def _advance(self, l, t, dt):
+        n, lv, cf = self._arrays()
+        check(lib().ocfd_amr_advance_level(n, lv, cf, self.st, l, t, dt, self.scheme, ctypes.byref(self.parm),
+                                           self.flags, int(self.do_reflux), _stream(self.stream)),
+              "ocfd_amr_advance_level")
+
+    def _time_step(self, l, t):
+        if self.fixed is None and self.regrid_int > 0 and l < self.max_level:
+            i = l
+            while i <= min(self.finest, self.max_level - 1):
+                if self.count[i] >= self.regrid_int:
+                    self.regrid(i, t)
+                i += 1
+        self._advance(l, t, self.dt_level[l])
+        self.level_steps[l] += 1
+        self.count[l] += 1
+        if l < self.finest:
+            for it in range(RATIO):
+                self._time_step(l + 1, t + it * self.dt_level[l + 1])
+            if self.do_reflux:
+                self.cfs[l + 1].reflux(self.S[l], self.stream)
+            self.cfs[l + 1].avg_down(self.S[l + 1], self.S[l], NCONS, self.stream)
 
     def step(self):
-        """One coarse time step (Amr::coarseTimeStep with two subcycled fine steps)."""
+        """One coarse time step (Amr::coarseTimeStep)."""
         self._compute_dt()
-        if self.fixed_fine is None and self.regrid_int > 0 and self.count >= self.regrid_int:
-            self.regrid()
-            self.count = 0
-        t, dt0 = self.time, self.dt_level[0]
-        self.Oc, self.S[0] = self.S[0], self.Oc          # old <- current; the new state is written
-        b = self._bufs()
-        check(lib().ocfd_amr_advance(self.coarse.handle, self.cf.handle if self.cf is not None else None,
-                                     ctypes.byref(b), t, dt0, self.scheme, ctypes.byref(self.parm), self.flags,
-                                     int(self.do_reflux), _stream(self.stream)), "ocfd_amr_advance")
-        self.time = t + dt0
+        t = self.time
+        self._time_step(0, t)
+        self.time = t + self.dt_level[0]
         self.steps += 1
-        self.count += 1
-        return dt0
+        return self.dt_level[0]
 
     def run(self, nsteps=-1):
         n = 0
@@ -301,9 +488,15 @@ class TwoLevel:
             n += 1
         return n
 
-    def fill(self):
-        """Fill the ghost cells of both levels at the current time."""
-        self.coarse.fill_boundary(self.S[0], self.time, self.stream)
-        if self.fine is not None:
-            self.fine.fill_boundary(self.S[1], self.time, self.stream)
-            fill_from_coarse(self.fine, self.S[1], self.coarse, self.S[0], self.stream)
+    def cells(self):
+        """Valid cells per level."""
+        out = []
+        for l in range(self.finest + 1):
+            out.append(sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1)
+                           for lo, hi in self.levels[l].boxes))
+        return out
+
+
+def TwoLevel(coarse_boxes, geom, parm, cfl, kind=None, **kw):
+    """A hierarchy of one base level and at most one finer level."""
+    return Hierarchy(coarse_boxes, geom, parm, cfl, kind, max_level=1, **kw)

@@ -190,15 +190,8 @@ def merge_boxes(boxes):
     return bx
 
 
-def cluster(dense, lo, block=1, grid_eff=0.7, max_size=32):
-    """Boxes covering the tagged cells of dense (nz, ny, nx) (index origin lo)."""
-    nz, ny, nx = dense.shape
-    b = block
-    blk = dense.reshape(nz // b, b, ny // b, b, nx // b, b).max(axis=(1, 3, 5))
-    kk, jj, ii = np.nonzero(blk)
-    if len(ii) == 0:
-        return []
-    pts = np.stack([ii + lo[0] // b, jj + lo[1] // b, kk + lo[2] // b], axis=1)
+def _br(pts, grid_eff):
+    """Berger-Rigoutsos clusters of integer points (n, 3) at efficiency grid_eff."""
     clusters = [pts]
     q = 0
     while q < len(clusters):
@@ -226,6 +219,36 @@ def cluster(dense, lo, block=1, grid_eff=0.7, max_size=32):
         high = p[p[:, dirn] >= cuts[dirn]]
         clusters[q] = low
         clusters.append(high)
+    return clusters
+
+
+def cluster(dense, lo, block=1, grid_eff=0.7, max_size=32, mask=None):
+    """Boxes covering the tagged cells of dense (nz, ny, nx) (index origin lo),
+    within the region `mask` allows when given."""
+    nz, ny, nx = dense.shape
+    b = block
+    blk = dense.reshape(nz // b, b, ny // b, b, nx // b, b).max(axis=(1, 3, 5))
+    kk, jj, ii = np.nonzero(blk)
+    if len(ii) == 0:
+        return []
+    pts = np.stack([ii + lo[0] // b, jj + lo[1] // b, kk + lo[2] // b], axis=1)
+    clusters = _br(pts, grid_eff)
+    if mask is not None:
+        bm = mask.astype(bool).reshape(nz // b, b, ny // b, b, nx // b, b).all(axis=(1, 3, 5))
+        kept = []
+        for p in clusters:
+            blo, bhi = _bbox(p)
+            sub = bm[blo[2] - lo[2] // b:bhi[2] - lo[2] // b + 1, blo[1] - lo[1] // b:bhi[1] - lo[1] // b + 1,
+                     blo[0] - lo[0] // b:bhi[0] - lo[0] // b + 1]
+            if sub.all():
+                kept.append(p)
+                continue
+            zk, zj, zi = np.nonzero(sub)
+            if len(zi) == 0:
+                continue
+            inside = np.stack([zi + blo[0], zj + blo[1], zk + blo[2]], axis=1)
+            kept += _br(inside, 1.0)
+        clusters = kept
     boxes = []
     for p in clusters:
         blo, bhi = _bbox(p)
@@ -535,86 +558,243 @@ def copy_fabs(fabs):
     return out
 
 
-class TwoLevel:
-    """Oracle of the device TwoLevel driver (same arguments, oracle geometry)."""
+def dilate(mask, n, periodic):
+    """mask grown by n cells (Chebyshev neighbourhood); periodic axes wrap,
+    other sides contribute nothing."""
+    out = mask.copy()
+    if n <= 0:
+        return out
+    nz, ny, nx = mask.shape
+    for dz in range(-n, n + 1):
+        for dy in range(-n, n + 1):
+            for dx in range(-n, n + 1):
+                sh = mask
+                for ax, dd, d in ((0, dz, 2), (1, dy, 1), (2, dx, 0)):
+                    if dd == 0:
+                        continue
+                    if periodic[d]:
+                        sh = np.roll(sh, dd, axis=ax)
+                    else:
+                        z = np.zeros_like(sh)
+                        src = [slice(None)] * 3
+                        dst = [slice(None)] * 3
+                        if dd > 0:
+                            src[ax], dst[ax] = slice(0, sh.shape[ax] - dd), slice(dd, None)
+                        else:
+                            src[ax], dst[ax] = slice(-dd, None), slice(0, sh.shape[ax] + dd)
+                        z[tuple(dst)] = sh[tuple(src)]
+                        sh = z
+                out |= sh
+    return out
 
-    def __init__(self, cboxes, gc, p, cfl, kind, scheme=2, visc=0, refine_dengrad=0.05, regrid_int=2,
-                 blocking_factor=2, grid_eff=0.7, max_grid_size=64, fine_boxes=None, seed=0, stop_time=-1.0,
-                 do_reflux=True):
-        self.cboxes, self.gc, self.gf = list(cboxes), gc, refine_ogeom(gc)
+
+def coarsen_boxes(boxes):
+    return [(tuple(v // 2 for v in lo), tuple(v // 2 for v in hi)) for lo, hi in boxes]
+
+
+class Hierarchy:
+    """Oracle of the device Hierarchy driver (same arguments, oracle geometry):
+    Amr::coarseTimeStep / timeStep with subcycling, regrids at every level,
+    grid_places with proper nesting and projection, FillPatch from coarser
+    levels interpolated to the stage time, flux registers per interface,
+    reflux and average-down."""
+
+    def __init__(self, cboxes, gc, p, cfl, kind, scheme=2, visc=0, max_level=1, refine_dengrad=0.05, regrid_int=2,
+                 blocking_factor=2, grid_eff=0.7, max_grid_size=64, n_error_buf=0, n_proper=1, fine_boxes=None,
+                 seed=0, stop_time=-1.0, do_reflux=True):
         self.p, self.cfl, self.scheme, self.visc = p, cfl, scheme, visc
+        self.max_level = max_level
         self.thr, self.regrid_int = refine_dengrad, regrid_int
         self.block, self.eff, self.max_c = blocking_factor // 2, grid_eff, max_grid_size // 2
+        self.n_error_buf, self.n_proper = n_error_buf, n_proper
+        if fine_boxes is not None and fine_boxes and not isinstance(fine_boxes[0][0][0], (tuple, list)):
+            fine_boxes = [fine_boxes]
         self.fixed, self.kind, self.seed = fine_boxes, kind, seed
         self.stop_time, self.do_reflux = stop_time, do_reflux
-        self.S = [orc.new_level_fabs(self.cboxes), None]
-        self.fboxes = []
-        orc.init_case(kind, self.cboxes, self.S[0], gc, p, seed)
-        self.time, self.steps, self.count = 0.0, 0, 0
-        self.dt_level = [0.0, 0.0]
-        boxes = refine_boxes(fine_boxes) if fine_boxes is not None else self._tags_to_boxes()
-        self.fboxes = boxes
-        if boxes:
-            self.S[1] = orc.new_level_fabs(boxes)
-            orc.init_case(kind, boxes, self.S[1], self.gf, p, seed)
-            avg_down(boxes, self.S[1], self.cboxes, self.S[0], gc)
+        self.geoms = [gc]
+        for _ in range(max_level):
+            self.geoms.append(refine_ogeom(self.geoms[-1]))
+        n = max_level + 1
+        self.boxes = [list(cboxes)] + [None] * max_level
+        self.S = [orc.new_level_fabs(self.boxes[0])] + [None] * max_level
+        self.O = [None] * n
+        self.t_old = [0.0] * n
+        self.t_new = [0.0] * n
+        self.RC = [None] * n
+        self.RF = [None] * n
+        self.finest = 0
+        self.time, self.steps = 0.0, 0
+        self.dt_level = [0.0] * n
+        self.count = [0] * n
+        orc.init_case(kind, self.boxes[0], self.S[0], gc, p, seed)
+        self._start()
 
-    def _tags_to_boxes(self):
-        orc.fill_ghost(self.cboxes, self.S[0], self.gc, self.time)
-        t = tag_dengrad(self.cboxes, self.S[0], self.gc, self.thr)
-        lo, _ = dom(self.gc)
-        return refine_boxes(cluster(t, lo, self.block, self.eff, self.max_c))
+    # compatibility names of the two-level form
+    @property
+    def cboxes(self):
+        return self.boxes[0]
 
-    def regrid(self):
-        boxes = self._tags_to_boxes()
-        if not boxes:
-            self.fboxes, self.S[1] = [], None
-            return
+    @property
+    def fboxes(self):
+        return self.boxes[1] if self.finest >= 1 else []
+
+    @property
+    def gc(self):
+        return self.geoms[0]
+
+    @property
+    def gf(self):
+        return self.geoms[1]
+
+    # -- fills ------------------------------------------------------------------
+    def fill(self, l, X, ts):
+        orc.fill_ghost(self.boxes[l], X, self.geoms[l], ts)
+        if l > 0:
+            C = self.at(l - 1, ts)
+            Fint = interp_dense(dense_grown(self.boxes[l - 1], C, self.geoms[l - 1]), self.geoms[l - 1])
+            fill_from_coarse(self.boxes[l], X, self.geoms[l], Fint)
+
+    def at(self, m, ts):
+        t0, t1 = self.t_old[m], self.t_new[m]
+        span = t1 - t0
+        if abs(ts - t1) <= 1e-10 * span:
+            self.fill(m, self.S[m], t1)
+            return self.S[m]
+        if abs(ts - t0) <= 1e-10 * span:
+            return self.O[m]
+        T = copy_fabs(self.O[m])
+        for tf, of, nf in zip(T, self.O[m], self.S[m]):
+            tf.a[:] = ((t1 - ts) / span) * of.a + ((ts - t0) / span) * nf.a
+        self.fill(m, T, ts)
+        return T
+
+    # -- grids --------------------------------------------------------------------
+    def _tags(self, l, t):
+        self.fill(l, self.S[l], t)
+        g = self.geoms[l]
+        tg = tag_dengrad(self.boxes[l], self.S[l], g, self.thr).astype(bool)
+        return dilate(tg, self.n_error_buf, [g.periodic[d] for d in range(3)])
+
+    def grid_places(self, lbase, t):
+        new, new_finest = {}, lbase
+        for levc in range(min(self.finest, self.max_level - 1), lbase - 1, -1):
+            levf = levc + 1
+            g = self.geoms[levc]
+            per = [g.periodic[d] for d in range(3)]
+            tags = self._tags(levc, t)
+            if new.get(levf + 1):
+                gf = self.geoms[levf]
+                proj = dilate(owner_mask(coarsen_boxes(new[levf + 1]), gf), self.n_proper,
+                              [gf.periodic[d] for d in range(3)])
+                nz, ny, nx = proj.shape
+                tags |= proj.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).any(axis=(1, 3, 5))
+            pn = ~dilate(~owner_mask(self.boxes[levc], g), self.n_proper, per) if levc > 0 else None
+            if pn is not None:
+                tags &= pn
+            if not tags.any():
+                new[levf] = []
+                continue
+            lo, _ = dom(g)
+            new[levf] = refine_boxes(cluster(tags.astype(np.uint8), lo, self.block, self.eff, self.max_c, pn))
+            new_finest = max(new_finest, levf)
+        return {l: b for l, b in new.items() if l <= new_finest}, new_finest
+
+    def _build(self, l, boxes, t, from_case):
         new = orc.new_level_fabs(boxes)
-        skip = None
-        if self.fboxes:
-            skip = copy_level(boxes, new, self.fboxes, self.S[1], self.gf)
-        Fint = interp_dense(dense_grown(self.cboxes, self.S[0], self.gc), self.gc)
-        interp_valid(boxes, new, self.gf, Fint, skip)
-        self.fboxes, self.S[1] = boxes, new
+        if from_case:
+            orc.init_case(self.kind, boxes, new, self.geoms[l], self.p, self.seed)
+        else:
+            skip = None
+            if l <= self.finest and self.boxes[l]:
+                skip = copy_level(boxes, new, self.boxes[l], self.S[l], self.geoms[l])
+            self.fill(l - 1, self.S[l - 1], t)
+            Fint = interp_dense(dense_grown(self.boxes[l - 1], self.S[l - 1], self.geoms[l - 1]), self.geoms[l - 1])
+            interp_valid(boxes, new, self.geoms[l], Fint, skip)
+        self.boxes[l], self.S[l], self.O[l] = list(boxes), new, None
+        self.t_old[l] = self.t_new[l] = t
+        self.finest = max(self.finest, l)
 
+    def _drop(self, l):
+        for m in range(l, self.max_level + 1):
+            self.boxes[m], self.S[m], self.O[m] = None, None, None
+        self.finest = l - 1
+
+    def _start(self):
+        t = self.time
+        if self.fixed is not None:
+            for l, fb in enumerate(self.fixed, start=1):
+                if l > self.max_level:
+                    break
+                self._build(l, refine_boxes(fb), t, True)
+        else:
+            while self.finest < self.max_level:
+                new, nf = self.grid_places(self.finest, t)
+                if nf <= self.finest:
+                    break
+                self._build(nf, new[nf], t, True)
+            if self.max_level > 0:
+                for _ in range(4):
+                    new, nf = self.grid_places(0, t)
+                    if nf < self.finest:
+                        break
+                    same = True
+                    for l in range(1, nf + 1):
+                        if l > self.finest or sorted(new[l]) != sorted(self.boxes[l]):
+                            same = False
+                            self._build(l, new[l], t, True)
+                    if same:
+                        break
+        for l in range(self.finest - 1, -1, -1):
+            avg_down(self.boxes[l + 1], self.S[l + 1], self.boxes[l], self.S[l], self.geoms[l])
+
+    def regrid(self, lbase, t):
+        new, nf = self.grid_places(lbase, t)
+        for l in range(lbase + 1, nf + 1):
+            if new[l]:
+                self._build(l, new[l], t, False)
+        if nf < self.finest:
+            self._drop(nf + 1)
+        for l in range(lbase, self.max_level + 1):
+            self.count[l] = 0
+
+    # -- time stepping --------------------------------------------------------------
     def _compute_dt(self):
-        dxc = [self.gc.dx[d] for d in range(3)]
-        est = [orc.estdt(self.cboxes, self.S[0], dxc, self.cfl, self.p)]
-        if self.fboxes:
-            est.append(orc.estdt(self.fboxes, self.S[1], [v / 2 for v in dxc], self.cfl, self.p))
+        est = []
+        for l in range(self.finest + 1):
+            g = self.geoms[l]
+            est.append(orc.estdt(self.boxes[l], self.S[l], [g.dx[d] for d in range(3)], self.cfl, self.p))
         if self.steps > 0:
-            prev = [self.dt_level[0], self.dt_level[1] if self.dt_level[1] > 0 else self.dt_level[0] / 2]
-            est = [min(e, CHANGE_MAX * prev[i]) for i, e in enumerate(est)]
-        dt0 = est[0]
-        if len(est) > 1:
-            dt0 = min(dt0, 2 * est[1])
+            prev = [self.dt_level[l] if self.dt_level[l] > 0 else self.dt_level[0] / 2 ** l
+                    for l in range(self.finest + 1)]
+            est = [min(e, CHANGE_MAX * prev[l]) for l, e in enumerate(est)]
+        dt0 = min(2 ** l * e for l, e in enumerate(est))
         eps = 0.001 * dt0
         if self.stop_time >= 0.0 and self.time + dt0 > self.stop_time - eps:
             dt0 = self.stop_time - self.time
-        self.dt_level = [dt0, dt0 / 2]
+        self.dt_level = [dt0 / 2 ** l for l in range(self.max_level + 1)]
 
-    def _coarse_at(self, ts, t0, t1, O, N):
-        span = t1 - t0
-        if abs(ts - t0) <= 1e-10 * span:
-            return O
-        if abs(ts - t1) <= 1e-10 * span:
-            orc.fill_ghost(self.cboxes, N, self.gc, t1)
-            return N
-        T = copy_fabs(O)
-        for tf, of, nf in zip(T, O, N):
-            tf.a[:] = ((t1 - ts) / span) * of.a + ((ts - t0) / span) * nf.a
-        orc.fill_ghost(self.cboxes, T, self.gc, ts)
-        return T
-
-    def _fill_fine(self, X, ts, t0, t1, O, N):
-        orc.fill_ghost(self.fboxes, X, self.gf, ts)
-        C = self._coarse_at(ts, t0, t1, O, N)
-        Fint = interp_dense(dense_grown(self.cboxes, C, self.gc), self.gc)
-        fill_from_coarse(self.fboxes, X, self.gf, Fint)
+    def _advance(self, l, t, dt):
+        g = self.geoms[l]
+        self.O[l] = self.S[l]
+        N = copy_fabs(self.O[l])
+        self.t_old[l], self.t_new[l] = t, t + dt
+        if l + 1 <= self.finest:
+            self.RC[l + 1] = None
+            self.RF[l + 1] = None
+        regs = self._run_stages(self.boxes[l], self.O[l], g, t, dt, lambda X, ts: self.fill(l, X, ts), N)
+        self.S[l] = N
+        if not self.do_reflux:
+            return
+        for w, fc in regs:
+            if l + 1 <= self.finest:
+                D = dense_faces(self.boxes[l], fc, g)
+                self.RC[l + 1] = [w * D[d] if self.RC[l + 1] is None else self.RC[l + 1][d] + w * D[d]
+                                  for d in range(3)]
+            if l > 0:
+                D = dense_faces(self.boxes[l], fc, g)
+                self.RF[l] = [w * D[d] if self.RF[l] is None else self.RF[l][d] + w * D[d] for d in range(3)]
 
     def _run_stages(self, boxes, A, g, t, dt, fill, out):
-        """The stage sequence of one level from A; returns [(weight, faces)]."""
         p, v = self.p, self.visc
         B, C = orc.new_level_fabs(boxes), orc.new_level_fabs(boxes)
         st = STAGES[self.scheme]
@@ -633,39 +813,32 @@ class TwoLevel:
         regs.append((st[2][1] * dt, stage(boxes, C, A, out, 2.0 / 3.0, 1.0 / 3.0, (2.0 / 3.0) * dt, g, p, v)))
         return regs
 
+    def _time_step(self, l, t):
+        if self.fixed is None and self.regrid_int > 0 and l < self.max_level:
+            i = l
+            while i <= min(self.finest, self.max_level - 1):
+                if self.count[i] >= self.regrid_int:
+                    self.regrid(i, t)
+                i += 1
+        self._advance(l, t, self.dt_level[l])
+        self.count[l] += 1
+        if l < self.finest:
+            for it in range(2):
+                self._time_step(l + 1, t + it * self.dt_level[l + 1])
+            if self.do_reflux:
+                cells = interface(self.boxes[l], self.boxes[l + 1], self.geoms[l])
+                reflux(cells, self.boxes[l], self.S[l], self.geoms[l], self.geoms[l + 1], self.RC[l + 1],
+                       self.RF[l + 1])
+            avg_down(self.boxes[l + 1], self.S[l + 1], self.boxes[l], self.S[l], self.geoms[l])
+
     def step(self):
         self._compute_dt()
-        if self.fixed is None and self.regrid_int > 0 and self.count >= self.regrid_int:
-            self.regrid()
-            self.count = 0
-        t, dt0 = self.time, self.dt_level[0]
-        gc, gf = self.gc, self.gf
-        O = self.S[0]
-        N = copy_fabs(O)
-        regs_c = self._run_stages(self.cboxes, O, gc, t, dt0,
-                                  lambda X, ts: orc.fill_ghost(self.cboxes, X, gc, ts), N)
-        self.S[0] = N
-        if self.fboxes:
-            dt1 = dt0 / 2
-            RC = [np.zeros_like(a) for a in dense_faces(self.cboxes, regs_c[0][1], gc)]
-            for w, fc in regs_c:
-                D = dense_faces(self.cboxes, fc, gc)
-                RC = [RC[d] + w * D[d] for d in range(3)]
-            RF = None
-            for it in range(2):
-                A = self.S[1]
-                out = copy_fabs(A)
-                regs_f = self._run_stages(self.fboxes, A, gf, t + it * dt1, dt1,
-                                          lambda X, ts: self._fill_fine(X, ts, t, t + dt0, O, N), out)
-                self.S[1] = out
-                for w, fc in regs_f:
-                    D = dense_faces(self.fboxes, fc, gf)
-                    RF = [w * D[d] for d in range(3)] if RF is None else [RF[d] + w * D[d] for d in range(3)]
-            if self.do_reflux:
-                cells = interface(self.cboxes, self.fboxes, gc)
-                reflux(cells, self.cboxes, self.S[0], gc, gf, RC, RF)
-            avg_down(self.fboxes, self.S[1], self.cboxes, self.S[0], gc)
-        self.time = t + dt0
+        t = self.time
+        self._time_step(0, t)
+        self.time = t + self.dt_level[0]
         self.steps += 1
-        self.count += 1
-        return dt0
+        return self.dt_level[0]
+
+
+def TwoLevel(cboxes, gc, p, cfl, kind, **kw):
+    return Hierarchy(cboxes, gc, p, cfl, kind, max_level=1, **kw)

@@ -214,3 +214,105 @@ def test_fully_refined_equals_uniform_fine(orc):
         time += dt0
     for (lo, hi), f, a in zip(fb, T.S[1], A):
         assert np.array_equal(f.valid(lo, hi), a.valid(lo, hi))
+
+
+def _totals_h(H):
+    """Totals of a hierarchy: every level's cells not covered by the next finer
+    one, in units of the finest cell volume."""
+    tot = np.zeros(5)
+    for l in range(H.finest + 1):
+        g = H.geoms[l]
+        n = [g.dom_hi[d] + 1 for d in range(3)]
+        cov = np.zeros((n[2], n[1], n[0]), dtype=bool)
+        if l < H.finest:
+            for lo, hi in H.boxes[l + 1]:
+                cov[lo[2] // 2:hi[2] // 2 + 1, lo[1] // 2:hi[1] // 2 + 1, lo[0] // 2:hi[0] // 2 + 1] = True
+        w = 8.0 ** (H.finest - l)
+        for (lo, hi), f in zip(H.boxes[l], H.S[l]):
+            v = f.valid(lo, hi)
+            m = ~cov[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+            tot += w * v[:, m].sum(axis=1)
+    return tot
+
+
+def test_three_levels_conserve_on_a_periodic_domain(orc):
+    """Viscous shear layer, 8^3 base + properly nested fixed levels 1 and 2
+    (level 2 across a periodic face): three levels of subcycling with reflux
+    at both interfaces conserve every total to rounding."""
+    n = 8
+    g = _og(orc, (n, n, n))
+    fine = [[((0, 1, 1), (7, 6, 5))], [((0, 4, 4), (3, 9, 7)), ((14, 4, 4), (15, 9, 7)), ((6, 3, 3), (9, 7, 5))]]
+    H = ao.Hierarchy([((0, 0, 0), (n - 1, n - 1, n - 1))], g, orc.parm(), 0.4, 2, visc=1, max_level=2,
+                     regrid_int=0, fine_boxes=fine, seed=5)
+    assert H.finest == 2
+    t0 = _totals_h(H)
+    for _ in range(2):
+        H.step()
+    t1 = _totals_h(H)
+    scale = np.array([t0[0]] * 4 + [t0[4]])
+    assert (np.abs(t1 - t0) / scale < 2e-14).all(), (t1 - t0) / scale
+
+
+def test_three_levels_fully_refined_equal_uniform_finest(orc):
+    n = 4
+    g = _og(orc, (n, n, n))
+    fine = [[((0, 0, 0), (n - 1, n - 1, n - 1))], [((0, 0, 0), (2 * n - 1, 2 * n - 1, 2 * n - 1))]]
+    H = ao.Hierarchy([((0, 0, 0), (n - 1, n - 1, n - 1))], g, orc.parm(), 0.4, 2, visc=1, max_level=2,
+                     regrid_int=0, fine_boxes=fine, seed=3)
+    g2 = ao.refine_ogeom(ao.refine_ogeom(g))
+    fb = [((0, 0, 0), (4 * n - 1, 4 * n - 1, 4 * n - 1))]
+    A, B, C = (orc.new_level_fabs(fb) for _ in range(3))
+    orc.init_case(2, fb, A, g2, orc.parm(), 3)
+    time = 0.0
+    for _ in range(2):
+        dt0 = H.step()
+        for it in range(4):
+            orc.flow_advance(fb, A, B, C, g2, orc.parm(), time + it * dt0 / 4, dt0 / 4, 2, 1)
+        time += dt0
+    assert np.array_equal(H.S[2][0].valid(*fb[0]), A[0].valid(*fb[0]))
+
+
+def test_three_level_dmr_grids_nest(orc):
+    """Double Mach reflection with max_level 2: after regrids every level's
+    grids, coarsened and grown by n_proper, lie inside the level below (away
+    from the domain boundary), and every tag of each level's last regrid is
+    refined."""
+    from ocfd import cases
+    boxes, g, kind = cases.dmr(64, 16, 2, max_grid=(16, 16, 2))
+    og = orc.OGeom()
+    ctypes.memmove(ctypes.byref(og), ctypes.byref(g), ctypes.sizeof(og))
+    H = ao.Hierarchy(boxes, og, orc.parm(), 0.5, kind, max_level=2, regrid_int=2, max_grid_size=32)
+    for _ in range(5):
+        H.step()
+    assert H.finest == 2
+    for l in range(1, H.finest + 1):
+        gc = H.geoms[l - 1]
+        per = [gc.periodic[d] for d in range(3)]
+        cov_c = ao.owner_mask(H.boxes[l - 1], gc)
+        fine_c = ao.owner_mask(ao.coarsen_boxes(H.boxes[l]), gc)
+        assert not (fine_c & ~cov_c).any()
+        if l >= 2:
+            grown = ao.dilate(fine_c, 1, per)
+            assert not (grown & ~cov_c).any()
+
+
+@pytest.mark.parametrize("block", [1, 2])
+def test_clustering_within_a_mask(block):
+    """Clusters are cut back to the allowed region (the proper nesting domain):
+    every box lies inside the mask, every tag inside the mask is covered, and the
+    library gives the oracle's boxes."""
+    from ocfd.amr import cluster_tags
+    rng = np.random.default_rng(40 + block)
+    for trial in range(6):
+        shape = (2, 2 * int(rng.integers(6, 16)), 2 * int(rng.integers(8, 24)))
+        mask = np.zeros(shape, dtype=np.uint8)
+        for _ in range(3):
+            y0, x0 = 2 * int(rng.integers(0, shape[1] // 2 - 2)), 2 * int(rng.integers(0, shape[2] // 2 - 2))
+            mask[:, y0:y0 + 2 * int(rng.integers(2, 6)), x0:x0 + 2 * int(rng.integers(2, 8))] = 1
+        tags = ((rng.uniform(size=shape) < 0.3) & mask.astype(bool)).astype(np.uint8)
+        want = sorted(ao.cluster(tags, (0, 0, 0), block, 0.7, 64, mask))
+        got = sorted(cluster_tags(tags, (0, 0, 0), block, 0.7, 64, mask))
+        assert got == want, trial
+        cov = _covers(got, tags)
+        assert cov.max() <= 1 and (cov[tags == 1] == 1).all()
+        assert not (cov.astype(bool) & ~mask.astype(bool)).any()
