@@ -303,7 +303,13 @@ def amr_dmr_to_time(stop_time=0.2, nx=1024, ny=256, max_level=2):
         H = amr.Hierarchy(boxes, g, p, 0.5, kind, flags=flags, **kw)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        n = H.run()
+        n = 0
+        while H.time < stop_time:
+            H.step()
+            n += 1
+            if n % 200 == 0:
+                print(f"dmr {name}: step {n} t={H.time:.4f} cells per level {H.cells()}", file=sys.stderr,
+                      flush=True)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         res[name] = {"wall_s": el, "coarse_steps": n, "time": H.time, "cells_per_level": H.cells()}

@@ -213,30 +213,36 @@ extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigne
             return OCFD_EINVAL;
         }
     }
-    // tagged points at blocking-factor resolution
+    // tagged points at blocking-factor resolution (a block is tagged when any
+    // of its cells is); the allowed region likewise, where a block counts only
+    // when all its cells are allowed
     const int nb[3] = {n[0] / block, n[1] / block, n[2] / block};
-    std::vector<unsigned char> bt((size_t)nb[0] * nb[1] * nb[2], 0);
-    for (int k = 0; k < n[2]; ++k)
-        for (int j = 0; j < n[1]; ++j)
-            for (int i = 0; i < n[0]; ++i)
-                if (tags[((size_t)k * n[1] + j) * n[0] + i])
-                    bt[((size_t)(k / block) * nb[1] + j / block) * nb[0] + i / block] = 1;
-    Cluster all;
-    for (int k = 0; k < nb[2]; ++k)
-        for (int j = 0; j < nb[1]; ++j)
-            for (int i = 0; i < nb[0]; ++i)
-                if (bt[((size_t)k * nb[1] + j) * nb[0] + i])
-                    all.pts.push_back(Pt{{lo[0] / block + i, lo[1] / block + j, lo[2] / block + k}});
-    // the region boxes may cover, at blocking-factor resolution (a block only
-    // when all its cells are in the mask)
-    std::vector<unsigned char> bm;
-    if (mask) {
-        bm.assign(bt.size(), 1);
+    const size_t nblk = (size_t)nb[0] * nb[1] * nb[2];
+    std::vector<unsigned char> bt_store, bm_store;
+    const unsigned char* bt = tags;
+    const unsigned char* bmp = mask;
+    if (block > 1) {
+        bt_store.assign(nblk, 0);
+        if (mask) bm_store.assign(nblk, 1);
+        size_t c = 0;
         for (int k = 0; k < n[2]; ++k)
-            for (int j = 0; j < n[1]; ++j)
-                for (int i = 0; i < n[0]; ++i)
-                    if (!mask[((size_t)k * n[1] + j) * n[0] + i])
-                        bm[((size_t)(k / block) * nb[1] + j / block) * nb[0] + i / block] = 0;
+            for (int j = 0; j < n[1]; ++j) {
+                const size_t row = ((size_t)(k / block) * nb[1] + j / block) * nb[0];
+                for (int i = 0; i < n[0]; ++i, ++c) {
+                    if (tags[c]) bt_store[row + i / block] = 1;
+                    if (mask && !mask[c]) bm_store[row + i / block] = 0;
+                }
+            }
+        bt = bt_store.data();
+        bmp = mask ? bm_store.data() : nullptr;
+    }
+    Cluster all;
+    {
+        size_t c = 0;
+        for (int k = 0; k < nb[2]; ++k)
+            for (int j = 0; j < nb[1]; ++j)
+                for (int i = 0; i < nb[0]; ++i, ++c)
+                    if (bt[c]) all.pts.push_back(Pt{{lo[0] / block + i, lo[1] / block + j, lo[2] / block + k}});
     }
     std::vector<IBox> boxes;
     if (!all.pts.empty()) {
@@ -258,7 +264,7 @@ extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigne
                 for (int j = c.box.lo[1]; j <= c.box.hi[1]; ++j)
                     for (int i = c.box.lo[0]; i <= c.box.hi[0]; ++i) {
                         const int ii = i - lo[0] / block, jj = j - lo[1] / block, kk = k - lo[2] / block;
-                        if (bm[((size_t)kk * nb[1] + jj) * nb[0] + ii]) in.pts.push_back(Pt{{i, j, k}});
+                        if (bmp[((size_t)kk * nb[1] + jj) * nb[0] + ii]) in.pts.push_back(Pt{{i, j, k}});
                     }
             if ((long)in.pts.size() == c.box.volume()) { kept.push_back(std::move(c)); continue; }
             if (in.pts.empty()) continue;

@@ -82,16 +82,22 @@ def cluster_tags(dense, lo, block=1, grid_eff=0.7, max_size=32, mask=None):
     nz, ny, nx = dense.shape
     lo_c = (ctypes.c_int * 3)(*lo)
     hi_c = (ctypes.c_int * 3)(lo[0] + nx - 1, lo[1] + ny - 1, lo[2] + nz - 1)
-    cap = max(16, dense.size)
-    out = (ctypes.c_int * (6 * cap))()
     nb = ctypes.c_int()
     up = ctypes.POINTER(ctypes.c_ubyte)
     mk = None
     if mask is not None:
         mask = np.ascontiguousarray(mask, dtype=np.uint8)
         mk = mask.ctypes.data_as(up)
-    check(lib().ocfd_cluster_tags(lo_c, hi_c, dense.ctypes.data_as(up), mk, block, grid_eff,
-                                  max_size, cap, out, ctypes.byref(nb)), "ocfd_cluster_tags")
+    cap = 4096
+    while True:
+        out = (ctypes.c_int * (6 * cap))()
+        rc = lib().ocfd_cluster_tags(lo_c, hi_c, dense.ctypes.data_as(up), mk, block, grid_eff, max_size, cap, out,
+                                     ctypes.byref(nb))
+        if rc != 0 and nb.value > cap:
+            cap = nb.value          # more boxes than room: once more with the exact count
+            continue
+        check(rc, "ocfd_cluster_tags")
+        break
     return [(tuple(out[6 * b:6 * b + 3]), tuple(out[6 * b + 3:6 * b + 6])) for b in range(nb.value)]
 
 
@@ -100,20 +106,26 @@ def copy_level(dst, S_dst, src, S_src, stream=None):
 
 
 def _dilate(mask, n, periodic):
-    """mask grown by n cells (Chebyshev); periodic axes wrap, other sides see False."""
-    if n <= 0:
-        return mask.copy()
-    from scipy.ndimage import maximum_filter
-    pads = []
-    a = mask
+    """mask grown by n cells (Chebyshev; separable into one pass per axis);
+    periodic axes wrap, other sides see False."""
+    a = mask.copy()
     for ax, d in ((0, 2), (1, 1), (2, 0)):
-        mode = "wrap" if periodic[d] else "constant"
-        w = [(0, 0)] * 3
-        w[ax] = (n, n)
-        a = np.pad(a, w, mode=mode)
-        pads.append(n)
-    a = maximum_filter(a.astype(np.uint8), size=2 * n + 1, mode="constant", cval=0).astype(bool)
-    return a[n:-n, n:-n, n:-n]
+        m = a.shape[ax]
+        for _ in range(n):
+            b = a.copy()
+            lo = [slice(None)] * 3
+            hi = [slice(None)] * 3
+            lo[ax], hi[ax] = slice(0, m - 1), slice(1, m)
+            b[tuple(hi)] |= a[tuple(lo)]
+            b[tuple(lo)] |= a[tuple(hi)]
+            if periodic[d] and m > 1:
+                first = [slice(None)] * 3
+                last = [slice(None)] * 3
+                first[ax], last[ax] = slice(0, 1), slice(m - 1, m)
+                b[tuple(first)] |= a[tuple(last)]
+                b[tuple(last)] |= a[tuple(first)]
+            a = b
+    return a
 
 
 def coverage(boxes, geom):
@@ -307,8 +319,8 @@ class Hierarchy:
     def gf(self):
         return self.geoms[1]
 
-    def _arrays(self):
-        n = self.finest + 1
+    def _arrays(self, n):
+        """Handles of levels 0..n-1 and their registers."""
         lv = (ctypes.c_void_p * n)(*[self.levels[l].handle for l in range(n)])
         cf = (ctypes.c_void_p * n)(*[self.cfs[l].handle if l > 0 else None for l in range(n)])
         return n, lv, cf
@@ -316,7 +328,7 @@ class Hierarchy:
     def fill(self, l, X=None, t=None):
         """Ghost cells of level l's state (default: its current one) at time t."""
         X = self.S[l] if X is None else X
-        n, lv, cf = self._arrays()
+        n, lv, cf = self._arrays(l + 1)
         check(lib().ocfd_amr_fill_level(n, lv, cf, self.st, l, _ptr(X), self.time if t is None else t,
                                         _stream(self.stream)), "ocfd_amr_fill_level")
 
@@ -450,7 +462,7 @@ class Hierarchy:
         self.dt_level = [dt0 / RATIO ** l for l in range(self.max_level + 1)]
 
     def _advance(self, l, t, dt):
-        n, lv, cf = self._arrays()
+        n, lv, cf = self._arrays(min(l + 2, self.finest + 1))
         check(lib().ocfd_amr_advance_level(n, lv, cf, self.st, l, t, dt, self.scheme, ctypes.byref(self.parm),
                                            self.flags, int(self.do_reflux), _stream(self.stream)),
               "ocfd_amr_advance_level")
