@@ -370,6 +370,9 @@ def main():
 
     flags32 = ocfd.VISC | ocfd.FLUX_FP32
     lev.init_state(kind, A, p, seed=42)
+    lev.fill_boundary(A, 0.0)
+    ch32, tune32 = lev.autotune(A, p, flags32, reps=3)
+    lev.init_state(kind, A, p, seed=42)
     t32 = timed_graph(lev, A, B, C, p, flags32, args.steps, args.warmup)
     lev.init_state(kind, A, p, seed=42)
     t32h = timed_run(lev, A, B, C, p, flags32, args.steps, args.warmup)
@@ -380,11 +383,15 @@ def main():
     if not args.skip_fp64:
         flags64 = ocfd.VISC
         lev.init_state(kind, A, p, seed=42)
+        lev.fill_boundary(A, 0.0)
+        ch64, tune64 = lev.autotune(A, p, flags64, reps=3)
+        lev.init_state(kind, A, p, seed=42)
         t64 = timed_run(lev, A, B, C, p, flags64, args.steps, args.warmup)
         lev.init_state(kind, A, p, seed=42)
         t64g = timed_graph(lev, A, B, C, p, flags64, args.steps, args.warmup)
         _, _, kern64 = kernel_roofline(lev, A, B, C, p, flags64, ncell)
         res64 = {"value": ncell * args.steps / t64 / 1e6, "ms_per_step": 1e3 * t64 / args.steps,
+                 "autotune": {"march_chunk": ch64, "y_plus_z_sweep_ms": tune64},
                  "form": "host-driven steps (plain FP64 GPU form)",
                  "graph_value": ncell * args.steps / t64g / 1e6, "kernels": kern64}
     del A, B, C, lev
@@ -431,6 +438,7 @@ def main():
         "hbm_frac_step": 320.0 * value * 1e6 / 1e9 / HBM_PEAK_GBS,
         "kernels": kern32,
         "form": "FP32-flux device-driven steps replayed from a hipGraph",
+        "autotune": {"march_chunk": ch32, "y_plus_z_sweep_ms": tune32},
         "host_driven_fp32": {"value": ncell * args.steps / t32h / 1e6, "ms_per_step": 1e3 * t32h / args.steps},
         "fp64_form": res64,
         "fp32_over_fp64": (value / res64["value"]) if res64 else None,

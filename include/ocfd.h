@@ -172,6 +172,21 @@ int ocfd_rk_stage(ocfd_level lev, const double* S_cur, const double* S_old, doub
  * periodic images and the physical BCs at time `time`. */
 int ocfd_fill_boundary(ocfd_level lev, double* S, double time, void* stream);
 
+/* Tile shape of the y and z march sweeps: cells per march chunk (shorter
+ * chunks give more independent waves at the cost of a longer halo prologue
+ * per cell).  A level starts with 32, or 16 or 8 when it has few march tiles.
+ * Results do not depend on it. */
+int ocfd_level_set_march_chunk(ocfd_level lev, int ch);
+int ocfd_level_march_chunk(ocfd_level lev);
+
+/* Autotuner of the march tile shape: times the y and z sweeps of this level
+ * on state S (4 filled ghosts) for chunks of 8, 16, 32 and 64 cells (reps
+ * launches each; ms_out[4] gets the mean ms per candidate when not NULL),
+ * keeps the fastest and returns it in *ch_out.  Overwrites the level's
+ * scratch only.  Synchronises the stream. */
+int ocfd_level_autotune(ocfd_level lev, const double* S, const ocfd_parm* parm, int flags, int reps, int* ch_out,
+                        double* ms_out, void* stream);
+
 /* EBR::estTimeStep (Source/Advance.cpp:61-79): cfl * min over valid cells of
  * dx_d/(|u_d|+c).  Synchronises the stream and writes the result to *dt_out. */
 int ocfd_estdt(ocfd_level lev, const double* S, double cfl, const ocfd_parm* parm,

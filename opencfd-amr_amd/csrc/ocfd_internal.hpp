@@ -33,6 +33,7 @@ struct LevelDev {
     long mf_valid = 0;              // doubles in a 0-ghost 5-comp MultiFab
     long mf_state = 0;              // doubles in a 4-ghost 5-comp MultiFab
     long dev_bytes = 0;             // device memory held by the level
+    int march_ch = 32;              // cells per march chunk of the y and z sweeps
     // embedded boundary (cut boxes only, grown layout, box order)
     bool has_eb = false;
     ocfd_eb_spec eb_spec{};

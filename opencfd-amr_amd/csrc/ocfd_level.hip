@@ -292,6 +292,28 @@ int ocfd_parm_initialize(ocfd_parm* p)
     return OCFD_OK;
 }
 
+// march tiles of the y and z sweeps over the regular boxes: i segment fastest,
+// then chunk, then row group, so a contiguous range of the list (one XCD's
+// share) holds tiles that share chunk halos and transverse neighbour rows;
+// .w = first cell of the chunk (low 20 bits) | chunk length << 20
+static void march_tiles(const LevelDev& L, int ch, std::vector<int4>& ty, std::vector<int4>& tz)
+{
+    ty.clear();
+    tz.clear();
+    for (int b = 0; b < L.nbox; ++b) {
+        const BoxDesc& B = L.boxes[b];
+        if (B.type != 0) continue;
+        const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+        const int iseg = (nx + 63) / 64;
+        for (int kg = 0; kg < (nz + 3) / 4; ++kg)
+            for (int c = 0; c < (ny + ch - 1) / ch; ++c)
+                for (int s = 0; s < iseg; ++s) ty.push_back(make_int4(b, s, kg, (c * ch) | (ch << 20)));
+        for (int jg = 0; jg < (ny + 3) / 4; ++jg)
+            for (int c = 0; c < (nz + ch - 1) / ch; ++c)
+                for (int s = 0; s < iseg; ++s) tz.push_back(make_int4(b, s, jg, (c * ch) | (ch << 20)));
+    }
+}
+
 int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g, ocfd_level* out)
 {
     return ocfd_level_create_eb(nbox, lo, hi, g, nullptr, out);
@@ -435,7 +457,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     // tiles to fill the chip (small or thin levels), where shorter chunks
     // trade a longer halo prologue per cell for more independent waves
     int ch = tile_chunk();
-    auto march_tiles = [&](int c) {
+    auto march_count = [&](int c) {
         long ny_t = 0, nz_t = 0;
         for (const BoxDesc& B : L.boxes) {
             if (B.type != 0) continue;
@@ -446,7 +468,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         }
         return std::max(ny_t, nz_t);
     };
-    while (ch > 8 && march_tiles(ch) < 2048) ch /= 2;
+    while (ch > 8 && march_count(ch) < 2048) ch /= 2;
     std::vector<int4> tx, ty, tz, tg, tc, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
@@ -479,20 +501,15 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         // march tiles: i segment fastest, then chunk, then row group, so a
         // contiguous range of the list (one XCD's share) holds tiles that
         // share chunk halos and transverse neighbour rows
-        const int iseg = (nx + 63) / 64;
-        // .w = first cell of the chunk (low 20 bits) | chunk length << 20
-        for (int kg = 0; kg < (nz + 3) / 4; ++kg)
-            for (int c = 0; c < (ny + ch - 1) / ch; ++c)
-                for (int s = 0; s < iseg; ++s) ty.push_back(make_int4(b, s, kg, (c * ch) | (ch << 20)));
-        for (int jg = 0; jg < (ny + 3) / 4; ++jg)
-            for (int c = 0; c < (nz + ch - 1) / ch; ++c)
-                for (int s = 0; s < iseg; ++s) tz.push_back(make_int4(b, s, jg, (c * ch) | (ch << 20)));
+
         for (int gz : {3, 1}) {
             const long np = (long)(nx + 6) * (ny + 6) * (nz + 2 * gz);
             std::vector<int4>& tl = gz == 3 ? tp : tpp;
             for (long s = 0; s < np; s += 256) tl.push_back(make_int4(b, (int)s, (int)std::min<long>(256, np - s), gz));
         }
     }
+    L.march_ch = ch;
+    march_tiles(L, ch, ty, tz);
     int rc = OCFD_OK;
     if ((rc = upload(&L.d_map, L.owner_map)) || (rc = upload(&L.d_tile_x, tx)) || (rc = upload(&L.d_tile_y, ty)) ||
         (rc = upload(&L.d_tile_z, tz)) || (rc = upload(&L.d_tile_g, tg)) || (rc = upload(&L.d_tile_c, tc)) ||
@@ -701,6 +718,87 @@ int ocfd_profile_stage(ocfd_level lev, const double* S_cur, const double* S_old,
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     return check_launch("ocfd_profile_stage");
+}
+
+int ocfd_level_set_march_chunk(ocfd_level lev, int ch)
+{
+    if (!lev || ch < 1 || ch > 2047) {
+        set_error("ocfd_level_set_march_chunk: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    LevelDev& L = lev->dev;
+    std::vector<int4> ty, tz;
+    march_tiles(L, ch, ty, tz);
+    int4 *dy = nullptr, *dz = nullptr;
+    int rc;
+    if ((rc = upload(&dy, ty)) || (rc = upload(&dz, tz))) {
+        hipFree(dy);
+        hipFree(dz);
+        return rc;
+    }
+    OCFD_HIP(hipDeviceSynchronize());   // no launch may still read the old lists
+    hipFree(L.d_tile_y);
+    hipFree(L.d_tile_z);
+    L.d_tile_y = dy; L.ntile_y = (int)ty.size();
+    L.d_tile_z = dz; L.ntile_z = (int)tz.size();
+    L.march_ch = ch;
+    return OCFD_OK;
+}
+
+int ocfd_level_march_chunk(ocfd_level lev) { return lev ? lev->dev.march_ch : -1; }
+
+int ocfd_level_autotune(ocfd_level lev, const double* S, const ocfd_parm* parm, int flags, int reps, int* ch_out,
+                        double* ms_out, void* stream)
+{
+    if (!lev || !S || !parm || !ch_out || reps <= 0 || !valid_flags(flags)) {
+        set_error("ocfd_level_autotune: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    static const int cand[4] = {8, 16, 32, 64};
+    hipStream_t s = (hipStream_t)stream;
+    LevelDev& L = lev->dev;
+    if (!L.ntile_y || !L.ntile_p) {   // no regular boxes: nothing to tune
+        *ch_out = L.march_ch;
+        return OCFD_OK;
+    }
+    StageArgs a{};
+    a.scur = S; a.acc = L.d_acc;   // plain dSdt into level scratch: timing only
+    a.c_cur = 1.0; a.flags = flags & ~OCFD_PLANAR; a.ph = phys_of(*parm);
+    const bool planar = (flags & OCFD_PLANAR) != 0;
+    hipEvent_t e0, e1;
+    OCFD_HIP(hipEventCreate(&e0));
+    OCFD_HIP(hipEventCreate(&e1));
+    launch_sweep_one(L, a, 4, s);   // primitives the sweeps read
+    int best = L.march_ch;
+    double best_ms = 1e300;
+    int rc = OCFD_OK;
+    for (int c = 0; c < 4 && rc == OCFD_OK; ++c) {
+        if ((rc = ocfd_level_set_march_chunk(lev, cand[c]))) break;
+        launch_sweep_one(L, a, 1, s);   // warm-up of this tile list
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = OCFD_EHIP; break; }
+        for (int r = 0; r < reps; ++r) {
+            launch_sweep_one(L, a, 1, s);
+            if (!planar) launch_sweep_one(L, a, 2, s);
+        }
+        float ms = 0.f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = OCFD_EHIP;
+            break;
+        }
+        const double t = ms / reps;
+        if (ms_out) ms_out[c] = t;
+        if (t < best_ms) { best_ms = t; best = cand[c]; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) {
+        set_error("ocfd_level_autotune: timing failed");
+        return rc;
+    }
+    if ((rc = ocfd_level_set_march_chunk(lev, best))) return rc;
+    *ch_out = best;
+    return check_launch("ocfd_level_autotune");
 }
 
 // ---------------------------------------------------------------------------

@@ -153,6 +153,9 @@ def lib():
                                         ctypes.c_int, ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, vp]
         L.ocfd_init_species_state.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(Parm)]
         L.ocfd_tag_dengrad.argtypes = [vp, vp, ctypes.c_double, vp, vp]
+        L.ocfd_level_set_march_chunk.argtypes = [vp, ctypes.c_int]
+        L.ocfd_level_march_chunk.argtypes = [vp]
+        L.ocfd_level_autotune.argtypes = [vp, vp, ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, ip, dp, vp]
         L.ocfd_cluster_tags.argtypes = [ip, ip, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte),
                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ip, ip]
         L.ocfd_lincomb.argtypes = [ctypes.c_long, ctypes.c_double, vp, ctypes.c_double, vp, vp, vp]
@@ -328,6 +331,21 @@ class Level:
     def rk_stage(self, S_cur, S_old, S_new, c_cur, c_old, c_dt, parm, flags=0, stream=None):
         check(lib().ocfd_rk_stage(self.handle, _ptr(S_cur), _ptr(S_old), _ptr(S_new), c_cur, c_old, c_dt,
                                   ctypes.byref(parm), flags, _stream(stream)), "ocfd_rk_stage")
+
+    def march_chunk(self, ch=None):
+        """Cells per march chunk of the y/z sweeps (set it when ch is given)."""
+        if ch is not None:
+            check(lib().ocfd_level_set_march_chunk(self.handle, ch), "ocfd_level_set_march_chunk")
+        return lib().ocfd_level_march_chunk(self.handle)
+
+    def autotune(self, S, parm, flags=0, reps=5, stream=None):
+        """Time the march tile shapes on state S (ghosts filled) and keep the
+        fastest; returns (chunk, {chunk: ms})."""
+        ch = ctypes.c_int()
+        ms = (ctypes.c_double * 4)()
+        check(lib().ocfd_level_autotune(self.handle, _ptr(S), ctypes.byref(parm), flags, reps, ctypes.byref(ch), ms,
+                                        _stream(stream)), "ocfd_level_autotune")
+        return ch.value, dict(zip((8, 16, 32, 64), list(ms)))
 
     def fill_boundary(self, S, time, stream=None):
         check(lib().ocfd_fill_boundary(self.handle, _ptr(S), time, _stream(stream)), "ocfd_fill_boundary")
