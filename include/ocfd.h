@@ -469,6 +469,16 @@ int ocfd_plotfile_write(const char* dir, int nlev, const ocfd_level* levels, con
 int ocfd_plotfile_compare(const char* file_a, const char* file_b, int norm, double rtol, double atol,
                           int allow_diff_grids, int* result, char* report, long report_cap);
 
+/* Checkpoints (AmrLevel::checkPoint / restart, Source/IO.cpp:9-30): the valid
+ * cells of a level's MultiFab S (4 ghosts, ncomp comps) in VisMF form,
+ * dir/<name>_H and dir/<name>_D_00000 (the FAB format of the plotfiles), and
+ * back into S (ghost cells zeroed) for a level with the same boxes.  The
+ * hierarchy's times, steps, dts and grids go in a text header the driver
+ * writes beside them.  Synchronise the stream. */
+int ocfd_checkpoint_write_level(const char* dir, const char* name, ocfd_level lev, const double* S, int ncomp,
+                                void* stream);
+int ocfd_checkpoint_read_level(const char* dir, const char* name, ocfd_level lev, double* S, int ncomp, void* stream);
+
 /* Last error message of the calling thread ("" if none). */
 const char* ocfd_last_error(void);
 

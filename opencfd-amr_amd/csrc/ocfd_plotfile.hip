@@ -99,6 +99,59 @@ int fail(const std::string& m)
 
 } // namespace
 
+namespace {
+
+// A MultiFab on disk in VisMF form: <dir>/<name>_D_00000 holds every FAB (a
+// one-line FAB header, then the raw doubles, component-major, i fastest) and
+// <dir>/<name>_H the header (version, how, ncomp, nghost, BoxArray, FabOnDisk
+// entries, per-FAB minima and maxima).  h: the level's 0-ghost layout with
+// ncomp components.
+int write_vismf(const std::string& dir, const std::string& name, const LevelDev& L, const double* h, int ncomp)
+{
+    const std::string dname = name + "_D_00000";
+    std::ofstream df(dir + "/" + dname, std::ios::binary);
+    if (!df) return fail("cannot write " + dir + "/" + dname);
+    std::vector<long> offs;
+    std::vector<double> mn, mx;
+    for (const BoxDesc& B : L.boxes) {
+        offs.push_back((long)df.tellp());
+        const std::string hdr = std::string("FAB (") + kRealDesc + ")" + box_str(B.lo, B.hi) + " " +
+                                std::to_string(ncomp) + "\n";
+        df.write(hdr.data(), (std::streamsize)hdr.size());
+        const double* p = h + B.off_d / NC * ncomp;
+        df.write((const char*)p, (std::streamsize)(sizeof(double) * ncomp * B.nptv));
+        for (int n = 0; n < ncomp; ++n) {
+            const double* q = p + n * B.nptv;
+            mn.push_back(*std::min_element(q, q + B.nptv));
+            mx.push_back(*std::max_element(q, q + B.nptv));
+        }
+    }
+    if (!df) return fail("write error in " + dir + "/" + dname);
+    df.close();
+    std::ofstream hf(dir + "/" + name + "_H");
+    if (!hf) return fail("cannot write " + dir + "/" + name + "_H");
+    hf << std::setprecision(17);
+    hf << 1 << '\n' << 1 << '\n' << ncomp << '\n' << 0 << '\n';
+    hf << '(' << L.nbox << ' ' << 0 << '\n';
+    for (const BoxDesc& B : L.boxes) hf << box_str(B.lo, B.hi) << '\n';
+    hf << ')' << '\n';
+    hf << L.nbox << '\n';
+    for (int b = 0; b < L.nbox; ++b) hf << "FabOnDisk: " << dname << ' ' << offs[b] << '\n';
+    hf << '\n';
+    for (const std::vector<double>* v : {&mn, &mx}) {
+        hf << L.nbox << ',' << ncomp << '\n';
+        for (int b = 0; b < L.nbox; ++b) {
+            for (int n = 0; n < ncomp; ++n) hf << (*v)[(size_t)b * ncomp + n] << ',';
+            hf << '\n';
+        }
+        hf << '\n';
+    }
+    if (!hf) return fail("write error in " + dir + "/" + name + "_H");
+    return OCFD_OK;
+}
+
+} // namespace
+
 extern "C" int ocfd_plot_vars(ocfd_level lev, const double* S, double* out, const ocfd_parm* parm, void* stream)
 {
     if (!lev || !S || !out || !parm) return fail("ocfd_plot_vars: invalid arguments");
@@ -151,46 +204,7 @@ extern "C" int ocfd_plotfile_write(const char* dir, int nlev, const ocfd_level* 
         }
         hipFree(d_out);
         if (rc) return rc;
-        // data file: every FAB with its one-line header
-        std::ofstream df(ldir + "/Cell_D_00000", std::ios::binary);
-        if (!df) return fail("ocfd_plotfile_write: cannot write " + ldir + "/Cell_D_00000");
-        std::vector<long> offs;
-        std::vector<double> mn, mx;
-        for (const BoxDesc& B : L.boxes) {
-            offs.push_back((long)df.tellp());
-            const std::string hdr = std::string("FAB (") + kRealDesc + ")" + box_str(B.lo, B.hi) + " " +
-                                    std::to_string(NPLOT) + "\n";
-            df.write(hdr.data(), (std::streamsize)hdr.size());
-            const double* p = h.data() + B.off_d / NC * NPLOT;
-            df.write((const char*)p, (std::streamsize)(sizeof(double) * NPLOT * B.nptv));
-            for (int n = 0; n < NPLOT; ++n) {
-                const double* q = p + n * B.nptv;
-                mn.push_back(*std::min_element(q, q + B.nptv));
-                mx.push_back(*std::max_element(q, q + B.nptv));
-            }
-        }
-        if (!df) return fail("ocfd_plotfile_write: write error in " + ldir);
-        df.close();
-        // VisMF header
-        std::ofstream hf(ldir + "/Cell_H");
-        if (!hf) return fail("ocfd_plotfile_write: cannot write " + ldir + "/Cell_H");
-        hf << std::setprecision(17);
-        hf << 1 << '\n' << 1 << '\n' << NPLOT << '\n' << 0 << '\n';
-        hf << '(' << L.nbox << ' ' << 0 << '\n';
-        for (const BoxDesc& B : L.boxes) hf << box_str(B.lo, B.hi) << '\n';
-        hf << ')' << '\n';
-        hf << L.nbox << '\n';
-        for (int b = 0; b < L.nbox; ++b) hf << "FabOnDisk: Cell_D_00000 " << offs[b] << '\n';
-        hf << '\n';
-        for (const std::vector<double>* v : {&mn, &mx}) {
-            hf << L.nbox << ',' << NPLOT << '\n';
-            for (int b = 0; b < L.nbox; ++b) {
-                for (int n = 0; n < NPLOT; ++n) hf << (*v)[(size_t)b * NPLOT + n] << ',';
-                hf << '\n';
-            }
-            hf << '\n';
-        }
-        if (!hf) return fail("ocfd_plotfile_write: write error in " + ldir + "/Cell_H");
+        if ((rc = write_vismf(ldir, "Cell", L, h.data(), NPLOT))) return rc;
     }
     // plotfile header
     std::ofstream H(root + "/Header");
@@ -499,6 +513,95 @@ extern "C" int ocfd_plotfile_compare(const char* file_a, const char* file_b, int
         const size_t n = std::min((size_t)(report_cap - 1), s.size());
         std::memcpy(report, s.data(), n);
         report[n] = '\0';
+    }
+    return OCFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// checkpoints: a level's MultiFab (valid cells) in VisMF form
+// ---------------------------------------------------------------------------
+extern "C" int ocfd_checkpoint_write_level(const char* dir, const char* name, ocfd_level lev, const double* S,
+                                           int ncomp, void* stream)
+{
+    if (!dir || !name || !lev || !S || ncomp <= 0) return fail("ocfd_checkpoint_write_level: invalid arguments");
+    const LevelDev& L = lev->dev;
+    if (!make_dir(dir)) return fail(std::string("ocfd_checkpoint_write_level: cannot create ") + dir);
+    const long ngrown = L.mf_state / NC, nvalid = L.mf_valid / NC;
+    std::vector<double> g((size_t)ncomp * ngrown), v((size_t)ncomp * nvalid);
+    hipStream_t st = (hipStream_t)stream;
+    if (hipMemcpyAsync(g.data(), S, sizeof(double) * g.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("ocfd_checkpoint_write_level: copy to host failed");
+        return OCFD_EHIP;
+    }
+    for (const BoxDesc& B : L.boxes) {
+        const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+        const double* gs = g.data() + B.off_s / NC * ncomp;
+        double* vs = v.data() + B.off_d / NC * ncomp;
+        for (int n = 0; n < ncomp; ++n)
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j)
+                    std::memcpy(vs + n * B.nptv + ((long)k * ny + j) * nx,
+                                gs + n * B.nptg + ((long)(k + NG) * B.nyg + (j + NG)) * B.nxg + NG,
+                                sizeof(double) * nx);
+    }
+    return write_vismf(dir, name, L, v.data(), ncomp);
+}
+
+extern "C" int ocfd_checkpoint_read_level(const char* dir, const char* name, ocfd_level lev, double* S, int ncomp,
+                                          void* stream)
+{
+    if (!dir || !name || !lev || !S || ncomp <= 0) return fail("ocfd_checkpoint_read_level: invalid arguments");
+    const LevelDev& L = lev->dev;
+    const std::string base = std::string(dir) + "/" + name;
+    std::ifstream C(base + "_H");
+    if (!C) return fail("ocfd_checkpoint_read_level: cannot open " + base + "_H");
+    int vers, how, nc, ng, nb, hash;
+    char c;
+    C >> vers >> how >> nc >> ng >> c >> nb >> hash;
+    if (!C || nc != ncomp || nb != L.nbox) return fail("ocfd_checkpoint_read_level: " + base + "_H does not match the level");
+    for (int b = 0; b < nb; ++b) {
+        PBox x;
+        if (!parse_box(C, x) || !std::equal(x.lo, x.lo + 3, L.boxes[b].lo) || !std::equal(x.hi, x.hi + 3, L.boxes[b].hi))
+            return fail("ocfd_checkpoint_read_level: box " + std::to_string(b) + " differs from the level's");
+    }
+    C >> c;
+    int nfod;
+    C >> nfod;
+    std::vector<std::string> files(nfod);
+    std::vector<long> offs(nfod);
+    for (int f = 0; f < nfod; ++f) {
+        std::string tag;
+        C >> tag >> files[f] >> offs[f];
+    }
+    if (!C || nfod != nb) return fail("ocfd_checkpoint_read_level: bad FabOnDisk list in " + base + "_H");
+    const long ngrown = L.mf_state / NC;
+    std::vector<double> g((size_t)ncomp * ngrown, 0.0);
+    for (int b = 0; b < nb; ++b) {
+        const BoxDesc& B = L.boxes[b];
+        std::ifstream D(std::string(dir) + "/" + files[b], std::ios::binary);
+        if (!D) return fail("ocfd_checkpoint_read_level: cannot open " + files[b]);
+        D.seekg(offs[b]);
+        std::string fh;
+        std::getline(D, fh);
+        if (fh.compare(0, 5, "FAB (") != 0 || fh.find(kRealDesc) == std::string::npos)
+            return fail("ocfd_checkpoint_read_level: " + files[b] + " is not little-endian IEEE double");
+        const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+        std::vector<double> v((size_t)ncomp * B.nptv);
+        D.read((char*)v.data(), (std::streamsize)(sizeof(double) * v.size()));
+        if (!D) return fail("ocfd_checkpoint_read_level: truncated data in " + files[b]);
+        double* gs = g.data() + B.off_s / NC * ncomp;
+        for (int n = 0; n < ncomp; ++n)
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j)
+                    std::memcpy(gs + n * B.nptg + ((long)(k + NG) * B.nyg + (j + NG)) * B.nxg + NG,
+                                v.data() + n * B.nptv + ((long)k * ny + j) * nx, sizeof(double) * nx);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (hipMemcpyAsync(S, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("ocfd_checkpoint_read_level: copy to the device failed");
+        return OCFD_EHIP;
     }
     return OCFD_OK;
 }

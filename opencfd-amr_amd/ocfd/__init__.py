@@ -172,6 +172,8 @@ def lib():
         L.ocfd_cf_reflux.argtypes = [vp, vp, vp]
         L.ocfd_cf_avg_down.argtypes = [vp, vp, vp, ctypes.c_int, vp]
         L.ocfd_plot_vars.argtypes = [vp, vp, vp, ctypes.POINTER(Parm), vp]
+        L.ocfd_checkpoint_write_level.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, vp, ctypes.c_int, vp]
+        L.ocfd_checkpoint_read_level.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, vp, ctypes.c_int, vp]
         L.ocfd_plotfile_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp),
                                           ctypes.c_double, ip, ctypes.POINTER(Parm), vp]
         L.ocfd_plotfile_compare.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,

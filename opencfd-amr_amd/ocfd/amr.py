@@ -1,4 +1,4 @@
-"""Two-level AMR time stepping on the device (refinement ratio 2, subcycled).
+"""AMR hierarchies on the device (any depth, refinement ratio 2, subcycled).
 
 The driver follows the AmrLevel/Amr semantics the reference runs under
 (Source/EBR.cpp, Source/Advance.cpp):
@@ -22,6 +22,7 @@ the dt reductions and the tags of a regrid.
 """
 import ctypes
 import importlib
+import os
 
 import numpy as np
 
@@ -499,6 +500,80 @@ class Hierarchy:
             self.step()
             n += 1
         return n
+
+    # -- checkpoint / restart -----------------------------------------------------
+    def checkpoint(self, path):
+        """Amr::checkPoint (AmrLevel::checkPoint per level, Source/IO.cpp:27-30):
+        a text Header with the hierarchy's clock and grids, and each level's
+        current state (valid cells) as Level_<l>/SD_0_New in VisMF form.  The
+        old states are not kept: every advance overwrites them before use."""
+        os.makedirs(path, exist_ok=True)
+        with open(os.path.join(path, "Header"), "w") as f:
+            f.write("ocfd-checkpoint 1\n")
+            f.write("max_level %d\nfinest_level %d\n" % (self.max_level, self.finest))
+            f.write("time %r\nsteps %d\n" % (self.time, self.steps))
+            f.write("dt_level %s\n" % " ".join(repr(float(x)) for x in self.dt_level))
+            f.write("level_steps %s\n" % " ".join("%d" % x for x in self.level_steps))
+            f.write("level_count %s\n" % " ".join("%d" % x for x in self.count))
+            for l in range(self.finest + 1):
+                f.write("level %d %d\n" % (l, len(self.levels[l].boxes)))
+                for lo, hi in self.levels[l].boxes:
+                    f.write("%d %d %d %d %d %d\n" % (*lo, *hi))
+        for l in range(self.finest + 1):
+            ldir = os.path.join(path, "Level_%d" % l)
+            check(lib().ocfd_checkpoint_write_level(ldir.encode(), b"SD_0_New", self.levels[l].handle,
+                                                    _ptr(self.S[l]), NCONS, _stream(self.stream)),
+                  "ocfd_checkpoint_write_level")
+
+    @classmethod
+    def restart(cls, path, geom, parm, cfl, **kw):
+        """Amr::restart: a hierarchy with the grids, clock and states of the
+        checkpoint at `path` (geom: level 0's geometry; the remaining keywords
+        as for the constructor, max_level defaulting to the checkpoint's)."""
+        with open(os.path.join(path, "Header")) as f:
+            lines = f.read().split("\n")
+        if lines[0] != "ocfd-checkpoint 1":
+            raise ValueError(f"{path} is not a checkpoint")
+        hdr, boxes, i = {}, [], 1
+        while i < len(lines) and lines[i]:
+            key, *vals = lines[i].split()
+            i += 1
+            if key == "level":
+                nb = int(vals[1])
+                boxes.append([(tuple(int(x) for x in lines[i + b].split()[:3]),
+                               tuple(int(x) for x in lines[i + b].split()[3:])) for b in range(nb)])
+                i += nb
+            else:
+                hdr[key] = vals
+        finest = int(hdr["finest_level"][0])
+        kw.setdefault("max_level", int(hdr["max_level"][0]))
+        if kw["max_level"] < finest:
+            raise ValueError(f"max_level {kw['max_level']} is below the checkpoint's finest level {finest}")
+        kw.pop("kind", None)
+        h = cls(boxes[0], geom, parm, cfl, None, **kw)
+        t = float(hdr["time"][0])
+        for l in range(finest + 1):
+            lev = h.levels[0] if l == 0 else Level(boxes[l], h.geoms[l])
+            S = h.S[0] if l == 0 else lev.new_mf()
+            check(lib().ocfd_checkpoint_read_level(os.path.join(path, "Level_%d" % l).encode(), b"SD_0_New",
+                                                   lev.handle, _ptr(S), NCONS, _stream(h.stream)),
+                  "ocfd_checkpoint_read_level")
+            if l == 0:
+                h.st[0].t_old = h.st[0].t_new = t
+            else:
+                h._set_level(l, lev, S, t)
+                h.finest = l
+        h._check_links()
+        h.time, h.steps = t, int(hdr["steps"][0])
+        n = h.max_level + 1
+
+        def ext(vals, conv, fill):
+            v = [conv(x) for x in vals][:n]
+            return v + [fill] * (n - len(v))
+        h.dt_level = ext(hdr["dt_level"], float, 0.0)
+        h.level_steps = ext(hdr["level_steps"], int, 0)
+        h.count = ext(hdr["level_count"], int, 0)
+        return h
 
     def cells(self):
         """Valid cells per level."""
