@@ -2,10 +2,13 @@
 // path, Source/Advance.cpp:231-342), one kernel per direction over the box
 // work-list of a whole level (or of many levels/cases at once).
 //
-//   sweep_x : wave = one x-row segment of 58 cells + 3 halo cells each side.
-//             Each lane recovers primitives and splits its cell (FP64 prims,
-//             FT split), the split fluxes are exchanged through LDS, lanes
-//             3..61 evaluate the 59 faces, and the x-part of -div F is stored.
+//   sweep_x : wave = one x-row segment of 58 cells + 3 halo cells each side,
+//             over every row of the primitive region.  Each lane recovers
+//             the primitives of its cell from the state (FP64, c2prim) and
+//             stores them for the y and z sweeps; on rows inside the box it
+//             also splits its cell (FT), the split fluxes are exchanged
+//             through LDS, lanes 3..61 evaluate the 59 faces, and the x-part
+//             of -div F is stored.
 //   sweep_y : lane = one i column, marches j through a chunk of cells with
 //             a register window of split fluxes (6 cells of f+, 5 of f-), and
 //             adds the y-part into the accumulator.
@@ -16,7 +19,7 @@
 // dx-term + dy-term + dz-term (Source/kernel/Kernels.H:94-96); the partial
 // sums between sweeps are stored in the flux type FT (FP64 in the plain form,
 // FP32 in the FP32-flux form, where they are differences of FP32 fluxes).
-// Every sweep reads the primitive MultiFab written once per stage by k_prims.
+// The y and z sweeps read the primitive MultiFab the x sweep wrote.
 #include "ocfd_device.hpp"
 #include "ocfd_internal.hpp"
 
@@ -35,40 +38,55 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
     const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const BoxDesc B = boxes[t.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    const int ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    // rows of the primitive region grow(bx, (3,3,gz)); rows inside the box
+    // also carry the x faces
+    const int gz = t.w, ey = ny + 6;
     const int row = t.z * 4 + w;
-    const bool rowok = row < ny * nz;
-    const int j = B.lo[1] + (rowok ? row % ny : 0);
-    const int k = B.lo[2] + (rowok ? row / ny : 0);
+    const bool rowok = row < ey * (nz + 2 * gz);
+    const int j = B.lo[1] - 3 + (rowok ? row % ey : 3);
+    const int k = B.lo[2] - gz + (rowok ? row / ey : gz);
+    const bool frow = rowok && j >= B.lo[1] && j <= B.hi[1] && k >= B.lo[2] && k <= B.hi[2];
     const int i = B.lo[0] + t.y * XSEG - 3 + lane;
     const bool cellok = rowok && i <= B.hi[0] + 3;
     const FT gam = (FT)a.ph.gamma;
-    (void)nx;
 
-    const FT* __restrict__ P = (const FT*)a.prim;
     if (cellok) {
-        const long s0 = sidx(B, i, j, k) - B.off_s + B.off_p;
-        Prim<FT> q = ldprim<FT>(P, s0, B.nptg);
-        FT fp[5], fm[5];
-        sw_split<0, FT>(q, gam, fp, fm);
+        const long s0 = sidx(B, i, j, k);
+        const long st = B.nptg;
+        const Prim<FT> q = c2prim<FT>(a.scur, s0, st, a.ph);
+        // each primitive is stored once: interior lanes, and the halo lanes
+        // that lie outside the box in x
+        if ((lane >= 3 && lane <= XSEG + 2) || i < B.lo[0] || i > B.hi[0]) {
+            FT* __restrict__ P = (FT*)a.prim;
+            const long p0 = s0 - B.off_s + B.off_p;
+            P[p0] = q.r; P[p0 + st] = q.u; P[p0 + 2 * st] = q.v; P[p0 + 3 * st] = q.w;
+            P[p0 + 4 * st] = q.c; P[p0 + 5 * st] = q.T;
+        }
+        if (frow) {
+            FT fp[5], fm[5];
+            sw_split<0, FT>(q, gam, fp, fm);
 #pragma unroll
-        for (int n = 0; n < 5; ++n) { sf[w][n][lane] = fp[n]; sf[w][5 + n][lane] = fm[n]; }
-        if (VISC) {
-            sv[w][0][lane] = q.T; sv[w][1][lane] = q.u; sv[w][2][lane] = q.v; sv[w][3][lane] = q.w;
-            FT u, v, ww;
-            ldvel<FT>(P, s0 + B.nxg, B.nptg, u, v, ww);              // (i, j+1)
-            sv[w][4][lane] = u; sv[w][5][lane] = v;
-            ldvel<FT>(P, s0 - B.nxg, B.nptg, u, v, ww);              // (i, j-1)
-            sv[w][6][lane] = u; sv[w][7][lane] = v;
-            ldvel<FT>(P, s0 + (long)B.nxg * B.nyg, B.nptg, u, v, ww); // (i, k+1)
-            sv[w][8][lane] = u; sv[w][9][lane] = ww;
-            ldvel<FT>(P, s0 - (long)B.nxg * B.nyg, B.nptg, u, v, ww); // (i, k-1)
-            sv[w][10][lane] = u; sv[w][11][lane] = ww;
+            for (int n = 0; n < 5; ++n) { sf[w][n][lane] = fp[n]; sf[w][5 + n][lane] = fm[n]; }
+            if (VISC) {
+                // tangential neighbours' velocities from the state itself
+                // (their primitives are being written by other waves)
+                sv[w][0][lane] = q.T; sv[w][1][lane] = q.u; sv[w][2][lane] = q.v; sv[w][3][lane] = q.w;
+                FT u, v, ww;
+                c2vel<FT>(a.scur, s0 + B.nxg, st, a.ph, u, v, ww);               // (i, j+1)
+                sv[w][4][lane] = u; sv[w][5][lane] = v;
+                c2vel<FT>(a.scur, s0 - B.nxg, st, a.ph, u, v, ww);               // (i, j-1)
+                sv[w][6][lane] = u; sv[w][7][lane] = v;
+                c2vel<FT>(a.scur, s0 + (long)B.nxg * B.nyg, st, a.ph, u, v, ww);  // (i, k+1)
+                sv[w][8][lane] = u; sv[w][9][lane] = ww;
+                c2vel<FT>(a.scur, s0 - (long)B.nxg * B.nyg, st, a.ph, u, v, ww);  // (i, k-1)
+                sv[w][10][lane] = u; sv[w][11][lane] = ww;
+            }
         }
     }
     __syncthreads();
 
-    const bool faceok = rowok && lane >= 3 && lane <= 61 && i <= B.hi[0] + 1;
+    const bool faceok = frow && lane >= 3 && lane <= 61 && i <= B.hi[0] + 1;
     FT F[5];
     if (faceok) {
 #pragma unroll
@@ -94,7 +112,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
         for (int n = 0; n < 5; ++n) sf[w][n][lane] = F[n];
     }
     __syncthreads();
-    if (rowok && lane >= 3 && lane <= 60 && i <= B.hi[0]) {
+    if (frow && lane >= 3 && lane <= 60 && i <= B.hi[0]) {
         const long d0 = didx(B, i, j, k);
         FT* __restrict__ part = (FT*)a.part;
 #pragma unroll
@@ -267,10 +285,9 @@ static void launch_sweeps_t(const LevelDev& L, const StageArgs& a0, bool planar,
     StageArgs a = a0;
     a.prim = sizeof(FT) == 4 ? (const void*)L.d_prim32 : (const void*)L.d_prim64;
     a.part = (void*)L.d_acc;
-    const int np = planar ? L.ntile_pp : L.ntile_p;
-    const int4* tp = planar ? L.d_tile_pp : L.d_tile_p;
-    if (np) hipLaunchKernelGGL((k_prims<FT>), dim3(np), dim3(256), 0, s, L.d_boxes, tp, a);
-    if (L.ntile_x) hipLaunchKernelGGL((k_sweep_x<FT, VISC>), dim3(L.ntile_x), dim3(256), 0, s, L.d_boxes, L.d_tile_x, a);
+    const int nx = planar ? L.ntile_xp : L.ntile_x;
+    if (nx) hipLaunchKernelGGL((k_sweep_x<FT, VISC>), dim3(nx), dim3(256), 0, s, L.d_boxes,
+                               planar ? L.d_tile_xp : L.d_tile_x, a);
     if (planar) {
         if (L.ntile_y) hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, true>), dim3(L.ntile_y), dim3(256), 0, s,
                                           L.d_boxes, L.d_tile_y, a);

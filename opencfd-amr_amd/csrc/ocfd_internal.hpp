@@ -17,7 +17,8 @@ struct LevelDev {
     BoxDesc* d_boxes = nullptr;
     GeomDesc* d_geom = nullptr;
     int* d_map = nullptr;
-    int4* d_tile_x = nullptr; int ntile_x = 0;
+    int4* d_tile_x = nullptr; int ntile_x = 0;     // x sweep: (box, segment, row group, z growth 3)
+    int4* d_tile_xp = nullptr; int ntile_xp = 0;   // planar x sweep (z growth 1)
     int4* d_tile_y = nullptr; int ntile_y = 0;
     int4* d_tile_z = nullptr; int ntile_z = 0;
     int4* d_tile_g = nullptr; int ntile_g = 0;   // ghost fill: (box, first ghost cell, count, 0)

@@ -251,7 +251,7 @@ static int upload(T** dptr, const std::vector<T>& h)
 static void free_level(LevelDev& L)
 {
     hipFree(L.d_boxes); hipFree(L.d_geom); hipFree(L.d_map);
-    hipFree(L.d_tile_x); hipFree(L.d_tile_y); hipFree(L.d_tile_z); hipFree(L.d_tile_g); hipFree(L.d_tile_c);
+    hipFree(L.d_tile_x); hipFree(L.d_tile_xp); hipFree(L.d_tile_y); hipFree(L.d_tile_z); hipFree(L.d_tile_g); hipFree(L.d_tile_c);
     hipFree(L.d_acc); hipFree(L.d_red); hipFree(L.d_tile_p); hipFree(L.d_tile_pp);
     hipFree(L.d_prim32); hipFree(L.d_prim64); hipFree(L.d_tile_v);
     hipFree(L.d_ebflag); hipFree(L.d_ebgeo); hipFree(L.d_ebq); hipFree(L.d_tile_eq); hipFree(L.d_tile_ed);
@@ -469,7 +469,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         return std::max(ny_t, nz_t);
     };
     while (ch > 8 && march_count(ch) < 2048) ch /= 2;
-    std::vector<int4> tx, ty, tz, tg, tc, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
+    std::vector<int4> tx, txp, ty, tz, tg, tc, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
         const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
@@ -495,9 +495,13 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
             for (long s = 0; s < nf; s += 256) tmf[d].push_back(make_int4(b, (int)s, (int)std::min<long>(256, nf - s), d));
         }
         for (long s = 0; s < B.nptv; s += 256) tmd.push_back(make_int4(b, (int)s, (int)std::min<long>(256, B.nptv - s), 0));
-        const int nseg = (nx + xseg - 1) / xseg, nrg = (ny * nz + 3) / 4;
-        for (int r = 0; r < nrg; ++r)
-            for (int s = 0; s < nseg; ++s) tx.push_back(make_int4(b, s, r, 0));
+        // x sweep rows: the whole primitive region grow(bx, (3,3,gz))
+        const int nseg = (nx + xseg - 1) / xseg;
+        for (int gz : {3, 1}) {
+            const int nrg = ((ny + 6) * (nz + 2 * gz) + 3) / 4;
+            for (int r = 0; r < nrg; ++r)
+                for (int s = 0; s < nseg; ++s) (gz == 3 ? tx : txp).push_back(make_int4(b, s, r, gz));
+        }
         // march tiles: i segment fastest, then chunk, then row group, so a
         // contiguous range of the list (one XCD's share) holds tiles that
         // share chunk halos and transverse neighbour rows
@@ -511,7 +515,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     L.march_ch = ch;
     march_tiles(L, ch, ty, tz);
     int rc = OCFD_OK;
-    if ((rc = upload(&L.d_map, L.owner_map)) || (rc = upload(&L.d_tile_x, tx)) || (rc = upload(&L.d_tile_y, ty)) ||
+    if ((rc = upload(&L.d_map, L.owner_map)) || (rc = upload(&L.d_tile_x, tx)) || (rc = upload(&L.d_tile_xp, txp)) || (rc = upload(&L.d_tile_y, ty)) ||
         (rc = upload(&L.d_tile_z, tz)) || (rc = upload(&L.d_tile_g, tg)) || (rc = upload(&L.d_tile_c, tc)) ||
         (rc = upload(&L.d_tile_p, tp)) || (rc = upload(&L.d_tile_pp, tpp)) || (rc = upload(&L.d_tile_v, tv)) ||
         (rc = upload(&L.d_tile_eq, teq)) || (rc = upload(&L.d_tile_ef[0], tef[0])) ||
@@ -529,7 +533,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         for (int d = 0; d < 3 && ok; ++d) ok = hipMalloc((void**)&L.d_ebF[d], sizeof(double) * NC * L.mf_eb) == hipSuccess;
         if (!ok) { set_error("ocfd_level_create_eb: out of device memory"); free_level(L); delete lev; return OCFD_ENOMEM; }
     }
-    L.ntile_x = (int)tx.size(); L.ntile_y = (int)ty.size(); L.ntile_z = (int)tz.size();
+    L.ntile_x = (int)tx.size(); L.ntile_xp = (int)txp.size(); L.ntile_y = (int)ty.size(); L.ntile_z = (int)tz.size();
     L.ntile_g = (int)tg.size(); L.ntile_c = (int)tc.size();
     L.ntile_p = (int)tp.size(); L.ntile_pp = (int)tpp.size(); L.ntile_v = (int)tv.size();
     L.mf_prim = L.mf_state / NC * 6;
@@ -544,7 +548,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         free_level(L); delete lev; return OCFD_ENOMEM;
     }
     L.dev_bytes = (long)sizeof(int) * (long)L.owner_map.size() +
-                  (long)sizeof(int4) * (L.ntile_x + L.ntile_y + L.ntile_z + L.ntile_g + L.ntile_c + L.ntile_p +
+                  (long)sizeof(int4) * (L.ntile_x + L.ntile_xp + L.ntile_y + L.ntile_z + L.ntile_g + L.ntile_c + L.ntile_p +
                                         L.ntile_pp + L.ntile_v) +
                   (long)sizeof(GeomDesc) + (long)sizeof(BoxDesc) * L.nbox + (long)sizeof(double) * L.mf_valid +
                   (long)(sizeof(float) + sizeof(double)) * L.mf_prim + (long)sizeof(unsigned long long) +
