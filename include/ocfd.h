@@ -363,6 +363,14 @@ int ocfd_tag_dengrad(ocfd_level lev, const double* S, double threshold, unsigned
 int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigned char* tags, const unsigned char* mask, int block,
                       double grid_eff, int max_size_cells, int cap, int* boxes_out, int* nbox_out);
 
+/* A dense 0/1 mask over a level's domain (nz*ny*nx bytes, i fastest, host
+ * memory) grown in place by n cells in every direction (Chebyshev: the tag
+ * buffer amr.n_error_buf), or with shrink != 0 shrunk by n cells (the proper
+ * nesting domain of a level's coverage, amr.n_proper).  Periodic axes wrap;
+ * outside the domain elsewhere counts as unset when growing and set when
+ * shrinking.  Host only. */
+int ocfd_mask_grow(unsigned char* m, int nx, int ny, int nz, int n, const int* periodic, int shrink);
+
 /* out[0..n) = a*x + b*y: the two-time data combination of FillPatch at a time
  * between a level's old and new states (a = (t1-t)/(t1-t0), b = (t-t0)/(t1-t0)). */
 int ocfd_lincomb(long n, double a, const double* x, double b, const double* y, double* out, void* stream);

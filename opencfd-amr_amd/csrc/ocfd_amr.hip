@@ -146,12 +146,6 @@ __global__ __launch_bounds__(128) void k_reflux(const BoxDesc* __restrict__ boxe
     for (int n = 0; n < 5; ++n) S[s0 + n * st] = U[n];
 }
 
-__device__ __forceinline__ int owner(const GeomDesc& G, int i, int j, int k)
-{
-    return G.map[((long)((k - G.dom_lo[2]) >> G.map_shift) * G.map_n[1] + ((j - G.dom_lo[1]) >> G.map_shift)) *
-                     G.map_n[0] + ((i - G.dom_lo[0]) >> G.map_shift)];
-}
-
 __device__ __forceinline__ void valid_cell(const BoxDesc& B, int c, int& i, int& j, int& k)
 {
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
@@ -174,7 +168,7 @@ __global__ __launch_bounds__(256) void k_avg_down(const BoxDesc* __restrict__ fb
     const int nx = (Bf.hi[0] - Bf.lo[0] + 1) >> 1, ny = (Bf.hi[1] - Bf.lo[1] + 1) >> 1;
     const int c = t.y + threadIdx.x;
     const int I = (Bf.lo[0] >> 1) + c % nx, J = (Bf.lo[1] >> 1) + (c / nx) % ny, K = (Bf.lo[2] >> 1) + c / (nx * ny);
-    const int m = owner(*cgeom, I, J, K);
+    const int m = level_owner(*cgeom, cboxes, I, J, K);
     if (m < 0) return;
     const BoxDesc Bc = cboxes[m];
     const long dst = Bc.off_s / NC * ncomp + (sidx(Bc, I, J, K) - Bc.off_s);
@@ -229,8 +223,8 @@ __device__ __forceinline__ bool cons_interp(const BoxDesc* __restrict__ cboxes, 
     if (I < CG.dom_lo[0] || I > CG.dom_hi[0] || J < CG.dom_lo[1] || J > CG.dom_hi[1] || K < CG.dom_lo[2] ||
         K > CG.dom_hi[2])
         return false;
-    const long nx = CG.dom_hi[0] - CG.dom_lo[0] + 1, ny = CG.dom_hi[1] - CG.dom_lo[1] + 1;
-    const int m = cmap[((long)(K - CG.dom_lo[2]) * ny + (J - CG.dom_lo[1])) * nx + (I - CG.dom_lo[0])];
+    const long ntile = (long)CG.map_n[0] * CG.map_n[1] * CG.map_n[2];
+    const int m = tile_owner(cmap, cmap + ntile, CG.map_n, CG.dom_lo, CG.map_shift, cboxes, I, J, K, 3);
     if (m < 0) return false;
     const BoxDesc Bc = cboxes[m];
     const double* u = C + n * Bc.nptg;
@@ -318,7 +312,7 @@ __global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fbo
         if (src[0] < G.dom_lo[0] || src[0] > G.dom_hi[0] || src[1] < G.dom_lo[1] || src[1] > G.dom_hi[1] ||
             src[2] < G.dom_lo[2] || src[2] > G.dom_hi[2])
             continue;
-        if (owner(G, src[0], src[1], src[2]) >= 0) continue;   // the level fill copied it
+        if (level_owner(G, fboxes, src[0], src[1], src[2]) >= 0) continue;   // the level fill copied it
         double v;
         if (cons_interp(cboxes, *cgeom, cmap, Sc, src[0], src[1], src[2], n, v)) Sf[dst + n * B.nptg] = sign * v;
     }
@@ -338,7 +332,7 @@ __global__ __launch_bounds__(256) void k_interp_valid(const BoxDesc* __restrict_
         int i, j, k;
         valid_cell(B, t.y + c, i, j, k);
         if (ogeom) {
-            const int m = owner(*ogeom, i, j, k);
+            const int m = level_owner(*ogeom, oboxes, i, j, k);
             if (m >= 0) {
                 const BoxDesc& O = oboxes[m];
                 if (i >= O.lo[0] && i <= O.hi[0] && j >= O.lo[1] && j <= O.hi[1] && k >= O.lo[2] && k <= O.hi[2])
@@ -364,7 +358,7 @@ __global__ __launch_bounds__(256) void k_copy_level(const BoxDesc* __restrict__ 
     for (int c = threadIdx.x; c < t.z; c += 256) {
         int i, j, k;
         valid_cell(B, t.y + c, i, j, k);
-        const int m = owner(*ogeom, i, j, k);
+        const int m = level_owner(*ogeom, oboxes, i, j, k);
         if (m < 0) continue;
         const BoxDesc O = oboxes[m];
         if (i < O.lo[0] || i > O.hi[0] || j < O.lo[1] || j > O.hi[1] || k < O.lo[2] || k > O.hi[2]) continue;
@@ -424,8 +418,9 @@ struct ocfd_cf_s {
     double* d_regf = nullptr;    // 5 per fine sub-face
     int4* d_tile_avg = nullptr;  // coarsened fine boxes, 256 coarse cells per tile
     int ntile_avg = 0;
-    int* d_cmap = nullptr;       // coarse cell -> box to interpolate from (owner, else a box whose
-                                 // grown-by-3 region holds it), -1 none; over the coarse domain
+    int* d_cmap = nullptr;       // coarse cell -> box to interpolate from (owner, else the first box
+                                 // whose grown-by-3 region holds it): a tile map of the coarse
+                                 // level's tiles, its box lists following the tiles
 };
 
 namespace {
@@ -435,12 +430,8 @@ int find_box(const LevelDev& L, int i, int j, int k)
     const GeomDesc& G = L.geom;
     if (i < G.dom_lo[0] || i > G.dom_hi[0] || j < G.dom_lo[1] || j > G.dom_hi[1] || k < G.dom_lo[2] || k > G.dom_hi[2])
         return -1;
-    const int m = L.owner_map[((size_t)((k - G.dom_lo[2]) >> G.map_shift) * G.map_n[1] +
-                               ((j - G.dom_lo[1]) >> G.map_shift)) * G.map_n[0] + ((i - G.dom_lo[0]) >> G.map_shift)];
-    if (m < 0) return -1;
-    const BoxDesc& B = L.boxes[m];
-    if (i < B.lo[0] || i > B.hi[0] || j < B.lo[1] || j > B.hi[1] || k < B.lo[2] || k > B.hi[2]) return -1;
-    return m;
+    return tile_owner(L.owner_map.data(), L.owner_list.data(), G.map_n, G.dom_lo, G.map_shift, L.boxes.data(), i, j,
+                      k, 0);
 }
 
 int wrap(const GeomDesc& G, int d, int v)
@@ -604,25 +595,10 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
     // coarse cell -> box to interpolate from: the owner, else the first box
     // whose grown-by-3 region holds the cell (its 27 neighbours then lie in
     // that box's filled ghost region)
-    std::vector<int> cmap((size_t)(n0 * n1 * n2), -1);
-    for (int b = 0; b < C.nbox; ++b) {
-        const BoxDesc& B = C.boxes[b];
-        for (int K = B.lo[2]; K <= B.hi[2]; ++K)
-            for (int J = B.lo[1]; J <= B.hi[1]; ++J)
-                for (int I = B.lo[0]; I <= B.hi[0]; ++I) cmap[lin(I, J, K)] = b;
-    }
-    for (int b = 0; b < C.nbox; ++b) {
-        const BoxDesc& B = C.boxes[b];
-        int lo3[3], hi3[3];
-        for (int d = 0; d < 3; ++d) {
-            lo3[d] = std::max(B.lo[d] - 3, CG.dom_lo[d]);
-            hi3[d] = std::min(B.hi[d] + 3, CG.dom_hi[d]);
-        }
-        for (int K = lo3[2]; K <= hi3[2]; ++K)
-            for (int J = lo3[1]; J <= hi3[1]; ++J)
-                for (int I = lo3[0]; I <= hi3[0]; ++I)
-                    if (cmap[lin(I, J, K)] < 0) cmap[lin(I, J, K)] = b;
-    }
+    std::vector<int> cmap, clist;
+    int cmn[3];
+    build_tile_map(CG.dom_lo, CG.dom_hi, CG.map_shift, C.boxes, 3, cmn, cmap, clist);
+    cmap.insert(cmap.end(), clist.begin(), clist.end());
     // the fine level must lie in the coarse level's valid cells
     for (const BoxDesc& B : F.boxes)
         for (int K = B.lo[2] >> 1; K <= B.hi[2] >> 1; ++K)

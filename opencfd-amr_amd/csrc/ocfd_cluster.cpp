@@ -300,3 +300,71 @@ extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigne
         }
     return OCFD_OK;
 }
+
+// Dense mask grown by n cells in every direction (Chebyshev distance, the
+// TagBoxArray::buffer of amr.n_error_buf), or shrunk by n cells (the proper
+// nesting domain of a level: cells whose n-neighbourhood the level covers).
+// Separable: one windowed OR per axis.  Periodic axes wrap; outside the
+// domain elsewhere counts as unset when growing and as set when shrinking.
+extern "C" int ocfd_mask_grow(unsigned char* m, int nx, int ny, int nz, int n, const int* periodic, int shrink)
+{
+    if (!m || nx <= 0 || ny <= 0 || nz <= 0 || n < 0 || !periodic) {
+        ocfd::set_error("ocfd_mask_grow: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    if (n == 0) return OCFD_OK;
+    const size_t plane = (size_t)nx * ny, vol = plane * nz;
+    if (shrink)
+        for (size_t c = 0; c < vol; ++c) m[c] = !m[c];
+    else
+        for (size_t c = 0; c < vol; ++c) m[c] = m[c] != 0;
+    // window [i-n, i+n] of one axis of length len, lines `stride` apart
+    // (element step `step` along the axis), for `nl` lines at bases base(l)
+    std::vector<unsigned char> tmp;
+    auto pass = [&](int len, bool per, size_t step, auto&& line_base, size_t nl, size_t line_len) {
+        // line_len elements move together (a row or a plane): OR whole rows
+        tmp.resize((size_t)len * line_len);
+        for (size_t l = 0; l < nl; ++l) {
+            unsigned char* b = m + line_base(l);
+            for (int a = 0; a < len; ++a) std::copy(b + a * step, b + a * step + line_len, tmp.data() + (size_t)a * line_len);
+            for (int a = 0; a < len; ++a) {
+                unsigned char* o = b + a * step;
+                for (int s = -n; s <= n; ++s) {
+                    if (s == 0) continue;
+                    int q = a + s;
+                    if (q < 0 || q >= len) {
+                        if (!per || len == 1) continue;
+                        q = ((q % len) + len) % len;
+                    }
+                    const unsigned char* src = tmp.data() + (size_t)q * line_len;
+                    for (size_t e = 0; e < line_len; ++e) o[e] |= src[e];
+                }
+            }
+        }
+    };
+    // x: each row on its own, as shifted ORs of the row's copy
+    {
+        std::vector<unsigned char> row((size_t)nx);
+        const bool per = periodic[0] != 0 && nx > 1;
+        for (size_t l = 0; l < (size_t)ny * nz; ++l) {
+            unsigned char* o = m + l * nx;
+            std::copy(o, o + nx, row.data());
+            for (int s = -n; s <= n; ++s) {
+                if (s == 0) continue;
+                // o[i] |= row[i + s] where i + s lies inside the row
+                const int a0 = std::max(0, -s), a1 = std::min(nx, nx - s);
+                for (int i = a0; i < a1; ++i) o[i] |= row[i + s];
+                if (per)
+                    for (int i = 0; i < nx; ++i)
+                        if (i + s < 0 || i + s >= nx) o[i] |= row[(((i + s) % nx) + nx) % nx];
+            }
+        }
+    }
+    // y: rows of a plane move together
+    pass(ny, periodic[1] != 0, nx, [&](size_t l) { return l * plane; }, (size_t)nz, (size_t)nx);
+    // z: whole planes move together
+    pass(nz, periodic[2] != 0, plane, [&](size_t) { return (size_t)0; }, 1, plane);
+    if (shrink)
+        for (size_t c = 0; c < vol; ++c) m[c] = !m[c];
+    return OCFD_OK;
+}

@@ -46,10 +46,41 @@ struct GeomDesc {
     int probtype;
     double prob_lo[3], dx[3];
     double left[NC], right[NC];
-    int map_shift;              // log2 of the owner-map granularity
-    int map_n[3];
-    const int* map;             // coarse cell -> box index, -1 if not covered
+    int map_shift;              // log2 of the owner-map tile edge
+    int map_n[3];               // owner-map tiles per direction
+    const int* map;             // tile -> the box wholly covering it (>= 0), -1 none, <= -2 a list:
+    const int* map_list;        //   at -(v+2): count, then the boxes meeting the tile, ascending
 };
+
+// Box of a tile map holding cell (i,j,k) of the domain: the box whose valid
+// region contains it, else (grow > 0) the first box whose region grown by
+// `grow` contains it; -1 if none.  Host and device.
+__host__ __device__ inline int tile_owner(const int* map, const int* list, const int* mn, const int* dlo, int shift,
+                                          const BoxDesc* boxes, int i, int j, int k, int grow)
+{
+    const int v = map[((long)((k - dlo[2]) >> shift) * mn[1] + ((j - dlo[1]) >> shift)) * mn[0] +
+                      ((i - dlo[0]) >> shift)];
+    if (v >= -1) return v;
+    const int* l = list + (-v - 2);
+    for (int q = 1; q <= l[0]; ++q) {
+        const BoxDesc& B = boxes[l[q]];
+        if (i >= B.lo[0] && i <= B.hi[0] && j >= B.lo[1] && j <= B.hi[1] && k >= B.lo[2] && k <= B.hi[2]) return l[q];
+    }
+    if (grow > 0)
+        for (int q = 1; q <= l[0]; ++q) {
+            const BoxDesc& B = boxes[l[q]];
+            if (i >= B.lo[0] - grow && i <= B.hi[0] + grow && j >= B.lo[1] - grow && j <= B.hi[1] + grow &&
+                k >= B.lo[2] - grow && k <= B.hi[2] + grow)
+                return l[q];
+        }
+    return -1;
+}
+
+// owner box of a cell inside the level's domain, -1 if not covered
+__host__ __device__ inline int level_owner(const GeomDesc& G, const BoxDesc* boxes, int i, int j, int k)
+{
+    return tile_owner(G.map, G.map_list, G.map_n, G.dom_lo, G.map_shift, boxes, i, j, k, 0);
+}
 
 // Parm members the kernels use, by value in the launch arguments
 struct PhysArgs {
@@ -185,6 +216,27 @@ __device__ __forceinline__ void c2vel(const double* __restrict__ s, long idx, lo
     u = (FT)(s[idx + stride] * rhoinv);
     v = (FT)(s[idx + 2 * stride] * rhoinv);
     w = (FT)(s[idx + 3 * stride] * rhoinv);
+}
+
+// Velocity of a neighbouring cell for the tangential derivatives of a
+// viscous face: exact (as c2vel) in the FP64 form; in the FP32-flux form,
+// where the face flux is evaluated in FP32, from the FP32-rounded momenta and
+// the hardware reciprocal of the density (a few FP32 ulps).
+template <class FT>
+__device__ __forceinline__ void c2vel_nb(const double* __restrict__ s, long idx, long stride, const PhysArgs& ph,
+                                         FT& u, FT& v, FT& w)
+{
+    c2vel<FT>(s, idx, stride, ph, u, v, w);
+}
+
+template <>
+__device__ __forceinline__ void c2vel_nb<float>(const double* __restrict__ s, long idx, long stride,
+                                                const PhysArgs& ph, float& u, float& v, float& w)
+{
+    const float ri = __builtin_amdgcn_rcpf((float)fmax(s[idx], ph.smallr));
+    u = (float)s[idx + stride] * ri;
+    v = (float)s[idx + 2 * stride] * ri;
+    w = (float)s[idx + 3 * stride] * ri;
 }
 
 // Steger-Warming split in direction D, Source/kernel/FluxSplit.H:11-193.

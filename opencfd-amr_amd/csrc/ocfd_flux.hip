@@ -73,13 +73,13 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
                 // (their primitives are being written by other waves)
                 sv[w][0][lane] = q.T; sv[w][1][lane] = q.u; sv[w][2][lane] = q.v; sv[w][3][lane] = q.w;
                 FT u, v, ww;
-                c2vel<FT>(a.scur, s0 + B.nxg, st, a.ph, u, v, ww);               // (i, j+1)
+                c2vel_nb<FT>(a.scur, s0 + B.nxg, st, a.ph, u, v, ww);               // (i, j+1)
                 sv[w][4][lane] = u; sv[w][5][lane] = v;
-                c2vel<FT>(a.scur, s0 - B.nxg, st, a.ph, u, v, ww);               // (i, j-1)
+                c2vel_nb<FT>(a.scur, s0 - B.nxg, st, a.ph, u, v, ww);               // (i, j-1)
                 sv[w][6][lane] = u; sv[w][7][lane] = v;
-                c2vel<FT>(a.scur, s0 + (long)B.nxg * B.nyg, st, a.ph, u, v, ww);  // (i, k+1)
+                c2vel_nb<FT>(a.scur, s0 + (long)B.nxg * B.nyg, st, a.ph, u, v, ww);  // (i, k+1)
                 sv[w][8][lane] = u; sv[w][9][lane] = ww;
-                c2vel<FT>(a.scur, s0 - (long)B.nxg * B.nyg, st, a.ph, u, v, ww);  // (i, k-1)
+                c2vel_nb<FT>(a.scur, s0 - (long)B.nxg * B.nyg, st, a.ph, u, v, ww);  // (i, k-1)
                 sv[w][10][lane] = u; sv[w][11][lane] = ww;
             }
         }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <string>
 #include <vector>
 #include "ocfd_device.hpp"
@@ -13,10 +14,12 @@ struct LevelDev {
     int nbox = 0;
     std::vector<BoxDesc> boxes;     // host copy
     GeomDesc geom{};                // host copy (map pointer is the device map)
-    std::vector<int> owner_map;     // host copy of the owner map
+    std::vector<int> owner_map;     // host copy of the owner map (tiles of 8^3 cells)
+    std::vector<int> owner_list;    // host copy of its box lists
     BoxDesc* d_boxes = nullptr;
     GeomDesc* d_geom = nullptr;
     int* d_map = nullptr;
+    int* d_map_list = nullptr;
     int4* d_tile_x = nullptr; int ntile_x = 0;     // x sweep: (box, segment, row group, z growth 3)
     int4* d_tile_xp = nullptr; int ntile_xp = 0;   // planar x sweep (z growth 1)
     int4* d_tile_y = nullptr; int ntile_y = 0;
@@ -55,6 +58,77 @@ struct LevelDev {
 };
 
 void launch_eb(const LevelDev& L, const StageArgs& a, hipStream_t s);
+
+constexpr int kMapShift = 3;   // owner-map tiles of 8^3 cells
+
+// Tile map of a box list over the domain [dlo, dhi] (tiles of 2^shift cells,
+// the lookup of tile_owner): a tile that one box (grown by `grow`, clipped to
+// the domain) wholly covers and no other box meets maps to that box, an empty
+// tile to -1, any other to a list of the boxes meeting it in ascending order.
+// Returns false when two boxes overlap (checked for grow 0).
+inline bool build_tile_map(const int* dlo, const int* dhi, int shift, const std::vector<BoxDesc>& boxes, int grow,
+                           int* mn, std::vector<int>& map, std::vector<int>& list)
+{
+    for (int d = 0; d < 3; ++d) mn[d] = ((dhi[d] - dlo[d] + 1) + (1 << shift) - 1) >> shift;
+    const size_t nt = (size_t)mn[0] * mn[1] * mn[2];
+    auto range = [&](const BoxDesc& B, int* t0, int* t1, int* blo, int* bhi) {
+        for (int d = 0; d < 3; ++d) {
+            blo[d] = std::max(B.lo[d] - grow, dlo[d]);
+            bhi[d] = std::min(B.hi[d] + grow, dhi[d]);
+            t0[d] = (blo[d] - dlo[d]) >> shift;
+            t1[d] = (bhi[d] - dlo[d]) >> shift;
+        }
+    };
+    std::vector<int> cnt(nt + 1, 0);
+    for (const BoxDesc& B : boxes) {
+        int t0[3], t1[3], blo[3], bhi[3];
+        range(B, t0, t1, blo, bhi);
+        for (int k = t0[2]; k <= t1[2]; ++k)
+            for (int j = t0[1]; j <= t1[1]; ++j)
+                for (int i = t0[0]; i <= t1[0]; ++i) ++cnt[((size_t)k * mn[1] + j) * mn[0] + i];
+    }
+    std::vector<int> off(nt + 1, 0);
+    for (size_t t = 0; t < nt; ++t) off[t + 1] = off[t] + cnt[t];
+    std::vector<int> ent(off[nt]);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int b = 0; b < (int)boxes.size(); ++b) {
+        int t0[3], t1[3], blo[3], bhi[3];
+        range(boxes[b], t0, t1, blo, bhi);
+        for (int k = t0[2]; k <= t1[2]; ++k)
+            for (int j = t0[1]; j <= t1[1]; ++j)
+                for (int i = t0[0]; i <= t1[0]; ++i) ent[fill[((size_t)k * mn[1] + j) * mn[0] + i]++] = b;
+    }
+    map.assign(nt, -1);
+    list.clear();
+    for (size_t t = 0; t < nt; ++t) {
+        const int c = cnt[t];
+        if (c == 0) continue;
+        const int ti[3] = {(int)(t % mn[0]), (int)((t / mn[0]) % mn[1]), (int)(t / ((size_t)mn[0] * mn[1]))};
+        if (c == 1) {
+            int t0[3], t1[3], blo[3], bhi[3];
+            range(boxes[ent[off[t]]], t0, t1, blo, bhi);
+            bool whole = true;
+            for (int d = 0; d < 3; ++d) {
+                const int tlo = dlo[d] + (ti[d] << shift), thi = std::min(tlo + (1 << shift) - 1, dhi[d]);
+                whole = whole && blo[d] <= tlo && bhi[d] >= thi;
+            }
+            if (whole) { map[t] = ent[off[t]]; continue; }
+        } else if (grow == 0) {
+            for (int a = off[t]; a < off[t + 1]; ++a)
+                for (int b = a + 1; b < off[t + 1]; ++b) {
+                    const BoxDesc& A = boxes[ent[a]];
+                    const BoxDesc& C = boxes[ent[b]];
+                    bool meet = true;
+                    for (int d = 0; d < 3; ++d) meet = meet && A.lo[d] <= C.hi[d] && C.lo[d] <= A.hi[d];
+                    if (meet) return false;
+                }
+        }
+        map[t] = -(int)list.size() - 2;
+        list.push_back(c);
+        list.insert(list.end(), ent.begin() + off[t], ent.begin() + off[t + 1]);
+    }
+    return true;
+}
 
 inline PhysArgs phys_of(const ocfd_parm& p)
 {

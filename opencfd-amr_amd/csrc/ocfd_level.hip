@@ -93,10 +93,7 @@ __global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes,
         if (src[0] < G.dom_lo[0] || src[0] > G.dom_hi[0] || src[1] < G.dom_lo[1] || src[1] > G.dom_hi[1] ||
             src[2] < G.dom_lo[2] || src[2] > G.dom_hi[2])
             continue;
-        const int mi = (src[0] - G.dom_lo[0]) >> G.map_shift;
-        const int mj = (src[1] - G.dom_lo[1]) >> G.map_shift;
-        const int mk = (src[2] - G.dom_lo[2]) >> G.map_shift;
-        const int m = G.map[((long)mk * G.map_n[1] + mj) * G.map_n[0] + mi];
+        const int m = level_owner(G, boxes, src[0], src[1], src[2]);
         if (m < 0) continue;   // not covered on this level (coarse/fine boundary)
         const BoxDesc& O = boxes[m];
         S[dst + n * B.nptg] = sign * S[sidx(O, src[0], src[1], src[2]) + n * O.nptg];
@@ -250,7 +247,7 @@ static int upload(T** dptr, const std::vector<T>& h)
 
 static void free_level(LevelDev& L)
 {
-    hipFree(L.d_boxes); hipFree(L.d_geom); hipFree(L.d_map);
+    hipFree(L.d_boxes); hipFree(L.d_geom); hipFree(L.d_map); hipFree(L.d_map_list);
     hipFree(L.d_tile_x); hipFree(L.d_tile_xp); hipFree(L.d_tile_y); hipFree(L.d_tile_z); hipFree(L.d_tile_g); hipFree(L.d_tile_c);
     hipFree(L.d_acc); hipFree(L.d_red); hipFree(L.d_tile_p); hipFree(L.d_tile_pp);
     hipFree(L.d_prim32); hipFree(L.d_prim64); hipFree(L.d_tile_v);
@@ -428,27 +425,12 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     G.probtype = g->probtype;
     for (int n = 0; n < NC; ++n) { G.left[n] = g->left_state[n]; G.right[n] = g->right_state[n]; }
 
-    // owner map at the coarsest granularity every box boundary is aligned to
-    int gran = 8;
-    for (int b = 0; b < nbox; ++b)
-        for (int d = 0; d < 3; ++d) {
-            while (gran > 1 && (((L.boxes[b].lo[d] - G.dom_lo[d]) % gran) || ((L.boxes[b].hi[d] + 1 - G.dom_lo[d]) % gran)))
-                gran >>= 1;
-        }
-    int sh = 0;
-    while ((1 << sh) < gran) ++sh;
-    G.map_shift = sh;
-    for (int d = 0; d < 3; ++d) G.map_n[d] = ((G.dom_hi[d] - G.dom_lo[d] + 1) + gran - 1) / gran;
-    L.owner_map.assign((size_t)G.map_n[0] * G.map_n[1] * G.map_n[2], -1);
-    for (int b = 0; b < nbox; ++b) {
-        const BoxDesc& B = L.boxes[b];
-        for (int k = (B.lo[2] - G.dom_lo[2]) >> sh; k <= (B.hi[2] - G.dom_lo[2]) >> sh; ++k)
-            for (int j = (B.lo[1] - G.dom_lo[1]) >> sh; j <= (B.hi[1] - G.dom_lo[1]) >> sh; ++j)
-                for (int i = (B.lo[0] - G.dom_lo[0]) >> sh; i <= (B.hi[0] - G.dom_lo[0]) >> sh; ++i) {
-                    int& m = L.owner_map[((size_t)k * G.map_n[1] + j) * G.map_n[0] + i];
-                    if (m >= 0) { set_error("ocfd_level_create: overlapping boxes"); delete lev; return OCFD_EINVAL; }
-                    m = b;
-                }
+    // owner map: tiles of 8^3 cells, with box lists where boxes share a tile
+    G.map_shift = kMapShift;
+    if (!build_tile_map(G.dom_lo, G.dom_hi, kMapShift, L.boxes, 0, G.map_n, L.owner_map, L.owner_list)) {
+        set_error("ocfd_level_create: overlapping boxes");
+        delete lev;
+        return OCFD_EINVAL;
     }
 
     // work-lists
@@ -515,7 +497,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     L.march_ch = ch;
     march_tiles(L, ch, ty, tz);
     int rc = OCFD_OK;
-    if ((rc = upload(&L.d_map, L.owner_map)) || (rc = upload(&L.d_tile_x, tx)) || (rc = upload(&L.d_tile_xp, txp)) || (rc = upload(&L.d_tile_y, ty)) ||
+    if ((rc = upload(&L.d_map, L.owner_map)) || (rc = upload(&L.d_map_list, L.owner_list)) || (rc = upload(&L.d_tile_x, tx)) || (rc = upload(&L.d_tile_xp, txp)) || (rc = upload(&L.d_tile_y, ty)) ||
         (rc = upload(&L.d_tile_z, tz)) || (rc = upload(&L.d_tile_g, tg)) || (rc = upload(&L.d_tile_c, tc)) ||
         (rc = upload(&L.d_tile_p, tp)) || (rc = upload(&L.d_tile_pp, tpp)) || (rc = upload(&L.d_tile_v, tv)) ||
         (rc = upload(&L.d_tile_eq, teq)) || (rc = upload(&L.d_tile_ef[0], tef[0])) ||
@@ -538,6 +520,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     L.ntile_p = (int)tp.size(); L.ntile_pp = (int)tpp.size(); L.ntile_v = (int)tv.size();
     L.mf_prim = L.mf_state / NC * 6;
     G.map = L.d_map;
+    G.map_list = L.d_map_list;
     std::vector<GeomDesc> gv(1, G);
     if ((rc = upload(&L.d_geom, gv)) || (rc = upload(&L.d_boxes, L.boxes))) { free_level(L); delete lev; return rc; }
     if (hipMalloc((void**)&L.d_acc, sizeof(double) * L.mf_valid) != hipSuccess ||
@@ -547,7 +530,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         set_error("ocfd_level_create: out of device memory");
         free_level(L); delete lev; return OCFD_ENOMEM;
     }
-    L.dev_bytes = (long)sizeof(int) * (long)L.owner_map.size() +
+    L.dev_bytes = (long)sizeof(int) * (long)(L.owner_map.size() + L.owner_list.size()) +
                   (long)sizeof(int4) * (L.ntile_x + L.ntile_xp + L.ntile_y + L.ntile_z + L.ntile_g + L.ntile_c + L.ntile_p +
                                         L.ntile_pp + L.ntile_v) +
                   (long)sizeof(GeomDesc) + (long)sizeof(BoxDesc) * L.nbox + (long)sizeof(double) * L.mf_valid +

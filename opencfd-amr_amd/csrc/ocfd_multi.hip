@@ -495,10 +495,7 @@ __global__ __launch_bounds__(256) void k_fill_spec(const BoxDesc* __restrict__ b
     if (src[0] < G.dom_lo[0] || src[0] > G.dom_hi[0] || src[1] < G.dom_lo[1] || src[1] > G.dom_hi[1] ||
         src[2] < G.dom_lo[2] || src[2] > G.dom_hi[2])
         return;
-    const int mi = (src[0] - G.dom_lo[0]) >> G.map_shift;
-    const int mj = (src[1] - G.dom_lo[1]) >> G.map_shift;
-    const int mk = (src[2] - G.dom_lo[2]) >> G.map_shift;
-    const int o = G.map[((long)mk * G.map_n[1] + mj) * G.map_n[0] + mi];
+    const int o = level_owner(G, boxes, src[0], src[1], src[2]);
     if (o < 0) return;
     const BoxDesc& O = boxes[o];
     const long s0 = O.off_s / NC * NS + gl(O, src[0], src[1], src[2]);

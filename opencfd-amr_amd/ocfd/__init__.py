@@ -158,6 +158,8 @@ def lib():
         L.ocfd_level_autotune.argtypes = [vp, vp, ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, ip, dp, vp]
         L.ocfd_cluster_tags.argtypes = [ip, ip, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte),
                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ip, ip]
+        L.ocfd_mask_grow.argtypes = [ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ip, ctypes.c_int]
         L.ocfd_lincomb.argtypes = [ctypes.c_long, ctypes.c_double, vp, ctypes.c_double, vp, vp, vp]
         L.ocfd_cf_fill_fine.argtypes = [vp, vp, vp, vp]
         L.ocfd_cf_interp_fine.argtypes = [vp, vp, vp, vp, vp]

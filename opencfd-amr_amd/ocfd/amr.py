@@ -106,27 +106,20 @@ def copy_level(dst, S_dst, src, S_src, stream=None):
     check(lib().ocfd_copy_level(dst.handle, _ptr(S_dst), src.handle, _ptr(S_src), _stream(stream)), "ocfd_copy_level")
 
 
+def _grow(mask, n, periodic, shrink=False):
+    """mask grown (or shrunk) by n cells, Chebyshev distance; periodic axes
+    wrap, other sides see unset (grow) or set (shrink) cells."""
+    a = np.ascontiguousarray(mask, dtype=np.uint8).copy()
+    if n > 0:
+        nz, ny, nx = a.shape
+        per = (ctypes.c_int * 3)(*[int(bool(p)) for p in periodic])
+        check(lib().ocfd_mask_grow(a.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), nx, ny, nz, n, per,
+                                   int(shrink)), "ocfd_mask_grow")
+    return a.view(bool)
+
+
 def _dilate(mask, n, periodic):
-    """mask grown by n cells (Chebyshev; separable into one pass per axis);
-    periodic axes wrap, other sides see False."""
-    a = mask.copy()
-    for ax, d in ((0, 2), (1, 1), (2, 0)):
-        m = a.shape[ax]
-        for _ in range(n):
-            b = a.copy()
-            lo = [slice(None)] * 3
-            hi = [slice(None)] * 3
-            lo[ax], hi[ax] = slice(0, m - 1), slice(1, m)
-            b[tuple(hi)] |= a[tuple(lo)]
-            b[tuple(lo)] |= a[tuple(hi)]
-            if periodic[d] and m > 1:
-                first = [slice(None)] * 3
-                last = [slice(None)] * 3
-                first[ax], last[ax] = slice(0, 1), slice(m - 1, m)
-                b[tuple(first)] |= a[tuple(last)]
-                b[tuple(last)] |= a[tuple(first)]
-            a = b
-    return a
+    return _grow(mask, n, periodic)
 
 
 def coverage(boxes, geom):
@@ -143,7 +136,43 @@ def proper_nesting(boxes, geom, n_proper):
     """Cells of a level whose n_proper-neighbourhood (inside the domain, periodic
     images included) is covered by the level: where the next finer level may go."""
     per = [geom.periodic[d] for d in range(3)]
-    return ~_dilate(~coverage(boxes, geom), n_proper, per)
+    return _grow(coverage(boxes, geom), n_proper, per, shrink=True)
+
+
+def projected_coverage(boxes_f2, geom_f, geom_c, n_proper):
+    """Cells of level c (2 coarser than the boxes' level f+1) under the grids
+    of level f+1 coarsened to level f, grown by n_proper there (periodic images
+    included) and coarsened again: the tags that keep level f's new grids
+    around the finer level's (Amr::grid_places, projection of new grids).
+    Growing and coarsening distribute over a union of boxes, so this is box
+    arithmetic and one coverage fill."""
+    lo_f = [geom_f.dom_lo[d] for d in range(3)]
+    n_f = [geom_f.dom_hi[d] - lo_f[d] + 1 for d in range(3)]
+    out = []
+    for blo, bhi in coarsen_boxes(boxes_f2):
+        glo = [blo[d] - n_proper for d in range(3)]
+        ghi = [bhi[d] + n_proper for d in range(3)]
+        # periodic images of the parts that leave the domain; elsewhere clip
+        parts = [(glo, ghi)]
+        for d in range(3):
+            nxt = []
+            for plo, phi in parts:
+                if geom_f.periodic[d]:
+                    for sh in (-n_f[d], 0, n_f[d]):
+                        a, b = max(plo[d] + sh, lo_f[d]), min(phi[d] + sh, lo_f[d] + n_f[d] - 1)
+                        if a <= b:
+                            q0, q1 = list(plo), list(phi)
+                            q0[d], q1[d] = a, b
+                            nxt.append((q0, q1))
+                else:
+                    a, b = max(plo[d], lo_f[d]), min(phi[d], lo_f[d] + n_f[d] - 1)
+                    if a <= b:
+                        q0, q1 = list(plo), list(phi)
+                        q0[d], q1[d] = a, b
+                        nxt.append((q0, q1))
+            parts = nxt
+        out.extend((tuple(p0), tuple(p1)) for p0, p1 in parts)
+    return coverage(coarsen_boxes(out), geom_c)
 
 
 def coarsen_boxes(boxes):
@@ -354,10 +383,7 @@ class Hierarchy:
             g = self.geoms[levc]
             tags = self._tags(levc, t)
             if levf + 1 in new and new[levf + 1]:
-                proj = coverage(coarsen_boxes(new[levf + 1]), self.geoms[levf])
-                proj = _dilate(proj, self.n_proper, [self.geoms[levf].periodic[d] for d in range(3)])
-                nz, ny, nx = proj.shape
-                tags |= proj.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).any(axis=(1, 3, 5))
+                tags |= projected_coverage(new[levf + 1], self.geoms[levf], g, self.n_proper)
             pn = proper_nesting(self.levels[levc].boxes, g, self.n_proper) if levc > 0 else None
             if pn is not None:
                 tags &= pn

@@ -316,3 +316,78 @@ def test_clustering_within_a_mask(block):
         cov = _covers(got, tags)
         assert cov.max() <= 1 and (cov[tags == 1] == 1).all()
         assert not (cov.astype(bool) & ~mask.astype(bool)).any()
+
+
+def _brute_grow(m, n, periodic):
+    """Chebyshev growth by n, one cell at a time (pure Python loops)."""
+    nz, ny, nx = m.shape
+    out = np.zeros_like(m)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                hit = False
+                for dk in range(-n, n + 1):
+                    for dj in range(-n, n + 1):
+                        for di in range(-n, n + 1):
+                            q = [i + di, j + dj, k + dk]
+                            ok = True
+                            for d, size in ((0, nx), (1, ny), (2, nz)):
+                                if 0 <= q[d] < size:
+                                    continue
+                                if periodic[d]:
+                                    q[d] %= size
+                                else:
+                                    ok = False
+                            if ok and m[q[2], q[1], q[0]]:
+                                hit = True
+                out[k, j, i] = hit
+    return out
+
+
+@pytest.mark.parametrize("shape,periodic,n", [((3, 4, 5), (0, 0, 0), 1), ((3, 4, 5), (1, 0, 1), 1),
+                                              ((2, 5, 6), (1, 1, 0), 2)])
+def test_oracle_dilation_matches_brute_force(shape, periodic, n):
+    rng = np.random.default_rng(sum(shape) + n)
+    m = rng.random(shape) < 0.15
+    assert np.array_equal(ao.dilate(m, n, periodic), _brute_grow(m, n, periodic))
+
+
+@pytest.mark.parametrize("periodic", [(0, 0, 0), (1, 0, 0), (0, 1, 1), (1, 1, 1)])
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_library_mask_grow_and_shrink_match_oracle(periodic, n):
+    """ocfd_mask_grow against the oracle's 27-shift dilation: growth of random
+    tags, and shrinking (the proper nesting domain) as the complement of the
+    growth of the complement."""
+    from ocfd import amr
+    rng = np.random.default_rng(n * 7 + sum(periodic))
+    for shape in ((4, 9, 11), (1, 6, 13), (7, 1, 5)):
+        m = rng.random(shape) < 0.2
+        assert np.array_equal(amr._grow(m, n, periodic), ao.dilate(m, n, periodic))
+        c = rng.random(shape) < 0.8
+        assert np.array_equal(amr._grow(c, n, periodic, shrink=True), ~ao.dilate(~c, n, periodic))
+
+
+def test_projected_coverage_matches_dense_projection():
+    """Grown-and-coarsened finer grids as box arithmetic equal the dense
+    route: coverage at level f, Chebyshev growth (periodic images), any-reduce
+    by the refinement ratio."""
+    import ocfd
+    from ocfd import amr
+    rng = np.random.default_rng(3)
+    for per in ((0, 0, 0), (1, 1, 0), (1, 1, 1)):
+        gc = ocfd.Geom()
+        for d in range(3):
+            gc.dom_lo[d], gc.periodic[d] = 0, per[d]
+        gc.dom_hi[0], gc.dom_hi[1], gc.dom_hi[2] = 15, 11, 7
+        gf = amr.refine_geom(gc)
+        gf2 = amr.refine_geom(gf)
+        for n_proper in (1, 2):
+            boxes = []
+            for _ in range(6):
+                lo = [int(rng.integers(0, (gf2.dom_hi[d] + 1) - 4)) for d in range(3)]
+                hi = [lo[d] + int(rng.integers(1, 4)) for d in range(3)]
+                boxes.append((tuple(lo), tuple(hi)))
+            want = ao.dilate(amr.coverage(amr.coarsen_boxes(boxes), gf), n_proper, per)
+            nz, ny, nx = want.shape
+            want = want.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).any(axis=(1, 3, 5))
+            assert np.array_equal(amr.projected_coverage(boxes, gf, gc, n_proper), want), (per, n_proper)
