@@ -21,6 +21,8 @@
 #include <string>
 #include <vector>
 
+#include <array>
+
 #include "ocfd_device.hpp"
 #include "ocfd_internal.hpp"
 
@@ -418,6 +420,7 @@ struct ocfd_cf_s {
     double* d_regf = nullptr;    // 5 per fine sub-face
     int4* d_tile_avg = nullptr;  // coarsened fine boxes, 256 coarse cells per tile
     int ntile_avg = 0;
+    void* d_arena = nullptr;     // one allocation holding the lists and the map below
     int* d_cmap = nullptr;       // coarse cell -> box to interpolate from (owner, else the first box
                                  // whose grown-by-3 region holds it): a tile map of the coarse
                                  // level's tiles, its box lists following the tiles
@@ -441,16 +444,6 @@ int wrap(const GeomDesc& G, int d, int v)
     return G.dom_lo[d] + (((v - G.dom_lo[d]) % len) + len) % len;
 }
 
-template <class T>
-int upload_vec(T** d, const std::vector<T>& h)
-{
-    *d = nullptr;
-    if (h.empty()) return OCFD_OK;
-    if (hipMalloc((void**)d, sizeof(T) * h.size()) != hipSuccess) return OCFD_ENOMEM;
-    if (hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice) != hipSuccess) return OCFD_EHIP;
-    return OCFD_OK;
-}
-
 int launched(const char* what)
 {
     hipError_t e = hipGetLastError();
@@ -463,8 +456,8 @@ int launched(const char* what)
 
 void free_cf(ocfd_cf c)
 {
-    hipFree(c->d_cf); hipFree(c->d_ff); hipFree(c->d_cells); hipFree(c->d_cellfaces);
-    hipFree(c->d_regc); hipFree(c->d_regf); hipFree(c->d_tile_avg); hipFree(c->d_cmap);
+    // the interface lists and the interpolation map live in c->d_arena
+    hipFree(c->d_arena); hipFree(c->d_regc); hipFree(c->d_regf);
 }
 
 bool refined_by_2(const LevelDev& C, const LevelDev& F)
@@ -504,19 +497,19 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
         set_error("ocfd_cf_create: the flux register covers regular levels only");
         return OCFD_EUNSUPPORTED;
     }
-    // coverage of the coarse domain by the fine level, and the uncovered coarse
-    // cells next to it (the only ones that can own an interface face)
+    // coverage of the coarse domain by the fine level (a tile map of the
+    // coarsened fine boxes), and the uncovered coarse cells next to it (the
+    // only ones that can own an interface face)
     const GeomDesc& CG = C.geom;
-    const long n0 = CG.dom_hi[0] - CG.dom_lo[0] + 1, n1 = CG.dom_hi[1] - CG.dom_lo[1] + 1,
-               n2 = CG.dom_hi[2] - CG.dom_lo[2] + 1;
-    auto lin = [&](int I, int J, int K) {
-        return ((long)(K - CG.dom_lo[2]) * n1 + (J - CG.dom_lo[1])) * n0 + (I - CG.dom_lo[0]);
-    };
-    std::vector<unsigned char> cov((size_t)(n0 * n1 * n2), 0), cand((size_t)(n0 * n1 * n2), 0);
-    for (const BoxDesc& B : F.boxes)
-        for (int K = B.lo[2] >> 1; K <= B.hi[2] >> 1; ++K)
-            for (int J = B.lo[1] >> 1; J <= B.hi[1] >> 1; ++J)
-                for (int I = B.lo[0] >> 1; I <= B.hi[0] >> 1; ++I) cov[lin(I, J, K)] = 1;
+    std::vector<BoxDesc> fcrs(F.boxes.size());
+    for (size_t b = 0; b < F.boxes.size(); ++b)
+        for (int d = 0; d < 3; ++d) {
+            fcrs[b].lo[d] = F.boxes[b].lo[d] >> 1;
+            fcrs[b].hi[d] = F.boxes[b].hi[d] >> 1;
+        }
+    std::vector<int> fmap, flist;
+    int fmn[3];
+    build_tile_map(CG.dom_lo, CG.dom_hi, CG.map_shift, fcrs, 0, fmn, fmap, flist);
     // 1 = covered by the fine level, 0 = not, -1 = outside a non-periodic domain side
     auto covered = [&](int I, int J, int K) {
         int iv[3] = {I, J, K};
@@ -524,67 +517,71 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
             iv[d] = wrap(CG, d, iv[d]);
             if (iv[d] < CG.dom_lo[d] || iv[d] > CG.dom_hi[d]) return -1;
         }
-        return (int)cov[lin(iv[0], iv[1], iv[2])];
+        return tile_owner(fmap.data(), flist.data(), fmn, CG.dom_lo, CG.map_shift, fcrs.data(), iv[0], iv[1], iv[2],
+                          0) >= 0 ? 1 : 0;
     };
-    for (const BoxDesc& B : F.boxes)
-        for (int K = (B.lo[2] >> 1) - 1; K <= (B.hi[2] >> 1) + 1; ++K)
-            for (int J = (B.lo[1] >> 1) - 1; J <= (B.hi[1] >> 1) + 1; ++J)
-                for (int I = (B.lo[0] >> 1) - 1; I <= (B.hi[0] >> 1) + 1; ++I) {
+    // candidates (coarse box, K, J, I), visited in coarse-box then k, j, i order
+    std::vector<std::array<int, 4>> cand;
+    for (const BoxDesc& B : fcrs)
+        for (int K = B.lo[2] - 1; K <= B.hi[2] + 1; ++K)
+            for (int J = B.lo[1] - 1; J <= B.hi[1] + 1; ++J)
+                for (int I = B.lo[0] - 1; I <= B.hi[0] + 1; ++I) {
+                    if (K >= B.lo[2] && K <= B.hi[2] && J >= B.lo[1] && J <= B.hi[1] && I > B.lo[0] && I < B.hi[0])
+                        I = B.hi[0];   // skip the box interior of this row
                     int iv[3] = {I, J, K};
                     bool in = true;
                     for (int d = 0; d < 3; ++d) {
                         iv[d] = wrap(CG, d, iv[d]);
                         in = in && iv[d] >= CG.dom_lo[d] && iv[d] <= CG.dom_hi[d];
                     }
-                    if (in && !cov[lin(iv[0], iv[1], iv[2])]) cand[lin(iv[0], iv[1], iv[2])] = 1;
+                    if (!in || covered(iv[0], iv[1], iv[2]) == 1) continue;
+                    const int cb = find_box(C, iv[0], iv[1], iv[2]);
+                    if (cb >= 0) cand.push_back({cb, iv[2], iv[1], iv[0]});
                 }
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     std::vector<CFFace> cf, ff;
     std::vector<int4> cells;
     std::vector<int> cellfaces;
-    for (int b = 0; b < C.nbox; ++b) {
-        const BoxDesc& B = C.boxes[b];
-        for (int K = B.lo[2]; K <= B.hi[2]; ++K)
-            for (int J = B.lo[1]; J <= B.hi[1]; ++J)
-                for (int I = B.lo[0]; I <= B.hi[0]; ++I) {
-                    if (!cand[lin(I, J, K)]) continue;
-                    int fidx[6] = {-1, -1, -1, -1, -1, -1};
-                    for (int d = 0; d < 3; ++d)
-                        for (int side = 0; side < 2; ++side) {
-                            int nb[3] = {I, J, K};
-                            nb[d] += side ? 1 : -1;
-                            if (covered(nb[0], nb[1], nb[2]) != 1) continue;
-                            CFFace f{};
-                            f.box = b; f.i = I; f.j = J; f.k = K; f.dir = d; f.up = side;
-                            fidx[2 * d + side] = (int)cf.size();
-                            cf.push_back(f);
-                            // the 4 fine cells across the face: the first fine layer of the neighbour
-                            int fc[3] = {2 * I, 2 * J, 2 * K};
-                            fc[d] = side ? 2 * nb[d] : 2 * nb[d] + 1;
-                            const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-                            for (int a2 = 0; a2 < 2; ++a2)
-                                for (int a1 = 0; a1 < 2; ++a1) {
-                                    int g[3] = {fc[0], fc[1], fc[2]};
-                                    g[t1] += a1;
-                                    g[t2] += a2;
-                                    for (int e = 0; e < 3; ++e) g[e] = wrap(F.geom, e, g[e]);
-                                    const int fb = find_box(F, g[0], g[1], g[2]);
-                                    if (fb < 0) {
-                                        set_error("ocfd_cf_create: a fine cell across an interface face has no box");
-                                        return OCFD_EINVAL;
-                                    }
-                                    CFFace h{};
-                                    h.box = fb; h.i = g[0]; h.j = g[1]; h.k = g[2]; h.dir = d;
-                                    h.up = side ? 0 : 1;   // fine level above the face: its low face
-                                    ff.push_back(h);
-                                }
+    for (const auto& q : cand) {
+        const int b = q[0], K = q[1], J = q[2], I = q[3];
+        int fidx[6] = {-1, -1, -1, -1, -1, -1};
+        for (int d = 0; d < 3; ++d)
+            for (int side = 0; side < 2; ++side) {
+                int nb[3] = {I, J, K};
+                nb[d] += side ? 1 : -1;
+                if (covered(nb[0], nb[1], nb[2]) != 1) continue;
+                CFFace f{};
+                f.box = b; f.i = I; f.j = J; f.k = K; f.dir = d; f.up = side;
+                fidx[2 * d + side] = (int)cf.size();
+                cf.push_back(f);
+                // the 4 fine cells across the face: the first fine layer of the neighbour
+                int fc[3] = {2 * I, 2 * J, 2 * K};
+                fc[d] = side ? 2 * nb[d] : 2 * nb[d] + 1;
+                const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+                for (int a2 = 0; a2 < 2; ++a2)
+                    for (int a1 = 0; a1 < 2; ++a1) {
+                        int g[3] = {fc[0], fc[1], fc[2]};
+                        g[t1] += a1;
+                        g[t2] += a2;
+                        for (int e = 0; e < 3; ++e) g[e] = wrap(F.geom, e, g[e]);
+                        const int fb = find_box(F, g[0], g[1], g[2]);
+                        if (fb < 0) {
+                            set_error("ocfd_cf_create: a fine cell across an interface face has no box");
+                            return OCFD_EINVAL;
                         }
-                    bool any = false;
-                    for (int m = 0; m < 6; ++m) any = any || fidx[m] >= 0;
-                    if (any) {
-                        cells.push_back(make_int4(b, I, J, K));
-                        cellfaces.insert(cellfaces.end(), fidx, fidx + 6);
+                        CFFace h{};
+                        h.box = fb; h.i = g[0]; h.j = g[1]; h.k = g[2]; h.dir = d;
+                        h.up = side ? 0 : 1;   // fine level above the face: its low face
+                        ff.push_back(h);
                     }
-                }
+            }
+        bool any = false;
+        for (int m = 0; m < 6; ++m) any = any || fidx[m] >= 0;
+        if (any) {
+            cells.push_back(make_int4(b, I, J, K));
+            cellfaces.insert(cellfaces.end(), fidx, fidx + 6);
+        }
     }
     std::vector<int4> ta;
     for (int b = 0; b < F.nbox; ++b) {
@@ -613,9 +610,10 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
     c->nface = (int)cf.size(); c->ncell = (int)cells.size();
     c->ntile_avg = (int)ta.size();
     int rc;
-    if ((rc = upload_vec(&c->d_cf, cf)) || (rc = upload_vec(&c->d_ff, ff)) || (rc = upload_vec(&c->d_cells, cells)) ||
-        (rc = upload_vec(&c->d_cellfaces, cellfaces)) || (rc = upload_vec(&c->d_tile_avg, ta)) ||
-        (rc = upload_vec(&c->d_cmap, cmap))) {
+    Staging stg;
+    stg.add(&c->d_cf, cf); stg.add(&c->d_ff, ff); stg.add(&c->d_cells, cells); stg.add(&c->d_cellfaces, cellfaces);
+    stg.add(&c->d_tile_avg, ta); stg.add(&c->d_cmap, cmap);
+    if ((rc = stg.commit(&c->d_arena))) {
         set_error("ocfd_cf_create: device allocation failed");
         free_cf(c);
         delete c;

@@ -11,9 +11,12 @@
 // (the BoxList::simplify step), and chopped to the maximum grid size
 // (balanced pieces, AMReX BoxList::maxSize).  Host code; no device work.
 #include <algorithm>
+#include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <list>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "ocfd.h"
@@ -128,38 +131,72 @@ Cluster chop(Cluster& c)
 }
 
 // join boxes that share their extent in two directions and touch in the
-// third, until no pair joins (list kept sorted by lower corner, z major);
-// the union is unchanged, the boxes get fewer and larger
+// third, until no pair joins: each pass sorts the list by lower corner (z
+// major, stable) and lets every box, in list order, absorb the boxes after it
+// that it can join, in list order, continuing after the last one absorbed.
+// A box after box a in the list has a lower corner no smaller than a's, so
+// the only boxes a can absorb start at (a.hi0+1, a.lo1, a.lo2),
+// (a.lo0, a.hi1+1, a.lo2) or (a.lo0, a.lo1, a.hi2+1): they are found through
+// an index by lower corner instead of a scan of the rest of the list.
+struct LoKey {
+    int v[3];
+    bool operator==(const LoKey& o) const { return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2]; }
+};
+struct LoHash {
+    size_t operator()(const LoKey& k) const
+    {
+        return ((size_t)(unsigned)k.v[0] * 0x9E3779B97F4A7C15ull) ^ ((size_t)(unsigned)k.v[1] * 0xC2B2AE3D27D4EB4Full) ^
+               ((size_t)(unsigned)k.v[2] * 0x165667B19E3779F9ull);
+    }
+};
+
 void merge(std::vector<IBox>& bx)
 {
     bool changed = true;
     while (changed) {
         changed = false;
-        std::sort(bx.begin(), bx.end(), [](const IBox& a, const IBox& b) {
+        std::stable_sort(bx.begin(), bx.end(), [](const IBox& a, const IBox& b) {
             if (a.lo[2] != b.lo[2]) return a.lo[2] < b.lo[2];
             if (a.lo[1] != b.lo[1]) return a.lo[1] < b.lo[1];
             return a.lo[0] < b.lo[0];
         });
-        for (size_t i = 0; i < bx.size(); ++i) {
-            size_t j = i + 1;
-            while (j < bx.size()) {
-                IBox& a = bx[i];
-                const IBox& b = bx[j];
-                int same = 0, d = -1;
-                for (int q = 0; q < 3; ++q) {
-                    if (a.lo[q] == b.lo[q] && a.hi[q] == b.hi[q]) ++same;
-                    else d = q;
+        const size_t n = bx.size();
+        std::unordered_map<LoKey, std::vector<size_t>, LoHash> by_lo;
+        by_lo.reserve(2 * n);
+        for (size_t i = 0; i < n; ++i) by_lo[LoKey{{bx[i].lo[0], bx[i].lo[1], bx[i].lo[2]}}].push_back(i);
+        std::vector<char> dead(n, 0);
+        for (size_t i = 0; i < n; ++i) {
+            if (dead[i]) continue;
+            IBox& a = bx[i];
+            size_t p = i;   // the scan has passed every index up to p
+            for (;;) {
+                size_t best = n;
+                for (int d = 0; d < 3; ++d) {
+                    LoKey key{{a.lo[0], a.lo[1], a.lo[2]}};
+                    key.v[d] = a.hi[d] + 1;
+                    auto it = by_lo.find(key);
+                    if (it == by_lo.end()) continue;
+                    const std::vector<size_t>& v = it->second;
+                    for (auto q = std::upper_bound(v.begin(), v.end(), p); q != v.end() && *q < best; ++q) {
+                        if (dead[*q]) continue;
+                        const IBox& b = bx[*q];
+                        bool same = true;
+                        for (int e = 0; e < 3; ++e)
+                            if (e != d) same = same && b.hi[e] == a.hi[e];
+                        if (same) { best = *q; break; }
+                    }
                 }
-                if (same == 2 && (b.lo[d] == a.hi[d] + 1 || a.lo[d] == b.hi[d] + 1)) {
-                    a.lo[d] = std::min(a.lo[d], b.lo[d]);
-                    a.hi[d] = std::max(a.hi[d], b.hi[d]);
-                    bx.erase(bx.begin() + j);
-                    changed = true;
-                    continue;
-                }
-                ++j;
+                if (best == n) break;
+                for (int d = 0; d < 3; ++d) a.hi[d] = std::max(a.hi[d], bx[best].hi[d]);
+                dead[best] = 1;
+                changed = true;
+                p = best;
             }
         }
+        size_t w = 0;
+        for (size_t i = 0; i < n; ++i)
+            if (!dead[i]) bx[w++] = bx[i];
+        bx.resize(w);
     }
 }
 
@@ -238,11 +275,22 @@ extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigne
     }
     Cluster all;
     {
-        size_t c = 0;
-        for (int k = 0; k < nb[2]; ++k)
-            for (int j = 0; j < nb[1]; ++j)
-                for (int i = 0; i < nb[0]; ++i, ++c)
-                    if (bt[c]) all.pts.push_back(Pt{{lo[0] / block + i, lo[1] / block + j, lo[2] / block + k}});
+        // tagged points, skipping untagged runs eight bytes at a time
+        const size_t nrow = (size_t)nb[1] * nb[2];
+        for (size_t r = 0; r < nrow; ++r) {
+            const unsigned char* row = bt + r * nb[0];
+            const int j = (int)(r % nb[1]), k = (int)(r / nb[1]);
+            int i = 0;
+            while (i < nb[0]) {
+                if (i + 8 <= nb[0]) {
+                    uint64_t w;
+                    std::memcpy(&w, row + i, 8);
+                    if (!w) { i += 8; continue; }
+                }
+                if (row[i]) all.pts.push_back(Pt{{lo[0] / block + i, lo[1] / block + j, lo[2] / block + k}});
+                ++i;
+            }
+        }
     }
     std::vector<IBox> boxes;
     if (!all.pts.empty()) {

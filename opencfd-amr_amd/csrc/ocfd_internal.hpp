@@ -3,11 +3,14 @@
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cstring>
 #include <string>
 #include <vector>
 #include "ocfd_device.hpp"
 
 namespace ocfd {
+
+void set_error(const std::string& msg);
 
 // A level: box list + geometry, with the device work-lists of every kernel.
 struct LevelDev {
@@ -16,6 +19,7 @@ struct LevelDev {
     GeomDesc geom{};                // host copy (map pointer is the device map)
     std::vector<int> owner_map;     // host copy of the owner map (tiles of 8^3 cells)
     std::vector<int> owner_list;    // host copy of its box lists
+    void* d_arena = nullptr;        // one allocation holding the read-only tables below
     BoxDesc* d_boxes = nullptr;
     GeomDesc* d_geom = nullptr;
     int* d_map = nullptr;
@@ -60,6 +64,40 @@ struct LevelDev {
 void launch_eb(const LevelDev& L, const StageArgs& a, hipStream_t s);
 
 constexpr int kMapShift = 3;   // owner-map tiles of 8^3 cells
+
+// Host tables gathered into one device allocation filled by one copy; each
+// table starts on a 256-byte boundary.  commit() sets every registered
+// pointer into the allocation (null for empty tables).
+struct Staging {
+    std::vector<char> host;
+    std::vector<std::pair<void**, size_t>> slots;
+    template <class T>
+    void add(T** slot, const std::vector<T>& v)
+    {
+        *slot = nullptr;
+        if (v.empty()) return;
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + sizeof(T) * v.size());
+        std::memcpy(host.data() + off, v.data(), sizeof(T) * v.size());
+        slots.push_back({(void**)slot, off});
+    }
+    int commit(void** base)
+    {
+        *base = nullptr;
+        if (host.empty()) return OCFD_OK;
+        if (hipMalloc(base, host.size()) != hipSuccess) {
+            *base = nullptr;
+            set_error("out of device memory for the level tables");
+            return OCFD_ENOMEM;
+        }
+        if (hipMemcpy(*base, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("copy of the level tables to the device failed");
+            return OCFD_EHIP;
+        }
+        for (auto& sl : slots) *sl.first = (char*)*base + sl.second;
+        return OCFD_OK;
+    }
+};
 
 // Tile map of a box list over the domain [dlo, dhi] (tiles of 2^shift cells,
 // the lookup of tile_owner): a tile that one box (grown by `grow`, clipped to
@@ -142,8 +180,6 @@ void launch_sweeps(const LevelDev& L, const StageArgs& a, hipStream_t s);
 void launch_sweep_one(const LevelDev& L, const StageArgs& a, int which, hipStream_t s);
 int tile_xseg();
 int tile_chunk();
-
-void set_error(const std::string& msg);
 
 } // namespace ocfd
 

@@ -245,15 +245,18 @@ static int upload(T** dptr, const std::vector<T>& h)
     return OCFD_OK;
 }
 
+// The level's read-only tables (owner map, work-lists, box descriptors, EB
+// geometry) share one device allocation (Staging): a level is rebuilt at
+// every regrid, and an allocation and a copy per table cost more than the
+// tables themselves.
 static void free_level(LevelDev& L)
 {
-    hipFree(L.d_boxes); hipFree(L.d_geom); hipFree(L.d_map); hipFree(L.d_map_list);
-    hipFree(L.d_tile_x); hipFree(L.d_tile_xp); hipFree(L.d_tile_y); hipFree(L.d_tile_z); hipFree(L.d_tile_g); hipFree(L.d_tile_c);
-    hipFree(L.d_acc); hipFree(L.d_red); hipFree(L.d_tile_p); hipFree(L.d_tile_pp);
-    hipFree(L.d_prim32); hipFree(L.d_prim64); hipFree(L.d_tile_v);
-    hipFree(L.d_ebflag); hipFree(L.d_ebgeo); hipFree(L.d_ebq); hipFree(L.d_tile_eq); hipFree(L.d_tile_ed);
-    for (int d = 0; d < 3; ++d) { hipFree(L.d_ebF[d]); hipFree(L.d_tile_ef[d]); hipFree(L.d_tile_mf[d]); hipFree(L.d_mF[d]); }
-    hipFree(L.d_tile_md); hipFree(L.d_mq);
+    // tables inside the arena are not freed on their own
+    hipFree(L.d_arena); hipFree(L.d_geom); hipFree(L.d_tile_y); hipFree(L.d_tile_z);
+    hipFree(L.d_acc); hipFree(L.d_red); hipFree(L.d_prim32); hipFree(L.d_prim64);
+    hipFree(L.d_ebq);
+    for (int d = 0; d < 3; ++d) { hipFree(L.d_ebF[d]); hipFree(L.d_mF[d]); }
+    hipFree(L.d_mq);
     L = LevelDev{};
 }
 
@@ -497,14 +500,16 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     L.march_ch = ch;
     march_tiles(L, ch, ty, tz);
     int rc = OCFD_OK;
-    if ((rc = upload(&L.d_map, L.owner_map)) || (rc = upload(&L.d_map_list, L.owner_list)) || (rc = upload(&L.d_tile_x, tx)) || (rc = upload(&L.d_tile_xp, txp)) || (rc = upload(&L.d_tile_y, ty)) ||
-        (rc = upload(&L.d_tile_z, tz)) || (rc = upload(&L.d_tile_g, tg)) || (rc = upload(&L.d_tile_c, tc)) ||
-        (rc = upload(&L.d_tile_p, tp)) || (rc = upload(&L.d_tile_pp, tpp)) || (rc = upload(&L.d_tile_v, tv)) ||
-        (rc = upload(&L.d_tile_eq, teq)) || (rc = upload(&L.d_tile_ef[0], tef[0])) ||
-        (rc = upload(&L.d_tile_ef[1], tef[1])) || (rc = upload(&L.d_tile_ef[2], tef[2])) ||
-        (rc = upload(&L.d_tile_ed, ted)) || (rc = upload(&L.d_ebflag, eb_flag)) || (rc = upload(&L.d_ebgeo, eb_geo)) ||
-        (rc = upload(&L.d_tile_mf[0], tmf[0])) || (rc = upload(&L.d_tile_mf[1], tmf[1])) ||
-        (rc = upload(&L.d_tile_mf[2], tmf[2])) || (rc = upload(&L.d_tile_md, tmd))) {
+    Staging stg;
+    stg.add(&L.d_map, L.owner_map); stg.add(&L.d_map_list, L.owner_list);
+    stg.add(&L.d_tile_x, tx); stg.add(&L.d_tile_xp, txp); stg.add(&L.d_tile_g, tg); stg.add(&L.d_tile_c, tc);
+    stg.add(&L.d_tile_p, tp); stg.add(&L.d_tile_pp, tpp); stg.add(&L.d_tile_v, tv); stg.add(&L.d_tile_eq, teq);
+    for (int d = 0; d < 3; ++d) stg.add(&L.d_tile_ef[d], tef[d]);
+    stg.add(&L.d_tile_ed, ted); stg.add(&L.d_ebflag, eb_flag); stg.add(&L.d_ebgeo, eb_geo);
+    for (int d = 0; d < 3; ++d) stg.add(&L.d_tile_mf[d], tmf[d]);
+    stg.add(&L.d_tile_md, tmd);
+    stg.add(&L.d_boxes, L.boxes);
+    if ((rc = stg.commit(&L.d_arena)) || (rc = upload(&L.d_tile_y, ty)) || (rc = upload(&L.d_tile_z, tz))) {
         free_level(L); delete lev; return rc;
     }
     L.ntile_eq = (int)teq.size(); L.ntile_ed = (int)ted.size(); L.ntile_md = (int)tmd.size();
@@ -522,7 +527,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     G.map = L.d_map;
     G.map_list = L.d_map_list;
     std::vector<GeomDesc> gv(1, G);
-    if ((rc = upload(&L.d_geom, gv)) || (rc = upload(&L.d_boxes, L.boxes))) { free_level(L); delete lev; return rc; }
+    if ((rc = upload(&L.d_geom, gv))) { free_level(L); delete lev; return rc; }
     if (hipMalloc((void**)&L.d_acc, sizeof(double) * L.mf_valid) != hipSuccess ||
         hipMalloc((void**)&L.d_prim32, sizeof(float) * L.mf_prim) != hipSuccess ||
         hipMalloc((void**)&L.d_prim64, sizeof(double) * L.mf_prim) != hipSuccess ||

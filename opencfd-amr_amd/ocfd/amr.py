@@ -79,7 +79,8 @@ def cluster_tags(dense, lo, block=1, grid_eff=0.7, max_size=32, mask=None):
     """Boxes (in the tags' index space, domain lower corner `lo`) covering the
     tagged cells of a dense (nz, ny, nx) array, within the region `mask`
     allows (same shape) when given."""
-    dense = np.ascontiguousarray(dense, dtype=np.uint8)
+    dense = np.ascontiguousarray(dense)
+    dense = dense.view(np.uint8) if dense.dtype == np.bool_ else dense.astype(np.uint8)
     nz, ny, nx = dense.shape
     lo_c = (ctypes.c_int * 3)(*lo)
     hi_c = (ctypes.c_int * 3)(lo[0] + nx - 1, lo[1] + ny - 1, lo[2] + nz - 1)
@@ -87,7 +88,8 @@ def cluster_tags(dense, lo, block=1, grid_eff=0.7, max_size=32, mask=None):
     up = ctypes.POINTER(ctypes.c_ubyte)
     mk = None
     if mask is not None:
-        mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        mask = np.ascontiguousarray(mask)
+        mask = mask.view(np.uint8) if mask.dtype == np.bool_ else mask.astype(np.uint8)
         mk = mask.ctypes.data_as(up)
     cap = 4096
     while True:
@@ -292,14 +294,16 @@ class Hierarchy:
             self.start()
 
     # -- buffers ------------------------------------------------------------
-    def _link(self, l):
-        cf = FluxRegister(self.levels[l - 1], self.levels[l])
+    def _link(self, l, cf=None):
+        if cf is None:
+            cf = FluxRegister(self.levels[l - 1], self.levels[l])
         if (self.flags & PLANAR) and cf.nfaces and (cf.faces()[:, 4] == 2).any():
             raise ValueError("planar stepping needs fine grids that span the domain in z")
         self.cfs[l] = cf
 
-    def _set_level(self, l, level, S=None, t=0.0):
-        """Install level l at time t (None removes it and everything finer)."""
+    def _set_level(self, l, level, S=None, t=0.0, cf=None):
+        """Install level l at time t (None removes it and everything finer);
+        cf: its register with level l-1 when already built."""
         if level is None:
             for m in range(l, self.max_level + 1):
                 self.levels[m], self.bufs[m], self.cfs[m] = None, None, None
@@ -316,7 +320,7 @@ class Hierarchy:
         q.s1, q.s2, q.tmp = b["s1"].data_ptr(), b["s2"].data_ptr(), b["tmp"].data_ptr()
         q.t_old = q.t_new = t
         if l > 0:
-            self._link(l)
+            self._link(l, cf)
         # the next finer level now hangs off a new level; it is relinked, or
         # rebuilt by the caller when it no longer nests
         if l + 1 <= self.max_level and self.levels[l + 1] is not None:
@@ -419,8 +423,7 @@ class Hierarchy:
                 copy_level(lev, Sn, old_level, old_S, self.stream)
             self.fill(l - 1, None, t)
             cf.interp_fine(Sn, self.S[l - 1], old_level, self.stream)
-        del cf
-        self._set_level(l, lev, Sn, t)
+        self._set_level(l, lev, Sn, t, cf)
         self.finest = max(self.finest, l)
 
     def start(self):

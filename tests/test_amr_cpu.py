@@ -391,3 +391,29 @@ def test_projected_coverage_matches_dense_projection():
             nz, ny, nx = want.shape
             want = want.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).any(axis=(1, 3, 5))
             assert np.array_equal(amr.projected_coverage(boxes, gf, gc, n_proper), want), (per, n_proper)
+
+
+@pytest.mark.parametrize("masked", [False, True])
+def test_library_clustering_matches_oracle_on_fronts(masked):
+    """Oblique tagged fronts (the shape shock tags take), where clustering
+    leaves many thin boxes and the box merge joins many pairs: library and
+    oracle give the same boxes, with and without a nesting mask."""
+    from ocfd.amr import cluster_tags
+    rng = np.random.default_rng(21 + masked)
+    for trial in range(8):
+        shape = (2, 2 * int(rng.integers(16, 40)), 2 * int(rng.integers(24, 64)))
+        d = np.zeros(shape, np.uint8)
+        for _ in range(int(rng.integers(1, 4))):
+            x0, sl, w = int(rng.integers(0, shape[2])), rng.uniform(-1.5, 1.5), int(rng.integers(1, 4))
+            for j in range(shape[1]):
+                x = int(x0 + sl * j)
+                if 0 <= x < shape[2] - w:
+                    d[:, j, x:x + w] = 1
+        mask = None
+        if masked:
+            mask = np.ones(shape, np.uint8)
+            mask[:, :, shape[2] // 2:] = 0
+        for block, eff, mx in ((1, 0.7, 32), (2, 0.7, 8)):
+            want = sorted(ao.cluster(d, (0, 0, 0), block, eff, mx, mask))
+            got = sorted(cluster_tags(d, (0, 0, 0), block, eff, mx, mask))
+            assert got == want, (trial, block)
