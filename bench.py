@@ -247,7 +247,7 @@ def amr_dmr(steps, warmup=40, nx=1024, ny=256, max_level=2, cpu_sample=True, uni
     if uniform:
         r = 2 ** max_level
         fb, fg, _ = cases.dmr(r * nx, r * ny, 2 * r, max_grid=(128, 128, 2 * r))
-        lev = ocfd.Level(fb, fg)
+        lev = ocfd.Level(fb, fg).set_planar(True)
         A, B, C = lev.new_mf(), lev.new_mf(), lev.new_mf()
         lev.init_state(kind, A, p)
         lev.advance(A, B, C, 0.0, dts[0] / r, 2, p, ocfd.PLANAR)

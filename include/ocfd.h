@@ -172,6 +172,14 @@ int ocfd_rk_stage(ocfd_level lev, const double* S_cur, const double* S_old, doub
  * periodic images and the physical BCs at time `time`. */
 int ocfd_fill_boundary(ocfd_level lev, double* S, double time, void* stream);
 
+/* Planar fill depth (no counterpart in the reference, whose 2D runs carry all
+ * four z ghost layers): with on != 0 the level's ghost fills (this level's
+ * FillPatch and the coarse/fine fill of an AMR level) write one z ghost layer
+ * instead of four, which is all a z-uniform level stepped with OCFD_PLANAR
+ * reads.  Stages without OCFD_PLANAR and the multispecies entry points then
+ * return OCFD_EINVAL; levels with cut boxes return OCFD_EUNSUPPORTED. */
+int ocfd_level_set_planar(ocfd_level lev, int on);
+
 /* Tile shape of the y and z march sweeps: cells per march chunk (shorter
  * chunks give more independent waves at the cost of a longer halo prologue
  * per cell).  A level starts with 32, or 16 or 8 when it has few march tiles.

@@ -277,7 +277,7 @@ __global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fbo
     const BoxDesc B = fboxes[t.x];
     const GeomDesc& G = *fgeom;
     int i, j, k;
-    ghost_cell(B, t.y + threadIdx.x, i, j, k);
+    ghost_cell(B, t.y + threadIdx.x, t.w, i, j, k);
     if (G.probtype == 1 && j < G.dom_lo[1] && G.bc[0][1][0] == BC_EVEN) {
         // the DMR post-shock wall region x < 1/6 below y-lo is set by the case
         const double x = G.prob_lo[0] + (i + 0.5) * G.dx[0];
@@ -400,6 +400,26 @@ __global__ void k_lincomb(long n, double a, const double* __restrict__ x, double
 {
 #pragma clang fp contract(off)
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) out[i] = a * x[i] + b * y[i];
+}
+
+// the same over the valid cells of a level's boxes only (the ghost cells are
+// filled afterwards); tiles: (box, first valid cell, count, 0)
+__global__ __launch_bounds__(256) void k_lincomb_valid(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+                                                       double a, const double* __restrict__ x, double b,
+                                                       const double* __restrict__ y, double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int4 t = tiles[blockIdx.x];
+    const BoxDesc B = boxes[t.x];
+    for (int c = threadIdx.x; c < t.z; c += 256) {
+        int i, j, k;
+        valid_cell(B, t.y + c, i, j, k);
+        const long s0 = sidx(B, i, j, k);
+        for (int n = 0; n < NC; ++n) {
+            const long q = s0 + n * B.nptg;
+            out[q] = a * x[q] + b * y[q];
+        }
+    }
 }
 
 } // namespace ocfd
@@ -747,8 +767,10 @@ int ocfd_cf_fill_fine(ocfd_cf c, double* S_fine, const double* S_coarse, void* s
     }
     const LevelDev& F = c->fine->dev;
     const LevelDev& C = c->coarse->dev;
-    if (F.ntile_g)
-        hipLaunchKernelGGL(k_fill_cf, dim3(F.ntile_g), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_geom, F.d_tile_g,
+    int ng;
+    const int4* tg = fill_tiles(F, ng);
+    if (ng)
+        hipLaunchKernelGGL(k_fill_cf, dim3(ng), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_geom, tg,
                            S_fine, C.d_boxes, C.d_geom, c->d_cmap, S_coarse);
     return launched("ocfd_cf_fill_fine");
 }
@@ -880,9 +902,12 @@ struct Hier {
             out = q.old;        // ghosts filled at t_old by the level's first stage
             return OCFD_OK;
         }
-        if ((rc = ocfd_lincomb(levels[m]->dev.mf_state, (q.t_new - ts) / span, q.old, (ts - q.t_old) / span, q.cur,
-                               q.tmp, s)))
-            return rc;
+        // valid cells combined in time, then the ghost cells filled at ts
+        const LevelDev& L = levels[m]->dev;
+        if (L.ntile_c)
+            hipLaunchKernelGGL(k_lincomb_valid, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c,
+                               (q.t_new - ts) / span, q.old, (ts - q.t_old) / span, q.cur, q.tmp);
+        if ((rc = launched("ocfd_amr_advance_level"))) return rc;
         if ((rc = fill(m, q.tmp, ts))) return rc;
         out = q.tmp;
         return OCFD_OK;

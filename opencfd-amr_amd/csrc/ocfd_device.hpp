@@ -127,17 +127,18 @@ __device__ __forceinline__ long didx(const BoxDesc& B, int i, int j, int k)
     return B.off_d + ((long)(k - B.lo[2]) * ny + (j - B.lo[1])) * nx + (i - B.lo[0]);
 }
 
-// Decode the g-th ghost cell of box B: z slabs (full grown x-y extent), then
-// y slabs (valid k), then x slabs (valid j,k).
-__device__ __forceinline__ void ghost_cell(const BoxDesc& B, int g, int& i, int& j, int& k)
+// Decode the g-th ghost cell of box B: z slabs gz deep (full grown x-y
+// extent), then y slabs (valid k), then x slabs (valid j,k).  gz = NG covers
+// every ghost cell; gz = 1 the ghost region a planar (z-uniform) level reads.
+__device__ __forceinline__ void ghost_cell(const BoxDesc& B, int g, int gz, int& i, int& j, int& k)
 {
-    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    const int ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
     const int nxg = B.nxg, nyg = B.nyg;
-    const int zs = NG * nxg * nyg;
+    const int zs = gz * nxg * nyg;
     if (g < 2 * zs) {
         const int side = g / zs, r = g - side * zs;
         const int kk = r / (nxg * nyg), rem = r - kk * nxg * nyg;
-        k = side ? B.hi[2] + 1 + kk : B.lo[2] - NG + kk;
+        k = side ? B.hi[2] + 1 + kk : B.lo[2] - gz + kk;
         j = B.lo[1] - NG + rem / nxg;
         i = B.lo[0] - NG + rem % nxg;
         return;
@@ -161,7 +162,6 @@ __device__ __forceinline__ void ghost_cell(const BoxDesc& B, int g, int& i, int&
     j = B.lo[1] + rem / NG;
     const int ii = rem % NG;
     i = side ? B.hi[0] + 1 + ii : B.lo[0] - NG + ii;
-    (void)nx;
 }
 
 // Primitive state used by the flux kernels

@@ -28,7 +28,9 @@ struct LevelDev {
     int4* d_tile_xp = nullptr; int ntile_xp = 0;   // planar x sweep (z growth 1)
     int4* d_tile_y = nullptr; int ntile_y = 0;
     int4* d_tile_z = nullptr; int ntile_z = 0;
-    int4* d_tile_g = nullptr; int ntile_g = 0;   // ghost fill: (box, first ghost cell, count, 0)
+    int4* d_tile_g = nullptr; int ntile_g = 0;   // ghost fill: (box, first ghost cell, count, z depth NG)
+    int4* d_tile_gp = nullptr; int ntile_gp = 0; // planar ghost fill: z depth 1
+    int fill_gz = NG;               // z depth of the level's ghost fills (1: planar levels only)
     int4* d_tile_c = nullptr; int ntile_c = 0;   // valid-cell chunks: (box, first cell, count, 0)
     int4* d_tile_v = nullptr; int ntile_v = 0;   // valid-cell chunks of 64 (chemistry)
     int4* d_tile_p = nullptr; int ntile_p = 0;   // primitive region grow(bx,3): (box, first, count, 3)
@@ -64,6 +66,13 @@ struct LevelDev {
 void launch_eb(const LevelDev& L, const StageArgs& a, hipStream_t s);
 
 constexpr int kMapShift = 3;   // owner-map tiles of 8^3 cells
+
+// ghost-fill work-list of a level at its fill depth
+inline const int4* fill_tiles(const LevelDev& L, int& n)
+{
+    n = L.fill_gz == 1 ? L.ntile_gp : L.ntile_g;
+    return L.fill_gz == 1 ? L.d_tile_gp : L.d_tile_g;
+}
 
 // Host tables gathered into one device allocation filled by one copy; each
 // table starts on a 256-byte boundary.  commit() sets every registered

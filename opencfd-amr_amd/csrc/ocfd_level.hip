@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes,
     const BoxDesc B = boxes[t.x];
     const GeomDesc& G = geoms[B.geom];
     int i, j, k;
-    ghost_cell(B, t.y + threadIdx.x, i, j, k);
+    ghost_cell(B, t.y + threadIdx.x, t.w, i, j, k);
     const long dst = sidx(B, i, j, k);
     int iv[3] = {i, j, k};
     int outside[3];
@@ -454,12 +454,15 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         return std::max(ny_t, nz_t);
     };
     while (ch > 8 && march_count(ch) < 2048) ch /= 2;
-    std::vector<int4> tx, txp, ty, tz, tg, tc, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
+    std::vector<int4> tx, txp, ty, tz, tg, tgp, tc, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
         const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
         const long ngh = B.nptg - B.nptv;
-        for (long s = 0; s < ngh; s += 256) tg.push_back(make_int4(b, (int)s, (int)std::min<long>(256, ngh - s), 0));
+        for (long s = 0; s < ngh; s += 256) tg.push_back(make_int4(b, (int)s, (int)std::min<long>(256, ngh - s), NG));
+        // planar fill: z ghosts one layer deep
+        const long ngp = 2L * B.nxg * B.nyg + 2L * NG * B.nxg * nz + 2L * NG * ny * nz;
+        for (long s = 0; s < ngp; s += 256) tgp.push_back(make_int4(b, (int)s, (int)std::min<long>(256, ngp - s), 1));
         for (long s = 0; s < B.nptv; s += 2048) tc.push_back(make_int4(b, (int)s, (int)std::min<long>(2048, B.nptv - s), 0));
         for (long s = 0; s < B.nptv; s += 64) tv.push_back(make_int4(b, (int)s, (int)std::min<long>(64, B.nptv - s), 0));
         if (B.type != 0) {
@@ -502,7 +505,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     int rc = OCFD_OK;
     Staging stg;
     stg.add(&L.d_map, L.owner_map); stg.add(&L.d_map_list, L.owner_list);
-    stg.add(&L.d_tile_x, tx); stg.add(&L.d_tile_xp, txp); stg.add(&L.d_tile_g, tg); stg.add(&L.d_tile_c, tc);
+    stg.add(&L.d_tile_x, tx); stg.add(&L.d_tile_xp, txp); stg.add(&L.d_tile_g, tg); stg.add(&L.d_tile_gp, tgp); stg.add(&L.d_tile_c, tc);
     stg.add(&L.d_tile_p, tp); stg.add(&L.d_tile_pp, tpp); stg.add(&L.d_tile_v, tv); stg.add(&L.d_tile_eq, teq);
     for (int d = 0; d < 3; ++d) stg.add(&L.d_tile_ef[d], tef[d]);
     stg.add(&L.d_tile_ed, ted); stg.add(&L.d_ebflag, eb_flag); stg.add(&L.d_ebgeo, eb_geo);
@@ -521,7 +524,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         if (!ok) { set_error("ocfd_level_create_eb: out of device memory"); free_level(L); delete lev; return OCFD_ENOMEM; }
     }
     L.ntile_x = (int)tx.size(); L.ntile_xp = (int)txp.size(); L.ntile_y = (int)ty.size(); L.ntile_z = (int)tz.size();
-    L.ntile_g = (int)tg.size(); L.ntile_c = (int)tc.size();
+    L.ntile_g = (int)tg.size(); L.ntile_gp = (int)tgp.size(); L.ntile_c = (int)tc.size();
     L.ntile_p = (int)tp.size(); L.ntile_pp = (int)tpp.size(); L.ntile_v = (int)tv.size();
     L.mf_prim = L.mf_state / NC * 6;
     G.map = L.d_map;
@@ -536,7 +539,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         free_level(L); delete lev; return OCFD_ENOMEM;
     }
     L.dev_bytes = (long)sizeof(int) * (long)(L.owner_map.size() + L.owner_list.size()) +
-                  (long)sizeof(int4) * (L.ntile_x + L.ntile_xp + L.ntile_y + L.ntile_z + L.ntile_g + L.ntile_c + L.ntile_p +
+                  (long)sizeof(int4) * (L.ntile_x + L.ntile_xp + L.ntile_y + L.ntile_z + L.ntile_g + L.ntile_gp + L.ntile_c + L.ntile_p +
                                         L.ntile_pp + L.ntile_v) +
                   (long)sizeof(GeomDesc) + (long)sizeof(BoxDesc) * L.nbox + (long)sizeof(double) * L.mf_valid +
                   (long)(sizeof(float) + sizeof(double)) * L.mf_prim + (long)sizeof(unsigned long long) +
@@ -555,6 +558,17 @@ int ocfd_level_destroy(ocfd_level lev)
 }
 
 int ocfd_level_nbox(ocfd_level lev) { return lev ? lev->dev.nbox : 0; }
+
+int ocfd_level_set_planar(ocfd_level lev, int on)
+{
+    if (!lev) { set_error("ocfd_level_set_planar: invalid arguments"); return OCFD_EINVAL; }
+    if (on && lev->dev.n_cut) {
+        set_error("ocfd_level_set_planar: cut boxes read two z ghost layers; keep the full fill depth");
+        return OCFD_EUNSUPPORTED;
+    }
+    lev->dev.fill_gz = on ? 1 : NG;
+    return OCFD_OK;
+}
 
 long ocfd_level_device_bytes(ocfd_level lev) { return lev ? lev->dev.dev_bytes : -1; }
 
@@ -583,8 +597,10 @@ long ocfd_level_fab_offset(ocfd_level lev, int box, int ngrow, int ncomp)
     return n;
 }
 
-static int valid_flags(int flags)
+// stage flags, and a planar fill depth only with planar stepping
+static int valid_flags(const LevelDev& L, int flags)
 {
+    if (L.fill_gz != NG && !(flags & OCFD_PLANAR)) return 0;
     return (flags & ~(OCFD_VISC | OCFD_GRAVITY | OCFD_FLUX_FP32 | OCFD_PLANAR | OCFD_EB_REDIST)) == 0;
 }
 
@@ -592,7 +608,7 @@ int ocfd_compute_dsdt(ocfd_level lev, const double* S, double* dSdt, double dt, 
                       void* stream)
 {
     (void)dt;
-    if (!lev || !S || !dSdt || !parm || !valid_flags(flags)) { set_error("ocfd_compute_dsdt: invalid arguments"); return OCFD_EINVAL; }
+    if (!lev || !S || !dSdt || !parm || !valid_flags(lev->dev, flags)) { set_error("ocfd_compute_dsdt: invalid arguments"); return OCFD_EINVAL; }
     StageArgs a{};
     a.scur = S; a.sold = nullptr; a.snew = nullptr; a.acc = dSdt;
     a.c_cur = 1.0; a.c_old = 0.0; a.c_dt = 0.0;
@@ -604,7 +620,7 @@ int ocfd_compute_dsdt(ocfd_level lev, const double* S, double* dSdt, double dt, 
 int ocfd_rk_stage(ocfd_level lev, const double* S_cur, const double* S_old, double* S_new, double c_cur, double c_old,
                   double c_dt, const ocfd_parm* parm, int flags, void* stream)
 {
-    if (!lev || !S_cur || !S_new || !parm || !valid_flags(flags) || S_cur == S_new) {
+    if (!lev || !S_cur || !S_new || !parm || !valid_flags(lev->dev, flags) || S_cur == S_new) {
         set_error("ocfd_rk_stage: invalid arguments (S_new must differ from S_cur)");
         return OCFD_EINVAL;
     }
@@ -620,8 +636,10 @@ int ocfd_fill_boundary(ocfd_level lev, double* S, double time, void* stream)
 {
     if (!lev || !S) { set_error("ocfd_fill_boundary: invalid arguments"); return OCFD_EINVAL; }
     const LevelDev& L = lev->dev;
-    if (L.ntile_g)
-        hipLaunchKernelGGL(k_fill, dim3(L.ntile_g), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_geom, L.d_tile_g, S, time,
+    int ng;
+    const int4* tg = fill_tiles(L, ng);
+    if (ng)
+        hipLaunchKernelGGL(k_fill, dim3(ng), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_geom, tg, S, time,
                            (const ocfd_step_ctl*)nullptr, 0.0);
     return check_launch("ocfd_fill_boundary");
 }
@@ -679,7 +697,7 @@ int ocfd_profile_stage(ocfd_level lev, const double* S_cur, const double* S_old,
                        double c_old, double c_dt, const ocfd_parm* parm, int flags, int reps, double* ms_out,
                        void* stream)
 {
-    if (!lev || !S_cur || !S_new || !parm || !ms_out || reps <= 0 || !valid_flags(flags) || S_cur == S_new ||
+    if (!lev || !S_cur || !S_new || !parm || !ms_out || reps <= 0 || !valid_flags(lev->dev, flags) || S_cur == S_new ||
         (flags & OCFD_PLANAR)) {
         set_error("ocfd_profile_stage: invalid arguments");
         return OCFD_EINVAL;
@@ -742,7 +760,7 @@ int ocfd_level_march_chunk(ocfd_level lev) { return lev ? lev->dev.march_ch : -1
 int ocfd_level_autotune(ocfd_level lev, const double* S, const ocfd_parm* parm, int flags, int reps, int* ch_out,
                         double* ms_out, void* stream)
 {
-    if (!lev || !S || !parm || !ch_out || reps <= 0 || !valid_flags(flags)) {
+    if (!lev || !S || !parm || !ch_out || reps <= 0 || !valid_flags(lev->dev, flags)) {
         set_error("ocfd_level_autotune: invalid arguments");
         return OCFD_EINVAL;
     }
@@ -816,8 +834,10 @@ static int step_device(ocfd_level lev, double* A, double* B, double* C, ocfd_ste
     StageArgs a{};
     a.flags = flags; a.ph = phys_of(*parm); a.ctl = ctl; a.acc = L.d_acc;
     auto fill = [&](double* S, double toff) {
-        if (L.ntile_g)
-            hipLaunchKernelGGL(k_fill, dim3(L.ntile_g), dim3(256), 0, s, L.d_boxes, L.d_geom, L.d_tile_g, S, 0.0,
+        int ng;
+        const int4* tg = fill_tiles(L, ng);
+        if (ng)
+            hipLaunchKernelGGL(k_fill, dim3(ng), dim3(256), 0, s, L.d_boxes, L.d_geom, tg, S, 0.0,
                                (const ocfd_step_ctl*)ctl, toff);
     };
     auto stage = [&](const double* cur, const double* old, double* nw, double cc, double co, double cdt) {
@@ -851,7 +871,7 @@ static int step_device(ocfd_level lev, double* A, double* B, double* C, ocfd_ste
 static bool step_args_ok(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
                          const ocfd_parm* parm, int flags)
 {
-    return lev && A && B && ctl && parm && valid_flags(flags) && scheme >= 0 && scheme <= 2 && (scheme != 2 || C) &&
+    return lev && A && B && ctl && parm && valid_flags(lev->dev, flags) && scheme >= 0 && scheme <= 2 && (scheme != 2 || C) &&
            A != B && (scheme != 2 || (C != A && C != B));
 }
 

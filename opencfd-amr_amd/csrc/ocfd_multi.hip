@@ -469,7 +469,7 @@ __global__ __launch_bounds__(256) void k_fill_spec(const BoxDesc* __restrict__ b
     const BoxDesc B = boxes[t.x];
     const GeomDesc& G = geoms[B.geom];
     int i, j, k;
-    ghost_cell(B, t.y + threadIdx.x, i, j, k);
+    ghost_cell(B, t.y + threadIdx.x, t.w, i, j, k);
     const long dst = B.off_s / NC * NS + gl(B, i, j, k);
     int iv[3] = {i, j, k}, outside[3];
 #pragma unroll
@@ -788,6 +788,10 @@ static MultiArgs margs(LevelDev& L, const double* S, const double* P, const doub
 
 int ocfd_fill_boundary_spec(ocfd_level lev, double* Spec, double time, void* stream)
 {
+    if (lev && lev->dev.fill_gz != NG) {
+        set_error("multispecies stepping needs the full ghost fill depth (ocfd_level_set_planar off)");
+        return OCFD_EINVAL;
+    }
     (void)time;
     if (!lev || !Spec) { set_error("ocfd_fill_boundary_spec: invalid arguments"); return OCFD_EINVAL; }
     launch_fill_spec(lev->dev, Spec, nullptr, (hipStream_t)stream);
@@ -797,6 +801,10 @@ int ocfd_fill_boundary_spec(ocfd_level lev, double* Spec, double time, void* str
 int ocfd_compute_dsdt_multi(ocfd_level lev, const double* S, const double* Spec, double* dSdt, double* dSdt_spec,
                             double dt, const ocfd_parm* parm, int flags, void* stream)
 {
+    if (lev && lev->dev.fill_gz != NG) {
+        set_error("multispecies stepping needs the full ghost fill depth (ocfd_level_set_planar off)");
+        return OCFD_EINVAL;
+    }
     (void)dt;
     if (!lev || !S || !Spec || !dSdt || !dSdt_spec || !parm) {
         set_error("ocfd_compute_dsdt_multi: invalid arguments");
@@ -814,6 +822,10 @@ int ocfd_rk_stage_multi(ocfd_level lev, const double* S_cur, const double* Spec_
                         const double* Spec_old, double* S_new, double* Spec_new, double c_cur, double c_old,
                         double c_dt, const ocfd_parm* parm, int flags, void* stream)
 {
+    if (lev && lev->dev.fill_gz != NG) {
+        set_error("multispecies stepping needs the full ghost fill depth (ocfd_level_set_planar off)");
+        return OCFD_EINVAL;
+    }
     if (!lev || !S_cur || !Spec_cur || !S_new || !Spec_new || !parm || S_cur == S_new || Spec_cur == Spec_new ||
         (!S_old) != (!Spec_old)) {
         set_error("ocfd_rk_stage_multi: invalid arguments");
@@ -830,6 +842,10 @@ int ocfd_rk_stage_multi(ocfd_level lev, const double* S_cur, const double* Spec_
 int ocfd_advance_multi(ocfd_level lev, double* A, double* Aspec, double* B, double* Bspec, double time, double dt,
                        int scheme, const ocfd_parm* parm, int flags, void* stream)
 {
+    if (lev && lev->dev.fill_gz != NG) {
+        set_error("multispecies stepping needs the full ghost fill depth (ocfd_level_set_planar off)");
+        return OCFD_EINVAL;
+    }
     if (!lev || !A || !Aspec || !B || !Bspec || !parm || scheme < 0 || scheme > 1) {
         set_error("ocfd_advance_multi: invalid arguments (scheme 0 Euler or 1 RK2)");
         return OCFD_EINVAL;
@@ -854,6 +870,10 @@ int ocfd_advance_multi(ocfd_level lev, double* A, double* Aspec, double* B, doub
 int ocfd_advance_chem(ocfd_level lev, double* A, double* Aspec, double* B, double* Bspec, double time, double dt,
                       int level, int scheme, const ocfd_parm* parm, int flags, int renorm, void* stream)
 {
+    if (lev && lev->dev.fill_gz != NG) {
+        set_error("multispecies stepping needs the full ghost fill depth (ocfd_level_set_planar off)");
+        return OCFD_EINVAL;
+    }
     if (!lev || !A || !Aspec || !B || !Bspec || !parm || level < 0) {
         set_error("ocfd_advance_chem: invalid arguments");
         return OCFD_EINVAL;

@@ -158,6 +158,7 @@ def lib():
         L.ocfd_level_autotune.argtypes = [vp, vp, ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, ip, dp, vp]
         L.ocfd_cluster_tags.argtypes = [ip, ip, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte),
                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ip, ip]
+        L.ocfd_level_set_planar.argtypes = [vp, ctypes.c_int]
         L.ocfd_mask_grow.argtypes = [ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ip, ctypes.c_int]
         L.ocfd_lincomb.argtypes = [ctypes.c_long, ctypes.c_double, vp, ctypes.c_double, vp, vp, vp]
@@ -286,6 +287,12 @@ class Level:
         check(lib().ocfd_level_create_eb(n, lo, hi, ctypes.byref(geom), ctypes.byref(eb) if eb is not None else None,
                                          ctypes.byref(h)), "ocfd_level_create_eb")
         self.handle = h
+
+    def set_planar(self, on=True):
+        """Fill only one z ghost layer (a z-uniform level stepped with PLANAR);
+        stages without PLANAR are then refused."""
+        check(lib().ocfd_level_set_planar(self.handle, int(bool(on))), "ocfd_level_set_planar")
+        return self
 
     def box_types(self):
         """Per box: 0 regular, 1 cut (embedded-boundary path), 2 covered."""

@@ -311,6 +311,8 @@ class Hierarchy:
             self.finest = l - 1
             return
         self.levels[l] = level
+        if self.flags & PLANAR:
+            level.set_planar(True)
         b = {k: level.new_mf() for k in ("a", "b", "s1", "s2", "tmp")}
         if S is not None:
             b["a"] = S

@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (CTL_BYTES, NCONS, NGROW, Level, Reactors, StepCtl, default_parm, ignition_batch, new_ctl,
+from . import (CTL_BYTES, NCONS, NGROW, PLANAR, Level, Reactors, StepCtl, default_parm, ignition_batch, new_ctl,
                REACTOR_NCOMP)
 
 MiB = 1 << 20
@@ -76,6 +76,8 @@ class _FlowJob:
         self.stream = stream
         boxes, geom, kind = req.make()
         self.level = Level(boxes, geom)
+        if req.flags & PLANAR:
+            self.level.set_planar(True)
         self.parm = default_parm(**req.parm_kw)
         with torch.cuda.stream(stream):
             self.A = self.level.new_mf()

@@ -205,3 +205,29 @@ def test_dmr_uneven_steps_fp64(torch_cuda):
     (_, _), (_, _), errs_p = _run_pair(case, 0.5, -1.0, 10, ocfd.PLANAR)
     worst_p = max(max(e) for e in errs_p)
     assert worst_p < TOL64, worst_p
+
+
+@pytest.mark.parametrize("flags_name", ["fp64", "fp32_visc"])
+def test_planar_fill_depth_gives_the_same_steps(torch_cuda, flags_name):
+    """A planar level that fills one z ghost layer (ocfd_level_set_planar)
+    steps bit for bit like the same level filling all four; a stage without
+    PLANAR is refused on it."""
+    import ocfd
+    from ocfd import cases
+    flags = ocfd.PLANAR | (ocfd.FLUX_FP32 | ocfd.VISC if flags_name == "fp32_visc" else 0)
+    boxes, g, kind = cases.dmr(128, 32, 2, sizes=(8, 16, 32), seed=3)
+    p = ocfd.default_parm()
+    out = []
+    for planar in (False, True):
+        lev = ocfd.Level(boxes, g)
+        if planar:
+            lev.set_planar(True)
+        A = lev.new_mf(); B = lev.new_mf(); C = lev.new_mf()
+        lev.init_state(kind, A, p, seed=42)
+        ocfd.run(lev, A, B, C, p, 0.5, 2, flags, stop_time=-1.0, nsteps=8)
+        out.append([x[:, 4:-4, 4:-4, 4:-4] for x in gu.download(lev, A)])
+        if planar:
+            with pytest.raises(RuntimeError, match="invalid arguments"):
+                lev.rk_stage(A, None, B, 1.0, 0.0, 1e-6, p, flags & ~ocfd.PLANAR)
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
