@@ -16,6 +16,7 @@
 // primitive scratch the stage's sweeps have just used, so the fused sweeps
 // carry no flux-register work.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -217,18 +218,23 @@ __device__ __forceinline__ double mc_limit(double dc, double um, double u0, doub
 // cell, the coarse box to read it from: its owner, or else a box whose filled
 // ghost region holds the cell and its neighbours (a coarse level that does not
 // cover the domain).  Returns false when there is none.
-__device__ __forceinline__ bool cons_interp(const BoxDesc* __restrict__ cboxes, const GeomDesc& CG,
-                                            const int* __restrict__ cmap, const double* __restrict__ C, int i, int j,
-                                            int k, int n, double& v)
+// coarse box the interpolation of fine cell (i,j,k) reads, -1 if none
+__device__ __forceinline__ int interp_box(const BoxDesc* __restrict__ cboxes, const GeomDesc& CG,
+                                          const int* __restrict__ cmap, int i, int j, int k)
 {
     const int I = i >> 1, J = j >> 1, K = k >> 1;
     if (I < CG.dom_lo[0] || I > CG.dom_hi[0] || J < CG.dom_lo[1] || J > CG.dom_hi[1] || K < CG.dom_lo[2] ||
         K > CG.dom_hi[2])
-        return false;
+        return -1;
     const long ntile = (long)CG.map_n[0] * CG.map_n[1] * CG.map_n[2];
-    const int m = tile_owner(cmap, cmap + ntile, CG.map_n, CG.dom_lo, CG.map_shift, cboxes, I, J, K, 3);
-    if (m < 0) return false;
-    const BoxDesc Bc = cboxes[m];
+    return tile_owner(cmap, cmap + ntile, CG.map_n, CG.dom_lo, CG.map_shift, cboxes, I, J, K, 3);
+}
+
+// component n of fine cell (i,j,k) interpolated from coarse box Bc
+__device__ __forceinline__ double interp_value(const BoxDesc& Bc, const GeomDesc& CG, const double* __restrict__ C,
+                                               int i, int j, int k, int n)
+{
+    const int I = i >> 1, J = j >> 1, K = k >> 1;
     const double* u = C + n * Bc.nptg;
     const long s = sidx(Bc, I, J, K);
     const long dy = Bc.nxg, dz = (long)Bc.nxg * Bc.nyg;
@@ -258,8 +264,7 @@ __device__ __forceinline__ bool cons_interp(const BoxDesc* __restrict__ cboxes, 
         if (dumax * alpha > u0 - umin) alpha = (u0 - umin) / dumax;
     }
     const double xo = (i & 1) ? 0.25 : -0.25, yo = (j & 1) ? 0.25 : -0.25, zo = (k & 1) ? 0.25 : -0.25;
-    v = u0 + xo * (sl[0] * alpha) + yo * (sl[1] * alpha) + zo * (sl[2] * alpha);
-    return true;
+    return u0 + xo * (sl[0] * alpha) + yo * (sl[1] * alpha) + zo * (sl[2] * alpha);
 }
 
 // the coarse part of the two-level ghost fill: fine ghost cells whose source
@@ -295,6 +300,10 @@ __global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fbo
         }
         outside[d] = iv[d] < G.dom_lo[d] ? -1 : (iv[d] > G.dom_hi[d] ? 1 : 0);
     }
+    // the source cell is normally the same for every component (the BC maps
+    // mirror or clamp alike): its owner and coarse box are looked up once
+    int last[3] = {INT_MIN, INT_MIN, INT_MIN};
+    int own = -1, m = -1;
     for (int n = 0; n < NC; ++n) {
         int src[3] = {iv[0], iv[1], iv[2]};
         double sign = 1.0;
@@ -314,9 +323,13 @@ __global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fbo
         if (src[0] < G.dom_lo[0] || src[0] > G.dom_hi[0] || src[1] < G.dom_lo[1] || src[1] > G.dom_hi[1] ||
             src[2] < G.dom_lo[2] || src[2] > G.dom_hi[2])
             continue;
-        if (level_owner(G, fboxes, src[0], src[1], src[2]) >= 0) continue;   // the level fill copied it
-        double v;
-        if (cons_interp(cboxes, *cgeom, cmap, Sc, src[0], src[1], src[2], n, v)) Sf[dst + n * B.nptg] = sign * v;
+        if (src[0] != last[0] || src[1] != last[1] || src[2] != last[2]) {
+            own = level_owner(G, fboxes, src[0], src[1], src[2]);
+            m = own >= 0 ? -1 : interp_box(cboxes, *cgeom, cmap, src[0], src[1], src[2]);
+            last[0] = src[0]; last[1] = src[1]; last[2] = src[2];
+        }
+        if (own >= 0 || m < 0) continue;   // the level fill copied it / nothing to interpolate from
+        Sf[dst + n * B.nptg] = sign * interp_value(cboxes[m], *cgeom, Sc, src[0], src[1], src[2], n);
     }
 }
 
@@ -342,10 +355,9 @@ __global__ __launch_bounds__(256) void k_interp_valid(const BoxDesc* __restrict_
             }
         }
         const long dst = sidx(B, i, j, k);
-        for (int n = 0; n < NC; ++n) {
-            double v;
-            if (cons_interp(cboxes, *cgeom, cmap, Sc, i, j, k, n, v)) Sf[dst + n * B.nptg] = v;
-        }
+        const int m = interp_box(cboxes, *cgeom, cmap, i, j, k);
+        if (m < 0) continue;
+        for (int n = 0; n < NC; ++n) Sf[dst + n * B.nptg] = interp_value(cboxes[m], *cgeom, Sc, i, j, k, n);
     }
 }
 

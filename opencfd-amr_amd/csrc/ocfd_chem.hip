@@ -129,12 +129,27 @@ __global__ void k_counter_init(unsigned long long* c) { *c = 0ull; }
 // one step of h (y1) against two of h/2 (y2), error max_k |y2-y1| /
 // (atol rho + rtol |y2|); on acceptance keep the Richardson value 2 y2 - y1
 // (second order, densities clipped at 0); rescale h by clamp(0.9/sqrt(err), 0.2, 4).
+//
+// LPR = 2 (batches too small to fill the chip): two lanes per reactor, the
+// even lane takes the step of h, the odd lane the two steps of h/2 (its
+// first one alongside the even lane's step), then the pair swaps results and
+// both make the same decision on the same values -- a substep costs two
+// implicit steps of wall time instead of three.  LPR = 1 (full batches): one
+// lane takes all three.  Both give the same results bit for bit.
+template <int LPR>
 __global__ __launch_bounds__(64) void k_reactor_adaptive(long n, double* __restrict__ st, double t_end, double h0,
                                                          double rtol, double atol, int max_steps,
                                                          int* __restrict__ steps_out, int* __restrict__ rej_out,
                                                          unsigned long long* __restrict__ counter)
 {
-    long r = (long)atomicAdd(counter, 1ull);
+    const int lane = threadIdx.x & 63;
+    const bool odd = LPR == 2 && (lane & 1);
+    // the pair's reactor: taken by the even lane, shared with the odd one
+    auto next = [&]() {
+        long q = odd ? 0 : (long)atomicAdd(counter, 1ull);
+        return LPR == 2 ? (long)__shfl(q, lane & ~1) : q;
+    };
+    long r = next();
     double rhoi[NS], E = 0, rho = 0, t = 0, h = h0;
     int acc = 0, rej = 0;
     auto load = [&](long q) {
@@ -151,12 +166,32 @@ __global__ __launch_bounds__(64) void k_reactor_adaptive(long n, double* __restr
         else if (acc + rej >= max_steps) { done = true; failed = true; }
         if (!done) {
             if (t + h > t_end) h = t_end - t;
-            double r1[NS], r2[NS];
+            double r1[NS], r2[NS], E1, E2;
+            if (LPR == 2) {
+                // even: y1 = one step of h; odd: y2 = two steps of h/2 (one
+                // code path for both, so the step is inlined once)
+                double rr[NS];
 #pragma unroll
-            for (int k = 0; k < NS; ++k) { r1[k] = rhoi[k]; r2[k] = rhoi[k]; }
-            const double E1 = E + chem::cell_step(E, r1, h);
-            double E2 = E + chem::cell_step(E, r2, 0.5 * h);
-            E2 += chem::cell_step(E2, r2, 0.5 * h);
+                for (int k = 0; k < NS; ++k) rr[k] = rhoi[k];
+                double Ex = E;
+                const double hs = odd ? 0.5 * h : h;
+                for (int pass = 0; pass < (odd ? 2 : 1); ++pass) Ex += chem::cell_step(Ex, rr, hs);
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    const double other = __shfl_xor(rr[k], 1);
+                    r1[k] = odd ? other : rr[k];
+                    r2[k] = odd ? rr[k] : other;
+                }
+                const double Eo = __shfl_xor(Ex, 1);
+                E1 = odd ? Eo : Ex;
+                E2 = odd ? Ex : Eo;
+            } else {
+#pragma unroll
+                for (int k = 0; k < NS; ++k) { r1[k] = rhoi[k]; r2[k] = rhoi[k]; }
+                E1 = E + chem::cell_step(E, r1, h);
+                E2 = E + chem::cell_step(E, r2, 0.5 * h);
+                E2 += chem::cell_step(E2, r2, 0.5 * h);
+            }
             double err = 0.0;
 #pragma unroll
             for (int k = 0; k < NS; ++k) err = fmax(err, fabs(r2[k] - r1[k]) / (atol * rho + rtol * fabs(r2[k])));
@@ -174,12 +209,14 @@ __global__ __launch_bounds__(64) void k_reactor_adaptive(long n, double* __restr
             }
         }
         if (done) {
+            if (!odd) {
 #pragma unroll
-            for (int k = 0; k < NS; ++k) st[(RROW_Y + k) * n + r] = rhoi[k];
-            st[RROW_E * n + r] = E;
-            if (steps_out) steps_out[r] = failed ? -1 : acc;
-            if (rej_out) rej_out[r] = rej;
-            r = (long)atomicAdd(counter, 1ull);
+                for (int k = 0; k < NS; ++k) st[(RROW_Y + k) * n + r] = rhoi[k];
+                st[RROW_E * n + r] = E;
+                if (steps_out) steps_out[r] = failed ? -1 : acc;
+                if (rej_out) rej_out[r] = rej;
+            }
+            r = next();
             if (r < n) load(r);
         }
     }
@@ -260,10 +297,18 @@ int ocfd_reactor_advance_adaptive(long n, double* state, double t_end, double h0
     int dev = 0, ncu = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    // persistent grid: a few 64-lane workgroups per CU, never more lanes than reactors
-    long wgs = std::min<long>((n + 63) / 64, (long)std::max(ncu, 1) * 8);
-    hipLaunchKernelGGL(k_reactor_adaptive, dim3((unsigned)wgs), dim3(64), 0, s, n, state, t_end, h0, rtol, atol,
-                       max_steps, steps_out, rejects_out, ctr);
+    // persistent grid of 64-lane workgroups; one lane per reactor once the
+    // batch fills one wave per SIMD (4 per CU), else two lanes per reactor
+    const long lanes_full = 64L * 4 * std::max(ncu, 1);
+    if (n >= lanes_full) {
+        const long wgs = std::min<long>((n + 63) / 64, (long)std::max(ncu, 1) * 8);
+        hipLaunchKernelGGL(k_reactor_adaptive<1>, dim3((unsigned)wgs), dim3(64), 0, s, n, state, t_end, h0, rtol,
+                           atol, max_steps, steps_out, rejects_out, ctr);
+    } else {
+        const long wgs = (n + 31) / 32;
+        hipLaunchKernelGGL(k_reactor_adaptive<2>, dim3((unsigned)wgs), dim3(64), 0, s, n, state, t_end, h0, rtol,
+                           atol, max_steps, steps_out, rejects_out, ctr);
+    }
     return launch_check("ocfd_reactor_advance_adaptive");
 }
 

@@ -136,3 +136,23 @@ def test_level_chem_advance_and_renorm(torch_cuda, orc):
                             for b in range(len(boxes))])
     ref = orc.renorm(gu_, before)
     assert np.array_equal(after, ref)
+
+
+def test_reactor_adaptive_lane_modes_agree(torch_cuda):
+    """Small batches run two lanes per reactor, full batches one: the same
+    reactors give the same states, step and rejection counts bit for bit."""
+    import numpy as np
+    import torch
+    import ocfd
+    T0, p0, phi = ocfd.ignition_batch(256, seed=13)
+    small = ocfd.Reactors(T0, p0, phi)
+    small.advance_adaptive(2e-4, h0=1e-8, rtol=1e-4, atol=1e-9)
+    n = 64 * 4 * torch.cuda.get_device_properties(0).multi_processor_count   # one wave per SIMD: one lane each
+    reps = -(-n // 256)
+    big = ocfd.Reactors(np.tile(T0, reps)[:n], np.tile(p0, reps)[:n], np.tile(phi, reps)[:n])
+    big.advance_adaptive(2e-4, h0=1e-8, rtol=1e-4, atol=1e-9)
+    a = small.rows().cpu().numpy()
+    b = big.rows().cpu().numpy()[:, :256]
+    assert np.array_equal(a, b)
+    assert np.array_equal(small.steps.cpu().numpy(), big.steps.cpu().numpy()[:256])
+    assert np.array_equal(small.rejects.cpu().numpy(), big.rejects.cpu().numpy()[:256])
