@@ -3,7 +3,8 @@
 // work-list of a whole level (or of many levels/cases at once).
 //
 //   sweep_x : wave = one x-row segment of 58 cells + 3 halo cells each side,
-//             over every row of the primitive region.  Each lane recovers
+//             over every row of the primitive region; a block holds 4
+//             consecutive y rows of one z plane.  Each lane recovers
 //             the primitives of its cell from the state (FP64, c2prim) and
 //             stores them for the y and z sweeps; on rows inside the box it
 //             also splits its cell (FT), the split fluxes are exchanged
@@ -41,11 +42,12 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
     const int ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
     // rows of the primitive region grow(bx, (3,3,gz)); rows inside the box
     // also carry the x faces
-    const int gz = t.w, ey = ny + 6;
-    const int row = t.z * 4 + w;
-    const bool rowok = row < ey * (nz + 2 * gz);
-    const int j = B.lo[1] - 3 + (rowok ? row % ey : 3);
-    const int k = B.lo[2] - gz + (rowok ? row / ey : gz);
+    // t.z = (z row) * (y groups) + (y group): 4 consecutive y rows per block
+    const int gz = t.w, ey = ny + 6, ngy = (ey + 3) / 4;
+    const int jr = (t.z % ngy) * 4 + w, kr = t.z / ngy;
+    const bool rowok = jr < ey;
+    const int j = B.lo[1] - 3 + (rowok ? jr : 3);
+    const int k = B.lo[2] - gz + kr;
     const bool frow = rowok && j >= B.lo[1] && j <= B.hi[1] && k >= B.lo[2] && k <= B.hi[2];
     const int i = B.lo[0] + t.y * XSEG - 3 + lane;
     const bool cellok = rowok && i <= B.hi[0] + 3;
