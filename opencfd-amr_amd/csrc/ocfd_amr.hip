@@ -452,6 +452,7 @@ struct ocfd_cf_s {
     double* d_regf = nullptr;    // 5 per fine sub-face
     int4* d_tile_avg = nullptr;  // coarsened fine boxes, 256 coarse cells per tile
     int ntile_avg = 0;
+    std::vector<CFFace> h_cf;    // host copy of the coarse faces
     void* d_arena = nullptr;     // one allocation holding the lists and the map below
     int* d_cmap = nullptr;       // coarse cell -> box to interpolate from (owner, else the first box
                                  // whose grown-by-3 region holds it): a tile map of the coarse
@@ -542,15 +543,16 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
     std::vector<int> fmap, flist;
     int fmn[3];
     build_tile_map(CG.dom_lo, CG.dom_hi, CG.map_shift, fcrs, 0, fmn, fmap, flist);
-    // 1 = covered by the fine level, 0 = not, -1 = outside a non-periodic domain side
-    auto covered = [&](int I, int J, int K) {
+    // the fine box covering coarse cell (I,J,K) after the periodic wrap (fine
+    // boxes are aligned to 2, so it holds all 8 fine cells under it), -1 none,
+    // -2 outside a non-periodic domain side
+    auto covering = [&](int I, int J, int K) {
         int iv[3] = {I, J, K};
         for (int d = 0; d < 3; ++d) {
             iv[d] = wrap(CG, d, iv[d]);
-            if (iv[d] < CG.dom_lo[d] || iv[d] > CG.dom_hi[d]) return -1;
+            if (iv[d] < CG.dom_lo[d] || iv[d] > CG.dom_hi[d]) return -2;
         }
-        return tile_owner(fmap.data(), flist.data(), fmn, CG.dom_lo, CG.map_shift, fcrs.data(), iv[0], iv[1], iv[2],
-                          0) >= 0 ? 1 : 0;
+        return tile_owner(fmap.data(), flist.data(), fmn, CG.dom_lo, CG.map_shift, fcrs.data(), iv[0], iv[1], iv[2], 0);
     };
     // candidates (coarse box, K, J, I), visited in coarse-box then k, j, i order
     std::vector<std::array<int, 4>> cand;
@@ -566,7 +568,7 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
                         iv[d] = wrap(CG, d, iv[d]);
                         in = in && iv[d] >= CG.dom_lo[d] && iv[d] <= CG.dom_hi[d];
                     }
-                    if (!in || covered(iv[0], iv[1], iv[2]) == 1) continue;
+                    if (!in || covering(iv[0], iv[1], iv[2]) >= 0) continue;
                     const int cb = find_box(C, iv[0], iv[1], iv[2]);
                     if (cb >= 0) cand.push_back({cb, iv[2], iv[1], iv[0]});
                 }
@@ -582,7 +584,8 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
             for (int side = 0; side < 2; ++side) {
                 int nb[3] = {I, J, K};
                 nb[d] += side ? 1 : -1;
-                if (covered(nb[0], nb[1], nb[2]) != 1) continue;
+                const int fb = covering(nb[0], nb[1], nb[2]);
+                if (fb < 0) continue;
                 CFFace f{};
                 f.box = b; f.i = I; f.j = J; f.k = K; f.dir = d; f.up = side;
                 fidx[2 * d + side] = (int)cf.size();
@@ -597,12 +600,7 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
                         g[t1] += a1;
                         g[t2] += a2;
                         for (int e = 0; e < 3; ++e) g[e] = wrap(F.geom, e, g[e]);
-                        const int fb = find_box(F, g[0], g[1], g[2]);
-                        if (fb < 0) {
-                            set_error("ocfd_cf_create: a fine cell across an interface face has no box");
-                            return OCFD_EINVAL;
-                        }
-                        CFFace h{};
+                        CFFace h{};   // in fine box fb, which covers the coarse neighbour
                         h.box = fb; h.i = g[0]; h.j = g[1]; h.k = g[2]; h.dir = d;
                         h.up = side ? 0 : 1;   // fine level above the face: its low face
                         ff.push_back(h);
@@ -640,6 +638,7 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
     ocfd_cf c = new ocfd_cf_s;
     c->coarse = coarse; c->fine = fine;
     c->nface = (int)cf.size(); c->ncell = (int)cells.size();
+    c->h_cf = cf;
     c->ntile_avg = (int)ta.size();
     int rc;
     Staging stg;
@@ -678,13 +677,8 @@ int ocfd_cf_faces(ocfd_cf c, int* out)
         set_error("ocfd_cf_faces: invalid arguments");
         return OCFD_EINVAL;
     }
-    std::vector<CFFace> h(c->nface);
-    if (c->nface && hipMemcpy(h.data(), c->d_cf, sizeof(CFFace) * c->nface, hipMemcpyDeviceToHost) != hipSuccess) {
-        set_error("ocfd_cf_faces: copy failed");
-        return OCFD_EHIP;
-    }
     for (int f = 0; f < c->nface; ++f) {
-        const CFFace& x = h[f];
+        const CFFace& x = c->h_cf[f];
         const int v[6] = {x.box, x.i, x.j, x.k, x.dir, x.up};
         std::copy(v, v + 6, out + 6 * f);
     }

@@ -74,6 +74,12 @@ inline const int4* fill_tiles(const LevelDev& L, int& n)
     return L.fill_gz == 1 ? L.d_tile_gp : L.d_tile_g;
 }
 
+// A non-blocking stream of the current device for copies of freshly allocated
+// tables: a copy on the legacy default stream would wait for every kernel
+// already queued there (the AMR driver builds levels while the previous ones
+// still run).
+hipStream_t upload_stream();
+
 // Host tables gathered into one device allocation filled by one copy; each
 // table starts on a 256-byte boundary.  commit() sets every registered
 // pointer into the allocation (null for empty tables).
@@ -99,7 +105,8 @@ struct Staging {
             set_error("out of device memory for the level tables");
             return OCFD_ENOMEM;
         }
-        if (hipMemcpy(*base, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        if (hipMemcpyAsync(*base, host.data(), host.size(), hipMemcpyHostToDevice, upload_stream()) != hipSuccess ||
+            hipStreamSynchronize(upload_stream()) != hipSuccess) {
             set_error("copy of the level tables to the device failed");
             return OCFD_EHIP;
         }

@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -236,12 +238,28 @@ static const signed char scalar_bc[6]   = {BC_INT, BC_EXT, BC_FOEXT, BC_EVEN, BC
 static const signed char norm_vel_bc[6] = {BC_INT, BC_EXT, BC_FOEXT, BC_ODD,  BC_ODD,  BC_ODD};
 static const signed char tang_vel_bc[6] = {BC_INT, BC_EXT, BC_FOEXT, BC_EVEN, BC_EVEN, BC_ODD};
 
+hipStream_t upload_stream()
+{
+    static std::mutex mu;
+    static std::map<int, hipStream_t> streams;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = streams.find(dev);
+    if (it != streams.end()) return it->second;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    streams[dev] = s;
+    return s;
+}
+
 template <class T>
 static int upload(T** dptr, const std::vector<T>& h)
 {
     if (h.empty()) { *dptr = nullptr; return OCFD_OK; }
     OCFD_HIP(hipMalloc((void**)dptr, sizeof(T) * h.size()));
-    OCFD_HIP(hipMemcpy(*dptr, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    OCFD_HIP(hipMemcpyAsync(*dptr, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, upload_stream()));
+    OCFD_HIP(hipStreamSynchronize(upload_stream()));
     return OCFD_OK;
 }
 

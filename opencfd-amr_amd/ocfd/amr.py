@@ -278,6 +278,7 @@ class Hierarchy:
         for _ in range(max_level):
             self.geoms.append(refine_geom(self.geoms[-1]))
         n = max_level + 1
+        self._retired = []
         self.levels = [None] * n
         self.cfs = [None] * n
         self.bufs = [None] * n
@@ -294,9 +295,17 @@ class Hierarchy:
             self.start()
 
     # -- buffers ------------------------------------------------------------
+    def _retire(self, *objs):
+        """Keep replaced levels and registers alive until the next coarse step
+        starts: freeing device memory waits for the whole device, which is idle
+        then (the dt reductions have just synchronised) and busy during a
+        regrid."""
+        self._retired.extend(o for o in objs if o is not None)
+
     def _link(self, l, cf=None):
         if cf is None:
             cf = FluxRegister(self.levels[l - 1], self.levels[l])
+        self._retire(self.cfs[l])
         if (self.flags & PLANAR) and cf.nfaces and (cf.faces()[:, 4] == 2).any():
             raise ValueError("planar stepping needs fine grids that span the domain in z")
         self.cfs[l] = cf
@@ -306,10 +315,12 @@ class Hierarchy:
         cf: its register with level l-1 when already built."""
         if level is None:
             for m in range(l, self.max_level + 1):
+                self._retire(self.levels[m], self.cfs[m])
                 self.levels[m], self.bufs[m], self.cfs[m] = None, None, None
                 self.st[m] = LevelState()
             self.finest = l - 1
             return
+        self._retire(self.levels[l])
         self.levels[l] = level
         if self.flags & PLANAR:
             level.set_planar(True)
@@ -326,6 +337,7 @@ class Hierarchy:
         # the next finer level now hangs off a new level; it is relinked, or
         # rebuilt by the caller when it no longer nests
         if l + 1 <= self.max_level and self.levels[l + 1] is not None:
+            self._retire(self.cfs[l + 1])
             self.cfs[l + 1] = None
             try:
                 self._link(l + 1)
@@ -519,6 +531,7 @@ class Hierarchy:
     def step(self):
         """One coarse time step (Amr::coarseTimeStep)."""
         self._compute_dt()
+        self._retired.clear()
         t = self.time
         self._time_step(0, t)
         self.time = t + self.dt_level[0]
