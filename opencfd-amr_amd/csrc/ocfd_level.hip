@@ -1,6 +1,7 @@
 // Level set-up, ghost fill (FillPatch on one level), CFL reduction and the
 // C ABI of include/ocfd.h.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -75,6 +76,10 @@ __global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes,
         }
         outside[d] = iv[d] < G.dom_lo[d] ? -1 : (iv[d] > G.dom_hi[d] ? 1 : 0);
     }
+    // the BC maps mirror or clamp every component alike, so the source cell and
+    // its owner are normally found once for all five
+    int last[3] = {INT_MIN, INT_MIN, INT_MIN};
+    long so = -1, sst = 0;
     for (int n = 0; n < NC; ++n) {
         int src[3] = {iv[0], iv[1], iv[2]};
         double sign = 1.0;
@@ -95,10 +100,18 @@ __global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes,
         if (src[0] < G.dom_lo[0] || src[0] > G.dom_hi[0] || src[1] < G.dom_lo[1] || src[1] > G.dom_hi[1] ||
             src[2] < G.dom_lo[2] || src[2] > G.dom_hi[2])
             continue;
-        const int m = level_owner(G, boxes, src[0], src[1], src[2]);
-        if (m < 0) continue;   // not covered on this level (coarse/fine boundary)
-        const BoxDesc& O = boxes[m];
-        S[dst + n * B.nptg] = sign * S[sidx(O, src[0], src[1], src[2]) + n * O.nptg];
+        if (src[0] != last[0] || src[1] != last[1] || src[2] != last[2]) {
+            const int m = level_owner(G, boxes, src[0], src[1], src[2]);
+            so = -1;
+            if (m >= 0) {   // else not covered on this level (coarse/fine boundary)
+                const BoxDesc& O = boxes[m];
+                so = sidx(O, src[0], src[1], src[2]);
+                sst = O.nptg;
+            }
+            last[0] = src[0]; last[1] = src[1]; last[2] = src[2];
+        }
+        if (so < 0) continue;
+        S[dst + n * B.nptg] = sign * S[so + n * sst];
     }
     if (G.probtype == 1) {
         const double x = G.prob_lo[0] + (i + 0.5) * G.dx[0];
