@@ -91,15 +91,18 @@ struct CFFace {
 // reg[f] += w * F(face f) for every listed face; one thread per face, each
 // slot written by one thread per launch, so the sums are deterministic
 template <class FT, bool VISC>
+// zrep: a z-uniform level whose sweeps computed the primitives of the first
+// z layer only; every x or y face takes the flux of its first-layer twin
 __global__ __launch_bounds__(128) void k_cf_flux(const BoxDesc* __restrict__ boxes, const CFFace* __restrict__ faces,
                                                  int nface, const void* prim, PhysArgs ph, double w,
-                                                 double* __restrict__ reg)
+                                                 double* __restrict__ reg, int zrep)
 {
     const int f = blockIdx.x * 128 + threadIdx.x;
     if (f >= nface) return;
     const CFFace c = faces[f];
     const BoxDesc B = boxes[c.box];
-    const int i = c.i + (c.up && c.dir == 0), j = c.j + (c.up && c.dir == 1), k = c.k + (c.up && c.dir == 2);
+    const int i = c.i + (c.up && c.dir == 0), j = c.j + (c.up && c.dir == 1);
+    const int k = zrep && c.dir != 2 ? B.lo[2] : c.k + (c.up && c.dir == 2);
     const long s = sidx(B, i, j, k) - B.off_s + B.off_p;
     const FT* P = (const FT*)prim;
     double F[5];
@@ -715,13 +718,14 @@ int ocfd_cf_add(ocfd_cf c, int fine_side, double w_dt, const ocfd_parm* parm, in
     const bool f32 = (flags & OCFD_FLUX_FP32) != 0, visc = (flags & OCFD_VISC) != 0;
     const void* prim = f32 ? (const void*)L.d_prim32 : (const void*)L.d_prim64;
     const dim3 g((n + 127) / 128), blk(128);
+    const int zrep = (flags & OCFD_PLANAR) && L.fill_gz == 1;   // as the level's planar sweeps
     hipStream_t s = (hipStream_t)stream;
     if (f32) {
-        if (visc) hipLaunchKernelGGL((k_cf_flux<float, true>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg);
-        else hipLaunchKernelGGL((k_cf_flux<float, false>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg);
+        if (visc) hipLaunchKernelGGL((k_cf_flux<float, true>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg, zrep);
+        else hipLaunchKernelGGL((k_cf_flux<float, false>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg, zrep);
     } else {
-        if (visc) hipLaunchKernelGGL((k_cf_flux<double, true>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg);
-        else hipLaunchKernelGGL((k_cf_flux<double, false>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg);
+        if (visc) hipLaunchKernelGGL((k_cf_flux<double, true>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg, zrep);
+        else hipLaunchKernelGGL((k_cf_flux<double, false>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg, zrep);
     }
     return launched("ocfd_cf_add");
 }

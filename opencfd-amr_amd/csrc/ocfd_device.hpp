@@ -98,6 +98,7 @@ struct StageArgs {
     int flags;
     PhysArgs ph;
     const ocfd_step_ctl* ctl;    // device time control, or null (host-driven)
+    int zrep;                    // z-uniform level: sweep its first z layer only and copy the result to the others
 };
 
 // The control block's fields are written by single-thread kernels with plain
@@ -431,6 +432,26 @@ __device__ __forceinline__ void finish_cell(const StageArgs& a, const BoxDesc& B
         const long d0 = didx(B, i, j, k);
 #pragma unroll
         for (int n = 0; n < NC; ++n) a.acc[d0 + n * B.nptv] = L[n];
+    }
+}
+
+// z-uniform level: the stage result of cell (i,j,k) (first z layer) copied to
+// the box's other z layers
+__device__ __forceinline__ void copy_to_layers(const StageArgs& a, const BoxDesc& B, int i, int j, int k)
+{
+    if (a.snew) {
+        const long s0 = sidx(B, i, j, k), st = B.nptg, dz = (long)B.nxg * B.nyg;
+        for (int n = 0; n < NC; ++n) {
+            const double v = a.snew[s0 + n * st];
+            for (int kk = B.lo[2] + 1; kk <= B.hi[2]; ++kk) a.snew[s0 + (kk - k) * dz + n * st] = v;
+        }
+    } else {
+        const long d0 = didx(B, i, j, k);
+        const long dz = (long)(B.hi[0] - B.lo[0] + 1) * (B.hi[1] - B.lo[1] + 1);
+        for (int n = 0; n < NC; ++n) {
+            const double v = a.acc[d0 + n * B.nptv];
+            for (int kk = B.lo[2] + 1; kk <= B.hi[2]; ++kk) a.acc[d0 + (kk - k) * dz + n * B.nptv] = v;
+        }
     }
 }
 

@@ -48,7 +48,10 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
     const bool rowok = jr < ey;
     const int j = B.lo[1] - 3 + (rowok ? jr : 3);
     const int k = B.lo[2] - gz + kr;
-    const bool frow = rowok && j >= B.lo[1] && j <= B.hi[1] && k >= B.lo[2] && k <= B.hi[2];
+    // a z-uniform level (zrep) needs the first layer and its two neighbours
+    // only; the test is uniform over the block (one z plane per block)
+    if (a.zrep && (k < B.lo[2] - 1 || k > B.lo[2] + 1)) return;
+    const bool frow = rowok && j >= B.lo[1] && j <= B.hi[1] && k >= B.lo[2] && k <= (a.zrep ? B.lo[2] : B.hi[2]);
     const int i = B.lo[0] + t.y * XSEG - 3 + lane;
     const bool cellok = rowok && i <= B.hi[0] + 3;
     const FT gam = (FT)a.ph.gamma;
@@ -137,7 +140,7 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
     constexpr int O = D == 1 ? 2 : 1;
     const int i = B.lo[0] + t.y * 64 + lane;
     const int o = B.lo[O] + t.z * 4 + w;
-    if (i > B.hi[0] || o > B.hi[O]) return;
+    if (i > B.hi[0] || o > B.hi[O] || (a.zrep && o != B.lo[O])) return;
     const int m0 = B.lo[D] + (t.w & 0xfffff);
     const int m1 = min(m0 + (t.w >> 20) - 1, B.hi[D]);
     const FT gam = (FT)a.ph.gamma;
@@ -231,6 +234,7 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
             if (FINAL) {
                 const int c = f - 1;
                 finish_cell(a, B, i, D == 1 ? c : o, D == 1 ? o : c, L);
+                if (a.zrep) copy_to_layers(a, B, i, D == 1 ? c : o, D == 1 ? o : c);
             } else {
 #pragma unroll
                 for (int n = 0; n < 5; ++n) ((FT*)a.part)[d0 + n * B.nptv] = (FT)L[n];
@@ -287,12 +291,16 @@ static void launch_sweeps_t(const LevelDev& L, const StageArgs& a0, bool planar,
     StageArgs a = a0;
     a.prim = sizeof(FT) == 4 ? (const void*)L.d_prim32 : (const void*)L.d_prim64;
     a.part = (void*)L.d_acc;
+    // a planar level (one z ghost layer filled) holds z-uniform data: its
+    // sweeps take the first z layer of each box and copy the result to the rest
+    a.zrep = planar && L.fill_gz == 1;
     const int nx = planar ? L.ntile_xp : L.ntile_x;
     if (nx) hipLaunchKernelGGL((k_sweep_x<FT, VISC>), dim3(nx), dim3(256), 0, s, L.d_boxes,
                                planar ? L.d_tile_xp : L.d_tile_x, a);
     if (planar) {
-        if (L.ntile_y) hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, true>), dim3(L.ntile_y), dim3(256), 0, s,
-                                          L.d_boxes, L.d_tile_y, a);
+        // one wave per march tile when only the first z layer is swept
+        if (L.ntile_y) hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, true>), dim3(L.ntile_y), dim3(a.zrep ? 64 : 256),
+                                          0, s, L.d_boxes, L.d_tile_y, a);
     } else {
         if (L.ntile_y) hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, false>), dim3(L.ntile_y), dim3(256), 0, s,
                                           L.d_boxes, L.d_tile_y, a);
