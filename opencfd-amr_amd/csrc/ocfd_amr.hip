@@ -492,8 +492,8 @@ int launched(const char* what)
 
 void free_cf(ocfd_cf c)
 {
-    // the interface lists and the interpolation map live in c->d_arena
-    hipFree(c->d_arena); hipFree(c->d_regc); hipFree(c->d_regf);
+    // the interface lists, the interpolation map and the registers live in c->d_arena
+    hipFree(c->d_arena);
 }
 
 bool refined_by_2(const LevelDev& C, const LevelDev& F)
@@ -647,18 +647,13 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
     Staging stg;
     stg.add(&c->d_cf, cf); stg.add(&c->d_ff, ff); stg.add(&c->d_cells, cells); stg.add(&c->d_cellfaces, cellfaces);
     stg.add(&c->d_tile_avg, ta); stg.add(&c->d_cmap, cmap);
+    stg.reserve(&c->d_regc, 5 * (size_t)c->nface);
+    stg.reserve(&c->d_regf, 20 * (size_t)c->nface);
     if ((rc = stg.commit(&c->d_arena))) {
         set_error("ocfd_cf_create: device allocation failed");
         free_cf(c);
         delete c;
         return rc;
-    }
-    if (c->nface && (hipMalloc((void**)&c->d_regc, sizeof(double) * 5 * c->nface) != hipSuccess ||
-                     hipMalloc((void**)&c->d_regf, sizeof(double) * 20 * c->nface) != hipSuccess)) {
-        set_error("ocfd_cf_create: out of device memory");
-        free_cf(c);
-        delete c;
-        return OCFD_ENOMEM;
     }
     *out = c;
     return OCFD_OK;

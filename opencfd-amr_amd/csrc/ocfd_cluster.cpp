@@ -14,7 +14,6 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
-#include <list>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -38,22 +37,25 @@ struct Pt {
 
 enum CutKind { HOLE = 0, INFLECTION = 1, BISECT = 2, NONE = 3 };
 
+// A cluster: a range [b, e) of a point array and the range's bounding box.
+// Chopping partitions the range in place, so clusters never copy points.
 struct Cluster {
-    std::vector<Pt> pts;
+    size_t b = 0, e = 0;
     IBox box{};
-    void min_box()
+    void min_box(const std::vector<Pt>& pts)
     {
         for (int d = 0; d < 3; ++d) {
-            box.lo[d] = pts[0].v[d];
-            box.hi[d] = pts[0].v[d];
+            box.lo[d] = pts[b].v[d];
+            box.hi[d] = pts[b].v[d];
         }
-        for (const Pt& p : pts)
+        for (size_t q = b; q < e; ++q)
             for (int d = 0; d < 3; ++d) {
-                box.lo[d] = std::min(box.lo[d], p.v[d]);
-                box.hi[d] = std::max(box.hi[d], p.v[d]);
+                box.lo[d] = std::min(box.lo[d], pts[q].v[d]);
+                box.hi[d] = std::max(box.hi[d], pts[q].v[d]);
             }
     }
-    double eff() const { return (double)pts.size() / (double)box.volume(); }
+    size_t size() const { return e - b; }
+    double eff() const { return (double)size() / (double)box.volume(); }
 };
 
 // cut index along one direction from the signature hist[0..len-1] over [lo, lo+len-1]
@@ -99,12 +101,12 @@ int find_cut(const std::vector<int>& hist, int lo, CutKind& kind)
 }
 
 // split cluster c in two; returns the high part, c keeps the low part
-Cluster chop(Cluster& c)
+Cluster chop(Cluster& c, std::vector<Pt>& pts)
 {
     std::vector<int> hist[3];
     for (int d = 0; d < 3; ++d) hist[d].assign(c.box.hi[d] - c.box.lo[d] + 1, 0);
-    for (const Pt& p : c.pts)
-        for (int d = 0; d < 3; ++d) hist[d][p.v[d] - c.box.lo[d]]++;
+    for (size_t q = c.b; q < c.e; ++q)
+        for (int d = 0; d < 3; ++d) hist[d][pts[q].v[d] - c.box.lo[d]]++;
     CutKind kind[3];
     int cut[3];
     CutKind best = NONE;
@@ -121,13 +123,30 @@ Cluster chop(Cluster& c)
             minlen = m;
         }
     }
+    const int cd = cut[dir];
+    const auto mid = std::partition(pts.begin() + c.b, pts.begin() + c.e, [&](const Pt& p) { return p.v[dir] < cd; });
     Cluster hi;
-    std::vector<Pt> lo;
-    for (const Pt& p : c.pts) (p.v[dir] < cut[dir] ? lo : hi.pts).push_back(p);
-    c.pts.swap(lo);
-    c.min_box();
-    hi.min_box();
+    hi.b = (size_t)(mid - pts.begin());
+    hi.e = c.e;
+    c.e = hi.b;
+    c.min_box(pts);
+    hi.min_box(pts);
     return hi;
+}
+
+// Berger-Rigoutsos: clusters (in list order) chopped until each reaches eff;
+// a chopped cluster keeps its place and is checked again, its high part goes
+// to the end of the list
+void refine(std::vector<Cluster>& lst, std::vector<Pt>& pts, double eff)
+{
+    for (size_t q = 0; q < lst.size();) {
+        if (lst[q].eff() < eff) {
+            Cluster hi = chop(lst[q], pts);
+            lst.push_back(hi);
+        } else {
+            ++q;
+        }
+    }
 }
 
 // join boxes that share their extent in two directions and touch in the
@@ -273,7 +292,7 @@ extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigne
         bt = bt_store.data();
         bmp = mask ? bm_store.data() : nullptr;
     }
-    Cluster all;
+    std::vector<Pt> all;
     {
         // tagged points, skipping untagged runs eight bytes at a time
         const size_t nrow = (size_t)nb[1] * nb[2];
@@ -287,46 +306,43 @@ extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigne
                     std::memcpy(&w, row + i, 8);
                     if (!w) { i += 8; continue; }
                 }
-                if (row[i]) all.pts.push_back(Pt{{lo[0] / block + i, lo[1] / block + j, lo[2] / block + k}});
+                if (row[i]) all.push_back(Pt{{lo[0] / block + i, lo[1] / block + j, lo[2] / block + k}});
                 ++i;
             }
         }
     }
     std::vector<IBox> boxes;
-    if (!all.pts.empty()) {
-        all.min_box();
-        std::list<Cluster> lst;
-        lst.push_back(std::move(all));
-        for (auto it = lst.begin(); it != lst.end();) {
-            if (it->eff() < grid_eff) lst.push_back(chop(*it));
-            else ++it;
-        }
+    if (!all.empty()) {
+        std::vector<Cluster> lst(1);
+        lst[0].b = 0;
+        lst[0].e = all.size();
+        lst[0].min_box(all);
+        refine(lst, all, grid_eff);
         // clusters intersected with the mask region (ClusterList::intersect with
         // the proper nesting domain): a box partly outside is replaced by an
         // exact box cover of its part inside
-        std::list<Cluster> kept;
-        for (Cluster& c : lst) {
-            if (!mask) { kept.push_back(std::move(c)); continue; }
-            Cluster in;
+        std::vector<IBox> kept;
+        std::vector<Pt> in;
+        for (const Cluster& c : lst) {
+            if (!mask) { kept.push_back(c.box); continue; }
+            in.clear();
             for (int k = c.box.lo[2]; k <= c.box.hi[2]; ++k)
                 for (int j = c.box.lo[1]; j <= c.box.hi[1]; ++j)
                     for (int i = c.box.lo[0]; i <= c.box.hi[0]; ++i) {
                         const int ii = i - lo[0] / block, jj = j - lo[1] / block, kk = k - lo[2] / block;
-                        if (bmp[((size_t)kk * nb[1] + jj) * nb[0] + ii]) in.pts.push_back(Pt{{i, j, k}});
+                        if (bmp[((size_t)kk * nb[1] + jj) * nb[0] + ii]) in.push_back(Pt{{i, j, k}});
                     }
-            if ((long)in.pts.size() == c.box.volume()) { kept.push_back(std::move(c)); continue; }
-            if (in.pts.empty()) continue;
-            in.min_box();
-            std::list<Cluster> part;
-            part.push_back(std::move(in));
-            for (auto it = part.begin(); it != part.end();) {
-                if (it->eff() < 1.0) part.push_back(chop(*it));
-                else ++it;
-            }
-            for (Cluster& q : part) kept.push_back(std::move(q));
+            if ((long)in.size() == c.box.volume()) { kept.push_back(c.box); continue; }
+            if (in.empty()) continue;
+            std::vector<Cluster> part(1);
+            part[0].b = 0;
+            part[0].e = in.size();
+            part[0].min_box(in);
+            refine(part, in, 1.0);
+            for (const Cluster& q : part) kept.push_back(q.box);
         }
-        for (const Cluster& c : kept) {
-            IBox b = c.box;
+        for (const IBox& kb : kept) {
+            IBox b = kb;
             for (int d = 0; d < 3; ++d) {
                 b.lo[d] *= block;
                 b.hi[d] = (b.hi[d] + 1) * block - 1;

@@ -20,6 +20,8 @@ struct LevelDev {
     std::vector<int> owner_map;     // host copy of the owner map (tiles of 8^3 cells)
     std::vector<int> owner_list;    // host copy of its box lists
     void* d_arena = nullptr;        // one allocation holding the read-only tables below
+    void* d_bufs = nullptr;         // one allocation holding the stage scratch (d_acc, d_prim32, d_prim64, d_red)
+    bool own_tile_yz = false;       // d_tile_y / d_tile_z allocated on their own (after a chunk change)
     BoxDesc* d_boxes = nullptr;
     GeomDesc* d_geom = nullptr;
     int* d_map = nullptr;
@@ -81,22 +83,39 @@ inline const int4* fill_tiles(const LevelDev& L, int& n)
 hipStream_t upload_stream();
 
 // Host tables gathered into one device allocation filled by one copy; each
-// table starts on a 256-byte boundary.  commit() sets every registered
-// pointer into the allocation (null for empty tables).
+// table starts on a 256-byte boundary.  allocate() sets every registered
+// pointer into the allocation (null for empty tables); reserved space can be
+// written (put) between allocate() and copy(), for tables that hold device
+// pointers into the allocation itself.
 struct Staging {
     std::vector<char> host;
     std::vector<std::pair<void**, size_t>> slots;
+    size_t place(size_t bytes)
+    {
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + bytes);
+        return off;
+    }
     template <class T>
     void add(T** slot, const std::vector<T>& v)
     {
         *slot = nullptr;
         if (v.empty()) return;
-        const size_t off = (host.size() + 255) & ~(size_t)255;
-        host.resize(off + sizeof(T) * v.size());
+        const size_t off = place(sizeof(T) * v.size());
         std::memcpy(host.data() + off, v.data(), sizeof(T) * v.size());
         slots.push_back({(void**)slot, off});
     }
-    int commit(void** base)
+    template <class T>
+    size_t reserve(T** slot, size_t count)
+    {
+        *slot = nullptr;
+        const size_t off = place(sizeof(T) * count);
+        slots.push_back({(void**)slot, off});
+        return off;
+    }
+    template <class T>
+    void put(size_t off, const T& v) { std::memcpy(host.data() + off, &v, sizeof(T)); }
+    int allocate(void** base)
     {
         *base = nullptr;
         if (host.empty()) return OCFD_OK;
@@ -105,13 +124,23 @@ struct Staging {
             set_error("out of device memory for the level tables");
             return OCFD_ENOMEM;
         }
-        if (hipMemcpyAsync(*base, host.data(), host.size(), hipMemcpyHostToDevice, upload_stream()) != hipSuccess ||
+        for (auto& sl : slots) *sl.first = (char*)*base + sl.second;
+        return OCFD_OK;
+    }
+    int copy(void* base)
+    {
+        if (host.empty()) return OCFD_OK;
+        if (hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, upload_stream()) != hipSuccess ||
             hipStreamSynchronize(upload_stream()) != hipSuccess) {
             set_error("copy of the level tables to the device failed");
             return OCFD_EHIP;
         }
-        for (auto& sl : slots) *sl.first = (char*)*base + sl.second;
         return OCFD_OK;
+    }
+    int commit(void** base)
+    {
+        int rc = allocate(base);
+        return rc ? rc : copy(*base);
     }
 };
 

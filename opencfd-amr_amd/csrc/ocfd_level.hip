@@ -283,8 +283,8 @@ static int upload(T** dptr, const std::vector<T>& h)
 static void free_level(LevelDev& L)
 {
     // tables inside the arena are not freed on their own
-    hipFree(L.d_arena); hipFree(L.d_geom); hipFree(L.d_tile_y); hipFree(L.d_tile_z);
-    hipFree(L.d_acc); hipFree(L.d_red); hipFree(L.d_prim32); hipFree(L.d_prim64);
+    hipFree(L.d_arena); hipFree(L.d_bufs);
+    if (L.own_tile_yz) { hipFree(L.d_tile_y); hipFree(L.d_tile_z); }
     hipFree(L.d_ebq);
     for (int d = 0; d < 3; ++d) { hipFree(L.d_ebF[d]); hipFree(L.d_mF[d]); }
     hipFree(L.d_mq);
@@ -549,9 +549,9 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     for (int d = 0; d < 3; ++d) stg.add(&L.d_tile_mf[d], tmf[d]);
     stg.add(&L.d_tile_md, tmd);
     stg.add(&L.d_boxes, L.boxes);
-    if ((rc = stg.commit(&L.d_arena)) || (rc = upload(&L.d_tile_y, ty)) || (rc = upload(&L.d_tile_z, tz))) {
-        free_level(L); delete lev; return rc;
-    }
+    stg.add(&L.d_tile_y, ty); stg.add(&L.d_tile_z, tz);
+    const size_t geom_off = stg.reserve(&L.d_geom, 1);
+    if ((rc = stg.allocate(&L.d_arena))) { free_level(L); delete lev; return rc; }
     L.ntile_eq = (int)teq.size(); L.ntile_ed = (int)ted.size(); L.ntile_md = (int)tmd.size();
     for (int d = 0; d < 3; ++d) L.ntile_mf[d] = (int)tmf[d].size();
     for (int d = 0; d < 3; ++d) L.ntile_ef[d] = (int)tef[d].size();
@@ -566,15 +566,22 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     L.mf_prim = L.mf_state / NC * 6;
     G.map = L.d_map;
     G.map_list = L.d_map_list;
-    std::vector<GeomDesc> gv(1, G);
-    if ((rc = upload(&L.d_geom, gv))) { free_level(L); delete lev; return rc; }
-    if (hipMalloc((void**)&L.d_acc, sizeof(double) * L.mf_valid) != hipSuccess ||
-        hipMalloc((void**)&L.d_prim32, sizeof(float) * L.mf_prim) != hipSuccess ||
-        hipMalloc((void**)&L.d_prim64, sizeof(double) * L.mf_prim) != hipSuccess ||
-        hipMalloc((void**)&L.d_red, sizeof(unsigned long long)) != hipSuccess) {
+    stg.put(geom_off, G);
+    if ((rc = stg.copy(L.d_arena))) { free_level(L); delete lev; return rc; }
+    // stage scratch in one allocation: partial divergence, FP32 and FP64
+    // primitives, the reduction slot
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_p32 = al(sizeof(double) * L.mf_valid), o_p64 = o_p32 + al(sizeof(float) * L.mf_prim),
+                 o_red = o_p64 + al(sizeof(double) * L.mf_prim), nbufs = o_red + 256;
+    if (hipMalloc(&L.d_bufs, nbufs) != hipSuccess) {
+        L.d_bufs = nullptr;
         set_error("ocfd_level_create: out of device memory");
         free_level(L); delete lev; return OCFD_ENOMEM;
     }
+    L.d_acc = (double*)L.d_bufs;
+    L.d_prim32 = (float*)((char*)L.d_bufs + o_p32);
+    L.d_prim64 = (double*)((char*)L.d_bufs + o_p64);
+    L.d_red = (unsigned long long*)((char*)L.d_bufs + o_red);
     L.dev_bytes = (long)sizeof(int) * (long)(L.owner_map.size() + L.owner_list.size()) +
                   (long)sizeof(int4) * (L.ntile_x + L.ntile_xp + L.ntile_y + L.ntile_z + L.ntile_g + L.ntile_gp + L.ntile_c + L.ntile_p +
                                         L.ntile_pp + L.ntile_v) +
@@ -784,10 +791,13 @@ int ocfd_level_set_march_chunk(ocfd_level lev, int ch)
         return rc;
     }
     OCFD_HIP(hipDeviceSynchronize());   // no launch may still read the old lists
-    hipFree(L.d_tile_y);
-    hipFree(L.d_tile_z);
+    if (L.own_tile_yz) {   // else the old lists live in the level's table allocation
+        hipFree(L.d_tile_y);
+        hipFree(L.d_tile_z);
+    }
     L.d_tile_y = dy; L.ntile_y = (int)ty.size();
     L.d_tile_z = dz; L.ntile_z = (int)tz.size();
+    L.own_tile_yz = true;
     L.march_ch = ch;
     return OCFD_OK;
 }
