@@ -557,24 +557,36 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
         }
         return tile_owner(fmap.data(), flist.data(), fmn, CG.dom_lo, CG.map_shift, fcrs.data(), iv[0], iv[1], iv[2], 0);
     };
-    // candidates (coarse box, K, J, I), visited in coarse-box then k, j, i order
+    // candidates (coarse box, K, J, I), visited in coarse-box then k, j, i
+    // order: the cells face-adjacent to a coarsened fine box (an edge or corner
+    // neighbour of one box that owns a face is face-adjacent to another box).
+    // A side on a non-periodic domain boundary has none, nor does a side of a
+    // box spanning a periodic direction (its image is the box itself).
     std::vector<std::array<int, 4>> cand;
     for (const BoxDesc& B : fcrs)
-        for (int K = B.lo[2] - 1; K <= B.hi[2] + 1; ++K)
-            for (int J = B.lo[1] - 1; J <= B.hi[1] + 1; ++J)
-                for (int I = B.lo[0] - 1; I <= B.hi[0] + 1; ++I) {
-                    if (K >= B.lo[2] && K <= B.hi[2] && J >= B.lo[1] && J <= B.hi[1] && I > B.lo[0] && I < B.hi[0])
-                        I = B.hi[0];   // skip the box interior of this row
-                    int iv[3] = {I, J, K};
-                    bool in = true;
-                    for (int d = 0; d < 3; ++d) {
-                        iv[d] = wrap(CG, d, iv[d]);
-                        in = in && iv[d] >= CG.dom_lo[d] && iv[d] <= CG.dom_hi[d];
-                    }
-                    if (!in || covering(iv[0], iv[1], iv[2]) >= 0) continue;
-                    const int cb = find_box(C, iv[0], iv[1], iv[2]);
-                    if (cb >= 0) cand.push_back({cb, iv[2], iv[1], iv[0]});
-                }
+        for (int d = 0; d < 3; ++d) {
+            const int len = CG.dom_hi[d] - CG.dom_lo[d] + 1;
+            if (CG.periodic[d] && B.hi[d] - B.lo[d] + 1 >= len) continue;
+            for (int side = 0; side < 2; ++side) {
+                const int at = side ? B.hi[d] + 1 : B.lo[d] - 1;
+                if (!CG.periodic[d] && (at < CG.dom_lo[d] || at > CG.dom_hi[d])) continue;
+                int lo3[3] = {B.lo[0], B.lo[1], B.lo[2]}, hi3[3] = {B.hi[0], B.hi[1], B.hi[2]};
+                lo3[d] = hi3[d] = at;
+                for (int K = lo3[2]; K <= hi3[2]; ++K)
+                    for (int J = lo3[1]; J <= hi3[1]; ++J)
+                        for (int I = lo3[0]; I <= hi3[0]; ++I) {
+                            int iv[3] = {I, J, K};
+                            bool in = true;
+                            for (int e = 0; e < 3; ++e) {
+                                iv[e] = wrap(CG, e, iv[e]);
+                                in = in && iv[e] >= CG.dom_lo[e] && iv[e] <= CG.dom_hi[e];
+                            }
+                            if (!in || covering(iv[0], iv[1], iv[2]) >= 0) continue;
+                            const int cb = find_box(C, iv[0], iv[1], iv[2]);
+                            if (cb >= 0) cand.push_back({cb, iv[2], iv[1], iv[0]});
+                        }
+            }
+        }
     std::sort(cand.begin(), cand.end());
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     std::vector<CFFace> cf, ff;
