@@ -274,6 +274,18 @@ int ocfd_reactor_advance_fixed(long n, double* state, double dt, int nsteps, voi
 int ocfd_reactor_advance_adaptive(long n, double* state, double t_end, double h0, double rtol, double atol,
                                   int max_steps, int* steps_out, int* rejects_out, void* work, void* stream);
 
+/* The same integration in resumable pieces (no counterpart in the reference):
+ * begin sets every reactor's time to 0 and step to h0 (th: device double[2n],
+ * times then steps) and zeroes the counts; each chunk lets every reactor
+ * take at most `budget` substep attempts and stores into *remaining (device
+ * unsigned) how many reactors have not reached t_end.  Chunks in a row give
+ * results identical to ocfd_reactor_advance_adaptive; a served batch runs in
+ * short launches that do not hold a hardware queue for its whole integration. */
+int ocfd_reactor_adaptive_begin(long n, double* th, double h0, int* steps, int* rejects, void* stream);
+int ocfd_reactor_adaptive_chunk(long n, double* state, double* th, double t_end, double rtol, double atol,
+                                int max_steps, int budget, int* steps, int* rejects, unsigned int* remaining,
+                                void* stream);
+
 /* Temperature of each reactor (GET_T_GIVEN_EY, Source/CHEM/LiDryer.H:836-888). */
 int ocfd_reactor_temperature(long n, const double* state, double* T, void* stream);
 

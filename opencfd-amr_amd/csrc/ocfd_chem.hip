@@ -130,6 +130,59 @@ __global__ void k_counter_init(unsigned long long* c) { *c = 0ull; }
 // (atol rho + rtol |y2|); on acceptance keep the Richardson value 2 y2 - y1
 // (second order, densities clipped at 0); rescale h by clamp(0.9/sqrt(err), 0.2, 4).
 //
+// One attempt of an adaptive substep of a reactor: step doubling on the
+// linearised-implicit step; on acceptance t advances and the Richardson value
+// is kept, either way h is rescaled.  LPR = 2: the two lanes of a pair share
+// the work (the even lane takes the step of h, the odd lane the two of h/2)
+// and end with the same values.
+template <int LPR>
+__device__ __forceinline__ void adaptive_substep(bool odd, double* rhoi, double& E, double rho, double& t, double& h,
+                                                 int& acc, int& rej, double rtol, double atol, double t_end)
+{
+    if (t + h > t_end) h = t_end - t;
+    double r1[NS], r2[NS], E1, E2;
+    if (LPR == 2) {
+        // even: y1 = one step of h; odd: y2 = two steps of h/2 (one
+        // code path for both, so the step is inlined once)
+        double rr[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) rr[k] = rhoi[k];
+        double Ex = E;
+        const double hs = odd ? 0.5 * h : h;
+        for (int pass = 0; pass < (odd ? 2 : 1); ++pass) Ex += chem::cell_step(Ex, rr, hs);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const double other = __shfl_xor(rr[k], 1);
+            r1[k] = odd ? other : rr[k];
+            r2[k] = odd ? rr[k] : other;
+        }
+        const double Eo = __shfl_xor(Ex, 1);
+        E1 = odd ? Eo : Ex;
+        E2 = odd ? Ex : Eo;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) { r1[k] = rhoi[k]; r2[k] = rhoi[k]; }
+        E1 = E + chem::cell_step(E, r1, h);
+        E2 = E + chem::cell_step(E, r2, 0.5 * h);
+        E2 += chem::cell_step(E2, r2, 0.5 * h);
+    }
+    double err = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) err = fmax(err, fabs(r2[k] - r1[k]) / (atol * rho + rtol * fabs(r2[k])));
+    if (err <= 1.0) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) rhoi[k] = fmax(2.0 * r2[k] - r1[k], 0.0);
+        E = 2.0 * E2 - E1;
+        t += h;
+        ++acc;
+        const double f = err > 0.0 ? 0.9 / sqrt(err) : 4.0;
+        h *= fmin(4.0, fmax(0.2, f));
+    } else {
+        ++rej;
+        h *= fmax(0.2, 0.9 / sqrt(err));
+    }
+}
+
 // LPR = 2 (batches too small to fill the chip): two lanes per reactor, the
 // even lane takes the step of h, the odd lane the two steps of h/2 (its
 // first one alongside the even lane's step), then the pair swaps results and
@@ -164,50 +217,7 @@ __global__ __launch_bounds__(64) void k_reactor_adaptive(long n, double* __restr
         bool done = false, failed = false;
         if (t >= t_end) done = true;
         else if (acc + rej >= max_steps) { done = true; failed = true; }
-        if (!done) {
-            if (t + h > t_end) h = t_end - t;
-            double r1[NS], r2[NS], E1, E2;
-            if (LPR == 2) {
-                // even: y1 = one step of h; odd: y2 = two steps of h/2 (one
-                // code path for both, so the step is inlined once)
-                double rr[NS];
-#pragma unroll
-                for (int k = 0; k < NS; ++k) rr[k] = rhoi[k];
-                double Ex = E;
-                const double hs = odd ? 0.5 * h : h;
-                for (int pass = 0; pass < (odd ? 2 : 1); ++pass) Ex += chem::cell_step(Ex, rr, hs);
-#pragma unroll
-                for (int k = 0; k < NS; ++k) {
-                    const double other = __shfl_xor(rr[k], 1);
-                    r1[k] = odd ? other : rr[k];
-                    r2[k] = odd ? rr[k] : other;
-                }
-                const double Eo = __shfl_xor(Ex, 1);
-                E1 = odd ? Eo : Ex;
-                E2 = odd ? Ex : Eo;
-            } else {
-#pragma unroll
-                for (int k = 0; k < NS; ++k) { r1[k] = rhoi[k]; r2[k] = rhoi[k]; }
-                E1 = E + chem::cell_step(E, r1, h);
-                E2 = E + chem::cell_step(E, r2, 0.5 * h);
-                E2 += chem::cell_step(E2, r2, 0.5 * h);
-            }
-            double err = 0.0;
-#pragma unroll
-            for (int k = 0; k < NS; ++k) err = fmax(err, fabs(r2[k] - r1[k]) / (atol * rho + rtol * fabs(r2[k])));
-            if (err <= 1.0) {
-#pragma unroll
-                for (int k = 0; k < NS; ++k) rhoi[k] = fmax(2.0 * r2[k] - r1[k], 0.0);
-                E = 2.0 * E2 - E1;
-                t += h;
-                ++acc;
-                const double f = err > 0.0 ? 0.9 / sqrt(err) : 4.0;
-                h *= fmin(4.0, fmax(0.2, f));
-            } else {
-                ++rej;
-                h *= fmax(0.2, 0.9 / sqrt(err));
-            }
-        }
+        if (!done) adaptive_substep<LPR>(odd, rhoi, E, rho, t, h, acc, rej, rtol, atol, t_end);
         if (done) {
             if (!odd) {
 #pragma unroll
@@ -221,6 +231,62 @@ __global__ __launch_bounds__(64) void k_reactor_adaptive(long n, double* __restr
         }
     }
 }
+
+// Resumable form of the adaptive integration, for batches that must not hold
+// a hardware queue for their whole integration (a served reactor batch next
+// to flow cases): every reactor (a lane, or a pair of lanes) takes at most
+// `budget` substep attempts per launch from its saved time and step size
+// (th: t in the first n entries, h in the next n) and saves them again; the
+// reactors not yet at t_end are counted in *remaining.  The substeps are
+// those of k_reactor_adaptive, so the results are identical.
+template <int LPR>
+__global__ __launch_bounds__(64) void k_reactor_chunk(long n, double* __restrict__ st, double* __restrict__ th,
+                                                      double t_end, double rtol, double atol, int max_steps,
+                                                      int budget, int* __restrict__ acc_io, int* __restrict__ rej_io,
+                                                      unsigned int* __restrict__ remaining)
+{
+    const int lane = threadIdx.x & 63;
+    const bool odd = LPR == 2 && (lane & 1);
+    const long r = ((long)blockIdx.x * 64 + threadIdx.x) / LPR;   // the same for both lanes of a pair
+    if (r >= n) return;
+    int acc = acc_io[r], rej = rej_io[r];
+    double t = th[r], h = th[n + r];
+    if (acc < 0 || t >= t_end) return;   // stopped at max_steps, or done
+    double rhoi[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) rhoi[k] = st[(RROW_Y + k) * n + r];
+    double E = st[RROW_E * n + r];
+    const double rho = st[RROW_RHO * n + r];
+    bool failed = false;
+    for (int it = 0; it < budget; ++it) {
+        if (t >= t_end) break;
+        if (acc + rej >= max_steps) { failed = true; break; }
+        adaptive_substep<LPR>(odd, rhoi, E, rho, t, h, acc, rej, rtol, atol, t_end);
+    }
+    if (!odd) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) st[(RROW_Y + k) * n + r] = rhoi[k];
+        st[RROW_E * n + r] = E;
+        th[r] = t;
+        th[n + r] = h;
+        acc_io[r] = failed ? -1 : acc;
+        rej_io[r] = rej;
+        if (!failed && t < t_end) atomicAdd(remaining, 1u);
+    }
+}
+
+__global__ void k_reactor_chunk_init(long n, double* __restrict__ th, double h0, int* __restrict__ acc,
+                                     int* __restrict__ rej)
+{
+    for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+        th[r] = 0.0;
+        th[n + r] = h0;
+        acc[r] = 0;
+        rej[r] = 0;
+    }
+}
+
+__global__ void k_zero_u32(unsigned int* p) { *p = 0u; }
 
 __global__ void k_reactor_T(long n, const double* __restrict__ st, double* __restrict__ T)
 {
@@ -310,6 +376,42 @@ int ocfd_reactor_advance_adaptive(long n, double* state, double t_end, double h0
                            atol, max_steps, steps_out, rejects_out, ctr);
     }
     return launch_check("ocfd_reactor_advance_adaptive");
+}
+
+int ocfd_reactor_adaptive_begin(long n, double* th, double h0, int* steps, int* rejects, void* stream)
+{
+    if (n <= 0 || !th || !steps || !rejects || !(h0 > 0)) {
+        set_error("ocfd_reactor_adaptive_begin: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    const long nb = std::min<long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_reactor_chunk_init, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, n, th, h0, steps,
+                       rejects);
+    return launch_check("ocfd_reactor_adaptive_begin");
+}
+
+int ocfd_reactor_adaptive_chunk(long n, double* state, double* th, double t_end, double rtol, double atol,
+                                int max_steps, int budget, int* steps, int* rejects, unsigned int* remaining,
+                                void* stream)
+{
+    if (n <= 0 || !state || !th || !steps || !rejects || !remaining || t_end < 0 || rtol <= 0 || atol < 0 ||
+        max_steps <= 0 || budget <= 0) {
+        set_error("ocfd_reactor_adaptive_chunk: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, remaining);
+    int dev = 0, ncu = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const long lanes_full = 64L * 4 * std::max(ncu, 1);
+    if (n >= lanes_full)
+        hipLaunchKernelGGL(k_reactor_chunk<1>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, state, th, t_end,
+                           rtol, atol, max_steps, budget, steps, rejects, remaining);
+    else
+        hipLaunchKernelGGL(k_reactor_chunk<2>, dim3((unsigned)((n + 31) / 32)), dim3(64), 0, s, n, state, th, t_end,
+                           rtol, atol, max_steps, budget, steps, rejects, remaining);
+    return launch_check("ocfd_reactor_adaptive_chunk");
 }
 
 int ocfd_reactor_temperature(long n, const double* state, double* T, void* stream)

@@ -159,6 +159,9 @@ def lib():
         L.ocfd_cluster_tags.argtypes = [ip, ip, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte),
                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ip, ip]
         L.ocfd_level_set_planar.argtypes = [vp, ctypes.c_int]
+        L.ocfd_reactor_adaptive_begin.argtypes = [ctypes.c_long, vp, ctypes.c_double, vp, vp, vp]
+        L.ocfd_reactor_adaptive_chunk.argtypes = [ctypes.c_long, vp, vp, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
         L.ocfd_mask_grow.argtypes = [ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ip, ctypes.c_int]
         L.ocfd_lincomb.argtypes = [ctypes.c_long, ctypes.c_double, vp, ctypes.c_double, vp, vp, vp]
@@ -543,6 +546,21 @@ class Reactors:
         check(lib().ocfd_reactor_advance_adaptive(self.n, _ptr(self.state), t_end, h0, rtol, atol, max_steps,
                                                   _ptr(self.steps), _ptr(self.rejects), _ptr(self.work),
                                                   _stream(stream)), "ocfd_reactor_advance_adaptive")
+
+    def begin_chunks(self, h0=1e-8, stream=None):
+        """Resumable adaptive integration: per-reactor time 0 and step h0."""
+        import torch
+        self.th = torch.empty(2 * self.n, dtype=torch.float64, device="cuda")
+        self.remaining = torch.zeros(1, dtype=torch.int32, device="cuda")
+        check(lib().ocfd_reactor_adaptive_begin(self.n, _ptr(self.th), h0, _ptr(self.steps), _ptr(self.rejects),
+                                                _stream(stream)), "ocfd_reactor_adaptive_begin")
+
+    def chunk(self, t_end, budget=16, rtol=1e-4, atol=1e-9, max_steps=200000, stream=None):
+        """At most `budget` substep attempts per reactor; self.remaining (device)
+        then counts the reactors short of t_end."""
+        check(lib().ocfd_reactor_adaptive_chunk(self.n, _ptr(self.state), _ptr(self.th), t_end, rtol, atol,
+                                                max_steps, budget, _ptr(self.steps), _ptr(self.rejects),
+                                                _ptr(self.remaining), _stream(stream)), "ocfd_reactor_adaptive_chunk")
 
     def temperature(self, stream=None):
         import torch

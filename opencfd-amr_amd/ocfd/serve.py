@@ -123,34 +123,45 @@ class _FlowJob:
 
 
 class _ReactorJob:
-    def __init__(self, req, stream, torch):
+    """A reactor batch integrated in short resumable launches (a few substeps
+    per reactor each), so the batch shares its hardware queue with the flow
+    cases instead of holding it for its whole integration."""
+
+    def __init__(self, req, stream, torch, budget=16, ahead=8):
         self.req = req
         self.stream = stream
+        self.budget = budget
+        self.ahead = ahead     # launches queued per poll (launches after the end do nothing)
         T0, p0, phi = ignition_batch(req.n, req.seed)
         with torch.cuda.stream(stream):
             self.r = Reactors(T0, p0, phi, stream)
-            self.r.advance_adaptive(req.t_end, rtol=req.rtol, atol=req.atol, stream=stream)
+            self.r.begin_chunks(1e-8, stream)
+            self.host_rem = torch.zeros(1, dtype=torch.int32).pin_memory()
         self.event = torch.cuda.Event()
-        self.event.record(stream)
-        self.in_flight = True
+        self.in_flight = False
         self.cells = 0
 
     def launch(self, torch):
-        pass
+        for _ in range(self.ahead):
+            self.r.chunk(self.req.t_end, self.budget, rtol=self.req.rtol, atol=self.req.atol, stream=self.stream)
+        with torch.cuda.stream(self.stream):
+            self.host_rem.copy_(self.r.remaining, non_blocking=True)
+        self.event.record(self.stream)
+        self.in_flight = True
 
     def finished(self):
         if not self.event.query():
             return False
         self.in_flight = False
         self.steps = None
-        return True
+        return int(self.host_rem[0]) == 0
 
     def result(self):
         return self.r.state
 
     def release(self):
-        self.r = None
-        return ()
+        r, self.r = self.r, None
+        return (r,)
 
 
 class Server:
@@ -235,7 +246,7 @@ class Server:
                     del active[slot]
                     free_slots.append(slot)
                     self.records.append(rec)
-                elif not job.in_flight and isinstance(job, _FlowJob):
+                elif not job.in_flight:
                     job.launch(torch)
             if real_time and not active and not pending and nxt < len(order):
                 wait = requests[order[nxt]].arrival - (time.perf_counter() - t0)

@@ -156,3 +156,27 @@ def test_reactor_adaptive_lane_modes_agree(torch_cuda):
     assert np.array_equal(a, b)
     assert np.array_equal(small.steps.cpu().numpy(), big.steps.cpu().numpy()[:256])
     assert np.array_equal(small.rejects.cpu().numpy(), big.rejects.cpu().numpy()[:256])
+
+
+def test_reactor_chunks_equal_one_launch(torch_cuda):
+    """The resumable integration (a few substep attempts per reactor per
+    launch) gives the one-launch integration's states, step and rejection
+    counts bit for bit, and reports no reactor remaining at the end."""
+    import numpy as np
+    import ocfd
+    T0, p0, phi = ocfd.ignition_batch(300, seed=21)
+    one = ocfd.Reactors(T0, p0, phi)
+    one.advance_adaptive(2e-4, h0=1e-8, rtol=1e-4, atol=1e-9)
+    ch = ocfd.Reactors(T0, p0, phi)
+    ch.begin_chunks(1e-8)
+    launches = 0
+    while True:
+        ch.chunk(2e-4, budget=7, rtol=1e-4, atol=1e-9)
+        launches += 1
+        if int(ch.remaining.item()) == 0:
+            break
+        assert launches < 100000
+    assert launches > 1
+    assert np.array_equal(one.rows().cpu().numpy(), ch.rows().cpu().numpy())
+    assert np.array_equal(one.steps.cpu().numpy(), ch.steps.cpu().numpy())
+    assert np.array_equal(one.rejects.cpu().numpy(), ch.rejects.cpu().numpy())
