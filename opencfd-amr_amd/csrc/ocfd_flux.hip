@@ -127,8 +127,10 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
 }
 
 // Marching sweep along direction D (1 = y, 2 = z).  FINAL: this sweep
-// completes the divergence and finishes the cell.
-template <int D, class FT, bool VISC, bool FINAL>
+// completes the divergence and finishes the cell.  ZREP: a z-uniform level
+// whose sweep covers the first z layer and copies each finished cell to the
+// other layers (a template flag so the 3D instantiations keep their registers).
+template <int D, class FT, bool VISC, bool FINAL, bool ZREP = false>
 __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                      StageArgs a)
 {
@@ -140,7 +142,7 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
     constexpr int O = D == 1 ? 2 : 1;
     const int i = B.lo[0] + t.y * 64 + lane;
     const int o = B.lo[O] + t.z * 4 + w;
-    if (i > B.hi[0] || o > B.hi[O] || (a.zrep && o != B.lo[O])) return;
+    if (i > B.hi[0] || o > B.hi[O] || (ZREP && o != B.lo[O])) return;
     const int m0 = B.lo[D] + (t.w & 0xfffff);
     const int m1 = min(m0 + (t.w >> 20) - 1, B.hi[D]);
     const FT gam = (FT)a.ph.gamma;
@@ -234,7 +236,7 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
             if (FINAL) {
                 const int c = f - 1;
                 finish_cell(a, B, i, D == 1 ? c : o, D == 1 ? o : c, L);
-                if (a.zrep) copy_to_layers(a, B, i, D == 1 ? c : o, D == 1 ? o : c);
+                if (ZREP) copy_to_layers(a, B, i, D == 1 ? c : o, D == 1 ? o : c);
             } else {
 #pragma unroll
                 for (int n = 0; n < 5; ++n) ((FT*)a.part)[d0 + n * B.nptv] = (FT)L[n];
@@ -299,8 +301,12 @@ static void launch_sweeps_t(const LevelDev& L, const StageArgs& a0, bool planar,
                                planar ? L.d_tile_xp : L.d_tile_x, a);
     if (planar) {
         // one wave per march tile when only the first z layer is swept
-        if (L.ntile_y) hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, true>), dim3(L.ntile_y), dim3(a.zrep ? 64 : 256),
-                                          0, s, L.d_boxes, L.d_tile_y, a);
+        if (L.ntile_y && a.zrep)
+            hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, true, true>), dim3(L.ntile_y), dim3(64), 0, s, L.d_boxes,
+                               L.d_tile_y, a);
+        else if (L.ntile_y)
+            hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, true>), dim3(L.ntile_y), dim3(256), 0, s, L.d_boxes,
+                               L.d_tile_y, a);
     } else {
         if (L.ntile_y) hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, false>), dim3(L.ntile_y), dim3(256), 0, s,
                                           L.d_boxes, L.d_tile_y, a);
