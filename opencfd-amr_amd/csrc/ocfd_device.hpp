@@ -360,11 +360,12 @@ __device__ __forceinline__ FT lat(FT pR, FT pL, FT mR, FT mL) { return pR + pL -
 //   D=0: ta = lat(u,y) tb = lat(v,y) tc = lat(u,z) td = lat(w,z)
 //   D=1: ta = lat(u,x) tb = lat(v,x) tc = lat(v,z) td = lat(w,z)
 //   D=2: ta = lat(u,x) tb = lat(w,x) tc = lat(v,y) td = lat(w,y)
+// mutl, mutr: the Sutherland viscosities of the two cells (a march carries
+// the high cell's value to the next face).
 template <int D, class FT>
-__device__ __forceinline__ void visc_face(const Prim<FT>& L, const Prim<FT>& R, FT ta, FT tb, FT tc, FT td,
-                                          const double* dxinv, const PhysArgs& ph, FT* F)
+__device__ __forceinline__ void visc_face_mu(const Prim<FT>& L, const Prim<FT>& R, FT mutl, FT mutr, FT ta, FT tb,
+                                             FT tc, FT td, const double* dxinv, const PhysArgs& ph, FT* F)
 {
-    const FT mutl = sutherland(L.T, ph), mutr = sutherland(R.T, ph);
     const FT ktl = mutl * FT(ph.kOverMu), ktr = mutr * FT(ph.kOverMu);
     const FT dn = FT(dxinv[D]);
     const FT dT = (R.T - L.T) * dn;
@@ -406,6 +407,13 @@ __device__ __forceinline__ void visc_face(const Prim<FT>& L, const Prim<FT>& R, 
     F[UMY] += -t2;
     F[UMZ] += -t3;
     F[UEDEN] += FT(-0.5) * ((R.u + L.u) * t1 + (R.v + L.v) * t2 + (R.w + L.w) * t3 + (ktl + ktr) * dT);
+}
+
+template <int D, class FT>
+__device__ __forceinline__ void visc_face(const Prim<FT>& L, const Prim<FT>& R, FT ta, FT tb, FT tc, FT td,
+                                          const double* dxinv, const PhysArgs& ph, FT* F)
+{
+    visc_face_mu<D, FT>(L, R, sutherland(L.T, ph), sutherland(R.T, ph), ta, tb, tc, td, dxinv, ph, F);
 }
 
 // Gravity source (Source/Advance.cpp:331-342) and the stage combination of

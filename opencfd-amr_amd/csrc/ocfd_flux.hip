@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
                                                  StageArgs a)
 {
     __shared__ FT sf[4][10][64];
-    __shared__ FT sv[VISC ? 4 : 1][VISC ? 12 : 1][64];
+    __shared__ FT sv[VISC ? 4 : 1][VISC ? 13 : 1][64];
     if (ctl_done(a.ctl)) return;   // uniform over the block: before any barrier
     const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const BoxDesc B = boxes[t.x];
@@ -77,6 +77,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
                 // tangential neighbours' velocities from the state itself
                 // (their primitives are being written by other waves)
                 sv[w][0][lane] = q.T; sv[w][1][lane] = q.u; sv[w][2][lane] = q.v; sv[w][3][lane] = q.w;
+                sv[w][12][lane] = sutherland(q.T, a.ph);   // once per cell, read by both faces
                 FT u, v, ww;
                 c2vel_nb<FT>(a.scur, s0 + B.nxg, st, a.ph, u, v, ww);               // (i, j+1)
                 sv[w][4][lane] = u; sv[w][5][lane] = v;
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
             FT tb = lat(sv[w][5][lane], sv[w][5][lane - 1], sv[w][7][lane], sv[w][7][lane - 1]);
             FT tc = lat(sv[w][8][lane], sv[w][8][lane - 1], sv[w][10][lane], sv[w][10][lane - 1]);
             FT td = lat(sv[w][9][lane], sv[w][9][lane - 1], sv[w][11][lane], sv[w][11][lane - 1]);
-            visc_face<0, FT>(L, R, ta, tb, tc, td, B.dxinv, a.ph, F);
+            visc_face_mu<0, FT>(L, R, sv[w][12][lane - 1], sv[w][12][lane], ta, tb, tc, td, B.dxinv, a.ph, F);
         }
     }
     __syncthreads();
@@ -130,8 +131,11 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
 // completes the divergence and finishes the cell.  ZREP: a z-uniform level
 // whose sweep covers the first z layer and copies each finished cell to the
 // other layers (a template flag so the 3D instantiations keep their registers).
+// The FP32 forms of the 3D sweeps are held to 128 VGPRs: four waves per
+// SIMD hide the march's load latency (three waves measured about 19% slower).
 template <int D, class FT, bool VISC, bool FINAL, bool ZREP = false>
-__global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 && (D == 2 || !FINAL) ? 4 : 1)))
+void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                      StageArgs a)
 {
     if (ctl_done(a.ctl)) return;
@@ -173,10 +177,12 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
     }
     // viscous state of the low cell of the first face (m0-1)
     Prim<FT> PL;
+    FT muL = 0;
     FT lxpL_a = 0, lxpL_b = 0, lxmL_a = 0, lxmL_b = 0, lopL_a = 0, lopL_b = 0, lomL_a = 0, lomL_b = 0;
     if (VISC) {
         const long s = base - dstep;
         PL = ldprim<FT>(P, s, st);
+        muL = sutherland(PL.T, a.ph);
         FT u, v, w2;
         ldvel<FT>(P, s + 1, st, u, v, w2);
         lxpL_a = u; lxpL_b = D == 1 ? v : w2;
@@ -221,8 +227,10 @@ __global__ __launch_bounds__(256) void k_sweep_march(const BoxDesc* __restrict__
             FT tb = lat(xpb, lxpL_b, xmb, lxmL_b);
             FT tc = lat(opa, lopL_a, oma, lomL_a);
             FT td = lat(opb, lopL_b, omb, lomL_b);
-            visc_face<D, FT>(PL, PR, ta, tb, tc, td, B.dxinv, a.ph, F);
+            const FT muR = sutherland(PR.T, a.ph);
+            visc_face_mu<D, FT>(PL, PR, muL, muR, ta, tb, tc, td, B.dxinv, a.ph, F);
             PL = PR;
+            muL = muR;
             lxpL_a = xpa; lxpL_b = xpb; lxmL_a = xma; lxmL_b = xmb;
             lopL_a = opa; lopL_b = opb; lomL_a = oma; lomL_b = omb;
         }
