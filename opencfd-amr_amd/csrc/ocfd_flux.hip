@@ -29,40 +29,60 @@ namespace ocfd {
 constexpr int XSEG = 58;   // interior cells of one x-row wave
 constexpr int CH = 32;     // cells per march chunk (y and z sweeps); small levels use shorter ones
 
+// Orders the LDS accesses of one wave (each wave of the x sweep owns its own
+// LDS rows, so no block barrier is needed between its phases).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// x sweep over the primitive region grow(bx, (3,3,gz)) of one z plane, read
+// as one flattened run of rows (nx+6 cells each, y row after y row): wave
+// `win` takes cells [win*XSEG, win*XSEG+64) of the run, so a row's 6 halo
+// cells cost 6 lanes rather than a partly idle wave per row, and windows
+// straddle rows.  A face is taken by the window whose lanes 3..61 hold its
+// cell (its 6-cell stencil then lies inside the window and inside one row).
 template <class FT, bool VISC>
 __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                  StageArgs a)
 {
     __shared__ FT sf[4][10][64];
     __shared__ FT sv[VISC ? 4 : 1][VISC ? 13 : 1][64];
-    if (ctl_done(a.ctl)) return;   // uniform over the block: before any barrier
+    if (ctl_done(a.ctl)) return;
     const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const BoxDesc B = boxes[t.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
-    // rows of the primitive region grow(bx, (3,3,gz)); rows inside the box
-    // also carry the x faces
-    // t.z = (z row) * (y groups) + (y group): 4 consecutive y rows per block
-    const int gz = t.w, ey = ny + 6, ngy = (ey + 3) / 4;
-    const int jr = (t.z % ngy) * 4 + w, kr = t.z / ngy;
-    const bool rowok = jr < ey;
-    const int j = B.lo[1] - 3 + (rowok ? jr : 3);
-    const int k = B.lo[2] - gz + kr;
+    const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
+    // t.y = first window of the block, t.z = z row of the region, t.w = z growth
+    const int gz = t.w, R = nx + 6, ncell = (ny + 6) * R;
+    const int nwin = max(1, (ncell - 6 + XSEG - 1) / XSEG);
+    const int k = B.lo[2] - gz + t.z;
     // a z-uniform level (zrep) needs the first layer and its two neighbours
     // only; the test is uniform over the block (one z plane per block)
     if (a.zrep && (k < B.lo[2] - 1 || k > B.lo[2] + 1)) return;
-    const bool frow = rowok && j >= B.lo[1] && j <= B.hi[1] && k >= B.lo[2] && k <= (a.zrep ? B.lo[2] : B.hi[2]);
-    const int i = B.lo[0] + t.y * XSEG - 3 + lane;
-    const bool cellok = rowok && i <= B.hi[0] + 3;
+    const int win = __builtin_amdgcn_readfirstlane(t.y + w);
+    const int g0 = win * XSEG, jr0 = g0 / R;
+    // row and column of this lane's cell: off < R + 64, one exact correction
+    const int off = g0 - jr0 * R + lane;
+    int rq = (int)((float)off * __builtin_amdgcn_rcpf((float)R));
+    int ir = off - rq * R;
+    if (ir >= R) { ++rq; ir -= R; }
+    if (ir < 0) { --rq; ir += R; }
+    const int jr = jr0 + rq;
+    const bool cellok = win < nwin && g0 + lane < ncell;
+    const int i = B.lo[0] - 3 + ir, j = B.lo[1] - 3 + jr;
+    const bool frow = cellok && jr >= 3 && jr <= ny + 2 && k >= B.lo[2] && k <= (a.zrep ? B.lo[2] : B.hi[2]);
     const FT gam = (FT)a.ph.gamma;
 
     if (cellok) {
         const long s0 = sidx(B, i, j, k);
         const long st = B.nptg;
         const Prim<FT> q = c2prim<FT>(a.scur, s0, st, a.ph);
-        // each primitive is stored once: interior lanes, and the halo lanes
-        // that lie outside the box in x
-        if ((lane >= 3 && lane <= XSEG + 2) || i < B.lo[0] || i > B.hi[0]) {
+        // each primitive is stored once: by the first window that holds it
+        // in lanes 0..XSEG-1, the last window's other lanes by that window
+        if (lane < XSEG || win == nwin - 1) {
             FT* __restrict__ P = (FT*)a.prim;
             const long p0 = s0 - B.off_s + B.off_p;
             P[p0] = q.r; P[p0 + st] = q.u; P[p0 + 2 * st] = q.v; P[p0 + 3 * st] = q.w;
@@ -90,9 +110,9 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
             }
         }
     }
-    __syncthreads();
+    wave_sync();
 
-    const bool faceok = frow && lane >= 3 && lane <= 61 && i <= B.hi[0] + 1;
+    const bool faceok = frow && lane >= 3 && lane <= 61 && ir >= 3 && ir <= nx + 3;
     FT F[5];
     if (faceok) {
 #pragma unroll
@@ -112,13 +132,13 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
             visc_face_mu<0, FT>(L, R, sv[w][12][lane - 1], sv[w][12][lane], ta, tb, tc, td, B.dxinv, a.ph, F);
         }
     }
-    __syncthreads();
+    wave_sync();
     if (faceok) {
 #pragma unroll
         for (int n = 0; n < 5; ++n) sf[w][n][lane] = F[n];
     }
-    __syncthreads();
-    if (frow && lane >= 3 && lane <= 60 && i <= B.hi[0]) {
+    wave_sync();
+    if (frow && lane >= 3 && lane <= 60 && ir >= 3 && ir <= nx + 2) {
         const long d0 = didx(B, i, j, k);
         FT* __restrict__ part = (FT*)a.part;
 #pragma unroll

@@ -26,7 +26,7 @@ struct LevelDev {
     GeomDesc* d_geom = nullptr;
     int* d_map = nullptr;
     int* d_map_list = nullptr;
-    int4* d_tile_x = nullptr; int ntile_x = 0;     // x sweep: (box, segment, row group, z growth 3)
+    int4* d_tile_x = nullptr; int ntile_x = 0;     // x sweep: (box, first window, z row, z growth 3)
     int4* d_tile_xp = nullptr; int ntile_xp = 0;   // planar x sweep (z growth 1)
     int4* d_tile_y = nullptr; int ntile_y = 0;
     int4* d_tile_z = nullptr; int ntile_z = 0;

@@ -514,18 +514,20 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
             for (long s = 0; s < nf; s += 256) tmf[d].push_back(make_int4(b, (int)s, (int)std::min<long>(256, nf - s), d));
         }
         for (long s = 0; s < B.nptv; s += 256) tmd.push_back(make_int4(b, (int)s, (int)std::min<long>(256, B.nptv - s), 0));
-        // x sweep rows: the whole primitive region grow(bx, (3,3,gz)), blocks of
-        // 4 y rows.  Listed in slabs of 8 z planes, y group by y group, so the
-        // rows whose states a row's viscous faces read (y +- 1, z +- 1) are
-        // taken up close in time, while their lines are still in L2.
-        const int nseg = (nx + xseg - 1) / xseg, ngy = (ny + 6 + 3) / 4;
+        // x sweep: each z plane of the primitive region grow(bx, (3,3,gz)) is
+        // one flattened run of (ny+6) rows of nx+6 cells, cut into 64-cell
+        // windows that advance by xseg cells; a block takes 4 windows.  Listed
+        // in slabs of 8 z planes, window group by window group, so the rows
+        // whose states a row's viscous faces read (y +- 1, z +- 1) are taken
+        // up close in time, while their lines are still in L2.
+        const long nrun = (long)(ny + 6) * (nx + 6);
+        const int nwin = (int)std::max<long>(1, (nrun - 6 + xseg - 1) / xseg), ngw = (nwin + 3) / 4;
         for (int gz : {3, 1}) {
             const int ez = nz + 2 * gz;
             for (int k0 = 0; k0 < ez; k0 += 8)
-                for (int g = 0; g < ngy; ++g)
+                for (int g = 0; g < ngw; ++g)
                     for (int kr = k0; kr < std::min(k0 + 8, ez); ++kr)
-                        for (int s = 0; s < nseg; ++s)
-                            (gz == 3 ? tx : txp).push_back(make_int4(b, s, kr * ngy + g, gz));
+                        (gz == 3 ? tx : txp).push_back(make_int4(b, 4 * g, kr, gz));
         }
         // march tiles: i segment fastest, then chunk, then row group, so a
         // contiguous range of the list (one XCD's share) holds tiles that
