@@ -2,14 +2,15 @@
 // path, Source/Advance.cpp:231-342), one kernel per direction over the box
 // work-list of a whole level (or of many levels/cases at once).
 //
-//   sweep_x : wave = one x-row segment of 58 cells + 3 halo cells each side,
-//             over every row of the primitive region; a block holds 4
-//             consecutive y rows of one z plane.  Each lane recovers
-//             the primitives of its cell from the state (FP64, c2prim) and
-//             stores them for the y and z sweeps; on rows inside the box it
-//             also splits its cell (FT), the split fluxes are exchanged
-//             through LDS, lanes 3..61 evaluate the 59 faces, and the x-part
-//             of -div F is stored.
+//   sweep_x : each z plane of the primitive region is one flattened run of
+//             its rows; wave = 64 consecutive cells of that run (windows
+//             advance by 58 cells and may straddle rows), a block holds 4
+//             consecutive windows.  Each lane recovers the primitives of its
+//             cell from the state (FP64, c2prim) and stores them for the y
+//             and z sweeps; on rows inside the box it also splits its cell
+//             (FT), the split fluxes are exchanged through the wave's LDS
+//             rows, lanes 3..61 evaluate the faces whose stencil lies in one
+//             row, and the x-part of -div F is stored.
 //   sweep_y : lane = one i column, marches j through a chunk of cells with
 //             a register window of split fluxes (6 cells of f+, 5 of f-), and
 //             adds the y-part into the accumulator.
@@ -26,7 +27,7 @@
 
 namespace ocfd {
 
-constexpr int XSEG = 58;   // interior cells of one x-row wave
+constexpr int XSEG = 58;   // cells an x-sweep window advances (64 lanes less a 6-cell stencil halo)
 constexpr int CH = 32;     // cells per march chunk (y and z sweeps); small levels use shorter ones
 
 // Orders the LDS accesses of one wave (each wave of the x sweep owns its own
