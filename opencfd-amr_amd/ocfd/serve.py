@@ -171,6 +171,12 @@ class Server:
     slots: streams, i.e. the number of cases resident at once.
     chunk: device-driven time steps per graph launch of a flow case.
     keep_results: copy every finished case's state to the host (for checks).
+
+    HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by
+    default) and runs the kernels of one queue in order, so with more slots
+    than queues a small case waits behind another slot's kernels.  A serving
+    process should set GPU_MAX_HW_QUEUES to `slots` before the HIP runtime
+    starts (before the first GPU call).
     """
 
     def __init__(self, hbm_budget=64 << 30, slots=8, chunk=8, keep_results=False):
