@@ -17,6 +17,7 @@ serving form of its per-level advance (Source/Advance.cpp:81-177) and time
 loop (Source/EBR.cpp:189-306).
 """
 import time
+from collections import deque
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -70,7 +71,47 @@ class ReactorRequest:
         return self.n * (REACTOR_NCOMP * 8 + 3 * 8 + 2 * 4) + MiB
 
 
+class _Groups:
+    """Launch groups of one case in flight on its stream, each closed by a
+    copy of the case's progress to its own pinned buffer and an event.  With
+    depth 2 a second group is queued so the stream has work while the host
+    looks at the first; a group launched after the end does nothing on the
+    device."""
+
+    def __init__(self, torch, depth):
+        self.depth = depth
+        self.events = [torch.cuda.Event() for _ in range(depth)]
+        self.queue = deque()   # buffer indices in launch order
+        self.n = 0
+
+    def can_launch(self):
+        return len(self.queue) < self.depth
+
+    def next(self):
+        i = self.n % self.depth
+        self.n += 1
+        return i
+
+    def record(self, i, stream):
+        self.events[i].record(stream)
+        self.queue.append(i)
+
+    def completed(self):
+        """Buffer indices of the groups that completed since the last call, in order."""
+        out = []
+        while self.queue and self.events[self.queue[0]].query():
+            out.append(self.queue.popleft())
+        return out
+
+    def drained(self):
+        return not self.queue or all(self.events[i].query() for i in self.queue)
+
+
 class _FlowJob:
+    # two graphs of `chunk` steps queued (served mix, same box: p50 58 -> 57 ms,
+    # mean 163 -> 160 ms)
+    DEPTH = 2
+
     def __init__(self, req, stream, chunk, torch):
         self.req = req
         self.stream = stream
@@ -84,31 +125,31 @@ class _FlowJob:
             self.B = self.level.new_mf()
             self.C = self.level.new_mf()
             self.ctl = new_ctl()
-            self.host_ctl = torch.zeros(CTL_BYTES, dtype=torch.uint8).pin_memory()
+            # one pinned copy of the control block per launch group in flight
+            self.host = [torch.zeros(CTL_BYTES, dtype=torch.uint8).pin_memory() for _ in range(self.DEPTH)]
         stream.synchronize()   # the zero fills above ran on this stream
         self.level.init_state(kind, self.A, self.parm, req.seed)
         self.level.ctl_start(self.A, self.ctl, self.parm, req.cfl, 0.0, req.stop_time, req.max_steps, stream)
         self.graph = self.level.capture_steps(self.A, self.B, self.C, self.ctl, self.parm, chunk, req.scheme,
                                               req.flags, stream)
-        self.event = torch.cuda.Event()
-        self.in_flight = False
+        self.groups = _Groups(torch, self.DEPTH)
         self.cells = sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for lo, hi in boxes)
 
     def launch(self, torch):
+        i = self.groups.next()
         self.graph.launch(self.stream)
         with torch.cuda.stream(self.stream):
-            self.host_ctl.copy_(self.ctl, non_blocking=True)
-        self.event.record(self.stream)
-        self.in_flight = True
+            self.host[i].copy_(self.ctl, non_blocking=True)
+        self.groups.record(i, self.stream)
 
     def finished(self):
-        """True once the last launched chunk completed and the run is done."""
-        if not self.event.query():
-            return False
-        self.in_flight = False
-        c = StepCtl.from_buffer_copy(self.host_ctl.numpy().tobytes())
-        self.steps, self.time = c.steps, c.time
-        return bool(c.done)
+        """True once a completed launch group found the run done."""
+        done = False
+        for i in self.groups.completed():
+            c = StepCtl.from_buffer_copy(self.host[i].numpy().tobytes())
+            self.steps, self.time = c.steps, c.time
+            done = done or bool(c.done)
+        return done
 
     def result(self):
         return self.A
@@ -125,7 +166,11 @@ class _FlowJob:
 class _ReactorJob:
     """A reactor batch integrated in short resumable launches (a few substeps
     per reactor each), so the batch shares its hardware queue with the flow
-    cases instead of holding it for its whole integration."""
+    cases instead of holding it for its whole integration.  One group of
+    chunks at a time: a second queued group shortens the batches (p99 509 ->
+    440 ms) but makes the small flow cases wait behind them (p50 58 -> 74 ms)."""
+
+    DEPTH = 1
 
     def __init__(self, req, stream, torch, budget=16, ahead=8):
         self.req = req
@@ -136,25 +181,21 @@ class _ReactorJob:
         with torch.cuda.stream(stream):
             self.r = Reactors(T0, p0, phi, stream)
             self.r.begin_chunks(1e-8, stream)
-            self.host_rem = torch.zeros(1, dtype=torch.int32).pin_memory()
-        self.event = torch.cuda.Event()
-        self.in_flight = False
+            self.host = [torch.zeros(1, dtype=torch.int32).pin_memory() for _ in range(self.DEPTH)]
+        self.groups = _Groups(torch, self.DEPTH)
         self.cells = 0
 
     def launch(self, torch):
+        i = self.groups.next()
         for _ in range(self.ahead):
             self.r.chunk(self.req.t_end, self.budget, rtol=self.req.rtol, atol=self.req.atol, stream=self.stream)
         with torch.cuda.stream(self.stream):
-            self.host_rem.copy_(self.r.remaining, non_blocking=True)
-        self.event.record(self.stream)
-        self.in_flight = True
+            self.host[i].copy_(self.r.remaining, non_blocking=True)
+        self.groups.record(i, self.stream)
 
     def finished(self):
-        if not self.event.query():
-            return False
-        self.in_flight = False
         self.steps = None
-        return int(self.host_rem[0]) == 0
+        return any(int(self.host[i][0]) == 0 for i in self.groups.completed())
 
     def result(self):
         return self.r.state
@@ -235,11 +276,16 @@ class Server:
                         pending.pop(pi)
                         admitted = True
                         break
-            # keep every flow case's stream fed, then collect finished cases
+            # keep every case's stream fed, collect finished cases, and free
+            # a finished case's slot and memory once its queued groups drained
             for slot, (job, rec) in list(active.items()):
-                if not job.in_flight:
-                    job.launch(torch)
-                if job.finished():
+                if "done" not in rec:
+                    while job.groups.can_launch():
+                        job.launch(torch)
+                    if not job.finished():
+                        while job.groups.can_launch():
+                            job.launch(torch)
+                        continue
                     now = time.perf_counter() - t0
                     rec["done"] = now
                     rec["latency"] = now - rec["arrival"]
@@ -247,13 +293,12 @@ class Server:
                     if self.keep_results:
                         with torch.cuda.stream(job.stream):
                             self.results[rec["name"]] = job.result().cpu()
+                    self.records.append(rec)
+                if job.groups.drained():
                     graveyard.append(job.release())
                     used -= rec["footprint"]
                     del active[slot]
                     free_slots.append(slot)
-                    self.records.append(rec)
-                elif not job.in_flight:
-                    job.launch(torch)
             if real_time and not active and not pending and nxt < len(order):
                 wait = requests[order[nxt]].arrival - (time.perf_counter() - t0)
                 if wait > 0:
