@@ -475,7 +475,7 @@ int find_box(const LevelDev& L, int i, int j, int k)
 
 int wrap(const GeomDesc& G, int d, int v)
 {
-    if (!G.periodic[d]) return v;
+    if (!G.periodic[d] || (v >= G.dom_lo[d] && v <= G.dom_hi[d])) return v;
     const int len = G.dom_hi[d] - G.dom_lo[d] + 1;
     return G.dom_lo[d] + (((v - G.dom_lo[d]) % len) + len) % len;
 }

@@ -110,7 +110,10 @@ def copy_level(dst, S_dst, src, S_src, stream=None):
 
 def _grow(mask, n, periodic, shrink=False):
     """mask grown (or shrunk) by n cells, Chebyshev distance; periodic axes
-    wrap, other sides see unset (grow) or set (shrink) cells."""
+    wrap, other sides see unset (grow) or set (shrink) cells.  n == 0: the
+    mask itself as bool (no copy when it already is one)."""
+    if n == 0:
+        return np.asarray(mask, dtype=bool)
     a = np.ascontiguousarray(mask, dtype=np.uint8).copy()
     if n > 0:
         nz, ny, nx = a.shape
