@@ -94,6 +94,32 @@ def test_compute_dsdt(torch_cuda, orc, kind, visc, grav, fp32):
     assert worst < tol, worst
 
 
+@pytest.mark.parametrize("max_grid", [(2, 2, 8), (6, 4, 4), (24, 2, 2)])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_compute_dsdt_small_boxes(torch_cuda, orc, max_grid, fp32):
+    """Boxes whose x rows (nx + 6 cells) are shorter than a wave: the x
+    sweep's 64-cell windows straddle several rows and box edges."""
+    import ocfd
+    from ocfd import cases
+    boxes, g, _ = cases.kh3d(24, max_grid=max_grid)
+    lev = ocfd.Level(boxes, g)
+    rng = np.random.default_rng(13)
+    fabs = orc.new_level_fabs(boxes)
+    for f in fabs:
+        gu.random_state(f, rng, "rough")
+    S = lev.new_mf()
+    gu.upload(lev, S, fabs)
+    D = lev.new_mf(ngrow=0)
+    p = ocfd.default_parm()
+    lev.compute_dsdt(S, D, p, ocfd.VISC | (ocfd.FLUX_FP32 if fp32 else 0))
+    got = gu.download(lev, D, ngrow=0)
+    op = orc.parm()
+    dxinv = [1.0 / g.dx[d] for d in range(3)]
+    refs = [orc.compute_dsdt_box(lo, hi, fabs[b], op, dxinv, 1, 0)[0] for b, (lo, hi) in enumerate(boxes)]
+    worst = gu.level_max_rel(got, refs)
+    assert worst < (2e-5 if fp32 else TOL64), worst
+
+
 def test_planar_equals_full(torch_cuda, orc):
     """z-uniform data: skipping the z sweep gives the full 3D result."""
     import ocfd
