@@ -484,7 +484,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         }
         return std::max(ny_t, nz_t);
     };
-    while (ch > 8 && march_count(ch) < 2048) ch /= 2;
+    while (ch > 2 && march_count(ch) < 2048) ch /= 2;
     std::vector<int4> tx, txp, ty, tz, tg, tgp, tc, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
