@@ -165,10 +165,20 @@ void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ t
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // the "other" transverse direction: z for the y sweep, y for the z sweep
     constexpr int O = D == 1 ? 2 : 1;
-    const int i = B.lo[0] + t.y * 64 + lane;
-    const int o = B.lo[O] + t.z * 4 + w;
-    if (i > B.hi[0] || o > B.hi[O] || (ZREP && o != B.lo[O])) return;
-    const int m0 = B.lo[D] + (t.w & 0xfffff);
+    int i, o, m0;
+    if (ZREP) {
+        // first z layer only; sub-rows of 2^lw lanes, sub-row k on chunk k of the tile
+        const int lw = t.y >> 16, sub = lane >> lw;
+        i = B.lo[0] + ((t.y & 0xffff) << lw) + (lane & ((1 << lw) - 1));
+        o = B.lo[O];
+        m0 = B.lo[D] + (t.w & 0xfffff) + sub * (t.w >> 20);
+        if (i > B.hi[0] || m0 > B.hi[D]) return;
+    } else {
+        i = B.lo[0] + t.y * 64 + lane;
+        o = B.lo[O] + t.z * 4 + w;
+        m0 = B.lo[D] + (t.w & 0xfffff);
+        if (i > B.hi[0] || o > B.hi[O]) return;
+    }
     const int m1 = min(m0 + (t.w >> 20) - 1, B.hi[D]);
     const FT gam = (FT)a.ph.gamma;
     const long st = B.nptg;
@@ -330,10 +340,11 @@ static void launch_sweeps_t(const LevelDev& L, const StageArgs& a0, bool planar,
                                planar ? L.d_tile_xp : L.d_tile_x, a);
     if (planar) {
         // one wave per march tile when only the first z layer is swept
-        if (L.ntile_y && a.zrep)
-            hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, true, true>), dim3(L.ntile_y), dim3(64), 0, s, L.d_boxes,
-                               L.d_tile_y, a);
-        else if (L.ntile_y)
+        if (a.zrep) {
+            if (L.ntile_yp)
+                hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, true, true>), dim3(L.ntile_yp), dim3(64), 0, s,
+                                   L.d_boxes, L.d_tile_y + L.ntile_y, a);
+        } else if (L.ntile_y)
             hipLaunchKernelGGL((k_sweep_march<1, FT, VISC, true>), dim3(L.ntile_y), dim3(256), 0, s, L.d_boxes,
                                L.d_tile_y, a);
     } else {

@@ -327,7 +327,12 @@ int ocfd_parm_initialize(ocfd_parm* p)
 // then chunk, then row group, so a contiguous range of the list (one XCD's
 // share) holds tiles that share chunk halos and transverse neighbour rows;
 // .w = first cell of the chunk (low 20 bits) | chunk length << 20
-static void march_tiles(const LevelDev& L, int ch, std::vector<int4>& ty, std::vector<int4>& tz)
+// ty holds the 3D y-march tiles, then (from index ny3) those of the planar
+// first-layer y march: one wave per tile, packing 64 / w sub-rows of w lanes
+// (w the power of two >= the box width, at most 64), sub-row k taking chunk
+// c0 + k of the same x segment, so narrow boxes do not leave lanes idle;
+// .y = x segment | log2(w) << 16
+static void march_tiles(const LevelDev& L, int ch, std::vector<int4>& ty, std::vector<int4>& tz, int& ny3)
 {
     ty.clear();
     tz.clear();
@@ -342,6 +347,17 @@ static void march_tiles(const LevelDev& L, int ch, std::vector<int4>& ty, std::v
         for (int jg = 0; jg < (ny + 3) / 4; ++jg)
             for (int c = 0; c < (nz + ch - 1) / ch; ++c)
                 for (int s = 0; s < iseg; ++s) tz.push_back(make_int4(b, s, jg, (c * ch) | (ch << 20)));
+    }
+    ny3 = (int)ty.size();
+    for (int b = 0; b < L.nbox; ++b) {
+        const BoxDesc& B = L.boxes[b];
+        if (B.type != 0) continue;
+        const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
+        int lw = 1;
+        while (lw < 6 && (1 << lw) < nx) ++lw;
+        const int nsub = 64 >> lw, iseg = (nx + (1 << lw) - 1) >> lw, nc = (ny + ch - 1) / ch;
+        for (int c0 = 0; c0 < nc; c0 += nsub)
+            for (int s = 0; s < iseg; ++s) ty.push_back(make_int4(b, s | (lw << 16), 0, (c0 * ch) | (ch << 20)));
     }
 }
 
@@ -540,7 +556,8 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         }
     }
     L.march_ch = ch;
-    march_tiles(L, ch, ty, tz);
+    int ny3 = 0;
+    march_tiles(L, ch, ty, tz, ny3);
     int rc = OCFD_OK;
     Staging stg;
     stg.add(&L.d_map, L.owner_map); stg.add(&L.d_map_list, L.owner_list);
@@ -562,7 +579,8 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         for (int d = 0; d < 3 && ok; ++d) ok = hipMalloc((void**)&L.d_ebF[d], sizeof(double) * NC * L.mf_eb) == hipSuccess;
         if (!ok) { set_error("ocfd_level_create_eb: out of device memory"); free_level(L); delete lev; return OCFD_ENOMEM; }
     }
-    L.ntile_x = (int)tx.size(); L.ntile_xp = (int)txp.size(); L.ntile_y = (int)ty.size(); L.ntile_z = (int)tz.size();
+    L.ntile_x = (int)tx.size(); L.ntile_xp = (int)txp.size(); L.ntile_y = ny3; L.ntile_z = (int)tz.size();
+    L.ntile_yp = (int)ty.size() - ny3;
     L.ntile_g = (int)tg.size(); L.ntile_gp = (int)tgp.size(); L.ntile_c = (int)tc.size();
     L.ntile_p = (int)tp.size(); L.ntile_pp = (int)tpp.size(); L.ntile_v = (int)tv.size();
     L.mf_prim = L.mf_state / NC * 6;
@@ -585,7 +603,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     L.d_prim64 = (double*)((char*)L.d_bufs + o_p64);
     L.d_red = (unsigned long long*)((char*)L.d_bufs + o_red);
     L.dev_bytes = (long)sizeof(int) * (long)(L.owner_map.size() + L.owner_list.size()) +
-                  (long)sizeof(int4) * (L.ntile_x + L.ntile_xp + L.ntile_y + L.ntile_z + L.ntile_g + L.ntile_gp + L.ntile_c + L.ntile_p +
+                  (long)sizeof(int4) * (L.ntile_x + L.ntile_xp + L.ntile_y + L.ntile_yp + L.ntile_z + L.ntile_g + L.ntile_gp + L.ntile_c + L.ntile_p +
                                         L.ntile_pp + L.ntile_v) +
                   (long)sizeof(GeomDesc) + (long)sizeof(BoxDesc) * L.nbox + (long)sizeof(double) * L.mf_valid +
                   (long)(sizeof(float) + sizeof(double)) * L.mf_prim + (long)sizeof(unsigned long long) +
@@ -784,7 +802,8 @@ int ocfd_level_set_march_chunk(ocfd_level lev, int ch)
     }
     LevelDev& L = lev->dev;
     std::vector<int4> ty, tz;
-    march_tiles(L, ch, ty, tz);
+    int ny3 = 0;
+    march_tiles(L, ch, ty, tz, ny3);
     int4 *dy = nullptr, *dz = nullptr;
     int rc;
     if ((rc = upload(&dy, ty)) || (rc = upload(&dz, tz))) {
@@ -797,7 +816,7 @@ int ocfd_level_set_march_chunk(ocfd_level lev, int ch)
         hipFree(L.d_tile_y);
         hipFree(L.d_tile_z);
     }
-    L.d_tile_y = dy; L.ntile_y = (int)ty.size();
+    L.d_tile_y = dy; L.ntile_y = ny3; L.ntile_yp = (int)ty.size() - ny3;
     L.d_tile_z = dz; L.ntile_z = (int)tz.size();
     L.own_tile_yz = true;
     L.march_ch = ch;
