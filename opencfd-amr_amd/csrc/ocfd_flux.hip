@@ -335,7 +335,7 @@ static void launch_sweeps_t(const LevelDev& L, const StageArgs& a0, bool planar,
     // a planar level (one z ghost layer filled) holds z-uniform data: its
     // sweeps take the first z layer of each box and copy the result to the rest
     a.zrep = planar && L.fill_gz == 1;
-    const int nx = planar ? L.ntile_xp : L.ntile_x;
+    const int nx = a.zrep ? L.ntile_xz : (planar ? L.ntile_xp : L.ntile_x);
     if (nx) hipLaunchKernelGGL((k_sweep_x<FT, VISC>), dim3(nx), dim3(256), 0, s, L.d_boxes,
                                planar ? L.d_tile_xp : L.d_tile_x, a);
     if (planar) {

@@ -28,6 +28,7 @@ struct LevelDev {
     int* d_map_list = nullptr;
     int4* d_tile_x = nullptr; int ntile_x = 0;     // x sweep: (box, first window, z row, z growth 3)
     int4* d_tile_xp = nullptr; int ntile_xp = 0;   // planar x sweep (z growth 1)
+    int ntile_xz = 0;                               // its first ntile_xz tiles: the first z layer and its neighbours
     int4* d_tile_y = nullptr; int ntile_y = 0;   // 3D y march; then ntile_yp planar first-layer tiles
     int ntile_yp = 0;
     int4* d_tile_z = nullptr; int ntile_z = 0;

@@ -501,7 +501,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         return std::max(ny_t, nz_t);
     };
     while (ch > 2 && march_count(ch) < 2048) ch /= 2;
-    std::vector<int4> tx, txp, ty, tz, tg, tgp, tc, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
+    std::vector<int4> tx, txz, txp, ty, tz, tg, tgp, tc, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
         const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
@@ -538,13 +538,16 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         // up close in time, while their lines are still in L2.
         const long nrun = (long)(ny + 6) * (nx + 6);
         const int nwin = (int)std::max<long>(1, (nrun - 6 + xseg - 1) / xseg), ngw = (nwin + 3) / 4;
-        for (int gz : {3, 1}) {
-            const int ez = nz + 2 * gz;
-            for (int k0 = 0; k0 < ez; k0 += 8)
-                for (int g = 0; g < ngw; ++g)
-                    for (int kr = k0; kr < std::min(k0 + 8, ez); ++kr)
-                        (gz == 3 ? tx : txp).push_back(make_int4(b, 4 * g, kr, gz));
-        }
+        for (int k0 = 0; k0 < nz + 6; k0 += 8)
+            for (int g = 0; g < ngw; ++g)
+                for (int kr = k0; kr < std::min(k0 + 8, nz + 6); ++kr) tx.push_back(make_int4(b, 4 * g, kr, 3));
+        // planar (z growth 1): the first layer and its two neighbours, which a
+        // z-uniform level sweeps, for all boxes first (txz), then the other layers
+        for (int g = 0; g < ngw; ++g)
+            for (int kr = 0; kr < 3; ++kr) txz.push_back(make_int4(b, 4 * g, kr, 1));
+        for (int k0 = 3; k0 < nz + 2; k0 += 8)
+            for (int g = 0; g < ngw; ++g)
+                for (int kr = k0; kr < std::min(k0 + 8, nz + 2); ++kr) txp.push_back(make_int4(b, 4 * g, kr, 1));
         // march tiles: i segment fastest, then chunk, then row group, so a
         // contiguous range of the list (one XCD's share) holds tiles that
         // share chunk halos and transverse neighbour rows
@@ -555,6 +558,9 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
             for (long s = 0; s < np; s += 256) tl.push_back(make_int4(b, (int)s, (int)std::min<long>(256, np - s), gz));
         }
     }
+    L.ntile_xz = (int)txz.size();
+    txz.insert(txz.end(), txp.begin(), txp.end());
+    txp.swap(txz);
     L.march_ch = ch;
     int ny3 = 0;
     march_tiles(L, ch, ty, tz, ny3);
