@@ -233,15 +233,17 @@ def test_dmr_uneven_steps_fp64(torch_cuda):
     assert worst_p < TOL64, worst_p
 
 
+@pytest.mark.parametrize("nz", [2, 8])
 @pytest.mark.parametrize("flags_name", ["fp64", "fp32_visc"])
-def test_planar_fill_depth_gives_the_same_steps(torch_cuda, flags_name):
+def test_planar_fill_depth_gives_the_same_steps(torch_cuda, flags_name, nz):
     """A planar level that fills one z ghost layer (ocfd_level_set_planar)
     steps bit for bit like the same level filling all four; a stage without
     PLANAR is refused on it."""
     import ocfd
     from ocfd import cases
     flags = ocfd.PLANAR | (ocfd.FLUX_FP32 | ocfd.VISC if flags_name == "fp32_visc" else 0)
-    boxes, g, kind = cases.dmr(128, 32, 2, sizes=(8, 16, 32), seed=3)
+    # uneven boxes 8-32 wide (first-layer y tiles of 8, 16 and 32 lanes); 2 or 8 z layers
+    boxes, g, kind = cases.dmr(128, 32, nz, sizes=(8, 16, 32), seed=3)
     p = ocfd.default_parm()
     out = []
     for planar in (False, True):
