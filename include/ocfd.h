@@ -26,6 +26,7 @@ extern "C" {
 #define OCFD_EHIP         2
 #define OCFD_ENOMEM       3
 #define OCFD_EUNSUPPORTED 4
+#define OCFD_ENONFINITE   5   /* a state holds a non-finite value (NaN or inf) */
 
 /* flags for the flux entry points */
 #define OCFD_VISC       1   /* ebr.do_visc    (Source/EBR.cpp:25)  */
@@ -92,14 +93,16 @@ typedef struct ocfd_step_ctl {
     double stop_time;       /* < 0: none */
     int steps;              /* steps taken */
     int max_steps;          /* < 0: none */
-    int done;               /* set when stop_time or max_steps is reached */
-    int pad0;
+    int done;               /* set when stop_time or max_steps is reached, or on a non-finite state */
+    int status;             /* OCFD_OK, or OCFD_ENONFINITE once a step produced a non-finite cell */
     char pad1[80];
-    /* CFL reduction slot (bit pattern of a positive double), on a cache line
-     * of its own: it is only ever accessed with device-scope atomics, the
-     * fields above only with plain loads and stores */
+    /* CFL reduction slot (bit pattern of a positive double) and the
+     * non-finite flag of the reduction, on a cache line of their own: they
+     * are only ever accessed with device-scope atomics, the fields above only
+     * with plain loads and stores */
     unsigned long long red;
-    char pad2[120];
+    unsigned int bad;
+    char pad2[116];
 } ocfd_step_ctl;
 
 /* Parm defaults + Parm::Initialize (Source/Constants.H:9-29, Source/Constants.cpp:7-11).
@@ -182,21 +185,27 @@ int ocfd_level_set_planar(ocfd_level lev, int on);
 
 /* Tile shape of the y and z march sweeps: cells per march chunk (shorter
  * chunks give more independent waves at the cost of a longer halo prologue
- * per cell).  A level starts with 32, or 16 or 8 when it has few march tiles.
- * Results do not depend on it. */
+ * per cell).  A level starts with 32, halved down to 2 while it has fewer
+ * than 2048 march tiles.  Results do not depend on it.  Refused
+ * (OCFD_EINVAL) while step graphs of the level exist: a graph holds the tile
+ * lists' addresses and launch sizes it was captured with. */
 int ocfd_level_set_march_chunk(ocfd_level lev, int ch);
 int ocfd_level_march_chunk(ocfd_level lev);
 
 /* Autotuner of the march tile shape: times the y and z sweeps of this level
- * on state S (4 filled ghosts) for chunks of 8, 16, 32 and 64 cells (reps
- * launches each; ms_out[4] gets the mean ms per candidate when not NULL),
- * keeps the fastest and returns it in *ch_out.  Overwrites the level's
- * scratch only.  Synchronises the stream. */
+ * on state S (4 filled ghosts) for chunks of 2, 4, 8, 16, 32 and 64 cells
+ * (reps launches each; ms_out[6] gets the mean ms per candidate when not
+ * NULL), keeps the fastest and returns it in *ch_out.  With OCFD_PLANAR on a
+ * level with a one-layer fill it times the first-layer y march that such a
+ * stage runs.  Overwrites the level's scratch only.  Synchronises the stream.
+ * Refused while step graphs of the level exist. */
 int ocfd_level_autotune(ocfd_level lev, const double* S, const ocfd_parm* parm, int flags, int reps, int* ch_out,
                         double* ms_out, void* stream);
 
 /* EBR::estTimeStep (Source/Advance.cpp:61-79): cfl * min over valid cells of
- * dx_d/(|u_d|+c).  Synchronises the stream and writes the result to *dt_out. */
+ * dx_d/(|u_d|+c).  Synchronises the stream and writes the result to *dt_out.
+ * Returns OCFD_ENONFINITE (and still writes *dt_out) when a valid cell of S
+ * holds a NaN or inf, which the reference's min would pass over silently. */
 int ocfd_estdt(ocfd_level lev, const double* S, double cfl, const ocfd_parm* parm,
                double* dt_out, void* stream);
 
@@ -342,15 +351,21 @@ int ocfd_ctl_start(ocfd_level lev, const double* A, ocfd_step_ctl* ctl, double t
 /* One time step driven by ctl (the stage sequence of ocfd_advance for the
  * scheme, with dt and time read from ctl), followed by the CFL reduction of
  * the new state and the dt update of EBR::computeNewDt (change_max 1.1,
- * stop_time clip, Source/EBR.cpp:231-306).  Once ctl->done is set every kernel
- * of the step returns at once, so extra steps are no-ops.  No host
- * synchronisation, no allocation. */
+ * stop_time clip, Source/EBR.cpp:231-306).  On a level of regular boxes the
+ * reduction is taken by the sweep that finishes the last stage, as each new
+ * cell is written.  Once ctl->done is set every kernel of the step returns at
+ * once, so extra steps are no-ops.  A non-finite cell in the new state sets
+ * ctl->status = OCFD_ENONFINITE and ctl->done.  No host synchronisation, no
+ * allocation. */
 int ocfd_step_device(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
                      const ocfd_parm* parm, int flags, void* stream);
 
 /* hipGraph of nsteps device-driven time steps of one level, captured on
  * `stream` (which must not be the NULL stream) and replayed with
- * ocfd_graph_launch on any stream. */
+ * ocfd_graph_launch on any stream.  The graph holds the level's work-list
+ * addresses and launch sizes: while it exists the level refuses
+ * ocfd_level_set_march_chunk, ocfd_level_autotune and ocfd_level_destroy
+ * (OCFD_EINVAL); destroy the graphs first. */
 typedef struct ocfd_graph_s* ocfd_graph;
 int ocfd_graph_capture_steps(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
                              const ocfd_parm* parm, int flags, int nsteps, void* stream, ocfd_graph* out);

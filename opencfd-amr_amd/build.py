@@ -7,6 +7,7 @@ file so it ships with the source tree.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -42,17 +43,28 @@ def _run(cmd):
     return r.stdout
 
 
+def _jobs():
+    # at most 16 compilers at once, fewer where the machine has fewer cores
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n, int(os.environ.get("MAX_JOBS", "16"))))
+
+
 def build(verbose=False):
     BUILD.mkdir(exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in sources():
         obj = BUILD / (src.name + ".o")
         lang = ["-x", "hip"] if src.suffix == ".hip" else []
-        cmd = [HIPCC] + HIP_FLAGS + EXTRA.get(src.name, []) + lang + ["-c", str(src), "-o", str(obj)]
-        if verbose:
-            print(" ".join(cmd))
-        _run(cmd)
+        cmds.append([HIPCC] + HIP_FLAGS + EXTRA.get(src.name, []) + lang + ["-c", str(src), "-o", str(obj)])
         objs.append(str(obj))
+    if verbose:
+        for cmd in cmds:
+            print(" ".join(cmd))
+    with ThreadPoolExecutor(max_workers=_jobs()) as ex:
+        list(ex.map(_run, cmds))
     tmp = str(LIB) + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)

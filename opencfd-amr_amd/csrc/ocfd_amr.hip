@@ -53,6 +53,8 @@ __device__ __forceinline__ void face_flux(const FT* __restrict__ P, long s, cons
     for (int n = 0; n < 5; ++n)
         F[n] = weno_p<FT>(fp[0][n], fp[1][n], fp[2][n], fp[3][n], fp[4][n]) +
                weno_m<FT>(fm[1][n], fm[2][n], fm[3][n], fm[4][n], fm[5][n]);
+    if (redo_scaled(F))   // as the sweeps do
+        weno_face_scaled<FT>([&](int c, int n) { return fp[c][n]; }, [&](int c, int n) { return fm[c + 1][n]; }, F);
     if (VISC) {
         const long sL = s - ds;
         const Prim<FT> L = ldprim<FT>(P, sL, st), R = ldprim<FT>(P, s, st);

@@ -99,7 +99,39 @@ struct StageArgs {
     PhysArgs ph;
     const ocfd_step_ctl* ctl;    // device time control, or null (host-driven)
     int zrep;                    // z-uniform level: sweep its first z layer only and copy the result to the others
+    // CFL reduction folded into the stage that finishes a time step (null:
+    // none): atomic minimum of the cells' dt bit patterns, non-finite flag
+    unsigned long long* red;
+    unsigned int* bad;
 };
+
+// CFL time step of one cell, min_d dx_d/(|u_d| + c) (ebr_estdt,
+// Source/Advance.cpp:10-52), in the reference's operation order without FMA
+// contraction, so every kernel that reduces it gives the same bits.  With
+// dx = dy = dz one division serves: rounded division by a positive number is
+// monotone, so min_d dx/a_d equals dx/max_d a_d exactly.
+__device__ __forceinline__ double cell_dt(double rho, double mx, double my, double mz, double E, const PhysArgs& ph,
+                                          const double* dx, bool dx_equal)
+{
+#pragma clang fp contract(off)
+    const double rhoinv = 1.0 / fmax(rho, ph.smallr);
+    const double vx = mx * rhoinv;
+    const double vy = my * rhoinv;
+    const double vz = mz * rhoinv;
+    const double p = fmax((ph.gamma - 1.0) * (E - 0.5 * rho * (vx * vx + vy * vy + vz * vz)), ph.smallp);
+    const double cs = sqrt(ph.gamma * p * rhoinv);
+    const double ax = fabs(vx) + cs, ay = fabs(vy) + cs, az = fabs(vz) + cs;
+    if (dx_equal) return dx[0] / fmax(ax, fmax(ay, az));
+    return fmin(dx[0] / ax, fmin(dx[1] / ay, dx[2] / az));
+}
+
+// a cell whose conserved state holds a NaN or an inf
+__device__ __forceinline__ bool cell_nonfinite(double rho, double mx, double my, double mz, double E)
+{
+    return !__builtin_isfinite(rho + mx + my + mz + E);
+}
+
+__device__ __forceinline__ bool box_dx_equal(const BoxDesc& B) { return B.dx[0] == B.dx[1] && B.dx[1] == B.dx[2]; }
 
 // The control block's fields are written by single-thread kernels with plain
 // stores and read with plain loads by the following kernels, like any other
@@ -295,11 +327,63 @@ __device__ __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ double div_(double a, double b) { return a / b; }
 __device__ __forceinline__ float div_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
+// WENO-Z combination of the three candidate values v_k with the weights
+// alpha_k = d_k (1 + r_k^2), r_k = tau5/(eps + beta_k), d = (1, 6, 3)
+// (Source/kernel/Reconstruction.H:96-110), in the reference's operation order.
+__device__ __forceinline__ double wenoz_combine(double s11, double s22, double s33, double s55, double v1, double v2,
+                                                double v3)
+{
+    const double eps = 1e-6;
+    double r1 = s55 / (eps + s11), r2 = s55 / (eps + s22), r3 = s55 / (eps + s33);
+    double a1 = 1.0 + r1 * r1;
+    double a2 = 6.0 * (1.0 + r2 * r2);
+    double a3 = 3.0 * (1.0 + r3 * r3);
+    double invsum = 1.0 / (a1 + a2 + a3);
+    return (1.0 / 6.0) * invsum * (a1 * v1 + a2 * v2 + a3 * v3);
+}
+
+// FP32 form: the same arithmetic with hardware reciprocals.
+__device__ __forceinline__ float wenoz_combine(float s11, float s22, float s33, float s55, float v1, float v2,
+                                               float v3)
+{
+    const float eps = 1e-6f;
+    float r1 = s55 * __builtin_amdgcn_rcpf(eps + s11), r2 = s55 * __builtin_amdgcn_rcpf(eps + s22),
+          r3 = s55 * __builtin_amdgcn_rcpf(eps + s33);
+    float a1 = 1.0f + r1 * r1;
+    float a2 = 6.0f * (1.0f + r2 * r2);
+    float a3 = 3.0f * (1.0f + r3 * r3);
+    float invsum = __builtin_amdgcn_rcpf(a1 + a2 + a3);
+    return (1.0f / 6.0f) * invsum * (a1 * v1 + a2 * v2 + a3 * v3);
+}
+
+// The same weights in FP32 without overflow, for faces whose plain FP32 flux
+// came out non-finite: squaring r_k overflows single precision once
+// tau5/(eps + beta_k) exceeds about 1.8e19 -- a jump of a few 1e6 in a split
+// flux next to an exactly smooth sub-stencil, as in dimensional shear layers
+// and shocks (the 256^3 Kelvin-Helmholtz layer reaches it within four steps).
+// Each alpha_k is divided by (1 + r_max^2), r_max = tau5/z_min with
+// z_k = eps + beta_k: alpha_k' = d_k (g + u_k^2 (1 - g)), u_k = z_min/z_k in
+// (0, 1], g = 1/(1 + r_max^2) in [0, 1].  Every alpha_k' is at most 10 and the
+// largest u_k is 1, so nothing overflows whatever the scale of the data.
+__device__ __forceinline__ float wenoz_scaled(float s11, float s22, float s33, float s55, float v1, float v2, float v3)
+{
+    const float eps = 1e-6f;
+    const float z1 = eps + s11, z2 = eps + s22, z3 = eps + s33;
+    const float zm = fminf(z1, fminf(z2, z3));
+    const float u1 = zm * __builtin_amdgcn_rcpf(z1), u2 = zm * __builtin_amdgcn_rcpf(z2),
+                u3 = zm * __builtin_amdgcn_rcpf(z3);
+    const float rm = s55 * __builtin_amdgcn_rcpf(zm);
+    const float g = __builtin_amdgcn_rcpf(1.0f + rm * rm);
+    const float h = 1.0f - g;
+    const float a1 = g + u1 * u1 * h, a2 = 6.0f * (g + u2 * u2 * h), a3 = 3.0f * (g + u3 * u3 * h);
+    const float inv = __builtin_amdgcn_rcpf(a1 + a2 + a3);
+    return (1.0f / 6.0f) * ((a1 * inv) * v1 + (a2 * inv) * v2 + (a3 * inv) * v3);
+}
+
 // WENO-Z5 from f+ at cells (f-3..f+1) of face f, Source/kernel/Reconstruction.H:83-110
-template <class FT>
+template <class FT, bool SCALED = false>
 __device__ __forceinline__ FT weno_p(FT V1, FT V2, FT V3, FT V4, FT V5)
 {
-    const FT eps = FT(1e-6);
     const FT c13 = FT(13.0 / 12.0);
     FT t1 = V1 - FT(2.0) * V2 + V3, t2 = V1 - FT(4.0) * V2 + FT(3.0) * V3;
     FT s11 = c13 * t1 * t1 + FT(0.25) * t2 * t2;
@@ -308,22 +392,17 @@ __device__ __forceinline__ FT weno_p(FT V1, FT V2, FT V3, FT V4, FT V5)
     FT t5 = V3 - FT(2.0) * V4 + V5, t6 = FT(3.0) * V3 - FT(4.0) * V4 + V5;
     FT s33 = c13 * t5 * t5 + FT(0.25) * t6 * t6;
     FT s55 = fabs(s11 - s33);
-    FT r1 = div_(s55, eps + s11), r2 = div_(s55, eps + s22), r3 = div_(s55, eps + s33);
-    FT a1 = FT(1.0) + r1 * r1;
-    FT a2 = FT(6.0) * (FT(1.0) + r2 * r2);
-    FT a3 = FT(3.0) * (FT(1.0) + r3 * r3);
-    FT invsum = rcp_(a1 + a2 + a3);
     FT v1 = FT(2.0) * V1 - FT(7.0) * V2 + FT(11.0) * V3;
     FT v2 = -V2 + FT(5.0) * V3 + FT(2.0) * V4;
     FT v3 = FT(2.0) * V3 + FT(5.0) * V4 - V5;
-    return FT(1.0 / 6.0) * invsum * (a1 * v1 + a2 * v2 + a3 * v3);
+    if constexpr (SCALED && sizeof(FT) == 4) return wenoz_scaled(s11, s22, s33, s55, v1, v2, v3);
+    else return wenoz_combine(s11, s22, s33, s55, v1, v2, v3);
 }
 
 // WENO-Z5 from f- at cells (f-2..f+2) of face f, Source/kernel/Reconstruction.H:112-137
-template <class FT>
+template <class FT, bool SCALED = false>
 __device__ __forceinline__ FT weno_m(FT V1, FT V2, FT V3, FT V4, FT V5)
 {
-    const FT eps = FT(1e-6);
     const FT c13 = FT(13.0 / 12.0);
     FT t1 = V5 - FT(2.0) * V4 + V3, t2 = V5 - FT(4.0) * V4 + FT(3.0) * V3;
     FT s11 = c13 * t1 * t1 + FT(0.25) * t2 * t2;
@@ -332,15 +411,43 @@ __device__ __forceinline__ FT weno_m(FT V1, FT V2, FT V3, FT V4, FT V5)
     FT t5 = V3 - FT(2.0) * V2 + V1, t6 = FT(3.0) * V3 - FT(4.0) * V2 + V1;
     FT s33 = c13 * t5 * t5 + FT(0.25) * t6 * t6;
     FT s55 = fabs(s11 - s33);
-    FT r1 = div_(s55, eps + s11), r2 = div_(s55, eps + s22), r3 = div_(s55, eps + s33);
-    FT a1 = FT(1.0) + r1 * r1;
-    FT a2 = FT(6.0) * (FT(1.0) + r2 * r2);
-    FT a3 = FT(3.0) * (FT(1.0) + r3 * r3);
-    FT invsum = rcp_(a1 + a2 + a3);
     FT v1 = FT(11.0) * V3 - FT(7.0) * V4 + FT(2.0) * V5;
     FT v2 = -V4 + FT(5.0) * V3 + FT(2.0) * V2;
     FT v3 = FT(2.0) * V3 + FT(5.0) * V2 - V1;
-    return FT(1.0 / 6.0) * invsum * (a1 * v1 + a2 * v2 + a3 * v3);
+    if constexpr (SCALED && sizeof(FT) == 4) return wenoz_scaled(s11, s22, s33, s55, v1, v2, v3);
+    else return wenoz_combine(s11, s22, s33, s55, v1, v2, v3);
+}
+
+// Whether the FP32 form must redo a face's reconstructions with the scaled
+// weights: its flux (5 components) is not finite.  A sum that overflows only
+// sends a finite face through the scaled path, which gives the same weights.
+// Never in FP64.
+template <class FT>
+__device__ __forceinline__ bool redo_scaled(const FT* F)
+{
+    if (sizeof(FT) != 4) return false;
+    return __builtin_expect(!__builtin_isfinite(F[0] + F[1] + F[2] + F[3] + F[4]), 0);
+}
+
+// Convective face flux of the five components with the scaled weights, from
+// accessors of the split fluxes f+ at the face's cells -3..+1 (gp(c, n)) and
+// f- at -2..+2 (gm(c, n)).  The inputs pass through an empty asm so the
+// compiler recomputes the smoothness indicators here instead of keeping those
+// of the plain evaluation live across the (rare) branch.
+template <class FT, class GP, class GM>
+__device__ __forceinline__ void weno_face_scaled(GP gp, GM gm, FT* F)
+{
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+        FT p[5], m[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            p[c] = gp(c, n);
+            m[c] = gm(c, n);
+            asm volatile("" : "+v"(p[c]), "+v"(m[c]));
+        }
+        F[n] = weno_p<FT, true>(p[0], p[1], p[2], p[3], p[4]) + weno_m<FT, true>(m[0], m[1], m[2], m[3], m[4]);
+    }
 }
 
 // Sutherland viscosity, Source/kernel/Diffusion.H:21-22
@@ -418,7 +525,8 @@ __device__ __forceinline__ void visc_face(const Prim<FT>& L, const Prim<FT>& R, 
 
 // Gravity source (Source/Advance.cpp:331-342) and the stage combination of
 // MultiFab::LinComb / Saxpy (Source/Advance.cpp:117-148); with no output
-// state the divergence itself is stored (compute_dSdt).
+// state the divergence itself is stored (compute_dSdt).  With an output
+// state, L holds the cell's new state on return.
 __device__ __forceinline__ void finish_cell(const StageArgs& a, const BoxDesc& B, int i, int j, int k, double* L)
 {
     const long s0 = sidx(B, i, j, k);
@@ -434,12 +542,52 @@ __device__ __forceinline__ void finish_cell(const StageArgs& a, const BoxDesc& B
         for (int n = 0; n < NC; ++n) {
             double v = a.sold ? a.c_cur * a.scur[s0 + n * st] + a.c_old * a.sold[s0 + n * st]
                               : a.c_cur * a.scur[s0 + n * st];
-            a.snew[s0 + n * st] = v + cdt * L[n];
+            L[n] = v + cdt * L[n];
+            a.snew[s0 + n * st] = L[n];
         }
     } else {
         const long d0 = didx(B, i, j, k);
 #pragma unroll
         for (int n = 0; n < NC; ++n) a.acc[d0 + n * B.nptv] = L[n];
+    }
+}
+
+// Per-wave part of the CFL reduction folded into a finishing sweep: each lane
+// keeps the minimum dt bit pattern of its cells and whether one of them was
+// not finite; the lanes still running meet in the wave's LDS slot (which
+// tolerates lanes that left early), and one of them updates the global slot.
+struct CflFold {
+    unsigned long long m = 0xffffffffffffffffull;
+    bool nf = false;
+    __device__ __forceinline__ void cell(const StageArgs& a, const BoxDesc& B, const double* U)
+    {
+        if (cell_nonfinite(U[0], U[1], U[2], U[3], U[4])) { nf = true; return; }
+        const double dt = cell_dt(U[0], U[1], U[2], U[3], U[4], a.ph, B.dx, box_dx_equal(B));
+        const unsigned long long b = (unsigned long long)__double_as_longlong(dt);
+        m = b < m ? b : m;
+    }
+};
+
+__device__ __forceinline__ void cfl_wave_init(unsigned long long* slot, int lane)
+{
+    if (lane == 0) *slot = 0xffffffffffffffffull;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void cfl_wave_finish(const StageArgs& a, unsigned long long* slot, const CflFold& f)
+{
+    atomicMin(slot, f.m);
+    const bool any_nf = __any(f.nf);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const unsigned long long act = __ballot(1);
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)act) - 1) {
+        const unsigned long long m = *slot;
+        if (m != 0xffffffffffffffffull) atomicMin(a.red, m);
+        if (any_nf) atomicOr(a.bad, 1u);
     }
 }
 

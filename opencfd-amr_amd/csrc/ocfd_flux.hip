@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
                  + weno_m<FT>(sf[w][5 + n][lane - 2], sf[w][5 + n][lane - 1], sf[w][5 + n][lane], sf[w][5 + n][lane + 1],
                               sf[w][5 + n][lane + 2]);
         }
-        if (VISC) {
+        auto add_visc = [&](FT* Fo) {
             Prim<FT> L, R;
             L.T = sv[w][0][lane - 1]; L.u = sv[w][1][lane - 1]; L.v = sv[w][2][lane - 1]; L.w = sv[w][3][lane - 1];
             R.T = sv[w][0][lane];     R.u = sv[w][1][lane];     R.v = sv[w][2][lane];     R.w = sv[w][3][lane];
@@ -130,7 +130,15 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
             FT tb = lat(sv[w][5][lane], sv[w][5][lane - 1], sv[w][7][lane], sv[w][7][lane - 1]);
             FT tc = lat(sv[w][8][lane], sv[w][8][lane - 1], sv[w][10][lane], sv[w][10][lane - 1]);
             FT td = lat(sv[w][9][lane], sv[w][9][lane - 1], sv[w][11][lane], sv[w][11][lane - 1]);
-            visc_face_mu<0, FT>(L, R, sv[w][12][lane - 1], sv[w][12][lane], ta, tb, tc, td, B.dxinv, a.ph, F);
+            visc_face_mu<0, FT>(L, R, sv[w][12][lane - 1], sv[w][12][lane], ta, tb, tc, td, B.dxinv, a.ph, Fo);
+        };
+        if (VISC) add_visc(F);
+        // checked after the viscous part, so the face's LDS reads and
+        // arithmetic stay in one block; the rare redo recomputes both parts
+        if (redo_scaled(F)) {
+            weno_face_scaled<FT>([&](int c, int n) { return sf[w][n][lane - 3 + c]; },
+                                 [&](int c, int n) { return sf[w][5 + n][lane - 2 + c]; }, F);
+            if (VISC) add_visc(F);
         }
     }
     wave_sync();
@@ -159,10 +167,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(FT) 
 void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                      StageArgs a)
 {
+    __shared__ unsigned long long cflslot[FINAL ? 4 : 1];
     if (ctl_done(a.ctl)) return;
     const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const BoxDesc B = boxes[t.x];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (FINAL && a.red) cfl_wave_init(&cflslot[w], lane);   // before any lane leaves
     // the "other" transverse direction: z for the y sweep, y for the z sweep
     constexpr int O = D == 1 ? 2 : 1;
     int i, o, m0;
@@ -226,6 +236,7 @@ void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ t
     }
 
     FT Fprev[5];
+    CflFold cfl;
     for (int f = m0; f <= m1 + 1; ++f) {
         // load cell f+2
         {
@@ -240,6 +251,10 @@ void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ t
         for (int n = 0; n < 5; ++n)
             F[n] = weno_p<FT>(Wp[0][n], Wp[1][n], Wp[2][n], Wp[3][n], Wp[4][n])
                  + weno_m<FT>(Wm[0][n], Wm[1][n], Wm[2][n], Wm[3][n], Wm[4][n]);
+        // checked before the viscous part: the window's first cell is dead
+        // after this point, which keeps the march within its 128 VGPRs
+        if (redo_scaled(F))
+            weno_face_scaled<FT>([&](int c, int n) { return Wp[c][n]; }, [&](int c, int n) { return Wm[c][n]; }, F);
         if (VISC) {
             const long s = base + (long)(f - m0) * dstep;
             Prim<FT> PR = ldprim<FT>(P, s, st);
@@ -276,6 +291,7 @@ void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ t
                 const int c = f - 1;
                 finish_cell(a, B, i, D == 1 ? c : o, D == 1 ? o : c, L);
                 if (ZREP) copy_to_layers(a, B, i, D == 1 ? c : o, D == 1 ? o : c);
+                if (a.red) cfl.cell(a, B, L);
             } else {
 #pragma unroll
                 for (int n = 0; n < 5; ++n) ((FT*)a.part)[d0 + n * B.nptv] = (FT)L[n];
@@ -295,6 +311,7 @@ void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ t
             for (int n = 0; n < 5; ++n) Wm[c][n] = Wm[c + 1][n];
         }
     }
+    if (FINAL && a.red) cfl_wave_finish(a, &cflslot[w], cfl);
 }
 
 // Primitive recovery (c2prim, Source/kernel/Kernels.H:13-40) once per cell
@@ -369,7 +386,8 @@ void launch_sweeps(const LevelDev& L, const StageArgs& a, hipStream_t s)
     launch_eb(L, a, s);   // cut and covered boxes
 }
 
-// One sweep alone (0 = x, 1 = y, 2 = z final), for per-kernel timing.
+// One sweep alone (0 = x, 1 = y, 2 = z final, 4 = primitives, 5 = the first-layer y
+// march of a z-uniform level with a one-layer fill), for per-kernel timing.
 void launch_sweep_one(const LevelDev& L, const StageArgs& a0, int which, hipStream_t s)
 {
     const bool visc = (a0.flags & OCFD_VISC) != 0;
@@ -385,6 +403,10 @@ void launch_sweep_one(const LevelDev& L, const StageArgs& a0, int which, hipStre
                                            L.d_tile_x, a);                                                       \
         else if (which == 1) hipLaunchKernelGGL((k_sweep_march<1, FT, V, false>), dim3(L.ntile_y), dim3(256), 0, \
                                                 s, L.d_boxes, L.d_tile_y, a);                                     \
+        else if (which == 5) {                                                                                   \
+            if (L.ntile_yp) hipLaunchKernelGGL((k_sweep_march<1, FT, V, true, true>), dim3(L.ntile_yp), dim3(64), \
+                                               0, s, L.d_boxes, L.d_tile_y + L.ntile_y, a);                     \
+        }                                                                                                        \
         else hipLaunchKernelGGL((k_sweep_march<2, FT, V, true>), dim3(L.ntile_z), dim3(256), 0, s, L.d_boxes,    \
                                 L.d_tile_z, a);                                                                  \
     } while (0)

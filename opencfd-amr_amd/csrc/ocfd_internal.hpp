@@ -48,6 +48,7 @@ struct LevelDev {
     long mf_state = 0;              // doubles in a 4-ghost 5-comp MultiFab
     long dev_bytes = 0;             // device memory held by the level
     int march_ch = 32;              // cells per march chunk of the y and z sweeps
+    int live_graphs = 0;            // instantiated step graphs holding this level's work-lists and sizes
     // embedded boundary (cut boxes only, grown layout, box order)
     bool has_eb = false;
     ocfd_eb_spec eb_spec{};

@@ -136,22 +136,28 @@ __global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes,
 // ---------------------------------------------------------------------------
 // CFL reduction, Source/Advance.cpp:10-79
 // ---------------------------------------------------------------------------
-__global__ void k_red_init(unsigned long long* r)
+__global__ void k_red_init(unsigned long long* r, unsigned int* bad)
 {
     const double big = std::numeric_limits<double>::max();
     *r = (unsigned long long)__double_as_longlong(big);
+    *bad = 0;
 }
 
+// bad: set to 1 when a valid cell holds a NaN or inf (such cells do not
+// enter the minimum)
 __global__ __launch_bounds__(256) void k_estdt(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                const double* __restrict__ S, PhysArgs ph,
-                                               unsigned long long* __restrict__ out, const ocfd_step_ctl* ctl)
+                                               unsigned long long* __restrict__ out, unsigned int* __restrict__ bad,
+                                               const ocfd_step_ctl* ctl)
 {
     __shared__ double red[4];
     if (ctl_done(ctl)) return;
     const int4 t = tiles[blockIdx.x];
     const BoxDesc B = boxes[t.x];
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
+    const bool dxe = box_dx_equal(B);
     double dt = std::numeric_limits<double>::max();
+    int nf = 0;
     for (int c = threadIdx.x; c < t.z; c += 256) {
         const int g = t.y + c;
         const int i = B.lo[0] + g % nx;
@@ -159,29 +165,18 @@ __global__ __launch_bounds__(256) void k_estdt(const BoxDesc* __restrict__ boxes
         const int k = B.lo[2] + g / (nx * ny);
         const long s0 = sidx(B, i, j, k);
         const long st = B.nptg;
-        double rho = S[s0];
-        double mx = S[s0 + st];
-        double my = S[s0 + 2 * st];
-        double mz = S[s0 + 3 * st];
-        double E = S[s0 + 4 * st];
-        double rhoinv = 1.0 / fmax(rho, ph.smallr);
-        double vx = mx * rhoinv;
-        double vy = my * rhoinv;
-        double vz = mz * rhoinv;
-        double p = fmax((ph.gamma - 1.0) * (E - 0.5 * rho * (vx * vx + vy * vy + vz * vz)), ph.smallp);
-        double cs = sqrt(ph.gamma * p * rhoinv);
-        double dtx = B.dx[0] / (fabs(vx) + cs);
-        double dty = B.dx[1] / (fabs(vy) + cs);
-        double dtz = B.dx[2] / (fabs(vz) + cs);
-        dt = fmin(dt, fmin(dtx, fmin(dty, dtz)));
+        const double rho = S[s0], mx = S[s0 + st], my = S[s0 + 2 * st], mz = S[s0 + 3 * st], E = S[s0 + 4 * st];
+        if (cell_nonfinite(rho, mx, my, mz, E)) nf = 1;
+        else dt = fmin(dt, cell_dt(rho, mx, my, mz, E, ph, B.dx, dxe));
     }
     for (int off = 32; off > 0; off >>= 1) dt = fmin(dt, __shfl_xor(dt, off));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dt;
-    __syncthreads();
+    nf = __syncthreads_or(nf);
     if (threadIdx.x == 0) {
         double m = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
         // positive doubles order like their bit patterns
         atomicMin(out, (unsigned long long)__double_as_longlong(m));
+        if (nf) atomicOr(bad, 1u);
     }
 }
 
@@ -197,8 +192,18 @@ __device__ __forceinline__ unsigned long long red_big()
 __global__ void k_ctl_init(ocfd_step_ctl* c, double time, double cfl, double stop_time, int max_steps)
 {
     c->time = time; c->dt = 0.0; c->cfl = cfl; c->stop_time = stop_time;
-    c->steps = 0; c->max_steps = max_steps; c->done = 0; c->pad0 = 0;
+    c->steps = 0; c->max_steps = max_steps; c->done = 0; c->status = OCFD_OK;
     atomicExch(&c->red, red_big());
+    atomicExch(&c->bad, 0u);
+}
+
+// a non-finite cell seen by the last reduction ends the run with a status
+__device__ void ctl_check_bad(ocfd_step_ctl* c)
+{
+    if (atomicExch(&c->bad, 0u)) {
+        c->status = OCFD_ENONFINITE;
+        c->done = 1;
+    }
 }
 
 // dt_new from the CFL minimum, clipped so the run ends on stop_time; then the
@@ -223,6 +228,7 @@ __global__ void k_ctl_first(ocfd_step_ctl* c)
 #pragma clang fp contract(off)
     const double dt = red_value(atomicExch(&c->red, red_big())) * c->cfl;
     ctl_set_dt(c, dt, c->time, c->steps);
+    ctl_check_bad(c);
 }
 
 // after a step: t += dt, then dt_new = min(cfl*estdt, change_max*dt) with
@@ -235,6 +241,7 @@ __global__ void k_ctl_update(ocfd_step_ctl* c)
     const double dt = c->dt;
     const double time = c->time + dt;
     ctl_set_dt(c, fmin(est, 1.1 * dt), time, c->steps + 1);
+    ctl_check_bad(c);
 }
 
 // A <- B for the forward-Euler step under device control (a no-op once done)
@@ -622,6 +629,10 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
 int ocfd_level_destroy(ocfd_level lev)
 {
     if (!lev) return OCFD_OK;
+    if (lev->dev.live_graphs > 0) {
+        set_error("ocfd_level_destroy: step graphs of this level still exist (destroy them first)");
+        return OCFD_EINVAL;
+    }
     free_level(lev->dev);
     delete lev;
     return OCFD_OK;
@@ -719,17 +730,22 @@ int ocfd_estdt(ocfd_level lev, const double* S, double cfl, const ocfd_parm* par
     if (!lev || !S || !parm || !dt_out) { set_error("ocfd_estdt: invalid arguments"); return OCFD_EINVAL; }
     const LevelDev& L = lev->dev;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_red_init, dim3(1), dim3(1), 0, s, L.d_red);
+    unsigned int* bad = (unsigned int*)(L.d_red + 1);
+    hipLaunchKernelGGL(k_red_init, dim3(1), dim3(1), 0, s, L.d_red, bad);
     hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, S, phys_of(*parm), L.d_red,
-                       (const ocfd_step_ctl*)nullptr);
+                       bad, (const ocfd_step_ctl*)nullptr);
     int rc = check_launch("ocfd_estdt");
     if (rc) return rc;
-    unsigned long long bits = 0;
-    OCFD_HIP(hipMemcpyAsync(&bits, L.d_red, sizeof(bits), hipMemcpyDeviceToHost, s));
+    unsigned long long res[2] = {0, 0};   // the slot and the flag word after it
+    OCFD_HIP(hipMemcpyAsync(res, L.d_red, sizeof(res), hipMemcpyDeviceToHost, s));
     OCFD_HIP(hipStreamSynchronize(s));
     double m;
-    std::memcpy(&m, &bits, sizeof(m));
+    std::memcpy(&m, &res[0], sizeof(m));
     *dt_out = m * cfl;
+    if ((unsigned int)res[1]) {
+        set_error("ocfd_estdt: the state holds a non-finite value");
+        return OCFD_ENONFINITE;
+    }
     return OCFD_OK;
 }
 
@@ -807,6 +823,12 @@ int ocfd_level_set_march_chunk(ocfd_level lev, int ch)
         return OCFD_EINVAL;
     }
     LevelDev& L = lev->dev;
+    if (L.live_graphs > 0) {
+        // a captured graph holds the current lists' addresses and grid sizes
+        set_error("ocfd_level_set_march_chunk: step graphs of this level exist; destroy them before changing the "
+                  "tile shape");
+        return OCFD_EINVAL;
+    }
     std::vector<int4> ty, tz;
     int ny3 = 0;
     march_tiles(L, ch, ty, tz, ny3);
@@ -838,17 +860,25 @@ int ocfd_level_autotune(ocfd_level lev, const double* S, const ocfd_parm* parm, 
         set_error("ocfd_level_autotune: invalid arguments");
         return OCFD_EINVAL;
     }
-    static const int cand[4] = {8, 16, 32, 64};
+    static const int cand[6] = {2, 4, 8, 16, 32, 64};
     hipStream_t s = (hipStream_t)stream;
     LevelDev& L = lev->dev;
     if (!L.ntile_y || !L.ntile_p) {   // no regular boxes: nothing to tune
         *ch_out = L.march_ch;
         return OCFD_OK;
     }
+    if (L.live_graphs > 0) {
+        set_error("ocfd_level_autotune: step graphs of this level exist; destroy them before tuning");
+        return OCFD_EINVAL;
+    }
     StageArgs a{};
     a.scur = S; a.acc = L.d_acc;   // plain dSdt into level scratch: timing only
     a.c_cur = 1.0; a.flags = flags & ~OCFD_PLANAR; a.ph = phys_of(*parm);
     const bool planar = (flags & OCFD_PLANAR) != 0;
+    // a z-uniform level with a one-layer fill runs the packed first-layer y
+    // march, which is what is timed for it
+    const bool zrep = planar && L.fill_gz == 1;
+    a.zrep = zrep;
     hipEvent_t e0, e1;
     OCFD_HIP(hipEventCreate(&e0));
     OCFD_HIP(hipEventCreate(&e1));
@@ -856,12 +886,13 @@ int ocfd_level_autotune(ocfd_level lev, const double* S, const ocfd_parm* parm, 
     int best = L.march_ch;
     double best_ms = 1e300;
     int rc = OCFD_OK;
-    for (int c = 0; c < 4 && rc == OCFD_OK; ++c) {
+    const int ymarch = zrep ? 5 : 1;
+    for (int c = 0; c < 6 && rc == OCFD_OK; ++c) {
         if ((rc = ocfd_level_set_march_chunk(lev, cand[c]))) break;
-        launch_sweep_one(L, a, 1, s);   // warm-up of this tile list
+        launch_sweep_one(L, a, ymarch, s);   // warm-up of this tile list
         if (hipEventRecord(e0, s) != hipSuccess) { rc = OCFD_EHIP; break; }
         for (int r = 0; r < reps; ++r) {
-            launch_sweep_one(L, a, 1, s);
+            launch_sweep_one(L, a, ymarch, s);
             if (!planar) launch_sweep_one(L, a, 2, s);
         }
         float ms = 0.f;
@@ -896,7 +927,7 @@ int ocfd_ctl_start(ocfd_level lev, const double* A, ocfd_step_ctl* ctl, double t
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, ctl, time, cfl, stop_time, max_steps);
     hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, A, phys_of(*parm), &ctl->red,
-                       (const ocfd_step_ctl*)nullptr);
+                       &ctl->bad, (const ocfd_step_ctl*)nullptr);
     hipLaunchKernelGGL(k_ctl_first, dim3(1), dim3(1), 0, s, ctl);
     return check_launch("ocfd_ctl_start");
 }
@@ -914,30 +945,38 @@ static int step_device(ocfd_level lev, double* A, double* B, double* C, ocfd_ste
             hipLaunchKernelGGL(k_fill, dim3(ng), dim3(256), 0, s, L.d_boxes, L.d_geom, tg, S, 0.0,
                                (const ocfd_step_ctl*)ctl, toff);
     };
-    auto stage = [&](const double* cur, const double* old, double* nw, double cc, double co, double cdt) {
+    // the CFL reduction of the new state rides on the stage that finishes
+    // the step when every box is regular (the finishing sweep sees each new
+    // cell once); levels with cut or covered boxes reduce in a pass of their own
+    const bool fold = L.n_cut == 0 && L.n_covered == 0 && L.ntile_c > 0;
+    auto stage = [&](const double* cur, const double* old, double* nw, double cc, double co, double cdt,
+                     bool last = false) {
         a.scur = cur; a.sold = old; a.snew = nw; a.c_cur = cc; a.c_old = co; a.c_dt = cdt;
+        a.red = last && fold ? &ctl->red : nullptr;
+        a.bad = last && fold ? &ctl->bad : nullptr;
         launch_sweeps(L, a, s);
     };
     // the same stage sequence as ocfd_advance, with dt and time read from ctl
     fill(A, 0.0);
     if (scheme == 0) {
-        stage(A, nullptr, B, 1.0, 0.0, 1.0);
+        stage(A, nullptr, B, 1.0, 0.0, 1.0, true);   // B is copied to A whole
         const long n = L.mf_state;
         const int nb = (int)std::min<long>((n + 255) / 256, 4096);
         hipLaunchKernelGGL(k_copy_ctl, dim3(nb), dim3(256), 0, s, A, B, n, (const ocfd_step_ctl*)ctl);
     } else if (scheme == 1) {
         stage(A, nullptr, B, 1.0, 0.0, 1.0);
         fill(B, 1.0);
-        stage(B, A, A, 0.5, 0.5, 0.5);
+        stage(B, A, A, 0.5, 0.5, 0.5, true);
     } else {
         stage(A, nullptr, B, 1.0, 0.0, 1.0);
         fill(B, 1.0);
         stage(B, A, C, 0.25, 0.75, 0.25);
         fill(C, 0.5);
-        stage(C, A, A, 2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0);
+        stage(C, A, A, 2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, true);
     }
-    hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, A, a.ph, &ctl->red,
-                       (const ocfd_step_ctl*)ctl);
+    if (!fold)
+        hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, A, a.ph, &ctl->red,
+                           &ctl->bad, (const ocfd_step_ctl*)ctl);
     hipLaunchKernelGGL(k_ctl_update, dim3(1), dim3(1), 0, s, ctl);
     return check_launch("ocfd_step_device");
 }
@@ -964,6 +1003,7 @@ int ocfd_step_device(ocfd_level lev, double* A, double* B, double* C, ocfd_step_
 struct ocfd_graph_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    ocfd_level lev = nullptr;   // its work-lists and launch sizes are baked into the graph
 };
 
 extern "C" {
@@ -996,6 +1036,8 @@ int ocfd_graph_capture_steps(ocfd_level lev, double* A, double* B, double* C, oc
     ocfd_graph G = new ocfd_graph_s;
     G->graph = g;
     G->exec = x;
+    G->lev = lev;
+    ++lev->dev.live_graphs;
     *out = G;
     return OCFD_OK;
 }
@@ -1012,6 +1054,7 @@ int ocfd_graph_destroy(ocfd_graph g)
     if (!g) return OCFD_OK;
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
+    if (g->lev) --g->lev->dev.live_graphs;
     delete g;
     return OCFD_OK;
 }

@@ -17,6 +17,9 @@ NCONS = 5
 NGROW = 4
 NSPEC = 9
 
+# status codes of the C ABI
+OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ENONFINITE = 0, 1, 2, 3, 4, 5
+
 VISC = 1
 GRAVITY = 2
 FLUX_FP32 = 4
@@ -87,8 +90,8 @@ class StepCtl(ctypes.Structure):
     """Device time control of one level (struct ocfd_step_ctl)."""
     _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("stop_time", ctypes.c_double), ("steps", ctypes.c_int), ("max_steps", ctypes.c_int),
-                ("done", ctypes.c_int), ("pad0", ctypes.c_int), ("pad1", ctypes.c_char * 80),
-                ("red", ctypes.c_ulonglong), ("pad2", ctypes.c_char * 120)]
+                ("done", ctypes.c_int), ("status", ctypes.c_int), ("pad1", ctypes.c_char * 80),
+                ("red", ctypes.c_ulonglong), ("bad", ctypes.c_uint), ("pad2", ctypes.c_char * 116)]
 
 
 CTL_BYTES = ctypes.sizeof(StepCtl)
@@ -356,10 +359,10 @@ class Level:
         """Time the march tile shapes on state S (ghosts filled) and keep the
         fastest; returns (chunk, {chunk: ms})."""
         ch = ctypes.c_int()
-        ms = (ctypes.c_double * 4)()
+        ms = (ctypes.c_double * 6)()
         check(lib().ocfd_level_autotune(self.handle, _ptr(S), ctypes.byref(parm), flags, reps, ctypes.byref(ch), ms,
                                         _stream(stream)), "ocfd_level_autotune")
-        return ch.value, dict(zip((8, 16, 32, 64), list(ms)))
+        return ch.value, dict(zip((2, 4, 8, 16, 32, 64), list(ms)))
 
     def fill_boundary(self, S, time, stream=None):
         check(lib().ocfd_fill_boundary(self.handle, _ptr(S), time, _stream(stream)), "ocfd_fill_boundary")
@@ -452,6 +455,28 @@ def run_chem(level, A, As, B, Bs, parm, cfl, nsteps=-1, stop_time=-1.0, scheme=1
             dt_new = stop_time - time
         dt = dt_new
     return steps, time, dt
+
+
+def level_error(level, got, ref, ncomp=NCONS):
+    """Whole-level relative (L1, Linf) error of state `got` against `ref`
+    (two MultiFabs of `level`, 4 ghosts) per conserved variable, over valid
+    cells, computed on the device: sum/max of |got - ref| over sum/max of
+    |ref|.  The momentum components share the momentum vector's norm."""
+    import torch
+    num1 = torch.zeros(ncomp, dtype=torch.float64, device=got.device)
+    numi = torch.zeros_like(num1); den1 = torch.zeros_like(num1); deni = torch.zeros_like(num1)
+    for b in range(level.nbox):
+        a = level.fab_view(got, b, NGROW, ncomp)[:, 4:-4, 4:-4, 4:-4].reshape(ncomp, -1)
+        r = level.fab_view(ref, b, NGROW, ncomp)[:, 4:-4, 4:-4, 4:-4].reshape(ncomp, -1)
+        d = (a - r).abs()
+        num1 += d.sum(1); numi = torch.maximum(numi, d.amax(1))
+        den1 += r.abs().sum(1); deni = torch.maximum(deni, r.abs().amax(1))
+    num1, numi, den1, deni = (x.cpu().numpy() for x in (num1, numi, den1, deni))
+    if ncomp >= 4:
+        den1[1:4] = den1[1:4].sum()
+        deni[1:4] = deni[1:4].max()
+    den1 = np.maximum(den1, 1e-300); deni = np.maximum(deni, 1e-300)
+    return [(float(x), float(y)) for x, y in zip(num1 / den1, numi / deni)]
 
 
 def new_ctl(device="cuda"):

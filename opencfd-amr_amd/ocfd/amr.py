@@ -259,7 +259,8 @@ class Hierarchy:
 
     def __init__(self, coarse_boxes, geom, parm, cfl, kind=None, scheme=2, flags=0, max_level=1,
                  refine_dengrad=0.05, regrid_int=2, blocking_factor=2, grid_eff=0.7, max_grid_size=64,
-                 n_error_buf=0, n_proper=1, fine_boxes=None, seed=0, stop_time=-1.0, do_reflux=True, stream=None):
+                 n_error_buf=0, n_proper=1, fine_boxes=None, seed=0, stop_time=-1.0, do_reflux=True, stream=None,
+                 planar_fill=True):
         if scheme not in (0, 1, 2):
             raise ValueError("scheme must be 0 (Euler), 1 (RK2) or 2 (SSP-RK3)")
         if blocking_factor % RATIO or max_grid_size % blocking_factor:
@@ -273,6 +274,9 @@ class Hierarchy:
         self.block, self.grid_eff, self.max_c = blocking_factor // RATIO, grid_eff, max_grid_size // RATIO
         self.n_error_buf, self.n_proper = n_error_buf, n_proper
         self.stop_time, self.do_reflux, self.stream = stop_time, do_reflux, stream
+        # PLANAR levels fill one z ghost layer and sweep their first z layer
+        # (ocfd_level_set_planar); planar_fill=False keeps the four-layer fill
+        self.planar_fill = planar_fill
         self.kind, self.seed = kind, seed
         if fine_boxes is not None and fine_boxes and not isinstance(fine_boxes[0][0][0], (tuple, list)):
             fine_boxes = [fine_boxes]          # one fixed fine level, boxes in level-0 index space
@@ -325,7 +329,7 @@ class Hierarchy:
             return
         self._retire(self.levels[l])
         self.levels[l] = level
-        if self.flags & PLANAR:
+        if (self.flags & PLANAR) and self.planar_fill:
             level.set_planar(True)
         b = {k: level.new_mf() for k in ("a", "b", "s1", "s2", "tmp")}
         if S is not None:

@@ -278,6 +278,31 @@ def test_two_level_fp32_flux_and_planar(torch_cuda, orc):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("flags", ["PLANAR|VISC", "PLANAR|FLUX_FP32|VISC"])
+def test_two_level_planar_fill_depth_viscous(torch_cuda, orc, flags):
+    """The one-layer planar fill (first-layer sweeps, first-layer register
+    faces whose viscous terms read the z neighbours the x tiles wrote) equals
+    the same planar stepping with the four-layer fill, bit for bit, in the
+    viscous FP64 and FP32-flux forms."""
+    import ocfd
+    from ocfd import amr, cases
+    f = 0
+    for name in flags.split("|"):
+        f |= getattr(ocfd, name)
+    boxes, g, kind = cases.dmr(64, 16, 2, max_grid=(16, 16, 2))
+    fine = [((0, 0, 0), (21, 9, 1)), ((22, 0, 0), (31, 5, 1))]
+    runs = []
+    for pf in (True, False):
+        T = amr.TwoLevel(boxes, g, ocfd.default_parm(), 0.5, kind, flags=f, regrid_int=0, fine_boxes=fine,
+                         planar_fill=pf)
+        for _ in range(4):
+            T.step()
+        runs.append([a[:, 4:-4, 4:-4, 4:-4] for lev, S in ((T.levels[0], T.S[0]), (T.fine, T.S[1]))
+                     for a in gu.download(lev, S)])
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+
+
 def _compare_hier(T, O, tol):
     key = lambda b: (tuple(b[0]), tuple(b[1]))
     assert T.finest == O.finest

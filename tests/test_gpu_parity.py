@@ -220,6 +220,34 @@ def test_kh3d_steps_fp64_and_fp32(torch_cuda):
     assert l1_rho < 1e-5
 
 
+def test_kh3d_256_fp32_flux_error_100_steps(torch_cuda):
+    """The precision target at its own size: KH 256^3 viscous, 100 SSP-RK3
+    steps at one fixed dt (CFL 0.5 of the initial state), FP32-flux form
+    against the FP64 form (which the oracle pins at 1e-10 on the same
+    kernels): every value finite and relative L1 density error <= 1e-4."""
+    import torch
+    import ocfd
+    from ocfd import cases
+    boxes, g, kind = cases.kh3d(256)
+    lev = ocfd.Level(boxes, g)
+    p = ocfd.default_parm()
+    A64, B, C = lev.new_mf(), lev.new_mf(), lev.new_mf()
+    lev.init_state(kind, A64, p, seed=42)
+    A32 = A64.clone()
+    dt = lev.estdt(A64, 0.5, p)
+    for A, flags in ((A64, ocfd.VISC), (A32, ocfd.VISC | ocfd.FLUX_FP32)):
+        t = 0.0
+        for _ in range(100):
+            lev.advance(A, B, C, t, dt, 2, p, flags)
+            t += dt
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(lev.fab_view(A32, 0)[:, 4:-4, 4:-4, 4:-4]).all())
+    e = ocfd.level_error(lev, A32, A64)
+    print("kh 256^3 fp32-flux vs fp64, 100 steps: (L1, Linf) per variable", e)
+    assert e[0][0] <= 1e-4
+    assert max(x[0] for x in e) <= 1e-4
+
+
 def test_dmr_uneven_steps_fp64(torch_cuda):
     import ocfd
     from ocfd import cases
