@@ -319,6 +319,19 @@ int ocfd_fill_boundary_spec(ocfd_level lev, double* Spec, double time, void* str
 int ocfd_compute_dsdt_multi(ocfd_level lev, const double* S, const double* Spec, double* dSdt, double* dSdt_spec,
                             double dt, const ocfd_parm* parm, int flags, void* stream);
 
+/* No counterpart in the reference: ocfd_compute_dsdt_multi for callers that
+ * cannot allocate device memory (code under hipGraph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and results; the scratch
+ * the multispecies path otherwise allocates on its first call (19-component
+ * primitives/transport and three 14-component face-flux buffers, 488 B per
+ * grown cell of the level) lives in a 64 MiB static device array, held by
+ * each GPU the library runs on until the process ends.  A level needing more
+ * (more than 137,518 grown cells) is refused with OCFD_EINVAL.  Calls no
+ * hipMalloc/hipFree.  Every call on one GPU uses the same array: calls that
+ * overlap on one GPU (different streams, concurrently) race. */
+int ocfd_compute_dsdt_multi_noalloc(ocfd_level lev, const double* S, const double* Spec, double* dSdt,
+                                    double* dSdt_spec, double dt, const ocfd_parm* parm, int flags, void* stream);
+
 /* One stage of EBR::flow_advance_multi for state and species together
  * (LinComb / Saxpy of Source/CHEM/FlowAdvance.cpp:59-86). */
 int ocfd_rk_stage_multi(ocfd_level lev, const double* S_cur, const double* Spec_cur, const double* S_old,

@@ -13,6 +13,10 @@
 //                viscous flux (ChemViscous.H)
 //   k_mdiv       per valid cell: divergence of the 14 face fluxes and the
 //                stage combination of state and species
+#include <map>
+#include <mutex>
+#include <string>
+
 #include "ocfd_device.hpp"
 #include "ocfd_internal.hpp"
 #include "ocfd_chem_device.hpp"
@@ -737,6 +741,47 @@ __global__ __launch_bounds__(256) void k_init_chemsod(const BoxDesc* __restrict_
     for (int n = 0; n < NS; ++n) p[n * st] = rho * Y[n];
 }
 
+// Scratch of the allocation-free multispecies entry point: the 19-component
+// primitive/transport buffer and the three 14-component face-flux buffers of
+// one level, laid out as ensure_multi lays out its allocations, in a
+// file-scope device array.  Each GPU holds its own copy from the first use of
+// this module on it until the process ends.  Host code never uses the array's
+// name as an address: the per-device address comes from hipGetSymbolAddress.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+__device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+// bytes of scratch the multispecies path needs on level L
+inline size_t multi_scratch_bytes(const LevelDev& L)
+{
+    return sizeof(double) * (size_t)(MQ_N + 3 * NF) * (size_t)(L.mf_state / NC);
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+static int noalloc_scratch(double** out)
+{
+    static std::mutex mu;
+    static std::map<int, double*> addr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("ocfd_compute_dsdt_multi_noalloc: no current device");
+        return OCFD_EHIP;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(dev);
+    if (it == addr.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("ocfd_compute_dsdt_multi_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return OCFD_EHIP;
+        }
+        it = addr.emplace(dev, (double*)p).first;
+    }
+    *out = it->second;
+    return OCFD_OK;
+}
+
 } // namespace ocfd
 
 using namespace ocfd;
@@ -816,6 +861,40 @@ int ocfd_compute_dsdt_multi(ocfd_level lev, const double* S, const double* Spec,
                         flags);
     launch_multi(lev->dev, m, (hipStream_t)stream);
     return mcheck("ocfd_compute_dsdt_multi");
+}
+
+int ocfd_compute_dsdt_multi_noalloc(ocfd_level lev, const double* S, const double* Spec, double* dSdt,
+                                    double* dSdt_spec, double dt, const ocfd_parm* parm, int flags, void* stream)
+{
+    if (lev && lev->dev.fill_gz != NG) {
+        set_error("multispecies stepping needs the full ghost fill depth (ocfd_level_set_planar off)");
+        return OCFD_EINVAL;
+    }
+    (void)dt;
+    if (!lev || !S || !Spec || !dSdt || !dSdt_spec || !parm) {
+        set_error("ocfd_compute_dsdt_multi_noalloc: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    const LevelDev& L = lev->dev;
+    const size_t need = multi_scratch_bytes(L);
+    if (need > kNoAllocScratchBytes) {
+        set_error("ocfd_compute_dsdt_multi_noalloc: the level needs " + std::to_string(need) +
+                  " bytes of scratch; the allocation-free entry point holds " + std::to_string(kNoAllocScratchBytes) +
+                  " (use ocfd_compute_dsdt_multi)");
+        return OCFD_EINVAL;
+    }
+    double* base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    MultiArgs m = margs(lev->dev, S, Spec, nullptr, nullptr, nullptr, nullptr, dSdt, dSdt_spec, 1.0, 0.0, 0.0, parm,
+                        flags);
+    // the layout of ensure_multi: primitives, then the x, y, z face fluxes;
+    // every element is written before it is read, so nothing is cleared
+    const long ng = L.mf_state / NC;
+    m.mq = base;
+    for (int d = 0; d < 3; ++d) m.mF[d] = base + (long)MQ_N * ng + (long)d * NF * ng;
+    launch_multi(L, m, (hipStream_t)stream);
+    return mcheck("ocfd_compute_dsdt_multi_noalloc");
 }
 
 int ocfd_rk_stage_multi(ocfd_level lev, const double* S_cur, const double* Spec_cur, const double* S_old,

@@ -148,6 +148,8 @@ def lib():
         L.ocfd_graph_destroy.argtypes = [vp]
         L.ocfd_fill_boundary_spec.argtypes = [vp, vp, ctypes.c_double, vp]
         L.ocfd_compute_dsdt_multi.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.POINTER(Parm), ctypes.c_int, vp]
+        L.ocfd_compute_dsdt_multi_noalloc.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.POINTER(Parm),
+                                                      ctypes.c_int, vp]
         L.ocfd_rk_stage_multi.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double, ctypes.POINTER(Parm), ctypes.c_int, vp]
         L.ocfd_advance_multi.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
