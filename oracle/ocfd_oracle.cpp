@@ -839,6 +839,50 @@ void orc_eb_flux_many(int dir, long n, const double* ql, const double* qr, doubl
     for (long m = 0; m < n; ++m) eb_flux_point(dir, ql + 7 * m, qr + 7 * m, F + 5 * m, *parm);
 }
 
+// eb_recon (EBkernels.H:10-71) on every face of surroundingNodes(bx,d), for
+// each component of q: ql, qr laid out (ncomp, faces) like a FAB over the
+// face box.  eb covers the cells the stencils reach (grow(bx,2) suffices).
+void orc_eb_recon_box(int d, const int* lo, const int* hi, const orc_fab* q, const orc_ebfab* eb, double* ql,
+                      double* qr)
+{
+    Box bx;
+    for (int e = 0; e < 3; ++e) { bx.lo[e] = lo[e]; bx.hi[e] = hi[e]; }
+    const Box fb = surrounding_nodes(bx, d);
+    const View qv(*q), L(ql, fb.lo, fb.hi, q->ncomp), R(qr, fb.lo, fb.hi, q->ncomp);
+    const EBV g(eb);
+    for_box(fb, [&](int i, int j, int k) {
+        for (int n = 0; n < q->ncomp; ++n) eb_recon(d, i, j, k, n, qv, g, L(i, j, k, n), R(i, j, k, n));
+    });
+}
+
+// eb_compute_visc (EBdiffusion.H:8-203) alone on every face of
+// surroundingNodes(bx,d): F (5 comps, zeroed first) receives the viscous and
+// heat-conduction terms.  q: primitive FAB (7 comps: rho,u,v,w,p,T,c).
+void orc_eb_visc_box(int d, const int* lo, const int* hi, const orc_fab* q, const orc_ebfab* eb, const double* dxinv,
+                     const orc_parm* parm, double* F)
+{
+    Box bx;
+    for (int e = 0; e < 3; ++e) { bx.lo[e] = lo[e]; bx.hi[e] = hi[e]; }
+    const Box fb = surrounding_nodes(bx, d);
+    const View qv(*q), Fv(F, fb.lo, fb.hi, NCONS);
+    const EBV g(eb);
+    for_box(fb, [&](int i, int j, int k) {
+        double f[NCONS] = {0, 0, 0, 0, 0};
+        eb_visc(d, i, j, k, qv, f, g, dxinv, *parm);
+        for (int n = 0; n < NCONS; ++n) Fv(i, j, k, n) = f[n];
+    });
+}
+
+// eb_wallflux (EBkernels.H:173-214) on n cells: qw n x 7 primitives, area
+// n x 6 (apx lo, apx hi, apy lo, apy hi, apz lo, apz hi), out n x 5
+void orc_eb_wallflux_many(long n, const double* qw, const double* area, double* out, const orc_parm* parm)
+{
+    for (long m = 0; m < n; ++m) {
+        const double* a = area + 6 * m;
+        eb_wallflux(qw + NPRIM * m, a[0], a[1], a[2], a[3], a[4], a[5], out + NCONS * m, *parm);
+    }
+}
+
 // Box kind as EBR::compute_dSdt decides it: 0 regular, 1 EB path, 2 covered.
 int orc_eb_box_kind(const int* lo, const int* hi, const orc_ebfab* eb)
 {
@@ -1352,7 +1396,7 @@ inline void sw_species(int d, const View& q, const View& spec, int i, int j, int
 
 // compute_dSdt_multi on one regular box (FlowAdvance.cpp:173-403)
 void compute_dsdt_multi_box(const Box& bx, const View& s, const View& spec, const View& dsdt, const View& dsdt_spec,
-                            const double* dxinv, int do_visc, const orc_parm& parm)
+                            const double* dxinv, int do_visc, const orc_parm& parm, double* const* face_out = nullptr)
 {
     const Box bxg = grow(bx, NUM_GROW);
     const size_t ng = (size_t)(bxg.hi[0]-bxg.lo[0]+1) * (bxg.hi[1]-bxg.lo[1]+1) * (bxg.hi[2]-bxg.lo[2]+1);
@@ -1480,6 +1524,14 @@ void compute_dsdt_multi_box(const Box& bx, const View& s, const View& spec, cons
             });
         }
     }
+    // face_out (optional): F[0..2] (5 comps), Fs[0..2] (9), Fd[0..2] (9), each
+    // on surroundingNodes(bx,d); null entries are skipped
+    if (face_out)
+        for (int d = 0; d < 3; ++d) {
+            if (face_out[d]) std::memcpy(face_out[d], fbuf[d].data(), fbuf[d].size() * sizeof(double));
+            if (face_out[3 + d]) std::memcpy(face_out[3 + d], fsbuf[d].data(), fsbuf[d].size() * sizeof(double));
+            if (face_out[6 + d]) std::memcpy(face_out[6 + d], fdbuf[d].data(), fdbuf[d].size() * sizeof(double));
+        }
     // divop for both, FlowAdvance.cpp:394-403
     for_box(bx, [&](int i, int j, int k) {
         for (int n = 0; n < NCONS; ++n)
@@ -1507,6 +1559,29 @@ inline void diffusion_face(int d, int i, int j, int k, const View& q, const View
         double dn = 0.5 * (D(i, j, k, n) + D(im, jm, km, n));
         J[n] = rho * dn * (Ytl - Ytr) * dxinv[d];
     }
+}
+
+// Van Leer mass-flux split of the partial densities at one face,
+// Source/CHEM/FlowAdvanceBox.cpp:174-223: normal velocities ul/ur and sound
+// speeds cL/cR of the reconstructed left/right states, partial densities
+// rl/rr; F = f+(left) + f-(right) per species.
+inline void vl_species_flux(double ul, double ur, double cL, double cR, const double* rl, const double* rr, double* F)
+{
+    double fp[NSP] = {0}, fm[NSP] = {0};
+    double ML = ul / cL, MR = ur / cR, UL = ul, UR = ur;
+    if (ML >= 1.0) {
+        for (int n = 0; n < NSP; ++n) fp[n] = rl[n] * UL;
+    } else if (std::fabs(ML) < 1.0) {
+        double Mp = 0.250 * (1.0 + ML) * (1.0 + ML);
+        for (int n = 0; n < NSP; ++n) fp[n] = rl[n] * cL * Mp;
+    }
+    if (std::fabs(MR) < 1.0) {
+        double Mm = -0.250 * (MR - 1.0) * (MR - 1.0);
+        for (int n = 0; n < NSP; ++n) fm[n] = rr[n] * cR * Mm;
+    } else if (MR <= -1.0) {
+        for (int n = 0; n < NSP; ++n) fm[n] = rr[n] * UR;
+    }
+    for (int n = 0; n < NSP; ++n) F[n] = fp[n] + fm[n];
 }
 
 // EBR::ebchem_compute_dSdt_box, Source/CHEM/FlowAdvanceBox.cpp:16-525: MUSCL
@@ -1543,25 +1618,12 @@ void ebchem_dsdt_box(const Box& bx, const View& s, const View& spec, const View&
         Fs[d] = View(fsbuf[d].data(), fb[d].lo, fb[d].hi, NSP);
         const View F = Fs[d];
         for_box(fb[d], [&](int i, int j, int k) {
-            double ul, ur, cL, cR, rl[NSP], rr[NSP];
+            double ul, ur, cL, cR, rl[NSP], rr[NSP], fs[NSP];
             eb_recon(d, i, j, k, QU + d, q, g, ul, ur);
             eb_recon(d, i, j, k, QC, q, g, cL, cR);
             for (int n = 0; n < NSP; ++n) eb_recon(d, i, j, k, n, spec, g, rl[n], rr[n]);
-            double fp[NSP] = {0}, fm[NSP] = {0};
-            double ML = ul / cL, MR = ur / cR, UL = ul, UR = ur;
-            if (ML >= 1.0) {
-                for (int n = 0; n < NSP; ++n) fp[n] = rl[n] * UL;
-            } else if (std::fabs(ML) < 1.0) {
-                double Mp = 0.250 * (1.0 + ML) * (1.0 + ML);
-                for (int n = 0; n < NSP; ++n) fp[n] = rl[n] * cL * Mp;
-            }
-            if (std::fabs(MR) < 1.0) {
-                double Mm = -0.250 * (MR - 1.0) * (MR - 1.0);
-                for (int n = 0; n < NSP; ++n) fm[n] = rr[n] * cR * Mm;
-            } else if (MR <= -1.0) {
-                for (int n = 0; n < NSP; ++n) fm[n] = rr[n] * UR;
-            }
-            for (int n = 0; n < NSP; ++n) F(i, j, k, n) = fp[n] + fm[n];
+            vl_species_flux(ul, ur, cL, cR, rl, rr, fs);
+            for (int n = 0; n < NSP; ++n) F(i, j, k, n) = fs[n];
             if (do_visc) {
                 double J[NSP];
                 diffusion_face(d, i, j, k, q, spec, D, dxinv, J);
@@ -1649,6 +1711,27 @@ void chem_level(const Level& S, const Level& Sp, double dt, const orc_parm& parm
 } // namespace
 
 extern "C" {
+
+// The face fluxes of compute_dSdt_multi on one regular box (test use): the
+// nine outputs F_x,y,z (5 comps), species F_x,y,z (9) and Fickian J_x,y,z
+// (9), each on surroundingNodes(bx,d) (null pointers skipped)
+void orc_compute_dsdt_multi_box_faces(const int* lo, const int* hi, const orc_fab* S, const orc_fab* Spec,
+                                      const double* dxinv, int do_visc, const orc_parm* parm, double* const* faces)
+{
+    Box bx;
+    for (int d = 0; d < 3; ++d) { bx.lo[d] = lo[d]; bx.hi[d] = hi[d]; }
+    std::vector<double> a(NCONS * (size_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1));
+    std::vector<double> b(NSP * (size_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1));
+    compute_dsdt_multi_box(bx, View(*S), View(*Spec), View(a.data(), lo, hi, NCONS), View(b.data(), lo, hi, NSP),
+                           dxinv, do_visc, *parm, faces);
+}
+
+// Van Leer species mass-flux split on n faces; rl, rr, F: n x 9
+void orc_vl_species_flux_many(long n, const double* ul, const double* ur, const double* cL, const double* cR,
+                              const double* rl, const double* rr, double* F)
+{
+    for (long m = 0; m < n; ++m) vl_species_flux(ul[m], ur[m], cL[m], cR[m], rl + NSP * m, rr + NSP * m, F + NSP * m);
+}
 
 // compute_dSdt_multi for one regular box; S, Spec cover grow(bx,4)
 void orc_compute_dsdt_multi_box(const int* lo, const int* hi, const orc_fab* S, const orc_fab* Spec,

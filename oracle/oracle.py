@@ -265,6 +265,10 @@ def _eb_sigs():
                                      ctypes.POINTER(OParm)]
     L.orc_inflow_state.argtypes = [dp]
     L.orc_eb_flux_many.argtypes = [ctypes.c_int, ctypes.c_long, dp, dp, dp, ctypes.POINTER(OParm)]
+    L.orc_eb_recon_box.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(OFab), ctypes.POINTER(OEbFab), dp, dp]
+    L.orc_eb_visc_box.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(OFab), ctypes.POINTER(OEbFab), dp,
+                                  ctypes.POINTER(OParm), dp]
+    L.orc_eb_wallflux_many.argtypes = [ctypes.c_long, dp, dp, dp, ctypes.POINTER(OParm)]
     L._eb_ready = True
     return L
 
@@ -288,6 +292,40 @@ def eb_flux(d, ql, qr, p):
     F = np.zeros((ql.shape[0], 5))
     L.orc_eb_flux_many(d, ql.shape[0], _dp(ql), _dp(qr), _dp(F), ctypes.byref(p))
     return F
+
+
+def eb_recon_box(d, lo, hi, q, eb):
+    """Left/right face states of eb_recon on surroundingNodes([lo,hi], d) for
+    every component of FAB q: (ql, qr), each (ncomp, faces)."""
+    L = _eb_sigs()
+    nc = q.a.shape[0]
+    ql = np.zeros(_face_shape(lo, hi, d, nc))
+    qr = np.zeros_like(ql)
+    L.orc_eb_recon_box(d, (ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi), ctypes.byref(q.c), ctypes.byref(eb.c),
+                       _dp(ql), _dp(qr))
+    return ql, qr
+
+
+def eb_visc_box(d, lo, hi, q, eb, dxinv, p):
+    """Viscous/heat part of the EB face flux (eb_compute_visc) on
+    surroundingNodes([lo,hi], d) from the 7-component primitive FAB q."""
+    L = _eb_sigs()
+    F = np.zeros(_face_shape(lo, hi, d, 5))
+    dxi = np.ascontiguousarray(dxinv, dtype=np.float64)
+    L.orc_eb_visc_box(d, (ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi), ctypes.byref(q.c), ctypes.byref(eb.c),
+                      _dp(dxi), ctypes.byref(p), _dp(F))
+    return F
+
+
+def eb_wallflux(qw, area, p):
+    """eb_wallflux of n cells: qw (n x 7 primitives), area (n x 6: apx lo/hi,
+    apy lo/hi, apz lo/hi) -> (n x 5)."""
+    L = _eb_sigs()
+    qw = np.ascontiguousarray(qw, dtype=np.float64)
+    area = np.ascontiguousarray(area, dtype=np.float64)
+    out = np.zeros((qw.shape[0], 5))
+    L.orc_eb_wallflux_many(qw.shape[0], _dp(qw), _dp(area), _dp(out), ctypes.byref(p))
+    return out
 
 
 def eb_box_kind(lo, hi, eb):
@@ -551,6 +589,9 @@ def _multi_sigs():
     L.orc_init_chemsod.argtypes = [ip, ip, F, F, ctypes.POINTER(OGeom)]
     L.orc_compute_dsdt_multi_box_eb.argtypes = [ip, ip, F, F, F, F, dp, ctypes.POINTER(OEbFab), ctypes.c_int,
                                                 ctypes.POINTER(OParm)]
+    L.orc_compute_dsdt_multi_box_faces.argtypes = [ip, ip, F, F, dp, ctypes.c_int, ctypes.POINTER(OParm),
+                                                   ctypes.POINTER(dp)]
+    L.orc_vl_species_flux_many.argtypes = [ctypes.c_long, dp, dp, dp, dp, dp, dp, dp]
     L._multi_ready = True
     return L
 
@@ -564,6 +605,35 @@ def compute_dsdt_multi_box(lo, hi, S, Sp, p, dxinv, do_visc=0):
                                  ctypes.byref(Sp.c), ctypes.byref(D.c), ctypes.byref(Ds.c), _dp(dxi), do_visc,
                                  ctypes.byref(p))
     return D.a, Ds.a
+
+
+def _face_shape(lo, hi, d, ncomp):
+    h = list(hi)
+    h[d] += 1
+    return (ncomp,) + tuple(h[e] - lo[e] + 1 for e in (2, 1, 0))
+
+
+def compute_multi_faces(lo, hi, S, Sp, p, dxinv, do_visc=0):
+    """Face fluxes of compute_dSdt_multi on one regular box: ([F_x, F_y, F_z]
+    (5 comps), [species F_x..z] (9), [Fickian J_x..z] (9)), each over
+    surroundingNodes(bx, d)."""
+    L = _multi_sigs()
+    outs = [[np.zeros(_face_shape(lo, hi, d, nc)) for d in range(3)] for nc in (5, NSPEC, NSPEC)]
+    ptrs = (ctypes.POINTER(ctypes.c_double) * 9)(*[_dp(a) for grp in outs for a in grp])
+    dxi = np.ascontiguousarray(dxinv, dtype=np.float64)
+    L.orc_compute_dsdt_multi_box_faces((ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi), ctypes.byref(S.c),
+                                       ctypes.byref(Sp.c), _dp(dxi), do_visc, ctypes.byref(p), ptrs)
+    return outs
+
+
+def vl_species_flux(ul, ur, cL, cR, rl, rr):
+    """Van Leer mass-flux split of the partial densities on n faces."""
+    L = _multi_sigs()
+    arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (ul, ur, cL, cR, rl, rr)]
+    n = arr[0].shape[0]
+    F = np.zeros((n, NSPEC))
+    L.orc_vl_species_flux_many(n, *[_dp(a) for a in arr], _dp(F))
+    return F
 
 
 def compute_dsdt_multi_box_eb(lo, hi, S, Sp, p, dxinv, eb, do_visc=0):
