@@ -18,6 +18,7 @@ extern "C" {
 #endif
 
 #define OCFD_NCONS 5   /* Source/IndexDefines.H:4-9   URHO,UMX,UMY,UMZ,UEDEN */
+#define OCFD_NSPEC 9   /* Source/CHEM/LiDryer.H       H2,O2,H2O,H,O,OH,HO2,H2O2,N2 */
 #define OCFD_NGROW 4   /* Source/EBR.H:207            NUM_GROW              */
 
 /* status codes */
@@ -58,6 +59,7 @@ typedef struct ocfd_geom {
 } ocfd_geom;
 
 typedef struct ocfd_level_s* ocfd_level;
+typedef struct ocfd_graph_s* ocfd_graph;   /* captured device-driven steps */
 
 /* Embedded boundary: a body extruded along z, given by an implicit function
  * (> 0 inside the body) -- the eb2.geom_type choices the cases use
@@ -311,6 +313,14 @@ int ocfd_reactor_temperature(long n, const double* state, double* T, void* strea
  * component (Source/Setup.cpp:241-247). */
 int ocfd_fill_boundary_spec(ocfd_level lev, double* Spec, double time, void* stream);
 
+/* Species on ext_dir (inflow) sides: ocfd_fill_boundary_spec and the device
+ * steps write `spec` (9 partial densities, host array; NULL clears) into
+ * every ext_dir ghost cell of the level's species MultiFabs.  Without it those
+ * ghost cells keep whatever they hold -- the reference's spec_bcfill leaves
+ * them alone too (Exec/9_CHEMEB/prob_bcfill.cpp:37-44), which is undefined for
+ * the scratch MultiFabs of its RK stages.  Synchronous. */
+int ocfd_level_set_inflow_species(ocfd_level lev, const double* spec);
+
 /* EBR::compute_dSdt_multi, regular boxes (Source/CHEM/FlowAdvance.cpp:92-496):
  * c2prim_rgas, mixture transport (viscous), Steger-Warming split of flow and
  * species with the cell's gamma, WENO-Z5 for 5 + 9 comps, Fickian diffusion,
@@ -355,6 +365,34 @@ int ocfd_advance_chem(ocfd_level lev, double* A, double* Aspec, double* B, doubl
  * Exec/5_CHEMSOD/prob.H:11-82).  Ghost cells are zeroed.  Synchronous. */
 int ocfd_init_species_state(ocfd_level lev, int kind, double* S, double* Spec, const ocfd_parm* parm);
 
+/* A uniform premixed state: every cell of S (5 comps) and Spec (9 partial
+ * densities), ghost cells included, set to `state` and `spec` (host arrays).
+ * The initial data and constant inflow of the reacting channel past a
+ * cylinder (the uniform-premixed counterpart of ebr_initdata/ebr_initspec,
+ * Exec/9_CHEMEB/prob.H:11-117); ghost cells on an inflow (ext_dir) side keep
+ * these species, which the species ghost fill leaves alone there.
+ * Synchronous. */
+int ocfd_init_species_uniform(ocfd_level lev, const double* state, const double* spec, double* S, double* Spec);
+
+/* Device-driven steps of a multispecies level (no host round trip, graph
+ * capturable): ocfd_ctl_start_chem sets time, stop and step limits and the
+ * CHEM build's dt = cfl * 1e-8 (ebr_estdt under CHEM, Source/Advance.cpp:54-57;
+ * EBR::computeInitialDt, Source/EBR.cpp:189-229).  ocfd_step_device_chem runs
+ * one EBR::advance of ocfd_advance_chem -- chemistry x iter, flow_advance_multi
+ * (scheme 0 Euler, 1 RK2), chemistry x iter, then the species renormalisation
+ * of EBR::post_timestep when renorm != 0 (Source/Advance.cpp:152-177,
+ * Source/EBR.cpp:333-357) -- with dt and time from ctl, then the dt update of
+ * EBR::computeNewDt; a non-finite cell ends the run with OCFD_ENONFINITE.
+ * Bit-identical to the host loop of ocfd_advance_chem calls. */
+int ocfd_ctl_start_chem(ocfd_level lev, const double* A, ocfd_step_ctl* ctl, double time, double cfl,
+                        double stop_time, int max_steps, const ocfd_parm* parm, void* stream);
+int ocfd_step_device_chem(ocfd_level lev, double* A, double* Aspec, double* B, double* Bspec, ocfd_step_ctl* ctl,
+                          int scheme, int level, const ocfd_parm* parm, int flags, int renorm, void* stream);
+/* hipGraph of nsteps such steps (rules of ocfd_graph_capture_steps). */
+int ocfd_graph_capture_steps_chem(ocfd_level lev, double* A, double* Aspec, double* B, double* Bspec,
+                                  ocfd_step_ctl* ctl, int scheme, int level, const ocfd_parm* parm, int flags,
+                                  int renorm, int nsteps, void* stream, ocfd_graph* out);
+
 /* Initialise device time control: time, cfl, stop_time, max_steps, and the
  * initial dt = cfl * estdt(A) clipped to stop_time (EBR::computeInitialDt,
  * Source/EBR.cpp:189-229,561-563).  No host synchronisation. */
@@ -379,7 +417,6 @@ int ocfd_step_device(ocfd_level lev, double* A, double* B, double* C, ocfd_step_
  * addresses and launch sizes: while it exists the level refuses
  * ocfd_level_set_march_chunk, ocfd_level_autotune and ocfd_level_destroy
  * (OCFD_EINVAL); destroy the graphs first. */
-typedef struct ocfd_graph_s* ocfd_graph;
 int ocfd_graph_capture_steps(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
                              const ocfd_parm* parm, int flags, int nsteps, void* stream, ocfd_graph* out);
 int ocfd_graph_launch(ocfd_graph g, void* stream);

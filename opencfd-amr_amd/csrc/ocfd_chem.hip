@@ -21,10 +21,15 @@ using chem::NS;
 // row 0 rho, row 1 sensible energy density E, rows 2..10 partial densities
 constexpr int RROW_RHO = 0, RROW_E = 1, RROW_Y = 2;
 
+// dt: the chemistry step; under device time control (ctl not null) it is
+// (0.5 ctl->dt)/iter instead, the Strang substep of EBR::advance
+// (Source/Advance.cpp:160-176)
 __global__ __launch_bounds__(64) void k_chem_level(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                    double* __restrict__ S, double* __restrict__ Spec, double dt,
-                                                   double smallr)
+                                                   double smallr, const ocfd_step_ctl* __restrict__ ctl, int iter)
 {
+    if (ctl_done(ctl)) return;
+    if (ctl) dt = (0.5 * ctl->dt) / iter;
     const int4 t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.z) return;
     const BoxDesc B = boxes[t.x];
@@ -51,8 +56,10 @@ __global__ __launch_bounds__(64) void k_chem_level(const BoxDesc* __restrict__ b
 }
 
 __global__ __launch_bounds__(256) void k_renorm_level(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
-                                                      const double* __restrict__ S, double* __restrict__ Spec)
+                                                      const double* __restrict__ S, double* __restrict__ Spec,
+                                                      const ocfd_step_ctl* __restrict__ ctl)
 {
+    if (ctl_done(ctl)) return;
     const int4 t = tiles[blockIdx.x];
     const BoxDesc B = boxes[t.x];
     const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
@@ -298,6 +305,20 @@ __global__ void k_reactor_T(long n, const double* __restrict__ st, double* __res
     T[r] = chem::T_given_ey(st[RROW_E * n + r], rhoi);
 }
 
+void launch_chem_level(const LevelDev& L, double* S, double* Spec, double dt, double smallr,
+                       const ocfd_step_ctl* ctl, int iter, hipStream_t s)
+{
+    if (L.ntile_v)
+        hipLaunchKernelGGL(k_chem_level, dim3(L.ntile_v), dim3(64), 0, s, L.d_boxes, L.d_tile_v, S, Spec, dt, smallr,
+                           ctl, iter);
+}
+
+void launch_renorm(const LevelDev& L, const double* S, double* Spec, const ocfd_step_ctl* ctl, hipStream_t s)
+{
+    if (L.ntile_c)
+        hipLaunchKernelGGL(k_renorm_level, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, S, Spec, ctl);
+}
+
 static int launch_check(const char* what)
 {
     hipError_t e = hipGetLastError();
@@ -317,20 +338,14 @@ extern "C" {
 int ocfd_chem_advance(ocfd_level lev, double* S, double* Spec, double dt, const ocfd_parm* parm, void* stream)
 {
     if (!lev || !S || !Spec || !parm) { set_error("ocfd_chem_advance: invalid arguments"); return OCFD_EINVAL; }
-    const LevelDev& L = lev->dev;
-    if (L.ntile_v)
-        hipLaunchKernelGGL(k_chem_level, dim3(L.ntile_v), dim3(64), 0, (hipStream_t)stream, L.d_boxes, L.d_tile_v, S,
-                           Spec, dt, parm->smallr);
+    launch_chem_level(lev->dev, S, Spec, dt, parm->smallr, nullptr, 1, (hipStream_t)stream);
     return launch_check("ocfd_chem_advance");
 }
 
 int ocfd_species_renorm(ocfd_level lev, const double* S, double* Spec, void* stream)
 {
     if (!lev || !S || !Spec) { set_error("ocfd_species_renorm: invalid arguments"); return OCFD_EINVAL; }
-    const LevelDev& L = lev->dev;
-    if (L.ntile_c)
-        hipLaunchKernelGGL(k_renorm_level, dim3(L.ntile_c), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_tile_c,
-                           S, Spec);
+    launch_renorm(lev->dev, S, Spec, nullptr, (hipStream_t)stream);
     return launch_check("ocfd_species_renorm");
 }
 

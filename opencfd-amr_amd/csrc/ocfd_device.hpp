@@ -46,6 +46,8 @@ struct GeomDesc {
     int probtype;
     double prob_lo[3], dx[3];
     double left[NC], right[NC];
+    double spec_in[OCFD_NSPEC];  // species written on ext_dir sides (has_spec_in), e.g. an inflow mixture
+    int has_spec_in;
     int map_shift;              // log2 of the owner-map tile edge
     int map_n[3];               // owner-map tiles per direction
     const int* map;             // tile -> the box wholly covering it (>= 0), -1 none, <= -2 a list:

@@ -70,6 +70,26 @@ struct LevelDev {
 
 void launch_eb(const LevelDev& L, const StageArgs& a, hipStream_t s);
 
+// chemistry of a level (ocfd_chem.hip): the linearised-implicit step of dt
+// (or, under device time control, of (0.5 ctl->dt)/iter) and the species
+// renormalisation
+void launch_chem_level(const LevelDev& L, double* S, double* Spec, double dt, double smallr,
+                       const ocfd_step_ctl* ctl, int iter, hipStream_t s);
+void launch_renorm(const LevelDev& L, const double* S, double* Spec, const ocfd_step_ctl* ctl, hipStream_t s);
+
+// device time control (ocfd_level.hip): the ghost fill at ctl->time +
+// toff ctl->dt; the end of a step (non-finite check of A, the CFL estimate --
+// or fixed_est when > 0 -- and the dt update); the start of a run
+void launch_fill_ctl(const LevelDev& L, double* S, const ocfd_step_ctl* ctl, double toff, hipStream_t s);
+void launch_step_end(const LevelDev& L, const double* A, const PhysArgs& ph, ocfd_step_ctl* ctl, double fixed_est,
+                     hipStream_t s);
+void launch_ctl_start(const LevelDev& L, const double* A, const PhysArgs& ph, ocfd_step_ctl* ctl, double time,
+                      double cfl, double stop_time, int max_steps, double fixed_est, hipStream_t s);
+// hipGraph of the launches `body` enqueues on stream s (a level's graph:
+// while it exists the level keeps its work-lists)
+int capture_graph(ocfd_level lev, hipStream_t s, int (*body)(void*, hipStream_t), void* arg, ocfd_graph* out,
+                  const char* what);
+
 constexpr int kMapShift = 3;   // owner-map tiles of 8^3 cells
 
 // ghost-fill work-list of a level at its fill depth

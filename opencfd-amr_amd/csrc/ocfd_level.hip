@@ -244,6 +244,14 @@ __global__ void k_ctl_update(ocfd_step_ctl* c)
     ctl_check_bad(c);
 }
 
+// the CFL estimate replaced by a fixed value (the CHEM build's estTimeStep,
+// Source/Advance.cpp:54-57), after a reduction that only checked finiteness
+__global__ void k_ctl_fixed_est(ocfd_step_ctl* c, double est)
+{
+    if (c->done) return;
+    atomicExch(&c->red, (unsigned long long)__double_as_longlong(est));
+}
+
 // A <- B for the forward-Euler step under device control (a no-op once done)
 __global__ __launch_bounds__(256) void k_copy_ctl(double* __restrict__ dst, const double* __restrict__ src, long n,
                                                   const ocfd_step_ctl* __restrict__ ctl)
@@ -481,6 +489,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     }
     G.probtype = g->probtype;
     for (int n = 0; n < NC; ++n) { G.left[n] = g->left_state[n]; G.right[n] = g->right_state[n]; }
+    G.has_spec_in = 0;
 
     // owner map: tiles of 8^3 cells, with box lists where boxes share a tile
     G.map_shift = kMapShift;
@@ -639,6 +648,17 @@ int ocfd_level_destroy(ocfd_level lev)
 }
 
 int ocfd_level_nbox(ocfd_level lev) { return lev ? lev->dev.nbox : 0; }
+
+int ocfd_level_set_inflow_species(ocfd_level lev, const double* spec)
+{
+    if (!lev) { set_error("ocfd_level_set_inflow_species: invalid arguments"); return OCFD_EINVAL; }
+    LevelDev& L = lev->dev;
+    L.geom.has_spec_in = spec != nullptr;
+    for (int n = 0; n < OCFD_NSPEC; ++n) L.geom.spec_in[n] = spec ? spec[n] : 0.0;
+    OCFD_HIP(hipDeviceSynchronize());   // no launch may still read the old table
+    OCFD_HIP(hipMemcpy(L.d_geom, &L.geom, sizeof(GeomDesc), hipMemcpyHostToDevice));
+    return OCFD_OK;
+}
 
 int ocfd_level_set_planar(ocfd_level lev, int on)
 {
@@ -919,16 +939,45 @@ int ocfd_level_autotune(ocfd_level lev, const double* S, const ocfd_parm* parm, 
 // ---------------------------------------------------------------------------
 // device-driven time steps and their graphs
 // ---------------------------------------------------------------------------
+} // extern "C"
+
+namespace ocfd {
+
+void launch_fill_ctl(const LevelDev& L, double* S, const ocfd_step_ctl* ctl, double toff, hipStream_t s)
+{
+    int ng;
+    const int4* tg = fill_tiles(L, ng);
+    if (ng) hipLaunchKernelGGL(k_fill, dim3(ng), dim3(256), 0, s, L.d_boxes, L.d_geom, tg, S, 0.0, ctl, toff);
+}
+
+void launch_ctl_start(const LevelDev& L, const double* A, const PhysArgs& ph, ocfd_step_ctl* ctl, double time,
+                      double cfl, double stop_time, int max_steps, double fixed_est, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, ctl, time, cfl, stop_time, max_steps);
+    hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, A, ph, &ctl->red, &ctl->bad,
+                       (const ocfd_step_ctl*)nullptr);
+    if (fixed_est > 0.0) hipLaunchKernelGGL(k_ctl_fixed_est, dim3(1), dim3(1), 0, s, ctl, fixed_est);
+    hipLaunchKernelGGL(k_ctl_first, dim3(1), dim3(1), 0, s, ctl);
+}
+
+void launch_step_end(const LevelDev& L, const double* A, const PhysArgs& ph, ocfd_step_ctl* ctl, double fixed_est,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, A, ph, &ctl->red, &ctl->bad,
+                       (const ocfd_step_ctl*)ctl);
+    if (fixed_est > 0.0) hipLaunchKernelGGL(k_ctl_fixed_est, dim3(1), dim3(1), 0, s, ctl, fixed_est);
+    hipLaunchKernelGGL(k_ctl_update, dim3(1), dim3(1), 0, s, ctl);
+}
+
+} // namespace ocfd
+
+extern "C" {
+
 int ocfd_ctl_start(ocfd_level lev, const double* A, ocfd_step_ctl* ctl, double time, double cfl, double stop_time,
                    int max_steps, const ocfd_parm* parm, void* stream)
 {
     if (!lev || !A || !ctl || !parm || !(cfl > 0.0)) { set_error("ocfd_ctl_start: invalid arguments"); return OCFD_EINVAL; }
-    const LevelDev& L = lev->dev;
-    hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1), 0, s, ctl, time, cfl, stop_time, max_steps);
-    hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, A, phys_of(*parm), &ctl->red,
-                       &ctl->bad, (const ocfd_step_ctl*)nullptr);
-    hipLaunchKernelGGL(k_ctl_first, dim3(1), dim3(1), 0, s, ctl);
+    launch_ctl_start(lev->dev, A, phys_of(*parm), ctl, time, cfl, stop_time, max_steps, 0.0, (hipStream_t)stream);
     return check_launch("ocfd_ctl_start");
 }
 
@@ -938,13 +987,7 @@ static int step_device(ocfd_level lev, double* A, double* B, double* C, ocfd_ste
     const LevelDev& L = lev->dev;
     StageArgs a{};
     a.flags = flags; a.ph = phys_of(*parm); a.ctl = ctl; a.acc = L.d_acc;
-    auto fill = [&](double* S, double toff) {
-        int ng;
-        const int4* tg = fill_tiles(L, ng);
-        if (ng)
-            hipLaunchKernelGGL(k_fill, dim3(ng), dim3(256), 0, s, L.d_boxes, L.d_geom, tg, S, 0.0,
-                               (const ocfd_step_ctl*)ctl, toff);
-    };
+    auto fill = [&](double* S, double toff) { launch_fill_ctl(L, S, ctl, toff, s); };
     // the CFL reduction of the new state rides on the stage that finishes
     // the step when every box is regular (the finishing sweep sees each new
     // cell once); levels with cut or covered boxes reduce in a pass of their own
@@ -975,9 +1018,9 @@ static int step_device(ocfd_level lev, double* A, double* B, double* C, ocfd_ste
         stage(C, A, A, 2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0, true);
     }
     if (!fold)
-        hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, A, a.ph, &ctl->red,
-                           &ctl->bad, (const ocfd_step_ctl*)ctl);
-    hipLaunchKernelGGL(k_ctl_update, dim3(1), dim3(1), 0, s, ctl);
+        launch_step_end(L, A, a.ph, ctl, 0.0, s);
+    else
+        hipLaunchKernelGGL(k_ctl_update, dim3(1), dim3(1), 0, s, ctl);
     return check_launch("ocfd_step_device");
 }
 
@@ -1006,31 +1049,29 @@ struct ocfd_graph_s {
     ocfd_level lev = nullptr;   // its work-lists and launch sizes are baked into the graph
 };
 
-extern "C" {
+namespace ocfd {
 
-int ocfd_graph_capture_steps(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
-                             const ocfd_parm* parm, int flags, int nsteps, void* stream, ocfd_graph* out)
+int capture_graph(ocfd_level lev, hipStream_t s, int (*body)(void*, hipStream_t), void* arg, ocfd_graph* out,
+                  const char* what)
 {
-    if (!step_args_ok(lev, A, B, C, ctl, scheme, parm, flags) || nsteps <= 0 || !out || !stream) {
-        set_error("ocfd_graph_capture_steps: invalid arguments (a non-null stream is required)");
-        return OCFD_EINVAL;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": hipStreamBeginCapture: " + hipGetErrorString(e));
+        return OCFD_EHIP;
     }
-    hipStream_t s = (hipStream_t)stream;
-    OCFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = OCFD_OK;
-    for (int n = 0; n < nsteps && rc == OCFD_OK; ++n) rc = step_device(lev, A, B, C, ctl, scheme, parm, flags, s);
+    int rc = body(arg, s);
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(s, &g);
+    e = hipStreamEndCapture(s, &g);
     if (rc != OCFD_OK || e != hipSuccess) {
         if (g) hipGraphDestroy(g);
-        if (rc == OCFD_OK) set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        if (rc == OCFD_OK) set_error(std::string(what) + ": hipStreamEndCapture: " + hipGetErrorString(e));
         return rc != OCFD_OK ? rc : OCFD_EHIP;
     }
     hipGraphExec_t x = nullptr;
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
         hipGraphDestroy(g);
-        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        set_error(std::string(what) + ": hipGraphInstantiate: " + hipGetErrorString(e));
         return OCFD_EHIP;
     }
     ocfd_graph G = new ocfd_graph_s;
@@ -1040,6 +1081,39 @@ int ocfd_graph_capture_steps(ocfd_level lev, double* A, double* B, double* C, oc
     ++lev->dev.live_graphs;
     *out = G;
     return OCFD_OK;
+}
+
+} // namespace ocfd
+
+namespace {
+struct StepsArgs {
+    ocfd_level lev;
+    double *A, *B, *C;
+    ocfd_step_ctl* ctl;
+    int scheme, flags, nsteps;
+    const ocfd_parm* parm;
+};
+int steps_body(void* p, hipStream_t s)
+{
+    const StepsArgs& a = *(const StepsArgs*)p;
+    int rc = OCFD_OK;
+    for (int n = 0; n < a.nsteps && rc == OCFD_OK; ++n)
+        rc = step_device(a.lev, a.A, a.B, a.C, a.ctl, a.scheme, a.parm, a.flags, s);
+    return rc;
+}
+} // namespace
+
+extern "C" {
+
+int ocfd_graph_capture_steps(ocfd_level lev, double* A, double* B, double* C, ocfd_step_ctl* ctl, int scheme,
+                             const ocfd_parm* parm, int flags, int nsteps, void* stream, ocfd_graph* out)
+{
+    if (!step_args_ok(lev, A, B, C, ctl, scheme, parm, flags) || nsteps <= 0 || !out || !stream) {
+        set_error("ocfd_graph_capture_steps: invalid arguments (a non-null stream is required)");
+        return OCFD_EINVAL;
+    }
+    StepsArgs a{lev, A, B, C, ctl, scheme, flags, nsteps, parm};
+    return capture_graph(lev, (hipStream_t)stream, steps_body, &a, out, "ocfd_graph_capture_steps");
 }
 
 int ocfd_graph_launch(ocfd_graph g, void* stream)

@@ -462,7 +462,8 @@ __global__ __launch_bounds__(256) void k_mdiv(const BoxDesc* __restrict__ boxes,
 }
 
 // ghost fill of a species MultiFab: every component takes the scalar BC of
-// the density (Source/Setup.cpp:241-247); ext_dir sides are left alone
+// the density (Source/Setup.cpp:241-247); ext_dir sides get the level's
+// inflow species when it has them, else they are left alone
 __global__ __launch_bounds__(256) void k_fill_spec(const BoxDesc* __restrict__ boxes, const GeomDesc* __restrict__ geoms,
                                                    const int4* __restrict__ tiles, double* __restrict__ P,
                                                    const ocfd_step_ctl* __restrict__ ctl)
@@ -494,7 +495,13 @@ __global__ __launch_bounds__(256) void k_fill_spec(const BoxDesc* __restrict__ b
         const int edge = outside[d] < 0 ? G.dom_lo[d] : G.dom_hi[d];
         if (bc == BC_FOEXT) src[d] = edge;
         else if (bc == BC_EVEN || bc == BC_ODD) src[d] = outside[d] < 0 ? 2 * edge - src[d] - 1 : 2 * edge - src[d] + 1;
-        else if (bc == BC_EXT) return;
+        else if (bc == BC_EXT) {
+            if (G.has_spec_in) {
+#pragma unroll
+                for (int n = 0; n < NS; ++n) P[dst + n * B.nptg] = G.spec_in[n];
+            }
+            return;
+        }
     }
     if (src[0] < G.dom_lo[0] || src[0] > G.dom_hi[0] || src[1] < G.dom_lo[1] || src[1] > G.dom_hi[1] ||
         src[2] < G.dom_lo[2] || src[2] > G.dom_hi[2])
@@ -703,6 +710,35 @@ void launch_fill_spec(const LevelDev& L, double* P, const ocfd_step_ctl* ctl, hi
 {
     if (L.ntile_g)
         hipLaunchKernelGGL(k_fill_spec, dim3(L.ntile_g), dim3(256), 0, s, L.d_boxes, L.d_geom, L.d_tile_g, P, ctl);
+}
+
+// every cell of every box (valid and ghost) set to one state and species
+// set: the premixed initial data and inflow of the reacting channel
+struct UniformState {
+    double u[NC];
+    double r[NS];
+};
+
+// grid: x over the grown cells of a box, y over the boxes
+__global__ __launch_bounds__(256) void k_init_uniform(const BoxDesc* __restrict__ boxes, double* __restrict__ S,
+                                                      double* __restrict__ P, UniformState v)
+{
+    const BoxDesc B = boxes[blockIdx.y];
+    const long p0 = B.off_s / NC * NS;
+    for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < B.nptg; c += (long)gridDim.x * 256) {
+#pragma unroll
+        for (int n = 0; n < NC; ++n) S[B.off_s + n * B.nptg + c] = v.u[n];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) P[p0 + n * B.nptg + c] = v.r[n];
+    }
+}
+
+// A <- B for the forward-Euler flow step under device control
+__global__ __launch_bounds__(256) void k_copy_ctl_m(double* __restrict__ dst, const double* __restrict__ src, long n,
+                                                    const ocfd_step_ctl* __restrict__ ctl)
+{
+    if (ctl_done(ctl)) return;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) dst[i] = src[i];
 }
 
 // Exec/5_CHEMSOD/prob.H:11-82 with prob_parm.H defaults: H2:O2 = 0.667:0.333
@@ -946,6 +982,117 @@ int ocfd_advance_multi(ocfd_level lev, double* A, double* Aspec, double* B, doub
     return ocfd_rk_stage_multi(lev, B, Bspec, A, Aspec, A, Aspec, 0.5, 0.5, 0.5 * dt, parm, flags, s);
 }
 
+} // extern "C"
+
+namespace {
+
+constexpr double kChemEstdt = 1e-8;   // the CHEM build's ebr_estdt (Source/Advance.cpp:54-57)
+
+struct ChemStep {
+    ocfd_level lev;
+    double *A, *Aspec, *B, *Bspec;
+    ocfd_step_ctl* ctl;
+    int scheme, level, flags, renorm, nsteps;
+    const ocfd_parm* parm;
+};
+
+// One Strang-split EBR::advance of a multispecies level with dt and time from
+// ctl (Source/Advance.cpp:152-177 with flow_advance_multi,
+// Source/CHEM/FlowAdvance.cpp:9-90), the renormalisation of post_timestep and
+// the dt update -- the launches of ocfd_advance_chem, under device control
+int chem_step(const ChemStep& c, hipStream_t s)
+{
+    LevelDev& L = c.lev->dev;
+    const int iter = c.level <= 2 ? 4 / (1 << c.level) : 1;
+    const double smallr = c.parm->smallr;
+    for (int n = 0; n < iter; ++n) launch_chem_level(L, c.A, c.Aspec, 0.0, smallr, c.ctl, iter, s);
+    auto stage = [&](const double* S, const double* P, const double* So, const double* Po, double* Sn, double* Pn,
+                     double cc, double co, double cdt) {
+        MultiArgs m = margs(L, S, P, So, Po, Sn, Pn, nullptr, nullptr, cc, co, cdt, c.parm, c.flags);
+        m.a.ctl = c.ctl;
+        launch_multi(L, m, s);
+    };
+    launch_fill_ctl(L, c.A, c.ctl, 0.0, s);
+    launch_fill_spec(L, c.Aspec, c.ctl, s);
+    stage(c.A, c.Aspec, nullptr, nullptr, c.B, c.Bspec, 1.0, 0.0, 1.0);
+    if (c.scheme == 0) {
+        const long n = L.mf_state, np = L.mf_state / NC * NS;
+        hipLaunchKernelGGL(k_copy_ctl_m, dim3((unsigned)std::min<long>((n + 255) / 256, 4096)), dim3(256), 0, s, c.A,
+                           c.B, n, (const ocfd_step_ctl*)c.ctl);
+        hipLaunchKernelGGL(k_copy_ctl_m, dim3((unsigned)std::min<long>((np + 255) / 256, 4096)), dim3(256), 0, s,
+                           c.Aspec, c.Bspec, np, (const ocfd_step_ctl*)c.ctl);
+    } else {
+        launch_fill_ctl(L, c.B, c.ctl, 1.0, s);
+        launch_fill_spec(L, c.Bspec, c.ctl, s);
+        stage(c.B, c.Bspec, c.A, c.Aspec, c.A, c.Aspec, 0.5, 0.5, 0.5);
+    }
+    for (int n = 0; n < iter; ++n) launch_chem_level(L, c.A, c.Aspec, 0.0, smallr, c.ctl, iter, s);
+    if (c.renorm) launch_renorm(L, c.A, c.Aspec, c.ctl, s);
+    launch_step_end(L, c.A, phys_of(*c.parm), c.ctl, kChemEstdt, s);
+    return mcheck("ocfd_step_device_chem");
+}
+
+int chem_steps_body(void* p, hipStream_t s)
+{
+    const ChemStep& c = *(const ChemStep*)p;
+    int rc = OCFD_OK;
+    for (int n = 0; n < c.nsteps && rc == OCFD_OK; ++n) rc = chem_step(c, s);
+    return rc;
+}
+
+bool chem_step_ok(const ChemStep& c, const char* what)
+{
+    if (c.lev && c.lev->dev.fill_gz != NG) {
+        set_error("multispecies stepping needs the full ghost fill depth (ocfd_level_set_planar off)");
+        return false;
+    }
+    const bool ok = c.lev && c.A && c.Aspec && c.B && c.Bspec && c.ctl && c.parm && c.level >= 0 &&
+                    (c.scheme == 0 || c.scheme == 1) && c.A != c.B && c.Aspec != c.Bspec;
+    if (!ok) set_error(std::string(what) + ": invalid arguments (scheme 0 Euler or 1 RK2)");
+    return ok;
+}
+
+} // namespace
+
+extern "C" {
+
+int ocfd_ctl_start_chem(ocfd_level lev, const double* A, ocfd_step_ctl* ctl, double time, double cfl,
+                        double stop_time, int max_steps, const ocfd_parm* parm, void* stream)
+{
+    if (!lev || !A || !ctl || !parm || !(cfl > 0.0)) {
+        set_error("ocfd_ctl_start_chem: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    launch_ctl_start(lev->dev, A, phys_of(*parm), ctl, time, cfl, stop_time, max_steps, kChemEstdt,
+                     (hipStream_t)stream);
+    return mcheck("ocfd_ctl_start_chem");
+}
+
+int ocfd_step_device_chem(ocfd_level lev, double* A, double* Aspec, double* B, double* Bspec, ocfd_step_ctl* ctl,
+                          int scheme, int level, const ocfd_parm* parm, int flags, int renorm, void* stream)
+{
+    ChemStep c{lev, A, Aspec, B, Bspec, ctl, scheme, level, flags, renorm, 1, parm};
+    if (!chem_step_ok(c, "ocfd_step_device_chem")) return OCFD_EINVAL;
+    int rc = ensure_multi(lev->dev);
+    if (rc) return rc;
+    return chem_step(c, (hipStream_t)stream);
+}
+
+int ocfd_graph_capture_steps_chem(ocfd_level lev, double* A, double* Aspec, double* B, double* Bspec,
+                                  ocfd_step_ctl* ctl, int scheme, int level, const ocfd_parm* parm, int flags,
+                                  int renorm, int nsteps, void* stream, ocfd_graph* out)
+{
+    ChemStep c{lev, A, Aspec, B, Bspec, ctl, scheme, level, flags, renorm, nsteps, parm};
+    if (!chem_step_ok(c, "ocfd_graph_capture_steps_chem")) return OCFD_EINVAL;
+    if (nsteps <= 0 || !out || !stream) {
+        set_error("ocfd_graph_capture_steps_chem: invalid arguments (a non-null stream is required)");
+        return OCFD_EINVAL;
+    }
+    int rc = ensure_multi(lev->dev);   // the scratch is allocated before the capture
+    if (rc) return rc;
+    return capture_graph(lev, (hipStream_t)stream, chem_steps_body, &c, out, "ocfd_graph_capture_steps_chem");
+}
+
 int ocfd_advance_chem(ocfd_level lev, double* A, double* Aspec, double* B, double* Bspec, double time, double dt,
                       int level, int scheme, const ocfd_parm* parm, int flags, int renorm, void* stream)
 {
@@ -968,6 +1115,28 @@ int ocfd_advance_chem(ocfd_level lev, double* A, double* Aspec, double* B, doubl
     for (int n = 0; n < iter; ++n)
         if ((rc = ocfd_chem_advance(lev, A, Aspec, dt2, parm, s))) return rc;
     if (renorm) return ocfd_species_renorm(lev, A, Aspec, s);
+    return OCFD_OK;
+}
+
+int ocfd_init_species_uniform(ocfd_level lev, const double* state, const double* spec, double* S, double* Spec)
+{
+    if (!lev || !state || !spec || !S || !Spec) {
+        set_error("ocfd_init_species_uniform: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    const LevelDev& L = lev->dev;
+    UniformState v;
+    for (int n = 0; n < NC; ++n) v.u[n] = state[n];
+    for (int n = 0; n < NS; ++n) v.r[n] = spec[n];
+    long maxg = 1;
+    for (const BoxDesc& B : L.boxes) maxg = std::max(maxg, B.nptg);
+    if (L.nbox > 0) {
+        const unsigned gx = (unsigned)std::min<long>((maxg + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_init_uniform, dim3(gx, (unsigned)L.nbox), dim3(256), 0, 0, L.d_boxes, S, Spec, v);
+    }
+    int rc = mcheck("ocfd_init_species_uniform");
+    if (rc) return rc;
+    if (hipDeviceSynchronize() != hipSuccess) return mcheck("ocfd_init_species_uniform");
     return OCFD_OK;
 }
 

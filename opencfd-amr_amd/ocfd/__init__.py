@@ -29,6 +29,7 @@ PLANAR = 8
 INTERIOR, INFLOW, OUTFLOW, SYMMETRY, SLIPWALL, NOSLIPWALL = range(6)
 
 CASE_SOD, CASE_DMR, CASE_KH3D, CASE_EB_INFLOW, CASE_CHEMSOD = 0, 1, 2, 3, 4
+CASE_CHEM_CHANNEL = 5   # premixed reacting channel: uniform data from ocfd_init_species_uniform
 
 
 class Parm(ctypes.Structure):
@@ -157,6 +158,15 @@ def lib():
         L.ocfd_advance_chem.argtypes = [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, vp]
         L.ocfd_init_species_state.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(Parm)]
+        L.ocfd_init_species_uniform.argtypes = [vp, dp, dp, vp, vp]
+        L.ocfd_level_set_inflow_species.argtypes = [vp, dp]
+        L.ocfd_ctl_start_chem.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int, ctypes.POINTER(Parm), vp]
+        L.ocfd_step_device_chem.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Parm),
+                                            ctypes.c_int, ctypes.c_int, vp]
+        L.ocfd_graph_capture_steps_chem.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                    vp, ctypes.POINTER(vp)]
         L.ocfd_tag_dengrad.argtypes = [vp, vp, ctypes.c_double, vp, vp]
         L.ocfd_level_set_march_chunk.argtypes = [vp, ctypes.c_int]
         L.ocfd_level_march_chunk.argtypes = [vp]
@@ -415,6 +425,36 @@ class Level:
         check(lib().ocfd_advance_chem(self.handle, _ptr(A), _ptr(As), _ptr(B), _ptr(Bs), time, dt, level, scheme,
                                       ctypes.byref(parm), flags, int(renorm), _stream(stream)), "ocfd_advance_chem")
 
+    def set_inflow_species(self, spec):
+        """Partial densities the species ghost fill writes on ext_dir sides (None clears)."""
+        dp = ctypes.POINTER(ctypes.c_double)
+        self._spec_in = None if spec is None else np.ascontiguousarray(spec, dtype=np.float64)
+        check(lib().ocfd_level_set_inflow_species(self.handle, None if spec is None else
+                                                  self._spec_in.ctypes.data_as(dp)), "ocfd_level_set_inflow_species")
+
+    def init_species_uniform(self, state, spec, S, Spec):
+        """Every cell (ghosts included) of S and Spec set to `state` (5) and `spec` (9)."""
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        sp = np.ascontiguousarray(spec, dtype=np.float64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        check(lib().ocfd_init_species_uniform(self.handle, st.ctypes.data_as(dp), sp.ctypes.data_as(dp), _ptr(S),
+                                              _ptr(Spec)), "ocfd_init_species_uniform")
+
+    def ctl_start_chem(self, A, ctl, parm, cfl, time=0.0, stop_time=-1.0, max_steps=-1, stream=None):
+        check(lib().ocfd_ctl_start_chem(self.handle, _ptr(A), _ptr(ctl), time, cfl, stop_time, max_steps,
+                                        ctypes.byref(parm), _stream(stream)), "ocfd_ctl_start_chem")
+
+    def step_device_chem(self, A, As, B, Bs, ctl, parm, scheme=1, level=0, flags=0, renorm=True, stream=None):
+        check(lib().ocfd_step_device_chem(self.handle, _ptr(A), _ptr(As), _ptr(B), _ptr(Bs), _ptr(ctl), scheme,
+                                          level, ctypes.byref(parm), flags, int(renorm), _stream(stream)),
+              "ocfd_step_device_chem")
+
+    def capture_steps_chem(self, A, As, B, Bs, ctl, parm, nsteps, scheme=1, level=0, flags=0, renorm=True,
+                           stream=None):
+        """hipGraph of nsteps device-driven multispecies steps (stream must not be None)."""
+        return StepGraph(self, A, B, None, ctl, parm, nsteps, scheme, flags, stream,
+                         chem=(As, Bs, level, int(renorm)))
+
     def init_species_state(self, kind, S, Spec, parm):
         check(lib().ocfd_init_species_state(self.handle, kind, _ptr(S), _ptr(Spec), ctypes.byref(parm)),
               "ocfd_init_species_state")
@@ -494,16 +534,23 @@ def read_ctl(ctl):
 class StepGraph:
     """hipGraph of device-driven time steps; replayed with launch()."""
 
-    def __init__(self, level, A, B, C, ctl, parm, nsteps, scheme=2, flags=0, stream=None):
+    def __init__(self, level, A, B, C, ctl, parm, nsteps, scheme=2, flags=0, stream=None, chem=None):
         if stream is None:
             raise ValueError("graph capture needs a non-default stream")
         self.level = level                     # keeps the level's work-lists alive
-        self.keep = (A, B, C, ctl)            # and the buffers the graph writes
+        self.keep = (A, B, C, ctl, chem)      # and the buffers the graph writes
         self.parm = parm
         h = ctypes.c_void_p()
-        check(lib().ocfd_graph_capture_steps(level.handle, _ptr(A), _ptr(B), _ptr(C), _ptr(ctl), scheme,
-                                             ctypes.byref(parm), flags, nsteps, _stream(stream), ctypes.byref(h)),
-              "ocfd_graph_capture_steps")
+        if chem is None:
+            check(lib().ocfd_graph_capture_steps(level.handle, _ptr(A), _ptr(B), _ptr(C), _ptr(ctl), scheme,
+                                                 ctypes.byref(parm), flags, nsteps, _stream(stream),
+                                                 ctypes.byref(h)), "ocfd_graph_capture_steps")
+        else:
+            As, Bs, lev, renorm = chem
+            check(lib().ocfd_graph_capture_steps_chem(level.handle, _ptr(A), _ptr(As), _ptr(B), _ptr(Bs), _ptr(ctl),
+                                                      scheme, lev, ctypes.byref(parm), flags, renorm, nsteps,
+                                                      _stream(stream), ctypes.byref(h)),
+                  "ocfd_graph_capture_steps_chem")
         self.handle = h
         self.nsteps = nsteps
 
@@ -595,6 +642,19 @@ class Reactors:
         check(lib().ocfd_reactor_temperature(self.n, _ptr(self.state), _ptr(T), _stream(stream)),
               "ocfd_reactor_temperature")
         return T
+
+
+def premixed_state(T0, p0, phi, u0=0.0):
+    """Conserved state (rho, rho u, rho v, rho w, E) and the 9 partial
+    densities of premixed H2/air (X(H2):X(O2):X(N2) = 2phi:1:3.76) at T0 [K],
+    p0 [Pa], moving at u0 [m/s] along x; E is the sensible energy density plus
+    the kinetic energy (the library's reactor initialisation, CKXTY/CKRHOY/
+    CKUBMS of Source/CHEM/LiDryer.H)."""
+    R = Reactors([T0], [p0], [phi])
+    rows = R.rows()[:, 0].cpu().numpy()
+    rho, E, rhoi = rows[0], rows[1], rows[2:].copy()
+    state = np.array([rho, rho * u0, 0.0, 0.0, E + 0.5 * rho * u0 * u0])
+    return state, rhoi
 
 
 def ignition_batch(n, seed=7):

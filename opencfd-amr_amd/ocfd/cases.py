@@ -96,6 +96,26 @@ def eb_channel(nx=512, ny=128, nz=2, max_grid=32, radius=0.1, center=(0.5, 0.25)
     return boxes, g, CASE_EB_INFLOW, eb_cylinder(center[0], center[1], radius, small_volfrac)
 
 
+def chem_channel(nx=512, ny=128, nz=2, max_grid=32, T0=1100.0, p0=101325.0, phi=1.0, u0=200.0, radius=0.1,
+                 center=(0.5, 0.25), small_volfrac=0.05):
+    """Reacting premixed H2/air channel past a cylinder: a 2 x 0.5 m channel
+    (nx x ny cells, nz periodic z layers), a cylinder of diameter 0.2 m,
+    premixed stoichiometric H2/air at 1100 K and 1 atm flowing in at x-lo at
+    200 m/s (constant inflow of the same mixture), outflow at x-hi, slip walls
+    in y.  9 species, the multispecies path with the EB-CHEM cut boxes.
+    The initial data fill the whole channel with the inflow mixture.
+    Returns (boxes, geom, kind, eb spec, (state, partial densities))."""
+    from . import premixed_state, CASE_CHEM_CHANNEL
+    n = (nx, ny, nz)
+    dx = 2.0 / nx
+    state, spec = premixed_state(T0, p0, phi, u0)
+    g = make_geom(n, (0.0, 0.0, 0.0), (2.0, 0.5 * ny / (nx / 4), nz * dx), periodic=(0, 0, 1),
+                  lo_bc=(INFLOW, SLIPWALL, INTERIOR), hi_bc=(OUTFLOW, SLIPWALL, INTERIOR), probtype=2,
+                  left_state=state, right_state=state)
+    boxes = chop(n, (max_grid, max_grid, nz))
+    return boxes, g, CASE_CHEM_CHANNEL, eb_cylinder(center[0], center[1], radius, small_volfrac), (state, spec)
+
+
 def chemsod(nx=1280, ny=4, nz=4, max_grid=128):
     """H2/O2 shock tube with a hot high-pressure driver (Exec/5_CHEMSOD/inputs):
     dx = 1e-4 m, slip wall at x-lo, outflow at x-hi, periodic in y and z."""

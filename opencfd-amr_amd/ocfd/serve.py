@@ -22,15 +22,20 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (CTL_BYTES, NCONS, NGROW, PLANAR, Level, Reactors, StepCtl, default_parm, ignition_batch, new_ctl,
-               REACTOR_NCOMP)
+from . import (CASE_CHEM_CHANNEL, CTL_BYTES, NCONS, NGROW, NSPEC, PLANAR, Level, Reactors, StepCtl, default_parm,
+               ignition_batch, new_ctl, REACTOR_NCOMP)
 
 MiB = 1 << 20
 
 
 @dataclass
 class FlowRequest:
-    """A flow case: `make()` returns (boxes, geom, init kind) as in ocfd.cases."""
+    """A flow case: `make()` returns (boxes, geom, init kind) as in ocfd.cases,
+    or (boxes, geom, kind, eb spec) for an embedded body, or for a reacting
+    multispecies case (kind CASE_CHEM_CHANNEL) (boxes, geom, kind, eb spec,
+    (state, partial densities)) -- the uniform initial data.  A reacting case
+    steps with the Strang-split multispecies advance (scheme 0 Euler or
+    1 RK2 for its flow part) and the CHEM build's dt = cfl * 1e-8."""
     name: str
     make: object
     flags: int = 0
@@ -42,18 +47,37 @@ class FlowRequest:
     parm_kw: dict = field(default_factory=dict)
     arrival: float = 0.0          # seconds after the loop starts
 
+    def made(self):
+        """The case set-up, made once (it may run device work)."""
+        if getattr(self, "_made", None) is None:
+            self._made = self.make()
+        return self._made
+
+    def is_chem(self):
+        return self.made()[2] == CASE_CHEM_CHANNEL
+
     def footprint(self, boxes=None):
-        """Device bytes while resident: 3 state MultiFabs (4 ghosts, 5 comps,
-        FP64), the level's FP32+FP64 primitive scratch (grow 4, 6 comps), the
-        partial divergence scratch (5 comps valid) and the small tables."""
+        """Device bytes while resident.  Flow cases: 3 state MultiFabs (4 ghosts,
+        5 comps, FP64), the level's FP32+FP64 primitive scratch (grow 4, 6
+        comps), the partial divergence scratch (5 comps valid) and the small
+        tables.  Reacting cases: 2 state and 2 species MultiFabs (9 comps), the
+        same level scratch, the multispecies scratch (19 primitive/transport
+        and 3 x 14 face-flux comps per grown cell) and, bounded by every box,
+        the embedded-boundary fields and scratch (flags, 4 geometry, 7
+        primitive and 3 x 5 face-flux comps)."""
+        chem = boxes is None and self.is_chem()
         if boxes is None:
-            boxes = self.make()[0]
+            boxes = self.made()[0]
         ng = nv = 0
         for lo, hi in boxes:
             n = [hi[d] - lo[d] + 1 for d in range(3)]
             nv += n[0] * n[1] * n[2]
             ng += (n[0] + 2 * NGROW) * (n[1] + 2 * NGROW) * (n[2] + 2 * NGROW)
-        return 3 * 8 * NCONS * ng + 12 * 6 * ng + 8 * NCONS * nv + 128 * len(boxes) * 16 + MiB
+        tables = 128 * len(boxes) * 16 + MiB
+        if chem:
+            return (2 * 8 * NCONS * ng + 2 * 8 * NSPEC * ng + 12 * 6 * ng + 8 * NCONS * nv + 8 * (19 + 3 * 14) * ng +
+                    (4 + 8 * (4 + 7 + 3 * NCONS)) * ng + tables)
+        return 3 * 8 * NCONS * ng + 12 * 6 * ng + 8 * NCONS * nv + tables
 
 
 @dataclass
@@ -115,23 +139,37 @@ class _FlowJob:
     def __init__(self, req, stream, chunk, torch):
         self.req = req
         self.stream = stream
-        boxes, geom, kind = req.make()
-        self.level = Level(boxes, geom)
+        made = req.made()
+        boxes, geom, kind = made[:3]
+        eb = made[3] if len(made) > 3 else None
+        chem = kind == CASE_CHEM_CHANNEL
+        self.level = Level(boxes, geom, eb=eb)
         if req.flags & PLANAR:
             self.level.set_planar(True)
         self.parm = default_parm(**req.parm_kw)
         with torch.cuda.stream(stream):
             self.A = self.level.new_mf()
             self.B = self.level.new_mf()
-            self.C = self.level.new_mf()
+            self.C = None if chem else self.level.new_mf()
+            self.As = self.level.new_spec_mf() if chem else None
+            self.Bs = self.level.new_spec_mf() if chem else None
             self.ctl = new_ctl()
             # one pinned copy of the control block per launch group in flight
             self.host = [torch.zeros(CTL_BYTES, dtype=torch.uint8).pin_memory() for _ in range(self.DEPTH)]
         stream.synchronize()   # the zero fills above ran on this stream
-        self.level.init_state(kind, self.A, self.parm, req.seed)
-        self.level.ctl_start(self.A, self.ctl, self.parm, req.cfl, 0.0, req.stop_time, req.max_steps, stream)
-        self.graph = self.level.capture_steps(self.A, self.B, self.C, self.ctl, self.parm, chunk, req.scheme,
-                                              req.flags, stream)
+        if chem:
+            state, spec = made[4]
+            self.level.set_inflow_species(spec)
+            self.level.init_species_uniform(state, spec, self.A, self.As)
+            self.level.ctl_start_chem(self.A, self.ctl, self.parm, req.cfl, 0.0, req.stop_time, req.max_steps,
+                                      stream)
+            self.graph = self.level.capture_steps_chem(self.A, self.As, self.B, self.Bs, self.ctl, self.parm, chunk,
+                                                       req.scheme, 0, req.flags, True, stream)
+        else:
+            self.level.init_state(kind, self.A, self.parm, req.seed)
+            self.level.ctl_start(self.A, self.ctl, self.parm, req.cfl, 0.0, req.stop_time, req.max_steps, stream)
+            self.graph = self.level.capture_steps(self.A, self.B, self.C, self.ctl, self.parm, chunk, req.scheme,
+                                                  req.flags, stream)
         self.groups = _Groups(torch, self.DEPTH)
         self.cells = sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for lo, hi in boxes)
 
@@ -143,22 +181,26 @@ class _FlowJob:
         self.groups.record(i, self.stream)
 
     def finished(self):
-        """True once a completed launch group found the run done."""
+        """True once a completed launch group found the run done; a run that
+        stopped on a non-finite state raises."""
         done = False
         for i in self.groups.completed():
             c = StepCtl.from_buffer_copy(self.host[i].numpy().tobytes())
             self.steps, self.time = c.steps, c.time
+            if c.status:
+                raise RuntimeError(f"served case {self.req.name} stopped at step {c.steps}: status {c.status} "
+                                   "(non-finite state)")
             done = done or bool(c.done)
         return done
 
     def result(self):
-        return self.A
+        return self.A if self.As is None else (self.A, self.As)
 
     def release(self):
         """Keep what the device may still be reading until the loop ends:
         a level's destructor frees with hipFree, which waits for the device."""
-        g = (self.level, self.graph)
-        self.A = self.B = self.C = self.ctl = None
+        g = (self.graph, self.level)
+        self.A = self.B = self.C = self.As = self.Bs = self.ctl = None
         self.level = self.graph = None
         return g
 
@@ -292,7 +334,9 @@ class Server:
                     rec["steps"] = job.steps
                     if self.keep_results:
                         with torch.cuda.stream(job.stream):
-                            self.results[rec["name"]] = job.result().cpu()
+                            res = job.result()
+                            self.results[rec["name"]] = (tuple(x.cpu() for x in res) if isinstance(res, tuple)
+                                                         else res.cpu())
                     self.records.append(rec)
                 if job.groups.drained():
                     graveyard.append(job.release())
@@ -318,13 +362,15 @@ def latency_stats(records):
 def poisson_mix(n, rate, seed=99, kinds=None):
     """n requests with exponential inter-arrival times at `rate` per second
     over the served case mix: 2D Sod/DMR slabs (64^2 .. 1024x256, FP32 flux),
-    3D periodic KH boxes (64^3 .. 128^3, viscous, FP32 flux) and 0D ignition
-    batches (1k .. 16k reactors)."""
+    3D periodic KH boxes (64^3 .. 128^3, viscous, FP32 flux), the 2D reacting
+    H2/air channel past a cylinder (512x128, 9 species, viscous, RK2, cfl 20
+    as in Exec/9_CHEMEB/inputs, 10 steps) and 0D ignition batches
+    (1k .. 16k reactors)."""
     from . import FLUX_FP32, PLANAR, VISC, cases
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
-    kinds = kinds or ("sod", "dmr", "kh", "reactors")
+    kinds = kinds or ("sod", "dmr", "kh", "chemeb", "reactors")
     for q in range(n):
         t += rng.exponential(1.0 / rate)
         kind = kinds[int(rng.integers(len(kinds)))]
@@ -343,6 +389,9 @@ def poisson_mix(n, rate, seed=99, kinds=None):
             n3 = int(rng.choice([64, 96, 128]))
             out.append(FlowRequest(f"kh{q}", (lambda n3=n3: cases.kh3d(n3)), flags=FLUX_FP32 | VISC, cfl=0.5,
                                    max_steps=10, seed=42, arrival=t))
+        elif kind == "chemeb":
+            out.append(FlowRequest(f"chemeb{q}", (lambda: cases.chem_channel(512, 128, 2, max_grid=64)), flags=VISC,
+                                   cfl=20.0, scheme=1, max_steps=10, arrival=t))
         else:
             nr = int(rng.choice([1024, 4096, 16384]))
             out.append(ReactorRequest(f"ign{q}", nr, seed=int(rng.integers(1 << 30)), t_end=1e-4, arrival=t))

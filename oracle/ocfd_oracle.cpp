@@ -976,6 +976,11 @@ struct Level {
 // then the case's ext_dir fill (Exec/1_DMR/prob_bcfill.cpp:14-58).
 // species: fill a 9-component species MultiFab; every component takes the
 // scalar (density) BC, Source/Setup.cpp:241-247, and no ext_dir values exist.
+// species written into ext_dir ghost cells of species MultiFabs (set with
+// orc_set_inflow_species; the library's ocfd_level_set_inflow_species)
+bool g_spec_in_set = false;
+double g_spec_in[9];
+
 int fill_level(const Level& lev, const orc_geom& g, double time, int species = 0)
 {
     int bad = 0;
@@ -1026,7 +1031,10 @@ int fill_level(const Level& lev, const orc_geom& g, double time, int species = 0
                     }
                     (void)any_out;
                     if (species) {
-                        if (ext) bad += 1;
+                        // ext_dir sides of a species MultiFab: the inflow
+                        // species when set, else they keep their values
+                        if (ext && g_spec_in_set)
+                            for (int n = 0; n < f.nc; ++n) f(i, j, k, n) = g_spec_in[n];
                     } else if (g.probtype == 1) {
                         // Exec/1_DMR/prob_bcfill.cpp:27-57; bc of component 0 decides
                         double x = g.prob_lo[0] + (i + 0.5) * g.dx[0];
@@ -1731,6 +1739,13 @@ void orc_vl_species_flux_many(long n, const double* ul, const double* ur, const 
                               const double* rl, const double* rr, double* F)
 {
     for (long m = 0; m < n; ++m) vl_species_flux(ul[m], ur[m], cL[m], cR[m], rl + NSP * m, rr + NSP * m, F + NSP * m);
+}
+
+// inflow species of the species ghost fill (NULL clears)
+void orc_set_inflow_species(const double* spec)
+{
+    g_spec_in_set = spec != nullptr;
+    for (int n = 0; n < 9; ++n) g_spec_in[n] = spec ? spec[n] : 0.0;
 }
 
 // compute_dSdt_multi for one regular box; S, Spec cover grow(bx,4)

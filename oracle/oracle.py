@@ -592,6 +592,7 @@ def _multi_sigs():
     L.orc_compute_dsdt_multi_box_faces.argtypes = [ip, ip, F, F, dp, ctypes.c_int, ctypes.POINTER(OParm),
                                                    ctypes.POINTER(dp)]
     L.orc_vl_species_flux_many.argtypes = [ctypes.c_long, dp, dp, dp, dp, dp, dp, dp]
+    L.orc_set_inflow_species.argtypes = [dp]
     L._multi_ready = True
     return L
 
@@ -624,6 +625,21 @@ def compute_multi_faces(lo, hi, S, Sp, p, dxinv, do_visc=0):
     L.orc_compute_dsdt_multi_box_faces((ctypes.c_int * 3)(*lo), (ctypes.c_int * 3)(*hi), ctypes.byref(S.c),
                                        ctypes.byref(Sp.c), _dp(dxi), do_visc, ctypes.byref(p), ptrs)
     return outs
+
+
+_spec_in_keep = None
+
+
+def set_inflow_species(spec):
+    """Species the ghost fill writes on ext_dir sides (None clears)."""
+    global _spec_in_keep
+    L = _multi_sigs()
+    if spec is None:
+        _spec_in_keep = None
+        L.orc_set_inflow_species(None)
+        return
+    _spec_in_keep = np.ascontiguousarray(spec, dtype=np.float64)
+    L.orc_set_inflow_species(_dp(_spec_in_keep))
 
 
 def vl_species_flux(ul, ur, cL, cR, rl, rr):
