@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--amr-steps", type=int, default=10)
     ap.add_argument("--dmr-to-t", action="store_true",
                     help="also run the AMR double Mach reflection to t = 0.2 in both precision forms")
+    ap.add_argument("--no-channel", action="store_true", help="skip the reacting channel")
+    ap.add_argument("--channel-only", action="store_true", help="only the reacting channel (for profiling)")
+    ap.add_argument("--channel-steps", type=int, default=20)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (HBM counter calibration)")
     return ap.parse_args()
@@ -251,6 +254,88 @@ def serving(nreq, rate, seed=99):
             "p50_ms_by_kind": {k: float(np.percentile(v, 50)) for k, v in kinds.items()},
             "hbm_budget_GiB": 32, "slots": 8, "peak_resident": srv.peak_active,
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))}
+
+
+def chem_channel(steps=20, warmup=5, nx=512, ny=128, max_grid=64, cpu_sample=True):
+    """The reacting H2/air channel past a cylinder (nx x ny x 2 cells,
+    max_grid boxes, regular boxes through the multispecies WENO path, cut
+    boxes through the EB-CHEM path): Strang-split RK2 steps at CFL 20 x 1e-8 s,
+    device-driven and replayed from a hipGraph of `steps` steps.  The oracle's
+    identical step is timed on the 128 x 32 x 2 channel for the CPU line."""
+    import torch
+    import ocfd
+    from ocfd import cases
+    boxes, g, kind, sp, (state, spec) = cases.chem_channel(nx, ny, 2, max_grid=max_grid)
+    lev = ocfd.Level(boxes, g, eb=sp)
+    lev.set_inflow_species(spec)
+    A, As, B, Bs = lev.new_mf(), lev.new_spec_mf(), lev.new_mf(), lev.new_spec_mf()
+    p = ocfd.default_parm()
+    cfl = 20.0
+    lev.init_species_uniform(state, spec, A, As)
+    s = torch.cuda.Stream()
+    ctl = ocfd.new_ctl()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        lev.ctl_start_chem(A, ctl, p, cfl, stream=s)
+        Gw = lev.capture_steps_chem(A, As, B, Bs, ctl, p, warmup, scheme=1, flags=ocfd.VISC, stream=s)
+        G = lev.capture_steps_chem(A, As, B, Bs, ctl, p, steps, scheme=1, flags=ocfd.VISC, stream=s)
+        Gw.launch(s)
+    s.synchronize()
+    t0 = time.perf_counter()
+    G.launch(s)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    c = ocfd.read_ctl(ctl)
+    del G, Gw
+    ncell = nx * ny * 2
+    kinds = lev.box_types()
+    out = {"workload": f"reacting H2/air channel past a cylinder, {nx} x {ny} x 2 cells, {len(boxes)} boxes "
+                       f"({kinds.count(0)} regular, {kinds.count(1)} cut, {kinds.count(2)} covered), "
+                       "Strang-split RK2 + linearised-implicit chemistry, dt = 20 x 1e-8 s",
+           "steps": steps, "ms_per_step": 1e3 * el / steps, "Mcell_updates_per_s": ncell * steps / el / 1e6,
+           "status": c.status, "steps_done": c.steps, "state_finite": bool(torch.isfinite(A).all()),
+           "form": "device-driven steps replayed from a hipGraph"}
+    if cpu_sample:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import ctypes
+        import oracle as orc
+        cb, cg, _, csp, (cst, csn) = cases.chem_channel(128, 32, 2, max_grid=16)
+        og = orc.OGeom()
+        ctypes.memmove(ctypes.byref(og), ctypes.byref(cg), ctypes.sizeof(og))
+        ebs = []
+        for lo, hi in cb:
+            glo, ghi = orc.grown(lo, hi)
+            gm = ocfd.eb_geometry(csp, cg, lo, hi, 4)
+            ebs.append(orc.EbFab(glo, ghi, gm["flag"], gm["vfrac"], gm["apx"], gm["apy"], gm["apz"]))
+        fa, fs = orc.new_level_fabs(cb), orc.new_spec_fabs(cb)
+        fb, fbs = orc.new_level_fabs(cb), orc.new_spec_fabs(cb)
+        for f, fsp in zip(fa, fs):
+            f.a[:] = cst[:, None, None, None]
+            fsp.a[:] = csn[:, None, None, None]
+        orc.set_eb(ebs)
+        orc.set_inflow_species(csn)
+        dt = cfl * ocfd.CHEM_ESTDT
+        state = {"t": 0.0}
+
+        nsamp = 10
+
+        def ten_steps():
+            for _ in range(nsamp):
+                assert orc.advance_chem(cb, fa, fs, fb, fbs, og, orc.parm(), state["t"], dt, 0, 1, 1, 1) == 0
+                state["t"] += dt
+        try:
+            med, ts = median_of(ten_steps)
+        finally:
+            orc.set_eb(None)
+            orc.set_inflow_species(None)
+        ncpu, model = host_cpu()
+        out["cpu_baseline"] = {"value": 128 * 32 * 2 * nsamp / med / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle Strang-split steps of the 128 x 32 x 2 channel (16 x 16 boxes), "
+                                         f"{nsamp} steps per run, "
+                                         f"median of 5 after 1 warm-up ({', '.join(f'{t:.2f}' for t in ts)} s)",
+                               "nproc": ncpu, "cpu_model": model}
+    return out
 
 
 def _cells(boxes):
@@ -483,6 +568,9 @@ def main():
     if args.calib_copy:
         calib_copy()
         return
+    if args.channel_only:
+        print(json.dumps({"chem_channel": chem_channel(args.channel_steps, cpu_sample=False)}))
+        return
     if args.amr_only:
         out = {"amr_dmr": amr_dmr(args.amr_steps, cpu_sample=False)}
         if args.dmr_to_t:
@@ -537,6 +625,10 @@ def main():
     if not args.no_serve:
         serve_res = serving(args.serve_requests, args.serve_rate)
 
+    channel = None
+    if not args.no_channel:
+        channel = chem_channel(args.channel_steps, cpu_sample=not args.no_cpu_baseline)
+
     chem = None
     if not args.no_chem:
         chem = chemistry_0d(args.reactors)
@@ -582,6 +674,7 @@ def main():
         "amr_dmr": amr_res,
         "cpu_baseline": cpu,
         "chemistry_0d": chem,
+        "chem_channel": channel,
     }
     print(json.dumps(out))
 

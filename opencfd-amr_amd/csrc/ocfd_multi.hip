@@ -221,12 +221,12 @@ __device__ __forceinline__ long gl(const BoxDesc& B, int i, int j, int k)
 
 // c2prim_rgas (Source/kernel/Kernels.H:42-81) and the transport properties
 // (FlowAdvance.cpp:188-202) on grow(bx,3) of a regular box
-__global__ __launch_bounds__(64) void k_mprims(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+__global__ __launch_bounds__(256) void k_mprims(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                MultiArgs m)
 {
     if (ctl_done(m.a.ctl)) return;
     const int4 t = tiles[blockIdx.x];
-    for (int c = threadIdx.x; c < t.z; c += 64) {
+    for (int c = threadIdx.x; c < t.z; c += 256) {   // tiles hold at most 256 cells
         const BoxDesc B = boxes[t.x];
         const int g = t.w;
         const int nx = B.hi[0] - B.lo[0] + 1 + 2 * g, ny = B.hi[1] - B.lo[1] + 1 + 2 * g;
@@ -252,7 +252,11 @@ __global__ __launch_bounds__(64) void k_mprims(const BoxDesc* __restrict__ boxes
         double* q = m.mq + B.off_s / NC * MQ_N + l0;
         q[MQ_R * st] = rho; q[MQ_U * st] = ux; q[MQ_V * st] = uy; q[MQ_W * st] = uz;
         q[MQ_P * st] = p; q[MQ_T * st] = T; q[MQ_C * st] = sqrt(gama * p / rho); q[MQ_G * st] = gama;
-        if (m.a.flags & OCFD_VISC) {
+        // the transport properties are read at the two cells of each face,
+        // all within grow(bx,1)
+        const bool face_cell = i >= B.lo[0] - 1 && i <= B.hi[0] + 1 && j >= B.lo[1] - 1 && j <= B.hi[1] + 1 &&
+                               k >= B.lo[2] - 1 && k <= B.hi[2] + 1;
+        if ((m.a.flags & OCFD_VISC) && face_cell) {
             double Y[NS], X[NS];
 #pragma unroll
             for (int n = 0; n < NS; ++n) Y[n] = rhoi[n] / rho;
@@ -321,12 +325,12 @@ __device__ __forceinline__ void msplit(const double* q, long st, double* fp, dou
 #pragma clang fp contract(on)
 
 template <int D>
-__global__ __launch_bounds__(64) void k_mface(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+__global__ __launch_bounds__(256) void k_mface(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                               MultiArgs m)
 {
     if (ctl_done(m.a.ctl)) return;
     const int4 t = tiles[blockIdx.x];
-    for (int c = threadIdx.x; c < t.z; c += 64) {
+    for (int c = threadIdx.x; c < t.z; c += 256) {   // tiles hold at most 256 cells
         const BoxDesc B = boxes[t.x];
         const int di = D == 0, dj = D == 1, dk = D == 2;
         const int nx = B.hi[0] - B.lo[0] + 1 + di, ny = B.hi[1] - B.lo[1] + 1 + dj;
@@ -546,12 +550,12 @@ __device__ __forceinline__ void mrecon(const double* x, long l0, long ds, bool c
 // species face flux of a cut box: MUSCL states of u_D, c and rho_k, Van Leer
 // mass-flux split (FlowAdvanceBox.cpp:174-223), Fickian flux when viscous
 template <int D>
-__global__ __launch_bounds__(64) void k_ebchem_face(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
+__global__ __launch_bounds__(256) void k_ebchem_face(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                     MultiArgs m, const uint32_t* __restrict__ ebflag)
 {
     if (ctl_done(m.a.ctl)) return;
     const int4 t = tiles[blockIdx.x];
-    for (int c = threadIdx.x; c < t.z; c += 64) {
+    for (int c = threadIdx.x; c < t.z; c += 256) {   // tiles hold at most 256 cells
         const BoxDesc B = boxes[t.x];
         const int di = D == 0, dj = D == 1, dk = D == 2;
         const int nx = B.hi[0] - B.lo[0] + 1 + di, ny = B.hi[1] - B.lo[1] + 1 + dj;
@@ -684,21 +688,21 @@ __global__ __launch_bounds__(256) void k_ebchem_div(const BoxDesc* __restrict__ 
 
 void launch_multi(const LevelDev& L, const MultiArgs& m, hipStream_t s)
 {
-    if (L.ntile_p) hipLaunchKernelGGL(k_mprims, dim3(L.ntile_p), dim3(64), 0, s, L.d_boxes, L.d_tile_p, m);
+    if (L.ntile_p) hipLaunchKernelGGL(k_mprims, dim3(L.ntile_p), dim3(256), 0, s, L.d_boxes, L.d_tile_p, m);
     if (L.ntile_mf[0]) {
-        hipLaunchKernelGGL(k_mface<0>, dim3(L.ntile_mf[0]), dim3(64), 0, s, L.d_boxes, L.d_tile_mf[0], m);
-        hipLaunchKernelGGL(k_mface<1>, dim3(L.ntile_mf[1]), dim3(64), 0, s, L.d_boxes, L.d_tile_mf[1], m);
-        hipLaunchKernelGGL(k_mface<2>, dim3(L.ntile_mf[2]), dim3(64), 0, s, L.d_boxes, L.d_tile_mf[2], m);
+        hipLaunchKernelGGL(k_mface<0>, dim3(L.ntile_mf[0]), dim3(256), 0, s, L.d_boxes, L.d_tile_mf[0], m);
+        hipLaunchKernelGGL(k_mface<1>, dim3(L.ntile_mf[1]), dim3(256), 0, s, L.d_boxes, L.d_tile_mf[1], m);
+        hipLaunchKernelGGL(k_mface<2>, dim3(L.ntile_mf[2]), dim3(256), 0, s, L.d_boxes, L.d_tile_mf[2], m);
     }
     if (L.ntile_md) hipLaunchKernelGGL(k_mdiv, dim3(L.ntile_md), dim3(256), 0, s, L.d_boxes, L.d_tile_md, m);
     // cut and covered boxes
     if (L.ntile_eq) {
-        hipLaunchKernelGGL(k_mprims, dim3(L.ntile_eq), dim3(64), 0, s, L.d_boxes, L.d_tile_eq, m);
-        hipLaunchKernelGGL(k_ebchem_face<0>, dim3(L.ntile_ef[0]), dim3(64), 0, s, L.d_boxes, L.d_tile_ef[0], m,
+        hipLaunchKernelGGL(k_mprims, dim3(L.ntile_eq), dim3(256), 0, s, L.d_boxes, L.d_tile_eq, m);
+        hipLaunchKernelGGL(k_ebchem_face<0>, dim3(L.ntile_ef[0]), dim3(256), 0, s, L.d_boxes, L.d_tile_ef[0], m,
                            (const uint32_t*)L.d_ebflag);
-        hipLaunchKernelGGL(k_ebchem_face<1>, dim3(L.ntile_ef[1]), dim3(64), 0, s, L.d_boxes, L.d_tile_ef[1], m,
+        hipLaunchKernelGGL(k_ebchem_face<1>, dim3(L.ntile_ef[1]), dim3(256), 0, s, L.d_boxes, L.d_tile_ef[1], m,
                            (const uint32_t*)L.d_ebflag);
-        hipLaunchKernelGGL(k_ebchem_face<2>, dim3(L.ntile_ef[2]), dim3(64), 0, s, L.d_boxes, L.d_tile_ef[2], m,
+        hipLaunchKernelGGL(k_ebchem_face<2>, dim3(L.ntile_ef[2]), dim3(256), 0, s, L.d_boxes, L.d_tile_ef[2], m,
                            (const uint32_t*)L.d_ebflag);
     }
     if (L.ntile_ed)
