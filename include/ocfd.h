@@ -479,6 +479,16 @@ int ocfd_cf_reset(ocfd_cf c, void* stream);
  * primitives (w_dt = the stage's dt weight: RK3 dt/6, dt/6, 2dt/3; RK2 dt/2;
  * Euler dt).  flags as for the stage. */
 int ocfd_cf_add(ocfd_cf c, int fine_side, double w_dt, const ocfd_parm* parm, int flags, void* stream);
+/* EBR::compute_dSdt with its flux-register arguments (Source/Advance.cpp:182,
+ * the CrseAdd/FineAdd block at 344-360; declared at Source/EBR.H:167-169):
+ * ocfd_compute_dsdt, then w_dt = dt times this level's interface fluxes added
+ * to `fine` (the register joining lev to the next finer level: lev is its
+ * coarse side, CrseAdd) and to `current` (the register joining lev to the next
+ * coarser level: lev is its fine side, FineAdd).  Either may be NULL.  dt is
+ * the stage weight the reference passes (Euler dt, RK2 dt/2, RK3 dt/6, dt/6,
+ * 2dt/3).  OCFD_EINVAL when a register does not border lev on that side. */
+int ocfd_compute_dsdt_fr(ocfd_level lev, const double* S, double* dSdt, double dt, ocfd_cf fine, ocfd_cf current,
+                         const ocfd_parm* parm, int flags, void* stream);
 /* copy the registers to host: regc[5*nface], regf[20*nface] (4 sub-faces per face) */
 int ocfd_cf_registers(ocfd_cf c, double* regc, double* regf);
 /* S_coarse += the flux mismatch of every interface face on the coarse side */

@@ -739,6 +739,23 @@ int ocfd_cf_add(ocfd_cf c, int fine_side, double w_dt, const ocfd_parm* parm, in
     return launched("ocfd_cf_add");
 }
 
+int ocfd_compute_dsdt_fr(ocfd_level lev, const double* S, double* dSdt, double dt, ocfd_cf fine, ocfd_cf current,
+                         const ocfd_parm* parm, int flags, void* stream)
+{
+    // compute_dSdt(S, dSdt, dt, fine, current), Advance.cpp:182 and 344-360:
+    // `fine` joins this level to the next finer one (this level is its coarse
+    // side: CrseAdd), `current` joins it to the next coarser one (fine side:
+    // FineAdd); dt is the stage's weight
+    if (!lev || (fine && fine->coarse != lev) || (current && current->fine != lev)) {
+        set_error("ocfd_compute_dsdt_fr: a register does not border this level on the named side");
+        return OCFD_EINVAL;
+    }
+    int rc = ocfd_compute_dsdt(lev, S, dSdt, dt, parm, flags, stream);
+    if (!rc && fine) rc = ocfd_cf_add(fine, 0, dt, parm, flags, stream);
+    if (!rc && current) rc = ocfd_cf_add(current, 1, dt, parm, flags, stream);
+    return rc;
+}
+
 int ocfd_cf_registers(ocfd_cf c, double* regc, double* regf)
 {
     if (!c || !regc || !regf) {

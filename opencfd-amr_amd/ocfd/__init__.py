@@ -190,6 +190,8 @@ def lib():
         L.ocfd_cf_reset.argtypes = [vp, vp]
         L.ocfd_cf_add.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Parm), ctypes.c_int, vp]
         L.ocfd_cf_registers.argtypes = [vp, dp, dp]
+        L.ocfd_compute_dsdt_fr.argtypes = [vp, vp, vp, ctypes.c_double, vp, vp, ctypes.POINTER(Parm), ctypes.c_int,
+                                           vp]
         L.ocfd_cf_reflux.argtypes = [vp, vp, vp]
         L.ocfd_cf_avg_down.argtypes = [vp, vp, vp, ctypes.c_int, vp]
         L.ocfd_plot_vars.argtypes = [vp, vp, vp, ctypes.POINTER(Parm), vp]
@@ -356,6 +358,14 @@ class Level:
     def compute_dsdt(self, S, dSdt, parm, flags=0, dt=0.0, stream=None):
         check(lib().ocfd_compute_dsdt(self.handle, _ptr(S), _ptr(dSdt), dt, ctypes.byref(parm), flags,
                                       _stream(stream)), "ocfd_compute_dsdt")
+
+    def compute_dsdt_fr(self, S, dSdt, parm, flags=0, dt=0.0, fine=None, current=None, stream=None):
+        """compute_dsdt, then dt x the interface fluxes added to the register
+        `fine` (this level its coarse side) and `current` (its fine side);
+        registers are amr.FluxRegister objects or None."""
+        h = lambda r: None if r is None else r.handle
+        check(lib().ocfd_compute_dsdt_fr(self.handle, _ptr(S), _ptr(dSdt), dt, h(fine), h(current),
+                                         ctypes.byref(parm), flags, _stream(stream)), "ocfd_compute_dsdt_fr")
 
     def rk_stage(self, S_cur, S_old, S_new, c_cur, c_old, c_dt, parm, flags=0, stream=None):
         check(lib().ocfd_rk_stage(self.handle, _ptr(S_cur), _ptr(S_old), _ptr(S_new), c_cur, c_old, c_dt,

@@ -189,6 +189,54 @@ def test_flux_registers_match_oracle(torch_cuda, orc, visc, f32):
     assert ec < tol and ef < tol
 
 
+@pytest.mark.parametrize("flags", [0, 1, 5])   # inviscid, viscous, viscous FP32-flux
+def test_compute_dsdt_feeds_flux_registers(torch_cuda, orc, flags):
+    """compute_dSdt with its register arguments through the C ABI: the dSdt
+    equals the plain compute, and the registers equal an explicit add after
+    each compute (bit for bit), both sides of one interface, two stages each;
+    a register on the wrong side of the level is refused."""
+    import torch
+    import ocfd
+    from ocfd import amr, cases
+    boxes, g, kind = cases.kh3d(12, max_grid=(6, 6, 12))
+    og = gu.to_ogeom(g)
+    A = orc.new_level_fabs(boxes)
+    orc.init_case(kind, boxes, A, og, orc.parm(), 1)
+    orc.fill_ghost(boxes, A, og, 0.0)
+    fine_c = [((0, 2, 0), (3, 7, 4)), ((8, 0, 6), (11, 5, 11))]
+    gf, ogf, fb, ff = _fine_setup(orc, fine_c, boxes, g, og, A)
+    orc.fill_ghost(fb, ff, ogf, 0.0)
+    ao.fill_from_coarse(fb, ff, ogf, ao.interp_dense(ao.dense_grown(boxes, A, og), og))
+    lc, lf = ocfd.Level(boxes, g), ocfd.Level(fb, gf)
+    Sc, Sf = lc.new_mf(), lf.new_mf()
+    gu.upload(lc, Sc, A)
+    gu.upload(lf, Sf, ff)
+    assert ocfd.VISC == 1 and ocfd.FLUX_FP32 == 4
+    p = ocfd.default_parm()
+    cf1, cf2 = amr.FluxRegister(lc, lf), amr.FluxRegister(lc, lf)
+    assert cf1.nfaces > 0
+    cf1.reset()
+    cf2.reset()
+    Dc1, Dc2, Df1, Df2 = lc.new_mf(ngrow=0), lc.new_mf(ngrow=0), lf.new_mf(ngrow=0), lf.new_mf(ngrow=0)
+    for w in (0.25, 0.5):
+        lc.compute_dsdt_fr(Sc, Dc1, p, flags, dt=w, fine=cf1)
+        lf.compute_dsdt_fr(Sf, Df1, p, flags, dt=0.5 * w, current=cf1)
+        lc.compute_dsdt(Sc, Dc2, p, flags)
+        cf2.add(0, w, p, flags)
+        lf.compute_dsdt(Sf, Df2, p, flags)
+        cf2.add(1, 0.5 * w, p, flags)
+    torch.cuda.synchronize()
+    assert torch.equal(Dc1, Dc2) and torch.equal(Df1, Df2)
+    (rc1, rf1), (rc2, rf2) = cf1.registers(), cf2.registers()
+    assert np.array_equal(rc1, rc2) and np.array_equal(rf1, rf2)
+    assert np.abs(rc1).max() > 0 and np.abs(rf1).max() > 0
+    # wrong side: the fine level is not the coarse side of its own register
+    with pytest.raises(RuntimeError, match="status 1"):
+        lf.compute_dsdt_fr(Sf, Df1, p, flags, dt=1.0, fine=cf1)
+    with pytest.raises(RuntimeError, match="status 1"):
+        lc.compute_dsdt_fr(Sc, Dc1, p, flags, dt=1.0, current=cf1)
+
+
 def _compare_runs(T, O, tol):
     key = lambda b: (tuple(b[0]), tuple(b[1]))
     assert sorted(map(key, T.fine.boxes)) == sorted(map(key, O.fboxes))
