@@ -149,6 +149,16 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
 /* Box classification of the level: types[b] = 0 regular, 1 cut, 2 covered. */
 int ocfd_level_box_types(ocfd_level lev, int* types);
 
+/* The library's own device tables and scratch (levels, interface registers,
+ * multispecies scratch) come from a per-device caching pool: freed blocks are
+ * kept and handed out again, since an AMR regrid or a serving loop rebuilds
+ * levels all the time (the role of AMReX's caching arena under the reference's
+ * EBR levels, Source/EBR.cpp:43-65).  ocfd_cached_device_bytes: bytes held
+ * free in the pool; ocfd_release_cached_memory: give them back to HIP (waits
+ * for nothing the library still uses). */
+long ocfd_cached_device_bytes(void);
+int ocfd_release_cached_memory(void);
+
 /* Device memory held by the level itself (work-lists, primitive and partial
  * divergence scratch), in bytes; the caller's MultiFabs are not included.
  * For admission control by HBM footprint. */
@@ -437,6 +447,16 @@ int ocfd_graph_destroy(ocfd_graph g);
  * valid cell in the level's 0-ghost layout (box order, i fastest). */
 int ocfd_tag_dengrad(ocfd_level lev, const double* S, double threshold, unsigned char* tags, void* stream);
 
+/* ocfd_tag_dengrad with the proper nesting region of the level folded in:
+ * flags (one byte per valid cell, the same layout) hold the tag in bit 0 and,
+ * in bit 1, whether every cell within Chebyshev distance n_proper of the cell
+ * (periodic images included; beyond a non-periodic domain side nothing is
+ * required) lies in the level's boxes: the region where the next finer level
+ * may place grids (amr.n_proper; Amr::grid_places intersects the tags and the
+ * new grids with it).  n_proper 0: bit 1 is set everywhere. */
+int ocfd_tag_dengrad_nest(ocfd_level lev, const double* S, double threshold, int n_proper, unsigned char* flags,
+                          void* stream);
+
 /* New fine boxes from coarse tags (the AmrLevel regrid's grid_places that
  * errorEst feeds; amr.blocking_factor / amr.grid_eff / amr.max_grid_size of
  * Exec/<case>/inputs): tags dense over [lo, hi] (i fastest, host memory),
@@ -447,6 +467,20 @@ int ocfd_tag_dengrad(ocfd_level lev, const double* S, double threshold, unsigned
  * *nbox_out = count.  Host only. */
 int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigned char* tags, const unsigned char* mask, int block,
                       double grid_eff, int max_size_cells, int cap, int* boxes_out, int* nbox_out);
+
+/* The same clustering from tags held in a level's own layout instead of dense
+ * domain arrays (a level covers a few percent of its domain): flags, one byte
+ * per valid cell of the nbox boxes (lo/hi triples) in the 0-ghost layout
+ * (box order, i fastest), as ocfd_tag_dengrad_nest writes them; the cells of
+ * the nextra boxes elo/ehi count as tagged too (the projection of the next
+ * finer level's new grids in Amr::grid_places).  A cell is tagged when bit 0
+ * (or an extra box) and bit 1 hold; with use_mask the clusters are cut back to
+ * the cells with bit 1 (cells outside the level's boxes are outside that
+ * region).  Returns the boxes ocfd_cluster_tags returns for the equivalent
+ * dense tags ((tag OR extra) AND region) and mask (region).  Host only. */
+int ocfd_cluster_box_tags(int nbox, const int* lo, const int* hi, const unsigned char* flags, int nextra,
+                          const int* elo, const int* ehi, int use_mask, int block, double grid_eff,
+                          int max_size_cells, int cap, int* boxes_out, int* nbox_out);
 
 /* A dense 0/1 mask over a level's domain (nz*ny*nx bytes, i fastest, host
  * memory) grown in place by n cells in every direction (Chebyshev: the tag

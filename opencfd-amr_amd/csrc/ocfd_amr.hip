@@ -411,6 +411,49 @@ __global__ __launch_bounds__(256) void k_tag(const BoxDesc* __restrict__ boxes, 
     }
 }
 
+// k_tag's flag in bit 0 and, in bit 1, whether the cell lies in the level's
+// proper nesting region: every cell within Chebyshev distance n_proper
+// (periodic images included, cells beyond a non-periodic domain side not
+// counted) is covered by the level's boxes
+__global__ __launch_bounds__(256) void k_tag_nest(const BoxDesc* __restrict__ boxes, const GeomDesc* __restrict__ geom,
+                                                  const int4* __restrict__ tiles, const double* __restrict__ S,
+                                                  double thr, int n_proper, uint8_t* __restrict__ flags)
+{
+    const int4 t = tiles[blockIdx.x];
+    const BoxDesc B = boxes[t.x];
+    const GeomDesc& G = *geom;
+    for (int c = threadIdx.x; c < t.z; c += 256) {
+        int i, j, k;
+        valid_cell(B, t.y + c, i, j, k);
+        uint8_t tg = 0;
+        if (B.type != 2) {
+            const long s = sidx(B, i, j, k);
+            const long sy = B.nxg, sz = (long)B.nxg * B.nyg;
+            const double r = S[s];
+            double ax = fabs(S[s + 1] - r), ay = fabs(S[s + sy] - r), az = fabs(S[s + sz] - r);
+            ax = fmax(ax, fabs(r - S[s - 1]));
+            ay = fmax(ay, fabs(r - S[s - sy]));
+            az = fmax(az, fabs(r - S[s - sz]));
+            tg = fmax(ax, fmax(ay, az)) >= thr ? 1 : 0;
+        }
+        bool in = true;
+        for (int c2 = -n_proper; c2 <= n_proper && in; ++c2)
+            for (int c1 = -n_proper; c1 <= n_proper && in; ++c1)
+                for (int c0 = -n_proper; c0 <= n_proper && in; ++c0) {
+                    int q[3] = {i + c0, j + c1, k + c2};
+                    bool inside = true;
+                    for (int d = 0; d < 3; ++d) {
+                        const int len = G.dom_hi[d] - G.dom_lo[d] + 1;
+                        if (q[d] >= G.dom_lo[d] && q[d] <= G.dom_hi[d]) continue;
+                        if (G.periodic[d] && len > 1) q[d] = G.dom_lo[d] + (((q[d] - G.dom_lo[d]) % len) + len) % len;
+                        else inside = false;
+                    }
+                    if (inside) in = level_owner(G, boxes, q[0], q[1], q[2]) >= 0;
+                }
+        flags[B.off_d / NC + (didx(B, i, j, k) - B.off_d)] = (uint8_t)(tg | (in ? 2 : 0));
+    }
+}
+
 // out = a*x + b*y, the two-time combination of FillPatchSingleLevel
 __global__ void k_lincomb(long n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
                           double* __restrict__ out)
@@ -495,7 +538,7 @@ int launched(const char* what)
 void free_cf(ocfd_cf c)
 {
     // the interface lists, the interpolation map and the registers live in c->d_arena
-    hipFree(c->d_arena);
+    dev_free({c->d_arena});
 }
 
 bool refined_by_2(const LevelDev& C, const LevelDev& F)
@@ -863,6 +906,20 @@ int ocfd_tag_dengrad(ocfd_level lev, const double* S, double threshold, unsigned
         hipLaunchKernelGGL(k_tag, dim3(L.ntile_c), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_tile_c, S,
                            threshold, (uint8_t*)tags);
     return launched("ocfd_tag_dengrad");
+}
+
+int ocfd_tag_dengrad_nest(ocfd_level lev, const double* S, double threshold, int n_proper, unsigned char* flags,
+                          void* stream)
+{
+    if (!lev || !S || !flags || n_proper < 0) {
+        set_error("ocfd_tag_dengrad_nest: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    const LevelDev& L = lev->dev;
+    if (L.ntile_c)
+        hipLaunchKernelGGL(k_tag_nest, dim3(L.ntile_c), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_geom,
+                           L.d_tile_c, S, threshold, n_proper, (uint8_t*)flags);
+    return launched("ocfd_tag_dengrad_nest");
 }
 
 int ocfd_lincomb(long n, double a, const double* x, double b, const double* y, double* out, void* stream)

@@ -250,6 +250,69 @@ void max_size(std::vector<IBox>& boxes, int chunk)
     }
 }
 
+
+// floor division (box corners may be negative)
+int fdiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+// The clustering shared by both entry points: points (blocking-factor
+// coordinates) grouped by Berger-Rigoutsos to grid_eff; when has_mask, each
+// cluster intersected with the allowed region (ClusterList::intersect with the
+// proper nesting domain: a box partly outside is replaced by an exact box
+// cover of its part inside; in_box(box, out) lists the allowed points of a
+// box); refined back by the blocking factor, joined, chopped to max size.
+template <class InBox>
+void cluster_points(std::vector<Pt>& all, bool has_mask, InBox&& in_box, int block, double grid_eff,
+                    int max_size_cells, std::vector<IBox>& boxes)
+{
+    boxes.clear();
+    if (all.empty()) return;
+    std::vector<Cluster> lst(1);
+    lst[0].b = 0;
+    lst[0].e = all.size();
+    lst[0].min_box(all);
+    refine(lst, all, grid_eff);
+    std::vector<IBox> kept;
+    std::vector<Pt> in;
+    for (const Cluster& c : lst) {
+        if (!has_mask) { kept.push_back(c.box); continue; }
+        in.clear();
+        in_box(c.box, in);
+        if ((long)in.size() == c.box.volume()) { kept.push_back(c.box); continue; }
+        if (in.empty()) continue;
+        std::vector<Cluster> part(1);
+        part[0].b = 0;
+        part[0].e = in.size();
+        part[0].min_box(in);
+        refine(part, in, 1.0);
+        for (const Cluster& q : part) kept.push_back(q.box);
+    }
+    for (const IBox& kb : kept) {
+        IBox b = kb;
+        for (int d = 0; d < 3; ++d) {
+            b.lo[d] *= block;
+            b.hi[d] = (b.hi[d] + 1) * block - 1;
+        }
+        boxes.push_back(b);
+    }
+    merge(boxes);
+    max_size(boxes, max_size_cells);
+}
+
+int write_boxes(const std::vector<IBox>& boxes, int cap, int* boxes_out, int* nbox_out, const char* who)
+{
+    *nbox_out = (int)boxes.size();
+    if ((int)boxes.size() > cap) {
+        ocfd::set_error(std::string(who) + ": more boxes than the output capacity");
+        return OCFD_EINVAL;
+    }
+    for (size_t b = 0; b < boxes.size(); ++b)
+        for (int d = 0; d < 3; ++d) {
+            boxes_out[6 * b + d] = boxes[b].lo[d];
+            boxes_out[6 * b + 3 + d] = boxes[b].hi[d];
+        }
+    return OCFD_OK;
+}
+
 } // namespace
 
 extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigned char* tags, const unsigned char* mask,
@@ -311,58 +374,134 @@ extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigne
             }
         }
     }
+    auto in_box = [&](const IBox& b, std::vector<Pt>& out) {
+        for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+            for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+                for (int i = b.lo[0]; i <= b.hi[0]; ++i) {
+                    const int ii = i - lo[0] / block, jj = j - lo[1] / block, kk = k - lo[2] / block;
+                    if (bmp[((size_t)kk * nb[1] + jj) * nb[0] + ii]) out.push_back(Pt{{i, j, k}});
+                }
+    };
     std::vector<IBox> boxes;
-    if (!all.empty()) {
-        std::vector<Cluster> lst(1);
-        lst[0].b = 0;
-        lst[0].e = all.size();
-        lst[0].min_box(all);
-        refine(lst, all, grid_eff);
-        // clusters intersected with the mask region (ClusterList::intersect with
-        // the proper nesting domain): a box partly outside is replaced by an
-        // exact box cover of its part inside
-        std::vector<IBox> kept;
-        std::vector<Pt> in;
-        for (const Cluster& c : lst) {
-            if (!mask) { kept.push_back(c.box); continue; }
-            in.clear();
-            for (int k = c.box.lo[2]; k <= c.box.hi[2]; ++k)
-                for (int j = c.box.lo[1]; j <= c.box.hi[1]; ++j)
-                    for (int i = c.box.lo[0]; i <= c.box.hi[0]; ++i) {
-                        const int ii = i - lo[0] / block, jj = j - lo[1] / block, kk = k - lo[2] / block;
-                        if (bmp[((size_t)kk * nb[1] + jj) * nb[0] + ii]) in.push_back(Pt{{i, j, k}});
-                    }
-            if ((long)in.size() == c.box.volume()) { kept.push_back(c.box); continue; }
-            if (in.empty()) continue;
-            std::vector<Cluster> part(1);
-            part[0].b = 0;
-            part[0].e = in.size();
-            part[0].min_box(in);
-            refine(part, in, 1.0);
-            for (const Cluster& q : part) kept.push_back(q.box);
-        }
-        for (const IBox& kb : kept) {
-            IBox b = kb;
-            for (int d = 0; d < 3; ++d) {
-                b.lo[d] *= block;
-                b.hi[d] = (b.hi[d] + 1) * block - 1;
-            }
-            boxes.push_back(b);
-        }
-        merge(boxes);
-        max_size(boxes, max_size_cells);
-    }
-    *nbox_out = (int)boxes.size();
-    if ((int)boxes.size() > cap) {
-        set_error("ocfd_cluster_tags: more boxes than the output capacity");
+    cluster_points(all, mask != nullptr, in_box, block, grid_eff, max_size_cells, boxes);
+    return write_boxes(boxes, cap, boxes_out, nbox_out, "ocfd_cluster_tags");
+}
+
+extern "C" int ocfd_cluster_box_tags(int nbox, const int* lo, const int* hi, const unsigned char* flags_in,
+                                     int nextra, const int* elo, const int* ehi, int use_mask, int block,
+                                     double grid_eff, int max_size_cells, int cap, int* boxes_out, int* nbox_out)
+{
+    using ocfd::set_error;
+    if (nbox < 0 || (nbox > 0 && (!lo || !hi || !flags_in)) || nextra < 0 || (nextra > 0 && (!elo || !ehi)) ||
+        !nbox_out || block < 1 || max_size_cells < block || max_size_cells % block || grid_eff <= 0.0 ||
+        grid_eff > 1.0 || cap < 0 || (cap > 0 && !boxes_out)) {
+        set_error("ocfd_cluster_box_tags: invalid arguments");
         return OCFD_EINVAL;
     }
-    for (size_t b = 0; b < boxes.size(); ++b)
+    std::vector<IBox> lv(nbox);
+    std::vector<size_t> off(nbox + 1, 0);
+    for (int b = 0; b < nbox; ++b) {
         for (int d = 0; d < 3; ++d) {
-            boxes_out[6 * b + d] = boxes[b].lo[d];
-            boxes_out[6 * b + 3 + d] = boxes[b].hi[d];
+            lv[b].lo[d] = lo[3 * b + d];
+            lv[b].hi[d] = hi[3 * b + d];
+            if (lv[b].hi[d] < lv[b].lo[d]) {
+                set_error("ocfd_cluster_box_tags: empty box");
+                return OCFD_EINVAL;
+            }
         }
-    return OCFD_OK;
+        off[b + 1] = off[b] + (size_t)lv[b].volume();
+    }
+    // the extra boxes' cells count as tagged (tags |= their coverage)
+    std::vector<unsigned char> fcopy;
+    const unsigned char* flags = flags_in;
+    if (nextra > 0) {
+        fcopy.assign(flags_in, flags_in + off[nbox]);
+        for (int e = 0; e < nextra; ++e)
+            for (int b = 0; b < nbox; ++b) {
+                const IBox& B = lv[b];
+                int a[3], z[3];
+                bool hit = true;
+                for (int d = 0; d < 3; ++d) {
+                    a[d] = std::max(B.lo[d], elo[3 * e + d]);
+                    z[d] = std::min(B.hi[d], ehi[3 * e + d]);
+                    hit = hit && a[d] <= z[d];
+                }
+                if (!hit) continue;
+                const int n0 = B.hi[0] - B.lo[0] + 1, n1 = B.hi[1] - B.lo[1] + 1;
+                for (int k = a[2]; k <= z[2]; ++k)
+                    for (int j = a[1]; j <= z[1]; ++j) {
+                        unsigned char* f = fcopy.data() + off[b] + ((size_t)(k - B.lo[2]) * n1 + (j - B.lo[1])) * n0;
+                        for (int i = a[0]; i <= z[0]; ++i) f[i - B.lo[0]] |= 1;
+                    }
+            }
+        flags = fcopy.data();
+    }
+    // points: blocks holding a cell flagged both tagged and allowed
+    std::vector<Pt> all;
+    for (int b = 0; b < nbox; ++b) {
+        const IBox& B = lv[b];
+        const unsigned char* f = flags + off[b];
+        const int n0 = B.hi[0] - B.lo[0] + 1;
+        size_t c = 0;
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+            for (int j = B.lo[1]; j <= B.hi[1]; ++j, c += n0) {
+                for (int i = 0; i < n0; ++i)
+                    if ((f[c + i] & 3) == 3) all.push_back(Pt{{fdiv(B.lo[0] + i, block), fdiv(j, block), fdiv(k, block)}});
+            }
+    }
+    if (block > 1) {
+        std::sort(all.begin(), all.end(), [](const Pt& a, const Pt& b) {
+            return std::lexicographical_compare(a.v, a.v + 3, b.v, b.v + 3);
+        });
+        all.erase(std::unique(all.begin(), all.end(),
+                              [](const Pt& a, const Pt& b) { return a.v[0] == b.v[0] && a.v[1] == b.v[1] && a.v[2] == b.v[2]; }),
+                  all.end());
+    }
+    // allowed points of a cluster box: blocks whose cells all lie in the
+    // level and carry the allowed flag
+    std::vector<unsigned char> loc;
+    auto in_box = [&](const IBox& cb, std::vector<Pt>& out) {
+        IBox cr;
+        for (int d = 0; d < 3; ++d) {
+            cr.lo[d] = cb.lo[d] * block;
+            cr.hi[d] = (cb.hi[d] + 1) * block - 1;
+        }
+        const int m0 = cr.hi[0] - cr.lo[0] + 1, m1 = cr.hi[1] - cr.lo[1] + 1;
+        loc.assign((size_t)cr.volume(), 0);
+        for (int b = 0; b < nbox; ++b) {
+            const IBox& B = lv[b];
+            int a[3], z[3];
+            bool hit = true;
+            for (int d = 0; d < 3; ++d) {
+                a[d] = std::max(B.lo[d], cr.lo[d]);
+                z[d] = std::min(B.hi[d], cr.hi[d]);
+                hit = hit && a[d] <= z[d];
+            }
+            if (!hit) continue;
+            const int n0 = B.hi[0] - B.lo[0] + 1, n1 = B.hi[1] - B.lo[1] + 1;
+            const unsigned char* f = flags + off[b];
+            for (int k = a[2]; k <= z[2]; ++k)
+                for (int j = a[1]; j <= z[1]; ++j) {
+                    const unsigned char* src = f + ((size_t)(k - B.lo[2]) * n1 + (j - B.lo[1])) * n0 + (a[0] - B.lo[0]);
+                    unsigned char* dst = loc.data() + ((size_t)(k - cr.lo[2]) * m1 + (j - cr.lo[1])) * m0 + (a[0] - cr.lo[0]);
+                    for (int i = 0; i <= z[0] - a[0]; ++i) dst[i] = (src[i] >> 1) & 1;
+                }
+        }
+        for (int k = cb.lo[2]; k <= cb.hi[2]; ++k)
+            for (int j = cb.lo[1]; j <= cb.hi[1]; ++j)
+                for (int i = cb.lo[0]; i <= cb.hi[0]; ++i) {
+                    bool ok = true;
+                    for (int kk = 0; kk < block && ok; ++kk)
+                        for (int jj = 0; jj < block && ok; ++jj)
+                            for (int ii = 0; ii < block && ok; ++ii)
+                                ok = loc[((size_t)(k * block + kk - cr.lo[2]) * m1 + (j * block + jj - cr.lo[1])) * m0 +
+                                         (i * block + ii - cr.lo[0])] != 0;
+                    if (ok) out.push_back(Pt{{i, j, k}});
+                }
+    };
+    std::vector<IBox> boxes;
+    cluster_points(all, use_mask != 0, in_box, block, grid_eff, max_size_cells, boxes);
+    return write_boxes(boxes, cap, boxes_out, nbox_out, "ocfd_cluster_box_tags");
 }
 
 // Dense mask grown by n cells in every direction (Chebyshev distance, the

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 #include "ocfd_device.hpp"
@@ -105,6 +106,15 @@ inline const int4* fill_tiles(const LevelDev& L, int& n)
 // still run).
 hipStream_t upload_stream();
 
+// Device blocks for the library's tables and scratch from a per-device
+// caching pool (ocfd_pool.hip): dev_alloc false when out of device memory;
+// dev_free waits for the device (as hipFree would) and keeps the blocks for
+// reuse.  upload_bytes copies host bytes to the device through a pinned
+// staging buffer on upload_stream and waits for that copy only.
+bool dev_alloc(void** out, size_t bytes);
+void dev_free(std::initializer_list<void*> ps);
+int upload_bytes(void* dst, const void* src, size_t n);
+
 // Host tables gathered into one device allocation filled by one copy; each
 // table starts on a 256-byte boundary.  allocate() sets every registered
 // pointer into the allocation (null for empty tables); reserved space can be
@@ -142,7 +152,7 @@ struct Staging {
     {
         *base = nullptr;
         if (host.empty()) return OCFD_OK;
-        if (hipMalloc(base, host.size()) != hipSuccess) {
+        if (!dev_alloc(base, host.size())) {
             *base = nullptr;
             set_error("out of device memory for the level tables");
             return OCFD_ENOMEM;
@@ -153,12 +163,7 @@ struct Staging {
     int copy(void* base)
     {
         if (host.empty()) return OCFD_OK;
-        if (hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, upload_stream()) != hipSuccess ||
-            hipStreamSynchronize(upload_stream()) != hipSuccess) {
-            set_error("copy of the level tables to the device failed");
-            return OCFD_EHIP;
-        }
-        return OCFD_OK;
+        return upload_bytes(base, host.data(), host.size());
     }
     int commit(void** base)
     {

@@ -285,10 +285,11 @@ template <class T>
 static int upload(T** dptr, const std::vector<T>& h)
 {
     if (h.empty()) { *dptr = nullptr; return OCFD_OK; }
-    OCFD_HIP(hipMalloc((void**)dptr, sizeof(T) * h.size()));
-    OCFD_HIP(hipMemcpyAsync(*dptr, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, upload_stream()));
-    OCFD_HIP(hipStreamSynchronize(upload_stream()));
-    return OCFD_OK;
+    if (!dev_alloc((void**)dptr, sizeof(T) * h.size())) {
+        set_error("out of device memory for the level tables");
+        return OCFD_ENOMEM;
+    }
+    return upload_bytes(*dptr, h.data(), sizeof(T) * h.size());
 }
 
 // The level's read-only tables (owner map, work-lists, box descriptors, EB
@@ -298,11 +299,9 @@ static int upload(T** dptr, const std::vector<T>& h)
 static void free_level(LevelDev& L)
 {
     // tables inside the arena are not freed on their own
-    hipFree(L.d_arena); hipFree(L.d_bufs);
-    if (L.own_tile_yz) { hipFree(L.d_tile_y); hipFree(L.d_tile_z); }
-    hipFree(L.d_ebq);
-    for (int d = 0; d < 3; ++d) { hipFree(L.d_ebF[d]); hipFree(L.d_mF[d]); }
-    hipFree(L.d_mq);
+    dev_free({L.d_arena, L.d_bufs, L.own_tile_yz ? (void*)L.d_tile_y : nullptr,
+              L.own_tile_yz ? (void*)L.d_tile_z : nullptr, L.d_ebq, L.d_ebF[0], L.d_ebF[1], L.d_ebF[2], L.d_mF[0],
+              L.d_mF[1], L.d_mF[2], L.d_mq});
     L = LevelDev{};
 }
 
@@ -597,8 +596,8 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     for (int d = 0; d < 3; ++d) L.ntile_mf[d] = (int)tmf[d].size();
     for (int d = 0; d < 3; ++d) L.ntile_ef[d] = (int)tef[d].size();
     if (L.mf_eb) {
-        bool ok = hipMalloc((void**)&L.d_ebq, sizeof(double) * 7 * L.mf_eb) == hipSuccess;
-        for (int d = 0; d < 3 && ok; ++d) ok = hipMalloc((void**)&L.d_ebF[d], sizeof(double) * NC * L.mf_eb) == hipSuccess;
+        bool ok = dev_alloc((void**)&L.d_ebq, sizeof(double) * 7 * L.mf_eb);
+        for (int d = 0; d < 3 && ok; ++d) ok = dev_alloc((void**)&L.d_ebF[d], sizeof(double) * NC * L.mf_eb);
         if (!ok) { set_error("ocfd_level_create_eb: out of device memory"); free_level(L); delete lev; return OCFD_ENOMEM; }
     }
     L.ntile_x = (int)tx.size(); L.ntile_xp = (int)txp.size(); L.ntile_y = ny3; L.ntile_z = (int)tz.size();
@@ -615,7 +614,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t o_p32 = al(sizeof(double) * L.mf_valid), o_p64 = o_p32 + al(sizeof(float) * L.mf_prim),
                  o_red = o_p64 + al(sizeof(double) * L.mf_prim), nbufs = o_red + 256;
-    if (hipMalloc(&L.d_bufs, nbufs) != hipSuccess) {
+    if (!dev_alloc(&L.d_bufs, nbufs)) {
         L.d_bufs = nullptr;
         set_error("ocfd_level_create: out of device memory");
         free_level(L); delete lev; return OCFD_ENOMEM;
@@ -855,15 +854,12 @@ int ocfd_level_set_march_chunk(ocfd_level lev, int ch)
     int4 *dy = nullptr, *dz = nullptr;
     int rc;
     if ((rc = upload(&dy, ty)) || (rc = upload(&dz, tz))) {
-        hipFree(dy);
-        hipFree(dz);
+        dev_free({dy, dz});
         return rc;
     }
     OCFD_HIP(hipDeviceSynchronize());   // no launch may still read the old lists
-    if (L.own_tile_yz) {   // else the old lists live in the level's table allocation
-        hipFree(L.d_tile_y);
-        hipFree(L.d_tile_z);
-    }
+    if (L.own_tile_yz)   // else the old lists live in the level's table allocation
+        dev_free({L.d_tile_y, L.d_tile_z});
     L.d_tile_y = dy; L.ntile_y = ny3; L.ntile_yp = (int)ty.size() - ny3;
     L.d_tile_z = dz; L.ntile_z = (int)tz.size();
     L.own_tile_yz = true;

@@ -842,15 +842,16 @@ static int ensure_multi(LevelDev& L)
 {
     if (L.d_mq) return OCFD_OK;
     const long ng = L.mf_state / NC;
-    if (hipMalloc((void**)&L.d_mq, sizeof(double) * MQ_N * ng) != hipSuccess) {
+    double *mq = nullptr, *mF[3] = {nullptr, nullptr, nullptr};
+    bool ok = dev_alloc((void**)&mq, sizeof(double) * MQ_N * ng);
+    for (int d = 0; d < 3 && ok; ++d) ok = dev_alloc((void**)&mF[d], sizeof(double) * NF * ng);
+    if (!ok) {   // all or nothing: a later call tries again from scratch
+        dev_free({mq, mF[0], mF[1], mF[2]});
         set_error("multispecies flow: out of device memory");
         return OCFD_ENOMEM;
     }
-    for (int d = 0; d < 3; ++d)
-        if (hipMalloc((void**)&L.d_mF[d], sizeof(double) * NF * ng) != hipSuccess) {
-            set_error("multispecies flow: out of device memory");
-            return OCFD_ENOMEM;
-        }
+    L.d_mq = mq;
+    for (int d = 0; d < 3; ++d) L.d_mF[d] = mF[d];
     L.dev_bytes += (long)sizeof(double) * (MQ_N + 3 * NF) * ng;
     return OCFD_OK;
 }

@@ -122,6 +122,9 @@ def lib():
         L.ocfd_inflow_state.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
         L.ocfd_level_device_bytes.argtypes = [vp]
         L.ocfd_level_device_bytes.restype = ctypes.c_long
+        L.ocfd_cached_device_bytes.argtypes = []
+        L.ocfd_cached_device_bytes.restype = ctypes.c_long
+        L.ocfd_release_cached_memory.argtypes = []
         L.ocfd_compute_dsdt.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.POINTER(Parm), ctypes.c_int, vp]
         L.ocfd_rk_stage.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     ctypes.POINTER(Parm), ctypes.c_int, vp]
@@ -174,6 +177,10 @@ def lib():
         L.ocfd_cluster_tags.argtypes = [ip, ip, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte),
                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ip, ip]
         L.ocfd_level_set_planar.argtypes = [vp, ctypes.c_int]
+        L.ocfd_tag_dengrad_nest.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, vp, vp]
+        L.ocfd_cluster_box_tags.argtypes = [ctypes.c_int, ip, ip, ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int, ip,
+                                            ip, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_int, ip, ip]
         L.ocfd_reactor_adaptive_begin.argtypes = [ctypes.c_long, vp, ctypes.c_double, vp, vp, vp]
         L.ocfd_reactor_adaptive_chunk.argtypes = [ctypes.c_long, vp, vp, ctypes.c_double, ctypes.c_double,
                                                   ctypes.c_double, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
@@ -529,6 +536,16 @@ def level_error(level, got, ref, ncomp=NCONS):
         deni[1:4] = deni[1:4].max()
     den1 = np.maximum(den1, 1e-300); deni = np.maximum(deni, 1e-300)
     return [(float(x), float(y)) for x, y in zip(num1 / den1, numi / deni)]
+
+
+def cached_device_bytes():
+    """Device bytes the library's pool holds free for reuse."""
+    return lib().ocfd_cached_device_bytes()
+
+
+def release_cached_memory():
+    """Return the pool's free device blocks to HIP."""
+    check(lib().ocfd_release_cached_memory(), "ocfd_release_cached_memory")
 
 
 def new_ctl(device="cuda"):

@@ -60,6 +60,44 @@ def tag_dengrad(level, S, threshold, stream=None):
     return tags
 
 
+def tag_dengrad_nest(level, S, threshold, n_proper, stream=None):
+    """Device flags, one byte per valid cell (0-ghost layout): bit 0 the
+    density-gradient tag, bit 1 inside the level's proper nesting region."""
+    import torch
+    flags = torch.empty(level.mf_size(0, 1), dtype=torch.uint8, device=S.device)
+    check(lib().ocfd_tag_dengrad_nest(level.handle, _ptr(S), threshold, int(n_proper), _ptr(flags), _stream(stream)),
+          "ocfd_tag_dengrad_nest")
+    return flags
+
+
+def _box_arrays(boxes):
+    n = len(boxes)
+    lo = (ctypes.c_int * max(1, 3 * n))(*[v for b in boxes for v in b[0]])
+    hi = (ctypes.c_int * max(1, 3 * n))(*[v for b in boxes for v in b[1]])
+    return n, lo, hi
+
+
+def cluster_box_tags(boxes, flags, extra=(), use_mask=True, block=1, grid_eff=0.7, max_size=32):
+    """cluster_tags from flags in the level layout of `boxes` (host uint8,
+    bit 0 tag, bit 1 allowed region) plus the cells of the `extra` boxes
+    as tags; the clusters are cut back to the allowed region when use_mask."""
+    flags = np.ascontiguousarray(flags, dtype=np.uint8)
+    n, lo, hi = _box_arrays(boxes)
+    ne, elo, ehi = _box_arrays(list(extra))
+    nb = ctypes.c_int()
+    cap = 4096
+    while True:
+        out = (ctypes.c_int * (6 * cap))()
+        rc = lib().ocfd_cluster_box_tags(n, lo, hi, flags.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), ne, elo, ehi,
+                                         int(bool(use_mask)), block, grid_eff, max_size, cap, out, ctypes.byref(nb))
+        if rc != 0 and nb.value > cap:
+            cap = nb.value
+            continue
+        check(rc, "ocfd_cluster_box_tags")
+        break
+    return [(tuple(out[6 * b:6 * b + 3]), tuple(out[6 * b + 3:6 * b + 6])) for b in range(nb.value)]
+
+
 def dense_tags(level, tags):
     """Tags of a level (host array in the 0-ghost layout) as a dense (nz, ny, nx)
     array over the level's domain."""
@@ -151,6 +189,11 @@ def projected_coverage(boxes_f2, geom_f, geom_c, n_proper):
     around the finer level's (Amr::grid_places, projection of new grids).
     Growing and coarsening distribute over a union of boxes, so this is box
     arithmetic and one coverage fill."""
+    return coverage(projected_boxes(boxes_f2, geom_f, n_proper), geom_c)
+
+
+def projected_boxes(boxes_f2, geom_f, n_proper):
+    """The boxes of projected_coverage (level c index space, may overlap)."""
     lo_f = [geom_f.dom_lo[d] for d in range(3)]
     n_f = [geom_f.dom_hi[d] - lo_f[d] + 1 for d in range(3)]
     out = []
@@ -177,7 +220,7 @@ def projected_coverage(boxes_f2, geom_f, geom_c, n_proper):
                         nxt.append((q0, q1))
             parts = nxt
         out.extend((tuple(p0), tuple(p1)) for p0, p1 in parts)
-    return coverage(coarsen_boxes(out), geom_c)
+    return coarsen_boxes(out)
 
 
 def coarsen_boxes(boxes):
@@ -317,9 +360,11 @@ class Hierarchy:
             raise ValueError("planar stepping needs fine grids that span the domain in z")
         self.cfs[l] = cf
 
-    def _set_level(self, l, level, S=None, t=0.0, cf=None):
+    def _set_level(self, l, level, S=None, t=0.0, cf=None, relink=True):
         """Install level l at time t (None removes it and everything finer);
-        cf: its register with level l-1 when already built."""
+        cf: its register with level l-1 when already built.  relink False:
+        the next finer level is about to be rebuilt too, so it is not joined
+        to the new level first."""
         if level is None:
             for m in range(l, self.max_level + 1):
                 self._retire(self.levels[m], self.cfs[m])
@@ -346,10 +391,11 @@ class Hierarchy:
         if l + 1 <= self.max_level and self.levels[l + 1] is not None:
             self._retire(self.cfs[l + 1])
             self.cfs[l + 1] = None
-            try:
-                self._link(l + 1)
-            except RuntimeError:
-                pass
+            if relink:
+                try:
+                    self._link(l + 1)
+                except RuntimeError:
+                    pass
 
     def _tensor(self, l, ptr):
         for t in self.bufs[l].values():
@@ -406,6 +452,21 @@ class Hierarchy:
         for levc in range(max_crse, lbase - 1, -1):
             levf = levc + 1
             g = self.geoms[levc]
+            if self.n_error_buf == 0:
+                # tags and nesting region in the level's own layout (a level
+                # covers a few percent of its domain), clustered from there
+                lev = self.levels[levc]
+                self.fill(levc, None, t)
+                flags = tag_dengrad_nest(lev, self.S[levc], self.thr, self.n_proper if levc > 0 else 0,
+                                         self.stream).cpu().numpy()
+                extra = []
+                if levf + 1 in new and new[levf + 1]:
+                    extra = projected_boxes(new[levf + 1], self.geoms[levf], self.n_proper)
+                boxes = cluster_box_tags(lev.boxes, flags, extra, levc > 0, self.block, self.grid_eff, self.max_c)
+                new[levf] = refine_boxes(boxes)
+                if boxes:
+                    new_finest = max(new_finest, levf)
+                continue
             tags = self._tags(levc, t)
             if levf + 1 in new and new[levf + 1]:
                 tags |= projected_coverage(new[levf + 1], self.geoms[levf], g, self.n_proper)
@@ -429,7 +490,7 @@ class Hierarchy:
             out[l] = refine_boxes(fb)
         return out
 
-    def _build(self, l, boxes, t, init_from_case):
+    def _build(self, l, boxes, t, init_from_case, relink=True):
         """(Re)build level l on `boxes` at time t: case data, or the old level's
         cells where it had them and the interpolated coarser level elsewhere."""
         old_level = self.levels[l] if l <= self.finest else None
@@ -444,7 +505,7 @@ class Hierarchy:
                 copy_level(lev, Sn, old_level, old_S, self.stream)
             self.fill(l - 1, None, t)
             cf.interp_fine(Sn, self.S[l - 1], old_level, self.stream)
-        self._set_level(l, lev, Sn, t, cf)
+        self._set_level(l, lev, Sn, t, cf, relink)
         self.finest = max(self.finest, l)
 
     def start(self):
@@ -487,7 +548,7 @@ class Hierarchy:
         new, nf = self.grid_places(lbase, t)
         for l in range(lbase + 1, nf + 1):
             if new[l]:
-                self._build(l, new[l], t, False)
+                self._build(l, new[l], t, False, relink=not (l + 1 <= nf and new.get(l + 1)))
         if nf < self.finest:
             self._set_level(nf + 1, None)
         self._check_links()

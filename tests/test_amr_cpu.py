@@ -417,3 +417,65 @@ def test_library_clustering_matches_oracle_on_fronts(masked):
             want = sorted(ao.cluster(d, (0, 0, 0), block, eff, mx, mask))
             got = sorted(cluster_tags(d, (0, 0, 0), block, eff, mx, mask))
             assert got == want, (trial, block)
+
+
+def _level_flags(boxes, dense_bits):
+    """Flags in the level's 0-ghost layout (box order, i fastest) from a dense
+    (nz, ny, nx) array over a domain starting at 0."""
+    out = []
+    for lo, hi in boxes:
+        out.append(dense_bits[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1].reshape(-1))
+    return np.concatenate(out).astype(np.uint8)
+
+
+@pytest.mark.parametrize("block", [1, 2])
+def test_box_layout_clustering_equals_dense(block):
+    """Clustering from tags in a level's own layout (with the proper nesting
+    region in bit 1 and the projected finer grids as extra boxes) returns the
+    boxes of the dense clustering of (tags | extra) & region within the
+    region, and those of the oracle."""
+    import ocfd
+    from ocfd.amr import cluster_box_tags, cluster_tags, coverage, proper_nesting
+    rng = np.random.default_rng(40 + block)
+    for trial in range(10):
+        nb = (int(rng.integers(3, 7)), int(rng.integers(2, 5)), 1)
+        bs = 8
+        n = (nb[0] * bs, nb[1] * bs, 4)
+        periodic = (int(rng.integers(0, 2)), 0, 1)
+        g = ocfd.make_geom(n, (0, 0, 0), (1.0, n[1] / n[0], n[2] / n[0]), periodic=periodic)
+        full = trial % 3 == 0          # a base level covering the domain: no region
+        boxes = []
+        for bj in range(nb[1]):
+            for bi in range(nb[0]):
+                if full or rng.random() < 0.6:
+                    boxes.append(((bi * bs, bj * bs, 0), (bi * bs + bs - 1, bj * bs + bs - 1, n[2] - 1)))
+        if not boxes:
+            continue
+        shape = (n[2], n[1], n[0])
+        tags = np.zeros(shape, np.uint8)
+        for _ in range(int(rng.integers(1, 4))):
+            x0, sl, w = int(rng.integers(0, n[0])), rng.uniform(-1.5, 1.5), int(rng.integers(1, 3))
+            for j in range(n[1]):
+                x = int(x0 + sl * j)
+                if 0 <= x < n[0] - w:
+                    tags[:, j, x:x + w] = 1
+        tags |= (rng.random(shape) < 0.01).astype(np.uint8)
+        cov = coverage(boxes, g).astype(np.uint8)
+        tags &= cov                                   # a level tags its own cells only
+        region = cov.copy() if full else proper_nesting(boxes, g, 1).astype(np.uint8)
+        extra = []
+        for _ in range(int(rng.integers(0, 3))):
+            lo = [int(rng.integers(0, n[d] - 2)) for d in range(3)]
+            hi = [min(n[d] - 1, lo[d] + int(rng.integers(0, 5))) for d in range(3)]
+            extra.append((tuple(lo), tuple(hi)))
+        dense = tags | coverage(extra, g).astype(np.uint8) if extra else tags.copy()
+        dense &= region
+        flags = _level_flags(boxes, tags | (region << 1))
+        for eff, mx in ((0.7, 16), (0.9, 8)):
+            if not dense.any():
+                continue
+            mask = None if full else region
+            want = sorted(cluster_tags(dense, (0, 0, 0), block, eff, mx, mask))
+            got = sorted(cluster_box_tags(boxes, flags, extra, not full, block, eff, mx))
+            assert got == want, (trial, block, eff)
+            assert got == sorted(ao.cluster(dense, (0, 0, 0), block, eff, mx, mask)), (trial, block, eff)

@@ -237,6 +237,33 @@ def test_compute_dsdt_feeds_flux_registers(torch_cuda, orc, flags):
         lc.compute_dsdt_fr(Sc, Dc1, p, flags, dt=1.0, current=cf1)
 
 
+@pytest.mark.parametrize("periodic_x", [0, 1])
+def test_tag_nest_flags_match_host(torch_cuda, orc, periodic_x):
+    """The device flags of a partial level: bit 0 equals ocfd_tag_dengrad's
+    tag, bit 1 the host's dense proper nesting region (n_proper 1 and 2) at
+    the level's cells."""
+    import torch
+    import ocfd
+    from ocfd import amr, cases
+    boxes, g, og, kind, A = _dmr(orc)
+    g.periodic[0] = periodic_x
+    part = [b for q, b in enumerate(boxes) if q % 3 != 1]
+    keep = [q for q, b in enumerate(boxes) if q % 3 != 1]
+    lev = ocfd.Level(part, g)
+    S = lev.new_mf()
+    gu.upload(lev, S, [A[q] for q in keep])
+    tags = amr.tag_dengrad(lev, S, 0.05).cpu().numpy()
+    for n_proper in (1, 2):
+        flags = amr.tag_dengrad_nest(lev, S, 0.05, n_proper).cpu().numpy()
+        assert np.array_equal(flags & 1, tags)
+        pn = amr.proper_nesting(part, g, n_proper).astype(np.uint8)
+        want = np.concatenate([pn[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1].reshape(-1) for lo, hi in part])
+        assert np.array_equal(flags >> 1, want), n_proper
+        assert 0 < want.sum() < want.size
+    assert tags.sum() > 0
+    torch.cuda.synchronize()
+
+
 def _compare_runs(T, O, tol):
     key = lambda b: (tuple(b[0]), tuple(b[1]))
     assert sorted(map(key, T.fine.boxes)) == sorted(map(key, O.fboxes))

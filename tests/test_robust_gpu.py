@@ -206,3 +206,37 @@ def test_graph_holds_the_tile_shape(torch_cuda):
     for a, b in zip(gu.download(lev, A), gu.download(lev, E)):
         assert np.array_equal(a[:, 4:-4, 4:-4, 4:-4], b[:, 4:-4, 4:-4, 4:-4])
     del G
+
+
+def test_device_pool_reuses_level_blocks(torch_cuda):
+    """A destroyed level's tables and scratch go back to the library's pool and
+    a level of the same boxes takes them again; its dSdt equals that of a
+    level built on fresh blocks; releasing the pool empties it."""
+    import gc
+    import torch
+    import ocfd
+    from ocfd import cases
+    boxes, g, kind = cases.kh3d(32, max_grid=(16, 16, 32))
+    p = ocfd.default_parm()
+    ocfd.release_cached_memory()
+    assert ocfd.cached_device_bytes() == 0
+    lev = ocfd.Level(boxes, g)
+    S = lev.new_mf()
+    lev.init_state(kind, S, p, seed=11)
+    D1 = lev.new_mf(ngrow=0)
+    lev.compute_dsdt(S, D1, p, ocfd.VISC)
+    torch.cuda.synchronize()
+    del lev
+    gc.collect()
+    held = ocfd.cached_device_bytes()
+    assert held > 0
+    lev2 = ocfd.Level(boxes, g)
+    assert ocfd.cached_device_bytes() < held          # blocks taken from the pool
+    D2 = lev2.new_mf(ngrow=0)
+    lev2.compute_dsdt(S, D2, p, ocfd.VISC)
+    torch.cuda.synchronize()
+    assert torch.equal(D1, D2)
+    del lev2
+    gc.collect()
+    ocfd.release_cached_memory()
+    assert ocfd.cached_device_bytes() == 0
