@@ -345,7 +345,7 @@ def _cells(boxes):
     return n
 
 
-def amr_dmr(steps, warmup=40, nx=1024, ny=256, max_level=2, cpu_sample=True, uniform=True):
+def amr_dmr(steps, warmup=40, nx=1024, ny=256, max_level=2, cpu_sample=True, uniform=True, fp32=False):
     """Double Mach reflection with AMR: base 1024x256x2 in uneven boxes of 32 to
     128 cells (seeded), max_level 2 at ratio 2, tagged on density gradient
     (refine_dengrad 0.05), regrid every 2 steps of each level, blocking factor 2,
@@ -353,7 +353,7 @@ def amr_dmr(steps, warmup=40, nx=1024, ny=256, max_level=2, cpu_sample=True, uni
     FP64, planar stepping of the z-uniform flow.  A cell update is one cell
     advanced by its level's dt (a level-l cell counts 2^l times per coarse
     step).  The same time span on the uniform grid at the finest resolution is
-    timed beside it."""
+    timed beside it.  fp32: both runs in the FP32-flux / FP64-state form."""
     import torch
     import ocfd
     from ocfd import amr, cases
@@ -361,7 +361,8 @@ def amr_dmr(steps, warmup=40, nx=1024, ny=256, max_level=2, cpu_sample=True, uni
     boxes, g, kind = cases.dmr(nx, ny, 2, sizes=(32, 64, 128), seed=1234)
     kw = dict(scheme=2, max_level=max_level, refine_dengrad=0.05, regrid_int=2, blocking_factor=2, grid_eff=0.7,
               max_grid_size=64)
-    H = amr.Hierarchy(boxes, g, p, 0.5, kind, flags=ocfd.PLANAR, **kw)
+    flags = ocfd.PLANAR | (ocfd.FLUX_FP32 if fp32 else 0)
+    H = amr.Hierarchy(boxes, g, p, 0.5, kind, flags=flags, **kw)
     for _ in range(warmup):
         H.step()
     torch.cuda.synchronize()
@@ -378,7 +379,8 @@ def amr_dmr(steps, warmup=40, nx=1024, ny=256, max_level=2, cpu_sample=True, uni
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     nc = _cells(boxes)
-    out = {"workload": f"DMR {nx}x{ny}x2 base (uneven boxes 32-128) + {max_level} levels of ratio 2, SSP-RK3, FP64",
+    form = "FP32 fluxes / FP64 state" if fp32 else "FP64"
+    out = {"workload": f"DMR {nx}x{ny}x2 base (uneven boxes 32-128) + {max_level} levels of ratio 2, SSP-RK3, {form}",
            "coarse_steps": steps, "after_steps": warmup, "ms_per_coarse_step": 1e3 * el / steps,
            "cell_updates_per_s": upd / el, "cell_updates_2d_per_s": upd2 / el,
            "note": "cell_updates counts every z layer (2, 4, 8 per level) as the reference's 3D build does; the "
@@ -394,13 +396,13 @@ def amr_dmr(steps, warmup=40, nx=1024, ny=256, max_level=2, cpu_sample=True, uni
         lev = ocfd.Level(fb, fg).set_planar(True)
         A, B, C = lev.new_mf(), lev.new_mf(), lev.new_mf()
         lev.init_state(kind, A, p)
-        lev.advance(A, B, C, 0.0, dts[0] / r, 2, p, ocfd.PLANAR)
+        lev.advance(A, B, C, 0.0, dts[0] / r, 2, p, flags)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         tt = 0.0
         for dt in dts:
             for _ in range(r):
-                lev.advance(A, B, C, tt, dt / r, 2, p, ocfd.PLANAR)
+                lev.advance(A, B, C, tt, dt / r, 2, p, flags)
                 tt += dt / r
         torch.cuda.synchronize()
         elu = time.perf_counter() - t0
@@ -572,7 +574,8 @@ def main():
         print(json.dumps({"chem_channel": chem_channel(args.channel_steps, cpu_sample=False)}))
         return
     if args.amr_only:
-        out = {"amr_dmr": amr_dmr(args.amr_steps, cpu_sample=False)}
+        out = {"amr_dmr": amr_dmr(args.amr_steps, cpu_sample=False),
+               "amr_dmr_fp32_flux": amr_dmr(args.amr_steps, cpu_sample=False, fp32=True)}
         if args.dmr_to_t:
             out["amr_dmr_to_t02"] = amr_dmr_to_time()
         print(json.dumps(out))
@@ -618,6 +621,7 @@ def main():
     amr_res = None
     if not args.no_amr:
         amr_res = amr_dmr(args.amr_steps, cpu_sample=not args.no_cpu_baseline)
+        amr_res["fp32_flux"] = amr_dmr(args.amr_steps, cpu_sample=False, fp32=True)
         if args.dmr_to_t:
             amr_res["to_t02"] = amr_dmr_to_time()
 

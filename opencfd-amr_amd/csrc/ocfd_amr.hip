@@ -588,9 +588,12 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
             fcrs[b].lo[d] = F.boxes[b].lo[d] >> 1;
             fcrs[b].hi[d] = F.boxes[b].hi[d] >> 1;
         }
+    // (tiles of 4^3 cells: the lookups below all lie at fine-box boundaries,
+    // where coarser tiles would hold several boxes each)
+    const int fshift = 2;
     std::vector<int> fmap, flist;
     int fmn[3];
-    build_tile_map(CG.dom_lo, CG.dom_hi, CG.map_shift, fcrs, 0, fmn, fmap, flist);
+    build_tile_map(CG.dom_lo, CG.dom_hi, fshift, fcrs, 0, fmn, fmap, flist);
     // the fine box covering coarse cell (I,J,K) after the periodic wrap (fine
     // boxes are aligned to 2, so it holds all 8 fine cells under it), -1 none,
     // -2 outside a non-periodic domain side
@@ -600,7 +603,7 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
             iv[d] = wrap(CG, d, iv[d]);
             if (iv[d] < CG.dom_lo[d] || iv[d] > CG.dom_hi[d]) return -2;
         }
-        return tile_owner(fmap.data(), flist.data(), fmn, CG.dom_lo, CG.map_shift, fcrs.data(), iv[0], iv[1], iv[2], 0);
+        return tile_owner(fmap.data(), flist.data(), fmn, CG.dom_lo, fshift, fcrs.data(), iv[0], iv[1], iv[2], 0);
     };
     // candidates (coarse box, K, J, I), visited in coarse-box then k, j, i
     // order: the cells face-adjacent to a coarsened fine box (an edge or corner
@@ -637,6 +640,10 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
     std::vector<CFFace> cf, ff;
     std::vector<int4> cells;
     std::vector<int> cellfaces;
+    cf.reserve(2 * cand.size());
+    ff.reserve(8 * cand.size());
+    cells.reserve(cand.size());
+    cellfaces.reserve(6 * cand.size());
     for (const auto& q : cand) {
         const int b = q[0], K = q[1], J = q[2], I = q[3];
         int fidx[6] = {-1, -1, -1, -1, -1, -1};
@@ -686,15 +693,22 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
     int cmn[3];
     build_tile_map(CG.dom_lo, CG.dom_hi, CG.map_shift, C.boxes, 3, cmn, cmap, clist);
     cmap.insert(cmap.end(), clist.begin(), clist.end());
-    // the fine level must lie in the coarse level's valid cells
-    for (const BoxDesc& B : F.boxes)
-        for (int K = B.lo[2] >> 1; K <= B.hi[2] >> 1; ++K)
-            for (int J = B.lo[1] >> 1; J <= B.hi[1] >> 1; ++J)
-                for (int I = B.lo[0] >> 1; I <= B.hi[0] >> 1; ++I)
-                    if (find_box(C, I, J, K) < 0) {
-                        set_error("ocfd_cf_create: the fine level is not nested in the coarse level");
-                        return OCFD_EINVAL;
-                    }
+    // the fine level must lie in the coarse level's valid cells: the coarse
+    // boxes are disjoint, so a coarsened fine box is covered when its
+    // intersections with them add up to its volume
+    for (const BoxDesc& B : fcrs) {
+        long vol = 1, got = 0;
+        for (int d = 0; d < 3; ++d) vol *= B.hi[d] - B.lo[d] + 1;
+        for (const BoxDesc& A : C.boxes) {
+            long v = 1;
+            for (int d = 0; d < 3 && v > 0; ++d) v *= std::max(0, std::min(A.hi[d], B.hi[d]) - std::max(A.lo[d], B.lo[d]) + 1);
+            got += v;
+        }
+        if (got != vol) {
+            set_error("ocfd_cf_create: the fine level is not nested in the coarse level");
+            return OCFD_EINVAL;
+        }
+    }
     ocfd_cf c = new ocfd_cf_s;
     c->coarse = coarse; c->fine = fine;
     c->nface = (int)cf.size(); c->ncell = (int)cells.size();
