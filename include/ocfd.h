@@ -307,6 +307,28 @@ int ocfd_reactor_adaptive_chunk(long n, double* state, double* th, double t_end,
                                 int max_steps, int budget, int* steps, int* rejects, unsigned int* remaining,
                                 void* stream);
 
+/* Integration to t_end by a Rosenbrock method (no counterpart in the
+ * reference, whose chemistry takes fixed linearised-implicit steps,
+ * Source/CHEM/ChemAdvance.cpp:48-81): RODAS3 (4 stages, order 3 with an
+ * embedded order-2 solution, L-stable; Sandu et al., Atmos. Environ. 31
+ * (1997) 3459) on d rho_k/dt = W_k wdot_k with T from the sensible energy,
+ * one 9x9 LU per step of the full Jacobian (complete fixed-T derivative of
+ * the rates, third bodies and fall-off included, plus the temperature
+ * coupling).  Error max_k |K4_k|/(atol rho + rtol max(|y_k|,|y1_k|)); h <- h
+ * clamp(0.9 err^(-1/3), 0.2, 6), not growing right after a rejection.
+ * Arguments as ocfd_reactor_advance_adaptive (work: 8 bytes). */
+int ocfd_reactor_advance_rosenbrock(long n, double* state, double t_end, double h0, double rtol, double atol,
+                                    int max_steps, int* steps_out, int* rejects_out, void* work, void* stream);
+
+/* Resumable pieces of the Rosenbrock integration, as the adaptive ones
+ * (no counterpart in the reference): th is device double[3n] (times, steps,
+ * temperature guesses); chunks in a row give the results of
+ * ocfd_reactor_advance_rosenbrock. */
+int ocfd_reactor_rosenbrock_begin(long n, double* th, double h0, int* steps, int* rejects, void* stream);
+int ocfd_reactor_rosenbrock_chunk(long n, double* state, double* th, double t_end, double rtol, double atol,
+                                  int max_steps, int budget, int* steps, int* rejects, unsigned int* remaining,
+                                  void* stream);
+
 /* Temperature of each reactor (GET_T_GIVEN_EY, Source/CHEM/LiDryer.H:836-888). */
 int ocfd_reactor_temperature(long n, const double* state, double* T, void* stream);
 

@@ -295,6 +295,126 @@ __global__ void k_reactor_chunk_init(long n, double* __restrict__ th, double h0,
 
 __global__ void k_zero_u32(unsigned int* p) { *p = 0u; }
 
+// One attempt of an adaptive RODAS3 step of a reactor: on acceptance the
+// densities are clipped at 0 (the energy follows the clipped change) and
+// h <- h clamp(0.9 err^(-1/3), 0.2, 6), not above h after a rejection; on
+// rejection h <- h max(0.2, 0.9 err^(-1/3)).  Tg: the temperature guess of the
+// next step (the last accepted stage's temperature; 0 before the first).
+__device__ __forceinline__ void rosenbrock_substep(double* rhoi, double& E, double rho, double& t, double& h, double& Tg,
+                                                   bool& last_rej, int& acc, int& rej, double rtol, double atol,
+                                                   double t_end, chem::LaneMat M)
+{
+#pragma clang fp contract(off)
+    if (t + h > t_end) h = t_end - t;
+    double yn[NS], En, Tl;
+    const double err = chem::rodas3_step(rhoi, E, Tg, h, rho, rtol, atol, yn, En, Tl, M);
+    const double fac = 0.9 / cbrt(err);
+    if (err <= 1.0) {
+        double dy[NS];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            const double v = fmax(yn[n], 0.0);
+            dy[n] = v - rhoi[n];
+            rhoi[n] = v;
+        }
+        E += chem::energy_of(dy);
+        Tg = Tl;
+        t += h;
+        ++acc;
+        double f = fmin(6.0, fmax(0.2, fac));
+        if (last_rej) f = fmin(f, 1.0);
+        h *= f;
+        last_rej = false;
+    } else {
+        ++rej;
+        h *= fmax(0.2, fac);
+        last_rej = true;
+    }
+}
+
+// Persistent RODAS3 integration, one lane per reactor taken from a device
+// work counter (lanes that finish an easy reactor take the next one)
+__global__ __launch_bounds__(64) void k_reactor_rosenbrock(long n, double* __restrict__ st, double t_end, double h0,
+                                                           double rtol, double atol, int max_steps,
+                                                           int* __restrict__ steps_out, int* __restrict__ rej_out,
+                                                           unsigned long long* __restrict__ counter)
+{
+    __shared__ double mat[NS * NS * 64];   // the iteration matrix of each lane
+    const chem::LaneMat M{mat + threadIdx.x};
+    long r = (long)atomicAdd(counter, 1ull);
+    while (r < n) {
+        double rhoi[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) rhoi[k] = st[(RROW_Y + k) * n + r];
+        double E = st[RROW_E * n + r];
+        const double rho = st[RROW_RHO * n + r];
+        double t = 0.0, h = h0, Tg = 0.0;
+        bool last_rej = false, failed = false;
+        int acc = 0, rej = 0;
+        while (t < t_end) {
+            if (acc + rej >= max_steps) { failed = true; break; }
+            rosenbrock_substep(rhoi, E, rho, t, h, Tg, last_rej, acc, rej, rtol, atol, t_end, M);
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) st[(RROW_Y + k) * n + r] = rhoi[k];
+        st[RROW_E * n + r] = E;
+        if (steps_out) steps_out[r] = failed ? -1 : acc;
+        if (rej_out) rej_out[r] = rej;
+        r = (long)atomicAdd(counter, 1ull);
+    }
+}
+
+// Resumable RODAS3 integration (see k_reactor_chunk): th holds t, h and the
+// temperature guess (n entries each); a rejected last attempt is kept as a
+// negative h.  The substeps are those of k_reactor_rosenbrock.
+__global__ __launch_bounds__(64) void k_rosenbrock_chunk(long n, double* __restrict__ st, double* __restrict__ th,
+                                                         double t_end, double rtol, double atol, int max_steps,
+                                                         int budget, int* __restrict__ acc_io, int* __restrict__ rej_io,
+                                                         unsigned int* __restrict__ remaining)
+{
+    __shared__ double mat[NS * NS * 64];   // the iteration matrix of each lane
+    const chem::LaneMat M{mat + threadIdx.x};
+    const long r = (long)blockIdx.x * 64 + threadIdx.x;
+    if (r >= n) return;
+    int acc = acc_io[r], rej = rej_io[r];
+    double t = th[r], h = th[n + r], Tg = th[2 * n + r];
+    if (acc < 0 || t >= t_end) return;
+    bool last_rej = h < 0.0;
+    h = fabs(h);
+    double rhoi[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) rhoi[k] = st[(RROW_Y + k) * n + r];
+    double E = st[RROW_E * n + r];
+    const double rho = st[RROW_RHO * n + r];
+    bool failed = false;
+    for (int it = 0; it < budget; ++it) {
+        if (t >= t_end) break;
+        if (acc + rej >= max_steps) { failed = true; break; }
+        rosenbrock_substep(rhoi, E, rho, t, h, Tg, last_rej, acc, rej, rtol, atol, t_end, M);
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) st[(RROW_Y + k) * n + r] = rhoi[k];
+    st[RROW_E * n + r] = E;
+    th[r] = t;
+    th[n + r] = last_rej ? -h : h;
+    th[2 * n + r] = Tg;
+    acc_io[r] = failed ? -1 : acc;
+    rej_io[r] = rej;
+    if (!failed && t < t_end) atomicAdd(remaining, 1u);
+}
+
+__global__ void k_rosenbrock_chunk_init(long n, double* __restrict__ th, double h0, int* __restrict__ acc,
+                                        int* __restrict__ rej)
+{
+    for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+        th[r] = 0.0;
+        th[n + r] = h0;
+        th[2 * n + r] = 0.0;
+        acc[r] = 0;
+        rej[r] = 0;
+    }
+}
+
 __global__ void k_reactor_T(long n, const double* __restrict__ st, double* __restrict__ T)
 {
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -427,6 +547,53 @@ int ocfd_reactor_adaptive_chunk(long n, double* state, double* th, double t_end,
         hipLaunchKernelGGL(k_reactor_chunk<2>, dim3((unsigned)((n + 31) / 32)), dim3(64), 0, s, n, state, th, t_end,
                            rtol, atol, max_steps, budget, steps, rejects, remaining);
     return launch_check("ocfd_reactor_adaptive_chunk");
+}
+
+int ocfd_reactor_advance_rosenbrock(long n, double* state, double t_end, double h0, double rtol, double atol,
+                                    int max_steps, int* steps_out, int* rejects_out, void* work, void* stream)
+{
+    if (n <= 0 || !state || !work || t_end < 0 || h0 <= 0 || rtol <= 0 || atol < 0 || max_steps <= 0) {
+        set_error("ocfd_reactor_advance_rosenbrock: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long* ctr = (unsigned long long*)work;
+    hipLaunchKernelGGL(k_counter_init, dim3(1), dim3(1), 0, s, ctr);
+    int dev = 0, ncu = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const long wgs = std::min<long>((n + 63) / 64, (long)std::max(ncu, 1) * 8);
+    hipLaunchKernelGGL(k_reactor_rosenbrock, dim3((unsigned)wgs), dim3(64), 0, s, n, state, t_end, h0, rtol, atol,
+                       max_steps, steps_out, rejects_out, ctr);
+    return launch_check("ocfd_reactor_advance_rosenbrock");
+}
+
+int ocfd_reactor_rosenbrock_begin(long n, double* th, double h0, int* steps, int* rejects, void* stream)
+{
+    if (n <= 0 || !th || !steps || !rejects || !(h0 > 0)) {
+        set_error("ocfd_reactor_rosenbrock_begin: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    const long nb = std::min<long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_rosenbrock_chunk_init, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, n, th, h0,
+                       steps, rejects);
+    return launch_check("ocfd_reactor_rosenbrock_begin");
+}
+
+int ocfd_reactor_rosenbrock_chunk(long n, double* state, double* th, double t_end, double rtol, double atol,
+                                  int max_steps, int budget, int* steps, int* rejects, unsigned int* remaining,
+                                  void* stream)
+{
+    if (n <= 0 || !state || !th || !steps || !rejects || !remaining || t_end < 0 || rtol <= 0 || atol < 0 ||
+        max_steps <= 0 || budget <= 0) {
+        set_error("ocfd_reactor_rosenbrock_chunk: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, remaining);
+    hipLaunchKernelGGL(k_rosenbrock_chunk, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, state, th, t_end, rtol,
+                       atol, max_steps, budget, steps, rejects, remaining);
+    return launch_check("ocfd_reactor_rosenbrock_chunk");
 }
 
 int ocfd_reactor_temperature(long n, const double* state, double* T, void* stream)

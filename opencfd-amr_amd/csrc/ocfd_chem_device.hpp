@@ -190,7 +190,8 @@ __device__ __forceinline__ double sc2(const double* sc, int a, int b) { return b
 // d wdot_l / d c_n at fixed T, Source/CHEM/LiDryer.H:156-529.  Species that do
 // not take part in a reaction contribute exact zeros in the reference's dense
 // loops; skipping them leaves every sum unchanged.
-__device__ __forceinline__ void production_rate(const double* sc, double T, double* wdot, double (*A)[NS])
+template <bool JAC>
+__device__ __forceinline__ void production_rate_t(const double* sc, double T, double* wdot, double (*A)[NS])
 {
     const double eps = 1e-20;
     const double lnT = log(T);
@@ -216,8 +217,10 @@ __device__ __forceinline__ void production_rate(const double* sc, double T, doub
     for (int n = 0; n < NS; ++n) {
         mixture += sc[n];
         wdot[n] = 0.0;
+        if constexpr (JAC) {
 #pragma unroll
-        for (int l = 0; l < NS; ++l) A[n][l] = 0.0;
+            for (int l = 0; l < NS; ++l) A[n][l] = 0.0;
+        }
     }
 #pragma unroll
     for (int m = 0; m < NR; ++m) {
@@ -262,20 +265,27 @@ __device__ __forceinline__ void production_rate(const double* sc, double T, doub
             const int n = R.sp[a];
             wdot[n] += (qf - qr) * (double)(R.nur[a] - R.nuf[a]);
         }
+        if constexpr (JAC) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            if (a >= R.np) break;
-            const int n = R.sp[a];
-            const double Awf = R.nuf[a] * qf / (sc[n] + eps);
-            const double Awr = R.nur[a] * qr / (sc[n] + eps);
+            for (int a = 0; a < 4; ++a) {
+                if (a >= R.np) break;
+                const int n = R.sp[a];
+                const double Awf = R.nuf[a] * qf / (sc[n] + eps);
+                const double Awr = R.nur[a] * qr / (sc[n] + eps);
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                if (b >= R.np) break;
-                const int l = R.sp[b];
-                A[l][n] += (Awf - Awr) * (double)(R.nur[b] - R.nuf[b]);
+                for (int b = 0; b < 4; ++b) {
+                    if (b >= R.np) break;
+                    const int l = R.sp[b];
+                    A[l][n] += (Awf - Awr) * (double)(R.nur[b] - R.nuf[b]);
+                }
             }
         }
     }
+}
+
+__device__ __forceinline__ void production_rate(const double* sc, double T, double* wdot, double (*A)[NS])
+{
+    production_rate_t<true>(sc, T, wdot, A);
 }
 
 // Gaussian elimination with partial pivoting (first maximum), in place on the
@@ -350,6 +360,339 @@ __device__ __forceinline__ double cell_step(double ei, double* rhoi, double dt)
         dei += -lo[n][5] * Ru / mw[n] * drho[n];
     }
     return dei;
+}
+
+// ---------------------------------------------------------------------------
+// Rosenbrock (RODAS3) integration of one constant-volume reactor: no
+// counterpart in the reference, whose chemistry takes fixed linearised-
+// implicit steps.  The ODE d rho_k/dt = W_k wdot_k(c, T), T from the sensible
+// energy E(rho) = E0 - sum_k hf_k (rho_k - rho_k0); RODAS3 (Sandu et al.,
+// Atmos. Environ. 31 (1997) 3459-3472: 4 stages, order 3, embedded order 2,
+// L-stable, stiffly accurate) in the transformed form
+//   (I/(h gamma) - J) K_i = f(y + sum_j a_ij K_j) + sum_j (c_ij/h) K_j,
+// J the full Jacobian: the complete fixed-T derivative plus (df/dT)(dT/drho).
+// ---------------------------------------------------------------------------
+
+// Troe factor P(Pr) = Pr/(1+Pr) F(Pr) (troe above) and its slope dP/dPr
+__device__ __forceinline__ void troe_and_slope(double redP, double Fc, double& P, double& dP)
+{
+    const double logPred = log10(redP);
+    const double logFcent = log10(Fc);
+    const double troe_c = -0.4 - 0.67 * logFcent;
+    const double troe_n = 0.75 - 1.27 * logFcent;
+    const double uu = troe_c + logPred;
+    const double den = troe_n - 0.14 * uu;
+    const double t = uu / den;
+    const double F = pow(10.0, logFcent / (1.0 + t * t));
+    P = (redP / (1.0 + redP)) * F;
+    const double q = 1.0 + t * t;
+    dP = F / ((1.0 + redP) * (1.0 + redP)) + F / (1.0 + redP) * (-2.0 * t * logFcent * troe_n) / (q * q * den * den);
+}
+
+// A 9x9 matrix of one lane in LDS: element (i, j) at base[(9 i + j) * 64]
+// (a block's 64 lanes interleaved, so a wave's accesses never conflict)
+struct LaneMat {
+    double* base;
+    __device__ __forceinline__ double& operator()(int i, int j) const { return base[(i * NS + j) * 64]; }
+};
+
+// the complete d wdot/d c (CGS) at fixed T: mass action by the product rule
+// (exact where a species is absent), third-body efficiencies and the Troe
+// fall-off, with the rates of production_rate
+__device__ __forceinline__ void jacobian_fixed_T(const double* sc, double T, LaneMat J)
+{
+    const double lnT = log(T);
+    const double invT = 1.0 / T;
+    double kfs[21];
+    rate_constants(lnT, invT, kfs);
+    double g[NS];
+    {
+        const double tc1 = T, tc2 = T * (T), tc3 = tc2 * tc1, tc4 = tc2 * tc2;
+        const double ginv = 1 / T;
+        const bool low = T < 1000;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const double* a = low ? lo[s] : hi[s];
+            g[s] = a[0] * (1.0 - lnT) - a[1] / 2.0 * tc1 - a[2] / 6.0 * tc2 - a[3] / 12.0 * tc3 -
+                   a[4] / 20.0 * tc4 + a[5] * ginv - a[6];
+        }
+    }
+    const double RsT = Ru / Patm * 1e6 * T;
+    double mixture = 0.0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        mixture += sc[n];
+#pragma unroll
+        for (int l = 0; l < NS; ++l) J(n, l) = 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+        const Rx& R = rx[m];
+        const double phi_f = sc2(sc, R.f0, R.f1), phi_r = sc2(sc, R.r0, R.r1);
+        double dpf[NS], dpr[NS];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) { dpf[n] = 0.0; dpr[n] = 0.0; }
+        if (R.f1 < 0) dpf[R.f0] += 1.0;
+        else { dpf[R.f0] += sc[R.f1]; dpf[R.f1] += sc[R.f0]; }
+        if (R.r1 < 0) dpr[R.r0] += 1.0;
+        else { dpr[R.r0] += sc[R.r1]; dpr[R.r1] += sc[R.r0]; }
+        const double k0 = kfs[R.kf];
+        double keff = k0, dk[NS];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) dk[n] = 0.0;
+        if (R.tb != 0) {
+            double eff[NS];
+#pragma unroll
+            for (int n = 0; n < NS; ++n) eff[n] = 1.0;
+            double alpha;
+            if (R.tb == 2) {
+                eff[0] += 1.0; eff[2] += 10.0; eff[1] += -0.22;
+                alpha = mixture + sc[0] + 10 * sc[2] + -0.22 * sc[1];
+            } else {
+                eff[0] += 1.5; eff[2] += 11.0;
+                alpha = mixture + 1.5 * sc[0] + 11 * sc[2];
+            }
+            if (R.tb == 1) {
+                keff = alpha * k0;
+#pragma unroll
+                for (int n = 0; n < NS; ++n) dk[n] = eff[n] * k0;
+            } else {
+                const double kl = R.tb == 2 ? 6.37e+20 * exp(-1.72 * lnT - 261.6741574193582 * invT)
+                                            : 1.20e+17 * exp(-22846.166820843966 * invT);
+                const double redP = alpha / k0 * kl;
+                double P, dP;
+                troe_and_slope(redP, R.tb == 2 ? 0.8 : 0.5, P, dP);
+                keff = k0 * P;
+#pragma unroll
+                for (int n = 0; n < NS; ++n) dk[n] = dP * kl * eff[n];
+            }
+        }
+        double Kc;
+        {
+            const KcDef& d = kcd[R.kf];
+            const double sa = d.a1 < 0 ? g[d.a0] : g[d.a0] + g[d.a1];
+            const double sb = d.b1 < 0 ? g[d.b0] : g[d.b0] + g[d.b1];
+            const double e = exp((sa) - (sb));
+            Kc = d.fac == 0 ? e : (d.fac == 1 ? 1.0 / RsT * e : RsT * e);
+        }
+        const double diff = phi_f - phi_r / Kc;
+        double dq[NS];
+#pragma unroll
+        for (int n = 0; n < NS; ++n) dq[n] = keff * (dpf[n] - dpr[n] / Kc) + dk[n] * diff;
+        if (R.dup >= 0) {
+            const KcDef& d = kcd[R.dup];
+            const double sa = d.a1 < 0 ? g[d.a0] : g[d.a0] + g[d.a1];
+            const double sb = d.b1 < 0 ? g[d.b0] : g[d.b0] + g[d.b1];
+            const double Kc2 = exp((sa) - (sb));
+            const double k2 = kfs[R.dup];
+#pragma unroll
+            for (int n = 0; n < NS; ++n) dq[n] += k2 * (dpf[n] - dpr[n] / Kc2);
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            if (b >= R.np) break;
+            const int l = R.sp[b];
+            const int nu = R.nur[b] - R.nuf[b];
+            if (nu == 0) continue;
+#pragma unroll
+            for (int n = 0; n < NS; ++n) J(l, n) += nu * dq[n];
+        }
+    }
+}
+
+// Newton inversion of the sensible energy from a temperature guess (the
+// iteration of T_given_ey); T_given_ey itself when the guess or an iterate
+// leaves [250, 3500] K
+__device__ __forceinline__ double T_newton(double ein, const double* rhoi, double Tg)
+{
+    const double tol = 1e-7;
+    if (!(Tg >= 250.0 && Tg <= 3500.0)) return T_given_ey(ein, rhoi);
+    double t1 = Tg;
+    for (int i = 0; i < 50; ++i) {
+        double e1, cv;
+        e_cv(t1, rhoi, e1, cv, true);
+        double dt = (ein - e1) / cv;
+        if (dt > 100.0) dt = 100.0;
+        else if (dt < -100.0) dt = -100.0;
+        else if (fabs(dt) < tol) return t1;
+        else if (t1 + dt == t1) return t1;
+        t1 += dt;
+        if (t1 < 250.0 || t1 > 3500.0) return T_given_ey(ein, rhoi);
+    }
+    return t1;
+}
+
+// f = d rho/dt (kg/m^3/s) at T
+__device__ __forceinline__ void reactor_f(const double* y, double T, double* f)
+{
+    double c[NS], wdot[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) c[n] = y[n] / mw[n] * 1e-6;
+    production_rate_t<false>(c, T, wdot, nullptr);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) f[i] = wdot[i] * mw[i] * 1e6;
+}
+
+// LU with partial pivoting (first maximum) in place on the lane's LDS matrix;
+// a row exchange at step k moves columns k.. only
+__device__ __forceinline__ void lu_factor(LaneMat M, int* piv)
+{
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        int p = k;
+        double mx = 0.0;
+#pragma unroll
+        for (int i = k; i < NS; ++i) {
+            const double v = fabs(M(i, k));
+            if (v > mx) { mx = v; p = i; }
+        }
+        piv[k] = p;
+        if (p != k) {
+#pragma unroll
+            for (int j = k; j < NS; ++j) {
+                const double a = M(p, j);
+                M(p, j) = M(k, j);
+                M(k, j) = a;
+            }
+        }
+        const double pv = M(k, k);
+#pragma unroll
+        for (int i = k + 1; i < NS; ++i) {
+            const double m = M(i, k) / pv;
+            M(i, k) = m;
+#pragma unroll
+            for (int j = k + 1; j < NS; ++j) M(i, j) -= m * M(k, j);
+        }
+    }
+}
+
+// b <- the solution of (the factored M) x = b; b in registers (exchanges by
+// selects: no dynamic register indexing)
+__device__ __forceinline__ void lu_solve(LaneMat M, const int* piv, double* b)
+{
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+#pragma unroll
+        for (int i = k + 1; i < NS; ++i) {
+            const bool sw = (piv[k] == i);
+            const double a = b[i], c = b[k];
+            b[i] = sw ? c : a;
+            b[k] = sw ? a : c;
+        }
+#pragma unroll
+        for (int i = k + 1; i < NS; ++i) b[i] -= M(i, k) * b[k];
+    }
+#pragma unroll
+    for (int k = NS - 1; k >= 0; --k) {
+        double sum = b[k];
+#pragma unroll
+        for (int j = k + 1; j < NS; ++j) sum -= M(k, j) * b[j];
+        b[k] = sum / M(k, k);
+    }
+}
+
+// energy change of a change dy of the partial densities (formation enthalpies)
+__device__ __forceinline__ double energy_of(const double* dy)
+{
+    double de = 0.0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) de += -lo[n][5] * Ru / mw[n] * dy[n];
+    return de;
+}
+
+// One RODAS3 step of h from (y, E) with temperature guess Tg: the new state in
+// ynew/Enew, the last stage's temperature in Tlast; returns the error
+// max_k |K4_k| / (atol rho + rtol max(|y_k|, |ynew_k|)).  The four rate
+// evaluations (y at T0, y at T0 + dT, the stages at y + 2 K1 and y + 2 K1 +
+// K3) run as one loop, so the kernel holds one copy of the rate code.
+__device__ __forceinline__ double rodas3_step(const double* y, double E, double Tg, double h, double rho, double rtol,
+                                              double atol, double* ynew, double& Enew, double& Tlast, LaneMat M)
+{
+    double f0[NS], K1[NS], K2[NS], K3[NS], K4[NS], dy[NS];
+    int piv[NS];
+    const double hinv = 1.0 / h;
+    double T0 = 0.0, Tprev = Tg, T4 = 0.0;
+#pragma unroll 1
+    for (int it = 0; it < 4; ++it) {
+        double ys[NS], fs[NS], Ecur = E;
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            dy[n] = it == 2 ? 2.0 * K1[n] : (it == 3 ? 2.0 * K1[n] + K3[n] : 0.0);
+            ys[n] = it >= 2 ? y[n] + dy[n] : y[n];
+        }
+        if (it >= 2) Ecur = E + energy_of(dy);
+        double Tc;
+        if (it == 1) Tc = T0 + 1e-7 * T0;
+        else Tc = T_newton(Ecur, ys, Tprev);
+        reactor_f(ys, Tc, fs);
+        if (it == 0) {
+            T0 = Tc;
+#pragma unroll
+            for (int n = 0; n < NS; ++n) f0[n] = fs[n];
+        } else if (it == 1) {
+            // M = I/(h gamma) - J, J = the fixed-T Jacobian + (df/dT)(dT/drho)
+            const double dT = 1e-7 * T0;
+            double c[NS];
+#pragma unroll
+            for (int n = 0; n < NS; ++n) c[n] = y[n] / mw[n] * 1e-6;
+            jacobian_fixed_T(c, T0, M);
+            double dTdy[NS];
+            {
+                const double tc1 = T0, tc2 = T0 * T0, tc3 = T0 * T0 * T0, tc4 = T0 * T0 * T0 * T0;
+                const double invT = 1.0 / T0;
+                const bool low = T0 < 1000;
+                const double cv = ckcvbs(T0, y);
+#pragma unroll
+                for (int s2 = 0; s2 < NS; ++s2) {
+                    const double* a = low ? lo[s2] : hi[s2];
+                    double u = a[0] - 1.0 + a[1] / 2.0 * tc1 + a[2] / 3.0 * tc2 + a[3] / 4.0 * tc3 + a[4] / 5.0 * tc4;
+                    if (!low) u = u + (hi[s2][5] - lo[s2][5]) * invT;
+                    dTdy[s2] = (-lo[s2][5] * Ru / mw[s2] - u * Ru * T0 / mw[s2]) / cv;
+                }
+            }
+            const double gh = 1.0 / (h * 0.5);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+                const double dfdT = (fs[i] - f0[i]) / dT;
+#pragma unroll
+                for (int j = 0; j < NS; ++j)
+                    M(i, j) = -(M(i, j) * mw[i] / mw[j] + dfdT * dTdy[j]) + (i == j ? gh : 0.0);
+            }
+            lu_factor(M, piv);
+#pragma unroll
+            for (int n = 0; n < NS; ++n) K1[n] = f0[n];
+            lu_solve(M, piv, K1);
+#pragma unroll
+            for (int n = 0; n < NS; ++n) K2[n] = f0[n] + (4.0 * hinv) * K1[n];
+            lu_solve(M, piv, K2);
+            Tprev = T0;
+        } else if (it == 2) {
+#pragma unroll
+            for (int n = 0; n < NS; ++n) K3[n] = fs[n] + hinv * K1[n] - hinv * K2[n];
+            lu_solve(M, piv, K3);
+            Tprev = Tc;
+        } else {
+#pragma unroll
+            for (int n = 0; n < NS; ++n) K4[n] = fs[n] + hinv * K1[n] - hinv * K2[n] - (8.0 / 3.0 * hinv) * K3[n];
+            lu_solve(M, piv, K4);
+            T4 = Tc;
+        }
+    }
+    double err = 0.0;
+    bool finite = true;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        dy[n] = 2.0 * K1[n] + K3[n] + K4[n];
+        ynew[n] = y[n] + dy[n];
+        const double e = fabs(K4[n]) / (atol * rho + rtol * fmax(fabs(y[n]), fabs(ynew[n])));
+        finite = finite && __builtin_isfinite(e) && __builtin_isfinite(ynew[n]);
+        err = fmax(err, e);
+    }
+    Enew = E + energy_of(dy);
+    Tlast = T4;
+    // a stage that left the thermodynamic tables (a non-physical temperature,
+    // non-finite rates) fails the step: it is retried with a smaller h
+    return finite ? err : __builtin_huge_val();
 }
 
 #pragma clang fp contract(on)

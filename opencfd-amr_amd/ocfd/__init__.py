@@ -182,6 +182,11 @@ def lib():
                                             ip, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                             ctypes.c_int, ip, ip]
         L.ocfd_reactor_adaptive_begin.argtypes = [ctypes.c_long, vp, ctypes.c_double, vp, vp, vp]
+        L.ocfd_reactor_advance_rosenbrock.argtypes = [ctypes.c_long, vp, ctypes.c_double, ctypes.c_double,
+                                                      ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp, vp, vp]
+        L.ocfd_reactor_rosenbrock_begin.argtypes = [ctypes.c_long, vp, ctypes.c_double, vp, vp, vp]
+        L.ocfd_reactor_rosenbrock_chunk.argtypes = [ctypes.c_long, vp, vp, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
         L.ocfd_reactor_adaptive_chunk.argtypes = [ctypes.c_long, vp, vp, ctypes.c_double, ctypes.c_double,
                                                   ctypes.c_double, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
         L.ocfd_mask_grow.argtypes = [ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -648,20 +653,31 @@ class Reactors:
                                                   _ptr(self.steps), _ptr(self.rejects), _ptr(self.work),
                                                   _stream(stream)), "ocfd_reactor_advance_adaptive")
 
-    def begin_chunks(self, h0=1e-8, stream=None):
-        """Resumable adaptive integration: per-reactor time 0 and step h0."""
+    def advance_rosenbrock(self, t_end, h0=1e-8, rtol=1e-4, atol=1e-9, max_steps=200000, stream=None):
+        """RODAS3 integration of every reactor to t_end."""
+        check(lib().ocfd_reactor_advance_rosenbrock(self.n, _ptr(self.state), t_end, h0, rtol, atol, max_steps,
+                                                    _ptr(self.steps), _ptr(self.rejects), _ptr(self.work),
+                                                    _stream(stream)), "ocfd_reactor_advance_rosenbrock")
+
+    def begin_chunks(self, h0=1e-8, stream=None, method="adaptive"):
+        """Resumable integration: per-reactor time 0 and step h0; method
+        "adaptive" (step doubling) or "rosenbrock" (RODAS3)."""
         import torch
-        self.th = torch.empty(2 * self.n, dtype=torch.float64, device="cuda")
+        if method not in ("adaptive", "rosenbrock"):
+            raise ValueError("method must be 'adaptive' or 'rosenbrock'")
+        self.method = method
+        self.th = torch.empty((3 if method == "rosenbrock" else 2) * self.n, dtype=torch.float64, device="cuda")
         self.remaining = torch.zeros(1, dtype=torch.int32, device="cuda")
-        check(lib().ocfd_reactor_adaptive_begin(self.n, _ptr(self.th), h0, _ptr(self.steps), _ptr(self.rejects),
-                                                _stream(stream)), "ocfd_reactor_adaptive_begin")
+        begin = lib().ocfd_reactor_rosenbrock_begin if method == "rosenbrock" else lib().ocfd_reactor_adaptive_begin
+        check(begin(self.n, _ptr(self.th), h0, _ptr(self.steps), _ptr(self.rejects), _stream(stream)),
+              "ocfd_reactor_" + method + "_begin")
 
     def chunk(self, t_end, budget=16, rtol=1e-4, atol=1e-9, max_steps=200000, stream=None):
         """At most `budget` substep attempts per reactor; self.remaining (device)
         then counts the reactors short of t_end."""
-        check(lib().ocfd_reactor_adaptive_chunk(self.n, _ptr(self.state), _ptr(self.th), t_end, rtol, atol,
-                                                max_steps, budget, _ptr(self.steps), _ptr(self.rejects),
-                                                _ptr(self.remaining), _stream(stream)), "ocfd_reactor_adaptive_chunk")
+        fn = lib().ocfd_reactor_rosenbrock_chunk if self.method == "rosenbrock" else lib().ocfd_reactor_adaptive_chunk
+        check(fn(self.n, _ptr(self.state), _ptr(self.th), t_end, rtol, atol, max_steps, budget, _ptr(self.steps),
+                 _ptr(self.rejects), _ptr(self.remaining), _stream(stream)), "ocfd_reactor_" + self.method + "_chunk")
 
     def temperature(self, stream=None):
         import torch

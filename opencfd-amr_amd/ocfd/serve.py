@@ -90,6 +90,7 @@ class ReactorRequest:
     rtol: float = 1e-4
     atol: float = 1e-9
     arrival: float = 0.0
+    method: str = "rosenbrock"     # or "adaptive" (step doubling on the linearised-implicit step)
 
     def footprint(self):
         return self.n * (REACTOR_NCOMP * 8 + 3 * 8 + 2 * 4) + MiB
@@ -222,7 +223,7 @@ class _ReactorJob:
         T0, p0, phi = ignition_batch(req.n, req.seed)
         with torch.cuda.stream(stream):
             self.r = Reactors(T0, p0, phi, stream)
-            self.r.begin_chunks(1e-8, stream)
+            self.r.begin_chunks(1e-8, stream, method=req.method)
             self.host = [torch.zeros(1, dtype=torch.int32).pin_memory() for _ in range(self.DEPTH)]
         self.groups = _Groups(torch, self.DEPTH)
         self.cells = 0

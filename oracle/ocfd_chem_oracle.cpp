@@ -611,9 +611,353 @@ int reactor_adaptive(double* u, double* rhoi, double t_end, double h0, double rt
     return acc;
 }
 
+// ---------------------------------------------------------------------------
+// Rosenbrock integration of one constant-volume reactor (no counterpart in the
+// reference, whose chemistry takes fixed linearised-implicit steps): the ODE
+// d rho_k/dt = W_k wdot_k(c, T) with T from the sensible energy E(rho) =
+// E0 - sum_k hf_k (rho_k - rho_k0) (the energy bookkeeping of
+// Source/CHEM/ChemAdvance.cpp:48-81), by RODAS3 (Sandu, Verwer, Blom,
+// Spee, Carmichael, Potra, Atmos. Environ. 31 (1997) 3459-3472): 4 stages,
+// order 3 with an embedded order-2 solution, L-stable and stiffly accurate,
+// in the transformed form
+//   (I/(h gamma) - J) K_i = f(y + sum_j a_ij K_j) + sum_j (c_ij/h) K_j,
+//   y1 = y + sum_i m_i K_i,  error estimate K_4.
+// J is the full Jacobian: the mass-action part at fixed T plus
+// (d f/d T)(d T/d rho), d f/d T by a forward difference of 1e-7 T and
+// d T/d rho_j = (-hf_j - e_j(T)) / cv from the energy relation.
+// ---------------------------------------------------------------------------
+const double kRosGamma = 0.5;
+
+// Troe fall-off factor P(Pr) = Pr/(1+Pr) F(Pr) of production_rate and dP/dPr
+void troe_and_slope(double redP, double Fc, double& P, double& dP)
+{
+    const double logPred = std::log10(redP);
+    const double logFcent = std::log10(Fc);
+    const double troe_c = -0.4 - 0.67 * logFcent;
+    const double troe_n = 0.75 - 1.27 * logFcent;
+    const double uu = troe_c + logPred;
+    const double den = troe_n - 0.14 * uu;
+    const double t = uu / den;
+    const double F = std::pow(10.0, logFcent / (1.0 + t * t));
+    P = (redP / (1.0 + redP)) * F;
+    const double q = 1.0 + t * t;
+    dP = F / ((1.0 + redP) * (1.0 + redP)) +
+         F / (1.0 + redP) * (-2.0 * t * logFcent * troe_n) / (q * q * den * den);
+}
+
+// The complete d wdot / d c (CGS) at fixed T: mass action by the product
+// rule (exact where a species is absent), the third-body efficiencies and
+// the Troe fall-off, with the rates of production_rate
+void jacobian_fixed_T(const double* sc, double T, double (*J)[NS])
+{
+    double tc[5];
+    tc[0] = std::log(T);
+    tc[1] = T;
+    tc[2] = T * (T);
+    tc[3] = tc[2] * tc[1];
+    tc[4] = tc[2] * tc[2];
+    const double invT = 1.0 / T;
+    double kfs[21];
+    for (int r = 0; r < 21; ++r) {
+        const Arr& a = kf_tab[r];
+        if (a.form == 0) kfs[r] = a.A * std::exp(a.beta * tc[0] - a.c * invT);
+        else if (a.form == 1) kfs[r] = a.A * std::exp(a.beta * tc[0]);
+        else if (a.form == 2) kfs[r] = a.A * std::exp(a.c * invT);
+        else kfs[r] = a.A;
+    }
+    double g[NS];
+    gibbs(g, tc);
+    const double RsT = Ru / Patm * 1e6 * T;
+    double Kc[21];
+    for (int r = 0; r < 21; ++r) {
+        const KcDef& d = kc_tab[r];
+        double sa = d.a1 < 0 ? g[d.a0] : g[d.a0] + g[d.a1];
+        double sb = d.b1 < 0 ? g[d.b0] : g[d.b0] + g[d.b1];
+        double e = std::exp((sa) - (sb));
+        Kc[r] = d.fac == 0 ? e : (d.fac == 1 ? 1.0 / RsT * e : RsT * e);
+    }
+    double mixture = 0.0;
+    for (int n = 0; n < NS; ++n) {
+        mixture += sc[n];
+        for (int l = 0; l < NS; ++l) J[n][l] = 0.0;
+    }
+    for (int m = 0; m < NR; ++m) {
+        const Reac& R = rx[m];
+        const double phi_f = prod(sc, R.f0, R.f1), phi_r = prod(sc, R.r0, R.r1);
+        double dpf[NS] = {0}, dpr[NS] = {0};
+        if (R.f1 < 0) dpf[R.f0] += 1.0;
+        else { dpf[R.f0] += sc[R.f1]; dpf[R.f1] += sc[R.f0]; }
+        if (R.r1 < 0) dpr[R.r0] += 1.0;
+        else { dpr[R.r0] += sc[R.r1]; dpr[R.r1] += sc[R.r0]; }
+        const double k0 = kfs[R.kf];
+        double keff = k0, dk[NS] = {0};
+        if (R.tb != 0) {
+            double eff[NS];
+            for (int n = 0; n < NS; ++n) eff[n] = 1.0;
+            double alpha;
+            if (R.tb == 2) {
+                eff[0] += 1.0; eff[2] += 10.0; eff[1] += -0.22;
+                alpha = mixture + sc[0] + 10 * sc[2] + -0.22 * sc[1];
+            } else {
+                eff[0] += 1.5; eff[2] += 11.0;
+                alpha = mixture + 1.5 * sc[0] + 11 * sc[2];
+            }
+            if (R.tb == 1) {
+                keff = alpha * k0;
+                for (int n = 0; n < NS; ++n) dk[n] = eff[n] * k0;
+            } else {
+                const double kl = R.tb == 2 ? 6.37e+20 * std::exp(-1.72 * tc[0] - 261.6741574193582 * invT)
+                                            : 1.20e+17 * std::exp(-22846.166820843966 * invT);
+                const double redP = alpha / k0 * kl;
+                double P, dP;
+                troe_and_slope(redP, R.tb == 2 ? 0.8 : 0.5, P, dP);
+                keff = k0 * P;
+                for (int n = 0; n < NS; ++n) dk[n] = dP * kl * eff[n];
+            }
+        }
+        const double diff = phi_f - phi_r / Kc[R.kf];
+        double dq[NS];
+        for (int n = 0; n < NS; ++n) dq[n] = keff * (dpf[n] - dpr[n] / Kc[R.kf]) + dk[n] * diff;
+        if (R.dup >= 0) {
+            const double k2 = kfs[R.dup];
+            for (int n = 0; n < NS; ++n) dq[n] += k2 * (dpf[n] - dpr[n] / Kc[R.dup]);
+        }
+        for (int l = 0; l < NS; ++l) {
+            const int nu = R.nur[l] - R.nuf[l];
+            if (nu == 0) continue;
+            for (int n = 0; n < NS; ++n) J[l][n] += nu * dq[n];
+        }
+    }
+}
+
+// Newton inversion of the sensible energy from a temperature guess (the
+// iteration of T_given_ey), T_given_ey itself when the guess or an iterate
+// leaves its table range
+double T_newton(double e, const double* rhoi, double Tg)
+{
+    const double tol = 1e-7;
+    if (!(Tg >= 250.0 && Tg <= 3500.0)) return T_given_ey(e, rhoi, nullptr);
+    double t1 = Tg;
+    for (int i = 0; i < 50; ++i) {
+        double e1 = CKUBMS(t1, rhoi);
+        double cv = CKCVBS(t1, rhoi);
+        double dt = (e - e1) / cv;
+        if (dt > 100.0) dt = 100.0;
+        else if (dt < -100.0) dt = -100.0;
+        else if (std::fabs(dt) < tol) return t1;
+        else if (t1 + dt == t1) return t1;
+        t1 += dt;
+        if (t1 < 250.0 || t1 > 3500.0) return T_given_ey(e, rhoi, nullptr);
+    }
+    return t1;
+}
+
+// f = d rho/dt (kg/m^3/s) at T (production_rate); J = d f/d rho at fixed T
+// (jacobian_fixed_T) when not null
+void reactor_f(const double* y, double T, double* f, double (*J)[NS])
+{
+    double c[NS], wdot[NS], Ar[NS][NS];
+    for (int n = 0; n < NS; ++n) c[n] = y[n] / mw[n] * 1e-6;
+    production_rate(wdot, Ar, c, T);
+    for (int i = 0; i < NS; ++i) f[i] = wdot[i] * mw[i] * 1e6;
+    if (J) {
+        double Jc[NS][NS];
+        jacobian_fixed_T(c, T, Jc);
+        for (int i = 0; i < NS; ++i)
+            for (int j = 0; j < NS; ++j) J[i][j] = Jc[i][j] * mw[i] / mw[j];
+    }
+}
+
+// LU with partial pivoting (first maximum), row exchanges in piv.  A row
+// exchange at step k moves columns k.. only: the multipliers of the earlier
+// columns stay with the rows they were applied to, which is the order in
+// which lu_solve replays exchanges and eliminations.
+void lu_factor(double (*M)[NS], int* piv)
+{
+    for (int k = 0; k < NS; ++k) {
+        int p = k;
+        double mx = 0.0;
+        for (int i = k; i < NS; ++i)
+            if (std::fabs(M[i][k]) > mx) { mx = std::fabs(M[i][k]); p = i; }
+        piv[k] = p;
+        if (p != k)
+            for (int j = k; j < NS; ++j) std::swap(M[p][j], M[k][j]);
+        for (int i = k + 1; i < NS; ++i) {
+            const double m = M[i][k] / M[k][k];
+            M[i][k] = m;
+            for (int j = k + 1; j < NS; ++j) M[i][j] -= m * M[k][j];
+        }
+    }
+}
+
+void lu_solve(const double (*M)[NS], const int* piv, double* b)
+{
+    for (int k = 0; k < NS; ++k) {
+        if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+        for (int i = k + 1; i < NS; ++i) b[i] -= M[i][k] * b[k];
+    }
+    for (int k = NS - 1; k >= 0; --k) {
+        double sum = b[k];
+        for (int j = k + 1; j < NS; ++j) sum -= M[k][j] * b[j];
+        b[k] = sum / M[k][k];
+    }
+}
+
+// energy change of a change dy of the partial densities (formation enthalpies)
+double energy_of(const double* dy)
+{
+    double de = 0.0;
+    for (int n = 0; n < NS; ++n) de += -HP(n) * Ru / mw[n] * dy[n];
+    return de;
+}
+
+// One RODAS3 step of h from (y, E) with temperature guess Tg: the new state
+// in ynew/Enew, the temperature of the last stage in Tlast; returns the
+// error max_k |K4_k| / (atol rho + rtol max(|y_k|, |ynew_k|)).
+double rodas3_step(const double* y, double E, double Tg, double h, double rho, double rtol, double atol, double* ynew,
+                   double& Enew, double& Tlast)
+{
+    const double T0 = T_newton(E, y, Tg);
+    double f0[NS], J[NS][NS], f1[NS];
+    reactor_f(y, T0, f0, J);
+    const double dT = 1e-7 * T0;
+    reactor_f(y, T0 + dT, f1, nullptr);
+    const double tc[5] = {0.0, T0, T0 * T0, T0 * T0 * T0, T0 * T0 * T0 * T0};
+    double ue[NS];
+    species_internal_energy(ue, tc);
+    const double cv = CKCVBS(T0, y);
+    double dTdy[NS], dfdT[NS];
+    for (int j = 0; j < NS; ++j) dTdy[j] = (-HP(j) * Ru / mw[j] - ue[j] * Ru * T0 / mw[j]) / cv;
+    for (int i = 0; i < NS; ++i) dfdT[i] = (f1[i] - f0[i]) / dT;
+    double M[NS][NS];
+    const double gh = 1.0 / (h * kRosGamma);
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) M[i][j] = -(J[i][j] + dfdT[i] * dTdy[j]) + (i == j ? gh : 0.0);
+    int piv[NS];
+    lu_factor(M, piv);
+    const double hinv = 1.0 / h;
+    double K1[NS], K2[NS], K3[NS], K4[NS], ys[NS], fs[NS], dy[NS];
+    for (int n = 0; n < NS; ++n) K1[n] = f0[n];
+    lu_solve(M, piv, K1);
+    for (int n = 0; n < NS; ++n) K2[n] = f0[n] + (4.0 * hinv) * K1[n];
+    lu_solve(M, piv, K2);
+    // stage 3 at y + 2 K1
+    for (int n = 0; n < NS; ++n) { dy[n] = 2.0 * K1[n]; ys[n] = y[n] + dy[n]; }
+    const double T3 = T_newton(E + energy_of(dy), ys, T0);
+    reactor_f(ys, T3, fs, nullptr);
+    for (int n = 0; n < NS; ++n) K3[n] = fs[n] + hinv * K1[n] - hinv * K2[n];
+    lu_solve(M, piv, K3);
+    // stage 4 at y + 2 K1 + K3
+    for (int n = 0; n < NS; ++n) { dy[n] = 2.0 * K1[n] + K3[n]; ys[n] = y[n] + dy[n]; }
+    const double T4 = T_newton(E + energy_of(dy), ys, T3);
+    reactor_f(ys, T4, fs, nullptr);
+    for (int n = 0; n < NS; ++n) K4[n] = fs[n] + hinv * K1[n] - hinv * K2[n] - (8.0 / 3.0 * hinv) * K3[n];
+    lu_solve(M, piv, K4);
+    double err = 0.0;
+    bool finite = true;
+    for (int n = 0; n < NS; ++n) {
+        dy[n] = 2.0 * K1[n] + K3[n] + K4[n];
+        ynew[n] = y[n] + dy[n];
+        const double e = std::fabs(K4[n]) / (atol * rho + rtol * std::max(std::fabs(y[n]), std::fabs(ynew[n])));
+        finite = finite && std::isfinite(e) && std::isfinite(ynew[n]);
+        err = std::max(err, e);
+    }
+    Enew = E + energy_of(dy);
+    Tlast = T4;
+    // a stage that left the thermodynamic tables (a non-physical temperature,
+    // non-finite rates) fails the step: it is retried with a smaller h
+    return finite ? err : HUGE_VAL;
+}
+
+// Adaptive RODAS3 to t_end: on acceptance the densities are clipped at 0 (the
+// energy follows the clipped change), h <- h clamp(0.9 err^(-1/3), 0.2, 6), not
+// above h after a rejection; on rejection h <- h max(0.2, 0.9 err^(-1/3)).
+// Returns accepted steps, or -1 when max_steps attempts did not reach t_end.
+int reactor_rosenbrock(double* u, double* rhoi, double t_end, double h0, double rtol, double atol, int max_steps,
+                       int* rejects)
+{
+    double t = 0.0, h = h0, E = u[4], Tg = 0.0;
+    int acc = 0, rej = 0;
+    bool last_rej = false;
+    const double rho = u[0];
+    while (t < t_end) {
+        if (acc + rej >= max_steps) { if (rejects) *rejects = rej; return -1; }
+        if (t + h > t_end) h = t_end - t;
+        double yn[NS], En, Tl;
+        const double err = rodas3_step(rhoi, E, Tg, h, rho, rtol, atol, yn, En, Tl);
+        const double fac = 0.9 / std::cbrt(err);
+        if (err <= 1.0) {
+            double dy[NS];
+            for (int n = 0; n < NS; ++n) {
+                const double v = std::max(yn[n], 0.0);
+                dy[n] = v - rhoi[n];
+                rhoi[n] = v;
+            }
+            E += energy_of(dy);
+            Tg = Tl;
+            t += h;
+            ++acc;
+            double f = std::min(6.0, std::max(0.2, fac));
+            if (last_rej) f = std::min(f, 1.0);
+            h *= f;
+            last_rej = false;
+        } else {
+            ++rej;
+            h *= std::max(0.2, fac);
+            last_rej = true;
+        }
+    }
+    u[4] = E;
+    if (rejects) *rejects = rej;
+    return acc;
+}
+
 } // namespace
 
 extern "C" {
+
+// Rosenbrock (RODAS3) integration to t_end; steps[m] = accepted steps (-1 on failure)
+void orc_reactor_rosenbrock(long n, double* u, double* rhoi, double t_end, double h0, double rtol, double atol,
+                            int max_steps, int* steps, int* rejects)
+{
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long m = 0; m < n; ++m)
+        steps[m] = reactor_rosenbrock(u + 5 * m, rhoi + NS * m, t_end, h0, rtol, atol, max_steps,
+                                      rejects ? rejects + m : nullptr);
+}
+
+// nsteps RODAS3 steps of h without error control (order checks); u, rhoi in place
+void orc_rodas3_fixed(double* u, double* rhoi, double h, int nsteps)
+{
+    double E = u[4], Tg = 0.0;
+    for (int s = 0; s < nsteps; ++s) {
+        double yn[NS], En, Tl;
+        rodas3_step(rhoi, E, Tg, h, u[0], 1e-4, 1e-9, yn, En, Tl);
+        std::memcpy(rhoi, yn, sizeof(yn));
+        E = En;
+        Tg = Tl;
+    }
+    u[4] = E;
+}
+
+// the complete d wdot/d c at fixed T (CGS), row-major 9x9
+void orc_jacobian_fixed_T(const double* c, double T, double* J)
+{
+    double Jc[NS][NS];
+    jacobian_fixed_T(c, T, Jc);
+    std::memcpy(J, Jc, sizeof(Jc));
+}
+
+// the reactor ODE's right-hand side at densities y, the energy following from
+// (E0, y0): f = d rho/dt; *T = the temperature at y
+void orc_reactor_rhs(double E0, const double* y0, const double* y, double* f, double* T)
+{
+    double dy[NS];
+    for (int n = 0; n < NS; ++n) dy[n] = y[n] - y0[n];
+    const double Ty = T_given_ey(E0 + energy_of(dy), y, nullptr);
+    reactor_f(y, Ty, f, nullptr);
+    if (T) *T = Ty;
+}
 
 void orc_chem_tables(double* mw_out, double* lo_out, double* hi_out)
 {

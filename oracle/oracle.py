@@ -403,6 +403,11 @@ def _chem_sigs():
     L.orc_reactor_adaptive.argtypes = [ctypes.c_long, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_int, ip, ip]
     L.orc_reactor_T.argtypes = [ctypes.c_long, dp, dp, dp]
+    L.orc_reactor_rosenbrock.argtypes = [ctypes.c_long, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int, ip, ip]
+    L.orc_rodas3_fixed.argtypes = [dp, dp, ctypes.c_double, ctypes.c_int]
+    L.orc_reactor_rhs.argtypes = [ctypes.c_double, dp, dp, dp, dp]
+    L.orc_jacobian_fixed_T.argtypes = [dp, ctypes.c_double, dp]
     L._chem_ready = True
     return L
 
@@ -488,6 +493,39 @@ def reactor_adaptive(u, r, t_end, h0=1e-8, rtol=1e-6, atol=1e-12, max_steps=2000
                                       steps.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                       rej.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     return u, r, steps, rej
+
+
+def reactor_rosenbrock(u, r, t_end, h0=1e-8, rtol=1e-4, atol=1e-9, max_steps=200000):
+    """RODAS3 integration of each reactor to t_end (u, r copied); returns
+    (u, r, accepted steps, rejected steps)."""
+    u = _c(u).copy(); r = _c(r).copy()
+    n = len(u)
+    steps = np.zeros(n, dtype=np.int32); rej = np.zeros(n, dtype=np.int32)
+    _chem_sigs().orc_reactor_rosenbrock(n, _dp(u), _dp(r), t_end, h0, rtol, atol, max_steps,
+                                        steps.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                        rej.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return u, r, steps, rej
+
+
+def rodas3_fixed(u1, r1, h, nsteps):
+    """nsteps RODAS3 steps of h of one reactor (u1: 5, r1: 9), no error control."""
+    u = _c(u1).copy(); r = _c(r1).copy()
+    _chem_sigs().orc_rodas3_fixed(_dp(u), _dp(r), h, nsteps)
+    return u, r
+
+
+def reactor_rhs(E0, y0, y):
+    """d rho/dt of the reactor ODE at densities y (energy from E0 at y0) and the temperature there."""
+    y0 = _c(y0); y = _c(y); f = np.zeros(9); T = np.zeros(1)
+    _chem_sigs().orc_reactor_rhs(E0, _dp(y0), _dp(y), _dp(f), _dp(T))
+    return f, T[0]
+
+
+def jacobian_fixed_T(c, T):
+    """The complete d wdot/d c (CGS) at fixed T."""
+    c = _c(c); J = np.zeros(81)
+    _chem_sigs().orc_jacobian_fixed_T(_dp(c), T, _dp(J))
+    return J.reshape(9, 9)
 
 
 def reactor_T(u, r):

@@ -180,3 +180,54 @@ def test_reactor_chunks_equal_one_launch(torch_cuda):
     assert np.array_equal(one.rows().cpu().numpy(), ch.rows().cpu().numpy())
     assert np.array_equal(one.steps.cpu().numpy(), ch.steps.cpu().numpy())
     assert np.array_equal(one.rejects.cpu().numpy(), ch.rejects.cpu().numpy())
+
+
+def test_reactor_rosenbrock_matches_oracle(torch_cuda, orc):
+    """The RODAS3 reactor kernel against the oracle's identical integrator:
+    temperatures at 1 ms within the tolerance scale (the device libm can move
+    a step decision), most step counts equal, and at most 500 accepted steps
+    per reactor on average."""
+    import ocfd
+    T0, p0, phi = ocfd.ignition_batch(512, seed=9)
+    R = ocfd.Reactors(T0, p0, phi)
+    R.advance_rosenbrock(1e-3, h0=1e-8, rtol=1e-4, atol=1e-9)
+    Tg = R.temperature().cpu().numpy()
+    steps = R.steps.cpu().numpy()
+    u, r = _oracle_state(orc, T0, p0, phi)
+    u, r, so, ro = orc.reactor_rosenbrock(u, r, 1e-3, h0=1e-8, rtol=1e-4, atol=1e-9)
+    To = orc.reactor_T(u, r)
+    assert np.all(steps > 0) and np.all(so > 0)
+    rel = np.abs(Tg - To) / To
+    print("rosenbrock T rel diff: median", np.median(rel), "max", rel.max(), "same step counts",
+          np.mean(steps == so), "mean steps", steps.mean())
+    assert np.median(rel) < 1e-8 and rel.max() < 1e-4
+    assert np.mean(steps == so) > 0.9
+    assert steps.mean() <= 500
+    # a reactor whose trial step leaves the thermodynamic tables: rejected
+    # and retried, the integration ends finite
+    one = ocfd.Reactors([1060.5603219620868], [108796.87196404164], [0.5295738294307426])
+    one.advance_rosenbrock(1e-3, h0=1e-8, rtol=1e-4, atol=1e-9)
+    assert int(one.steps[0]) > 0 and int(one.rejects[0]) >= 1
+    assert np.isfinite(one.rows().cpu().numpy()).all()
+
+
+def test_rosenbrock_chunks_equal_one_launch(torch_cuda):
+    """The resumable RODAS3 integration gives the one-launch states, step and
+    rejection counts bit for bit."""
+    import ocfd
+    T0, p0, phi = ocfd.ignition_batch(300, seed=23)
+    one = ocfd.Reactors(T0, p0, phi)
+    one.advance_rosenbrock(5e-4, h0=1e-8, rtol=1e-4, atol=1e-9)
+    ch = ocfd.Reactors(T0, p0, phi)
+    ch.begin_chunks(1e-8, method="rosenbrock")
+    launches = 0
+    while True:
+        ch.chunk(5e-4, budget=5, rtol=1e-4, atol=1e-9)
+        launches += 1
+        if int(ch.remaining.item()) == 0:
+            break
+        assert launches < 100000
+    assert launches > 1
+    assert np.array_equal(one.rows().cpu().numpy(), ch.rows().cpu().numpy())
+    assert np.array_equal(one.steps.cpu().numpy(), ch.steps.cpu().numpy())
+    assert np.array_equal(one.rejects.cpu().numpy(), ch.rejects.cpu().numpy())
