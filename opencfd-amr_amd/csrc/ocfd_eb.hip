@@ -75,51 +75,50 @@ __global__ __launch_bounds__(256) void k_eb_prims(const BoxDesc* __restrict__ bo
     q[6 * st] = sqrt(ph.gamma * ph.Rg * T);
 }
 
-// Van Leer flux vector splitting, Source/EB/EBkernels.H:73-171 (non-CHEM).
-// l/r: rho, u, v, w, p, c of the left and right states.
+// One side of Van Leer's flux-vector splitting (B. van Leer, Flux-vector
+// splitting for the Euler equations, Lecture Notes in Physics 170 (1982)
+// 507-512) along direction D, for a state q = (rho, u, v, w, p, c): s = +1
+// gives the forward part F+ of the left state, s = -1 the backward part F- of
+// the right state.  Supersonic towards the part's side: the whole Euler
+// flux; subsonic: the split mass flux s rho c (1 + s M)^2 / 4 carrying the
+// normal momentum ((gamma-1) u_n + 2 s c)/gamma, the tangential velocities
+// and the energy ((gamma-1) u_n + 2 s c)^2 / (2 (gamma^2-1)) + |u_t|^2 / 2;
+// otherwise nothing.  Used as the cut-cell face flux of
+// Source/EB/EBkernels.H:73-171 (non-CHEM build, gamma constant).
+template <int D>
+__device__ __forceinline__ void vl_part(const double* q, double sgn, double gamma, double smallp, double* f)
+{
+    const double rho = q[0], p = fmax(q[4], smallp), c = q[5];
+    const double un = q[1 + D];
+    const double M = un / c;
+    const double uu = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (sgn * M >= 1.0) {
+        const double m = rho * un;
+        f[0] = m;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) f[1 + a] = m * q[1 + a] + (a == D ? p : 0.0);
+        f[4] = un * (gamma * p / (gamma - 1.0) + 0.5 * rho * uu);
+    } else if (fabs(M) < 1.0) {
+        const double w = 1.0 + sgn * M;
+        const double m = rho * c * (sgn * 0.250 * w * w);
+        const double a2 = (gamma - 1) * un + sgn * 2.0 * c;
+        f[0] = m;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) f[1 + a] = a == D ? m * ((-un + sgn * 2.0 * c) / gamma + un) : m * q[1 + a];
+        f[4] = m * (a2 * a2 / (2 * (gamma * gamma - 1)) + (uu - un * un) / 2);
+    } else {
+#pragma unroll
+        for (int n = 0; n < NC; ++n) f[n] = 0.0;
+    }
+}
+
+// the face flux F = F+(left) + F-(right)
 template <int D>
 __device__ __forceinline__ void vl_flux(const double* l, const double* r, double gamma, double smallp, double* F)
 {
-    const double dl = l[0], ul = l[1], vl = l[2], wl = l[3], pl = fmax(l[4], smallp);
-    const double dr = r[0], ur = r[1], vr = r[2], wr = r[3], pr = fmax(r[4], smallp);
-    const double cL = l[5], cR = r[5];
-    const double UL = D == 0 ? ul : (D == 1 ? vl : wl);
-    const double UR = D == 0 ? ur : (D == 1 ? vr : wr);
-    const double ML = UL / cL, MR = UR / cR;
-    const int nx = D == 0, ny = D == 1, nz = D == 2;
-    double fp[NC] = {0, 0, 0, 0, 0}, fm[NC] = {0, 0, 0, 0, 0};
-    if (ML >= 1.0) {
-        fp[0] = dl * UL;
-        fp[1] = dl * UL * ul + nx * pl;
-        fp[2] = dl * UL * vl + ny * pl;
-        fp[3] = dl * UL * wl + nz * pl;
-        fp[4] = UL * (gamma * pl / (gamma - 1.0) + 0.5 * dl * (ul * ul + vl * vl + wl * wl));
-    } else if (fabs(ML) < 1.0) {
-        const double Mp = 0.250 * (1.0 + ML) * (1.0 + ML);
-        const double tmp0 = dl * cL * Mp;
-        fp[0] = tmp0;
-        fp[1] = tmp0 * (nx * (-UL + 2.0 * cL) / gamma + ul);
-        fp[2] = tmp0 * (ny * (-UL + 2.0 * cL) / gamma + vl);
-        fp[3] = tmp0 * (nz * (-UL + 2.0 * cL) / gamma + wl);
-        fp[4] = tmp0 * (((gamma - 1) * UL + 2 * cL) * ((gamma - 1) * UL + 2 * cL) / (2 * (gamma * gamma - 1)) +
-                        (ul * ul + vl * vl + wl * wl - UL * UL) / 2);
-    }
-    if (fabs(MR) < 1.0) {
-        const double Mm = -0.250 * (MR - 1.0) * (MR - 1.0);
-        const double tmp0 = dr * cR * Mm;
-        fm[0] = tmp0;
-        fm[1] = tmp0 * (nx * (-UR - 2.0 * cR) / gamma + ur);
-        fm[2] = tmp0 * (ny * (-UR - 2.0 * cR) / gamma + vr);
-        fm[3] = tmp0 * (nz * (-UR - 2.0 * cR) / gamma + wr);
-        fm[4] = tmp0 * (((gamma - 1) * UR - 2 * cR) * ((gamma - 1) * UR - 2 * cR) / (2 * (gamma * gamma - 1)) +
-                        (ur * ur + vr * vr + wr * wr - UR * UR) / 2);
-    } else if (MR <= -1.0) {
-        fm[0] = dr * UR;
-        fm[1] = dr * UR * ur + nx * pr;
-        fm[2] = dr * UR * vr + ny * pr;
-        fm[3] = dr * UR * wr + nz * pr;
-        fm[4] = UR * (gamma * pr / (gamma - 1.0) + 0.5 * dr * (ur * ur + vr * vr + wr * wr));
-    }
+    double fp[NC], fm[NC];
+    vl_part<D>(l, 1.0, gamma, smallp, fp);
+    vl_part<D>(r, -1.0, gamma, smallp, fm);
 #pragma unroll
     for (int n = 0; n < NC; ++n) F[n] = fp[n] + fm[n];
 }

@@ -547,8 +547,25 @@ __device__ __forceinline__ void mrecon(const double* x, long l0, long ds, bool c
     }
 }
 
-// species face flux of a cut box: MUSCL states of u_D, c and rho_k, Van Leer
-// mass-flux split (FlowAdvanceBox.cpp:174-223), Fickian flux when viscous
+// The mass flux of Van Leer's splitting (B. van Leer, Lecture Notes in
+// Physics 170 (1982) 507-512) for a partial density r moving with normal
+// velocity U at sound speed c: sgn = +1 the forward part, -1 the backward
+// part (the whole flux r U when supersonic towards the side, the split
+// r c sgn (1 + sgn M)^2 / 4 when subsonic, else nothing)
+__device__ __forceinline__ double vl_mass_part(double r, double U, double c, double sgn)
+{
+    const double M = U / c;
+    if (sgn * M >= 1.0) return r * U;
+    if (fabs(M) < 1.0) {
+        const double w = 1.0 + sgn * M;
+        return r * c * (sgn * 0.250 * w * w);
+    }
+    return 0.0;
+}
+
+// species face flux of a cut box: MUSCL states of u_D, c and rho_k, the Van
+// Leer mass-flux split of each partial density (the species flux of
+// FlowAdvanceBox.cpp:174-223), Fickian flux when viscous
 template <int D>
 __global__ __launch_bounds__(256) void k_ebchem_face(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                     MultiArgs m, const uint32_t* __restrict__ ebflag)
@@ -571,18 +588,12 @@ __global__ __launch_bounds__(256) void k_ebchem_face(const BoxDesc* __restrict__
         double ul, ur, cL, cR;
         mrecon(q + (MQ_U + D) * st, l0, ds, cm, cm2, cp, ul, ur);
         mrecon(q + MQ_C * st, l0, ds, cm, cm2, cp, cL, cR);
-        const double ML = ul / cL, MR = ur / cR, UL = ul, UR = ur;
         double F[NS];
 #pragma unroll
         for (int n = 0; n < NS; ++n) {
             double rl, rr;
             mrecon(P + n * st, l0, ds, cm, cm2, cp, rl, rr);
-            double fp = 0.0, fm = 0.0;
-            if (ML >= 1.0) fp = rl * UL;
-            else if (fabs(ML) < 1.0) { const double Mp = 0.250 * (1.0 + ML) * (1.0 + ML); fp = rl * cL * Mp; }
-            if (fabs(MR) < 1.0) { const double Mm = -0.250 * (MR - 1.0) * (MR - 1.0); fm = rr * cR * Mm; }
-            else if (MR <= -1.0) fm = rr * UR;
-            F[n] = fp + fm;
+            F[n] = vl_mass_part(rl, ul, cL, 1.0) + vl_mass_part(rr, ur, cR, -1.0);
         }
         if (m.a.flags & OCFD_VISC) {
             const long lm = l0 - ds;

@@ -474,51 +474,46 @@ inline void eb_recon(int d, int i, int j, int k, int n, const View& q, const EBV
     }
 }
 
-// Van Leer flux vector splitting, Source/EB/EBkernels.H:73-171 (non-CHEM)
+// One side of Van Leer's flux-vector splitting (B. van Leer, Flux-vector
+// splitting for the Euler equations, Lecture Notes in Physics 170 (1982)
+// 507-512) along direction cdir for q = (rho, u, v, w, p, c, ...): sgn = +1
+// the forward part of the left state, -1 the backward part of the right one.
+// Supersonic towards the part's side: the whole Euler flux; subsonic: the
+// split mass flux sgn rho c (1 + sgn M)^2 / 4 carrying the normal momentum
+// ((gamma-1) u_n + 2 sgn c)/gamma, the tangential velocities and the energy
+// ((gamma-1) u_n + 2 sgn c)^2 / (2 (gamma^2-1)) + |u_t|^2 / 2; else nothing.
+inline void vl_part(int cdir, const double* q, double sgn, double* f, const orc_parm& parm)
+{
+    const double rho = q[QRHO], p = std::max(q[QPRES], parm.smallp), c = q[QC];
+    const double gamma = parm.eos_gamma;
+    const double vel[3] = {q[QU], q[QV], q[QW]};
+    const double un = vel[cdir];
+    const double M = un / c;
+    const double uu = vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2];
+    if (sgn * M >= 1.0) {
+        const double m = rho * un;
+        f[0] = m;
+        for (int a = 0; a < 3; ++a) f[1 + a] = m * vel[a] + (a == cdir ? p : 0.0);
+        f[4] = un * (gamma * p / (gamma - 1.0) + 0.5 * rho * uu);
+    } else if (std::fabs(M) < 1.0) {
+        const double w = 1.0 + sgn * M;
+        const double m = rho * c * (sgn * 0.250 * w * w);
+        const double a2 = (gamma - 1) * un + sgn * 2.0 * c;
+        f[0] = m;
+        for (int a = 0; a < 3; ++a) f[1 + a] = a == cdir ? m * ((-un + sgn * 2.0 * c) / gamma + un) : m * vel[a];
+        f[4] = m * (a2 * a2 / (2 * (gamma * gamma - 1)) + (uu - un * un) / 2);
+    } else {
+        for (int n = 0; n < NCONS; ++n) f[n] = 0.0;
+    }
+}
+
+// the cut-cell face flux of Source/EB/EBkernels.H:73-171 (non-CHEM build):
+// F = F+(left) + F-(right)
 inline void eb_flux_point(int cdir, const double* ql, const double* qr, double* F, const orc_parm& parm)
 {
-    double dl = ql[QRHO], ul = ql[QU], vl = ql[QV], wl = ql[QW], pl = std::max(ql[QPRES], parm.smallp);
-    double dr = qr[QRHO], ur = qr[QU], vr = qr[QV], wr = qr[QW], pr = std::max(qr[QPRES], parm.smallp);
-    double gamma = parm.eos_gamma;
-    double cL = ql[QC], cR = qr[QC];
-    double fp[NCONS] = {0, 0, 0, 0, 0}, fm[NCONS] = {0, 0, 0, 0, 0};
-    double ML, MR, UL, UR;
-    int nx = 0, ny = 0, nz = 0;
-    if (cdir == 0) { ML = ul / cL; MR = ur / cR; UL = ul; UR = ur; nx = 1; }
-    else if (cdir == 1) { ML = vl / cL; MR = vr / cR; UL = vl; UR = vr; ny = 1; }
-    else { ML = wl / cL; MR = wr / cR; UL = wl; UR = wr; nz = 1; }
-    if (ML >= 1.0) {
-        fp[0] = dl * UL;
-        fp[1] = dl * UL * ul + nx * pl;
-        fp[2] = dl * UL * vl + ny * pl;
-        fp[3] = dl * UL * wl + nz * pl;
-        fp[4] = UL * (gamma * pl / (gamma - 1.0) + 0.5 * dl * (ul * ul + vl * vl + wl * wl));
-    } else if (std::fabs(ML) < 1.0) {
-        double Mp = 0.250 * (1.0 + ML) * (1.0 + ML);
-        double tmp0 = dl * cL * Mp;
-        fp[0] = tmp0;
-        fp[1] = tmp0 * (nx * (-UL + 2.0 * cL) / gamma + ul);
-        fp[2] = tmp0 * (ny * (-UL + 2.0 * cL) / gamma + vl);
-        fp[3] = tmp0 * (nz * (-UL + 2.0 * cL) / gamma + wl);
-        fp[4] = tmp0 * (((gamma - 1) * UL + 2 * cL) * ((gamma - 1) * UL + 2 * cL) / (2 * (gamma * gamma - 1))
-                        + (ul * ul + vl * vl + wl * wl - UL * UL) / 2);
-    }
-    if (std::fabs(MR) < 1.0) {
-        double Mm = -0.250 * (MR - 1.0) * (MR - 1.0);
-        double tmp0 = dr * cR * Mm;
-        fm[0] = tmp0;
-        fm[1] = tmp0 * (nx * (-UR - 2.0 * cR) / gamma + ur);
-        fm[2] = tmp0 * (ny * (-UR - 2.0 * cR) / gamma + vr);
-        fm[3] = tmp0 * (nz * (-UR - 2.0 * cR) / gamma + wr);
-        fm[4] = tmp0 * (((gamma - 1) * UR - 2 * cR) * ((gamma - 1) * UR - 2 * cR) / (2 * (gamma * gamma - 1))
-                        + (ur * ur + vr * vr + wr * wr - UR * UR) / 2);
-    } else if (MR <= -1.0) {
-        fm[0] = dr * UR;
-        fm[1] = dr * UR * ur + nx * pr;
-        fm[2] = dr * UR * vr + ny * pr;
-        fm[3] = dr * UR * wr + nz * pr;
-        fm[4] = UR * (gamma * pr / (gamma - 1.0) + 0.5 * dr * (ur * ur + vr * vr + wr * wr));
-    }
+    double fp[NCONS], fm[NCONS];
+    vl_part(cdir, ql, 1.0, fp, parm);
+    vl_part(cdir, qr, -1.0, fm, parm);
     for (int n = 0; n < NCONS; ++n) F[n] = fp[n] + fm[n];
 }
 
@@ -1573,23 +1568,24 @@ inline void diffusion_face(int d, int i, int j, int k, const View& q, const View
 // Source/CHEM/FlowAdvanceBox.cpp:174-223: normal velocities ul/ur and sound
 // speeds cL/cR of the reconstructed left/right states, partial densities
 // rl/rr; F = f+(left) + f-(right) per species.
+// The mass flux of Van Leer's splitting for a partial density r moving with
+// normal velocity U at sound speed c: sgn = +1 the forward part, -1 the
+// backward part (the whole flux r U when supersonic towards the side, the
+// split r c sgn (1 + sgn M)^2 / 4 when subsonic, else nothing)
+inline double vl_mass_part(double r, double U, double c, double sgn)
+{
+    const double M = U / c;
+    if (sgn * M >= 1.0) return r * U;
+    if (std::fabs(M) < 1.0) {
+        const double w = 1.0 + sgn * M;
+        return r * c * (sgn * 0.250 * w * w);
+    }
+    return 0.0;
+}
+
 inline void vl_species_flux(double ul, double ur, double cL, double cR, const double* rl, const double* rr, double* F)
 {
-    double fp[NSP] = {0}, fm[NSP] = {0};
-    double ML = ul / cL, MR = ur / cR, UL = ul, UR = ur;
-    if (ML >= 1.0) {
-        for (int n = 0; n < NSP; ++n) fp[n] = rl[n] * UL;
-    } else if (std::fabs(ML) < 1.0) {
-        double Mp = 0.250 * (1.0 + ML) * (1.0 + ML);
-        for (int n = 0; n < NSP; ++n) fp[n] = rl[n] * cL * Mp;
-    }
-    if (std::fabs(MR) < 1.0) {
-        double Mm = -0.250 * (MR - 1.0) * (MR - 1.0);
-        for (int n = 0; n < NSP; ++n) fm[n] = rr[n] * cR * Mm;
-    } else if (MR <= -1.0) {
-        for (int n = 0; n < NSP; ++n) fm[n] = rr[n] * UR;
-    }
-    for (int n = 0; n < NSP; ++n) F[n] = fp[n] + fm[n];
+    for (int n = 0; n < NSP; ++n) F[n] = vl_mass_part(rl[n], ul, cL, 1.0) + vl_mass_part(rr[n], ur, cR, -1.0);
 }
 
 // EBR::ebchem_compute_dSdt_box, Source/CHEM/FlowAdvanceBox.cpp:16-525: MUSCL
