@@ -610,6 +610,16 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
     // neighbour of one box that owns a face is face-adjacent to another box).
     // A side on a non-periodic domain boundary has none, nor does a side of a
     // box spanning a periodic direction (its image is the box itself).
+    // candidates as 64-bit keys (coarse box, K, J, I), 16 bits each relative to
+    // the domain's low corner, so that sorting the keys orders them as the
+    // tuples; a domain or box count beyond 16 bits falls back to the tuples
+    bool narrow = C.nbox < (1 << 16);
+    for (int d = 0; d < 3; ++d) narrow = narrow && CG.dom_hi[d] - CG.dom_lo[d] < (1 << 16);
+    std::vector<uint64_t> ckey;
+    auto key_of = [&](int b, int K, int J, int I) {
+        return ((uint64_t)b << 48) | ((uint64_t)(K - CG.dom_lo[2]) << 32) | ((uint64_t)(J - CG.dom_lo[1]) << 16) |
+               (uint64_t)(I - CG.dom_lo[0]);
+    };
     std::vector<std::array<int, 4>> cand;
     for (const BoxDesc& B : fcrs)
         for (int d = 0; d < 3; ++d) {
@@ -631,12 +641,25 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
                             }
                             if (!in || covering(iv[0], iv[1], iv[2]) >= 0) continue;
                             const int cb = find_box(C, iv[0], iv[1], iv[2]);
-                            if (cb >= 0) cand.push_back({cb, iv[2], iv[1], iv[0]});
+                            if (cb < 0) continue;
+                            if (narrow) ckey.push_back(key_of(cb, iv[2], iv[1], iv[0]));
+                            else cand.push_back({cb, iv[2], iv[1], iv[0]});
                         }
             }
         }
-    std::sort(cand.begin(), cand.end());
-    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    if (narrow) {
+        std::sort(ckey.begin(), ckey.end());
+        ckey.erase(std::unique(ckey.begin(), ckey.end()), ckey.end());
+        cand.resize(ckey.size());
+        for (size_t q = 0; q < ckey.size(); ++q) {
+            const uint64_t k = ckey[q];
+            cand[q] = {(int)(k >> 48), (int)((k >> 32) & 0xffff) + CG.dom_lo[2], (int)((k >> 16) & 0xffff) + CG.dom_lo[1],
+                       (int)(k & 0xffff) + CG.dom_lo[0]};
+        }
+    } else {
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    }
     std::vector<CFFace> cf, ff;
     std::vector<int4> cells;
     std::vector<int> cellfaces;

@@ -37,12 +37,16 @@ struct Pt {
 
 enum CutKind { HOLE = 0, INFLECTION = 1, BISECT = 2, NONE = 3 };
 
-// A cluster: a range [b, e) of a point array and the range's bounding box.
-// Chopping partitions the range in place, so clusters never copy points.
+// A cluster: a range [b, e) of a point array, the range's bounding box and
+// its signatures (point counts per plane across each direction, over the
+// box).  Chopping partitions the range in place, so clusters never copy
+// points, and builds the two parts' boxes and signatures in the same pass.
 struct Cluster {
     size_t b = 0, e = 0;
     IBox box{};
-    void min_box(const std::vector<Pt>& pts)
+    std::vector<int> hist[3];
+    // bounding box and signatures from a pass over the points
+    void scan(const std::vector<Pt>& pts)
     {
         for (int d = 0; d < 3; ++d) {
             box.lo[d] = pts[b].v[d];
@@ -53,13 +57,16 @@ struct Cluster {
                 box.lo[d] = std::min(box.lo[d], pts[q].v[d]);
                 box.hi[d] = std::max(box.hi[d], pts[q].v[d]);
             }
+        for (int d = 0; d < 3; ++d) hist[d].assign(box.hi[d] - box.lo[d] + 1, 0);
+        for (size_t q = b; q < e; ++q)
+            for (int d = 0; d < 3; ++d) hist[d][pts[q].v[d] - box.lo[d]]++;
     }
     size_t size() const { return e - b; }
     double eff() const { return (double)size() / (double)box.volume(); }
 };
 
 // cut index along one direction from the signature hist[0..len-1] over [lo, lo+len-1]
-int find_cut(const std::vector<int>& hist, int lo, CutKind& kind)
+int find_cut(const std::vector<int>& hist, int lo, CutKind& kind, std::vector<int>& d2)
 {
     const int MINOFF = 2, CUT_THRESH = 2;
     kind = NONE;
@@ -77,7 +84,7 @@ int find_cut(const std::vector<int>& hist, int lo, CutKind& kind)
         }
     }
     if (kind == HOLE) return lo + cut;
-    std::vector<int> d2(len, 0);
+    d2.assign(len, 0);
     for (int i = 1; i < len - 1; ++i) d2[i] = hist[i + 1] - 2 * hist[i] + hist[i - 1];
     int locmax = -1;
     for (int i = MINOFF; i < len - MINOFF; ++i) {
@@ -100,18 +107,30 @@ int find_cut(const std::vector<int>& hist, int lo, CutKind& kind)
     return lo + cut;
 }
 
-// split cluster c in two; returns the high part, c keeps the low part
-Cluster chop(Cluster& c, std::vector<Pt>& pts)
+// The two parts' boxes and signatures: counted over the parent's box during
+// the partition pass, then trimmed to the planes that hold points (a box is
+// the span of its nonzero signature entries in each direction).
+void trim(Cluster& c, const IBox& parent, const int* const* h)
 {
-    std::vector<int> hist[3];
-    for (int d = 0; d < 3; ++d) hist[d].assign(c.box.hi[d] - c.box.lo[d] + 1, 0);
-    for (size_t q = c.b; q < c.e; ++q)
-        for (int d = 0; d < 3; ++d) hist[d][pts[q].v[d] - c.box.lo[d]]++;
+    for (int d = 0; d < 3; ++d) {
+        const int n = parent.hi[d] - parent.lo[d] + 1;
+        int a = 0, z = n - 1;
+        while (a < z && h[d][a] == 0) ++a;
+        while (z > a && h[d][z] == 0) --z;
+        c.box.lo[d] = parent.lo[d] + a;
+        c.box.hi[d] = parent.lo[d] + z;
+        c.hist[d].assign(h[d] + a, h[d] + z + 1);
+    }
+}
+
+// split cluster c in two; returns the high part, c keeps the low part
+Cluster chop(Cluster& c, std::vector<Pt>& pts, std::vector<int>& d2, std::vector<int>& work)
+{
     CutKind kind[3];
     int cut[3];
     CutKind best = NONE;
     for (int d = 0; d < 3; ++d) {
-        cut[d] = find_cut(hist[d], c.box.lo[d], kind[d]);
+        cut[d] = find_cut(c.hist[d], c.box.lo[d], kind[d], d2);
         best = std::min(best, kind[d]);
     }
     int dir = -1, minlen = -1;
@@ -124,13 +143,39 @@ Cluster chop(Cluster& c, std::vector<Pt>& pts)
         }
     }
     const int cd = cut[dir];
-    const auto mid = std::partition(pts.begin() + c.b, pts.begin() + c.e, [&](const Pt& p) { return p.v[dir] < cd; });
+    const IBox parent = c.box;
+    int n[3], off[3];
+    size_t tot = 0;
+    for (int d = 0; d < 3; ++d) {
+        n[d] = parent.hi[d] - parent.lo[d] + 1;
+        off[d] = (int)tot;
+        tot += n[d];
+    }
+    work.assign(2 * tot, 0);
+    int* lo_h = work.data();
+    int* hi_h = work.data() + tot;
+    // partition by the cut plane, counting each part's signatures on the way
+    size_t i = c.b, j = c.e;
+    while (true) {
+        while (i < j && pts[i].v[dir] < cd) {
+            for (int d = 0; d < 3; ++d) lo_h[off[d] + pts[i].v[d] - parent.lo[d]]++;
+            ++i;
+        }
+        while (i < j && pts[j - 1].v[dir] >= cd) {
+            --j;
+            for (int d = 0; d < 3; ++d) hi_h[off[d] + pts[j].v[d] - parent.lo[d]]++;
+        }
+        if (i >= j) break;
+        std::swap(pts[i], pts[j - 1]);
+    }
     Cluster hi;
-    hi.b = (size_t)(mid - pts.begin());
+    hi.b = i;
     hi.e = c.e;
-    c.e = hi.b;
-    c.min_box(pts);
-    hi.min_box(pts);
+    c.e = i;
+    const int* lh[3] = {lo_h + off[0], lo_h + off[1], lo_h + off[2]};
+    const int* hh[3] = {hi_h + off[0], hi_h + off[1], hi_h + off[2]};
+    trim(c, parent, lh);
+    trim(hi, parent, hh);
     return hi;
 }
 
@@ -139,10 +184,11 @@ Cluster chop(Cluster& c, std::vector<Pt>& pts)
 // to the end of the list
 void refine(std::vector<Cluster>& lst, std::vector<Pt>& pts, double eff)
 {
+    std::vector<int> d2, work;
     for (size_t q = 0; q < lst.size();) {
         if (lst[q].eff() < eff) {
-            Cluster hi = chop(lst[q], pts);
-            lst.push_back(hi);
+            Cluster hi = chop(lst[q], pts, d2, work);
+            lst.push_back(std::move(hi));
         } else {
             ++q;
         }
@@ -258,8 +304,9 @@ int fdiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 // coordinates) grouped by Berger-Rigoutsos to grid_eff; when has_mask, each
 // cluster intersected with the allowed region (ClusterList::intersect with the
 // proper nesting domain: a box partly outside is replaced by an exact box
-// cover of its part inside; in_box(box, out) lists the allowed points of a
-// box); refined back by the blocking factor, joined, chopped to max size.
+// cover of its part inside; in_box(box, out) returns true when every point of
+// the box is allowed, else lists the allowed points in out); refined back by
+// the blocking factor, joined, chopped to max size.
 template <class InBox>
 void cluster_points(std::vector<Pt>& all, bool has_mask, InBox&& in_box, int block, double grid_eff,
                     int max_size_cells, std::vector<IBox>& boxes)
@@ -269,20 +316,19 @@ void cluster_points(std::vector<Pt>& all, bool has_mask, InBox&& in_box, int blo
     std::vector<Cluster> lst(1);
     lst[0].b = 0;
     lst[0].e = all.size();
-    lst[0].min_box(all);
+    lst[0].scan(all);
     refine(lst, all, grid_eff);
     std::vector<IBox> kept;
     std::vector<Pt> in;
     for (const Cluster& c : lst) {
         if (!has_mask) { kept.push_back(c.box); continue; }
         in.clear();
-        in_box(c.box, in);
-        if ((long)in.size() == c.box.volume()) { kept.push_back(c.box); continue; }
+        if (in_box(c.box, in)) { kept.push_back(c.box); continue; }
         if (in.empty()) continue;
         std::vector<Cluster> part(1);
         part[0].b = 0;
         part[0].e = in.size();
-        part[0].min_box(in);
+        part[0].scan(in);
         refine(part, in, 1.0);
         for (const Cluster& q : part) kept.push_back(q.box);
     }
@@ -381,6 +427,7 @@ extern "C" int ocfd_cluster_tags(const int lo[3], const int hi[3], const unsigne
                     const int ii = i - lo[0] / block, jj = j - lo[1] / block, kk = k - lo[2] / block;
                     if (bmp[((size_t)kk * nb[1] + jj) * nb[0] + ii]) out.push_back(Pt{{i, j, k}});
                 }
+        return (long)out.size() == b.volume();
     };
     std::vector<IBox> boxes;
     cluster_points(all, mask != nullptr, in_box, block, grid_eff, max_size_cells, boxes);
@@ -438,6 +485,7 @@ extern "C" int ocfd_cluster_box_tags(int nbox, const int* lo, const int* hi, con
     }
     // points: blocks holding a cell flagged both tagged and allowed
     std::vector<Pt> all;
+    all.reserve(off[nbox] / ((size_t)block * block * block) + 1);
     for (int b = 0; b < nbox; ++b) {
         const IBox& B = lv[b];
         const unsigned char* f = flags + off[b];
@@ -445,6 +493,11 @@ extern "C" int ocfd_cluster_box_tags(int nbox, const int* lo, const int* hi, con
         size_t c = 0;
         for (int k = B.lo[2]; k <= B.hi[2]; ++k)
             for (int j = B.lo[1]; j <= B.hi[1]; ++j, c += n0) {
+                if (block == 1) {
+                    for (int i = 0; i < n0; ++i)
+                        if ((f[c + i] & 3) == 3) all.push_back(Pt{{B.lo[0] + i, j, k}});
+                    continue;
+                }
                 for (int i = 0; i < n0; ++i)
                     if ((f[c + i] & 3) == 3) all.push_back(Pt{{fdiv(B.lo[0] + i, block), fdiv(j, block), fdiv(k, block)}});
             }
@@ -487,6 +540,12 @@ extern "C" int ocfd_cluster_box_tags(int nbox, const int* lo, const int* hi, con
                     for (int i = 0; i <= z[0] - a[0]; ++i) dst[i] = (src[i] >> 1) & 1;
                 }
         }
+        if (block == 1) {
+            // the common case: every cell allowed, counted without listing
+            bool all_ok = true;
+            for (size_t q = 0; q < loc.size() && all_ok; ++q) all_ok = loc[q] != 0;
+            if (all_ok) return true;
+        }
         for (int k = cb.lo[2]; k <= cb.hi[2]; ++k)
             for (int j = cb.lo[1]; j <= cb.hi[1]; ++j)
                 for (int i = cb.lo[0]; i <= cb.hi[0]; ++i) {
@@ -498,6 +557,7 @@ extern "C" int ocfd_cluster_box_tags(int nbox, const int* lo, const int* hi, con
                                          (i * block + ii - cr.lo[0])] != 0;
                     if (ok) out.push_back(Pt{{i, j, k}});
                 }
+        return (long)out.size() == cb.volume();
     };
     std::vector<IBox> boxes;
     cluster_points(all, use_mask != 0, in_box, block, grid_eff, max_size_cells, boxes);
