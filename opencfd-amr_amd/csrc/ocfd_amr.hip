@@ -236,10 +236,13 @@ __device__ __forceinline__ int interp_box(const BoxDesc* __restrict__ cboxes, co
 }
 
 // component n of fine cell (i,j,k) interpolated from coarse box Bc
+// czuni: the coarse level is z-uniform (planar) and fills the ghost cells of
+// its first two layers only; every layer holds the first layer's values, so
+// the parent is read in the first layer of its box.
 __device__ __forceinline__ double interp_value(const BoxDesc& Bc, const GeomDesc& CG, const double* __restrict__ C,
-                                               int i, int j, int k, int n)
+                                               int i, int j, int k, int n, int czuni)
 {
-    const int I = i >> 1, J = j >> 1, K = k >> 1;
+    const int I = i >> 1, J = j >> 1, K = czuni ? Bc.lo[2] : k >> 1;
     const double* u = C + n * Bc.nptg;
     const long s = sidx(Bc, I, J, K);
     const long dy = Bc.nxg, dz = (long)Bc.nxg * Bc.nyg;
@@ -280,7 +283,7 @@ __device__ __forceinline__ double interp_value(const BoxDesc& Bc, const GeomDesc
 __global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fboxes, const GeomDesc* __restrict__ fgeom,
                                                  const int4* __restrict__ tiles, double* __restrict__ Sf,
                                                  const BoxDesc* __restrict__ cboxes, const GeomDesc* __restrict__ cgeom,
-                                                 const int* __restrict__ cmap, const double* __restrict__ Sc)
+                                                 const int* __restrict__ cmap, const double* __restrict__ Sc, int czuni)
 {
     const int4 t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.z) return;
@@ -334,7 +337,7 @@ __global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fbo
             last[0] = src[0]; last[1] = src[1]; last[2] = src[2];
         }
         if (own >= 0 || m < 0) continue;   // the level fill copied it / nothing to interpolate from
-        Sf[dst + n * B.nptg] = sign * interp_value(cboxes[m], *cgeom, Sc, src[0], src[1], src[2], n);
+        Sf[dst + n * B.nptg] = sign * interp_value(cboxes[m], *cgeom, Sc, src[0], src[1], src[2], n, czuni);
     }
 }
 
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(256) void k_interp_valid(const BoxDesc* __restrict_
                                                       double* __restrict__ Sf, const BoxDesc* __restrict__ cboxes,
                                                       const GeomDesc* __restrict__ cgeom, const int* __restrict__ cmap,
                                                       const double* __restrict__ Sc, const BoxDesc* __restrict__ oboxes,
-                                                      const GeomDesc* __restrict__ ogeom)
+                                                      const GeomDesc* __restrict__ ogeom, int czuni)
 {
     const int4 t = tiles[blockIdx.x];
     const BoxDesc B = fboxes[t.x];
@@ -362,7 +365,7 @@ __global__ __launch_bounds__(256) void k_interp_valid(const BoxDesc* __restrict_
         const long dst = sidx(B, i, j, k);
         const int m = interp_box(cboxes, *cgeom, cmap, i, j, k);
         if (m < 0) continue;
-        for (int n = 0; n < NC; ++n) Sf[dst + n * B.nptg] = interp_value(cboxes[m], *cgeom, Sc, i, j, k, n);
+        for (int n = 0; n < NC; ++n) Sf[dst + n * B.nptg] = interp_value(cboxes[m], *cgeom, Sc, i, j, k, n, czuni);
     }
 }
 
@@ -388,8 +391,12 @@ __global__ __launch_bounds__(256) void k_copy_level(const BoxDesc* __restrict__ 
 
 // tag_dengrad, Source/Tagging.H:8-36: the larger one-sided density jump in
 // any direction >= threshold (ghost cells of S filled); covered boxes untagged
+// zuni: a z-uniform (planar) level, whose ghost cells are filled in its first
+// two layers only: every cell takes the tag of its column's first-layer cell
+// (the same value, as the layers hold the same data)
 __global__ __launch_bounds__(256) void k_tag(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
-                                             const double* __restrict__ S, double thr, uint8_t* __restrict__ tags)
+                                             const double* __restrict__ S, double thr, uint8_t* __restrict__ tags,
+                                             int zuni)
 {
     const int4 t = tiles[blockIdx.x];
     const BoxDesc B = boxes[t.x];
@@ -398,7 +405,7 @@ __global__ __launch_bounds__(256) void k_tag(const BoxDesc* __restrict__ boxes, 
         valid_cell(B, t.y + c, i, j, k);
         uint8_t tg = 0;
         if (B.type != 2) {
-            const long s = sidx(B, i, j, k);
+            const long s = sidx(B, i, j, zuni ? B.lo[2] : k);
             const long sy = B.nxg, sz = (long)B.nxg * B.nyg;
             const double r = S[s];
             double ax = fabs(S[s + 1] - r), ay = fabs(S[s + sy] - r), az = fabs(S[s + sz] - r);
@@ -417,7 +424,7 @@ __global__ __launch_bounds__(256) void k_tag(const BoxDesc* __restrict__ boxes, 
 // counted) is covered by the level's boxes
 __global__ __launch_bounds__(256) void k_tag_nest(const BoxDesc* __restrict__ boxes, const GeomDesc* __restrict__ geom,
                                                   const int4* __restrict__ tiles, const double* __restrict__ S,
-                                                  double thr, int n_proper, uint8_t* __restrict__ flags)
+                                                  double thr, int n_proper, uint8_t* __restrict__ flags, int zuni)
 {
     const int4 t = tiles[blockIdx.x];
     const BoxDesc B = boxes[t.x];
@@ -427,7 +434,7 @@ __global__ __launch_bounds__(256) void k_tag_nest(const BoxDesc* __restrict__ bo
         valid_cell(B, t.y + c, i, j, k);
         uint8_t tg = 0;
         if (B.type != 2) {
-            const long s = sidx(B, i, j, k);
+            const long s = sidx(B, i, j, zuni ? B.lo[2] : k);
             const long sy = B.nxg, sz = (long)B.nxg * B.nyg;
             const double r = S[s];
             double ax = fabs(S[s + 1] - r), ay = fabs(S[s + sy] - r), az = fabs(S[s + sz] - r);
@@ -436,8 +443,11 @@ __global__ __launch_bounds__(256) void k_tag_nest(const BoxDesc* __restrict__ bo
             az = fmax(az, fabs(r - S[s - sz]));
             tg = fmax(ax, fmax(ay, az)) >= thr ? 1 : 0;
         }
+        // a neighbourhood inside the cell's own box is covered
         bool in = true;
-        for (int c2 = -n_proper; c2 <= n_proper && in; ++c2)
+        const bool own = i - n_proper >= B.lo[0] && i + n_proper <= B.hi[0] && j - n_proper >= B.lo[1] &&
+                         j + n_proper <= B.hi[1] && k - n_proper >= B.lo[2] && k + n_proper <= B.hi[2];
+        for (int c2 = -n_proper; c2 <= n_proper && in && !own; ++c2)
             for (int c1 = -n_proper; c1 <= n_proper && in; ++c1)
                 for (int c0 = -n_proper; c0 <= n_proper && in; ++c0) {
                     int q[3] = {i + c0, j + c1, k + c2};
@@ -887,7 +897,7 @@ int ocfd_cf_fill_fine(ocfd_cf c, double* S_fine, const double* S_coarse, void* s
     const int4* tg = fill_tiles(F, ng);
     if (ng)
         hipLaunchKernelGGL(k_fill_cf, dim3(ng), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_geom, tg,
-                           S_fine, C.d_boxes, C.d_geom, c->d_cmap, S_coarse);
+                           S_fine, C.d_boxes, C.d_geom, c->d_cmap, S_coarse, (int)(C.fill_gz == 1));
     return launched("ocfd_cf_fill_fine");
 }
 
@@ -909,7 +919,7 @@ int ocfd_cf_interp_fine(ocfd_cf c, double* S_fine, const double* S_coarse, ocfd_
     if (F.ntile_c)
         hipLaunchKernelGGL(k_interp_valid, dim3(F.ntile_c), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_tile_c,
                            S_fine, C.d_boxes, C.d_geom, c->d_cmap, S_coarse, old_fine ? old_fine->dev.d_boxes : nullptr,
-                           old_fine ? old_fine->dev.d_geom : nullptr);
+                           old_fine ? old_fine->dev.d_geom : nullptr, (int)(C.fill_gz == 1));
     return launched("ocfd_cf_interp_fine");
 }
 
@@ -941,7 +951,7 @@ int ocfd_tag_dengrad(ocfd_level lev, const double* S, double threshold, unsigned
     const LevelDev& L = lev->dev;
     if (L.ntile_c)
         hipLaunchKernelGGL(k_tag, dim3(L.ntile_c), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_tile_c, S,
-                           threshold, (uint8_t*)tags);
+                           threshold, (uint8_t*)tags, (int)(L.fill_gz == 1));
     return launched("ocfd_tag_dengrad");
 }
 
@@ -955,7 +965,7 @@ int ocfd_tag_dengrad_nest(ocfd_level lev, const double* S, double threshold, int
     const LevelDev& L = lev->dev;
     if (L.ntile_c)
         hipLaunchKernelGGL(k_tag_nest, dim3(L.ntile_c), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_geom,
-                           L.d_tile_c, S, threshold, n_proper, (uint8_t*)flags);
+                           L.d_tile_c, S, threshold, n_proper, (uint8_t*)flags, (int)(L.fill_gz == 1));
     return launched("ocfd_tag_dengrad_nest");
 }
 

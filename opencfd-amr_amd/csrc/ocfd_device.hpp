@@ -162,15 +162,36 @@ __device__ __forceinline__ long didx(const BoxDesc& B, int i, int j, int k)
     return B.off_d + ((long)(k - B.lo[2]) * ny + (j - B.lo[1])) * nx + (i - B.lo[0]);
 }
 
-// Decode the g-th ghost cell of box B: z slabs gz deep (full grown x-y
-// extent), then y slabs (valid k), then x slabs (valid j,k).  gz = NG covers
-// every ghost cell; gz = 1 the ghost region a planar (z-uniform) level reads.
+// Ghost cells of box B, numbered: z slabs gz deep (full grown x-y extent),
+// then y slabs, then x slabs, over the valid layers.  gz = NG: every ghost
+// cell.  gz = 1: the ghost region of a z-uniform (planar) level, whose sweeps
+// read the first z layer and its two neighbours only: the z slab below, the y
+// and x slabs of the first two layers, and the z slab above only when the box
+// has one layer.
+__host__ __device__ inline int ghost_zslabs(const BoxDesc& B, int gz)
+{
+    return (gz == 1 && B.hi[2] > B.lo[2]) ? 1 : 2;
+}
+
+__host__ __device__ inline int ghost_layers(const BoxDesc& B, int gz)
+{
+    const int nz = B.hi[2] - B.lo[2] + 1;
+    return gz == 1 ? (nz < 2 ? nz : 2) : nz;
+}
+
+__host__ __device__ inline long ghost_count(const BoxDesc& B, int gz)
+{
+    const long ny = B.hi[1] - B.lo[1] + 1, nk = ghost_layers(B, gz);
+    return (long)ghost_zslabs(B, gz) * gz * B.nxg * B.nyg + 2L * NG * B.nxg * nk + 2L * NG * ny * nk;
+}
+
+// Decode the g-th ghost cell of box B in that numbering.
 __device__ __forceinline__ void ghost_cell(const BoxDesc& B, int g, int gz, int& i, int& j, int& k)
 {
-    const int ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
+    const int ny = B.hi[1] - B.lo[1] + 1, nk = ghost_layers(B, gz);
     const int nxg = B.nxg, nyg = B.nyg;
     const int zs = gz * nxg * nyg;
-    if (g < 2 * zs) {
+    if (g < ghost_zslabs(B, gz) * zs) {
         const int side = g / zs, r = g - side * zs;
         const int kk = r / (nxg * nyg), rem = r - kk * nxg * nyg;
         k = side ? B.hi[2] + 1 + kk : B.lo[2] - gz + kk;
@@ -178,8 +199,8 @@ __device__ __forceinline__ void ghost_cell(const BoxDesc& B, int g, int gz, int&
         i = B.lo[0] - NG + rem % nxg;
         return;
     }
-    g -= 2 * zs;
-    const int ys = NG * nxg * nz;
+    g -= ghost_zslabs(B, gz) * zs;
+    const int ys = NG * nxg * nk;
     if (g < 2 * ys) {
         const int side = g / ys, r = g - side * ys;
         const int kk = r / (NG * nxg), rem = r - kk * NG * nxg;
@@ -190,7 +211,7 @@ __device__ __forceinline__ void ghost_cell(const BoxDesc& B, int g, int gz, int&
         return;
     }
     g -= 2 * ys;
-    const int xs = NG * ny * nz;
+    const int xs = NG * ny * nk;
     const int side = g / xs, r = g - side * xs;
     const int kk = r / (NG * ny), rem = r - kk * NG * ny;
     k = B.lo[2] + kk;

@@ -520,10 +520,11 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
         const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
-        const long ngh = B.nptg - B.nptv;
+        const long ngh = ghost_count(B, NG);
         for (long s = 0; s < ngh; s += 256) tg.push_back(make_int4(b, (int)s, (int)std::min<long>(256, ngh - s), NG));
-        // planar fill: z ghosts one layer deep
-        const long ngp = 2L * B.nxg * B.nyg + 2L * NG * B.nxg * nz + 2L * NG * ny * nz;
+        // planar fill: the z ghost layer below, the y and x ghosts of the
+        // first two layers (the cells the first-layer sweeps read)
+        const long ngp = ghost_count(B, 1);
         for (long s = 0; s < ngp; s += 256) tgp.push_back(make_int4(b, (int)s, (int)std::min<long>(256, ngp - s), 1));
         for (long s = 0; s < B.nptv; s += 2048) tc.push_back(make_int4(b, (int)s, (int)std::min<long>(2048, B.nptv - s), 0));
         for (long s = 0; s < B.nptv; s += 64) tv.push_back(make_int4(b, (int)s, (int)std::min<long>(64, B.nptv - s), 0));
