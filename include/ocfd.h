@@ -504,6 +504,14 @@ int ocfd_cluster_box_tags(int nbox, const int* lo, const int* hi, const unsigned
                           const int* elo, const int* ehi, int use_mask, int block, double grid_eff,
                           int max_size_cells, int cap, int* boxes_out, int* nbox_out);
 
+/* Boxes of level f+1 (lo/hi, inclusive, 3 ints each) projected to level f-1: coarsened to level f, grown by
+ * n_proper cells there, the parts beyond level f's domain (dom_lo..dom_hi) wrapped to their periodic
+ * images on periodic sides and clipped elsewhere, coarsened again; the tags that keep level f's new grids
+ * around the finer new grids (Amr::grid_places, AMReX; the reference's regrids run under it,
+ * Source/EBR.cpp:231-306).  The boxes may overlap.  Output and capacity as ocfd_cluster_tags.  Host only. */
+int ocfd_project_boxes(int nbox, const int* lo, const int* hi, const int dom_lo[3], const int dom_hi[3],
+                       const int periodic[3], int n_proper, int cap, int* boxes_out, int* nbox_out);
+
 /* A dense 0/1 mask over a level's domain (nz*ny*nx bytes, i fastest, host
  * memory) grown in place by n cells in every direction (Chebyshev: the tag
  * buffer amr.n_error_buf), or with shrink != 0 shrunk by n cells (the proper

@@ -68,7 +68,30 @@ def build(verbose=False):
     tmp = str(LIB) + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
+
+
+DRIVER = PKG / "ocfd_run"
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+
+
+def driver_cmd(src, out):
+    """The host-only compile of a C++ program over include/ocfd.hpp, linked
+    against libocfd.so (found next to the program at run time)."""
+    return [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+            f"-I{ROCM / 'include'}", f"-I{ROOT / 'include'}", str(src), f"-L{PKG}", "-locfd",
+            f"-L{ROCM / 'lib'}", "-lamdhip64", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM / 'lib'}", "-o", str(out)]
+
+
+def build_driver(verbose=False):
+    """ocfd_run: the C++ host driver of one level (host/ocfd_run.cpp)."""
+    cmd = driver_cmd(PKG / "host" / "ocfd_run.cpp", str(DRIVER) + ".tmp")
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(str(DRIVER) + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -564,6 +564,56 @@ extern "C" int ocfd_cluster_box_tags(int nbox, const int* lo, const int* hi, con
     return write_boxes(boxes, cap, boxes_out, nbox_out, "ocfd_cluster_box_tags");
 }
 
+// Boxes of level f+1 projected two levels down: coarsened to level f, grown by
+// n_proper there (the parts leaving the domain of level f wrapped to their
+// periodic images on periodic sides, clipped elsewhere), coarsened to level
+// f-1.  The tags Amr::grid_places adds so that level f's new grids keep the
+// finer new grids properly nested.  The output boxes may overlap.
+extern "C" int ocfd_project_boxes(int nbox, const int* lo, const int* hi, const int dom_lo[3], const int dom_hi[3],
+                                  const int periodic[3], int n_proper, int cap, int* boxes_out, int* nbox_out)
+{
+    if (nbox < 0 || (nbox > 0 && (!lo || !hi)) || !dom_lo || !dom_hi || !periodic || n_proper < 0 || cap < 0 ||
+        (cap > 0 && !boxes_out) || !nbox_out) {
+        ocfd::set_error("ocfd_project_boxes: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    std::vector<IBox> out, parts, next;
+    for (int b = 0; b < nbox; ++b) {
+        IBox g;
+        for (int d = 0; d < 3; ++d) {
+            g.lo[d] = fdiv(lo[3 * b + d], 2) - n_proper;
+            g.hi[d] = fdiv(hi[3 * b + d], 2) + n_proper;
+        }
+        parts.assign(1, g);
+        for (int d = 0; d < 3; ++d) {
+            const int n = dom_hi[d] - dom_lo[d] + 1;
+            next.clear();
+            for (const IBox& q : parts) {
+                const int shifts[3] = {-n, 0, n};
+                const int ns = periodic[d] ? 3 : 1;
+                for (int t = 0; t < ns; ++t) {
+                    const int sh = periodic[d] ? shifts[t] : 0;
+                    const int a = std::max(q.lo[d] + sh, dom_lo[d]), z = std::min(q.hi[d] + sh, dom_hi[d]);
+                    if (a > z) continue;
+                    IBox r = q;
+                    r.lo[d] = a;
+                    r.hi[d] = z;
+                    next.push_back(r);
+                }
+            }
+            parts.swap(next);
+        }
+        for (IBox r : parts) {
+            for (int d = 0; d < 3; ++d) {
+                r.lo[d] = fdiv(r.lo[d], 2);
+                r.hi[d] = fdiv(r.hi[d], 2);
+            }
+            out.push_back(r);
+        }
+    }
+    return write_boxes(out, cap, boxes_out, nbox_out, "ocfd_project_boxes");
+}
+
 // Dense mask grown by n cells in every direction (Chebyshev distance, the
 // TagBoxArray::buffer of amr.n_error_buf), or shrunk by n cells (the proper
 // nesting domain of a level: cells whose n-neighbourhood the level covers).

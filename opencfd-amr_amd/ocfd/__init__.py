@@ -189,6 +189,7 @@ def lib():
                                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
         L.ocfd_reactor_adaptive_chunk.argtypes = [ctypes.c_long, vp, vp, ctypes.c_double, ctypes.c_double,
                                                   ctypes.c_double, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+        L.ocfd_project_boxes.argtypes = [ctypes.c_int, ip, ip, ip, ip, ip, ctypes.c_int, ctypes.c_int, ip, ip]
         L.ocfd_mask_grow.argtypes = [ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ip, ctypes.c_int]
         L.ocfd_lincomb.argtypes = [ctypes.c_long, ctypes.c_double, vp, ctypes.c_double, vp, vp, vp]

@@ -194,33 +194,16 @@ def projected_coverage(boxes_f2, geom_f, geom_c, n_proper):
 
 def projected_boxes(boxes_f2, geom_f, n_proper):
     """The boxes of projected_coverage (level c index space, may overlap)."""
-    lo_f = [geom_f.dom_lo[d] for d in range(3)]
-    n_f = [geom_f.dom_hi[d] - lo_f[d] + 1 for d in range(3)]
-    out = []
-    for blo, bhi in coarsen_boxes(boxes_f2):
-        glo = [blo[d] - n_proper for d in range(3)]
-        ghi = [bhi[d] + n_proper for d in range(3)]
-        # periodic images of the parts that leave the domain; elsewhere clip
-        parts = [(glo, ghi)]
-        for d in range(3):
-            nxt = []
-            for plo, phi in parts:
-                if geom_f.periodic[d]:
-                    for sh in (-n_f[d], 0, n_f[d]):
-                        a, b = max(plo[d] + sh, lo_f[d]), min(phi[d] + sh, lo_f[d] + n_f[d] - 1)
-                        if a <= b:
-                            q0, q1 = list(plo), list(phi)
-                            q0[d], q1[d] = a, b
-                            nxt.append((q0, q1))
-                else:
-                    a, b = max(plo[d], lo_f[d]), min(phi[d], lo_f[d] + n_f[d] - 1)
-                    if a <= b:
-                        q0, q1 = list(plo), list(phi)
-                        q0[d], q1[d] = a, b
-                        nxt.append((q0, q1))
-            parts = nxt
-        out.extend((tuple(p0), tuple(p1)) for p0, p1 in parts)
-    return coarsen_boxes(out)
+    n, lo, hi = _box_arrays(boxes_f2)
+    dlo = (ctypes.c_int * 3)(*[geom_f.dom_lo[d] for d in range(3)])
+    dhi = (ctypes.c_int * 3)(*[geom_f.dom_hi[d] for d in range(3)])
+    per = (ctypes.c_int * 3)(*[int(bool(geom_f.periodic[d])) for d in range(3)])
+    nb = ctypes.c_int()
+    cap = max(64, 27 * n)
+    out = (ctypes.c_int * (6 * cap))()
+    check(lib().ocfd_project_boxes(n, lo, hi, dlo, dhi, per, int(n_proper), cap, out, ctypes.byref(nb)),
+          "ocfd_project_boxes")
+    return [(tuple(out[6 * b:6 * b + 3]), tuple(out[6 * b + 3:6 * b + 6])) for b in range(nb.value)]
 
 
 def coarsen_boxes(boxes):
