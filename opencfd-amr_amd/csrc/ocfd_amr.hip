@@ -26,6 +26,7 @@
 
 #include "ocfd_device.hpp"
 #include "ocfd_internal.hpp"
+#include "ocfd_host_pool.hpp"
 
 namespace ocfd {
 
@@ -630,34 +631,47 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
         return ((uint64_t)b << 48) | ((uint64_t)(K - CG.dom_lo[2]) << 32) | ((uint64_t)(J - CG.dom_lo[1]) << 16) |
                (uint64_t)(I - CG.dom_lo[0]);
     };
-    std::vector<std::array<int, 4>> cand;
-    for (const BoxDesc& B : fcrs)
-        for (int d = 0; d < 3; ++d) {
-            const int len = CG.dom_hi[d] - CG.dom_lo[d] + 1;
-            if (CG.periodic[d] && B.hi[d] - B.lo[d] + 1 >= len) continue;
-            for (int side = 0; side < 2; ++side) {
-                const int at = side ? B.hi[d] + 1 : B.lo[d] - 1;
-                if (!CG.periodic[d] && (at < CG.dom_lo[d] || at > CG.dom_hi[d])) continue;
-                int lo3[3] = {B.lo[0], B.lo[1], B.lo[2]}, hi3[3] = {B.hi[0], B.hi[1], B.hi[2]};
-                lo3[d] = hi3[d] = at;
-                for (int K = lo3[2]; K <= hi3[2]; ++K)
-                    for (int J = lo3[1]; J <= hi3[1]; ++J)
-                        for (int I = lo3[0]; I <= hi3[0]; ++I) {
-                            int iv[3] = {I, J, K};
-                            bool in = true;
-                            for (int e = 0; e < 3; ++e) {
-                                iv[e] = wrap(CG, e, iv[e]);
-                                in = in && iv[e] >= CG.dom_lo[e] && iv[e] <= CG.dom_hi[e];
+    // the candidates and the face lists are built on the host threads: the
+    // candidates of contiguous ranges of fine boxes (sorted afterwards), the
+    // faces of contiguous ranges of the sorted candidates (joined in order, so
+    // the lists are those a single thread builds)
+    const int nth = std::max(1, std::min(host_threads(), (int)fcrs.size() / 16));
+    std::vector<std::vector<uint64_t>> ckey_t(nth);
+    std::vector<std::vector<std::array<int, 4>>> cand_t(nth);
+    parallel_for(nth, [&](int t) {
+        const size_t b0 = fcrs.size() * t / nth, b1 = fcrs.size() * (t + 1) / nth;
+        for (size_t fb = b0; fb < b1; ++fb) {
+            const BoxDesc& B = fcrs[fb];
+            for (int d = 0; d < 3; ++d) {
+                const int len = CG.dom_hi[d] - CG.dom_lo[d] + 1;
+                if (CG.periodic[d] && B.hi[d] - B.lo[d] + 1 >= len) continue;
+                for (int side = 0; side < 2; ++side) {
+                    const int at = side ? B.hi[d] + 1 : B.lo[d] - 1;
+                    if (!CG.periodic[d] && (at < CG.dom_lo[d] || at > CG.dom_hi[d])) continue;
+                    int lo3[3] = {B.lo[0], B.lo[1], B.lo[2]}, hi3[3] = {B.hi[0], B.hi[1], B.hi[2]};
+                    lo3[d] = hi3[d] = at;
+                    for (int K = lo3[2]; K <= hi3[2]; ++K)
+                        for (int J = lo3[1]; J <= hi3[1]; ++J)
+                            for (int I = lo3[0]; I <= hi3[0]; ++I) {
+                                int iv[3] = {I, J, K};
+                                bool in = true;
+                                for (int e = 0; e < 3; ++e) {
+                                    iv[e] = wrap(CG, e, iv[e]);
+                                    in = in && iv[e] >= CG.dom_lo[e] && iv[e] <= CG.dom_hi[e];
+                                }
+                                if (!in || covering(iv[0], iv[1], iv[2]) >= 0) continue;
+                                const int cb = find_box(C, iv[0], iv[1], iv[2]);
+                                if (cb < 0) continue;
+                                if (narrow) ckey_t[t].push_back(key_of(cb, iv[2], iv[1], iv[0]));
+                                else cand_t[t].push_back({cb, iv[2], iv[1], iv[0]});
                             }
-                            if (!in || covering(iv[0], iv[1], iv[2]) >= 0) continue;
-                            const int cb = find_box(C, iv[0], iv[1], iv[2]);
-                            if (cb < 0) continue;
-                            if (narrow) ckey.push_back(key_of(cb, iv[2], iv[1], iv[0]));
-                            else cand.push_back({cb, iv[2], iv[1], iv[0]});
-                        }
+                }
             }
         }
+    });
+    std::vector<std::array<int, 4>> cand;
     if (narrow) {
+        for (int t = 0; t < nth; ++t) ckey.insert(ckey.end(), ckey_t[t].begin(), ckey_t[t].end());
         std::sort(ckey.begin(), ckey.end());
         ckey.erase(std::unique(ckey.begin(), ckey.end()), ckey.end());
         cand.resize(ckey.size());
@@ -667,51 +681,70 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
                        (int)(k & 0xffff) + CG.dom_lo[0]};
         }
     } else {
+        for (int t = 0; t < nth; ++t) cand.insert(cand.end(), cand_t[t].begin(), cand_t[t].end());
         std::sort(cand.begin(), cand.end());
         cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     }
+    struct FaceLists {
+        std::vector<CFFace> cf, ff;
+        std::vector<int4> cells;
+        std::vector<int> cellfaces;
+    };
+    std::vector<FaceLists> part(nth);
+    parallel_for(nth, [&](int t) {
+        FaceLists& P = part[t];
+        const size_t q0 = cand.size() * t / nth, q1 = cand.size() * (t + 1) / nth;
+        P.cf.reserve(2 * (q1 - q0));
+        P.ff.reserve(8 * (q1 - q0));
+        for (size_t qi = q0; qi < q1; ++qi) {
+            const auto& q = cand[qi];
+            const int b = q[0], K = q[1], J = q[2], I = q[3];
+            int fidx[6] = {-1, -1, -1, -1, -1, -1};
+            for (int d = 0; d < 3; ++d)
+                for (int side = 0; side < 2; ++side) {
+                    int nb[3] = {I, J, K};
+                    nb[d] += side ? 1 : -1;
+                    const int fb = covering(nb[0], nb[1], nb[2]);
+                    if (fb < 0) continue;
+                    CFFace f{};
+                    f.box = b; f.i = I; f.j = J; f.k = K; f.dir = d; f.up = side;
+                    fidx[2 * d + side] = (int)P.cf.size();   // local: offset when the parts are joined
+                    P.cf.push_back(f);
+                    // the 4 fine cells across the face: the first fine layer of the neighbour
+                    int fc[3] = {2 * I, 2 * J, 2 * K};
+                    fc[d] = side ? 2 * nb[d] : 2 * nb[d] + 1;
+                    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+                    for (int a2 = 0; a2 < 2; ++a2)
+                        for (int a1 = 0; a1 < 2; ++a1) {
+                            int g[3] = {fc[0], fc[1], fc[2]};
+                            g[t1] += a1;
+                            g[t2] += a2;
+                            for (int e = 0; e < 3; ++e) g[e] = wrap(F.geom, e, g[e]);
+                            CFFace h{};   // in fine box fb, which covers the coarse neighbour
+                            h.box = fb; h.i = g[0]; h.j = g[1]; h.k = g[2]; h.dir = d;
+                            h.up = side ? 0 : 1;   // fine level above the face: its low face
+                            P.ff.push_back(h);
+                        }
+                }
+            bool any = false;
+            for (int m = 0; m < 6; ++m) any = any || fidx[m] >= 0;
+            if (any) {
+                P.cells.push_back(make_int4(b, I, J, K));
+                P.cellfaces.insert(P.cellfaces.end(), fidx, fidx + 6);
+            }
+        }
+    });
     std::vector<CFFace> cf, ff;
     std::vector<int4> cells;
     std::vector<int> cellfaces;
-    cf.reserve(2 * cand.size());
-    ff.reserve(8 * cand.size());
-    cells.reserve(cand.size());
-    cellfaces.reserve(6 * cand.size());
-    for (const auto& q : cand) {
-        const int b = q[0], K = q[1], J = q[2], I = q[3];
-        int fidx[6] = {-1, -1, -1, -1, -1, -1};
-        for (int d = 0; d < 3; ++d)
-            for (int side = 0; side < 2; ++side) {
-                int nb[3] = {I, J, K};
-                nb[d] += side ? 1 : -1;
-                const int fb = covering(nb[0], nb[1], nb[2]);
-                if (fb < 0) continue;
-                CFFace f{};
-                f.box = b; f.i = I; f.j = J; f.k = K; f.dir = d; f.up = side;
-                fidx[2 * d + side] = (int)cf.size();
-                cf.push_back(f);
-                // the 4 fine cells across the face: the first fine layer of the neighbour
-                int fc[3] = {2 * I, 2 * J, 2 * K};
-                fc[d] = side ? 2 * nb[d] : 2 * nb[d] + 1;
-                const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-                for (int a2 = 0; a2 < 2; ++a2)
-                    for (int a1 = 0; a1 < 2; ++a1) {
-                        int g[3] = {fc[0], fc[1], fc[2]};
-                        g[t1] += a1;
-                        g[t2] += a2;
-                        for (int e = 0; e < 3; ++e) g[e] = wrap(F.geom, e, g[e]);
-                        CFFace h{};   // in fine box fb, which covers the coarse neighbour
-                        h.box = fb; h.i = g[0]; h.j = g[1]; h.k = g[2]; h.dir = d;
-                        h.up = side ? 0 : 1;   // fine level above the face: its low face
-                        ff.push_back(h);
-                    }
-            }
-        bool any = false;
-        for (int m = 0; m < 6; ++m) any = any || fidx[m] >= 0;
-        if (any) {
-            cells.push_back(make_int4(b, I, J, K));
-            cellfaces.insert(cellfaces.end(), fidx, fidx + 6);
-        }
+    for (int t = 0; t < nth; ++t) {
+        const int off = (int)cf.size();
+        for (int& v : part[t].cellfaces)
+            if (v >= 0) v += off;
+        cf.insert(cf.end(), part[t].cf.begin(), part[t].cf.end());
+        ff.insert(ff.end(), part[t].ff.begin(), part[t].ff.end());
+        cells.insert(cells.end(), part[t].cells.begin(), part[t].cells.end());
+        cellfaces.insert(cellfaces.end(), part[t].cellfaces.begin(), part[t].cellfaces.end());
     }
     std::vector<int4> ta;
     for (int b = 0; b < F.nbox; ++b) {

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "ocfd.h"
+#include "ocfd_host_pool.hpp"
 
 namespace ocfd {
 void set_error(const std::string& msg);
@@ -181,11 +182,16 @@ Cluster chop(Cluster& c, std::vector<Pt>& pts, std::vector<int>& d2, std::vector
 
 // Berger-Rigoutsos: clusters (in list order) chopped until each reaches eff;
 // a chopped cluster keeps its place and is checked again, its high part goes
-// to the end of the list
-void refine(std::vector<Cluster>& lst, std::vector<Pt>& pts, double eff)
+// to the end of the list.  Every cluster is refined from its own points only
+// (into its own range of pts), so once the list holds enough clusters still
+// to be chopped, they are refined on the host threads, each into a list of
+// its own, and the lists joined in order.  The set of final clusters is the
+// same either way; the boxes built from them are sorted before use.
+void refine_serial(std::vector<Cluster>& lst, std::vector<Pt>& pts, double eff, size_t stop_at,
+                   std::vector<int>& d2, std::vector<int>& work, size_t& q)
 {
-    std::vector<int> d2, work;
-    for (size_t q = 0; q < lst.size();) {
+    while (q < lst.size()) {
+        if (lst.size() >= stop_at) return;
         if (lst[q].eff() < eff) {
             Cluster hi = chop(lst[q], pts, d2, work);
             lst.push_back(std::move(hi));
@@ -193,6 +199,40 @@ void refine(std::vector<Cluster>& lst, std::vector<Pt>& pts, double eff)
             ++q;
         }
     }
+}
+
+void refine(std::vector<Cluster>& lst, std::vector<Pt>& pts, double eff)
+{
+    std::vector<int> d2, work;
+    size_t q = 0;
+    const int nt = ocfd::host_threads();
+    // serial until there are a few clusters per thread (small point sets stay serial)
+    const size_t npts = lst.empty() ? 0 : lst[0].e - lst[0].b;
+    const size_t stop_at = (nt > 1 && npts >= 4096) ? (size_t)4 * nt : (size_t)-1;
+    refine_serial(lst, pts, eff, stop_at, d2, work, q);
+    if (q >= lst.size()) return;
+    std::vector<size_t> todo;
+    for (size_t c = q; c < lst.size(); ++c)
+        if (lst[c].eff() < eff) todo.push_back(c);
+    std::vector<std::vector<Cluster>> sub(todo.size());
+    ocfd::parallel_for((int)todo.size(), [&](int t) {
+        std::vector<int> d2t, workt;
+        size_t qt = 0;
+        sub[t].push_back(std::move(lst[todo[t]]));
+        refine_serial(sub[t], pts, eff, (size_t)-1, d2t, workt, qt);
+    });
+    std::vector<Cluster> out;
+    out.reserve(lst.size());
+    size_t k = 0;
+    for (size_t c = 0; c < lst.size(); ++c) {
+        if (k < todo.size() && todo[k] == c) {
+            for (Cluster& x : sub[k]) out.push_back(std::move(x));
+            ++k;
+        } else {
+            out.push_back(std::move(lst[c]));
+        }
+    }
+    lst.swap(out);
 }
 
 // join boxes that share their extent in two directions and touch in the

@@ -16,6 +16,7 @@ machinery, so the oracle is pinned by what the mathematics fixes:
   for bit, which pins the subcycled stage sequence and time interpolation.
 """
 import ctypes
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -479,3 +480,37 @@ def test_box_layout_clustering_equals_dense(block):
             got = sorted(cluster_box_tags(boxes, flags, extra, not full, block, eff, mx))
             assert got == want, (trial, block, eff)
             assert got == sorted(ao.cluster(dense, (0, 0, 0), block, eff, mx, mask)), (trial, block, eff)
+
+
+def test_clustering_on_host_threads_matches_serial():
+    """Large tag sets are refined on the library's host threads: the boxes are
+    those of the serial refinement (OCFD_HOST_THREADS=1, a process of its own)
+    and of the oracle."""
+    import subprocess
+    import sys
+    rng = np.random.default_rng(5)
+    shape = (4, 256, 512)
+    d = np.zeros(shape, np.uint8)
+    for _ in range(6):
+        x0, sl, w = int(rng.integers(0, shape[2])), rng.uniform(-1.5, 1.5), int(rng.integers(3, 9))
+        for j in range(shape[1]):
+            x = int(x0 + sl * j)
+            if 0 <= x < shape[2] - w:
+                d[:, j, x:x + w] = 1
+    assert d.sum() > 4096 * 2
+    code = ("import sys, numpy as np; sys.path.insert(0, %r); from ocfd.amr import cluster_tags; "
+            "d = np.load(sys.argv[1]); print(sorted(cluster_tags(d, (0, 0, 0), 1, 0.7, 32)))"
+            % str(Path(__file__).resolve().parent.parent / "opencfd-amr_amd"))
+    import tempfile, os
+    with tempfile.TemporaryDirectory() as td:
+        f = os.path.join(td, "d.npy")
+        np.save(f, d)
+        outs = []
+        for nt in ("1", "8"):
+            env = dict(os.environ, OCFD_HOST_THREADS=nt)
+            r = subprocess.run([sys.executable, "-c", code, f], capture_output=True, text=True, env=env, timeout=120)
+            assert r.returncode == 0, r.stderr
+            outs.append(r.stdout)
+    assert outs[0] == outs[1]
+    want = str(sorted(ao.cluster(d, (0, 0, 0), 1, 0.7, 32))) + "\n"
+    assert outs[0] == want
