@@ -409,6 +409,27 @@ def test_three_level_dmr_matches_oracle(torch_cuda, orc):
     _compare_hier(T, O, TOL64)
 
 
+def test_three_level_dmr_uneven_planar_matches_oracle(torch_cuda, orc):
+    """The bench's hierarchy at a quarter of its base: DMR 256x64x2 in seeded
+    uneven boxes of 16-64 cells, max_level 2, regrid every 2 steps of each
+    level, blocking factor 2, max_grid_size 64, planar stepping (one-layer
+    ghost fills, first-layer sweeps) against the oracle's full 3D run: grids
+    of every level identical, states within 1e-10 after 4 coarse steps (two
+    regrids of each level)."""
+    import ocfd
+    from ocfd import amr, cases
+    boxes, g, kind = cases.dmr(256, 64, 2, sizes=(16, 32, 64), seed=1234)
+    kw = dict(scheme=2, max_level=2, refine_dengrad=0.05, regrid_int=2, blocking_factor=2, grid_eff=0.7,
+              max_grid_size=64)
+    T = amr.Hierarchy(boxes, g, ocfd.default_parm(), 0.5, kind, flags=ocfd.PLANAR, **kw)
+    O = ao.Hierarchy(boxes, gu.to_ogeom(g), orc.parm(), 0.5, kind, **kw)
+    for s in range(4):
+        dt_g, dt_o = T.step(), O.step()
+        assert abs(dt_g - dt_o) <= 1e-12 * dt_o, (s, dt_g, dt_o)
+    assert T.finest == 2 and len(T.levels[2].boxes) > 8
+    _compare_hier(T, O, TOL64)
+
+
 def test_three_level_viscous_periodic_conserves_and_matches(torch_cuda, orc):
     import ocfd
     from ocfd import amr, cases
