@@ -949,8 +949,8 @@ int ocfd_cf_interp_fine(ocfd_cf c, double* S_fine, const double* S_coarse, ocfd_
                 return OCFD_EINVAL;
             }
     }
-    if (F.ntile_c)
-        hipLaunchKernelGGL(k_interp_valid, dim3(F.ntile_c), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_tile_c,
+    if (F.ntile_c1)
+        hipLaunchKernelGGL(k_interp_valid, dim3(F.ntile_c1), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_tile_c1,
                            S_fine, C.d_boxes, C.d_geom, c->d_cmap, S_coarse, old_fine ? old_fine->dev.d_boxes : nullptr,
                            old_fine ? old_fine->dev.d_geom : nullptr, (int)(C.fill_gz == 1));
     return launched("ocfd_cf_interp_fine");
@@ -969,8 +969,8 @@ int ocfd_copy_level(ocfd_level dst, double* S_dst, ocfd_level src, const double*
             set_error("ocfd_copy_level: the levels must share the domain");
             return OCFD_EINVAL;
         }
-    if (N.ntile_c)
-        hipLaunchKernelGGL(k_copy_level, dim3(N.ntile_c), dim3(256), 0, (hipStream_t)stream, N.d_boxes, N.d_tile_c,
+    if (N.ntile_c1)
+        hipLaunchKernelGGL(k_copy_level, dim3(N.ntile_c1), dim3(256), 0, (hipStream_t)stream, N.d_boxes, N.d_tile_c1,
                            S_dst, O.d_boxes, O.d_geom, S_src);
     return launched("ocfd_copy_level");
 }
@@ -982,8 +982,8 @@ int ocfd_tag_dengrad(ocfd_level lev, const double* S, double threshold, unsigned
         return OCFD_EINVAL;
     }
     const LevelDev& L = lev->dev;
-    if (L.ntile_c)
-        hipLaunchKernelGGL(k_tag, dim3(L.ntile_c), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_tile_c, S,
+    if (L.ntile_c1)
+        hipLaunchKernelGGL(k_tag, dim3(L.ntile_c1), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_tile_c1, S,
                            threshold, (uint8_t*)tags, (int)(L.fill_gz == 1));
     return launched("ocfd_tag_dengrad");
 }
@@ -996,9 +996,9 @@ int ocfd_tag_dengrad_nest(ocfd_level lev, const double* S, double threshold, int
         return OCFD_EINVAL;
     }
     const LevelDev& L = lev->dev;
-    if (L.ntile_c)
-        hipLaunchKernelGGL(k_tag_nest, dim3(L.ntile_c), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_geom,
-                           L.d_tile_c, S, threshold, n_proper, (uint8_t*)flags, (int)(L.fill_gz == 1));
+    if (L.ntile_c1)
+        hipLaunchKernelGGL(k_tag_nest, dim3(L.ntile_c1), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_geom,
+                           L.d_tile_c1, S, threshold, n_proper, (uint8_t*)flags, (int)(L.fill_gz == 1));
     return launched("ocfd_tag_dengrad_nest");
 }
 
@@ -1077,8 +1077,8 @@ struct Hier {
         }
         // valid cells combined in time, then the ghost cells filled at ts
         const LevelDev& L = levels[m]->dev;
-        if (L.ntile_c)
-            hipLaunchKernelGGL(k_lincomb_valid, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c,
+        if (L.ntile_c1)
+            hipLaunchKernelGGL(k_lincomb_valid, dim3(L.ntile_c1), dim3(256), 0, s, L.d_boxes, L.d_tile_c1,
                                (q.t_new - ts) / span, q.old, (ts - q.t_old) / span, q.cur, q.tmp);
         if ((rc = launched("ocfd_amr_advance_level"))) return rc;
         if ((rc = fill(m, q.tmp, ts))) return rc;

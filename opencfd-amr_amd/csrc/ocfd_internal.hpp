@@ -37,6 +37,7 @@ struct LevelDev {
     int4* d_tile_gp = nullptr; int ntile_gp = 0; // planar ghost fill: z depth 1
     int fill_gz = NG;               // z depth of the level's ghost fills (1: planar levels only)
     int4* d_tile_c = nullptr; int ntile_c = 0;   // valid-cell chunks: (box, first cell, count, 0)
+    int4* d_tile_c1 = nullptr; int ntile_c1 = 0; // valid-cell chunks of 256, one cell per thread (per-cell kernels)
     int4* d_tile_v = nullptr; int ntile_v = 0;   // valid-cell chunks of 64 (chemistry)
     int4* d_tile_p = nullptr; int ntile_p = 0;   // primitive region grow(bx,3): (box, first, count, 3)
     int4* d_tile_pp = nullptr; int ntile_pp = 0; // planar: z grown by 1 only

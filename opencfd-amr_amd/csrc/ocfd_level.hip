@@ -516,7 +516,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         return std::max(ny_t, nz_t);
     };
     while (ch > 2 && march_count(ch) < 2048) ch /= 2;
-    std::vector<int4> tx, txz, txp, ty, tz, tg, tgp, tc, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
+    std::vector<int4> tx, txz, txp, ty, tz, tg, tgp, tc, tc1, tp, tpp, tv, teq, tef[3], ted, tmf[3], tmd;
     for (int b = 0; b < nbox; ++b) {
         const BoxDesc& B = L.boxes[b];
         const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1, nz = B.hi[2] - B.lo[2] + 1;
@@ -527,6 +527,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
         const long ngp = ghost_count(B, 1);
         for (long s = 0; s < ngp; s += 256) tgp.push_back(make_int4(b, (int)s, (int)std::min<long>(256, ngp - s), 1));
         for (long s = 0; s < B.nptv; s += 2048) tc.push_back(make_int4(b, (int)s, (int)std::min<long>(2048, B.nptv - s), 0));
+        for (long s = 0; s < B.nptv; s += 256) tc1.push_back(make_int4(b, (int)s, (int)std::min<long>(256, B.nptv - s), 0));
         for (long s = 0; s < B.nptv; s += 64) tv.push_back(make_int4(b, (int)s, (int)std::min<long>(64, B.nptv - s), 0));
         if (B.type != 0) {
             if (B.type == 1) {
@@ -583,7 +584,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     int rc = OCFD_OK;
     Staging stg;
     stg.add(&L.d_map, L.owner_map); stg.add(&L.d_map_list, L.owner_list);
-    stg.add(&L.d_tile_x, tx); stg.add(&L.d_tile_xp, txp); stg.add(&L.d_tile_g, tg); stg.add(&L.d_tile_gp, tgp); stg.add(&L.d_tile_c, tc);
+    stg.add(&L.d_tile_x, tx); stg.add(&L.d_tile_xp, txp); stg.add(&L.d_tile_g, tg); stg.add(&L.d_tile_gp, tgp); stg.add(&L.d_tile_c, tc); stg.add(&L.d_tile_c1, tc1);
     stg.add(&L.d_tile_p, tp); stg.add(&L.d_tile_pp, tpp); stg.add(&L.d_tile_v, tv); stg.add(&L.d_tile_eq, teq);
     for (int d = 0; d < 3; ++d) stg.add(&L.d_tile_ef[d], tef[d]);
     stg.add(&L.d_tile_ed, ted); stg.add(&L.d_ebflag, eb_flag); stg.add(&L.d_ebgeo, eb_geo);
@@ -603,7 +604,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     }
     L.ntile_x = (int)tx.size(); L.ntile_xp = (int)txp.size(); L.ntile_y = ny3; L.ntile_z = (int)tz.size();
     L.ntile_yp = (int)ty.size() - ny3;
-    L.ntile_g = (int)tg.size(); L.ntile_gp = (int)tgp.size(); L.ntile_c = (int)tc.size();
+    L.ntile_g = (int)tg.size(); L.ntile_gp = (int)tgp.size(); L.ntile_c = (int)tc.size(); L.ntile_c1 = (int)tc1.size();
     L.ntile_p = (int)tp.size(); L.ntile_pp = (int)tpp.size(); L.ntile_v = (int)tv.size();
     L.mf_prim = L.mf_state / NC * 6;
     G.map = L.d_map;
@@ -625,7 +626,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
     L.d_prim64 = (double*)((char*)L.d_bufs + o_p64);
     L.d_red = (unsigned long long*)((char*)L.d_bufs + o_red);
     L.dev_bytes = (long)sizeof(int) * (long)(L.owner_map.size() + L.owner_list.size()) +
-                  (long)sizeof(int4) * (L.ntile_x + L.ntile_xp + L.ntile_y + L.ntile_yp + L.ntile_z + L.ntile_g + L.ntile_gp + L.ntile_c + L.ntile_p +
+                  (long)sizeof(int4) * (L.ntile_x + L.ntile_xp + L.ntile_y + L.ntile_yp + L.ntile_z + L.ntile_g + L.ntile_gp + L.ntile_c + L.ntile_c1 + L.ntile_p +
                                         L.ntile_pp + L.ntile_v) +
                   (long)sizeof(GeomDesc) + (long)sizeof(BoxDesc) * L.nbox + (long)sizeof(double) * L.mf_valid +
                   (long)(sizeof(float) + sizeof(double)) * L.mf_prim + (long)sizeof(unsigned long long) +
