@@ -899,6 +899,7 @@ int ocfd_cf_reflux(ocfd_cf c, double* S_coarse, void* stream)
         set_error("ocfd_cf_reflux: invalid arguments");
         return OCFD_EINVAL;
     }
+    tcache_clear(c->coarse->dev);
     if (c->ncell)
         hipLaunchKernelGGL(k_reflux, dim3((c->ncell + 127) / 128), dim3(128), 0, (hipStream_t)stream,
                            c->coarse->dev.d_boxes, c->d_cells, c->d_cellfaces, c->ncell, c->d_cf, c->d_regc, c->d_regf,
@@ -912,6 +913,7 @@ int ocfd_cf_avg_down(ocfd_cf c, const double* S_fine, double* S_coarse, int ncom
         set_error("ocfd_cf_avg_down: invalid arguments");
         return OCFD_EINVAL;
     }
+    tcache_clear(c->coarse->dev);
     if (c->ntile_avg)
         hipLaunchKernelGGL(k_avg_down, dim3(c->ntile_avg), dim3(256), 0, (hipStream_t)stream, c->fine->dev.d_boxes,
                            c->coarse->dev.d_boxes, c->coarse->dev.d_geom, c->d_tile_avg, S_fine, S_coarse, ncomp);
@@ -949,6 +951,7 @@ int ocfd_cf_interp_fine(ocfd_cf c, double* S_fine, const double* S_coarse, ocfd_
                 return OCFD_EINVAL;
             }
     }
+    tcache_clear(c->fine->dev);
     if (F.ntile_c1)
         hipLaunchKernelGGL(k_interp_valid, dim3(F.ntile_c1), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_tile_c1,
                            S_fine, C.d_boxes, C.d_geom, c->d_cmap, S_coarse, old_fine ? old_fine->dev.d_boxes : nullptr,
@@ -969,6 +972,7 @@ int ocfd_copy_level(ocfd_level dst, double* S_dst, ocfd_level src, const double*
             set_error("ocfd_copy_level: the levels must share the domain");
             return OCFD_EINVAL;
         }
+    tcache_clear(dst->dev);
     if (N.ntile_c1)
         hipLaunchKernelGGL(k_copy_level, dim3(N.ntile_c1), dim3(256), 0, (hipStream_t)stream, N.d_boxes, N.d_tile_c1,
                            S_dst, O.d_boxes, O.d_geom, S_src);
@@ -1040,6 +1044,11 @@ void stage_coefs(int scheme, int s, double& c_cur, double& c_old, double& c_dt)
 
 namespace {
 
+// time-combined states a level keeps for its finer levels' stages: the
+// distinct stage times of one coarse step (up to 8 for two finer levels of
+// SSP-RK3 with subcycling)
+constexpr int kTimeStates = 8;
+
 // A hierarchy during one level advance: fills stage ghosts of level l from the
 // level itself and, recursively, from the coarser levels at the same time.
 struct Hier {
@@ -1075,14 +1084,51 @@ struct Hier {
             out = q.old;        // ghosts filled at t_old by the level's first stage
             return OCFD_OK;
         }
-        // valid cells combined in time, then the ghost cells filled at ts
-        const LevelDev& L = levels[m]->dev;
+        // valid cells combined in time, then the ghost cells filled at ts; the
+        // result is kept for the other finer stages at the same time (the
+        // stages of a level's second substep start where the first one ended,
+        // and the finest level's stage times repeat for every coarser level)
+        LevelDev& L = levels[m]->dev;
+        for (auto& e : L.tcache)
+            if (e.valid && e.ts == ts && e.t_old == q.t_old && e.t_new == q.t_new && e.old == q.old &&
+                e.cur == q.cur) {
+                e.use = ++L.tcache_clock;
+                out = e.buf;
+                return OCFD_OK;
+            }
+        double* dst = q.tmp;
+        LevelDev::TimeState* slot = nullptr;
+        // at most kTimeStates entries and 2 GiB per level
+        const size_t bytes = sizeof(double) * (size_t)L.mf_state;
+        const int cap = (int)std::min<size_t>(kTimeStates, ((size_t)2 << 30) / std::max<size_t>(bytes, 1));
+        if ((int)L.tcache.size() < cap) {
+            void* b = nullptr;
+            if (dev_alloc(&b, bytes)) {
+                L.tcache.emplace_back();
+                L.tcache.back().buf = (double*)b;
+                slot = &L.tcache.back();
+            }
+        }
+        if (!slot && !L.tcache.empty()) {   // the least recently used entry
+            slot = &L.tcache[0];
+            for (auto& e : L.tcache)
+                if (e.use < slot->use) slot = &e;
+        }
+        if (slot) {
+            slot->valid = false;
+            dst = slot->buf;
+        }
         if (L.ntile_c1)
             hipLaunchKernelGGL(k_lincomb_valid, dim3(L.ntile_c1), dim3(256), 0, s, L.d_boxes, L.d_tile_c1,
-                               (q.t_new - ts) / span, q.old, (ts - q.t_old) / span, q.cur, q.tmp);
+                               (q.t_new - ts) / span, q.old, (ts - q.t_old) / span, q.cur, dst);
         if ((rc = launched("ocfd_amr_advance_level"))) return rc;
-        if ((rc = fill(m, q.tmp, ts))) return rc;
-        out = q.tmp;
+        if ((rc = fill(m, dst, ts))) return rc;
+        if (slot) {
+            slot->ts = ts; slot->t_old = q.t_old; slot->t_new = q.t_new; slot->old = q.old; slot->cur = q.cur;
+            slot->valid = true;
+            slot->use = ++L.tcache_clock;
+        }
+        out = dst;
         return OCFD_OK;
     }
 };
@@ -1109,6 +1155,7 @@ extern "C" int ocfd_amr_advance_level(int nlev, const ocfd_level* levels, const 
     hipStream_t s = (hipStream_t)stream;
     Hier h{nlev, levels, cfs, st, s};
     ocfd_amr_level_state& q = st[l];
+    tcache_clear(levels[l]->dev);   // the level's states change now
     std::swap(q.old, q.cur);
     q.t_old = t;
     q.t_new = t + dt;

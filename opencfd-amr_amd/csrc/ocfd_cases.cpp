@@ -171,6 +171,7 @@ int ocfd_init_state(ocfd_level lev, int kind, double* S, const ocfd_parm* parm, 
         ocfd::set_error("ocfd_init_state: invalid arguments");
         return OCFD_EINVAL;
     }
+    ocfd::tcache_clear(lev->dev);   // new data in the level's states
     const ocfd::LevelDev& L = lev->dev;
     std::vector<double> h((size_t)L.mf_state, 0.0);
     if (kind == 0) init_sod(L, *parm, h);

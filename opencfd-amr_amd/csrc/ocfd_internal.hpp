@@ -50,6 +50,21 @@ struct LevelDev {
     long mf_state = 0;              // doubles in a 4-ghost 5-comp MultiFab
     long dev_bytes = 0;             // device memory held by the level
     int march_ch = 32;              // cells per march chunk of the y and z sweeps
+    // The level's state combined in time and ghost-filled at a stage time of
+    // a finer level (one entry per stage time, reused by the finer stages at
+    // the same time; ocfd_amr.hip).  Valid while the level's old and current
+    // states are unchanged: cleared when the level advances, is refluxed or
+    // averaged down, or gets new data.
+    struct TimeState {
+        double ts = 0.0, t_old = 0.0, t_new = 0.0;
+        const double* old = nullptr;
+        const double* cur = nullptr;
+        double* buf = nullptr;
+        bool valid = false;
+        unsigned long use = 0;
+    };
+    std::vector<TimeState> tcache;
+    unsigned long tcache_clock = 0;
     int live_graphs = 0;            // instantiated step graphs holding this level's work-lists and sizes
     // embedded boundary (cut boxes only, grown layout, box order)
     bool has_eb = false;
@@ -71,6 +86,12 @@ struct LevelDev {
 };
 
 void launch_eb(const LevelDev& L, const StageArgs& a, hipStream_t s);
+
+// drops the level's time-combined states (their buffers are kept for reuse)
+inline void tcache_clear(LevelDev& L)
+{
+    for (auto& e : L.tcache) e.valid = false;
+}
 
 // chemistry of a level (ocfd_chem.hip): the linearised-implicit step of dt
 // (or, under device time control, of (0.5 ctl->dt)/iter) and the species

@@ -298,6 +298,7 @@ static int upload(T** dptr, const std::vector<T>& h)
 // tables themselves.
 static void free_level(LevelDev& L)
 {
+    for (auto& e : L.tcache) dev_free({e.buf});
     // tables inside the arena are not freed on their own
     dev_free({L.d_arena, L.d_bufs, L.own_tile_yz ? (void*)L.d_tile_y : nullptr,
               L.own_tile_yz ? (void*)L.d_tile_z : nullptr, L.d_ebq, L.d_ebF[0], L.d_ebF[1], L.d_ebF[2], L.d_mF[0],

@@ -552,6 +552,7 @@ extern "C" int ocfd_checkpoint_read_level(const char* dir, const char* name, ocf
                                           void* stream)
 {
     if (!dir || !name || !lev || !S || ncomp <= 0) return fail("ocfd_checkpoint_read_level: invalid arguments");
+    tcache_clear(lev->dev);   // new data in the level's states
     const LevelDev& L = lev->dev;
     const std::string base = std::string(dir) + "/" + name;
     std::ifstream C(base + "_H");
