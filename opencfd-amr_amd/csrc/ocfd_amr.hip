@@ -88,21 +88,49 @@ __device__ __forceinline__ void face_flux(const FT* __restrict__ P, long s, cons
 struct CFFace {
     int box, i, j, k;
     int dir, up;
-    int pad0, pad1;
+    int fbox;   // coarse faces: the fine box across the face
+    int pad1;
 };
 
 // reg[f] += w * F(face f) for every listed face; one thread per face, each
 // slot written by one thread per launch, so the sums are deterministic
-template <class FT, bool VISC>
 // zrep: a z-uniform level whose sweeps computed the primitives of the first
-// z layer only; every x or y face takes the flux of its first-layer twin
+// z layer only; every x or y face takes the flux of its first-layer twin.
+// FINE: the fine side, 4 sub-faces per listed coarse face (thread 4f + s):
+// sub-face s of face f is the low or high face (across the interface) of
+// fine cell (2I, 2J, 2K) + (a1, a2) in the two tangential directions of the
+// first fine layer beyond the coarse cell, a1 = s & 1, a2 = s >> 1, wrapped
+// to the fine domain on periodic sides, in the fine box c.fbox.
+__device__ __forceinline__ int wrap_dev(const GeomDesc& G, int d, int v)
+{
+    if (!G.periodic[d] || (v >= G.dom_lo[d] && v <= G.dom_hi[d])) return v;
+    const int len = G.dom_hi[d] - G.dom_lo[d] + 1;
+    return G.dom_lo[d] + (((v - G.dom_lo[d]) % len) + len) % len;
+}
+
+template <class FT, bool VISC, bool FINE>
 __global__ __launch_bounds__(128) void k_cf_flux(const BoxDesc* __restrict__ boxes, const CFFace* __restrict__ faces,
                                                  int nface, const void* prim, PhysArgs ph, double w,
-                                                 double* __restrict__ reg, int zrep)
+                                                 double* __restrict__ reg, int zrep,
+                                                 const GeomDesc* __restrict__ fgeom)
 {
     const int f = blockIdx.x * 128 + threadIdx.x;
     if (f >= nface) return;
-    const CFFace c = faces[f];
+    CFFace c = faces[FINE ? f >> 2 : f];
+    if (FINE) {
+        const int d = c.dir, side = c.up;
+        int nb[3] = {c.i, c.j, c.k};
+        nb[d] += side ? 1 : -1;
+        int g[3] = {2 * c.i, 2 * c.j, 2 * c.k};
+        g[d] = side ? 2 * nb[d] : 2 * nb[d] + 1;
+        const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+        g[t1] += f & 1;
+        g[t2] += (f >> 1) & 1;
+        const GeomDesc& G = *fgeom;
+        c.box = c.fbox;
+        c.i = wrap_dev(G, 0, g[0]); c.j = wrap_dev(G, 1, g[1]); c.k = wrap_dev(G, 2, g[2]);
+        c.up = side ? 0 : 1;   // fine level above the face: its low face
+    }
     const BoxDesc B = boxes[c.box];
     const int i = c.i + (c.up && c.dir == 0), j = c.j + (c.up && c.dir == 1);
     const int k = zrep && c.dir != 2 ? B.lo[2] : c.k + (c.up && c.dir == 2);
@@ -504,7 +532,6 @@ struct ocfd_cf_s {
     ocfd_level coarse = nullptr, fine = nullptr;
     int nface = 0, ncell = 0;
     CFFace* d_cf = nullptr;      // coarse faces
-    CFFace* d_ff = nullptr;      // 4 fine sub-faces per coarse face
     int4* d_cells = nullptr;     // coarse cells next to the fine level (box, i, j, k)
     int* d_cellfaces = nullptr;  // 6 face indices per cell, -1 = none
     double* d_regc = nullptr;    // 5 per coarse face
@@ -686,7 +713,7 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
         cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     }
     struct FaceLists {
-        std::vector<CFFace> cf, ff;
+        std::vector<CFFace> cf;
         std::vector<int4> cells;
         std::vector<int> cellfaces;
     };
@@ -695,7 +722,6 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
         FaceLists& P = part[t];
         const size_t q0 = cand.size() * t / nth, q1 = cand.size() * (t + 1) / nth;
         P.cf.reserve(2 * (q1 - q0));
-        P.ff.reserve(8 * (q1 - q0));
         for (size_t qi = q0; qi < q1; ++qi) {
             const auto& q = cand[qi];
             const int b = q[0], K = q[1], J = q[2], I = q[3];
@@ -708,23 +734,9 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
                     if (fb < 0) continue;
                     CFFace f{};
                     f.box = b; f.i = I; f.j = J; f.k = K; f.dir = d; f.up = side;
+                    f.fbox = fb;   // covers the coarse neighbour; its 4 sub-faces are found on the device
                     fidx[2 * d + side] = (int)P.cf.size();   // local: offset when the parts are joined
                     P.cf.push_back(f);
-                    // the 4 fine cells across the face: the first fine layer of the neighbour
-                    int fc[3] = {2 * I, 2 * J, 2 * K};
-                    fc[d] = side ? 2 * nb[d] : 2 * nb[d] + 1;
-                    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-                    for (int a2 = 0; a2 < 2; ++a2)
-                        for (int a1 = 0; a1 < 2; ++a1) {
-                            int g[3] = {fc[0], fc[1], fc[2]};
-                            g[t1] += a1;
-                            g[t2] += a2;
-                            for (int e = 0; e < 3; ++e) g[e] = wrap(F.geom, e, g[e]);
-                            CFFace h{};   // in fine box fb, which covers the coarse neighbour
-                            h.box = fb; h.i = g[0]; h.j = g[1]; h.k = g[2]; h.dir = d;
-                            h.up = side ? 0 : 1;   // fine level above the face: its low face
-                            P.ff.push_back(h);
-                        }
                 }
             bool any = false;
             for (int m = 0; m < 6; ++m) any = any || fidx[m] >= 0;
@@ -734,7 +746,7 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
             }
         }
     });
-    std::vector<CFFace> cf, ff;
+    std::vector<CFFace> cf;
     std::vector<int4> cells;
     std::vector<int> cellfaces;
     for (int t = 0; t < nth; ++t) {
@@ -742,7 +754,6 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
         for (int& v : part[t].cellfaces)
             if (v >= 0) v += off;
         cf.insert(cf.end(), part[t].cf.begin(), part[t].cf.end());
-        ff.insert(ff.end(), part[t].ff.begin(), part[t].ff.end());
         cells.insert(cells.end(), part[t].cells.begin(), part[t].cells.end());
         cellfaces.insert(cellfaces.end(), part[t].cellfaces.begin(), part[t].cellfaces.end());
     }
@@ -782,7 +793,7 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
     c->ntile_avg = (int)ta.size();
     int rc;
     Staging stg;
-    stg.add(&c->d_cf, cf); stg.add(&c->d_ff, ff); stg.add(&c->d_cells, cells); stg.add(&c->d_cellfaces, cellfaces);
+    stg.add(&c->d_cf, cf); stg.add(&c->d_cells, cells); stg.add(&c->d_cellfaces, cellfaces);
     stg.add(&c->d_tile_avg, ta); stg.add(&c->d_cmap, cmap);
     stg.reserve(&c->d_regc, 5 * (size_t)c->nface);
     stg.reserve(&c->d_regf, 20 * (size_t)c->nface);
@@ -843,7 +854,8 @@ int ocfd_cf_add(ocfd_cf c, int fine_side, double w_dt, const ocfd_parm* parm, in
     }
     if (!c->nface) return OCFD_OK;
     const LevelDev& L = fine_side ? c->fine->dev : c->coarse->dev;
-    const CFFace* faces = fine_side ? c->d_ff : c->d_cf;
+    const CFFace* faces = c->d_cf;   // the fine side derives its 4 sub-faces per face
+    const GeomDesc* fg = c->fine->dev.d_geom;
     double* reg = fine_side ? c->d_regf : c->d_regc;
     const int n = fine_side ? 4 * c->nface : c->nface;
     const PhysArgs ph = phys_of(*parm);
@@ -852,13 +864,21 @@ int ocfd_cf_add(ocfd_cf c, int fine_side, double w_dt, const ocfd_parm* parm, in
     const dim3 g((n + 127) / 128), blk(128);
     const int zrep = (flags & OCFD_PLANAR) && L.fill_gz == 1;   // as the level's planar sweeps
     hipStream_t s = (hipStream_t)stream;
+#define OCFD_CF(FT, V)                                                                                   \
+    do {                                                                                                 \
+        if (fine_side)                                                                                   \
+            hipLaunchKernelGGL((k_cf_flux<FT, V, true>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg, \
+                               zrep, fg);                                                                \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_cf_flux<FT, V, false>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg, \
+                               zrep, fg);                                                                \
+    } while (0)
     if (f32) {
-        if (visc) hipLaunchKernelGGL((k_cf_flux<float, true>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg, zrep);
-        else hipLaunchKernelGGL((k_cf_flux<float, false>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg, zrep);
+        if (visc) OCFD_CF(float, true); else OCFD_CF(float, false);
     } else {
-        if (visc) hipLaunchKernelGGL((k_cf_flux<double, true>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg, zrep);
-        else hipLaunchKernelGGL((k_cf_flux<double, false>), g, blk, 0, s, L.d_boxes, faces, n, prim, ph, w_dt, reg, zrep);
+        if (visc) OCFD_CF(double, true); else OCFD_CF(double, false);
     }
+#undef OCFD_CF
     return launched("ocfd_cf_add");
 }
 
