@@ -309,57 +309,28 @@ __device__ __forceinline__ double interp_value(const BoxDesc& Bc, const GeomDesc
 // interpolated coarse value at that source; runs after the level's own fill,
 // which leaves exactly those cells untouched.  Cells the case's own boundary
 // function sets (external BCs, the DMR wall override) are left alone too.
-__global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fboxes, const GeomDesc* __restrict__ fgeom,
-                                                 const int4* __restrict__ tiles, double* __restrict__ Sf,
-                                                 const BoxDesc* __restrict__ cboxes, const GeomDesc* __restrict__ cgeom,
-                                                 const int* __restrict__ cmap, const double* __restrict__ Sc, int czuni)
+__device__ __forceinline__ void fill_cf_ghost(const BoxDesc* __restrict__ fboxes, const GeomDesc& G,
+                                              double* __restrict__ Sf, const BoxDesc* __restrict__ cboxes,
+                                              const GeomDesc* __restrict__ cgeom, const int* __restrict__ cmap,
+                                              const double* __restrict__ Sc, int czuni, const BoxDesc& B, int i,
+                                              int j, int k)
 {
-    const int4 t = tiles[blockIdx.x];
-    if ((int)threadIdx.x >= t.z) return;
-    const BoxDesc B = fboxes[t.x];
-    const GeomDesc& G = *fgeom;
-    int i, j, k;
-    ghost_cell(B, t.y + threadIdx.x, t.w, i, j, k);
     if (G.probtype == 1 && j < G.dom_lo[1] && G.bc[0][1][0] == BC_EVEN) {
         // the DMR post-shock wall region x < 1/6 below y-lo is set by the case
         const double x = G.prob_lo[0] + (i + 0.5) * G.dx[0];
         if (x < 1.0 / 6.0) return;
     }
     const long dst = sidx(B, i, j, k);
-    int iv[3] = {i, j, k}, outside[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        if (G.periodic[d]) {
-            const int len = G.dom_hi[d] - G.dom_lo[d] + 1;
-            int r = (iv[d] - G.dom_lo[d]) % len;
-            if (r < 0) r += len;
-            iv[d] = G.dom_lo[d] + r;
-        }
-        outside[d] = iv[d] < G.dom_lo[d] ? -1 : (iv[d] > G.dom_hi[d] ? 1 : 0);
-    }
+    int iv[3], outside[3];
+    ghost_wrap(G, i, j, k, iv, outside);
     // the source cell is normally the same for every component (the BC maps
     // mirror or clamp alike): its owner and coarse box are looked up once
     int last[3] = {INT_MIN, INT_MIN, INT_MIN};
     int own = -1, m = -1;
     for (int n = 0; n < NC; ++n) {
-        int src[3] = {iv[0], iv[1], iv[2]};
-        double sign = 1.0;
-        bool user = false;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if (!outside[d]) continue;
-            const int bc = G.bc[n][d][outside[d] > 0];
-            const int edge = outside[d] < 0 ? G.dom_lo[d] : G.dom_hi[d];
-            if (bc == BC_FOEXT) src[d] = edge;
-            else if (bc == BC_EVEN || bc == BC_ODD) {
-                src[d] = outside[d] < 0 ? 2 * edge - src[d] - 1 : 2 * edge - src[d] + 1;
-                if (bc == BC_ODD) sign = -sign;
-            } else if (bc == BC_EXT) user = true;
-        }
-        if (user) continue;
-        if (src[0] < G.dom_lo[0] || src[0] > G.dom_hi[0] || src[1] < G.dom_lo[1] || src[1] > G.dom_hi[1] ||
-            src[2] < G.dom_lo[2] || src[2] > G.dom_hi[2])
-            continue;
+        int src[3];
+        double sign;
+        if (!bc_source(G, n, iv, outside, src, sign)) continue;
         if (src[0] != last[0] || src[1] != last[1] || src[2] != last[2]) {
             own = level_owner(G, fboxes, src[0], src[1], src[2]);
             m = own >= 0 ? -1 : interp_box(cboxes, *cgeom, cmap, src[0], src[1], src[2]);
@@ -368,6 +339,37 @@ __global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fbo
         if (own >= 0 || m < 0) continue;   // the level fill copied it / nothing to interpolate from
         Sf[dst + n * B.nptg] = sign * interp_value(cboxes[m], *cgeom, Sc, src[0], src[1], src[2], n, czuni);
     }
+}
+
+__global__ __launch_bounds__(256) void k_fill_cf(const BoxDesc* __restrict__ fboxes, const GeomDesc* __restrict__ fgeom,
+                                                 const int4* __restrict__ tiles, double* __restrict__ Sf,
+                                                 const BoxDesc* __restrict__ cboxes, const GeomDesc* __restrict__ cgeom,
+                                                 const int* __restrict__ cmap, const double* __restrict__ Sc, int czuni)
+{
+    const int4 t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.z) return;
+    const BoxDesc B = fboxes[t.x];
+    int i, j, k;
+    ghost_cell(B, t.y + threadIdx.x, t.w, i, j, k);
+    fill_cf_ghost(fboxes, *fgeom, Sf, cboxes, cgeom, cmap, Sc, czuni, B, i, j, k);
+}
+
+// the same over the fine level's list of the ghost cells the coarse/fine fill
+// may write (box, ghost index at depth gz)
+__global__ __launch_bounds__(256) void k_fill_cf_list(const BoxDesc* __restrict__ fboxes,
+                                                      const GeomDesc* __restrict__ fgeom, const int2* __restrict__ list,
+                                                      const unsigned* __restrict__ nl, int gz, double* __restrict__ Sf,
+                                                      const BoxDesc* __restrict__ cboxes,
+                                                      const GeomDesc* __restrict__ cgeom, const int* __restrict__ cmap,
+                                                      const double* __restrict__ Sc, int czuni)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= (int)*nl) return;
+    const int2 e = list[q];
+    const BoxDesc B = fboxes[e.x];
+    int i, j, k;
+    ghost_cell(B, e.y, gz, i, j, k);
+    fill_cf_ghost(fboxes, *fgeom, Sf, cboxes, cgeom, cmap, Sc, czuni, B, i, j, k);
 }
 
 // valid cells of the fine level from the coarse level (a new fine region);
@@ -948,11 +950,20 @@ int ocfd_cf_fill_fine(ocfd_cf c, double* S_fine, const double* S_coarse, void* s
     }
     const LevelDev& F = c->fine->dev;
     const LevelDev& C = c->coarse->dev;
-    int ng;
-    const int4* tg = fill_tiles(F, ng);
-    if (ng)
-        hipLaunchKernelGGL(k_fill_cf, dim3(ng), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_geom, tg,
-                           S_fine, C.d_boxes, C.d_geom, c->d_cmap, S_coarse, (int)(C.fill_gz == 1));
+    const int czuni = C.fill_gz == 1;
+    const auto* gl = ghost_list(F, 1, (hipStream_t)stream);
+    if (gl) {   // the fine level's ghost cells without a same-level source
+        if (gl->cap)
+            hipLaunchKernelGGL(k_fill_cf_list, dim3((gl->cap + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                               F.d_boxes, F.d_geom, gl->d, gl->cnt, F.fill_gz, S_fine, C.d_boxes, C.d_geom, c->d_cmap,
+                               S_coarse, czuni);
+    } else {
+        int ng;
+        const int4* tg = fill_tiles(F, ng);
+        if (ng)
+            hipLaunchKernelGGL(k_fill_cf, dim3(ng), dim3(256), 0, (hipStream_t)stream, F.d_boxes, F.d_geom, tg,
+                               S_fine, C.d_boxes, C.d_geom, c->d_cmap, S_coarse, czuni);
+    }
     return launched("ocfd_cf_fill_fine");
 }
 

@@ -220,6 +220,51 @@ __device__ __forceinline__ void ghost_cell(const BoxDesc& B, int g, int gz, int&
     i = side ? B.hi[0] + 1 + ii : B.lo[0] - NG + ii;
 }
 
+// Periodic image of cell (i,j,k) (a domain thinner than the ghost width wraps
+// more than once) and, per direction, whether it lies below (-1) or above (+1)
+// the domain.
+__device__ __forceinline__ void ghost_wrap(const GeomDesc& G, int i, int j, int k, int iv[3], int outside[3])
+{
+    iv[0] = i; iv[1] = j; iv[2] = k;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (G.periodic[d]) {
+            const int len = G.dom_hi[d] - G.dom_lo[d] + 1;
+            int r = (iv[d] - G.dom_lo[d]) % len;
+            if (r < 0) r += len;
+            iv[d] = G.dom_lo[d] + r;
+        }
+        outside[d] = iv[d] < G.dom_lo[d] ? -1 : (iv[d] > G.dom_hi[d] ? 1 : 0);
+    }
+}
+
+// Source cell and sign of component n of a ghost cell outside the domain by
+// `outside` (AMReX FilccCell: foextrap copies the boundary cell,
+// reflect_even/odd mirror about the face with sign +-1, composed when the cell
+// is outside in several directions).  False for ext_dir (the case's boundary
+// fill sets it) and for a source outside a domain thinner than the ghost width.
+__device__ __forceinline__ bool bc_source(const GeomDesc& G, int n, const int iv[3], const int outside[3], int src[3],
+                                          double& sign)
+{
+    src[0] = iv[0]; src[1] = iv[1]; src[2] = iv[2];
+    sign = 1.0;
+    bool user = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (!outside[d]) continue;
+        const int bc = G.bc[n][d][outside[d] > 0];
+        const int edge = outside[d] < 0 ? G.dom_lo[d] : G.dom_hi[d];
+        if (bc == BC_FOEXT) src[d] = edge;
+        else if (bc == BC_EVEN || bc == BC_ODD) {
+            src[d] = outside[d] < 0 ? 2 * edge - src[d] - 1 : 2 * edge - src[d] + 1;
+            if (bc == BC_ODD) sign = -sign;
+        } else if (bc == BC_EXT) user = true;
+    }
+    if (user) return false;
+    return src[0] >= G.dom_lo[0] && src[0] <= G.dom_hi[0] && src[1] >= G.dom_lo[1] && src[1] <= G.dom_hi[1] &&
+           src[2] >= G.dom_lo[2] && src[2] <= G.dom_hi[2];
+}
+
 // Primitive state used by the flux kernels
 template <class FT>
 struct Prim {

@@ -13,6 +13,8 @@
 namespace ocfd {
 namespace {
 
+// set on the pool's workers, and on a caller while its job runs: a nested
+// parallel_for from either runs serially instead of waiting for the pool
 thread_local bool t_worker = false;
 
 class Pool {
@@ -44,7 +46,9 @@ public:
             ++gen_;
         }
         cv_.notify_all();
+        t_worker = true;
         work(fn, n);
+        t_worker = false;
         // the job ends when every index ran and no worker is still inside it
         std::unique_lock<std::mutex> lk(mu_);
         done_cv_.wait(lk, [this] { return left_ == 0 && active_ == 0; });

@@ -36,6 +36,17 @@ struct LevelDev {
     int4* d_tile_g = nullptr; int ntile_g = 0;   // ghost fill: (box, first ghost cell, count, z depth NG)
     int4* d_tile_gp = nullptr; int ntile_gp = 0; // planar ghost fill: z depth 1
     int fill_gz = NG;               // z depth of the level's ghost fills (1: planar levels only)
+    // ghost lists of a level that does not cover its domain (one with
+    // coarse/fine boundaries), per fill depth (0: NG, 1: planar): [0] the
+    // cells the level fill writes, [1] the cells the coarse/fine fill may write
+    // (k_ghost_lists), each sized for every ghost cell of that depth (cap) with
+    // its length in device memory (cnt).  Built on the upload stream at level
+    // creation; launches that read them first wait for gl_ready.
+    struct GhostList { int2* d = nullptr; const unsigned* cnt = nullptr; int cap = 0; };
+    GhostList gl[2][2];
+    void* d_gl = nullptr;           // one allocation holding the lists and their lengths
+    hipEvent_t gl_ready = nullptr;
+    mutable bool gl_pending = false;
     int4* d_tile_c = nullptr; int ntile_c = 0;   // valid-cell chunks: (box, first cell, count, 0)
     int4* d_tile_c1 = nullptr; int ntile_c1 = 0; // valid-cell chunks of 256, one cell per thread (per-cell kernels)
     int4* d_tile_v = nullptr; int ntile_v = 0;   // valid-cell chunks of 64 (chemistry)
@@ -103,7 +114,8 @@ void launch_renorm(const LevelDev& L, const double* S, double* Spec, const ocfd_
 // device time control (ocfd_level.hip): the ghost fill at ctl->time +
 // toff ctl->dt; the end of a step (non-finite check of A, the CFL estimate --
 // or fixed_est when > 0 -- and the dt update); the start of a run
-void launch_fill_ctl(const LevelDev& L, double* S, const ocfd_step_ctl* ctl, double toff, hipStream_t s);
+void launch_fill_ctl(const LevelDev& L, double* S, const ocfd_step_ctl* ctl, double toff, hipStream_t s,
+                     double time = 0.0);
 void launch_step_end(const LevelDev& L, const double* A, const PhysArgs& ph, ocfd_step_ctl* ctl, double fixed_est,
                      hipStream_t s);
 void launch_ctl_start(const LevelDev& L, const double* A, const PhysArgs& ph, ocfd_step_ctl* ctl, double time,
@@ -120,6 +132,26 @@ inline const int4* fill_tiles(const LevelDev& L, int& n)
 {
     n = L.fill_gz == 1 ? L.ntile_gp : L.ntile_g;
     return L.fill_gz == 1 ? L.d_tile_gp : L.d_tile_g;
+}
+
+// the level's ghost list `which` (0 level fill, 1 coarse/fine fill) at its
+// fill depth, or null when the level keeps none (fill by the tile list); a
+// launch on stream s that reads it is ordered after the lists' build
+inline const LevelDev::GhostList* ghost_list(const LevelDev& L, int which, hipStream_t s)
+{
+    if (!L.d_gl) return nullptr;
+    if (L.gl_pending) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipEventQuery(L.gl_ready) == hipSuccess) {
+            L.gl_pending = false;
+        } else if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+            (void)hipStreamWaitEvent(s, L.gl_ready, 0);
+        } else {   // a capture may not wait on an event recorded outside it
+            (void)hipEventSynchronize(L.gl_ready);
+            L.gl_pending = false;
+        }
+    }
+    return &L.gl[L.fill_gz == 1 ? 1 : 0][which];
 }
 
 // A non-blocking stream of the current device for copies of freshly allocated

@@ -48,58 +48,20 @@ static int check_launch(const char* what)
 // the case's ext_dir fill (Exec/1_DMR/prob_bcfill.cpp:14-58).
 // With device time control the fill time is ctl->time + toff * ctl->dt and a
 // finished run fills nothing.
-__global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes, const GeomDesc* __restrict__ geoms,
-                                              const int4* __restrict__ tiles, double* __restrict__ S, double time,
-                                              const ocfd_step_ctl* __restrict__ ctl, double toff)
+__device__ __forceinline__ void fill_ghost(const BoxDesc* __restrict__ boxes, const GeomDesc& G,
+                                           double* __restrict__ S, double time, const BoxDesc& B, int i, int j, int k)
 {
-    if (ctl) {
-        if (ctl->done) return;
-        time = ctl->time + toff * ctl->dt;
-    }
-    const int4 t = tiles[blockIdx.x];
-    if ((int)threadIdx.x >= t.z) return;
-    const BoxDesc B = boxes[t.x];
-    const GeomDesc& G = geoms[B.geom];
-    int i, j, k;
-    ghost_cell(B, t.y + threadIdx.x, t.w, i, j, k);
     const long dst = sidx(B, i, j, k);
-    int iv[3] = {i, j, k};
-    int outside[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        if (G.periodic[d]) {
-            // periodic image; a domain thinner than the ghost width wraps more than once
-            const int len = G.dom_hi[d] - G.dom_lo[d] + 1;
-            int r = (iv[d] - G.dom_lo[d]) % len;
-            if (r < 0) r += len;
-            iv[d] = G.dom_lo[d] + r;
-        }
-        outside[d] = iv[d] < G.dom_lo[d] ? -1 : (iv[d] > G.dom_hi[d] ? 1 : 0);
-    }
+    int iv[3], outside[3];
+    ghost_wrap(G, i, j, k, iv, outside);
     // the BC maps mirror or clamp every component alike, so the source cell and
     // its owner are normally found once for all five
     int last[3] = {INT_MIN, INT_MIN, INT_MIN};
     long so = -1, sst = 0;
     for (int n = 0; n < NC; ++n) {
-        int src[3] = {iv[0], iv[1], iv[2]};
-        double sign = 1.0;
-        bool user = false;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if (!outside[d]) continue;
-            const int bc = G.bc[n][d][outside[d] > 0];
-            const int edge = outside[d] < 0 ? G.dom_lo[d] : G.dom_hi[d];
-            if (bc == BC_FOEXT) src[d] = edge;
-            else if (bc == BC_EVEN || bc == BC_ODD) {
-                src[d] = outside[d] < 0 ? 2 * edge - src[d] - 1 : 2 * edge - src[d] + 1;
-                if (bc == BC_ODD) sign = -sign;
-            } else if (bc == BC_EXT) user = true;
-        }
-        if (user) continue;
-        // a domain thinner than the ghost width leaves the source outside: no fill
-        if (src[0] < G.dom_lo[0] || src[0] > G.dom_hi[0] || src[1] < G.dom_lo[1] || src[1] > G.dom_hi[1] ||
-            src[2] < G.dom_lo[2] || src[2] > G.dom_hi[2])
-            continue;
+        int src[3];
+        double sign;
+        if (!bc_source(G, n, iv, outside, src, sign)) continue;
         if (src[0] != last[0] || src[1] != last[1] || src[2] != last[2]) {
             const int m = level_owner(G, boxes, src[0], src[1], src[2]);
             so = -1;
@@ -131,6 +93,87 @@ __global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes,
         if (G.bc[0][0][0] == BC_EXT && i < G.dom_lo[0])
             for (int n = 0; n < NC; ++n) S[dst + n * B.nptg] = G.left[n];
     }
+}
+
+__global__ __launch_bounds__(256) void k_fill(const BoxDesc* __restrict__ boxes, const GeomDesc* __restrict__ geoms,
+                                              const int4* __restrict__ tiles, double* __restrict__ S, double time,
+                                              const ocfd_step_ctl* __restrict__ ctl, double toff)
+{
+    if (ctl) {
+        if (ctl->done) return;
+        time = ctl->time + toff * ctl->dt;
+    }
+    const int4 t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.z) return;
+    const BoxDesc B = boxes[t.x];
+    int i, j, k;
+    ghost_cell(B, t.y + threadIdx.x, t.w, i, j, k);
+    fill_ghost(boxes, geoms[B.geom], S, time, B, i, j, k);
+}
+
+// the same over a list of ghost cells (box, ghost index at depth gz): the
+// cells of a level the level fill writes (ocfd_level's ghost lists)
+__global__ __launch_bounds__(256) void k_fill_list(const BoxDesc* __restrict__ boxes,
+                                                   const GeomDesc* __restrict__ geoms, const int2* __restrict__ list,
+                                                   const unsigned* __restrict__ n, int gz, double* __restrict__ S,
+                                                   double time, const ocfd_step_ctl* __restrict__ ctl, double toff)
+{
+    if (ctl) {
+        if (ctl->done) return;
+        time = ctl->time + toff * ctl->dt;
+    }
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= (int)*n) return;
+    const int2 e = list[q];
+    const BoxDesc B = boxes[e.x];
+    int i, j, k;
+    ghost_cell(B, e.y, gz, i, j, k);
+    fill_ghost(boxes, geoms[B.geom], S, time, B, i, j, k);
+}
+
+// Ghost lists of a level at fill depth t.w: list A, the cells the level fill
+// writes for some component (a box of the level owns the BC-mapped source) or
+// may set (outside the domain: the case's boundary fill); list B, the cells
+// the coarse/fine fill may write (for some component the mapped source lies in
+// the domain and no box of the level owns it).  A cell can be in both.  Each
+// wave appends its cells with one atomic add per list; the order within a
+// list varies from run to run, the cells a fill writes and their values do not.
+__global__ __launch_bounds__(256) void k_ghost_lists(const BoxDesc* __restrict__ boxes,
+                                                     const GeomDesc* __restrict__ geoms, const int4* __restrict__ tiles,
+                                                     int2* __restrict__ la, int2* __restrict__ lb,
+                                                     unsigned* __restrict__ counts)
+{
+    const int4 t = tiles[blockIdx.x];
+    const int lane = threadIdx.x & 63;
+    const int g = t.y + (int)threadIdx.x;
+    bool a = false, b = false;
+    if ((int)threadIdx.x < t.z) {
+        const BoxDesc B = boxes[t.x];
+        const GeomDesc& G = geoms[B.geom];
+        int i, j, k;
+        ghost_cell(B, g, t.w, i, j, k);
+        int iv[3], outside[3];
+        ghost_wrap(G, i, j, k, iv, outside);
+        a = outside[0] || outside[1] || outside[2];
+        for (int n = 0; n < NC; ++n) {
+            int src[3];
+            double sign;
+            if (!bc_source(G, n, iv, outside, src, sign)) continue;
+            if (level_owner(G, boxes, src[0], src[1], src[2]) >= 0) a = true;
+            else b = true;
+        }
+    }
+    const unsigned long long ma = __ballot(a), mb = __ballot(b);
+    unsigned ba = 0, bb = 0;
+    if (lane == 0) {
+        if (ma) ba = atomicAdd(&counts[0], (unsigned)__popcll(ma));
+        if (mb) bb = atomicAdd(&counts[1], (unsigned)__popcll(mb));
+    }
+    ba = __shfl(ba, 0);
+    bb = __shfl(bb, 0);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (a) la[ba + __popcll(ma & below)] = make_int2(t.x, g);
+    if (b) lb[bb + __popcll(mb & below)] = make_int2(t.x, g);
 }
 
 // ---------------------------------------------------------------------------
@@ -299,6 +342,8 @@ static int upload(T** dptr, const std::vector<T>& h)
 static void free_level(LevelDev& L)
 {
     for (auto& e : L.tcache) dev_free({e.buf});
+    if (L.gl_ready) hipEventDestroy(L.gl_ready);
+    dev_free({L.d_gl});
     // tables inside the arena are not freed on their own
     dev_free({L.d_arena, L.d_bufs, L.own_tile_yz ? (void*)L.d_tile_y : nullptr,
               L.own_tile_yz ? (void*)L.d_tile_z : nullptr, L.d_ebq, L.d_ebF[0], L.d_ebF[1], L.d_ebF[2], L.d_mF[0],
@@ -374,6 +419,52 @@ static void march_tiles(const LevelDev& L, int ch, std::vector<int4>& ty, std::v
         for (int c0 = 0; c0 < nc; c0 += nsub)
             for (int s = 0; s < iseg; ++s) ty.push_back(make_int4(b, s | (lw << 16), 0, (c0 * ch) | (ch << 20)));
     }
+}
+
+// The ghost lists of a new level (k_ghost_lists), for both fill depths, when
+// the level does not cover its domain (only then can a ghost cell lack a
+// same-level or physical source).  Each list has room for every ghost cell of
+// its depth; the kernels run on the upload stream, after the level's table
+// copy, and the event gl_ready marks their end (no host wait).
+static int build_ghost_lists(LevelDev& L)
+{
+    if (!L.ntile_g) return OCFD_OK;
+    long vol = 0, dom = 1;
+    for (const BoxDesc& B : L.boxes)
+        vol += (long)(B.hi[0] - B.lo[0] + 1) * (B.hi[1] - B.lo[1] + 1) * (B.hi[2] - B.lo[2] + 1);
+    for (int d = 0; d < 3; ++d) dom *= L.geom.dom_hi[d] - L.geom.dom_lo[d] + 1;
+    if (vol >= dom) return OCFD_OK;   // boxes are disjoint: the level covers its domain
+    long cap[2] = {0, 0};
+    for (const BoxDesc& B : L.boxes) {
+        cap[0] += ghost_count(B, NG);
+        cap[1] += ghost_count(B, 1);
+    }
+    const size_t bytes = 256 + sizeof(int2) * 2 * (size_t)(cap[0] + cap[1]);
+    void* mem = nullptr;
+    if (!dev_alloc(&mem, bytes)) {
+        set_error("ocfd_level_create: out of device memory for the ghost lists");
+        return OCFD_ENOMEM;
+    }
+    unsigned* cnt = (unsigned*)mem;   // 4 lengths on the first 256 bytes
+    int2* p = (int2*)((char*)mem + 256);
+    for (int d = 0; d < 2; ++d)
+        for (int w = 0; w < 2; ++w) {
+            L.gl[d][w] = LevelDev::GhostList{p, cnt + 2 * d + w, (int)cap[d]};
+            p += cap[d];
+        }
+    hipStream_t s = upload_stream();
+    OCFD_HIP(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_ghost_lists, dim3(L.ntile_g), dim3(256), 0, s, L.d_boxes, L.d_geom, L.d_tile_g, L.gl[0][0].d,
+                       L.gl[0][1].d, cnt);
+    if (L.ntile_gp)
+        hipLaunchKernelGGL(k_ghost_lists, dim3(L.ntile_gp), dim3(256), 0, s, L.d_boxes, L.d_geom, L.d_tile_gp,
+                           L.gl[1][0].d, L.gl[1][1].d, cnt + 2);
+    OCFD_HIP(hipEventCreateWithFlags(&L.gl_ready, hipEventDisableTiming));
+    OCFD_HIP(hipEventRecord(L.gl_ready, s));
+    L.d_gl = mem;
+    L.gl_pending = true;
+    L.dev_bytes += (long)bytes;
+    return check_launch("ocfd_level_create");
 }
 
 int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g, ocfd_level* out)
@@ -633,6 +724,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
                   (long)(sizeof(float) + sizeof(double)) * L.mf_prim + (long)sizeof(unsigned long long) +
                   (long)sizeof(int4) * (L.ntile_eq + L.ntile_ef[0] + L.ntile_ef[1] + L.ntile_ef[2] + L.ntile_ed) +
                   L.mf_eb * (long)(sizeof(uint32_t) + sizeof(double) * (4 + 7 + 3 * NC));
+    if ((rc = build_ghost_lists(L))) { free_level(L); delete lev; return rc; }
     *out = lev;
     return OCFD_OK;
 }
@@ -738,12 +830,7 @@ int ocfd_rk_stage(ocfd_level lev, const double* S_cur, const double* S_old, doub
 int ocfd_fill_boundary(ocfd_level lev, double* S, double time, void* stream)
 {
     if (!lev || !S) { set_error("ocfd_fill_boundary: invalid arguments"); return OCFD_EINVAL; }
-    const LevelDev& L = lev->dev;
-    int ng;
-    const int4* tg = fill_tiles(L, ng);
-    if (ng)
-        hipLaunchKernelGGL(k_fill, dim3(ng), dim3(256), 0, (hipStream_t)stream, L.d_boxes, L.d_geom, tg, S, time,
-                           (const ocfd_step_ctl*)nullptr, 0.0);
+    launch_fill_ctl(lev->dev, S, nullptr, 0.0, (hipStream_t)stream, time);
     return check_launch("ocfd_fill_boundary");
 }
 
@@ -942,11 +1029,17 @@ int ocfd_level_autotune(ocfd_level lev, const double* S, const ocfd_parm* parm, 
 
 namespace ocfd {
 
-void launch_fill_ctl(const LevelDev& L, double* S, const ocfd_step_ctl* ctl, double toff, hipStream_t s)
+void launch_fill_ctl(const LevelDev& L, double* S, const ocfd_step_ctl* ctl, double toff, hipStream_t s, double time)
 {
+    if (const auto* gl = ghost_list(L, 0, s)) {   // a level with coarse/fine boundaries: its own ghost cells only
+        if (gl->cap)
+            hipLaunchKernelGGL(k_fill_list, dim3((gl->cap + 255) / 256), dim3(256), 0, s, L.d_boxes, L.d_geom, gl->d,
+                               gl->cnt, L.fill_gz, S, time, ctl, toff);
+        return;
+    }
     int ng;
     const int4* tg = fill_tiles(L, ng);
-    if (ng) hipLaunchKernelGGL(k_fill, dim3(ng), dim3(256), 0, s, L.d_boxes, L.d_geom, tg, S, 0.0, ctl, toff);
+    if (ng) hipLaunchKernelGGL(k_fill, dim3(ng), dim3(256), 0, s, L.d_boxes, L.d_geom, tg, S, time, ctl, toff);
 }
 
 void launch_ctl_start(const LevelDev& L, const double* A, const PhysArgs& ph, ocfd_step_ctl* ctl, double time,
