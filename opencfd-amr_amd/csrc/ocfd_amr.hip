@@ -47,13 +47,28 @@ __device__ __forceinline__ void face_flux(const FT* __restrict__ P, long s, cons
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
         Prim<FT> q = ldprim<FT>(P, s + (c - 3) * ds, st);
-        sw_split<D, FT>(q, gam, fp[c], fm[c]);
+        sw_split<D, FT>(q, gam, sw_t1<FT>(ph), sw_t3<FT>(ph), fp[c], fm[c]);
     }
     FT F[5];
+    if constexpr (D == 0 && sizeof(FT) == 4) {
+        // the FP32 x sweep's form: sub-stencil quantities by centre cell
 #pragma unroll
-    for (int n = 0; n < 5; ++n)
-        F[n] = weno_p<FT>(fp[0][n], fp[1][n], fp[2][n], fp[3][n], fp[4][n]) +
-               weno_m<FT>(fm[1][n], fm[2][n], fm[3][n], fm[4][n], fm[5][n]);
+        for (int n = 0; n < 5; ++n) {
+            const WenoCentre p1 = weno_centre(fp[0][n], fp[1][n], fp[2][n]);
+            const WenoCentre p2 = weno_centre(fp[1][n], fp[2][n], fp[3][n]);
+            const WenoCentre p3 = weno_centre(fp[2][n], fp[3][n], fp[4][n]);
+            const WenoCentre m2 = weno_centre(fm[1][n], fm[2][n], fm[3][n]);
+            const WenoCentre m3 = weno_centre(fm[2][n], fm[3][n], fm[4][n]);
+            const WenoCentre m4 = weno_centre(fm[3][n], fm[4][n], fm[5][n]);
+            F[n] = wenoz_centred(p1.b0, p2.b1, p3.b2, p1.p0, p2.p1, p3.p2) +
+                   wenoz_centred(m4.b2, m3.b1, m2.b0, m4.m0, m3.p2, m2.p1);
+        }
+    } else {
+#pragma unroll
+        for (int n = 0; n < 5; ++n)
+            F[n] = weno_p<FT>(fp[0][n], fp[1][n], fp[2][n], fp[3][n], fp[4][n]) +
+                   weno_m<FT>(fm[1][n], fm[2][n], fm[3][n], fm[4][n], fm[5][n]);
+    }
     if (redo_scaled(F))   // as the sweeps do
         weno_face_scaled<FT>([&](int c, int n) { return fp[c][n]; }, [&](int c, int n) { return fm[c + 1][n]; }, F);
     if (VISC) {

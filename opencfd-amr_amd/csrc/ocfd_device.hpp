@@ -87,7 +87,20 @@ __host__ __device__ inline int level_owner(const GeomDesc& G, const BoxDesc* box
 // Parm members the kernels use, by value in the launch arguments
 struct PhysArgs {
     double gamma, smallr, smallp, Rg, C_s, T_s, kOverMu, twoThirds;
+    // constants of the Steger-Warming split, 1/(2 gamma) and
+    // (3 - gamma)/(2 (gamma - 1)), rounded as the split would form them in
+    // FP64 and in FP32 (the host's IEEE division gives the same bits), and 1/Rg
+    double sw_t1, sw_t3;
+    float sw_t1f, sw_t3f;
+    double Rinv;
 };
+
+template <class FT> __device__ __forceinline__ FT sw_t1(const PhysArgs& ph);
+template <> __device__ __forceinline__ double sw_t1<double>(const PhysArgs& ph) { return ph.sw_t1; }
+template <> __device__ __forceinline__ float sw_t1<float>(const PhysArgs& ph) { return ph.sw_t1f; }
+template <class FT> __device__ __forceinline__ FT sw_t3(const PhysArgs& ph);
+template <> __device__ __forceinline__ double sw_t3<double>(const PhysArgs& ph) { return ph.sw_t3; }
+template <> __device__ __forceinline__ float sw_t3<float>(const PhysArgs& ph) { return ph.sw_t3f; }
 
 struct StageArgs {
     const void* prim;     // primitives of scur (FT, 6 comps r,u,v,w,c,T, grown layout)
@@ -271,11 +284,32 @@ struct Prim {
     FT r, u, v, w, c, T;
 };
 
-// c2prim, Source/kernel/Kernels.H:13-40 (always FP64, then narrowed)
+// c2prim, Source/kernel/Kernels.H:13-40 (always FP64, then narrowed).  The
+// FP32-flux form, whose primitives are narrowed to FP32, takes 1/rho as the
+// hardware FP64 reciprocal refined by two Newton steps (about an FP64 ulp),
+// T as p (1/rho)(1/Rg), and c as the hardware FP32 square root of the FP64
+// product gamma Rg T (positive, far from the denormal range).  The energy
+// difference, where the cancellation is, stays in FP64; every output keeps
+// its FP32 rounding but for a last-bit difference now and then.
 template <class FT>
 __device__ __forceinline__ Prim<FT> c2prim(const double* __restrict__ s, long idx, long stride, const PhysArgs& ph)
 {
     double rho = fmax(s[idx], ph.smallr);
+    if constexpr (sizeof(FT) == 4) {
+        double r = __builtin_amdgcn_rcp(rho);
+        r = __builtin_fma(r, __builtin_fma(-rho, r, 1.0), r);
+        r = __builtin_fma(r, __builtin_fma(-rho, r, 1.0), r);
+        const double ux = s[idx + stride] * r, uy = s[idx + 2 * stride] * r, uz = s[idx + 3 * stride] * r;
+        const double kineng = 0.5 * (ux * ux + uy * uy + uz * uz);
+        const double ei = s[idx + 4 * stride] - rho * kineng;
+        const double p = fmax((ph.gamma - 1.0) * ei, ph.smallp);
+        const double T = p * (r * ph.Rinv);
+        Prim<FT> q;
+        q.r = (FT)rho; q.u = (FT)ux; q.v = (FT)uy; q.w = (FT)uz;
+        q.T = (FT)T;
+        q.c = __builtin_amdgcn_sqrtf((float)(ph.gamma * ph.Rg * T));
+        return q;
+    }
     double rhoinv = 1.0 / rho;
     double ux = s[idx + stride] * rhoinv;
     double uy = s[idx + 2 * stride] * rhoinv;
@@ -342,12 +376,13 @@ __device__ __forceinline__ void c2vel_nb<float>(const double* __restrict__ s, lo
 
 // Steger-Warming split in direction D, Source/kernel/FluxSplit.H:11-193.
 template <int D, class FT>
-__device__ __forceinline__ void sw_split(const Prim<FT>& q, FT gamma, FT* __restrict__ fp, FT* __restrict__ fm)
+__device__ __forceinline__ void sw_split(const Prim<FT>& q, FT gamma, FT t1, FT t3, FT* __restrict__ fp,
+                                         FT* __restrict__ fm)
 {
     const FT d = q.r, u = q.u, v = q.v, w = q.w, c = q.c;
-    const FT tmp1 = FT(1.0) / (FT(2.0) * gamma);
+    const FT tmp1 = t1;      // 1/(2 gamma)
     const FT tmp2 = FT(2.0) * (gamma - FT(1.0));
-    const FT tmp3 = (FT(3.0) - gamma) / (FT(2.0) * (gamma - FT(1.0)));
+    const FT tmp3 = t3;      // (3 - gamma)/(2 (gamma - 1))
     const FT un = D == 0 ? u : (D == 1 ? v : w);
     const FT uc1 = un - c;
     const FT uc2 = un + c;
@@ -446,6 +481,72 @@ __device__ __forceinline__ float wenoz_scaled(float s11, float s22, float s33, f
     const float a1 = g + u1 * u1 * h, a2 = 6.0f * (g + u2 * u2 * h), a3 = 3.0f * (g + u3 * u3 * h);
     const float inv = __builtin_amdgcn_rcpf(a1 + a2 + a3);
     return (1.0f / 6.0f) * ((a1 * inv) * v1 + (a2 * inv) * v2 + (a3 * inv) * v3);
+}
+
+// WENO-Z5 by stencil centre (FP32 form of the x sweep).  Every 3-cell
+// sub-stencil (a, b, c) of Source/kernel/Reconstruction.H:83-137 is centred on
+// one cell and serves three faces; with D2 = a - 2b + c and D1 = c - a its
+// three smoothness indicators are
+//   beta1 = 13/12 D2^2 + 1/4 D1^2,
+//   beta0 = 13/12 D2^2 + 1/4 (2 D2 + D1)^2 = beta1 + D2 (D2 + D1),
+//   beta2 = 13/12 D2^2 + 1/4 (2 D2 - D1)^2 = beta1 + D2 (D2 - D1),
+// and its candidate values (over 6) are
+//   P0 = (2a - 7b + 11c)/6 = b + 13/12 D2 + 3/4 D1,
+//   P1 = (-a + 5b + 2c)/6  = b + 1/12 D2 + 1/4 D1,
+//   P2 = (2a + 5b - c)/6   = b + 1/12 D2 - 1/4 D1,
+//   M0 = (11a - 7b + 2c)/6 = b + 13/12 D2 - 3/4 D1.
+// The face between cells f-1 and f takes, for f+, (beta0, P0) of centre f-2,
+// (beta1, P1) of f-1 and (beta2, P2) of f; for f-, (beta2, M0) of centre f+1,
+// (beta1, P2) of f and (beta0, P1) of f-1 -- the sub-stencils, weights and
+// candidates of weno_p / weno_m below.  A cell's quantities are computed
+// once instead of once per face (11 + 8 operations per cell and series
+// instead of 32 per face); the results differ from weno_p / weno_m by
+// rounding only.
+struct WenoCentre {
+    float b0, b1, b2, p0, p1, p2, m0;
+};
+
+// (explicit FMAs and no contraction, so every kernel that evaluates a face
+// this way -- the x sweep and the flux registers -- gets the same bits)
+__device__ __forceinline__ WenoCentre weno_centre(float a, float b, float c)
+{
+#pragma clang fp contract(off)
+    const float c13 = 13.0f / 12.0f;
+    const float D2 = __builtin_fmaf(-2.0f, b, a) + c, D1 = c - a;
+    const float h = 0.5f * D1;
+    WenoCentre w;
+    w.b1 = __builtin_fmaf(c13 * D2, D2, h * h);
+    w.b0 = __builtin_fmaf(D2, D2 + D1, w.b1);
+    w.b2 = __builtin_fmaf(D2, D2 - D1, w.b1);
+    const float e = __builtin_fmaf(1.0f / 12.0f, D2, b), g = __builtin_fmaf(c13, D2, b);
+    w.p0 = __builtin_fmaf(0.75f, D1, g);
+    w.m0 = __builtin_fmaf(-0.75f, D1, g);
+    w.p1 = __builtin_fmaf(0.25f, D1, e);
+    w.p2 = __builtin_fmaf(-0.25f, D1, e);
+    return w;
+}
+
+// The value of x held by lane - 1 of the wave (a DPP wave shift; lane 0
+// gets 0).  Every lane must be active where it is called.
+__device__ __forceinline__ float lane_from_below(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, false));
+}
+
+// The WENO-Z combination of three sub-stencils (indicators s11, s22, s33 with
+// weights 1, 6, 3) of candidates already divided by 6.
+__device__ __forceinline__ float wenoz_centred(float s11, float s22, float s33, float v1, float v2, float v3)
+{
+#pragma clang fp contract(off)
+    const float eps = 1e-6f;
+    const float s55 = fabsf(s11 - s33);
+    const float r1 = s55 * __builtin_amdgcn_rcpf(eps + s11), r2 = s55 * __builtin_amdgcn_rcpf(eps + s22),
+                r3 = s55 * __builtin_amdgcn_rcpf(eps + s33);
+    const float a1 = __builtin_fmaf(r1, r1, 1.0f);
+    const float a2 = 6.0f * __builtin_fmaf(r2, r2, 1.0f);
+    const float a3 = 3.0f * __builtin_fmaf(r3, r3, 1.0f);
+    const float num = __builtin_fmaf(a3, v3, __builtin_fmaf(a2, v2, a1 * v1));
+    return __builtin_amdgcn_rcpf((a1 + a2) + a3) * num;
 }
 
 // WENO-Z5 from f+ at cells (f-3..f+1) of face f, Source/kernel/Reconstruction.H:83-110

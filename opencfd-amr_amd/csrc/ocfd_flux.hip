@@ -49,8 +49,12 @@ template <class FT, bool VISC>
 __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ boxes, const int4* __restrict__ tiles,
                                                  StageArgs a)
 {
+    // FP32: WENO quantities by stencil centre, the viscous state of the
+    // face's low cell from the neighbour lane by a wave shift (no LDS)
+    constexpr bool CENTRED = sizeof(FT) == 4;
     __shared__ FT sf[4][10][64];
-    __shared__ FT sv[VISC ? 4 : 1][VISC ? 13 : 1][64];
+    __shared__ FT sv[VISC && !CENTRED ? 4 : 1][VISC && !CENTRED ? 13 : 1][64];
+    __shared__ float sq[CENTRED ? 4 : 1][CENTRED ? 8 : 1][64];
     if (ctl_done(a.ctl)) return;
     const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const BoxDesc B = boxes[t.x];
@@ -76,6 +80,10 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
     const int i = B.lo[0] - 3 + ir, j = B.lo[1] - 3 + jr;
     const bool frow = cellok && jr >= 3 && jr <= ny + 2 && k >= B.lo[2] && k <= (a.zrep ? B.lo[2] : B.hi[2]);
     const FT gam = (FT)a.ph.gamma;
+    FT fp[5], fm[5];
+    // FP32 viscous state of the lane's cell: T, u, v, w, then the tangential
+    // neighbours' (u, v) at j+1, j-1 and (u, w) at k+1, k-1, then mu
+    FT vs[13];
 
     if (cellok) {
         const long s0 = sidx(B, i, j, k);
@@ -90,11 +98,24 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
             P[p0 + 4 * st] = q.c; P[p0 + 5 * st] = q.T;
         }
         if (frow) {
-            FT fp[5], fm[5];
-            sw_split<0, FT>(q, gam, fp, fm);
+            sw_split<0, FT>(q, gam, sw_t1<FT>(a.ph), sw_t3<FT>(a.ph), fp, fm);
 #pragma unroll
             for (int n = 0; n < 5; ++n) { sf[w][n][lane] = fp[n]; sf[w][5 + n][lane] = fm[n]; }
-            if (VISC) {
+            if (VISC && CENTRED) {
+                // tangential neighbours' velocities from the state itself
+                // (their primitives are being written by other waves)
+                vs[0] = q.T; vs[1] = q.u; vs[2] = q.v; vs[3] = q.w;
+                vs[12] = sutherland(q.T, a.ph);   // once per cell, used by both faces
+                FT u, v, ww;
+                c2vel_nb<FT>(a.scur, s0 + B.nxg, st, a.ph, u, v, ww);               // (i, j+1)
+                vs[4] = u; vs[5] = v;
+                c2vel_nb<FT>(a.scur, s0 - B.nxg, st, a.ph, u, v, ww);               // (i, j-1)
+                vs[6] = u; vs[7] = v;
+                c2vel_nb<FT>(a.scur, s0 + (long)B.nxg * B.nyg, st, a.ph, u, v, ww);  // (i, k+1)
+                vs[8] = u; vs[9] = ww;
+                c2vel_nb<FT>(a.scur, s0 - (long)B.nxg * B.nyg, st, a.ph, u, v, ww);  // (i, k-1)
+                vs[10] = u; vs[11] = ww;
+            } else if (VISC) {
                 // tangential neighbours' velocities from the state itself
                 // (their primitives are being written by other waves)
                 sv[w][0][lane] = q.T; sv[w][1][lane] = q.u; sv[w][2][lane] = q.v; sv[w][3][lane] = q.w;
@@ -115,14 +136,58 @@ __global__ __launch_bounds__(256) void k_sweep_x(const BoxDesc* __restrict__ box
 
     const bool faceok = frow && lane >= 3 && lane <= 61 && ir >= 3 && ir <= nx + 3;
     FT F[5];
-    if (faceok) {
+    // FP32 viscous terms of the face, formed before the WENO part (each is
+    // one addition to its component, so the sum is the same either way) from
+    // this lane's cell and the low cell's state shifted in from lane - 1
+    FT V[5] = {0, 0, 0, 0, 0};
+    if constexpr (CENTRED && VISC) {
+        FT vl[13];
+#pragma unroll
+        for (int m = 0; m < 13; ++m) vl[m] = lane_from_below(vs[m]);
+        Prim<FT> L, R;
+        L.T = vl[0]; L.u = vl[1]; L.v = vl[2]; L.w = vl[3];
+        R.T = vs[0]; R.u = vs[1]; R.v = vs[2]; R.w = vs[3];
+        const FT ta = lat(vs[4], vl[4], vs[6], vl[6]);
+        const FT tb = lat(vs[5], vl[5], vs[7], vl[7]);
+        const FT tc = lat(vs[8], vl[8], vs[10], vl[10]);
+        const FT td = lat(vs[9], vl[9], vs[11], vl[11]);
+        visc_face_mu<0, FT>(L, R, vl[12], vs[12], ta, tb, tc, td, B.dxinv, a.ph, V);
+    }
+    if constexpr (CENTRED) {
+        // per component: every lane forms the quantities of the sub-stencils
+        // centred on its cell and publishes those its neighbour faces take,
+        // then every lane combines the three centres of its face.  All lanes
+        // run both phases (lane indices clamped to the wave, no branches);
+        // only the faces of faceok lanes, whose 6 cells lie in their row and
+        // were split above, are used.
+        const int lm1 = max(lane - 1, 0), lm2 = max(lane - 2, 0), lp1 = min(lane + 1, 63);
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
-            F[n] = weno_p<FT>(sf[w][n][lane - 3], sf[w][n][lane - 2], sf[w][n][lane - 1], sf[w][n][lane], sf[w][n][lane + 1])
-                 + weno_m<FT>(sf[w][5 + n][lane - 2], sf[w][5 + n][lane - 1], sf[w][5 + n][lane], sf[w][5 + n][lane + 1],
-                              sf[w][5 + n][lane + 2]);
+            const WenoCentre P = weno_centre(sf[w][n][lm1], fp[n], sf[w][n][lp1]);
+            const WenoCentre M = weno_centre(sf[w][5 + n][lm1], fm[n], sf[w][5 + n][lp1]);
+            sq[w][0][lane] = P.b0; sq[w][1][lane] = P.p0; sq[w][2][lane] = P.b1; sq[w][3][lane] = P.p1;
+            sq[w][4][lane] = M.b2; sq[w][5][lane] = M.m0; sq[w][6][lane] = M.b0; sq[w][7][lane] = M.p1;
+            wave_sync();
+            F[n] = wenoz_centred(sq[w][0][lm2], sq[w][2][lm1], P.b2, sq[w][1][lm2], sq[w][3][lm1], P.p2)
+                 + wenoz_centred(sq[w][4][lp1], M.b1, sq[w][6][lm1], sq[w][5][lp1], M.p2, sq[w][7][lm1]);
+            wave_sync();
+        }
+    }
+    if (faceok) {
+        if constexpr (!CENTRED) {
+#pragma unroll
+            for (int n = 0; n < 5; ++n) {
+                F[n] = weno_p<FT>(sf[w][n][lane - 3], sf[w][n][lane - 2], sf[w][n][lane - 1], sf[w][n][lane], sf[w][n][lane + 1])
+                     + weno_m<FT>(sf[w][5 + n][lane - 2], sf[w][5 + n][lane - 1], sf[w][5 + n][lane], sf[w][5 + n][lane + 1],
+                                  sf[w][5 + n][lane + 2]);
+            }
         }
         auto add_visc = [&](FT* Fo) {
+            if constexpr (CENTRED) {
+#pragma unroll
+                for (int n = 1; n < 5; ++n) Fo[n] += V[n];
+                return;
+            }
             Prim<FT> L, R;
             L.T = sv[w][0][lane - 1]; L.u = sv[w][1][lane - 1]; L.v = sv[w][2][lane - 1]; L.w = sv[w][3][lane - 1];
             R.T = sv[w][0][lane];     R.u = sv[w][1][lane];     R.v = sv[w][2][lane];     R.w = sv[w][3][lane];
@@ -208,7 +273,7 @@ void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ t
     for (int c = 0; c < 5; ++c) {
         Prim<FT> q = ldprim<FT>(P, base + (c - 3) * dstep, st);
         FT fp[5], fm[5];
-        sw_split<D, FT>(q, gam, fp, fm);
+        sw_split<D, FT>(q, gam, sw_t1<FT>(a.ph), sw_t3<FT>(a.ph), fp, fm);
 #pragma unroll
         for (int n = 0; n < 5; ++n) Wp[c][n] = fp[n];
         if (c >= 1) {
@@ -242,7 +307,7 @@ void k_sweep_march(const BoxDesc* __restrict__ boxes, const int4* __restrict__ t
         {
             Prim<FT> q = ldprim<FT>(P, base + (long)(f + 2 - m0) * dstep, st);
             FT fp[5], fm[5];
-            sw_split<D, FT>(q, gam, fp, fm);
+            sw_split<D, FT>(q, gam, sw_t1<FT>(a.ph), sw_t3<FT>(a.ph), fp, fm);
 #pragma unroll
             for (int n = 0; n < 5; ++n) { Wp[5][n] = fp[n]; Wm[4][n] = fm[n]; }
         }

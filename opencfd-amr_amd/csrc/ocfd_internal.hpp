@@ -300,6 +300,12 @@ inline PhysArgs phys_of(const ocfd_parm& p)
     PhysArgs a;
     a.gamma = p.eos_gamma; a.smallr = p.smallr; a.smallp = p.smallp; a.Rg = p.Rg;
     a.C_s = p.C_s; a.T_s = p.T_s; a.kOverMu = p.kOverMu; a.twoThirds = p.twoThirds;
+    a.sw_t1 = 1.0 / (2.0 * a.gamma);
+    a.sw_t3 = (3.0 - a.gamma) / (2.0 * (a.gamma - 1.0));
+    const float gf = (float)a.gamma;
+    a.sw_t1f = 1.0f / (2.0f * gf);
+    a.sw_t3f = (3.0f - gf) / (2.0f * (gf - 1.0f));
+    a.Rinv = 1.0 / a.Rg;
     return a;
 }
 
