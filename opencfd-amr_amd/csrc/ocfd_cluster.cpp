@@ -124,8 +124,8 @@ void trim(Cluster& c, const IBox& parent, const int* const* h)
     }
 }
 
-// split cluster c in two; returns the high part, c keeps the low part
-Cluster chop(Cluster& c, std::vector<Pt>& pts, std::vector<int>& d2, std::vector<int>& work)
+// the cut plane of cluster c: direction and first index of the high part
+void choose_cut(const Cluster& c, std::vector<int>& d2, int& dir, int& cd)
 {
     CutKind kind[3];
     int cut[3];
@@ -134,7 +134,8 @@ Cluster chop(Cluster& c, std::vector<Pt>& pts, std::vector<int>& d2, std::vector
         cut[d] = find_cut(c.hist[d], c.box.lo[d], kind[d], d2);
         best = std::min(best, kind[d]);
     }
-    int dir = -1, minlen = -1;
+    int minlen = -1;
+    dir = -1;
     for (int d = 0; d < 3; ++d) {
         if (kind[d] != best) continue;
         const int m = std::min(cut[d] - c.box.lo[d], c.box.hi[d] - cut[d]);
@@ -143,7 +144,90 @@ Cluster chop(Cluster& c, std::vector<Pt>& pts, std::vector<int>& d2, std::vector
             minlen = m;
         }
     }
-    const int cd = cut[dir];
+    cd = cut[dir];
+}
+
+// points of a cluster from which its chop partitions on the host threads
+constexpr size_t kParChop = 16384;
+
+// chop of a large cluster on the host threads: each thread counts its slice's
+// low points and both parts' signatures, then scatters the slice into a
+// scratch array at its offsets (low parts first, slice by slice), which is
+// copied back.  The parts hold the same points as the serial chop's (in
+// another order), so their boxes, signatures and every later cut are the same.
+Cluster chop_parallel(Cluster& c, std::vector<Pt>& pts, int dir, int cd, int nt)
+{
+    const IBox parent = c.box;
+    int n[3], off[3];
+    size_t tot = 0;
+    for (int d = 0; d < 3; ++d) {
+        n[d] = parent.hi[d] - parent.lo[d] + 1;
+        off[d] = (int)tot;
+        tot += n[d];
+    }
+    const size_t N = c.e - c.b, per = (N + nt - 1) / nt;
+    std::vector<std::vector<int>> h(nt);
+    std::vector<size_t> nlo(nt, 0);
+    ocfd::parallel_for(nt, [&](int t) {
+        const size_t a = c.b + std::min(N, t * per), z = c.b + std::min(N, (t + 1) * per);
+        h[t].assign(2 * tot, 0);
+        int* lo_h = h[t].data();
+        int* hi_h = h[t].data() + tot;
+        size_t k = 0;
+        for (size_t q = a; q < z; ++q) {
+            const Pt& p = pts[q];
+            int* hh = p.v[dir] < cd ? (++k, lo_h) : hi_h;
+            for (int d = 0; d < 3; ++d) hh[off[d] + p.v[d] - parent.lo[d]]++;
+        }
+        nlo[t] = k;
+    });
+    size_t nl = 0;
+    std::vector<size_t> lo_at(nt), hi_at(nt);
+    for (int t = 0; t < nt; ++t) { lo_at[t] = nl; nl += nlo[t]; }
+    size_t nh = nl;
+    for (int t = 0; t < nt; ++t) {
+        const size_t a = std::min(N, t * per), z = std::min(N, (t + 1) * per);
+        hi_at[t] = nh;
+        nh += (z - a) - nlo[t];
+    }
+    // scratch kept between calls, per calling thread (the workers use the
+    // caller's through this reference)
+    static thread_local std::vector<Pt> scratch;
+    if (scratch.size() < N) scratch.resize(N);
+    std::vector<Pt>& tmp = scratch;
+    ocfd::parallel_for(nt, [&](int t) {
+        const size_t a = c.b + std::min(N, t * per), z = c.b + std::min(N, (t + 1) * per);
+        size_t l = lo_at[t], u = hi_at[t];
+        for (size_t q = a; q < z; ++q) {
+            if (pts[q].v[dir] < cd) tmp[l++] = pts[q];
+            else tmp[u++] = pts[q];
+        }
+    });
+    ocfd::parallel_for(nt, [&](int t) {
+        const size_t a = std::min(N, t * per), z = std::min(N, (t + 1) * per);
+        std::copy(tmp.begin() + a, tmp.begin() + z, pts.begin() + c.b + a);
+    });
+    std::vector<int> sum(2 * tot, 0);
+    for (int t = 0; t < nt; ++t)
+        for (size_t q = 0; q < 2 * tot; ++q) sum[q] += h[t][q];
+    Cluster hi;
+    hi.b = c.b + nl;
+    hi.e = c.e;
+    c.e = c.b + nl;
+    const int* lh[3] = {sum.data() + off[0], sum.data() + off[1], sum.data() + off[2]};
+    const int* hh[3] = {sum.data() + tot + off[0], sum.data() + tot + off[1], sum.data() + tot + off[2]};
+    trim(c, parent, lh);
+    trim(hi, parent, hh);
+    return hi;
+}
+
+// split cluster c in two; returns the high part, c keeps the low part
+Cluster chop(Cluster& c, std::vector<Pt>& pts, std::vector<int>& d2, std::vector<int>& work)
+{
+    int dir, cd;
+    choose_cut(c, d2, dir, cd);
+    const int nt = ocfd::host_threads();
+    if (nt > 1 && c.e - c.b >= kParChop) return chop_parallel(c, pts, dir, cd, nt);
     const IBox parent = c.box;
     int n[3], off[3];
     size_t tot = 0;
@@ -359,18 +443,25 @@ void cluster_points(std::vector<Pt>& all, bool has_mask, InBox&& in_box, int blo
     lst[0].scan(all);
     refine(lst, all, grid_eff);
     std::vector<IBox> kept;
-    std::vector<Pt> in;
-    for (const Cluster& c : lst) {
-        if (!has_mask) { kept.push_back(c.box); continue; }
-        in.clear();
-        if (in_box(c.box, in)) { kept.push_back(c.box); continue; }
-        if (in.empty()) continue;
-        std::vector<Cluster> part(1);
-        part[0].b = 0;
-        part[0].e = in.size();
-        part[0].scan(in);
-        refine(part, in, 1.0);
-        for (const Cluster& q : part) kept.push_back(q.box);
+    if (!has_mask) {
+        for (const Cluster& c : lst) kept.push_back(c.box);
+    } else {
+        // each cluster cut back to the allowed region on its own (host
+        // threads), the pieces joined in cluster order
+        std::vector<std::vector<IBox>> pieces(lst.size());
+        ocfd::parallel_for((int)lst.size(), [&](int q) {
+            const Cluster& c = lst[q];
+            std::vector<Pt> in;
+            if (in_box(c.box, in)) { pieces[q].push_back(c.box); return; }
+            if (in.empty()) return;
+            std::vector<Cluster> part(1);
+            part[0].b = 0;
+            part[0].e = in.size();
+            part[0].scan(in);
+            refine(part, in, 1.0);
+            for (const Cluster& p : part) pieces[q].push_back(p.box);
+        });
+        for (const auto& v : pieces) kept.insert(kept.end(), v.begin(), v.end());
     }
     for (const IBox& kb : kept) {
         IBox b = kb;
@@ -523,25 +614,35 @@ extern "C" int ocfd_cluster_box_tags(int nbox, const int* lo, const int* hi, con
             }
         flags = fcopy.data();
     }
-    // points: blocks holding a cell flagged both tagged and allowed
-    std::vector<Pt> all;
-    all.reserve(off[nbox] / ((size_t)block * block * block) + 1);
-    for (int b = 0; b < nbox; ++b) {
+    // points: blocks holding a cell flagged both tagged and allowed, in box
+    // order; the level boxes are counted, then listed at their offsets, on
+    // the host threads
+    std::vector<size_t> at(nbox + 1, 0);
+    ocfd::parallel_for(nbox, [&](int b) {
+        const unsigned char* f = flags + off[b];
+        size_t k = 0;
+        for (size_t c = 0; c < off[b + 1] - off[b]; ++c) k += (f[c] & 3) == 3;
+        at[b + 1] = k;
+    });
+    for (int b = 0; b < nbox; ++b) at[b + 1] += at[b];
+    std::vector<Pt> all(at[nbox]);
+    ocfd::parallel_for(nbox, [&](int b) {
         const IBox& B = lv[b];
         const unsigned char* f = flags + off[b];
         const int n0 = B.hi[0] - B.lo[0] + 1;
+        Pt* out = all.data() + at[b];
         size_t c = 0;
         for (int k = B.lo[2]; k <= B.hi[2]; ++k)
             for (int j = B.lo[1]; j <= B.hi[1]; ++j, c += n0) {
                 if (block == 1) {
                     for (int i = 0; i < n0; ++i)
-                        if ((f[c + i] & 3) == 3) all.push_back(Pt{{B.lo[0] + i, j, k}});
+                        if ((f[c + i] & 3) == 3) *out++ = Pt{{B.lo[0] + i, j, k}};
                     continue;
                 }
                 for (int i = 0; i < n0; ++i)
-                    if ((f[c + i] & 3) == 3) all.push_back(Pt{{fdiv(B.lo[0] + i, block), fdiv(j, block), fdiv(k, block)}});
+                    if ((f[c + i] & 3) == 3) *out++ = Pt{{fdiv(B.lo[0] + i, block), fdiv(j, block), fdiv(k, block)}};
             }
-    }
+    });
     if (block > 1) {
         std::sort(all.begin(), all.end(), [](const Pt& a, const Pt& b) {
             return std::lexicographical_compare(a.v, a.v + 3, b.v, b.v + 3);
@@ -551,9 +652,9 @@ extern "C" int ocfd_cluster_box_tags(int nbox, const int* lo, const int* hi, con
                   all.end());
     }
     // allowed points of a cluster box: blocks whose cells all lie in the
-    // level and carry the allowed flag
-    std::vector<unsigned char> loc;
+    // level and carry the allowed flag (called from several host threads)
     auto in_box = [&](const IBox& cb, std::vector<Pt>& out) {
+        std::vector<unsigned char> loc;
         IBox cr;
         for (int d = 0; d < 3; ++d) {
             cr.lo[d] = cb.lo[d] * block;

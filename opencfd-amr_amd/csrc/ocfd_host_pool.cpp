@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstdlib>
 #include <mutex>
@@ -17,6 +18,11 @@ namespace {
 // parallel_for from either runs serially instead of waiting for the pool
 thread_local bool t_worker = false;
 
+// spin time of an idle worker before it sleeps, and of a caller before it
+// waits for the job's end: jobs come in bursts (the chops and passes of one
+// clustering), and waking a sleeping thread costs tens of microseconds
+constexpr auto kSpin = std::chrono::microseconds(100);
+
 class Pool {
 public:
     explicit Pool(int nthreads)
@@ -27,7 +33,7 @@ public:
     {
         {
             std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
+            stop_.store(true);
         }
         cv_.notify_all();
         for (std::thread& t : workers_) t.join();
@@ -42,16 +48,19 @@ public:
             fn_ = &fn;
             n_ = n;
             next_.store(0);
-            left_ = n;
-            ++gen_;
+            left_.store(n);
+            gen_.fetch_add(1);
         }
         cv_.notify_all();
         t_worker = true;
         work(fn, n);
         t_worker = false;
         // the job ends when every index ran and no worker is still inside it
+        const auto until = std::chrono::steady_clock::now() + kSpin;
+        while ((left_.load() != 0 || active_.load() != 0) && std::chrono::steady_clock::now() < until)
+            std::this_thread::yield();
         std::unique_lock<std::mutex> lk(mu_);
-        done_cv_.wait(lk, [this] { return left_ == 0 && active_ == 0; });
+        done_cv_.wait(lk, [this] { return left_.load() == 0 && active_.load() == 0; });
         fn_ = nullptr;
     }
 
@@ -62,8 +71,10 @@ private:
             const int i = next_.fetch_add(1);
             if (i >= n) return;
             fn(i);
-            std::lock_guard<std::mutex> lk(mu_);
-            if (--left_ == 0) done_cv_.notify_all();
+            if (left_.fetch_sub(1) == 1) {
+                std::lock_guard<std::mutex> lk(mu_);
+                done_cv_.notify_all();
+            }
         }
     }
     void loop()
@@ -71,21 +82,27 @@ private:
         t_worker = true;
         unsigned long seen = 0;
         for (;;) {
+            // a new job is looked for by spinning first, then by sleeping;
+            // it is taken under the lock, so the job's function and size
+            // are read together with the worker's entry into it
+            const auto until = std::chrono::steady_clock::now() + kSpin;
+            while (gen_.load() == seen && !stop_.load() && std::chrono::steady_clock::now() < until)
+                std::this_thread::yield();
             const std::function<void(int)>* fn = nullptr;
             int n = 0;
             {
                 std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
-                seen = gen_;
+                cv_.wait(lk, [&] { return stop_.load() || gen_.load() != seen; });
+                if (stop_.load()) return;
+                seen = gen_.load();
                 if (!fn_) continue;
                 fn = fn_;
                 n = n_;
-                ++active_;
+                active_.fetch_add(1);
             }
             work(*fn, n);
             std::lock_guard<std::mutex> lk(mu_);
-            --active_;
+            active_.fetch_sub(1);
             done_cv_.notify_all();
         }
     }
@@ -94,10 +111,10 @@ private:
     std::mutex mu_, job_mu_;
     std::condition_variable cv_, done_cv_;
     const std::function<void(int)>* fn_ = nullptr;
-    int n_ = 0, left_ = 0, active_ = 0;
-    std::atomic<int> next_{0};
-    unsigned long gen_ = 0;
-    bool stop_ = false;
+    int n_ = 0;
+    std::atomic<int> next_{0}, left_{0}, active_{0};
+    std::atomic<unsigned long> gen_{0};
+    std::atomic<bool> stop_{false};
 };
 
 int configured_threads()

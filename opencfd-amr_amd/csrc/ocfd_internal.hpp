@@ -37,16 +37,17 @@ struct LevelDev {
     int4* d_tile_gp = nullptr; int ntile_gp = 0; // planar ghost fill: z depth 1
     int fill_gz = NG;               // z depth of the level's ghost fills (1: planar levels only)
     // ghost lists of a level that does not cover its domain (one with
-    // coarse/fine boundaries), per fill depth (0: NG, 1: planar): [0] the
-    // cells the level fill writes, [1] the cells the coarse/fine fill may write
-    // (k_ghost_lists), each sized for every ghost cell of that depth (cap) with
-    // its length in device memory (cnt).  Built on the upload stream at level
-    // creation; launches that read them first wait for gl_ready.
+    // coarse/fine boundaries, gl_need), per fill depth (0: NG, 1: planar):
+    // [0] the cells the level fill writes, [1] the cells the coarse/fine fill
+    // may write (k_ghost_lists), each sized for every ghost cell of that depth
+    // (cap) with its length in device memory (cnt).  A depth's lists are built
+    // on the stream of the first fill at that depth (ghost_list).
     struct GhostList { int2* d = nullptr; const unsigned* cnt = nullptr; int cap = 0; };
-    GhostList gl[2][2];
-    void* d_gl = nullptr;           // one allocation holding the lists and their lengths
-    hipEvent_t gl_ready = nullptr;
-    mutable bool gl_pending = false;
+    bool gl_need = false;
+    mutable GhostList gl[2][2];
+    mutable void* d_gl[2] = {nullptr, nullptr};   // per depth: the two lists and their lengths
+    mutable hipStream_t gl_stream[2] = {nullptr, nullptr};   // the stream that built them
+    mutable hipEvent_t gl_event[2] = {nullptr, nullptr};     // recorded there after the build
     int4* d_tile_c = nullptr; int ntile_c = 0;   // valid-cell chunks: (box, first cell, count, 0)
     int4* d_tile_c1 = nullptr; int ntile_c1 = 0; // valid-cell chunks of 256, one cell per thread (per-cell kernels)
     int4* d_tile_v = nullptr; int ntile_v = 0;   // valid-cell chunks of 64 (chemistry)
@@ -134,24 +135,28 @@ inline const int4* fill_tiles(const LevelDev& L, int& n)
     return L.fill_gz == 1 ? L.d_tile_gp : L.d_tile_g;
 }
 
+// Builds the ghost lists of depth d (0: NG, 1: planar) on stream s; false
+// when they cannot be had (no device memory), the fills then use the tiles.
+bool build_ghost_lists(const LevelDev& L, int d, hipStream_t s);
+
 // the level's ghost list `which` (0 level fill, 1 coarse/fine fill) at its
-// fill depth, or null when the level keeps none (fill by the tile list); a
-// launch on stream s that reads it is ordered after the lists' build
+// fill depth, or null when the level keeps none (fill by the tile list).  The
+// lists of a depth are built at its first use, on the stream of that fill
+// (ordered before it); a fill on another stream waits for the build's event.
+// A fill under stream capture that would need either takes the tile list.
 inline const LevelDev::GhostList* ghost_list(const LevelDev& L, int which, hipStream_t s)
 {
-    if (!L.d_gl) return nullptr;
-    if (L.gl_pending) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (hipEventQuery(L.gl_ready) == hipSuccess) {
-            L.gl_pending = false;
-        } else if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
-            (void)hipStreamWaitEvent(s, L.gl_ready, 0);
-        } else {   // a capture may not wait on an event recorded outside it
-            (void)hipEventSynchronize(L.gl_ready);
-            L.gl_pending = false;
-        }
+    if (!L.gl_need) return nullptr;
+    const int d = L.fill_gz == 1 ? 1 : 0;
+    if (L.d_gl[d] && s == L.gl_stream[d]) return &L.gl[d][which];
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+    if (!L.d_gl[d]) {
+        if (!build_ghost_lists(L, d, s)) return nullptr;
+    } else if (hipStreamWaitEvent(s, L.gl_event[d], 0) != hipSuccess) {
+        return nullptr;
     }
-    return &L.gl[L.fill_gz == 1 ? 1 : 0][which];
+    return &L.gl[d][which];
 }
 
 // A non-blocking stream of the current device for copies of freshly allocated

@@ -155,12 +155,25 @@ __global__ __launch_bounds__(256) void k_ghost_lists(const BoxDesc* __restrict__
         int iv[3], outside[3];
         ghost_wrap(G, i, j, k, iv, outside);
         a = outside[0] || outside[1] || outside[2];
-        for (int n = 0; n < NC; ++n) {
-            int src[3];
-            double sign;
-            if (!bc_source(G, n, iv, outside, src, sign)) continue;
-            if (level_owner(G, boxes, src[0], src[1], src[2]) >= 0) a = true;
+        if (!a) {
+            // inside the domain every component's source is the wrapped cell
+            if (level_owner(G, boxes, iv[0], iv[1], iv[2]) >= 0) a = true;
             else b = true;
+        } else {
+            // outside: one owner lookup per distinct source cell
+            int last[3] = {0, 0, 0};
+            bool have = false, own = false;
+            for (int n = 0; n < NC; ++n) {
+                int src[3];
+                double sign;
+                if (!bc_source(G, n, iv, outside, src, sign)) continue;
+                if (!have || src[0] != last[0] || src[1] != last[1] || src[2] != last[2]) {
+                    own = level_owner(G, boxes, src[0], src[1], src[2]) >= 0;
+                    last[0] = src[0]; last[1] = src[1]; last[2] = src[2];
+                    have = true;
+                }
+                if (!own) b = true;
+            }
         }
     }
     const unsigned long long ma = __ballot(a), mb = __ballot(b);
@@ -342,13 +355,60 @@ static int upload(T** dptr, const std::vector<T>& h)
 static void free_level(LevelDev& L)
 {
     for (auto& e : L.tcache) dev_free({e.buf});
-    if (L.gl_ready) hipEventDestroy(L.gl_ready);
-    dev_free({L.d_gl});
+    for (hipEvent_t e : L.gl_event)
+        if (e) (void)hipEventDestroy(e);
+    dev_free({L.d_gl[0], L.d_gl[1]});
     // tables inside the arena are not freed on their own
     dev_free({L.d_arena, L.d_bufs, L.own_tile_yz ? (void*)L.d_tile_y : nullptr,
               L.own_tile_yz ? (void*)L.d_tile_z : nullptr, L.d_ebq, L.d_ebF[0], L.d_ebF[1], L.d_ebF[2], L.d_mF[0],
               L.d_mF[1], L.d_mF[2], L.d_mq});
     L = LevelDev{};
+}
+
+__global__ void k_zero_u32x2(unsigned* p) { p[threadIdx.x] = 0u; }
+
+// The ghost lists of depth d (k_ghost_lists) of a level that does not cover
+// its domain, each with room for every ghost cell of that depth, built on
+// stream s: the lengths cleared by a kernel, then one classification pass.
+bool build_ghost_lists(const LevelDev& L, int d, hipStream_t s)
+{
+    const int4* tiles = d ? L.d_tile_gp : L.d_tile_g;
+    const int ntile = d ? L.ntile_gp : L.ntile_g;
+    if (!ntile) return false;
+    long cap = 0;
+    for (const BoxDesc& B : L.boxes) cap += ghost_count(B, d ? 1 : NG);
+    const size_t bytes = 256 + sizeof(int2) * 2 * (size_t)cap;
+    void* mem = nullptr;
+    if (!dev_alloc(&mem, bytes)) return false;
+    unsigned* cnt = (unsigned*)mem;   // the two lengths on the first 256 bytes
+    int2* p = (int2*)((char*)mem + 256);
+    L.gl[d][0] = LevelDev::GhostList{p, cnt, (int)cap};
+    L.gl[d][1] = LevelDev::GhostList{p + cap, cnt + 1, (int)cap};
+    hipLaunchKernelGGL(k_zero_u32x2, dim3(1), dim3(2), 0, s, cnt);
+    hipLaunchKernelGGL(k_ghost_lists, dim3(ntile), dim3(256), 0, s, L.d_boxes, L.d_geom, tiles, L.gl[d][0].d,
+                       L.gl[d][1].d, cnt);
+    hipEvent_t ev = nullptr;
+    if (hipGetLastError() != hipSuccess || hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(ev, s) != hipSuccess) {
+        if (ev) (void)hipEventDestroy(ev);
+        dev_free({mem});
+        return false;
+    }
+    L.d_gl[d] = mem;
+    L.gl_stream[d] = s;
+    L.gl_event[d] = ev;
+    return true;
+}
+
+// whether a level needs ghost lists: its boxes (disjoint) do not cover the
+// domain, so some ghost cells may have no same-level or physical source
+static void set_ghost_need(LevelDev& L)
+{
+    long vol = 0, dom = 1;
+    for (const BoxDesc& B : L.boxes)
+        vol += (long)(B.hi[0] - B.lo[0] + 1) * (B.hi[1] - B.lo[1] + 1) * (B.hi[2] - B.lo[2] + 1);
+    for (int d = 0; d < 3; ++d) dom *= L.geom.dom_hi[d] - L.geom.dom_lo[d] + 1;
+    L.gl_need = vol < dom && L.ntile_g > 0;
 }
 
 } // namespace ocfd
@@ -419,52 +479,6 @@ static void march_tiles(const LevelDev& L, int ch, std::vector<int4>& ty, std::v
         for (int c0 = 0; c0 < nc; c0 += nsub)
             for (int s = 0; s < iseg; ++s) ty.push_back(make_int4(b, s | (lw << 16), 0, (c0 * ch) | (ch << 20)));
     }
-}
-
-// The ghost lists of a new level (k_ghost_lists), for both fill depths, when
-// the level does not cover its domain (only then can a ghost cell lack a
-// same-level or physical source).  Each list has room for every ghost cell of
-// its depth; the kernels run on the upload stream, after the level's table
-// copy, and the event gl_ready marks their end (no host wait).
-static int build_ghost_lists(LevelDev& L)
-{
-    if (!L.ntile_g) return OCFD_OK;
-    long vol = 0, dom = 1;
-    for (const BoxDesc& B : L.boxes)
-        vol += (long)(B.hi[0] - B.lo[0] + 1) * (B.hi[1] - B.lo[1] + 1) * (B.hi[2] - B.lo[2] + 1);
-    for (int d = 0; d < 3; ++d) dom *= L.geom.dom_hi[d] - L.geom.dom_lo[d] + 1;
-    if (vol >= dom) return OCFD_OK;   // boxes are disjoint: the level covers its domain
-    long cap[2] = {0, 0};
-    for (const BoxDesc& B : L.boxes) {
-        cap[0] += ghost_count(B, NG);
-        cap[1] += ghost_count(B, 1);
-    }
-    const size_t bytes = 256 + sizeof(int2) * 2 * (size_t)(cap[0] + cap[1]);
-    void* mem = nullptr;
-    if (!dev_alloc(&mem, bytes)) {
-        set_error("ocfd_level_create: out of device memory for the ghost lists");
-        return OCFD_ENOMEM;
-    }
-    unsigned* cnt = (unsigned*)mem;   // 4 lengths on the first 256 bytes
-    int2* p = (int2*)((char*)mem + 256);
-    for (int d = 0; d < 2; ++d)
-        for (int w = 0; w < 2; ++w) {
-            L.gl[d][w] = LevelDev::GhostList{p, cnt + 2 * d + w, (int)cap[d]};
-            p += cap[d];
-        }
-    hipStream_t s = upload_stream();
-    OCFD_HIP(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned), s));
-    hipLaunchKernelGGL(k_ghost_lists, dim3(L.ntile_g), dim3(256), 0, s, L.d_boxes, L.d_geom, L.d_tile_g, L.gl[0][0].d,
-                       L.gl[0][1].d, cnt);
-    if (L.ntile_gp)
-        hipLaunchKernelGGL(k_ghost_lists, dim3(L.ntile_gp), dim3(256), 0, s, L.d_boxes, L.d_geom, L.d_tile_gp,
-                           L.gl[1][0].d, L.gl[1][1].d, cnt + 2);
-    OCFD_HIP(hipEventCreateWithFlags(&L.gl_ready, hipEventDisableTiming));
-    OCFD_HIP(hipEventRecord(L.gl_ready, s));
-    L.d_gl = mem;
-    L.gl_pending = true;
-    L.dev_bytes += (long)bytes;
-    return check_launch("ocfd_level_create");
 }
 
 int ocfd_level_create(int nbox, const int* lo, const int* hi, const ocfd_geom* g, ocfd_level* out)
@@ -724,7 +738,7 @@ int ocfd_level_create_eb(int nbox, const int* lo, const int* hi, const ocfd_geom
                   (long)(sizeof(float) + sizeof(double)) * L.mf_prim + (long)sizeof(unsigned long long) +
                   (long)sizeof(int4) * (L.ntile_eq + L.ntile_ef[0] + L.ntile_ef[1] + L.ntile_ef[2] + L.ntile_ed) +
                   L.mf_eb * (long)(sizeof(uint32_t) + sizeof(double) * (4 + 7 + 3 * NC));
-    if ((rc = build_ghost_lists(L))) { free_level(L); delete lev; return rc; }
+    set_ghost_need(L);
     *out = lev;
     return OCFD_OK;
 }

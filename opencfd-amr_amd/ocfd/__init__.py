@@ -357,9 +357,14 @@ class Level:
         lo, hi = self.boxes[b]
         return (ncomp,) + tuple(hi[d] - lo[d] + 1 + 2 * ngrow for d in (2, 1, 0))
 
-    def new_mf(self, ngrow=NGROW, ncomp=NCONS, device="cuda"):
+    def new_mf(self, ngrow=NGROW, ncomp=NCONS, device="cuda", zero=True):
+        """MultiFab storage of the level; zero=False leaves it uninitialised
+        (for buffers whose cells are written before they are read)."""
         import torch
-        return torch.zeros(self.mf_size(ngrow, ncomp), dtype=torch.float64, device=device)
+        n = self.mf_size(ngrow, ncomp)
+        if zero:
+            return torch.zeros(n, dtype=torch.float64, device=device)
+        return torch.empty(n, dtype=torch.float64, device=device)
 
     def fab_view(self, mf, b, ngrow=NGROW, ncomp=NCONS):
         """(ncomp, nz, ny, nx) view of box b's FAB inside MultiFab storage mf."""

@@ -359,9 +359,11 @@ class Hierarchy:
         self.levels[l] = level
         if (self.flags & PLANAR) and self.planar_fill:
             level.set_planar(True)
-        b = {k: level.new_mf() for k in ("a", "b", "s1", "s2", "tmp")}
-        if S is not None:
-            b["a"] = S
+        # the old state, the stage states and the time-combined state have
+        # their valid cells written and their ghost cells filled before any
+        # read, so they start uninitialised (no fill kernels in a regrid)
+        b = {k: level.new_mf(zero=False) for k in ("b", "s1", "s2", "tmp")}
+        b["a"] = S if S is not None else level.new_mf()
         self.bufs[l] = b
         q = self.st[l]
         q.cur, q.old = b["a"].data_ptr(), b["b"].data_ptr()
