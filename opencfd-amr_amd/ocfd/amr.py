@@ -426,11 +426,35 @@ class Hierarchy:
         g = self.geoms[l]
         return _dilate(tg, self.n_error_buf, [g.periodic[d] for d in range(3)])
 
-    def grid_places(self, lbase, t):
+    def _tag_flags(self, levc, t):
+        """Device tag and nesting flags of level levc at time t (ghosts filled)."""
+        self.fill(levc, None, t)
+        return tag_dengrad_nest(self.levels[levc], self.S[levc], self.thr, self.n_proper if levc > 0 else 0,
+                                self.stream)
+
+    def _prefetch_flags(self, lbase, t):
+        """The flags grid_places(lbase, t) reads, computed now and copied to
+        pinned host memory without waiting: {levc: (level, host flags)} and an
+        event that marks the copies' end."""
+        import torch
+        if self.n_error_buf != 0:
+            return None
+        out = {}
+        with torch.cuda.stream(self.stream):
+            for levc in range(min(self.finest, self.max_level - 1), lbase - 1, -1):
+                dev = self._tag_flags(levc, t)
+                host = torch.empty(dev.numel(), dtype=torch.uint8, pin_memory=True)
+                host.copy_(dev, non_blocking=True)
+                out[levc] = (self.levels[levc], host, dev)
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+        return out, ev
+
+    def grid_places(self, lbase, t, pre=None):
         """New grids for levels lbase+1.. (Amr::grid_places): top-down from the
         finest level that may refine; each level's tags are widened to cover
         the next finer new grids plus n_proper cells, and kept inside the
-        level's proper nesting region."""
+        level's proper nesting region.  pre: the flags of _prefetch_flags(lbase, t)."""
         new = {}
         new_finest = lbase
         max_crse = min(self.finest, self.max_level - 1)
@@ -441,9 +465,12 @@ class Hierarchy:
                 # tags and nesting region in the level's own layout (a level
                 # covers a few percent of its domain), clustered from there
                 lev = self.levels[levc]
-                self.fill(levc, None, t)
-                flags = tag_dengrad_nest(lev, self.S[levc], self.thr, self.n_proper if levc > 0 else 0,
-                                         self.stream).cpu().numpy()
+                got = pre[0].get(levc) if pre is not None else None
+                if got is not None and got[0] is lev:
+                    pre[1].synchronize()
+                    flags = got[1].numpy()
+                else:
+                    flags = self._tag_flags(levc, t).cpu().numpy()
                 extra = []
                 if levf + 1 in new and new[levf + 1]:
                     extra = projected_boxes(new[levf + 1], self.geoms[levf], self.n_proper)
@@ -527,11 +554,18 @@ class Hierarchy:
         for l in range(self.finest - 1, -1, -1):
             self.cfs[l + 1].avg_down(self.S[l + 1], self.S[l], NCONS, self.stream)
 
-    def regrid(self, lbase, t):
+    def regrid(self, lbase, t, pre=None, then=None):
         """Amr::regrid(lbase): new grids for levels lbase+1..; levels that lose
-        their grids are removed."""
-        new, nf = self.grid_places(lbase, t)
+        their grids are removed.  pre: flags from _prefetch_flags(lbase, t).
+        then: called once level lbase+1 is in place, before the finer levels
+        are built (the advance of level lbase, which reads levels up to
+        lbase+1 only; the finer levels read the coarser ones at time t, which
+        that advance keeps as its old state)."""
+        new, nf = self.grid_places(lbase, t, pre)
         for l in range(lbase + 1, nf + 1):
+            if l == lbase + 2 and then is not None:
+                then()
+                then = None
             if new[l]:
                 self._build(l, new[l], t, False, relink=not (l + 1 <= nf and new.get(l + 1)))
         if nf < self.finest:
@@ -539,6 +573,8 @@ class Hierarchy:
         self._check_links()
         for l in range(lbase, self.max_level + 1):
             self.count[l] = 0
+        if then is not None:
+            then()
 
     def _check_links(self):
         for l in range(1, self.finest + 1):
@@ -565,13 +601,40 @@ class Hierarchy:
               "ocfd_amr_advance_level")
 
     def _time_step(self, l, t):
+        # Amr::timeStep: regrid(i, t) at the first level i >= l whose count
+        # reached regrid_int (it resets the counts of i and above, so no other
+        # follows).  A regrid of a finer level i > l changes only levels above
+        # i and their registers, while this level's advance reads levels up to
+        # l+1 and the register of l+1, and the coarser states the regrid reads
+        # at time t are the same before and after the advance (the advanced
+        # level keeps its time-t state, ghosts filled, as its old state): so
+        # its tags are computed and copied to the host first, the advance is
+        # queued, and the clustering and the new levels are built on the host
+        # while the device runs the advance.  Same results as regridding first.
+        # A regrid of this level itself builds level l+1 first, queues the
+        # advance, then builds the finer levels.
+        pending = None
+        advanced = False
+
+        def advance():
+            nonlocal advanced
+            self._advance(l, t, self.dt_level[l])
+            advanced = True
+
         if self.fixed is None and self.regrid_int > 0 and l < self.max_level:
             i = l
             while i <= min(self.finest, self.max_level - 1):
                 if self.count[i] >= self.regrid_int:
-                    self.regrid(i, t)
+                    if i == l:
+                        self.regrid(i, t, then=advance)
+                    else:
+                        pending = (i, self._prefetch_flags(i, t))
+                    break
                 i += 1
-        self._advance(l, t, self.dt_level[l])
+        if not advanced:
+            self._advance(l, t, self.dt_level[l])
+        if pending is not None:
+            self.regrid(pending[0], t, pending[1])
         self.level_steps[l] += 1
         self.count[l] += 1
         if l < self.finest:
