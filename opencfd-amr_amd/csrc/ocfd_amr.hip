@@ -387,6 +387,33 @@ __global__ __launch_bounds__(256) void k_fill_cf_list(const BoxDesc* __restrict_
     fill_cf_ghost(fboxes, *fgeom, Sf, cboxes, cgeom, cmap, Sc, czuni, B, i, j, k);
 }
 
+// Both ghost fills of a level in one launch, over its two ghost lists: the
+// first ca threads take list A (the level fill), the others list B (the
+// coarse/fine fill).  The two write disjoint (cell, component) pairs -- the
+// level fill a component whose mapped source a level box owns, the
+// coarse/fine fill one whose source no box owns, neither an ext_dir component
+// the other sets -- so their order does not matter.
+__global__ __launch_bounds__(256) void k_fill_both(const BoxDesc* __restrict__ fboxes,
+                                                   const GeomDesc* __restrict__ fgeom, const int2* __restrict__ la,
+                                                   const unsigned* __restrict__ na, int ca,
+                                                   const int2* __restrict__ lb, const unsigned* __restrict__ nb, int gz,
+                                                   double* __restrict__ Sf, double time,
+                                                   const BoxDesc* __restrict__ cboxes,
+                                                   const GeomDesc* __restrict__ cgeom, const int* __restrict__ cmap,
+                                                   const double* __restrict__ Sc, int czuni)
+{
+    int q = blockIdx.x * 256 + threadIdx.x;
+    const bool a = q < ca;
+    if (!a) q -= ca;
+    if (q >= (int)(a ? *na : *nb)) return;
+    const int2 e = (a ? la : lb)[q];
+    const BoxDesc B = fboxes[e.x];
+    int i, j, k;
+    ghost_cell(B, e.y, gz, i, j, k);
+    if (a) fill_ghost(fboxes, fgeom[B.geom], Sf, time, B, i, j, k);
+    else fill_cf_ghost(fboxes, *fgeom, Sf, cboxes, cgeom, cmap, Sc, czuni, B, i, j, k);
+}
+
 // valid cells of the fine level from the coarse level (a new fine region);
 // cells that the old fine level (when given) covers are skipped
 __global__ __launch_bounds__(256) void k_interp_valid(const BoxDesc* __restrict__ fboxes, const int4* __restrict__ tiles,
@@ -1108,10 +1135,24 @@ struct Hier {
     int fill(int l, double* X, double ts)
     {
         int rc;
-        if ((rc = ocfd_fill_boundary(levels[l], X, ts, s))) return rc;
-        if (l == 0) return OCFD_OK;
+        if (l == 0) return ocfd_fill_boundary(levels[l], X, ts, s);
+        // the coarser state at ts first (it touches only coarser levels), then
+        // both fills of this level: one launch over its ghost lists when it
+        // keeps them, else the level fill and the coarse/fine fill
         const double* C = nullptr;
         if ((rc = at(l - 1, ts, C))) return rc;
+        const LevelDev& F = levels[l]->dev;
+        const LevelDev::GhostList* ga = ghost_list(F, 0, s);
+        const LevelDev::GhostList* gb = ga ? ghost_list(F, 1, s) : nullptr;
+        if (ga && gb) {
+            if (ga->cap + gb->cap > 0)
+                hipLaunchKernelGGL(k_fill_both, dim3((ga->cap + gb->cap + 255) / 256), dim3(256), 0, s, F.d_boxes,
+                                   F.d_geom, ga->d, ga->cnt, ga->cap, gb->d, gb->cnt, F.fill_gz, X, ts,
+                                   levels[l - 1]->dev.d_boxes, levels[l - 1]->dev.d_geom, cfs[l]->d_cmap, C,
+                                   levels[l - 1]->dev.fill_gz == 1);
+            return launched("ocfd_amr fill");
+        }
+        if ((rc = ocfd_fill_boundary(levels[l], X, ts, s))) return rc;
         return ocfd_cf_fill_fine(cfs[l], X, C, s);
     }
 

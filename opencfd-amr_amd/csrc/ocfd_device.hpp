@@ -11,6 +11,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <climits>
+#include <cmath>
 #include <cstdint>
 #include "ocfd.h"
 
@@ -276,6 +278,58 @@ __device__ __forceinline__ bool bc_source(const GeomDesc& G, int n, const int iv
     if (user) return false;
     return src[0] >= G.dom_lo[0] && src[0] <= G.dom_hi[0] && src[1] >= G.dom_lo[1] && src[1] <= G.dom_hi[1] &&
            src[2] >= G.dom_lo[2] && src[2] <= G.dom_hi[2];
+}
+
+// Level ghost fill of one ghost cell (i,j,k) of box B at time `time`:
+// periodic images and the per-direction BC maps from the level's own boxes,
+// then the case's ext_dir values (Exec/1_DMR/prob_bcfill.cpp:14-58,
+// Exec/3_FSTEP/prob_bcfill.cpp:8-33).  Components whose source no box of the
+// level owns are left alone (the coarse/fine fill's part).
+__device__ __forceinline__ void fill_ghost(const BoxDesc* __restrict__ boxes, const GeomDesc& G,
+                                           double* __restrict__ S, double time, const BoxDesc& B, int i, int j, int k)
+{
+    const long dst = sidx(B, i, j, k);
+    int iv[3], outside[3];
+    ghost_wrap(G, i, j, k, iv, outside);
+    // the BC maps mirror or clamp every component alike, so the source cell and
+    // its owner are normally found once for all five
+    int last[3] = {INT_MIN, INT_MIN, INT_MIN};
+    long so = -1, sst = 0;
+    for (int n = 0; n < NC; ++n) {
+        int src[3];
+        double sign;
+        if (!bc_source(G, n, iv, outside, src, sign)) continue;
+        if (src[0] != last[0] || src[1] != last[1] || src[2] != last[2]) {
+            const int m = level_owner(G, boxes, src[0], src[1], src[2]);
+            so = -1;
+            if (m >= 0) {   // else not covered on this level (coarse/fine boundary)
+                const BoxDesc& O = boxes[m];
+                so = sidx(O, src[0], src[1], src[2]);
+                sst = O.nptg;
+            }
+            last[0] = src[0]; last[1] = src[1]; last[2] = src[2];
+        }
+        if (so < 0) continue;
+        S[dst + n * B.nptg] = sign * S[so + n * sst];
+    }
+    if (G.probtype == 1) {
+        const double x = G.prob_lo[0] + (i + 0.5) * G.dx[0];
+        const double x0 = 1.0 / 6.0 + sqrt(1 / 3.0) + time * 10.0 / sin(M_PI * (1.0 / 3.0));
+        if (G.bc[0][1][1] == BC_EXT && j > G.dom_hi[1]) {
+            for (int n = 0; n < NC; ++n) S[dst + n * B.nptg] = x < x0 ? G.left[n] : G.right[n];
+        }
+        if (G.bc[0][1][0] == BC_EVEN && j < G.dom_lo[1]) {
+            if (x < 1.0 / 6.0)
+                for (int n = 0; n < NC; ++n) S[dst + n * B.nptg] = G.left[n];
+        }
+        if (G.bc[0][0][0] == BC_EXT && i < G.dom_lo[0]) {
+            for (int n = 0; n < NC; ++n) S[dst + n * B.nptg] = G.left[n];
+        }
+    } else if (G.probtype == 2) {
+        // inflow state at x-lo (Exec/3_FSTEP/prob_bcfill.cpp:8-33)
+        if (G.bc[0][0][0] == BC_EXT && i < G.dom_lo[0])
+            for (int n = 0; n < NC; ++n) S[dst + n * B.nptg] = G.left[n];
+    }
 }
 
 // Primitive state used by the flux kernels
