@@ -599,6 +599,15 @@ int ocfd_amr_advance_level(int nlev, const ocfd_level* levels, const ocfd_cf* cf
                            double t, double dt, int scheme, const ocfd_parm* parm, int flags, int do_reflux,
                            void* stream);
 
+/* EBR::estTimeStep of every level of a hierarchy in one call, the loop of
+ * Amr::computeNewDt over the levels (Source/Advance.cpp:61-79): dt_out[l] =
+ * cfl * min over level l's valid cells of dx_d/(|u_d|+c), for states[l] of
+ * levels[l].  All reductions are queued, then the results come back in one
+ * copy and one synchronisation of the stream.  Returns OCFD_ENONFINITE (all of
+ * dt_out still written) when a level's state holds a NaN or inf. */
+int ocfd_amr_estdt(int nlev, const ocfd_level* levels, const double* const* states, double cfl,
+                   const ocfd_parm* parm, double* dt_out, void* stream);
+
 /* Ghost cells of X, a state of level l at time t, filled as a stage's are in
  * ocfd_amr_advance_level (FillPatch of level l: the level, then the coarser
  * levels at time t, recursively).  For regrids and output. */

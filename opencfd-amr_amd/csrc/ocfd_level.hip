@@ -826,6 +826,59 @@ int ocfd_estdt(ocfd_level lev, const double* S, double cfl, const ocfd_parm* par
     return OCFD_OK;
 }
 
+int ocfd_amr_estdt(int nlev, const ocfd_level* levels, const double* const* states, double cfl,
+                   const ocfd_parm* parm, double* dt_out, void* stream)
+{
+    if (nlev < 1 || !levels || !states || !parm || !dt_out) {
+        set_error("ocfd_amr_estdt: invalid arguments");
+        return OCFD_EINVAL;
+    }
+    for (int l = 0; l < nlev; ++l)
+        if (!levels[l] || !states[l]) {
+            set_error("ocfd_amr_estdt: level " + std::to_string(l) + " or its state is null");
+            return OCFD_EINVAL;
+        }
+    hipStream_t s = (hipStream_t)stream;
+    const PhysArgs ph = phys_of(*parm);
+    for (int l = 0; l < nlev; ++l) {
+        const LevelDev& L = levels[l]->dev;
+        unsigned int* bad = (unsigned int*)(L.d_red + 1);
+        hipLaunchKernelGGL(k_red_init, dim3(1), dim3(1), 0, s, L.d_red, bad);
+        hipLaunchKernelGGL(k_estdt, dim3(L.ntile_c), dim3(256), 0, s, L.d_boxes, L.d_tile_c, states[l], ph, L.d_red,
+                           bad, (const ocfd_step_ctl*)nullptr);
+    }
+    int rc = check_launch("ocfd_amr_estdt");
+    if (rc) return rc;
+    // every level's slot and flag word into one pinned buffer, one wait
+    static std::mutex mu;
+    static unsigned long long* pin = nullptr;
+    static int cap = 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cap < nlev) {
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        cap = 0;
+        OCFD_HIP(hipHostMalloc((void**)&pin, sizeof(unsigned long long) * 2 * nlev, hipHostMallocDefault));
+        cap = nlev;
+    }
+    for (int l = 0; l < nlev; ++l)
+        OCFD_HIP(hipMemcpyAsync(pin + 2 * l, levels[l]->dev.d_red, 2 * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, s));
+    OCFD_HIP(hipStreamSynchronize(s));
+    bool bad = false;
+    for (int l = 0; l < nlev; ++l) {
+        double m;
+        std::memcpy(&m, &pin[2 * l], sizeof(m));
+        dt_out[l] = m * cfl;
+        bad = bad || (unsigned int)pin[2 * l + 1] != 0u;
+    }
+    if (bad) {
+        set_error("ocfd_amr_estdt: a level's state holds a non-finite value");
+        return OCFD_ENONFINITE;
+    }
+    return OCFD_OK;
+}
+
 int ocfd_advance(ocfd_level lev, double* A, double* B, double* C, double time, double dt, int scheme,
                  const ocfd_parm* parm, int flags, void* stream)
 {

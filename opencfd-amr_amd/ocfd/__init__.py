@@ -217,6 +217,7 @@ def lib():
         L.ocfd_amr_advance_level.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int, ctypes.POINTER(Parm), ctypes.c_int, ctypes.c_int, vp]
         L.ocfd_amr_fill_level.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int, vp, ctypes.c_double, vp]
+        L.ocfd_amr_estdt.argtypes = [ctypes.c_int, vp, vp, ctypes.c_double, vp, vp, vp]
         L.ocfd_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib

@@ -583,7 +583,13 @@ class Hierarchy:
 
     # -- time stepping ----------------------------------------------------------
     def _compute_dt(self):
-        est = [self.levels[l].estdt(self.S[l], self.cfl, self.parm, self.stream) for l in range(self.finest + 1)]
+        n = self.finest + 1
+        lv = (ctypes.c_void_p * n)(*[self.levels[l].handle for l in range(n)])
+        st = (ctypes.c_void_p * n)(*[self.S[l].data_ptr() for l in range(n)])
+        out = (ctypes.c_double * n)()
+        check(lib().ocfd_amr_estdt(n, lv, st, self.cfl, ctypes.byref(self.parm), out, _stream(self.stream)),
+              "ocfd_amr_estdt")
+        est = list(out)
         if self.steps > 0:
             prev = [self.dt_level[l] if self.dt_level[l] > 0 else self.dt_level[0] / RATIO ** l
                     for l in range(self.finest + 1)]
