@@ -127,17 +127,25 @@ __device__ __forceinline__ void mixture(double T, const double* X, double& mu, d
         sum2 += X[k] / lambdai[k];
     }
     lambda = 0.5 * (sum1 + 1.0 / sum2);
+    // Wilke's phi_kj = (1 + sqrt(mu_k/mu_j) (M_j/M_k)^(1/4))^2 / sqrt(8 (1 + M_k/M_j))
+    // and phi_jk = phi_kj (mu_j/mu_k)(M_k/M_j): the reference's formulas with
+    // one reciprocal per species root instead of three divisions per pair
+    // (the molecular-weight factors are compile-time constants); the values
+    // agree with the reference order to a few ulps
+    double rsq[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) rsq[k] = 1.0 / sqmui[k];
     double phi[NS][NS];
 #pragma unroll
     for (int k = 0; k < NS; k++) {
 #pragma unroll
         for (int j = 0; j < k; j++) {
             const double wratiojk = chem::mw[j] / chem::mw[k];
-            const double vratiokj = mui[k] / mui[j];
-            const double factor1 = 1.0 + (sqmui[k] / sqmui[j]) * sqrt(sqrt(wratiojk));
-            const double tmp = factor1 * factor1 / sqrt(8.0 + 8.0 * chem::mw[k] / chem::mw[j]);
+            const double factor1 = 1.0 + (sqmui[k] * rsq[j]) * sqrt(sqrt(wratiojk));
+            const double tmp = factor1 * factor1 * (1.0 / sqrt(8.0 + 8.0 * chem::mw[k] / chem::mw[j]));
+            const double rjk = sqmui[j] * rsq[k];
             phi[k][j] = tmp;
-            phi[j][k] = tmp / (vratiokj * wratiojk);
+            phi[j][k] = tmp * (rjk * rjk) * (chem::mw[k] / chem::mw[j]);
         }
         phi[k][k] = 1.0;
     }
@@ -160,10 +168,11 @@ __device__ __forceinline__ void diff_coeffs(double T, double p, const double* X,
     double mmw = 0;
 #pragma unroll
     for (int n = 0; n < NS; ++n) mmw += X[n] * chem::mw[n];
-    // binary coefficients of the upper triangle, row by row (the fit order)
-    double bd[45];
+    // reciprocals of the binary coefficients of the upper triangle, row by
+    // row (the fit order): each is divided by twice in the sums below
+    double rbd[45];
 #pragma unroll
-    for (int ic = 0; ic < 45; ++ic) bd[ic] = t15 * dot5(poly, diff_fit[ic]);
+    for (int ic = 0; ic < 45; ++ic) rbd[ic] = 1.0 / (t15 * dot5(poly, diff_fit[ic]));
 #pragma unroll
     for (int n = 0; n < NS; n++) {
         double sum1 = 0.0, sum2 = 0.0;
@@ -172,8 +181,8 @@ __device__ __forceinline__ void diff_coeffs(double T, double p, const double* X,
             if (nn == n) continue;
             const int a = n < nn ? n : nn, b = n < nn ? nn : n;
             const int ic = a * NS - a * (a - 1) / 2 + (b - a);   // index of (a, b) in the upper triangle
-            sum1 += X[nn] / bd[ic];
-            sum2 += X[nn] * chem::mw[nn] / bd[ic];
+            sum1 += X[nn] * rbd[ic];
+            sum2 += X[nn] * chem::mw[nn] * rbd[ic];
         }
         sum1 *= p;
         sum2 *= p * X[n] / (mmw - chem::mw[n] * X[n]);
@@ -244,9 +253,11 @@ __global__ __launch_bounds__(256) void k_mprims(const BoxDesc* __restrict__ boxe
 #pragma unroll
         for (int n = 0; n < NS; ++n) rhoi[n] = P[n * st];
         const double T = chem::T_given_ey(ei, rhoi);
+        // (divisions by the constant molecular weights and by rho as
+        // multiplications by their reciprocals: a last-bit difference at most)
         double yow = 0;
 #pragma unroll
-        for (int n = 0; n < NS; ++n) yow += rhoi[n] / chem::mw[n];
+        for (int n = 0; n < NS; ++n) yow += rhoi[n] * (1.0 / chem::mw[n]);
         const double p = chem::Ru * T * yow;
         const double gama = p / ei + 1.0;
         double* q = m.mq + B.off_s / NC * MQ_N + l0;
@@ -259,16 +270,16 @@ __global__ __launch_bounds__(256) void k_mprims(const BoxDesc* __restrict__ boxe
         if ((m.a.flags & OCFD_VISC) && face_cell) {
             double Y[NS], X[NS];
 #pragma unroll
-            for (int n = 0; n < NS; ++n) Y[n] = rhoi[n] / rho;
+            for (int n = 0; n < NS; ++n) Y[n] = rhoi[n] * rhoinv;
             // CKYTX, LiDryer.H:36-52
             double tmp[NS], yw = 0;
 #pragma unroll
-            for (int n = 0; n < NS; ++n) tmp[n] = Y[n] / chem::mw[n];
+            for (int n = 0; n < NS; ++n) tmp[n] = Y[n] * (1.0 / chem::mw[n]);
 #pragma unroll
             for (int n = 0; n < NS; ++n) yw += tmp[n];
             const double ywinv = 1.0 / yw;
 #pragma unroll
-            for (int n = 0; n < NS; ++n) X[n] = Y[n] / chem::mw[n] * ywinv;
+            for (int n = 0; n < NS; ++n) X[n] = tmp[n] * ywinv;
             double mu, lam, D[NS];
             tr::mixture(T, X, mu, lam);
             tr::diff_coeffs(T, p, X, D);
