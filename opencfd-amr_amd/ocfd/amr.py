@@ -561,6 +561,10 @@ class Hierarchy:
         are built (the advance of level lbase, which reads levels up to
         lbase+1 only; the finer levels read the coarser ones at time t, which
         that advance keeps as its old state)."""
+        if pre is None:
+            # every level's tags queued at once: one wait for the first
+            # clustering, the others' kernels run while it does
+            pre = self._prefetch_flags(lbase, t)
         new, nf = self.grid_places(lbase, t, pre)
         for l in range(lbase + 1, nf + 1):
             if l == lbase + 2 and then is not None:
