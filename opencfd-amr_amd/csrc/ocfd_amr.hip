@@ -687,88 +687,91 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
         }
         return tile_owner(fmap.data(), flist.data(), fmn, CG.dom_lo, fshift, fcrs.data(), iv[0], iv[1], iv[2], 0);
     };
-    // candidates (coarse box, K, J, I), visited in coarse-box then k, j, i
-    // order: the cells face-adjacent to a coarsened fine box (an edge or corner
-    // neighbour of one box that owns a face is face-adjacent to another box).
-    // A side on a non-periodic domain boundary has none, nor does a side of a
-    // box spanning a periodic direction (its image is the box itself).
-    // candidates as 64-bit keys (coarse box, K, J, I), 16 bits each relative to
-    // the domain's low corner, so that sorting the keys orders them as the
-    // tuples; a domain or box count beyond 16 bits falls back to the tuples
-    bool narrow = C.nbox < (1 << 16);
-    for (int d = 0; d < 3; ++d) narrow = narrow && CG.dom_hi[d] - CG.dom_lo[d] < (1 << 16);
-    std::vector<uint64_t> ckey;
-    auto key_of = [&](int b, int K, int J, int I) {
-        return ((uint64_t)b << 48) | ((uint64_t)(K - CG.dom_lo[2]) << 32) | ((uint64_t)(J - CG.dom_lo[1]) << 16) |
-               (uint64_t)(I - CG.dom_lo[0]);
+    // Interface cells and faces, coarse box by coarse box on the host threads.
+    // A candidate is a coarse cell face-adjacent to a coarsened fine box (an
+    // edge or corner neighbour of one box that owns a face is face-adjacent
+    // to another box): each side of each coarsened fine box gives a one-cell
+    // slab (its plane wrapped on a periodic side; none on a non-periodic
+    // domain side, nor for a box spanning a periodic direction, whose image
+    // is the box itself).  The slabs are binned to the coarse boxes they meet
+    // (through the coarse level's tile map); each coarse box then lists the
+    // uncovered slab cells it owns in (K, J, I) order, once each, with their
+    // faces toward covered neighbours in (direction, side) order, and the
+    // lists are joined in coarse-box order: the order a global sort of
+    // (coarse box, K, J, I) gives, whatever the number of threads.
+    struct Slab {
+        int lo[3], hi[3];
     };
-    // the candidates and the face lists are built on the host threads: the
-    // candidates of contiguous ranges of fine boxes (sorted afterwards), the
-    // faces of contiguous ranges of the sorted candidates (joined in order, so
-    // the lists are those a single thread builds)
-    const int nth = std::max(1, std::min(host_threads(), (int)fcrs.size() / 16));
-    std::vector<std::vector<uint64_t>> ckey_t(nth);
-    std::vector<std::vector<std::array<int, 4>>> cand_t(nth);
-    parallel_for(nth, [&](int t) {
-        const size_t b0 = fcrs.size() * t / nth, b1 = fcrs.size() * (t + 1) / nth;
-        for (size_t fb = b0; fb < b1; ++fb) {
-            const BoxDesc& B = fcrs[fb];
-            for (int d = 0; d < 3; ++d) {
-                const int len = CG.dom_hi[d] - CG.dom_lo[d] + 1;
-                if (CG.periodic[d] && B.hi[d] - B.lo[d] + 1 >= len) continue;
-                for (int side = 0; side < 2; ++side) {
-                    const int at = side ? B.hi[d] + 1 : B.lo[d] - 1;
-                    if (!CG.periodic[d] && (at < CG.dom_lo[d] || at > CG.dom_hi[d])) continue;
-                    int lo3[3] = {B.lo[0], B.lo[1], B.lo[2]}, hi3[3] = {B.hi[0], B.hi[1], B.hi[2]};
-                    lo3[d] = hi3[d] = at;
-                    for (int K = lo3[2]; K <= hi3[2]; ++K)
-                        for (int J = lo3[1]; J <= hi3[1]; ++J)
-                            for (int I = lo3[0]; I <= hi3[0]; ++I) {
-                                int iv[3] = {I, J, K};
-                                bool in = true;
-                                for (int e = 0; e < 3; ++e) {
-                                    iv[e] = wrap(CG, e, iv[e]);
-                                    in = in && iv[e] >= CG.dom_lo[e] && iv[e] <= CG.dom_hi[e];
-                                }
-                                if (!in || covering(iv[0], iv[1], iv[2]) >= 0) continue;
-                                const int cb = find_box(C, iv[0], iv[1], iv[2]);
-                                if (cb < 0) continue;
-                                if (narrow) ckey_t[t].push_back(key_of(cb, iv[2], iv[1], iv[0]));
-                                else cand_t[t].push_back({cb, iv[2], iv[1], iv[0]});
-                            }
-                }
+    std::vector<Slab> slabs;
+    slabs.reserve(6 * fcrs.size());
+    for (const BoxDesc& B : fcrs)
+        for (int d = 0; d < 3; ++d) {
+            const int len = CG.dom_hi[d] - CG.dom_lo[d] + 1;
+            if (CG.periodic[d] && B.hi[d] - B.lo[d] + 1 >= len) continue;
+            for (int side = 0; side < 2; ++side) {
+                const int at = side ? B.hi[d] + 1 : B.lo[d] - 1;
+                if (!CG.periodic[d] && (at < CG.dom_lo[d] || at > CG.dom_hi[d])) continue;
+                Slab sl;
+                for (int e = 0; e < 3; ++e) { sl.lo[e] = B.lo[e]; sl.hi[e] = B.hi[e]; }
+                sl.lo[d] = sl.hi[d] = wrap(CG, d, at);
+                slabs.push_back(sl);
             }
         }
-    });
-    std::vector<std::array<int, 4>> cand;
-    if (narrow) {
-        for (int t = 0; t < nth; ++t) ckey.insert(ckey.end(), ckey_t[t].begin(), ckey_t[t].end());
-        std::sort(ckey.begin(), ckey.end());
-        ckey.erase(std::unique(ckey.begin(), ckey.end()), ckey.end());
-        cand.resize(ckey.size());
-        for (size_t q = 0; q < ckey.size(); ++q) {
-            const uint64_t k = ckey[q];
-            cand[q] = {(int)(k >> 48), (int)((k >> 32) & 0xffff) + CG.dom_lo[2], (int)((k >> 16) & 0xffff) + CG.dom_lo[1],
-                       (int)(k & 0xffff) + CG.dom_lo[0]};
+    // slabs per coarse box, from the tiles of the coarse owner map each meets
+    std::vector<std::vector<int>> cslab(C.nbox);
+    {
+        const int sh = CG.map_shift;
+        std::vector<int> seen(C.nbox, -1);
+        for (int q = 0; q < (int)slabs.size(); ++q) {
+            const Slab& sl = slabs[q];
+            int t0[3], t1[3];
+            for (int e = 0; e < 3; ++e) {
+                t0[e] = (sl.lo[e] - CG.dom_lo[e]) >> sh;
+                t1[e] = (sl.hi[e] - CG.dom_lo[e]) >> sh;
+            }
+            auto take = [&](int cb) {
+                if (cb < 0 || seen[cb] == q) return;
+                seen[cb] = q;
+                cslab[cb].push_back(q);
+            };
+            for (int tk = t0[2]; tk <= t1[2]; ++tk)
+                for (int tj = t0[1]; tj <= t1[1]; ++tj)
+                    for (int ti = t0[0]; ti <= t1[0]; ++ti) {
+                        const int v = C.owner_map[((long)tk * CG.map_n[1] + tj) * CG.map_n[0] + ti];
+                        if (v >= -1) { take(v); continue; }
+                        const int* l = C.owner_list.data() + (-v - 2);
+                        for (int m = 1; m <= l[0]; ++m) take(l[m]);
+                    }
         }
-    } else {
-        for (int t = 0; t < nth; ++t) cand.insert(cand.end(), cand_t[t].begin(), cand_t[t].end());
-        std::sort(cand.begin(), cand.end());
-        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     }
     struct FaceLists {
         std::vector<CFFace> cf;
         std::vector<int4> cells;
         std::vector<int> cellfaces;
     };
-    std::vector<FaceLists> part(nth);
-    parallel_for(nth, [&](int t) {
-        FaceLists& P = part[t];
-        const size_t q0 = cand.size() * t / nth, q1 = cand.size() * (t + 1) / nth;
-        P.cf.reserve(2 * (q1 - q0));
-        for (size_t qi = q0; qi < q1; ++qi) {
-            const auto& q = cand[qi];
-            const int b = q[0], K = q[1], J = q[2], I = q[3];
+    std::vector<FaceLists> part(C.nbox);
+    parallel_for(C.nbox, [&](int cb) {
+        if (cslab[cb].empty()) return;
+        const BoxDesc& A = C.boxes[cb];
+        std::vector<std::array<int, 3>> cand;   // (K, J, I)
+        for (int q : cslab[cb]) {
+            const Slab& sl = slabs[q];
+            int a[3], z[3];
+            for (int e = 0; e < 3; ++e) {
+                a[e] = std::max(sl.lo[e], A.lo[e]);
+                z[e] = std::min(sl.hi[e], A.hi[e]);
+            }
+            for (int K = a[2]; K <= z[2]; ++K)
+                for (int J = a[1]; J <= z[1]; ++J)
+                    for (int I = a[0]; I <= z[0]; ++I)
+                        if (covering(I, J, K) < 0) cand.push_back({K, J, I});
+        }
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        FaceLists& P = part[cb];
+        P.cf.reserve(2 * cand.size());
+        for (const auto& q : cand) {
+            const int K = q[0], J = q[1], I = q[2];
             int fidx[6] = {-1, -1, -1, -1, -1, -1};
             for (int d = 0; d < 3; ++d)
                 for (int side = 0; side < 2; ++side) {
@@ -777,7 +780,7 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
                     const int fb = covering(nb[0], nb[1], nb[2]);
                     if (fb < 0) continue;
                     CFFace f{};
-                    f.box = b; f.i = I; f.j = J; f.k = K; f.dir = d; f.up = side;
+                    f.box = cb; f.i = I; f.j = J; f.k = K; f.dir = d; f.up = side;
                     f.fbox = fb;   // covers the coarse neighbour; its 4 sub-faces are found on the device
                     fidx[2 * d + side] = (int)P.cf.size();   // local: offset when the parts are joined
                     P.cf.push_back(f);
@@ -785,7 +788,7 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
             bool any = false;
             for (int m = 0; m < 6; ++m) any = any || fidx[m] >= 0;
             if (any) {
-                P.cells.push_back(make_int4(b, I, J, K));
+                P.cells.push_back(make_int4(cb, I, J, K));
                 P.cellfaces.insert(P.cellfaces.end(), fidx, fidx + 6);
             }
         }
@@ -793,13 +796,18 @@ int ocfd_cf_create(ocfd_level coarse, ocfd_level fine, ocfd_cf* out)
     std::vector<CFFace> cf;
     std::vector<int4> cells;
     std::vector<int> cellfaces;
-    for (int t = 0; t < nth; ++t) {
+    {
+        size_t nf = 0, nc = 0;
+        for (const FaceLists& P : part) { nf += P.cf.size(); nc += P.cells.size(); }
+        cf.reserve(nf); cells.reserve(nc); cellfaces.reserve(6 * nc);
+    }
+    for (int cb = 0; cb < C.nbox; ++cb) {
         const int off = (int)cf.size();
-        for (int& v : part[t].cellfaces)
+        for (int& v : part[cb].cellfaces)
             if (v >= 0) v += off;
-        cf.insert(cf.end(), part[t].cf.begin(), part[t].cf.end());
-        cells.insert(cells.end(), part[t].cells.begin(), part[t].cells.end());
-        cellfaces.insert(cellfaces.end(), part[t].cellfaces.begin(), part[t].cellfaces.end());
+        cf.insert(cf.end(), part[cb].cf.begin(), part[cb].cf.end());
+        cells.insert(cells.end(), part[cb].cells.begin(), part[cb].cells.end());
+        cellfaces.insert(cellfaces.end(), part[cb].cellfaces.begin(), part[cb].cellfaces.end());
     }
     std::vector<int4> ta;
     for (int b = 0; b < F.nbox; ++b) {
