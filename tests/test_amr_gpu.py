@@ -462,3 +462,48 @@ def test_three_level_viscous_periodic_conserves_and_matches(torch_cuda, orc):
     print("relative drift of the totals", (t1 - t0) / scale)
     assert (np.abs(t1 - t0) / scale < 1e-13).all()
     _compare_hier(T, O, TOL64)
+
+
+def test_hierarchy_one_call_forms_equal_per_level_calls(torch_cuda, orc):
+    """The hierarchy's batched forms against the per-level calls they stand
+    for, bit for bit, on the uneven planar DMR after two coarse steps (both
+    levels regridded): the CFL reductions of every level in one call
+    (ocfd_amr_estdt) against ocfd_estdt level by level; the finest level's
+    ghost fill at the current time (level and coarse/fine fill in one launch
+    over its ghost lists) against ocfd_fill_boundary followed by
+    ocfd_cf_fill_fine; and every register's faces listed in (coarse box, k, j,
+    i, direction, side) order, the order of a global sort."""
+    import torch
+    import ocfd
+    from ocfd import amr, cases
+    boxes, g, kind = cases.dmr(256, 64, 2, sizes=(16, 32, 64), seed=1234)
+    kw = dict(scheme=2, max_level=2, refine_dengrad=0.05, regrid_int=2, blocking_factor=2, grid_eff=0.7,
+              max_grid_size=64)
+    T = amr.Hierarchy(boxes, g, ocfd.default_parm(), 0.5, kind, flags=ocfd.PLANAR, **kw)
+    for _ in range(2):
+        T.step()
+    torch.cuda.synchronize()
+    n = T.finest + 1
+    assert n == 3
+    lv = (ctypes.c_void_p * n)(*[T.levels[l].handle for l in range(n)])
+    st = (ctypes.c_void_p * n)(*[T.S[l].data_ptr() for l in range(n)])
+    out = (ctypes.c_double * n)()
+    ocfd.check(ocfd.lib().ocfd_amr_estdt(n, lv, st, 0.5, ctypes.byref(T.parm), out, None), "ocfd_amr_estdt")
+    per = [T.levels[l].estdt(T.S[l], 0.5, T.parm) for l in range(n)]
+    assert list(out) == per, (list(out), per)
+
+    l, t = T.finest, T.time
+    X1 = T.S[l].clone()
+    T.fill(l, X1, t)
+    X2 = T.S[l].clone()
+    T.fill(l - 1, None, t)
+    T.levels[l].fill_boundary(X2, t)
+    T.cfs[l].fill_fine(X2, T.S[l - 1])
+    torch.cuda.synchronize()
+    assert torch.equal(X1, X2)
+
+    for m in range(1, n):
+        f = T.cfs[m].faces()
+        assert len(f) > 0
+        keys = [(b, k, j, i, d, up) for b, i, j, k, d, up in f.tolist()]
+        assert keys == sorted(keys) and len(set(keys)) == len(keys)
