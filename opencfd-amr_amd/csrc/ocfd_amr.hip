@@ -280,9 +280,11 @@ __device__ __forceinline__ int interp_box(const BoxDesc* __restrict__ cboxes, co
 }
 
 // component n of fine cell (i,j,k) interpolated from coarse box Bc
-// czuni: the coarse level is z-uniform (planar) and fills the ghost cells of
-// its first two layers only; every layer holds the first layer's values, so
-// the parent is read in the first layer of its box.
+// czuni: the coarse level is z-uniform (planar): every layer holds the first
+// layer's values, and its ghost fill covers the first layers only, so the
+// parent and its x/y neighbours are read in the first layer of its box, the z
+// slope is zero and the limiter's z neighbours repeat that layer (the values a
+// full read of z-uniform data gives, bit for bit, with a third of the loads).
 __device__ __forceinline__ double interp_value(const BoxDesc& Bc, const GeomDesc& CG, const double* __restrict__ C,
                                                int i, int j, int k, int n, int czuni)
 {
@@ -293,9 +295,10 @@ __device__ __forceinline__ double interp_value(const BoxDesc& Bc, const GeomDesc
     const long dd[3] = {1, dy, dz};
     const int iv[3] = {I, J, K};
     const double u0 = u[s];
-    double sl[3];
+    double sl[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
+        if (d == 2 && czuni) break;
         const bool elo = iv[d] == CG.dom_lo[d] && CG.bc[n][d][0] == BC_EXT && !CG.periodic[d];
         const bool ehi = iv[d] == CG.dom_hi[d] && CG.bc[n][d][1] == BC_EXT && !CG.periodic[d];
         const double dc = cc_slope(u, s, dd[d], elo, ehi);
@@ -305,7 +308,8 @@ __device__ __forceinline__ double interp_value(const BoxDesc& Bc, const GeomDesc
     if (sl[0] != 0.0 || sl[1] != 0.0 || sl[2] != 0.0) {
         const double dumax = fabs(sl[0]) * 0.25 + fabs(sl[1]) * 0.25 + fabs(sl[2]) * 0.25;
         double umax = u0, umin = u0;
-        for (int kk = -1; kk <= 1; ++kk)
+        const int kz = czuni ? 0 : 1;
+        for (int kk = -kz; kk <= kz; ++kk)
             for (int jj = -1; jj <= 1; ++jj)
                 for (int ii = -1; ii <= 1; ++ii) {
                     const double w = u[s + kk * dz + jj * dy + ii];
