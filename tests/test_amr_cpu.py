@@ -420,6 +420,32 @@ def test_library_clustering_matches_oracle_on_fronts(masked):
             assert got == want, (trial, block)
 
 
+@pytest.mark.parametrize("masked", [False, True])
+def test_library_clustering_matches_oracle_on_a_large_front_set(masked):
+    """Six oblique fronts over a 1024x256x2 domain, about 10 000 tagged
+    points: enough for the library to refine the first clusters serially and
+    the rest on its host threads. The boxes must still be the oracle's."""
+    from ocfd.amr import cluster_tags
+    rng = np.random.default_rng(5)
+    shape = (2, 256, 1024)
+    d = np.zeros(shape, np.uint8)
+    for _ in range(6):
+        x0, sl, w = int(rng.integers(0, shape[2])), rng.uniform(-1.5, 1.5), int(rng.integers(2, 6))
+        for j in range(shape[1]):
+            x = int(x0 + sl * j)
+            if 0 <= x < shape[2] - w:
+                d[:, j, x:x + w] = 1
+    mask = None
+    if masked:
+        mask = np.ones(shape, np.uint8)
+        mask[:, 96:160, 256:768] = 0
+    assert int(d.sum()) > 8192
+    want = sorted(ao.cluster(d, (0, 0, 0), 1, 0.7, 32, mask))
+    got = sorted(cluster_tags(d, (0, 0, 0), 1, 0.7, 32, mask))
+    assert len(want) > 256
+    assert got == want
+
+
 def _level_flags(boxes, dense_bits):
     """Flags in the level's 0-ghost layout (box order, i fastest) from a dense
     (nz, ny, nx) array over a domain starting at 0."""
