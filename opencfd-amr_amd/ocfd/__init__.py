@@ -5,6 +5,7 @@ Device memory and streams come from torch (plumbing only); every computation
 runs in the HIP library.  Loading fails loudly when the library is missing.
 """
 import ctypes
+import itertools
 import os
 from pathlib import Path
 
@@ -306,17 +307,41 @@ def _stream(stream):
     return ctypes.c_void_p(stream.cuda_stream)
 
 
+_INT_P = ctypes.POINTER(ctypes.c_int)
+
+
+def box_corners(boxes):
+    """(n, lo, hi): the boxes' lower and upper corners as int* arguments of the
+    C ABI, 3 ints per box (each pointer keeps its array alive). One pass over
+    the Python tuples, without a ctypes element per integer."""
+    n = len(boxes)
+    a = np.fromiter(itertools.chain.from_iterable(itertools.chain.from_iterable(boxes)), np.int32, 6 * n)
+    a = a.reshape(n, 2, 3)
+    lo = np.ascontiguousarray(a[:, 0]) if n else np.zeros(3, np.int32)
+    hi = np.ascontiguousarray(a[:, 1]) if n else np.zeros(3, np.int32)
+    return n, lo.ctypes.data_as(_INT_P), hi.ctypes.data_as(_INT_P)
+
+
+def box_buffer(cap):
+    """(array, int*): room for `cap` boxes written by the C ABI (6 ints each)."""
+    out = np.empty(6 * max(1, cap), np.int32)
+    return out, out.ctypes.data_as(_INT_P)
+
+
+def boxes_from_buffer(out, nb):
+    """The first `nb` boxes of a box buffer as ((lo), (hi)) tuples."""
+    return [(tuple(l), tuple(h)) for l, h in out[:6 * nb].reshape(nb, 2, 3).tolist()]
+
+
 class Level:
     """One AMR level's box list + geometry (drop-in for the BoxArray/Geometry
     an EBR level advances, Source/EBR.cpp:43-65)."""
 
     def __init__(self, boxes, geom, eb=None):
-        self.boxes = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
+        self.boxes = [((a, b, c), (d, e, f)) for (a, b, c), (d, e, f) in boxes]
         self.geom = geom
         self.eb = eb
-        n = len(self.boxes)
-        lo = (ctypes.c_int * (3 * n))(*[v for b in self.boxes for v in b[0]])
-        hi = (ctypes.c_int * (3 * n))(*[v for b in self.boxes for v in b[1]])
+        n, lo, hi = box_corners(self.boxes)
         h = ctypes.c_void_p()
         check(lib().ocfd_level_create_eb(n, lo, hi, ctypes.byref(geom), ctypes.byref(eb) if eb is not None else None,
                                          ctypes.byref(h)), "ocfd_level_create_eb")

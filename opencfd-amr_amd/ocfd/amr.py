@@ -26,7 +26,7 @@ import os
 
 import numpy as np
 
-from . import PLANAR, Geom, Level, NCONS, _ptr, _stream, check, lib
+from . import PLANAR, Geom, Level, NCONS, _ptr, _stream, box_buffer, box_corners, boxes_from_buffer, check, lib
 
 CHANGE_MAX = 1.1
 RATIO = 2
@@ -45,7 +45,8 @@ def refine_geom(g):
 
 
 def refine_boxes(boxes):
-    return [(tuple(RATIO * v for v in lo), tuple(RATIO * v + RATIO - 1 for v in hi)) for lo, hi in boxes]
+    r, e = RATIO, RATIO - 1
+    return [((r * a, r * b, r * c), (r * d + e, r * f + e, r * g + e)) for (a, b, c), (d, f, g) in boxes]
 
 
 def lincomb(a, x, b, y, out, stream=None):
@@ -70,32 +71,25 @@ def tag_dengrad_nest(level, S, threshold, n_proper, stream=None):
     return flags
 
 
-def _box_arrays(boxes):
-    n = len(boxes)
-    lo = (ctypes.c_int * max(1, 3 * n))(*[v for b in boxes for v in b[0]])
-    hi = (ctypes.c_int * max(1, 3 * n))(*[v for b in boxes for v in b[1]])
-    return n, lo, hi
-
-
 def cluster_box_tags(boxes, flags, extra=(), use_mask=True, block=1, grid_eff=0.7, max_size=32):
     """cluster_tags from flags in the level layout of `boxes` (host uint8,
     bit 0 tag, bit 1 allowed region) plus the cells of the `extra` boxes
     as tags; the clusters are cut back to the allowed region when use_mask."""
     flags = np.ascontiguousarray(flags, dtype=np.uint8)
-    n, lo, hi = _box_arrays(boxes)
-    ne, elo, ehi = _box_arrays(list(extra))
+    n, lo, hi = box_corners(boxes)
+    ne, elo, ehi = box_corners(list(extra))
     nb = ctypes.c_int()
     cap = 4096
     while True:
-        out = (ctypes.c_int * (6 * cap))()
+        out, outp = box_buffer(cap)
         rc = lib().ocfd_cluster_box_tags(n, lo, hi, flags.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), ne, elo, ehi,
-                                         int(bool(use_mask)), block, grid_eff, max_size, cap, out, ctypes.byref(nb))
+                                         int(bool(use_mask)), block, grid_eff, max_size, cap, outp, ctypes.byref(nb))
         if rc != 0 and nb.value > cap:
             cap = nb.value
             continue
         check(rc, "ocfd_cluster_box_tags")
         break
-    return [(tuple(out[6 * b:6 * b + 3]), tuple(out[6 * b + 3:6 * b + 6])) for b in range(nb.value)]
+    return boxes_from_buffer(out, nb.value)
 
 
 def dense_tags(level, tags):
@@ -131,15 +125,15 @@ def cluster_tags(dense, lo, block=1, grid_eff=0.7, max_size=32, mask=None):
         mk = mask.ctypes.data_as(up)
     cap = 4096
     while True:
-        out = (ctypes.c_int * (6 * cap))()
-        rc = lib().ocfd_cluster_tags(lo_c, hi_c, dense.ctypes.data_as(up), mk, block, grid_eff, max_size, cap, out,
+        out, outp = box_buffer(cap)
+        rc = lib().ocfd_cluster_tags(lo_c, hi_c, dense.ctypes.data_as(up), mk, block, grid_eff, max_size, cap, outp,
                                      ctypes.byref(nb))
         if rc != 0 and nb.value > cap:
             cap = nb.value          # more boxes than room: once more with the exact count
             continue
         check(rc, "ocfd_cluster_tags")
         break
-    return [(tuple(out[6 * b:6 * b + 3]), tuple(out[6 * b + 3:6 * b + 6])) for b in range(nb.value)]
+    return boxes_from_buffer(out, nb.value)
 
 
 def copy_level(dst, S_dst, src, S_src, stream=None):
@@ -194,20 +188,21 @@ def projected_coverage(boxes_f2, geom_f, geom_c, n_proper):
 
 def projected_boxes(boxes_f2, geom_f, n_proper):
     """The boxes of projected_coverage (level c index space, may overlap)."""
-    n, lo, hi = _box_arrays(boxes_f2)
+    n, lo, hi = box_corners(boxes_f2)
     dlo = (ctypes.c_int * 3)(*[geom_f.dom_lo[d] for d in range(3)])
     dhi = (ctypes.c_int * 3)(*[geom_f.dom_hi[d] for d in range(3)])
     per = (ctypes.c_int * 3)(*[int(bool(geom_f.periodic[d])) for d in range(3)])
     nb = ctypes.c_int()
     cap = max(64, 27 * n)
-    out = (ctypes.c_int * (6 * cap))()
-    check(lib().ocfd_project_boxes(n, lo, hi, dlo, dhi, per, int(n_proper), cap, out, ctypes.byref(nb)),
+    out, outp = box_buffer(cap)
+    check(lib().ocfd_project_boxes(n, lo, hi, dlo, dhi, per, int(n_proper), cap, outp, ctypes.byref(nb)),
           "ocfd_project_boxes")
-    return [(tuple(out[6 * b:6 * b + 3]), tuple(out[6 * b + 3:6 * b + 6])) for b in range(nb.value)]
+    return boxes_from_buffer(out, nb.value)
 
 
 def coarsen_boxes(boxes):
-    return [(tuple(v // RATIO for v in lo), tuple(v // RATIO for v in hi)) for lo, hi in boxes]
+    r = RATIO
+    return [((a // r, b // r, c // r), (d // r, f // r, g // r)) for (a, b, c), (d, f, g) in boxes]
 
 
 class FluxRegister:
