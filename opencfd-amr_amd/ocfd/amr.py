@@ -200,6 +200,14 @@ def projected_boxes(boxes_f2, geom_f, n_proper):
     return boxes_from_buffer(out, nb.value)
 
 
+def _spans_z(level):
+    """Every box of the level spans its domain in z: then no coarse/fine face
+    of the level faces z (a coarse column is wholly covered or not at all),
+    and the register's face list need not be searched for one."""
+    zlo, zhi = level.geom.dom_lo[2], level.geom.dom_hi[2]
+    return all(lo[2] == zlo and hi[2] == zhi for lo, hi in level.boxes)
+
+
 def coarsen_boxes(boxes):
     r = RATIO
     return [((a // r, b // r, c // r), (d // r, f // r, g // r)) for (a, b, c), (d, f, g) in boxes]
@@ -334,7 +342,7 @@ class Hierarchy:
         if cf is None:
             cf = FluxRegister(self.levels[l - 1], self.levels[l])
         self._retire(self.cfs[l])
-        if (self.flags & PLANAR) and cf.nfaces and (cf.faces()[:, 4] == 2).any():
+        if (self.flags & PLANAR) and cf.nfaces and not _spans_z(cf.fine) and (cf.faces()[:, 4] == 2).any():
             raise ValueError("planar stepping needs fine grids that span the domain in z")
         self.cfs[l] = cf
 

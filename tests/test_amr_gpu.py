@@ -353,6 +353,22 @@ def test_two_level_fp32_flux_and_planar(torch_cuda, orc):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.gpu
+def test_planar_hierarchy_refuses_fine_grids_short_of_the_z_domain():
+    """Planar stepping builds fine grids that span the domain in z, and refuses
+    one that covers half of it (its register would hold z faces)."""
+    import ocfd
+    from ocfd import amr, cases
+    boxes, g, kind = cases.dmr(64, 16, 2, max_grid=(16, 16, 2))
+    T = amr.TwoLevel(boxes, g, ocfd.default_parm(), 0.5, kind, flags=ocfd.PLANAR, regrid_int=0,
+                     fine_boxes=[((0, 0, 0), (21, 9, 1))])
+    assert T.cfs[1].nfaces > 0
+    del T
+    with pytest.raises(ValueError, match="span the domain in z"):
+        amr.TwoLevel(boxes, g, ocfd.default_parm(), 0.5, kind, flags=ocfd.PLANAR, regrid_int=0,
+                     fine_boxes=[((0, 0, 0), (21, 9, 0))])
+
+
 @pytest.mark.parametrize("flags", ["PLANAR|VISC", "PLANAR|FLUX_FP32|VISC"])
 def test_two_level_planar_fill_depth_viscous(torch_cuda, orc, flags):
     """The one-layer planar fill (first-layer sweeps, first-layer register
